@@ -1,0 +1,222 @@
+/*
+ * vkhrt C-ABI: MI355X (gfx950) drop-in for vkhrt's traceRayEXT hair path.
+ *
+ * Reference interfaces replaced (mmzala/vkhrt, file:line):
+ *   BottomLevelAccelerationStructure(const BLASInput&, ...)
+ *       include/bottom_level_acceleration_structure.hpp:34, BLASInput :19-29, BLASType :12-17
+ *       -> vkhrt_blas_build / vkhrt_blas_destroy
+ *   TopLevelAccelerationStructure(const std::vector<BottomLevelAccelerationStructure>&, ...)
+ *       include/top_level_acceleration_structure.hpp:12; instances
+ *       source/top_level_acceleration_structure.cpp:23-43 (customIndex = i, mask 0xFF)
+ *       -> vkhrt_tlas_build / vkhrt_tlas_destroy
+ *   traceRayEXT(tlas, gl_RayFlagsOpaqueEXT, 0xFF, 0, 0, 0, o, tMin, d, tMax, 0)
+ *       shaders/ray_gen.rgen:26-45, reportIntersectionEXT shaders/hair_intersection.rint:146-149
+ *       -> vkhrt_trace_rays
+ *   Renderer::RecordRayTracingCommands -> traceRaysKHR(W, H, 1) source/renderer.cpp:156-166
+ *       with ray_gen.rgen + hair_closest_hit.rchit / triangle_closest_hit.rchit + miss
+ *       -> vkhrt_render
+ *   CameraUniformData{viewInverse, projInverse} include/resources/camera_resource.hpp:7-11,
+ *   FlyCamera::ViewMatrix/ProjectionMatrix source/fly_camera.cpp:25-35
+ *       -> vkhrt_camera_look_at
+ *   ProcessHairCurves / GenerateCurves / GenerateLinearSweptSpheres /
+ *   GenerateDisjointOrthogonalTriangleStrips source/resources/model/geometry_processor.cpp:
+ *   123-156, 221-298, 604-654 -> vkhrt_generate_* (run on the GPU)
+ *
+ * Conventions: all functions return a vkhrt_status (0 = success, negative = error, no
+ * aborts, no exceptions across the ABI); vkhrt_last_error() describes the last failure of
+ * a context. Buffers are device pointers unless a flag says otherwise. `stream` is a
+ * hipStream_t (NULL = the context's own stream). A context is not internally locked:
+ * use one per host thread.
+ */
+#ifndef VKHRT_H
+#define VKHRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VKHRT_ABI_VERSION 1
+
+typedef struct vkhrt_context vkhrt_context;
+typedef struct vkhrt_blas vkhrt_blas;
+typedef struct vkhrt_tlas vkhrt_tlas;
+
+typedef enum vkhrt_status {
+    VKHRT_SUCCESS = 0,
+    VKHRT_ERROR_INVALID_ARGUMENT = -1,
+    VKHRT_ERROR_OUT_OF_MEMORY = -2,
+    VKHRT_ERROR_DEVICE = -3,
+    VKHRT_ERROR_UNSUPPORTED = -4,
+    VKHRT_ERROR_STACK_OVERFLOW = -5,
+    VKHRT_ERROR_NO_DEVICE = -6
+} vkhrt_status;
+
+/* Geometry modes: BLASType::eHair with AABB geometry + hair_intersection.rint (phantom),
+ * VK_NV_ray_tracing_linear_swept_spheres (LSS), or DOTS triangles
+ * (source/resources/model/model_loader.cpp:334 picks LSS or DOTS; phantom is wired here). */
+typedef enum vkhrt_geometry_mode {
+    VKHRT_GEOMETRY_PHANTOM_CURVES = 0,
+    VKHRT_GEOMETRY_LSS = 1,
+    VKHRT_GEOMETRY_DOTS = 2
+} vkhrt_geometry_mode;
+
+/* vkhrt_blas_desc.flags */
+#define VKHRT_BLAS_HOST_POINTERS (1u << 0)    /* `primitives` is host memory */
+#define VKHRT_BLAS_PER_POINT_RADIUS (1u << 1) /* use per-point radii (.w) instead of `radius` */
+#define VKHRT_BLAS_HAS_TRANSFORM (1u << 2)    /* `transform` is valid (else identity) */
+
+/* Mirrors BLASInput (bottom_level_acceleration_structure.hpp:19-29).
+ * primitives, `count` records of:
+ *   PHANTOM_CURVES: 16 floats = start, cp1, cp2, end, each (x, y, z, radius) -- the 48-byte
+ *                   Curve of include/resources/model/model.hpp:32-38 plus one radius per point
+ *   LSS:            8 floats = (a.xyz, ra, b.xyz, rb); list indexing, chained end caps
+ *                   (renderer.cpp:664-674): a start cap equal to the previous end is shared
+ *   DOTS:           8 floats = line (start.xyz, r0, end.xyz, r1); 2 orthogonal quads
+ *                   (4 triangles) per line are generated; primitive id = 4 * line + triangle
+ * radius: constant radius used when VKHRT_BLAS_PER_POINT_RADIUS is not set (reference 0.02).
+ * transform: row-major 3x4 world-from-object (BLASInput.transform, Node::GetWorldMatrix). */
+typedef struct vkhrt_blas_desc {
+    uint32_t mode;
+    uint32_t flags;
+    const float* primitives;
+    uint32_t count;
+    float radius;
+    uint32_t material_index;
+    uint32_t leaf_size; /* primitives per leaf child, 1..3; 0 = default (1) */
+    float transform[12];
+} vkhrt_blas_desc;
+
+typedef struct vkhrt_blas_info {
+    uint32_t mode;
+    uint32_t primitive_count; /* reported primitives (DOTS: triangles) */
+    uint32_t leaf_records;    /* curves / capsules / DOTS pieces */
+    uint32_t node_count;      /* 8-wide nodes */
+    uint32_t depth;           /* 8-wide levels */
+    uint64_t device_bytes;
+    double build_ms;
+} vkhrt_blas_info;
+
+/* Ray record (32 B): origin, tmin, direction, tmax. */
+typedef struct vkhrt_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} vkhrt_ray;
+
+/* Hit record (32 B). Miss: primitive_id = instance_id = 0xFFFFFFFF, t = 0.
+ * t: phantom reports the distance along the normalized direction like
+ *    hair_intersection.rint:70-71 (use unit directions); LSS/DOTS the ray parameter.
+ * u: phantom curve parameter / LSS segment parameter / DOTS barycentric of vertex 1.
+ * v: DOTS barycentric of vertex 2; phantom 1 if the root came from the FP64 polish.
+ * normal: world-space unit normal (phantom: hit - curve(u), hair_intersection.rint:74-76;
+ *    LSS: hit - mix(a, b, u), triangle_closest_hit.rchit:50-55; DOTS: geometric normal). */
+typedef struct vkhrt_hit {
+    float t;
+    uint32_t primitive_id;
+    uint32_t instance_id;
+    float u;
+    float normal[3];
+    float v;
+} vkhrt_hit;
+
+/* vkhrt_trace_rays flags */
+#define VKHRT_TRACE_TERMINATE_ON_FIRST_HIT (1u << 0) /* shadow / any-hit rays */
+#define VKHRT_TRACE_FP64_POLISH (1u << 3)            /* FP64 Newton polish of grazing phantom roots */
+
+/* Column-major inverse view / projection (CameraUniformData). */
+typedef struct vkhrt_camera {
+    float view_inverse[16];
+    float proj_inverse[16];
+} vkhrt_camera;
+
+/* vkhrt_render_desc.flags */
+#define VKHRT_RENDER_SHADOWS (1u << 0)       /* one shadow ray per hit toward (0, 1, 0) */
+#define VKHRT_RENDER_DEBUG_PALETTE (1u << 1) /* debug.glsl palette by primitive id */
+#define VKHRT_RENDER_RGBA8 (1u << 2)         /* image is RGBA8 (else RGBA32F) */
+#define VKHRT_RENDER_FP64_POLISH (1u << 3)
+
+typedef struct vkhrt_render_desc {
+    uint32_t width;
+    uint32_t height;
+    uint32_t flags;
+    void* image;                    /* device, width * height pixels */
+    uint32_t* primitive_ids;        /* device, optional (0xFFFFFFFF = miss) */
+    const float* light_visibility;  /* device, optional per-pixel light visibility in [0, 1]
+                                       replacing the shadow ray (neural shadow form) */
+} vkhrt_render_desc;
+
+/* ---- context */
+int vkhrt_context_create(int device, vkhrt_context** out);
+void vkhrt_context_destroy(vkhrt_context* ctx);
+const char* vkhrt_last_error(const vkhrt_context* ctx);
+int vkhrt_synchronize(vkhrt_context* ctx, void* stream);
+int vkhrt_abi_version(void);
+
+/* Options (autotuned parameters). */
+#define VKHRT_OPTION_STACK_KIND 1 /* LDS stack entries per lane: 0 -> 8, 1 -> 16, 2 -> 24 */
+#define VKHRT_OPTION_LEAF_SIZE 2  /* default leaf size for builds (1..3) */
+int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
+
+/* ---- acceleration structures */
+int vkhrt_blas_build(vkhrt_context* ctx, const vkhrt_blas_desc* desc, void* stream, vkhrt_blas** out);
+int vkhrt_blas_info_get(const vkhrt_blas* blas, vkhrt_blas_info* out);
+void vkhrt_blas_destroy(vkhrt_blas* blas);
+/* Instance i = blases[i] with its transform; customIndex = i. */
+int vkhrt_tlas_build(vkhrt_context* ctx, const vkhrt_blas* const* blases, uint32_t count, void* stream,
+                     vkhrt_tlas** out);
+void vkhrt_tlas_destroy(vkhrt_tlas* tlas);
+
+/* ---- tracing */
+int vkhrt_trace_rays(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* rays, uint32_t count,
+                     vkhrt_hit* hits, uint32_t flags, void* stream);
+int vkhrt_render(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_camera* camera,
+                 const vkhrt_render_desc* desc, void* stream);
+/* glm::lookAt + perspectiveRH_ZO with the Vulkan Y flip, inverted (host function). */
+int vkhrt_camera_look_at(const float eye[3], const float center[3], const float up[3], float tan_half_fovy,
+                         float aspect, float znear, float zfar, vkhrt_camera* out);
+
+/* ---- geometry front-end (device buffers, float4 records) */
+typedef struct vkhrt_strand_params {
+    float sphere_radius; /* 100 */
+    float step;          /* 1.5 */
+    float gravity;       /* 0.3 */
+    float curl;          /* 0.2 */
+    float r_root;        /* 0.04 */
+    float r_tip;         /* 0.01 */
+} vkhrt_strand_params;
+/* Seeded synthetic strands: nstrands x (nseg + 1) vertices of (x, y, z, radius). */
+int vkhrt_generate_strands(vkhrt_context* ctx, uint64_t seed, uint32_t nstrands, uint32_t nseg,
+                           const vkhrt_strand_params* params, float* vertices, void* stream);
+/* Strand polylines -> lines (start.xyz, r0, end.xyz, r1), GenerateLines. */
+int vkhrt_generate_lines(vkhrt_context* ctx, const float* vertices, uint32_t nstrands, uint32_t nseg, float* lines,
+                         void* stream);
+/* GenerateCurves: lines -> curves (16 floats each), tension 1 in the reference. */
+int vkhrt_generate_curves(vkhrt_context* ctx, const float* lines, uint32_t count, float tension, float* curves,
+                          void* stream);
+/* Cubic -> `pieces` lines per curve sampled at t = k / pieces. */
+int vkhrt_curves_to_lines(vkhrt_context* ctx, const float* curves, uint32_t count, uint32_t pieces, float* lines,
+                          void* stream);
+/* GenerateLinearSweptSpheres: lines -> capsules (8 floats), radius max(r, 0.001);
+ * constant_radius < 0 keeps per-end radii. */
+int vkhrt_generate_lss(vkhrt_context* ctx, const float* lines, uint32_t count, float constant_radius, float* capsules,
+                       void* stream);
+/* GenerateDisjointOrthogonalTriangleStrips frames: lines -> pieces (16 floats:
+ * start, r0, end, r1, s, 0, t, 0). constant_radius < 0 keeps per-end radii. */
+int vkhrt_generate_dots(vkhrt_context* ctx, const float* lines, uint32_t count, float constant_radius, float* pieces,
+                        void* stream);
+
+/* ---- profiling: per-kernel HIP-event timing on the launch stream */
+int vkhrt_profiling_enable(vkhrt_context* ctx, int enable);
+/* name: "render_primary", "render_shadow", "trace_rays", "blas_build". */
+int vkhrt_kernel_stats(vkhrt_context* ctx, const char* name, double* total_ms, uint32_t* launches);
+int vkhrt_status_counters(vkhrt_context* ctx, uint32_t* stack_overflows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VKHRT_H */

@@ -1,0 +1,967 @@
+// Oracle C API (test infrastructure only; see oracle_math.hpp). Loaded with ctypes by
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+//
+// Semantics restated from the reference:
+//  - traceRayEXT(tlas, gl_RayFlagsOpaqueEXT, 0xFF, ...) with tMin/tMax (ray_gen.rgen:26-45):
+//    closest opaque hit with t in [tMin, tMax]; the phantom intersector reports t > 0 only
+//    (hair_intersection.rint:146-149). Ties in t resolve to the lowest (instance, primitive),
+//    a rule this build defines (the Vulkan spec leaves it open).
+//  - Instance i of the TLAS has customIndex = i (top_level_acceleration_structure.cpp:30).
+//  - Hair BLAS primitives are curve segments with GenerateAABBs bounds
+//    (geometry_processor.cpp:422-436); the BVH is only an accelerator: brute force over all
+//    primitives gives the same answer (pinned in tests).
+// Additions with no reference counterpart: shadow rays toward -lightDirection
+// (shading.glsl:4), the shadowed Shade, FP64 polish, the per-point radius.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "oracle_geom.hpp"
+
+using namespace orc;
+
+namespace {
+
+enum { MODE_PHANTOM = 0, MODE_LSS = 1, MODE_DOTS = 2 };
+enum {
+    F_ANY_HIT = 1u << 0,
+    F_BRUTE = 1u << 1,
+    F_FP64 = 1u << 2,
+    F_POLISH = 1u << 3,
+    F_REF_NORMAL = 1u << 4,
+};
+
+struct Hit {
+    float t;
+    uint32_t prim;
+    uint32_t inst;
+    float u;
+    float n[3];
+    float v;
+};
+static_assert(sizeof(Hit) == 32, "hit record is 32 bytes");
+
+struct Box {
+    float mn[3], mx[3];
+    void reset() {
+        mn[0] = mn[1] = mn[2] = INFINITY;
+        mx[0] = mx[1] = mx[2] = -INFINITY;
+    }
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = std::min(mn[k], b.mn[k]);
+            mx[k] = std::max(mx[k], b.mx[k]);
+        }
+    }
+    float area() const {
+        float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        if (!(dx >= 0)) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Node {
+    Box b;
+    uint32_t left;   // internal: left child; leaf: first index into order[]
+    uint32_t count;  // 0 = internal (right = left + 1 is NOT assumed: right stored separately)
+    uint32_t right;
+};
+
+struct Blas {
+    int mode = MODE_PHANTOM;
+    bool per_point = false;
+    float radius = 0.02f;
+    std::vector<float> prims;        // phantom: 16/curve, LSS: 8/capsule, DOTS: 8/line
+    std::vector<uint8_t> cap_a;      // LSS chained caps
+    std::vector<DotsPiece> pieces;   // DOTS
+    uint32_t nprims = 0;             // primitive count (DOTS: triangles)
+    std::vector<Box> boxes;
+    std::vector<uint32_t> order;
+    std::vector<Node> nodes;
+};
+
+struct Instance {
+    std::shared_ptr<Blas> blas;
+    bool identity = true;
+    float xf[12];     // world-from-object, row-major 3x4
+    M4<float> inv;    // object-from-world
+};
+
+struct Scene {
+    std::vector<Instance> inst;
+};
+
+float pad_for(const Box& b) {
+    float m = 0;
+    for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(b.mn[k]), std::fabs(b.mx[k])));
+    return 1e-4f + 4e-7f * m;
+}
+
+void prim_box(const Blas& B, uint32_t i, Box& bx) {
+    if (B.mode == MODE_PHANTOM) {
+        CurveAABB(&B.prims[size_t(i) * 16], B.per_point, B.radius, bx.mn, bx.mx);
+    } else if (B.mode == MODE_LSS) {
+        const float* c = &B.prims[size_t(i) * 8];
+        for (int k = 0; k < 3; ++k) {
+            bx.mn[k] = std::min(c[k] - c[3], c[4 + k] - c[7]);
+            bx.mx[k] = std::max(c[k] + c[3], c[4 + k] + c[7]);
+        }
+    } else {
+        const DotsPiece& p = B.pieces[i / 4];
+        V3<float> v0, v1, v2;
+        DotsTriangle(p, (i >> 1) & 1, i & 1, v0, v1, v2);
+        for (int k = 0; k < 3; ++k) {
+            float a = comp(v0, k), b = comp(v1, k), c = comp(v2, k);
+            bx.mn[k] = std::min(a, std::min(b, c));
+            bx.mx[k] = std::max(a, std::max(b, c));
+        }
+    }
+    float p = pad_for(bx);
+    for (int k = 0; k < 3; ++k) {
+        bx.mn[k] -= p;
+        bx.mx[k] += p;
+    }
+}
+
+// ---------------------------------------------------------------- binned SAH BVH2
+struct Builder {
+    Blas& B;
+    int leaf_size;
+    std::vector<float> cen;  // 3 per prim
+    std::atomic<uint32_t> next_node{0};
+
+    uint32_t alloc() { return next_node.fetch_add(1); }
+
+    void build(uint32_t ni, uint32_t first, uint32_t count, int depth) {
+        Node& N = B.nodes[ni];
+        Box bb;
+        bb.reset();
+        Box cb;
+        cb.reset();
+        for (uint32_t i = first; i < first + count; ++i) {
+            uint32_t p = B.order[i];
+            bb.grow(B.boxes[p]);
+            for (int k = 0; k < 3; ++k) {
+                cb.mn[k] = std::min(cb.mn[k], cen[p * 3 + k]);
+                cb.mx[k] = std::max(cb.mx[k], cen[p * 3 + k]);
+            }
+        }
+        N.b = bb;
+        if (count <= uint32_t(leaf_size)) {
+            N.left = first;
+            N.count = count;
+            return;
+        }
+        const int NB = 16;
+        float best_cost = INFINITY;
+        int best_axis = -1, best_split = 0;
+        for (int ax = 0; ax < 3; ++ax) {
+            float lo = cb.mn[ax], hi = cb.mx[ax];
+            if (!(hi > lo)) continue;
+            Box bins[NB];
+            uint32_t cnt[NB] = {0};
+            for (int b = 0; b < NB; ++b) bins[b].reset();
+            float scale = NB / (hi - lo);
+            for (uint32_t i = first; i < first + count; ++i) {
+                uint32_t p = B.order[i];
+                int b = std::min(NB - 1, int((cen[p * 3 + ax] - lo) * scale));
+                cnt[b]++;
+                bins[b].grow(B.boxes[p]);
+            }
+            float rarea[NB];
+            uint32_t rcnt[NB];
+            Box acc;
+            acc.reset();
+            uint32_t c = 0;
+            for (int b = NB - 1; b > 0; --b) {
+                acc.grow(bins[b]);
+                c += cnt[b];
+                rarea[b] = acc.area();
+                rcnt[b] = c;
+            }
+            acc.reset();
+            c = 0;
+            for (int b = 0; b < NB - 1; ++b) {
+                acc.grow(bins[b]);
+                c += cnt[b];
+                if (c == 0 || rcnt[b + 1] == 0) continue;
+                float cost = acc.area() * c + rarea[b + 1] * rcnt[b + 1];
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = ax;
+                    best_split = b;
+                }
+            }
+        }
+        uint32_t mid;
+        if (best_axis < 0) {
+            mid = first + count / 2;  // all centroids equal: split by index
+        } else {
+            float lo = cb.mn[best_axis], hi = cb.mx[best_axis];
+            float scale = NB / (hi - lo);
+            auto it = std::partition(B.order.begin() + first, B.order.begin() + first + count, [&](uint32_t p) {
+                int b = std::min(NB - 1, int((cen[p * 3 + best_axis] - lo) * scale));
+                return b <= best_split;
+            });
+            mid = uint32_t(it - B.order.begin());
+            if (mid == first || mid == first + count) mid = first + count / 2;
+        }
+        uint32_t l = alloc(), r = alloc();
+        N.left = l;
+        N.right = r;
+        N.count = 0;
+        uint32_t lc = mid - first, rc = count - lc;
+        if (depth < 4 && count > 50000) {
+            std::thread th([&, l, first, lc, depth] { build(l, first, lc, depth + 1); });
+            build(r, mid, rc, depth + 1);
+            th.join();
+        } else {
+            build(l, first, lc, depth + 1);
+            build(r, mid, rc, depth + 1);
+        }
+    }
+};
+
+void build_blas(Blas& B, int leaf_size) {
+    uint32_t n = B.nprims;
+    B.boxes.resize(n);
+    B.order.resize(n);
+    Builder bd{B, leaf_size};
+    bd.cen.resize(size_t(n) * 3);
+    for (uint32_t i = 0; i < n; ++i) {
+        prim_box(B, i, B.boxes[i]);
+        B.order[i] = i;
+        for (int k = 0; k < 3; ++k) bd.cen[i * 3 + k] = 0.5f * (B.boxes[i].mn[k] + B.boxes[i].mx[k]);
+    }
+    B.nodes.assign(std::max<uint32_t>(1, 2 * n), Node{});
+    if (n == 0) {
+        B.nodes[0].b.reset();
+        B.nodes[0].count = 0;
+        B.nodes[0].left = 0;
+        return;
+    }
+    bd.next_node = 1;
+    bd.build(0, 0, n, 0);
+    B.nodes.resize(bd.next_node.load());
+}
+
+// ---------------------------------------------------------------- per-primitive tests
+template <class T>
+bool test_prim(const Blas& B, uint32_t p, const Ray<float>& ray, float tmin, float tmax, uint32_t flags, Hit& h) {
+    if (B.mode == MODE_PHANTOM) {
+        const float* c = &B.prims[size_t(p) * 16];
+        PrhiOpts o;
+        o.radius_mode = B.per_point ? RADIUS_PER_POINT : RADIUS_CONSTANT;
+        o.radius = B.radius;
+        o.fp64_polish = (flags & F_POLISH) != 0;
+        Curve<T> cv{mk(T(c[0]), T(c[1]), T(c[2])), mk(T(c[4]), T(c[5]), T(c[6])), mk(T(c[8]), T(c[9]), T(c[10])),
+                    mk(T(c[12]), T(c[13]), T(c[14]))};
+        T rr[4] = {T(c[3]), T(c[7]), T(c[11]), T(c[15])};
+        Ray<T> r{cvt<T>(ray.origin), cvt<T>(ray.direction)};
+        PrhiOut<T> out = Prhi(r, cv, rr, o);
+        if (!(out.t > T(0))) return false;
+        float t = float(out.t);
+        if (!(t >= tmin && t <= tmax)) return false;
+        h.t = t;
+        h.u = float(out.u);
+        V3<T> n = (flags & F_REF_NORMAL) ? out.n_ref : out.n_rcc;
+        h.n[0] = float(n.x);
+        h.n[1] = float(n.y);
+        h.n[2] = float(n.z);
+        h.v = out.polished ? 1.0f : 0.0f;
+        return true;
+    } else if (B.mode == MODE_LSS) {
+        const float* c = &B.prims[size_t(p) * 8];
+        LssOut<T> out;
+        bool ok = IntersectLSS<T>(cvt<T>(ray.origin), cvt<T>(ray.direction), mk(T(c[0]), T(c[1]), T(c[2])), T(c[3]),
+                                  mk(T(c[4]), T(c[5]), T(c[6])), T(c[7]), B.cap_a[p] != 0, T(tmin), T(tmax), out);
+        if (!ok) return false;
+        h.t = float(out.t);
+        h.u = float(out.u);
+        h.n[0] = float(out.n.x);
+        h.n[1] = float(out.n.y);
+        h.n[2] = float(out.n.z);
+        h.v = 0.0f;
+        return true;
+    } else {
+        const DotsPiece& pc = B.pieces[p / 4];
+        V3<float> v0, v1, v2;
+        DotsTriangle(pc, (p >> 1) & 1, p & 1, v0, v1, v2);
+        TriOut<float> out;
+        if (!IntersectTriangle(ray.origin, ray.direction, v0, v1, v2, tmin, tmax, out)) return false;
+        h.t = out.t;
+        h.u = out.b1;
+        h.v = out.b2;
+        V3<float> n = normalize(cross(v1 - v0, v2 - v0));
+        h.n[0] = n.x;
+        h.n[1] = n.y;
+        h.n[2] = n.z;
+        return true;
+    }
+}
+
+inline bool better(float t, uint32_t inst, uint32_t prim, const Hit& best) {
+    if (t < best.t) return true;
+    if (t == best.t) return inst < best.inst || (inst == best.inst && prim < best.prim);
+    return false;
+}
+
+inline bool slab(const Box& b, const Ray<float>& r, const float inv[3], float tmin, float tmax) {
+    float t0 = tmin, t1 = tmax;
+    const float o[3] = {r.origin.x, r.origin.y, r.origin.z};
+    for (int k = 0; k < 3; ++k) {
+        float a = (b.mn[k] - o[k]) * inv[k];
+        float c = (b.mx[k] - o[k]) * inv[k];
+        if (std::isnan(a) || std::isnan(c)) {  // origin on a slab plane with a zero direction
+            a = -INFINITY;
+            c = INFINITY;
+        }
+        float lo = std::min(a, c), hi = std::max(a, c);
+        t0 = std::max(t0, lo);
+        t1 = std::min(t1, hi * (1.0f + 4e-7f));
+    }
+    return t0 <= t1;
+}
+
+// Traces one ray against one instance. Returns the number of primitive tests.
+template <class T>
+uint32_t trace_blas(const Blas& B, uint32_t inst_id, const Ray<float>& r, float tmin, float& tmax, uint32_t flags,
+                    Hit& best, bool& found, std::vector<uint32_t>* cand) {
+    uint32_t tests = 0;
+    if (B.nprims == 0) return 0;
+    const bool any = (flags & F_ANY_HIT) != 0;
+    if (flags & F_BRUTE) {
+        for (uint32_t p = 0; p < B.nprims; ++p) {
+            Hit h{};
+            ++tests;
+            if (test_prim<T>(B, p, r, tmin, tmax, flags, h)) {
+                if (!found || better(h.t, inst_id, p, best)) {
+                    best = h;
+                    best.prim = p;
+                    best.inst = inst_id;
+                    found = true;
+                    if (!any) tmax = h.t;
+                }
+                if (any) return tests;
+            }
+        }
+        return tests;
+    }
+    float inv[3] = {1.0f / r.direction.x, 1.0f / r.direction.y, 1.0f / r.direction.z};
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const Node& N = B.nodes[stack[--sp]];
+        if (!slab(N.b, r, inv, tmin, tmax)) continue;
+        if (N.count) {
+            for (uint32_t i = N.left; i < N.left + N.count; ++i) {
+                uint32_t p = B.order[i];
+                if (cand) {
+                    if (slab(B.boxes[p], r, inv, tmin, tmax)) cand->push_back(p);
+                    continue;
+                }
+                Hit h{};
+                ++tests;
+                if (test_prim<T>(B, p, r, tmin, tmax, flags, h)) {
+                    if (!found || better(h.t, inst_id, p, best)) {
+                        best = h;
+                        best.prim = p;
+                        best.inst = inst_id;
+                        found = true;
+                        if (!any) tmax = h.t;
+                    }
+                    if (any) return tests;
+                }
+            }
+        } else {
+            stack[sp++] = N.left;
+            stack[sp++] = N.right;
+            if (sp > 126) {
+                std::fprintf(stderr, "oracle: BVH2 stack overflow\n");
+                std::abort();
+            }
+        }
+    }
+    return tests;
+}
+
+void obj_ray(const Instance& I, const Ray<float>& w, Ray<float>& o) {
+    if (I.identity) {
+        o = w;
+        return;
+    }
+    o.origin = xform_point(I.inv, w.origin);
+    float v[4] = {w.direction.x, w.direction.y, w.direction.z, 0.0f}, r[4];
+    mul4(I.inv, v, r);
+    o.direction = mk(r[0], r[1], r[2]);
+}
+
+void normal_to_world(const Instance& I, float n[3]) {
+    if (I.identity) return;
+    V3<float> a = mk(I.xf[0] * n[0] + I.xf[1] * n[1] + I.xf[2] * n[2], I.xf[4] * n[0] + I.xf[5] * n[1] + I.xf[6] * n[2],
+                     I.xf[8] * n[0] + I.xf[9] * n[1] + I.xf[10] * n[2]);
+    a = normalize(a);
+    n[0] = a.x;
+    n[1] = a.y;
+    n[2] = a.z;
+}
+
+template <class T>
+bool trace_scene(const Scene& S, const Ray<float>& r, float tmin, float tmax, uint32_t flags, Hit& best) {
+    bool found = false;
+    best = Hit{};
+    best.t = tmax;
+    float tcur = tmax;
+    for (uint32_t i = 0; i < S.inst.size(); ++i) {
+        const Instance& I = S.inst[i];
+        Ray<float> o;
+        obj_ray(I, r, o);
+        bool was = found;
+        uint32_t prev_inst = best.inst, prev_prim = best.prim;
+        float prev_t = best.t;
+        trace_blas<T>(*I.blas, i, o, tmin, tcur, flags, best, found, nullptr);
+        bool updated = found && (!was || best.inst != prev_inst || best.prim != prev_prim || best.t != prev_t);
+        if (updated) {
+            normal_to_world(I, best.n);
+            if (flags & F_ANY_HIT) return true;
+        }
+    }
+    if (!found) {
+        best.t = 0.0f;
+        best.prim = best.inst = 0xFFFFFFFFu;
+    }
+    return found;
+}
+
+template <class F>
+void parallel_for(uint32_t n, int nthreads, F f) {
+    if (nthreads <= 0) nthreads = int(std::max(1u, std::thread::hardware_concurrency()));
+    std::atomic<uint32_t> next{0};
+    const uint32_t chunk = 64;
+    auto work = [&] {
+        for (;;) {
+            uint32_t b = next.fetch_add(chunk);
+            if (b >= n) break;
+            uint32_t e = std::min(n, b + chunk);
+            for (uint32_t i = b; i < e; ++i) f(i);
+        }
+    };
+    if (nthreads == 1) {
+        work();
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(work);
+    for (auto& t : th) t.join();
+}
+
+inline Ray<float> load_ray(const float* r, float& tmin, float& tmax) {
+    tmin = r[3];
+    tmax = r[7];
+    return Ray<float>{mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6])};
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct orc_scene orc_scene;
+
+int orc_abi_version() { return 1; }
+int orc_hardware_threads() { return int(std::max(1u, std::thread::hardware_concurrency())); }
+
+// ---- generator / geometry processing
+int orc_generate_strands(uint64_t seed, uint32_t nstrands, uint32_t nseg, float sphere_radius, float* out,
+                         int nthreads) {
+    HairParams hp;
+    hp.sphere_radius = sphere_radius;
+    parallel_for(nstrands, nthreads, [&](uint32_t s) {
+        GenerateStrand(seed, s, int(nseg), hp, out + size_t(s) * (nseg + 1) * 4);
+    });
+    return 0;
+}
+
+// Strand polylines -> lines (GenerateLines over consecutive index pairs,
+// geometry_processor.cpp:45-67): line k of strand s = (v[k], v[k+1]); 8 floats per line.
+int orc_strands_to_lines(const float* verts, uint32_t nstrands, uint32_t nseg, float* lines) {
+    for (uint32_t s = 0; s < nstrands; ++s)
+        for (uint32_t k = 0; k < nseg; ++k) {
+            const float* a = verts + (size_t(s) * (nseg + 1) + k) * 4;
+            float* L = lines + (size_t(s) * nseg + k) * 8;
+            L[0] = a[0];
+            L[1] = a[1];
+            L[2] = a[2];
+            L[3] = a[3];
+            L[4] = a[4];
+            L[5] = a[5];
+            L[6] = a[6];
+            L[7] = a[7];
+        }
+    return 0;
+}
+
+static std::vector<LineR> lines_from(const float* lines, uint32_t n) {
+    std::vector<LineR> v(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* L = lines + size_t(i) * 8;
+        v[i].start = mk(L[0], L[1], L[2]);
+        v[i].r0 = L[3];
+        v[i].end = mk(L[4], L[5], L[6]);
+        v[i].r1 = L[7];
+    }
+    return v;
+}
+
+int orc_generate_curves(const float* lines, uint32_t n, float* curves) {
+    std::vector<float> out;
+    GenerateCurves(lines_from(lines, n), out);
+    std::memcpy(curves, out.data(), out.size() * sizeof(float));
+    return 0;
+}
+
+int orc_curves_to_lines(const float* curves, uint32_t n, uint32_t pieces, float* lines) {
+    std::vector<float> c(curves, curves + size_t(n) * 16);
+    std::vector<LineR> L;
+    CurvesToLines(c, int(pieces), L);
+    for (size_t i = 0; i < L.size(); ++i) {
+        float* o = lines + i * 8;
+        o[0] = L[i].start.x;
+        o[1] = L[i].start.y;
+        o[2] = L[i].start.z;
+        o[3] = L[i].r0;
+        o[4] = L[i].end.x;
+        o[5] = L[i].end.y;
+        o[6] = L[i].end.z;
+        o[7] = L[i].r1;
+    }
+    return 0;
+}
+
+// GenerateLinearSweptSpheres (geometry_processor.cpp:273-298): one capsule per line,
+// radius max(r, 0.001) (:285). constant_radius < 0 keeps the per-end radii.
+int orc_generate_lss(const float* lines, uint32_t n, float constant_radius, float* caps) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* L = lines + size_t(i) * 8;
+        float* c = caps + size_t(i) * 8;
+        for (int k = 0; k < 8; ++k) c[k] = L[k];
+        float r0 = constant_radius >= 0 ? constant_radius : L[3];
+        float r1 = constant_radius >= 0 ? constant_radius : L[7];
+        c[3] = gmax(r0, 0.001f);
+        c[7] = gmax(r1, 0.001f);
+    }
+    return 0;
+}
+
+// DOTS pieces: 16 floats (start, r0, end, r1, s, 0, t, 0).
+int orc_generate_dots(const float* lines, uint32_t n, float constant_radius, float* pieces) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* L = lines + size_t(i) * 8;
+        float r0 = constant_radius >= 0 ? constant_radius : L[3];
+        float r1 = constant_radius >= 0 ? constant_radius : L[7];
+        DotsPiece p = MakeDotsPiece(mk(L[0], L[1], L[2]), mk(L[4], L[5], L[6]), r0, r1);
+        float v[16] = {p.start.x, p.start.y, p.start.z, p.r0, p.end.x, p.end.y, p.end.z, p.r1,
+                       p.s.x,     p.s.y,     p.s.z,     0.0f, p.t.x,   p.t.y,   p.t.z,   0.0f};
+        std::memcpy(pieces + size_t(i) * 16, v, sizeof(v));
+    }
+    return 0;
+}
+
+// Expanded DOTS triangles (reference vertex layout, geometry_processor.cpp:240-266):
+// 4 triangles x 3 vertices x xyz per piece.
+int orc_dots_triangles(const float* pieces, uint32_t n, float* tris) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = pieces + size_t(i) * 16;
+        DotsPiece p;
+        p.start = mk(q[0], q[1], q[2]);
+        p.r0 = q[3];
+        p.end = mk(q[4], q[5], q[6]);
+        p.r1 = q[7];
+        p.s = mk(q[8], q[9], q[10]);
+        p.t = mk(q[12], q[13], q[14]);
+        for (int f = 0; f < 2; ++f)
+            for (int t = 0; t < 2; ++t) {
+                V3<float> v[3];
+                DotsTriangle(p, f, t, v[0], v[1], v[2]);
+                float* o = tris + (size_t(i) * 4 + f * 2 + t) * 9;
+                for (int j = 0; j < 3; ++j) {
+                    o[j * 3 + 0] = v[j].x;
+                    o[j * 3 + 1] = v[j].y;
+                    o[j * 3 + 2] = v[j].z;
+                }
+            }
+    }
+    return 0;
+}
+
+int orc_curve_aabbs(const float* curves, uint32_t n, int per_point, float radius, float* aabbs) {
+    for (uint32_t i = 0; i < n; ++i) CurveAABB(curves + size_t(i) * 16, per_point != 0, radius, aabbs + i * 6, aabbs + i * 6 + 3);
+    return 0;
+}
+
+// ---- camera / ray generation
+int orc_camera(const float eye[3], const float center[3], const float up[3], float tan_half_fovy, float aspect,
+               float znear, float zfar, float view_inv[16], float proj_inv[16]) {
+    M4<float> V = LookAtRH(mk(eye[0], eye[1], eye[2]), mk(center[0], center[1], center[2]), mk(up[0], up[1], up[2]));
+    M4<float> P = PerspectiveRH_ZO_FlipY(tan_half_fovy, aspect, znear, zfar);
+    M4<float> Vi = inverse(V), Pi = inverse(P);
+    std::memcpy(view_inv, Vi.m, 64);
+    std::memcpy(proj_inv, Pi.m, 64);
+    return 0;
+}
+
+int orc_raygen(const float view_inv[16], const float proj_inv[16], uint32_t W, uint32_t H, float tmin, float tmax,
+               float* rays) {
+    M4<float> Vi, Pi;
+    std::memcpy(Vi.m, view_inv, 64);
+    std::memcpy(Pi.m, proj_inv, 64);
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            V3<float> o, d;
+            RayGen(Vi, Pi, x, y, W, H, o, d);
+            float* r = rays + (size_t(y) * W + x) * 8;
+            r[0] = o.x;
+            r[1] = o.y;
+            r[2] = o.z;
+            r[3] = tmin;
+            r[4] = d.x;
+            r[5] = d.y;
+            r[6] = d.z;
+            r[7] = tmax;
+        }
+    return 0;
+}
+
+// ---- single-function evaluators (used by the closed-form tests)
+// fp64 != 0 evaluates the FP64 instantiation on the float inputs.
+int orc_prhi(const float* ray /*8*/, const float* curve /*16*/, int per_point, float radius, int polish, int fp64,
+             float* out /*t, u, n_ref[3], n_rcc[3], iters, polished*/) {
+    PrhiOpts o;
+    o.radius_mode = per_point ? RADIUS_PER_POINT : RADIUS_CONSTANT;
+    o.radius = radius;
+    o.fp64_polish = polish != 0;
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        Ray<T> r{mk(T(ray[0]), T(ray[1]), T(ray[2])), mk(T(ray[4]), T(ray[5]), T(ray[6]))};
+        Curve<T> c{mk(T(curve[0]), T(curve[1]), T(curve[2])), mk(T(curve[4]), T(curve[5]), T(curve[6])),
+                   mk(T(curve[8]), T(curve[9]), T(curve[10])), mk(T(curve[12]), T(curve[13]), T(curve[14]))};
+        T rr[4] = {T(curve[3]), T(curve[7]), T(curve[11]), T(curve[15])};
+        PrhiOut<T> p = Prhi(r, c, rr, o);
+        double v[10] = {double(p.t),       double(p.u),       double(p.n_ref.x), double(p.n_ref.y), double(p.n_ref.z),
+                        double(p.n_rcc.x), double(p.n_rcc.y), double(p.n_rcc.z), double(p.iters),   p.polished ? 1.0 : 0.0};
+        for (int i = 0; i < 10; ++i) out[i] = float(v[i]);
+    };
+    if (fp64)
+        run(0.0);
+    else
+        run(0.0f);
+    return 0;
+}
+
+// Same as orc_prhi but writes double results (for FP64 error references).
+int orc_prhi_f64(const float* ray, const float* curve, int per_point, float radius, double* out) {
+    PrhiOpts o;
+    o.radius_mode = per_point ? RADIUS_PER_POINT : RADIUS_CONSTANT;
+    o.radius = radius;
+    Ray<double> r{mk(double(ray[0]), double(ray[1]), double(ray[2])), mk(double(ray[4]), double(ray[5]), double(ray[6]))};
+    Curve<double> c{mk(double(curve[0]), double(curve[1]), double(curve[2])),
+                    mk(double(curve[4]), double(curve[5]), double(curve[6])),
+                    mk(double(curve[8]), double(curve[9]), double(curve[10])),
+                    mk(double(curve[12]), double(curve[13]), double(curve[14]))};
+    double rr[4] = {curve[3], curve[7], curve[11], curve[15]};
+    PrhiOut<double> p = Prhi(r, c, rr, o);
+    double v[10] = {p.t, p.u, p.n_ref.x, p.n_ref.y, p.n_ref.z, p.n_rcc.x, p.n_rcc.y, p.n_rcc.z, double(p.iters), 0.0};
+    std::memcpy(out, v, sizeof(v));
+    return 0;
+}
+
+int orc_rcc_matrix(const float* ray, int fp64, double* out /*16 column-major*/) {
+    if (fp64) {
+        Ray<double> r{mk(double(ray[0]), double(ray[1]), double(ray[2])), mk(double(ray[4]), double(ray[5]), double(ray[6]))};
+        M4<double> M = CreateRCCMatrix(r);
+        std::memcpy(out, M.m, 128);
+    } else {
+        Ray<float> r{mk(ray[0], ray[1], ray[2]), mk(ray[4], ray[5], ray[6])};
+        M4<float> M = CreateRCCMatrix(r);
+        for (int c = 0; c < 4; ++c)
+            for (int k = 0; k < 4; ++k) out[c * 4 + k] = M.m[c][k];
+    }
+    return 0;
+}
+
+int orc_inverse4(const float* m, float* out) {
+    M4<float> M;
+    std::memcpy(M.m, m, 64);
+    M4<float> R = inverse(M);
+    std::memcpy(out, R.m, 64);
+    return 0;
+}
+
+int orc_basis(const float* w, float* u, float* v) {
+    V3<float> uu, vv;
+    MakeOrthonormalBasis(uu, vv, mk(w[0], w[1], w[2]));
+    u[0] = uu.x;
+    u[1] = uu.y;
+    u[2] = uu.z;
+    v[0] = vv.x;
+    v[1] = vv.y;
+    v[2] = vv.z;
+    return 0;
+}
+
+int orc_sample_curve(const float* curve, float t, int fp64, double* p /*3*/, double* axis /*3*/) {
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        Curve<T> c{mk(T(curve[0]), T(curve[1]), T(curve[2])), mk(T(curve[4]), T(curve[5]), T(curve[6])),
+                   mk(T(curve[8]), T(curve[9]), T(curve[10])), mk(T(curve[12]), T(curve[13]), T(curve[14]))};
+        V3<T> a = SampleCurvePoint(c, T(t)), b = SampleCurveAxis(c, T(t));
+        p[0] = a.x;
+        p[1] = a.y;
+        p[2] = a.z;
+        axis[0] = b.x;
+        axis[1] = b.y;
+        axis[2] = b.z;
+    };
+    if (fp64)
+        run(0.0);
+    else
+        run(0.0f);
+    return 0;
+}
+
+int orc_cylinder(const float* ray, const float* p0, const float* p1, float radius) {
+    Ray<float> r{mk(ray[0], ray[1], ray[2]), mk(ray[4], ray[5], ray[6])};
+    return RayCylinderIntersect(r, mk(p0[0], p0[1], p0[2]), mk(p1[0], p1[1], p1[2]), radius) ? 1 : 0;
+}
+
+int orc_cone(const float* center, float radius, const float* axis, float slant, float* out /*isect, s, dt, dp, dc, sp*/) {
+    Cone<float> c{mk(center[0], center[1], center[2]), radius, mk(axis[0], axis[1], axis[2]), slant};
+    RCI<float> r = RayConeIntersectRCC(c);
+    out[0] = r.isIntersection ? 1.0f : 0.0f;
+    out[1] = r.s;
+    out[2] = r.dt;
+    out[3] = r.dp;
+    out[4] = r.dc;
+    out[5] = r.sp;
+    return 0;
+}
+
+int orc_lss(const float* ray, const float* cap, int cap_a, int fp64, double* out /*hit, t, u, n[3]*/) {
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        LssOut<T> o{};
+        bool ok = IntersectLSS<T>(mk(T(ray[0]), T(ray[1]), T(ray[2])), mk(T(ray[4]), T(ray[5]), T(ray[6])),
+                                  mk(T(cap[0]), T(cap[1]), T(cap[2])), T(cap[3]), mk(T(cap[4]), T(cap[5]), T(cap[6])),
+                                  T(cap[7]), cap_a != 0, T(ray[3]), T(ray[7]), o);
+        out[0] = ok ? 1 : 0;
+        out[1] = o.t;
+        out[2] = o.u;
+        out[3] = o.n.x;
+        out[4] = o.n.y;
+        out[5] = o.n.z;
+    };
+    if (fp64)
+        run(0.0);
+    else
+        run(0.0f);
+    return 0;
+}
+
+int orc_triangle(const float* ray, const float* v /*9*/, float* out /*hit, t, b1, b2*/) {
+    TriOut<float> o{};
+    bool ok = IntersectTriangle(mk(ray[0], ray[1], ray[2]), mk(ray[4], ray[5], ray[6]), mk(v[0], v[1], v[2]),
+                                mk(v[3], v[4], v[5]), mk(v[6], v[7], v[8]), ray[3], ray[7], o);
+    out[0] = ok ? 1.0f : 0.0f;
+    out[1] = o.t;
+    out[2] = o.b1;
+    out[3] = o.b2;
+    return 0;
+}
+
+int orc_perp_stark(const float* u, float* v) {
+    V3<float> r = PerpStark(mk(u[0], u[1], u[2]));
+    v[0] = r.x;
+    v[1] = r.y;
+    v[2] = r.z;
+    return 0;
+}
+
+int orc_shade(const float* n, float vis, float* rgb) {
+    V3<float> c = Shade(mk(n[0], n[1], n[2]), vis);
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+    return 0;
+}
+
+int orc_debug_color(int32_t prim, float* rgb) {
+    V3<float> c = GetDebugColor(prim);
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+    return 0;
+}
+
+// ---- scenes
+orc_scene* orc_scene_create() { return reinterpret_cast<orc_scene*>(new Scene()); }
+void orc_scene_destroy(orc_scene* s) { delete reinterpret_cast<Scene*>(s); }
+
+// mode 0: prims = curves (16 floats); 1: capsules (8 floats); 2: lines (8 floats) -> DOTS
+// (4 triangles per line). transform: row-major 3x4 world-from-object or NULL.
+int orc_scene_add_instance(orc_scene* sp, int mode, const float* prims, uint32_t n, int per_point, float radius,
+                           const float* transform, int leaf_size) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    auto B = std::make_shared<Blas>();
+    B->mode = mode;
+    B->per_point = per_point != 0;
+    B->radius = radius;
+    if (mode == MODE_PHANTOM) {
+        B->prims.assign(prims, prims + size_t(n) * 16);
+        B->nprims = n;
+    } else if (mode == MODE_LSS) {
+        B->prims.assign(prims, prims + size_t(n) * 8);
+        B->nprims = n;
+        B->cap_a.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* c = prims + size_t(i) * 8;
+            bool chained = i > 0 && prims[(i - 1) * 8 + 4] == c[0] && prims[(i - 1) * 8 + 5] == c[1] &&
+                           prims[(i - 1) * 8 + 6] == c[2];
+            B->cap_a[i] = chained ? 0 : 1;
+        }
+    } else if (mode == MODE_DOTS) {
+        B->pieces.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* L = prims + size_t(i) * 8;
+            B->pieces[i] = MakeDotsPiece(mk(L[0], L[1], L[2]), mk(L[4], L[5], L[6]), per_point ? L[3] : radius,
+                                         per_point ? L[7] : radius);
+        }
+        B->nprims = n * 4;
+    } else {
+        return -1;
+    }
+    build_blas(*B, leaf_size > 0 ? leaf_size : 4);
+    Instance I;
+    I.blas = B;
+    I.identity = true;
+    if (transform) {
+        std::memcpy(I.xf, transform, 48);
+        const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        I.identity = std::memcmp(transform, id, 48) == 0;
+        M4<float> M{};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) M.m[c][r] = transform[r * 4 + c];
+        M.m[0][3] = M.m[1][3] = M.m[2][3] = 0.0f;
+        M.m[3][3] = 1.0f;
+        I.inv = inverse(M);
+    }
+    S.inst.push_back(I);
+    return int(S.inst.size() - 1);
+}
+
+int orc_scene_stats(orc_scene* sp, uint32_t inst, uint32_t* out /*nprims, nnodes*/) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    if (inst >= S.inst.size()) return -1;
+    out[0] = S.inst[inst].blas->nprims;
+    out[1] = uint32_t(S.inst[inst].blas->nodes.size());
+    return 0;
+}
+
+// rays: 8 floats each (origin, tmin, dir, tmax); hits: 32-byte records.
+int orc_trace(orc_scene* sp, const float* rays, uint32_t n, void* hits, uint32_t flags, int nthreads) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    Hit* H = reinterpret_cast<Hit*>(hits);
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        float tmin, tmax;
+        Ray<float> r = load_ray(rays + size_t(i) * 8, tmin, tmax);
+        if (flags & F_FP64)
+            trace_scene<double>(S, r, tmin, tmax, flags, H[i]);
+        else
+            trace_scene<float>(S, r, tmin, tmax, flags, H[i]);
+    });
+    return 0;
+}
+
+// N*_prim: number of primitives whose (padded) AABB overlaps [tmin, t_end] where
+// t_end = hit t for hits, tmax for misses. Identity instances only.
+int orc_count_candidates(orc_scene* sp, const float* rays, const void* hits, uint32_t n, uint32_t* counts,
+                         int nthreads) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    const Hit* H = reinterpret_cast<const Hit*>(hits);
+    parallel_for(n, nthreads, [&](uint32_t i) {
+        float tmin, tmax;
+        Ray<float> r = load_ray(rays + size_t(i) * 8, tmin, tmax);
+        float tend = H[i].prim != 0xFFFFFFFFu ? H[i].t : tmax;
+        uint32_t c = 0;
+        for (uint32_t k = 0; k < S.inst.size(); ++k) {
+            std::vector<uint32_t> cand;
+            Hit dummy{};
+            bool f = false;
+            float te = tend;
+            Ray<float> o;
+            obj_ray(S.inst[k], r, o);
+            trace_blas<float>(*S.inst[k].blas, k, o, tmin, te, 0, dummy, f, &cand);
+            c += uint32_t(cand.size());
+        }
+        counts[i] = c;
+    });
+    return 0;
+}
+
+// Primary + shadow render (ray_gen.rgen + closest hit + the shadow extension).
+// flags: F_REF_NORMAL, F_POLISH. shadows: 0 = Shade(n) only, 1 = exact shadow ray.
+// pixel_stride > 1 traces every stride-th pixel (row-major index) for bounded CPU timing.
+// image: W*H*3 floats (untouched for skipped pixels); prim_ids: W*H (0xFFFFFFFF miss) or NULL.
+// Returns the number of rays traced; *seconds gets the wall time of the tracing loop.
+long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_inv[16], uint32_t W, uint32_t H,
+                     uint32_t flags, int shadows, float* image, uint32_t* prim_ids, int nthreads, uint32_t pixel_stride,
+                     double* seconds) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    M4<float> Vi, Pi;
+    std::memcpy(Vi.m, view_inv, 64);
+    std::memcpy(Pi.m, proj_inv, 64);
+    if (pixel_stride == 0) pixel_stride = 1;
+    uint32_t npix = W * H, nsample = (npix + pixel_stride - 1) / pixel_stride;
+    std::atomic<long long> nrays{0};
+    auto t0 = std::chrono::steady_clock::now();
+    parallel_for(nsample, nthreads, [&](uint32_t si) {
+        uint32_t pix = si * pixel_stride;
+        uint32_t x = pix % W, y = pix / W;
+        V3<float> o, d;
+        RayGen(Vi, Pi, x, y, W, H, o, d);
+        Ray<float> r{o, d};
+        Hit h;
+        long long nr = 1;
+        bool hit = trace_scene<float>(S, r, 0.001f, 10000.0f, flags & (F_REF_NORMAL | F_POLISH), h);
+        float rgb[3] = {0.0f, 0.0f, 0.0f};
+        if (hit) {
+            float vis = 1.0f;
+            if (shadows) {
+                V3<float> hp = o + h.t * d;
+                Ray<float> sr{hp, mk(0.0f, 1.0f, 0.0f)};
+                Hit sh;
+                ++nr;
+                if (trace_scene<float>(S, sr, 0.001f, 10000.0f, F_ANY_HIT | (flags & F_POLISH), sh)) vis = 0.0f;
+            }
+            V3<float> c = Shade(mk(h.n[0], h.n[1], h.n[2]), vis);
+            rgb[0] = c.x;
+            rgb[1] = c.y;
+            rgb[2] = c.z;
+        }
+        nrays += nr;
+        float* px = image + size_t(pix) * 3;
+        px[0] = rgb[0];
+        px[1] = rgb[1];
+        px[2] = rgb[2];
+        if (prim_ids) prim_ids[pix] = hit ? h.prim : 0xFFFFFFFFu;
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return nrays.load();
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+"""ctypes bindings to the oracle (oracle/build/liboracle.so).
+
+Test infrastructure: used by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline only. The product (vkhrt_amd) never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "build", "liboracle.so")
+
+MODE_PHANTOM, MODE_LSS, MODE_DOTS = 0, 1, 2
+F_ANY_HIT, F_BRUTE, F_FP64, F_POLISH, F_REF_NORMAL = 1, 2, 4, 8, 16
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("u", "<f4"),
+                      ("n", "<f4", (3,)), ("v", "<f4")])
+assert HIT_DTYPE.itemsize == 32
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    f32, u32, i32, u64 = ctypes.c_float, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64
+    sig = {
+        "orc_abi_version": ([], i32),
+        "orc_hardware_threads": ([], i32),
+        "orc_generate_strands": ([u64, u32, u32, f32, P, i32], i32),
+        "orc_strands_to_lines": ([P, u32, u32, P], i32),
+        "orc_generate_curves": ([P, u32, P], i32),
+        "orc_curves_to_lines": ([P, u32, u32, P], i32),
+        "orc_generate_lss": ([P, u32, f32, P], i32),
+        "orc_generate_dots": ([P, u32, f32, P], i32),
+        "orc_dots_triangles": ([P, u32, P], i32),
+        "orc_curve_aabbs": ([P, u32, i32, f32, P], i32),
+        "orc_camera": ([P, P, P, f32, f32, f32, f32, P, P], i32),
+        "orc_raygen": ([P, P, u32, u32, f32, f32, P], i32),
+        "orc_prhi": ([P, P, i32, f32, i32, i32, P], i32),
+        "orc_prhi_f64": ([P, P, i32, f32, P], i32),
+        "orc_rcc_matrix": ([P, i32, P], i32),
+        "orc_inverse4": ([P, P], i32),
+        "orc_basis": ([P, P, P], i32),
+        "orc_sample_curve": ([P, f32, i32, P, P], i32),
+        "orc_cylinder": ([P, P, P, f32], i32),
+        "orc_cone": ([P, f32, P, f32, P], i32),
+        "orc_lss": ([P, P, i32, i32, P], i32),
+        "orc_triangle": ([P, P, P], i32),
+        "orc_perp_stark": ([P, P], i32),
+        "orc_shade": ([P, f32, P], i32),
+        "orc_debug_color": ([ctypes.c_int32, P], i32),
+        "orc_scene_create": ([], P),
+        "orc_scene_destroy": ([P], None),
+        "orc_scene_add_instance": ([P, i32, P, u32, i32, f32, P, i32], i32),
+        "orc_scene_stats": ([P, u32, P], i32),
+        "orc_trace": ([P, P, u32, P, u32, i32], i32),
+        "orc_count_candidates": ([P, P, P, u32, P, i32], i32),
+        "orc_render": ([P, P, P, u32, u32, u32, i32, P, P, i32, u32, P], ctypes.c_longlong),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---------------------------------------------------------------- scene construction
+def generate_strands(seed, nstrands, nseg, sphere_radius=100.0, nthreads=0):
+    out = np.empty((nstrands, nseg + 1, 4), np.float32)
+    lib().orc_generate_strands(seed, nstrands, nseg, sphere_radius, ptr(out), nthreads)
+    return out
+
+
+def strands_to_lines(verts):
+    ns, nv, _ = verts.shape
+    lines = np.empty((ns * (nv - 1), 8), np.float32)
+    lib().orc_strands_to_lines(ptr(f32(verts)), ns, nv - 1, ptr(lines))
+    return lines
+
+
+def generate_curves(lines):
+    lines = f32(lines)
+    out = np.empty((lines.shape[0], 16), np.float32)
+    lib().orc_generate_curves(ptr(lines), lines.shape[0], ptr(out))
+    return out
+
+
+def curves_to_lines(curves, pieces):
+    curves = f32(curves)
+    out = np.empty((curves.shape[0] * pieces, 8), np.float32)
+    lib().orc_curves_to_lines(ptr(curves), curves.shape[0], pieces, ptr(out))
+    return out
+
+
+def generate_lss(lines, constant_radius=-1.0):
+    lines = f32(lines)
+    out = np.empty((lines.shape[0], 8), np.float32)
+    lib().orc_generate_lss(ptr(lines), lines.shape[0], constant_radius, ptr(out))
+    return out
+
+
+def generate_dots(lines, constant_radius=-1.0):
+    lines = f32(lines)
+    out = np.empty((lines.shape[0], 16), np.float32)
+    lib().orc_generate_dots(ptr(lines), lines.shape[0], constant_radius, ptr(out))
+    return out
+
+
+def dots_triangles(pieces):
+    pieces = f32(pieces)
+    out = np.empty((pieces.shape[0] * 4, 3, 3), np.float32)
+    lib().orc_dots_triangles(ptr(pieces), pieces.shape[0], ptr(out))
+    return out
+
+
+def curve_aabbs(curves, per_point=False, radius=0.02):
+    curves = f32(curves)
+    out = np.empty((curves.shape[0], 6), np.float32)
+    lib().orc_curve_aabbs(ptr(curves), curves.shape[0], int(per_point), radius, ptr(out))
+    return out
+
+
+def hair_curves(seed, nstrands, nseg):
+    """Seeded strands -> lines -> GenerateCurves: (nstrands*nseg, 16) float32."""
+    return generate_curves(strands_to_lines(generate_strands(seed, nstrands, nseg)))
+
+
+def camera(eye, center, up=(0.0, 1.0, 0.0), tan_half_fovy=0.5773502691896257, aspect=1.0,
+           znear=0.1, zfar=1000.0):
+    vi = np.empty(16, np.float32)
+    pi = np.empty(16, np.float32)
+    lib().orc_camera(ptr(f32(eye)), ptr(f32(center)), ptr(f32(up)), tan_half_fovy, aspect,
+                     znear, zfar, ptr(vi), ptr(pi))
+    return vi, pi
+
+
+def raygen(view_inv, proj_inv, W, H, tmin=0.001, tmax=10000.0):
+    rays = np.empty((H * W, 8), np.float32)
+    lib().orc_raygen(ptr(f32(view_inv)), ptr(f32(proj_inv)), W, H, tmin, tmax, ptr(rays))
+    return rays
+
+
+class Scene:
+    """CPU scene: instances of BLASes, traced with a binned-SAH BVH2 or brute force."""
+
+    def __init__(self):
+        self.h = lib().orc_scene_create()
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def add(self, mode, prims, per_point=False, radius=0.02, transform=None, leaf_size=4):
+        prims = f32(prims)
+        xf = None if transform is None else f32(np.asarray(transform).reshape(12))
+        r = lib().orc_scene_add_instance(self.h, mode, ptr(prims), prims.shape[0], int(per_point),
+                                         radius, ptr(xf), leaf_size)
+        assert r >= 0
+        return r
+
+    def trace(self, rays, flags=0, nthreads=0):
+        rays = f32(rays)
+        hits = np.zeros(rays.shape[0], HIT_DTYPE)
+        lib().orc_trace(self.h, ptr(rays), rays.shape[0], ptr(hits), flags, nthreads)
+        return hits
+
+    def count_candidates(self, rays, hits, nthreads=0):
+        rays = f32(rays)
+        counts = np.zeros(rays.shape[0], np.uint32)
+        lib().orc_count_candidates(self.h, ptr(rays), ptr(hits), rays.shape[0], ptr(counts), nthreads)
+        return counts
+
+    def render(self, view_inv, proj_inv, W, H, flags=0, shadows=1, nthreads=0, pixel_stride=1):
+        img = np.zeros((H, W, 3), np.float32)
+        ids = np.full((H, W), 0xFFFFFFFF, np.uint32)
+        secs = ctypes.c_double(0)
+        nrays = lib().orc_render(self.h, ptr(f32(view_inv)), ptr(f32(proj_inv)), W, H, flags,
+                                 shadows, ptr(img), ptr(ids), nthreads, pixel_stride,
+                                 ctypes.byref(secs))
+        return img, ids, int(nrays), secs.value
+
+
+def prhi(ray, curve, per_point=False, radius=0.02, polish=False, fp64=False):
+    out = np.zeros(10, np.float32)
+    lib().orc_prhi(ptr(f32(ray)), ptr(f32(curve)), int(per_point), radius, int(polish),
+                   int(fp64), ptr(out))
+    return out
+
+
+def prhi_f64(ray, curve, per_point=False, radius=0.02):
+    out = np.zeros(10, np.float64)
+    lib().orc_prhi_f64(ptr(f32(ray)), ptr(f32(curve)), int(per_point), radius, ptr(out))
+    return out

@@ -1,0 +1,46 @@
+"""Seeded test scenes and ray sets (small enough for the oracle to finish in seconds)."""
+import numpy as np
+
+import oracle_lib as O
+
+C0_SEED = 20240601
+
+
+def c0_curves(nstrands=1000, nseg=16, seed=C0_SEED):
+    """Config 0 hair: 1,000 strands x 16 cubic segments (49 control points per strand)."""
+    return O.hair_curves(seed, nstrands, nseg)
+
+
+def c0_camera(oracle_cam=True, W=256, H=256):
+    return O.camera((0.0, 50.0, 400.0), (0.0, 30.0, 0.0), aspect=W / H)
+
+
+def targeted_rays(curves, n, seed, spread=0.06, dist=(5.0, 400.0)):
+    """Rays aimed at points near random curve samples from random directions, so most rays
+    hit or graze hair. Returns (n, 8) float32 rays with tmin 0.001, tmax 10000."""
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, curves.shape[0], n)
+    t = rng.random(n).astype(np.float32)
+    c = curves[idx].reshape(n, 4, 4)[:, :, :3].astype(np.float64)
+    u = 1.0 - t
+    w = np.stack([u ** 3, 3 * u * u * t, 3 * u * t * t, t ** 3], axis=1)
+    p = np.einsum("nk,nkj->nj", w, c)
+    p += rng.normal(size=(n, 3)) * spread
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    L = rng.uniform(dist[0], dist[1], n)[:, None]
+    o = p - d * L
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 0.001
+    rays[:, 4:7] = d
+    rays[:, 7] = 10000.0
+    # re-normalize in float32 (unit directions as the reference's rays)
+    dd = rays[:, 4:7].astype(np.float32)
+    dd = dd / np.sqrt((dd * dd).sum(1, keepdims=True))
+    rays[:, 4:7] = dd
+    return rays
+
+
+def lines_from_curves(curves, pieces=4):
+    return O.curves_to_lines(curves, pieces)

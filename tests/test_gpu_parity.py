@@ -1,0 +1,174 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle on seeded inputs.
+
+Integer / index results (hit/miss, primitive id, instance id) must agree exactly; the FP32
+intersector arithmetic follows the same IEEE operation order on both sides, so t, u and
+normals are compared bit-for-bit as well (tolerance 0).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def G():
+    import gpu_util
+    return gpu_util
+
+
+@pytest.fixture(scope="module")
+def curves():
+    import scenes
+    return scenes.c0_curves()
+
+
+def _compare_hits(h, ref, what, exact=True):
+    miss_g = h["prim"] == 0xFFFFFFFF
+    miss_r = ref["prim"] == 0xFFFFFFFF
+    agree = (h["prim"] == ref["prim"]) & (h["inst"] == ref["inst"])
+    frac = agree.mean()
+    print(f"{what}: {len(h)} rays, hits {(~miss_r).sum()}, primId agreement {frac:.6f}")
+    assert (miss_g == miss_r).all(), f"{what}: hit/miss differs on {(miss_g != miss_r).sum()} rays"
+    assert agree.all(), f"{what}: primitive id differs on {(~agree).sum()} rays"
+    hit = ~miss_r
+    if exact:
+        assert np.array_equal(h["t"][hit].view(np.uint32), ref["t"][hit].view(np.uint32)), f"{what}: t differs"
+        assert np.array_equal(h["u"][hit].view(np.uint32), ref["u"][hit].view(np.uint32)), f"{what}: u differs"
+        assert np.array_equal(h["n"][hit].view(np.uint32), ref["n"][hit].view(np.uint32)), f"{what}: normal differs"
+
+
+@pytest.mark.parametrize("per_point", [False, True])
+def test_phantom_trace_targeted(gpu_ctx, G, oracle, curves, per_point):
+    import scenes
+    import vkhrt_amd as V
+    rays = scenes.targeted_rays(curves, 20000, seed=7 + per_point)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=per_point)
+    ref = S.trace(rays)
+    blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=per_point)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    h = G.trace(gpu_ctx, tlas, rays)
+    _compare_hits(h, ref, f"phantom per_point={per_point}")
+    # shadow (any-hit) form: visibility must agree exactly
+    hs = G.trace(gpu_ctx, tlas, rays, flags=V.TRACE_TERMINATE_ON_FIRST_HIT)
+    rs = S.trace(rays, flags=oracle.F_ANY_HIT)
+    assert np.array_equal(hs["prim"] == 0xFFFFFFFF, rs["prim"] == 0xFFFFFFFF)
+
+
+def test_phantom_render_c0(gpu_ctx, G, oracle, curves):
+    import vkhrt_amd as V
+    W = H = 256
+    cam = V.camera_look_at((0.0, 50.0, 400.0), (0.0, 30.0, 0.0), fov_deg=60.0, aspect=1.0)
+    vi, pi = G.cam_arrays(cam)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    ref_img, ref_ids, _, _ = S.render(vi, pi, W, H, shadows=1)
+    blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    img, ids = G.render(gpu_ctx, tlas, cam, W, H)
+    assert np.array_equal(ids, ref_ids)
+    p = G.psnr(img[..., :3], ref_img)
+    print("C0 render PSNR", p)
+    assert p >= 50.0
+    assert np.array_equal(img[..., :3].view(np.uint32), ref_img.view(np.uint32))
+
+
+def test_geometry_front_end(gpu_ctx, G, oracle):
+    """Seeded strands, GenerateLines, GenerateCurves, cubic->lines, LSS and DOTS on the GPU
+    are bit-identical to the oracle."""
+    import vkhrt_amd as V
+    ns, nseg, seed = 300, 16, 99
+    verts = torch.zeros(ns * (nseg + 1) * 4, dtype=torch.float32, device="cuda")
+    gpu_ctx.generate_strands(seed, ns, nseg, verts.data_ptr(), stream=G.stream_ptr())
+    lines = torch.zeros(ns * nseg * 8, dtype=torch.float32, device="cuda")
+    gpu_ctx.generate_lines(verts.data_ptr(), ns, nseg, lines.data_ptr(), stream=G.stream_ptr())
+    curves = torch.zeros(ns * nseg * 16, dtype=torch.float32, device="cuda")
+    gpu_ctx.generate_curves(lines.data_ptr(), ns * nseg, curves.data_ptr(), stream=G.stream_ptr())
+    pl = torch.zeros(ns * nseg * 4 * 8, dtype=torch.float32, device="cuda")
+    gpu_ctx.curves_to_lines(curves.data_ptr(), ns * nseg, 4, pl.data_ptr(), stream=G.stream_ptr())
+    caps = torch.zeros(ns * nseg * 4 * 8, dtype=torch.float32, device="cuda")
+    gpu_ctx.generate_lss(pl.data_ptr(), ns * nseg * 4, caps.data_ptr(), stream=G.stream_ptr())
+    dots = torch.zeros(ns * nseg * 4 * 16, dtype=torch.float32, device="cuda")
+    gpu_ctx.generate_dots(pl.data_ptr(), ns * nseg * 4, dots.data_ptr(), stream=G.stream_ptr())
+    torch.cuda.synchronize()
+
+    rv = oracle.generate_strands(seed, ns, nseg)
+    rl = oracle.strands_to_lines(rv)
+    rc = oracle.generate_curves(rl)
+    rpl = oracle.curves_to_lines(rc, 4)
+    rcaps = oracle.generate_lss(rpl)
+    rdots = oracle.generate_dots(rpl)
+    for name, g, r in [("strands", verts, rv), ("lines", lines, rl), ("curves", curves, rc),
+                       ("pieces", pl, rpl), ("lss", caps, rcaps), ("dots", dots, rdots)]:
+        gh = g.cpu().numpy().reshape(r.shape)
+        assert np.array_equal(gh.view(np.uint32), r.view(np.uint32)), name
+
+
+@pytest.mark.parametrize("mode", ["lss", "dots"])
+def test_lss_dots_trace(gpu_ctx, G, oracle, curves, mode):
+    import scenes
+    import vkhrt_amd as V
+    lines = scenes.lines_from_curves(curves[:4000], 4)
+    rays = scenes.targeted_rays(curves[:4000], 20000, seed=11)
+    S = oracle.Scene()
+    if mode == "lss":
+        caps = oracle.generate_lss(lines)
+        S.add(oracle.MODE_LSS, caps)
+        blas = G.build_blas(gpu_ctx, V.GEOMETRY_LSS, caps)
+    else:
+        S.add(oracle.MODE_DOTS, lines, per_point=True)
+        blas = G.build_blas(gpu_ctx, V.GEOMETRY_DOTS, lines, per_point=True)
+    ref = S.trace(rays)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    h = G.trace(gpu_ctx, tlas, rays)
+    _compare_hits(h, ref, mode)
+
+
+def test_instances_with_transforms(gpu_ctx, G, oracle, curves):
+    """Several instances (identity and rigid transforms): closest hit across instances with
+    the (instance, primitive) tie rule."""
+    import scenes
+    import vkhrt_amd as V
+    xf1 = np.array([1, 0, 0, 40.0, 0, 1, 0, -10.0, 0, 0, 1, 5.0], np.float32)
+    c, s = np.cos(0.3), np.sin(0.3)
+    xf2 = np.array([c, 0, s, -30.0, 0, 1, 0, 0.0, -s, 0, c, 0.0], np.float32)
+    sub = curves[:3000]
+    rays = scenes.targeted_rays(sub, 8000, seed=5, dist=(50.0, 300.0))
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, sub, per_point=True)
+    S.add(oracle.MODE_PHANTOM, sub, per_point=True, transform=xf1)
+    S.add(oracle.MODE_PHANTOM, sub, per_point=True, transform=xf2)
+    ref = S.trace(rays)
+    b0 = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, sub, per_point=True)
+    b1 = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, sub, per_point=True, transform=xf1)
+    b2 = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, sub, per_point=True, transform=xf2)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [b0, b1, b2])
+    h = G.trace(gpu_ctx, tlas, rays)
+    _compare_hits(h, ref, "instances")
+
+
+def test_edge_cases(gpu_ctx, G, oracle, curves):
+    """Empty BLAS, a single primitive, axis-aligned directions, tmin > tmax."""
+    import vkhrt_amd as V
+    empty = V.BottomLevelAccelerationStructure(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, None, 0)
+    one = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves[:1], per_point=True)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [empty, one])
+    c = curves[0].reshape(4, 4)
+    mid = (c[0, :3] + c[3, :3]) * 0.5
+    rays = []
+    for axis in range(3):
+        for sgn in (1.0, -1.0):
+            d = np.zeros(3, np.float32)
+            d[axis] = sgn
+            o = mid - d * 50.0
+            rays.append(np.r_[o, 0.001, d, 10000.0])
+    rays.append(np.r_[mid - np.array([0, 0, 50.0]), 20.0, np.array([0, 0, 1.0]), 10.0])  # empty interval
+    rays = np.array(rays, np.float32)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, np.zeros((0, 16), np.float32), per_point=True)
+    S.add(oracle.MODE_PHANTOM, curves[:1], per_point=True)
+    ref = S.trace(rays)
+    h = G.trace(gpu_ctx, tlas, rays)
+    _compare_hits(h, ref, "edge")

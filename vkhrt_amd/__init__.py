@@ -1,0 +1,281 @@
+"""vkhrt_amd: MI355X-native hair ray-tracing core behind a C-ABI (include/vkhrt.h).
+
+This module binds libvkhrt.so with ctypes and mirrors the reference's host interface:
+``BottomLevelAccelerationStructure`` (include/bottom_level_acceleration_structure.hpp:31-50),
+``TopLevelAccelerationStructure`` (include/top_level_acceleration_structure.hpp:9-23) and the
+``traceRaysKHR`` dispatch (source/renderer.cpp:156-166) as ``Context.render``.
+
+Device buffers are passed as integer device addresses (e.g. ``torch.Tensor.data_ptr()``)
+and streams as ``hipStream_t`` integers (``torch.cuda.Stream.cuda_stream``). There is no
+CPU fallback: every call goes to the HIP library, and loading fails loudly if it is absent.
+"""
+import ctypes
+import math
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvkhrt.so")
+
+GEOMETRY_PHANTOM_CURVES, GEOMETRY_LSS, GEOMETRY_DOTS = 0, 1, 2
+
+BLAS_HOST_POINTERS = 1 << 0
+BLAS_PER_POINT_RADIUS = 1 << 1
+BLAS_HAS_TRANSFORM = 1 << 2
+
+TRACE_TERMINATE_ON_FIRST_HIT = 1 << 0
+TRACE_FP64_POLISH = 1 << 3
+
+RENDER_SHADOWS = 1 << 0
+RENDER_DEBUG_PALETTE = 1 << 1
+RENDER_RGBA8 = 1 << 2
+RENDER_FP64_POLISH = 1 << 3
+
+OPTION_STACK_KIND = 1
+OPTION_LEAF_SIZE = 2
+
+STATUS = {0: "SUCCESS", -1: "INVALID_ARGUMENT", -2: "OUT_OF_MEMORY", -3: "DEVICE",
+          -4: "UNSUPPORTED", -5: "STACK_OVERFLOW", -6: "NO_DEVICE"}
+
+# Every symbol declared in include/vkhrt.h.
+EXPORTS = [
+    "vkhrt_abi_version", "vkhrt_context_create", "vkhrt_context_destroy", "vkhrt_last_error",
+    "vkhrt_synchronize", "vkhrt_set_option", "vkhrt_blas_build", "vkhrt_blas_info_get",
+    "vkhrt_blas_destroy", "vkhrt_tlas_build", "vkhrt_tlas_destroy", "vkhrt_trace_rays",
+    "vkhrt_render", "vkhrt_camera_look_at", "vkhrt_generate_strands", "vkhrt_generate_lines",
+    "vkhrt_generate_curves", "vkhrt_curves_to_lines", "vkhrt_generate_lss", "vkhrt_generate_dots",
+    "vkhrt_profiling_enable", "vkhrt_kernel_stats", "vkhrt_status_counters",
+]
+
+
+class VkhrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vkhrt {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class BlasDesc(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("primitives", ctypes.c_void_p), ("count", ctypes.c_uint32),
+                ("radius", ctypes.c_float), ("material_index", ctypes.c_uint32),
+                ("leaf_size", ctypes.c_uint32), ("transform", ctypes.c_float * 12)]
+
+
+class BlasInfo(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_uint32), ("primitive_count", ctypes.c_uint32),
+                ("leaf_records", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
+                ("depth", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64),
+                ("build_ms", ctypes.c_double)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("view_inverse", ctypes.c_float * 16), ("proj_inverse", ctypes.c_float * 16)]
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("image", ctypes.c_void_p), ("primitive_ids", ctypes.c_void_p),
+                ("light_visibility", ctypes.c_void_p)]
+
+
+class StrandParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("sphere_radius", "step", "gravity", "curl", "r_root", "r_tip")]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libvkhrt.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() or "
+                          "python vkhrt_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P, u32, i32, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float
+    sigs = {
+        "vkhrt_abi_version": ([], i32),
+        "vkhrt_context_create": ([i32, ctypes.POINTER(P)], i32),
+        "vkhrt_context_destroy": ([P], None),
+        "vkhrt_last_error": ([P], ctypes.c_char_p),
+        "vkhrt_synchronize": ([P, P], i32),
+        "vkhrt_set_option": ([P, i32, i32], i32),
+        "vkhrt_blas_build": ([P, ctypes.POINTER(BlasDesc), P, ctypes.POINTER(P)], i32),
+        "vkhrt_blas_info_get": ([P, ctypes.POINTER(BlasInfo)], i32),
+        "vkhrt_blas_destroy": ([P], None),
+        "vkhrt_tlas_build": ([P, ctypes.POINTER(P), u32, P, ctypes.POINTER(P)], i32),
+        "vkhrt_tlas_destroy": ([P], None),
+        "vkhrt_trace_rays": ([P, P, P, u32, P, u32, P], i32),
+        "vkhrt_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderDesc), P], i32),
+        "vkhrt_camera_look_at": ([P, P, P, f32, f32, f32, f32, ctypes.POINTER(Camera)], i32),
+        "vkhrt_generate_strands": ([P, ctypes.c_uint64, u32, u32, ctypes.POINTER(StrandParams), P, P], i32),
+        "vkhrt_generate_lines": ([P, P, u32, u32, P, P], i32),
+        "vkhrt_generate_curves": ([P, P, u32, f32, P, P], i32),
+        "vkhrt_curves_to_lines": ([P, P, u32, u32, P, P], i32),
+        "vkhrt_generate_lss": ([P, P, u32, f32, P, P], i32),
+        "vkhrt_generate_dots": ([P, P, u32, f32, P, P], i32),
+        "vkhrt_profiling_enable": ([P, i32], i32),
+        "vkhrt_kernel_stats": ([P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(u32)], i32),
+        "vkhrt_status_counters": ([P, ctypes.POINTER(u32)], i32),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def camera_look_at(eye, center, up=(0.0, 1.0, 0.0), fov_deg=60.0, aspect=1.0, znear=0.1,
+                   zfar=1000.0):
+    """FlyCamera view/projection (fly_camera.cpp:25-35) inverted like renderer.cpp:189-195."""
+    cam = Camera()
+    rc = lib().vkhrt_camera_look_at(_f3(eye), _f3(center), _f3(up),
+                                    math.tan(math.radians(fov_deg) * 0.5), aspect, znear, zfar,
+                                    ctypes.byref(cam))
+    if rc:
+        raise VkhrtError(rc, "camera_look_at")
+    return cam
+
+
+class Context:
+    """One device context (the analogue of VulkanContext for this path)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().vkhrt_context_create(device, ctypes.byref(h))
+        if rc:
+            raise VkhrtError(rc, "context_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vkhrt_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, rc, what=""):
+        if rc:
+            msg = lib().vkhrt_last_error(self.h)
+            raise VkhrtError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_option(self, option, value):
+        self.check(lib().vkhrt_set_option(self.h, option, value), "set_option")
+
+    def synchronize(self, stream=None):
+        self.check(lib().vkhrt_synchronize(self.h, stream), "synchronize")
+
+    def trace_rays(self, tlas, rays_ptr, count, hits_ptr, flags=0, stream=None):
+        """traceRayEXT over `count` vkhrt_ray records -> vkhrt_hit records (device)."""
+        self.check(lib().vkhrt_trace_rays(self.h, tlas.h, rays_ptr, count, hits_ptr, flags, stream),
+                   "trace_rays")
+
+    def render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS,
+               prim_ids_ptr=None, light_visibility_ptr=None, stream=None):
+        """traceRaysKHR(W, H, 1) with ray_gen.rgen and the closest-hit shaders."""
+        d = RenderDesc(width, height, flags, image_ptr, prim_ids_ptr, light_visibility_ptr)
+        self.check(lib().vkhrt_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), stream),
+                   "render")
+
+    # geometry front-end
+    def generate_strands(self, seed, nstrands, nseg, vertices_ptr, params=None, stream=None):
+        p = None if params is None else ctypes.byref(StrandParams(*params))
+        self.check(lib().vkhrt_generate_strands(self.h, seed, nstrands, nseg, p, vertices_ptr, stream),
+                   "generate_strands")
+
+    def generate_lines(self, vertices_ptr, nstrands, nseg, lines_ptr, stream=None):
+        self.check(lib().vkhrt_generate_lines(self.h, vertices_ptr, nstrands, nseg, lines_ptr, stream),
+                   "generate_lines")
+
+    def generate_curves(self, lines_ptr, count, curves_ptr, tension=1.0, stream=None):
+        self.check(lib().vkhrt_generate_curves(self.h, lines_ptr, count, tension, curves_ptr, stream),
+                   "generate_curves")
+
+    def curves_to_lines(self, curves_ptr, count, pieces, lines_ptr, stream=None):
+        self.check(lib().vkhrt_curves_to_lines(self.h, curves_ptr, count, pieces, lines_ptr, stream),
+                   "curves_to_lines")
+
+    def generate_lss(self, lines_ptr, count, caps_ptr, constant_radius=-1.0, stream=None):
+        self.check(lib().vkhrt_generate_lss(self.h, lines_ptr, count, constant_radius, caps_ptr, stream),
+                   "generate_lss")
+
+    def generate_dots(self, lines_ptr, count, pieces_ptr, constant_radius=-1.0, stream=None):
+        self.check(lib().vkhrt_generate_dots(self.h, lines_ptr, count, constant_radius, pieces_ptr,
+                                             stream), "generate_dots")
+
+    # profiling
+    def profiling(self, enable):
+        self.check(lib().vkhrt_profiling_enable(self.h, int(enable)), "profiling")
+
+    def kernel_stats(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_uint32()
+        self.check(lib().vkhrt_kernel_stats(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+                   "kernel_stats")
+        return ms.value, n.value
+
+    def stack_overflows(self):
+        n = ctypes.c_uint32()
+        self.check(lib().vkhrt_status_counters(self.h, ctypes.byref(n)), "status")
+        return n.value
+
+
+class BottomLevelAccelerationStructure:
+    """BLAS over hair primitives (bottom_level_acceleration_structure.hpp:34).
+
+    `primitives` is a device address (or a host address with BLAS_HOST_POINTERS), laid out per
+    include/vkhrt.h. `transform` is a row-major 3x4 world-from-object matrix (BLASInput.transform).
+    """
+
+    def __init__(self, ctx, mode, primitives_ptr, count, radius=0.02, flags=0, transform=None,
+                 leaf_size=0, material_index=0, stream=None):
+        d = BlasDesc()
+        d.mode, d.flags, d.primitives, d.count = mode, flags, primitives_ptr, count
+        d.radius, d.material_index, d.leaf_size = radius, material_index, leaf_size
+        if transform is not None:
+            d.flags |= BLAS_HAS_TRANSFORM
+            for i, v in enumerate(list(transform)[:12]):
+                d.transform[i] = float(v)
+        h = ctypes.c_void_p()
+        ctx.check(lib().vkhrt_blas_build(ctx.h, ctypes.byref(d), stream, ctypes.byref(h)), "blas_build")
+        self.ctx, self.h, self.mode = ctx, h, mode
+
+    def info(self):
+        i = BlasInfo()
+        lib().vkhrt_blas_info_get(self.h, ctypes.byref(i))
+        return {k: getattr(i, k) for k, _ in BlasInfo._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vkhrt_blas_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class TopLevelAccelerationStructure:
+    """TLAS: instance i = blases[i], customIndex = i (top_level_acceleration_structure.cpp:23-43)."""
+
+    def __init__(self, ctx, blases, stream=None):
+        arr = (ctypes.c_void_p * max(1, len(blases)))(*[b.h.value for b in blases])
+        h = ctypes.c_void_p()
+        ctx.check(lib().vkhrt_tlas_build(ctx.h, arr, len(blases), stream, ctypes.byref(h)), "tlas_build")
+        self.ctx, self.h, self.blases = ctx, h, list(blases)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vkhrt_tlas_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,666 @@
+// C-ABI of the vkhrt hair path (include/vkhrt.h): contexts, BLAS/TLAS build, tracing,
+// rendering, geometry front-end and profiling. Host C++ over hand-written HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vkhrt.h"
+#include "build.hpp"
+#include "geometry.hpp"
+#include "internal.hpp"
+
+namespace {
+
+struct PendingTiming {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct vkhrt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int stack_kind = 1;
+    int leaf_size = 1;
+    bool profiling = false;
+    std::vector<PendingTiming> pending;
+    std::map<std::string, std::pair<double, uint32_t>> stats;
+    uint32_t* status = nullptr;  // device counters
+    // render scratch
+    float4* hitbuf = nullptr;
+    uint32_t* queue = nullptr;
+    uint32_t* counters = nullptr;
+    size_t cap_pixels = 0;
+};
+
+struct vkhrt_blas {
+    vkhrt_context* ctx = nullptr;
+    uint32_t mode = 0;
+    uint32_t radius_mode = 0;
+    float radius = 0.02f;
+    uint32_t material_index = 0;
+    bool has_xf = false;
+    float xf[12];
+    vk::BuiltBvh bvh;
+    uint32_t records = 0;
+    float lo[3], hi[3];  // object-space bounds
+    vkhrt_blas_info info{};
+};
+
+struct vkhrt_tlas {
+    vkhrt_context* ctx = nullptr;
+    vk::InstanceDev* dev = nullptr;
+    uint32_t count = 0;
+    int variant = vk::kVarMixed;  // kernel variant without FP64 polish
+};
+
+namespace {
+// Kernel variant for a TLAS and the polish flag.
+int trace_variant(const vkhrt_tlas* t, bool polish) {
+    int v = t->variant;
+    if (polish && (v == vk::kVarPhConst || v == vk::kVarPhPoint)) ++v;
+    return v;
+}
+}  // namespace
+
+namespace {
+
+int fail(vkhrt_context* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(vkhrt_context* c, hipError_t e, const char* what) {
+    int code = e == hipErrorOutOfMemory ? VKHRT_ERROR_OUT_OF_MEMORY : VKHRT_ERROR_DEVICE;
+    return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+hipStream_t pick(vkhrt_context* c, void* s) { return s ? reinterpret_cast<hipStream_t>(s) : c->stream; }
+
+struct Timer {
+    vkhrt_context* c;
+    hipStream_t s;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer(vkhrt_context* c_, hipStream_t s_, const char* n) : c(c_), s(s_), name(n) {
+        if (c->profiling) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, s);
+        }
+    }
+    void stop() {
+        if (a) {
+            hipEventRecord(b, s);
+            c->pending.push_back(PendingTiming{name, a, b});
+            a = b = nullptr;
+        }
+    }
+    ~Timer() { stop(); }
+};
+
+void resolve_timings(vkhrt_context* c) {
+    for (auto& p : c->pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        auto& s = c->stats[p.name];
+        s.first += ms;
+        s.second += 1;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
+
+// ---- host 4x4 helpers (column-major m[c*4 + r], GLSL/glm conventions)
+void inverse4(const float* m, float* out) {
+#define A_(r, c) m[(c)*4 + (r)]
+    float s0 = A_(0, 0) * A_(1, 1) - A_(1, 0) * A_(0, 1);
+    float s1 = A_(0, 0) * A_(1, 2) - A_(1, 0) * A_(0, 2);
+    float s2 = A_(0, 0) * A_(1, 3) - A_(1, 0) * A_(0, 3);
+    float s3 = A_(0, 1) * A_(1, 2) - A_(1, 1) * A_(0, 2);
+    float s4 = A_(0, 1) * A_(1, 3) - A_(1, 1) * A_(0, 3);
+    float s5 = A_(0, 2) * A_(1, 3) - A_(1, 2) * A_(0, 3);
+    float c5 = A_(2, 2) * A_(3, 3) - A_(3, 2) * A_(2, 3);
+    float c4 = A_(2, 1) * A_(3, 3) - A_(3, 1) * A_(2, 3);
+    float c3 = A_(2, 1) * A_(3, 2) - A_(3, 1) * A_(2, 2);
+    float c2 = A_(2, 0) * A_(3, 3) - A_(3, 0) * A_(2, 3);
+    float c1 = A_(2, 0) * A_(3, 2) - A_(3, 0) * A_(2, 2);
+    float c0 = A_(2, 0) * A_(3, 1) - A_(3, 0) * A_(2, 1);
+    float det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+    float id = 1.0f / det;
+    float B[4][4];
+    B[0][0] = (A_(1, 1) * c5 - A_(1, 2) * c4 + A_(1, 3) * c3) * id;
+    B[0][1] = (-A_(0, 1) * c5 + A_(0, 2) * c4 - A_(0, 3) * c3) * id;
+    B[0][2] = (A_(3, 1) * s5 - A_(3, 2) * s4 + A_(3, 3) * s3) * id;
+    B[0][3] = (-A_(2, 1) * s5 + A_(2, 2) * s4 - A_(2, 3) * s3) * id;
+    B[1][0] = (-A_(1, 0) * c5 + A_(1, 2) * c2 - A_(1, 3) * c1) * id;
+    B[1][1] = (A_(0, 0) * c5 - A_(0, 2) * c2 + A_(0, 3) * c1) * id;
+    B[1][2] = (-A_(3, 0) * s5 + A_(3, 2) * s2 - A_(3, 3) * s1) * id;
+    B[1][3] = (A_(2, 0) * s5 - A_(2, 2) * s2 + A_(2, 3) * s1) * id;
+    B[2][0] = (A_(1, 0) * c4 - A_(1, 1) * c2 + A_(1, 3) * c0) * id;
+    B[2][1] = (-A_(0, 0) * c4 + A_(0, 1) * c2 - A_(0, 3) * c0) * id;
+    B[2][2] = (A_(3, 0) * s4 - A_(3, 1) * s2 + A_(3, 3) * s0) * id;
+    B[2][3] = (-A_(2, 0) * s4 + A_(2, 1) * s2 - A_(2, 3) * s0) * id;
+    B[3][0] = (-A_(1, 0) * c3 + A_(1, 1) * c1 - A_(1, 2) * c0) * id;
+    B[3][1] = (A_(0, 0) * c3 - A_(0, 1) * c1 + A_(0, 2) * c0) * id;
+    B[3][2] = (-A_(3, 0) * s3 + A_(3, 1) * s1 - A_(3, 2) * s0) * id;
+    B[3][3] = (A_(2, 0) * s3 - A_(2, 1) * s1 + A_(2, 2) * s0) * id;
+#undef A_
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[c * 4 + r] = B[r][c];
+}
+
+struct H3 {
+    float x, y, z;
+};
+H3 h_sub(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+float h_dot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+H3 h_cross(H3 a, H3 b) { return H3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+H3 h_norm(H3 a) {
+    float k = 1.0f / std::sqrt(h_dot(a, a));
+    return H3{a.x * k, a.y * k, a.z * k};
+}
+
+// Decodes the root node's child boxes into object-space bounds.
+void root_bounds(const vk::WideNode& N, float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = INFINITY;
+        hi[k] = -INFINITY;
+    }
+    const float org[3] = {N.px, N.py, N.pz};
+    const uint32_t* ql[3] = {N.qlo_x, N.qlo_y, N.qlo_z};
+    const uint32_t* qh[3] = {N.qhi_x, N.qhi_y, N.qhi_z};
+    for (int s = 0; s < 8; ++s) {
+        uint32_t meta = (N.meta[s >> 2] >> (8 * (s & 3))) & 0xffu;
+        if (!meta) continue;
+        for (int k = 0; k < 3; ++k) {
+            uint32_t e = (N.e_imask >> (8 * k)) & 0xffu;
+            float sc;
+            uint32_t bits = e << 23;
+            std::memcpy(&sc, &bits, 4);
+            float a = float((ql[k][s >> 2] >> (8 * (s & 3))) & 0xffu);
+            float b = float((qh[k][s >> 2] >> (8 * (s & 3))) & 0xffu);
+            lo[k] = std::min(lo[k], org[k] + a * sc);
+            hi[k] = std::max(hi[k], org[k] + b * sc);
+        }
+    }
+    for (int k = 0; k < 3; ++k) {  // absorb decode rounding
+        float m = std::max(std::fabs(lo[k]), std::fabs(hi[k]));
+        lo[k] -= 1e-4f + 1e-6f * m;
+        hi[k] += 1e-4f + 1e-6f * m;
+    }
+}
+
+int ensure_render_scratch(vkhrt_context* c, size_t npix) {
+    if (npix <= c->cap_pixels) return VKHRT_SUCCESS;
+    hipFree(c->hitbuf);
+    hipFree(c->queue);
+    c->hitbuf = nullptr;
+    c->queue = nullptr;
+    c->cap_pixels = 0;
+    hipError_t e = hipMalloc(&c->hitbuf, npix * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&c->queue, npix * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_fail(c, e, "render scratch");
+    c->cap_pixels = npix;
+    return VKHRT_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vkhrt_abi_version(void) { return VKHRT_ABI_VERSION; }
+
+int vkhrt_context_create(int device, vkhrt_context** out) {
+    if (!out) return VKHRT_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return VKHRT_ERROR_NO_DEVICE;
+    if (device < 0 || device >= n) return VKHRT_ERROR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return VKHRT_ERROR_DEVICE;
+    vkhrt_context* c = new (std::nothrow) vkhrt_context();
+    if (!c) return VKHRT_ERROR_OUT_OF_MEMORY;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->status, 16);
+    if (e == hipSuccess) e = hipMalloc(&c->counters, 16);
+    if (e == hipSuccess) e = hipMemset(c->status, 0, 16);
+    if (e != hipSuccess) {
+        vkhrt_context_destroy(c);
+        return VKHRT_ERROR_DEVICE;
+    }
+    *out = c;
+    return VKHRT_SUCCESS;
+}
+
+void vkhrt_context_destroy(vkhrt_context* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    hipFree(c->status);
+    hipFree(c->counters);
+    hipFree(c->hitbuf);
+    hipFree(c->queue);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* vkhrt_last_error(const vkhrt_context* c) { return c ? c->err.c_str() : "null context"; }
+
+int vkhrt_synchronize(vkhrt_context* c, void* stream) {
+    if (!c) return VKHRT_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(pick(c, stream));
+    if (e != hipSuccess) return hip_fail(c, e, "synchronize");
+    uint32_t st[4];
+    e = hipMemcpy(st, c->status, 16, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "status readback");
+    if (st[0]) {
+        hipMemset(c->status, 0, 16);
+        return fail(c, VKHRT_ERROR_STACK_OVERFLOW,
+                    "traversal stack overflow on " + std::to_string(st[0]) + " rays (results incomplete)");
+    }
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_set_option(vkhrt_context* c, int option, int value) {
+    if (!c) return VKHRT_ERROR_INVALID_ARGUMENT;
+    if (option == VKHRT_OPTION_STACK_KIND) {
+        if (value < 0 || value > 2) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "stack kind must be 0..2");
+        c->stack_kind = value;
+        return VKHRT_SUCCESS;
+    }
+    if (option == VKHRT_OPTION_LEAF_SIZE) {
+        if (value < 1 || value > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "leaf size must be 1..3");
+        c->leaf_size = value;
+        return VKHRT_SUCCESS;
+    }
+    return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown option");
+}
+
+int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, vkhrt_blas** out) {
+    if (!c || !d || !out) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (d->mode > VKHRT_GEOMETRY_DOTS) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown geometry mode");
+    if (d->count && !d->primitives) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null primitives");
+    if (d->count > (1u << 28)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "too many primitives");
+    if (d->leaf_size > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "leaf size must be 0..3");
+    const bool per_point = (d->flags & VKHRT_BLAS_PER_POINT_RADIUS) != 0;
+    if (!per_point && !(d->radius >= 0.0f)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "radius must be >= 0");
+    hipSetDevice(c->device);
+    hipStream_t st = pick(c, stream);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, st);
+    const uint32_t in_floats = d->mode == VKHRT_GEOMETRY_PHANTOM_CURVES ? 16u : 8u;
+    const size_t in_bytes = size_t(d->count) * in_floats * sizeof(float);
+    float4* input = nullptr;
+    float4* records = nullptr;
+    bool own_input = false;
+    vkhrt_blas* b = new (std::nothrow) vkhrt_blas();
+    if (!b) return fail(c, VKHRT_ERROR_OUT_OF_MEMORY, "blas");
+    b->ctx = c;
+    b->mode = d->mode;
+    b->radius_mode = per_point ? 1u : 0u;
+    b->radius = d->radius;
+    b->material_index = d->material_index;
+    b->has_xf = (d->flags & VKHRT_BLAS_HAS_TRANSFORM) != 0;
+    std::memcpy(b->xf, d->transform, sizeof(b->xf));
+    b->records = d->count;
+    hipError_t e = hipSuccess;
+    std::string err;
+    int rc = VKHRT_SUCCESS;
+    if (d->count) {
+        if (d->flags & VKHRT_BLAS_HOST_POINTERS) {
+            e = hipMalloc(&input, in_bytes);
+            if (e == hipSuccess) e = hipMemcpyAsync(input, d->primitives, in_bytes, hipMemcpyHostToDevice, st);
+            own_input = true;
+        } else {
+            input = reinterpret_cast<float4*>(const_cast<float*>(d->primitives));
+        }
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "blas upload");
+            goto out;
+        }
+        if (d->mode == VKHRT_GEOMETRY_PHANTOM_CURVES) {
+            records = input;
+        } else if (d->mode == VKHRT_GEOMETRY_LSS) {
+            e = hipMalloc(&records, size_t(d->count) * 2 * sizeof(float4));
+            if (e == hipSuccess) e = vk::lss_chain(input, d->count, records, st);
+        } else {
+            e = hipMalloc(&records, size_t(d->count) * 4 * sizeof(float4));
+            if (e == hipSuccess) e = vk::gen_dots(input, d->count, per_point ? -1.0f : d->radius, records, st);
+        }
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "blas records");
+            goto out;
+        }
+        {
+            Timer tm(c, st, "blas_build");
+            e = vk::build_bvh(int(d->mode), records, d->count, int(b->radius_mode), d->radius,
+                              d->leaf_size ? int(d->leaf_size) : c->leaf_size, st, &b->bvh, err);
+        }
+        if (e != hipSuccess) {
+            rc = fail(c, e == hipErrorOutOfMemory ? VKHRT_ERROR_OUT_OF_MEMORY : VKHRT_ERROR_DEVICE, "bvh build: " + err);
+            goto out;
+        }
+        vk::WideNode root;
+        e = hipMemcpyAsync(&root, b->bvh.nodes, sizeof(root), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "root readback");
+            goto out;
+        }
+        root_bounds(root, b->lo, b->hi);
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            b->lo[k] = INFINITY;
+            b->hi[k] = -INFINITY;
+        }
+    }
+    hipEventRecord(e1, st);
+    hipEventSynchronize(e1);
+    {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        b->info.build_ms = ms;
+    }
+    b->info.mode = d->mode;
+    b->info.leaf_records = d->count;
+    b->info.primitive_count = d->mode == VKHRT_GEOMETRY_DOTS ? d->count * 4u : d->count;
+    b->info.node_count = b->bvh.nnodes;
+    b->info.depth = b->bvh.depth;
+    b->info.device_bytes = uint64_t(b->bvh.nnodes) * sizeof(vk::WideNode) + uint64_t(d->count) * 4u +
+                           uint64_t(d->count) * (d->mode == VKHRT_GEOMETRY_LSS ? 32u : 64u);
+out:
+    if (own_input) hipFree(input);
+    if (records && records != input) hipFree(records);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != VKHRT_SUCCESS) {
+        vkhrt_blas_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_blas_info_get(const vkhrt_blas* b, vkhrt_blas_info* out) {
+    if (!b || !out) return VKHRT_ERROR_INVALID_ARGUMENT;
+    *out = b->info;
+    return VKHRT_SUCCESS;
+}
+
+void vkhrt_blas_destroy(vkhrt_blas* b) {
+    if (!b) return;
+    hipFree(b->bvh.nodes);
+    hipFree(b->bvh.prim_ids);
+    hipFree(b->bvh.prims);
+    delete b;
+}
+
+int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t count, void* stream,
+                     vkhrt_tlas** out) {
+    if (!c || !out || (count && !blases)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    std::vector<vk::InstanceDev> inst(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const vkhrt_blas* b = blases[i];
+        if (!b) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null blas");
+        vk::InstanceDev& I = inst[i];
+        std::memset(&I, 0, sizeof(I));
+        I.blas.prims = b->bvh.prims;
+        I.blas.prim_ids = b->bvh.prim_ids;
+        I.blas.nodes = b->bvh.nodes;
+        I.blas.nprims = b->records;
+        I.blas.nnodes = b->bvh.nnodes;
+        I.blas.mode = b->mode;
+        I.blas.radius_mode = b->radius_mode;
+        I.blas.radius = b->radius;
+        const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        const float* xf = b->has_xf ? b->xf : ident;
+        std::memcpy(I.xf, xf, sizeof(I.xf));
+        I.identity = std::memcmp(xf, ident, sizeof(ident)) == 0 ? 1u : 0u;
+        float M[16] = {xf[0], xf[4], xf[8], 0, xf[1], xf[5], xf[9], 0, xf[2], xf[6], xf[10], 0, xf[3], xf[7], xf[11], 1};
+        inverse4(M, I.inv);
+        // world bounds of the transformed object box
+        for (int k = 0; k < 3; ++k) {
+            I.lo[k] = INFINITY;
+            I.hi[k] = -INFINITY;
+        }
+        if (b->records) {
+            for (int corner = 0; corner < 8; ++corner) {
+                float p[3] = {(corner & 1) ? b->hi[0] : b->lo[0], (corner & 2) ? b->hi[1] : b->lo[1],
+                              (corner & 4) ? b->hi[2] : b->lo[2]};
+                for (int r = 0; r < 3; ++r) {
+                    float w = xf[r * 4 + 0] * p[0] + xf[r * 4 + 1] * p[1] + xf[r * 4 + 2] * p[2] + xf[r * 4 + 3];
+                    I.lo[r] = std::min(I.lo[r], w);
+                    I.hi[r] = std::max(I.hi[r], w);
+                }
+            }
+            if (!I.identity)
+                for (int r = 0; r < 3; ++r) {
+                    float m = std::max(std::fabs(I.lo[r]), std::fabs(I.hi[r]));
+                    I.lo[r] -= 1e-4f + 1e-5f * m;
+                    I.hi[r] += 1e-4f + 1e-5f * m;
+                }
+        }
+    }
+    vkhrt_tlas* t = new (std::nothrow) vkhrt_tlas();
+    if (!t) return fail(c, VKHRT_ERROR_OUT_OF_MEMORY, "tlas");
+    t->ctx = c;
+    t->count = count;
+    {
+        int v = -1;
+        for (uint32_t i = 0; i < count; ++i) {
+            const vkhrt_blas* b = blases[i];
+            int vi = b->mode == VKHRT_GEOMETRY_LSS    ? vk::kVarLss
+                     : b->mode == VKHRT_GEOMETRY_DOTS ? vk::kVarDots
+                     : b->radius_mode                 ? vk::kVarPhPoint
+                                                      : vk::kVarPhConst;
+            v = (v < 0 || v == vi) ? vi : vk::kVarMixed;
+        }
+        t->variant = v < 0 ? vk::kVarPhConst : v;
+    }
+    if (count) {
+        hipStream_t st = pick(c, stream);
+        hipError_t e = hipMalloc(&t->dev, sizeof(vk::InstanceDev) * count);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(t->dev, inst.data(), sizeof(vk::InstanceDev) * count, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            vkhrt_tlas_destroy(t);
+            return hip_fail(c, e, "tlas upload");
+        }
+    }
+    *out = t;
+    return VKHRT_SUCCESS;
+}
+
+void vkhrt_tlas_destroy(vkhrt_tlas* t) {
+    if (!t) return;
+    hipFree(t->dev);
+    delete t;
+}
+
+int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits,
+                     uint32_t flags, void* stream) {
+    if (!c || !t) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (count && (!rays || !hits)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null rays or hits");
+    if (flags & ~(VKHRT_TRACE_TERMINATE_ON_FIRST_HIT | VKHRT_TRACE_FP64_POLISH))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown trace flags");
+    hipStream_t st = pick(c, stream);
+    vk::TraceParams tp{t->dev, t->count, flags, c->status};
+    Timer tm(c, st, "trace_rays");
+    hipError_t e = vk::launch_trace_rays(tp, reinterpret_cast<const float*>(rays), count,
+                                         reinterpret_cast<vk::HitRec*>(hits),
+                                         trace_variant(t, (flags & VKHRT_TRACE_FP64_POLISH) != 0), c->stack_kind, st);
+    tm.stop();
+    if (e != hipSuccess) return hip_fail(c, e, "trace launch");
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam, const vkhrt_render_desc* d,
+                 void* stream) {
+    if (!c || !t || !cam || !d) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!d->image) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null image");
+    if (d->width == 0 || d->height == 0 || uint64_t(d->width) * d->height > (1ull << 28))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad image size");
+    const uint32_t known = VKHRT_RENDER_SHADOWS | VKHRT_RENDER_DEBUG_PALETTE | VKHRT_RENDER_RGBA8 | VKHRT_RENDER_FP64_POLISH;
+    if (d->flags & ~known) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown render flags");
+    const size_t npix = size_t(d->width) * d->height;
+    int rc = ensure_render_scratch(c, npix);
+    if (rc != VKHRT_SUCCESS) return rc;
+    hipStream_t st = pick(c, stream);
+    vk::RenderParams rp{};
+    rp.tp = vk::TraceParams{t->dev, t->count, (d->flags & VKHRT_RENDER_FP64_POLISH) ? uint32_t(vk::kTraceFp64Polish) : 0u,
+                            c->status};
+    std::memcpy(rp.cam.view_inv, cam->view_inverse, 64);
+    std::memcpy(rp.cam.proj_inv, cam->proj_inverse, 64);
+    rp.cam.width = d->width;
+    rp.cam.height = d->height;
+    rp.shadows = (d->flags & VKHRT_RENDER_SHADOWS) ? 1u : 0u;
+    rp.out_format = (d->flags & VKHRT_RENDER_RGBA8) ? 1u : 0u;
+    rp.debug_palette = (d->flags & VKHRT_RENDER_DEBUG_PALETTE) ? 1u : 0u;
+    rp.image = d->image;
+    rp.prim_out = d->primitive_ids;
+    rp.hitbuf = c->hitbuf;
+    rp.queue = c->queue;
+    rp.counters = c->counters;
+    rp.light_vis = const_cast<float*>(d->light_visibility);
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (c->profiling)
+        for (auto& x : ev) hipEventCreate(&x);
+    hipError_t e = vk::launch_render(rp, trace_variant(t, (d->flags & VKHRT_RENDER_FP64_POLISH) != 0), c->stack_kind,
+                                     st, c->profiling ? ev : nullptr);
+    if (c->profiling) {
+        c->pending.push_back(PendingTiming{"render_primary", ev[0], ev[1]});
+        c->pending.push_back(PendingTiming{"render_shadow", ev[2], ev[3]});
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "render launch");
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_camera_look_at(const float eye[3], const float center[3], const float up[3], float tan_half_fovy,
+                         float aspect, float znear, float zfar, vkhrt_camera* out) {
+    if (!eye || !center || !up || !out) return VKHRT_ERROR_INVALID_ARGUMENT;
+    if (!(tan_half_fovy > 0.0f) || !(aspect > 0.0f) || !(zfar != znear)) return VKHRT_ERROR_INVALID_ARGUMENT;
+    // glm::lookAtRH
+    H3 E{eye[0], eye[1], eye[2]}, C{center[0], center[1], center[2]}, U{up[0], up[1], up[2]};
+    H3 f = h_norm(h_sub(C, E));
+    H3 s = h_norm(h_cross(f, U));
+    H3 u = h_cross(s, f);
+    float V[16] = {0};
+    V[0] = s.x;
+    V[4] = s.y;
+    V[8] = s.z;
+    V[1] = u.x;
+    V[5] = u.y;
+    V[9] = u.z;
+    V[2] = -f.x;
+    V[6] = -f.y;
+    V[10] = -f.z;
+    V[12] = -h_dot(s, E);
+    V[13] = -h_dot(u, E);
+    V[14] = h_dot(f, E);
+    V[15] = 1.0f;
+    // glm::perspectiveRH_ZO, then projection[1][1] *= -1 (fly_camera.cpp:32-33)
+    float P[16] = {0};
+    P[0] = 1.0f / (aspect * tan_half_fovy);
+    P[5] = 1.0f / tan_half_fovy;
+    P[10] = zfar / (znear - zfar);
+    P[11] = -1.0f;
+    P[14] = -(zfar * znear) / (zfar - znear);
+    P[5] *= -1.0f;
+    inverse4(V, out->view_inverse);
+    inverse4(P, out->proj_inverse);
+    return VKHRT_SUCCESS;
+}
+
+// ---- geometry front-end
+int vkhrt_generate_strands(vkhrt_context* c, uint64_t seed, uint32_t nstrands, uint32_t nseg,
+                           const vkhrt_strand_params* p, float* vertices, void* stream) {
+    if (!c || (nstrands && !vertices) || nseg == 0) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    vk::StrandParams hp{100.0f, 1.5f, 0.3f, 0.2f, 0.04f, 0.01f};
+    if (p) hp = vk::StrandParams{p->sphere_radius, p->step, p->gravity, p->curl, p->r_root, p->r_tip};
+    hipError_t e = vk::gen_strands(seed, nstrands, nseg, hp, reinterpret_cast<float4*>(vertices), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "generate strands");
+}
+int vkhrt_generate_lines(vkhrt_context* c, const float* v, uint32_t nstrands, uint32_t nseg, float* lines,
+                         void* stream) {
+    if (!c || (nstrands && (!v || !lines))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    hipError_t e = vk::gen_strand_lines(reinterpret_cast<const float4*>(v), nstrands, nseg,
+                                        reinterpret_cast<float4*>(lines), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "generate lines");
+}
+int vkhrt_generate_curves(vkhrt_context* c, const float* lines, uint32_t n, float tension, float* curves,
+                          void* stream) {
+    if (!c || (n && (!lines || !curves))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    hipError_t e = vk::gen_curves(reinterpret_cast<const float4*>(lines), n, tension, reinterpret_cast<float4*>(curves),
+                                  pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "generate curves");
+}
+int vkhrt_curves_to_lines(vkhrt_context* c, const float* curves, uint32_t n, uint32_t pieces, float* lines,
+                          void* stream) {
+    if (!c || pieces == 0 || (n && (!curves || !lines))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    hipError_t e = vk::gen_curve_lines(reinterpret_cast<const float4*>(curves), n, pieces,
+                                       reinterpret_cast<float4*>(lines), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "curves to lines");
+}
+int vkhrt_generate_lss(vkhrt_context* c, const float* lines, uint32_t n, float constant_radius, float* caps,
+                       void* stream) {
+    if (!c || (n && (!lines || !caps))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    hipError_t e = vk::gen_lss(reinterpret_cast<const float4*>(lines), n, constant_radius,
+                               reinterpret_cast<float4*>(caps), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "generate lss");
+}
+int vkhrt_generate_dots(vkhrt_context* c, const float* lines, uint32_t n, float constant_radius, float* pieces,
+                        void* stream) {
+    if (!c || (n && (!lines || !pieces))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    hipError_t e = vk::gen_dots(reinterpret_cast<const float4*>(lines), n, constant_radius,
+                                reinterpret_cast<float4*>(pieces), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "generate dots");
+}
+
+// ---- profiling
+int vkhrt_profiling_enable(vkhrt_context* c, int enable) {
+    if (!c) return VKHRT_ERROR_INVALID_ARGUMENT;
+    resolve_timings(c);
+    c->profiling = enable != 0;
+    c->stats.clear();
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_kernel_stats(vkhrt_context* c, const char* name, double* total_ms, uint32_t* launches) {
+    if (!c || !name) return VKHRT_ERROR_INVALID_ARGUMENT;
+    resolve_timings(c);
+    auto it = c->stats.find(name);
+    if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == c->stats.end() ? 0u : it->second.second;
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_status_counters(vkhrt_context* c, uint32_t* overflows) {
+    if (!c || !overflows) return VKHRT_ERROR_INVALID_ARGUMENT;
+    uint32_t st[4];
+    hipError_t e = hipMemcpy(st, c->status, 16, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "status readback");
+    *overflows = st[0];
+    return VKHRT_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// Host-side entry points of the device code (C++ linkage, used by api.cpp only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "internal.hpp"
+
+namespace vk {
+
+struct BuiltBvh {
+    WideNode* nodes = nullptr;
+    uint32_t nnodes = 0;
+    uint32_t* prim_ids = nullptr;
+    float4* prims = nullptr;  // primitive records in leaf order
+    uint32_t depth = 0;       // wide levels
+};
+
+// Builds the 8-wide BVH over `n` primitive records (device pointer) on stream `st`.
+hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mode, float radius, int leaf_max,
+                     hipStream_t st, BuiltBvh* out, std::string& err);
+
+struct TraceParams {
+    const InstanceDev* inst;
+    uint32_t ninst;
+    uint32_t flags;
+    uint32_t* status;  // [0] stack overflow count
+};
+
+struct Camera {
+    float view_inv[16];
+    float proj_inv[16];
+    uint32_t width, height;
+};
+
+struct RenderParams {
+    TraceParams tp;
+    Camera cam;
+    uint32_t shadows;     // 0: Shade(n) only, 1: exact shadow ray
+    uint32_t out_format;  // 0: RGBA32F, 1: RGBA8
+    uint32_t debug_palette;
+    void* image;
+    uint32_t* prim_out;   // optional per-pixel primitive id
+    float4* hitbuf;       // per-pixel (n.xyz, t) scratch
+    uint32_t* queue;      // shadow queue (pixel indices)
+    uint32_t* counters;   // [0] queue length
+    float* light_vis;     // optional per-pixel visibility override (neural form), may be null
+};
+
+// Compile-time geometry variants of the traversal kernels. A TLAS whose instances share
+// one mode uses the matching specialized kernels; mixed TLASes use kVarMixed.
+enum TraceVariant {
+    kVarPhConst = 0,
+    kVarPhConstPolish = 1,
+    kVarPhPoint = 2,
+    kVarPhPointPolish = 3,
+    kVarLss = 4,
+    kVarDots = 5,
+    kVarMixed = 6
+};
+
+hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, int variant,
+                             int stack_kind, hipStream_t st);
+hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hipStream_t st,
+                         hipEvent_t* ev /*[4] or null*/);
+
+template <class G, int STACK>
+hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, hipStream_t st);
+template <class G, int STACK>
+hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEvent_t* ev);
+
+}  // namespace vk

@@ -1,0 +1,7 @@
+// Traversal kernels for the dots geometry variant(s) (instantiations of trace_impl.hpp).
+#include "trace_impl.hpp"
+
+namespace vk {
+using Variant0 = Geo<kDots, 0, false>;
+VKHRT_INSTANTIATE_TRACE(Variant0)
+}  // namespace vk

@@ -1,0 +1,9 @@
+// Traversal kernels for the phantom point geometry variant(s) (instantiations of trace_impl.hpp).
+#include "trace_impl.hpp"
+
+namespace vk {
+using Variant0 = Geo<kPhantom, kRadiusPerPoint, false>;
+VKHRT_INSTANTIATE_TRACE(Variant0)
+using Variant1 = Geo<kPhantom, kRadiusPerPoint, true>;
+VKHRT_INSTANTIATE_TRACE(Variant1)
+}  // namespace vk
