@@ -25,8 +25,8 @@
  * Conventions: all functions return a vkhrt_status (0 = success, negative = error, no
  * aborts, no exceptions across the ABI); vkhrt_last_error() describes the last failure of
  * a context. Buffers are device pointers unless a flag says otherwise. `stream` is a
- * hipStream_t (NULL = the context's own stream). A context is not internally locked:
- * use one per host thread.
+ * hipStream_t (NULL = the HIP null stream). A context is not internally locked: use one
+ * per host thread.
  */
 #ifndef VKHRT_H
 #define VKHRT_H
@@ -208,6 +208,22 @@ int vkhrt_generate_lss(vkhrt_context* ctx, const float* lines, uint32_t count, f
  * start, r0, end, r1, s, 0, t, 0). constant_radius < 0 keeps per-end radii. */
 int vkhrt_generate_dots(vkhrt_context* ctx, const float* lines, uint32_t count, float constant_radius, float* pieces,
                         void* stream);
+
+/* ---- auxiliary ray work */
+/* ray_gen.rgen as a standalone generator: width * height rays (row-major launch ids). */
+int vkhrt_generate_camera_rays(vkhrt_context* ctx, const vkhrt_camera* camera, uint32_t width, uint32_t height,
+                               float tmin, float tmax, vkhrt_ray* rays, void* stream);
+/* One ray toward `direction` from every hit (origin = o + t d); compacted. `count_out` and
+ * `source` (index of the source ray, may be NULL) are device pointers. */
+int vkhrt_generate_shadow_rays(vkhrt_context* ctx, const vkhrt_ray* rays, const vkhrt_hit* hits, uint32_t count,
+                               const float direction[3], float tmin, float tmax, vkhrt_ray* out, uint32_t* source,
+                               uint32_t* count_out, void* stream);
+/* Per ray, the number of primitives whose exact bounding box (GenerateAABBs,
+ * geometry_processor.cpp:422-436) overlaps [tmin, t_end], t_end = hit t or tmax on a miss. */
+int vkhrt_count_candidates(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* rays, const vkhrt_hit* hits,
+                           uint32_t count, uint32_t* counts, void* stream);
+/* Rays of the last vkhrt_render on this context (synchronizes its stream). */
+int vkhrt_render_ray_counts(vkhrt_context* ctx, uint64_t* primary, uint64_t* shadow);
 
 /* ---- profiling: per-kernel HIP-event timing on the launch stream */
 int vkhrt_profiling_enable(vkhrt_context* ctx, int enable);

@@ -102,7 +102,7 @@ float pad_for(const Box& b) {
     return 1e-4f + 4e-7f * m;
 }
 
-void prim_box(const Blas& B, uint32_t i, Box& bx) {
+void prim_box(const Blas& B, uint32_t i, Box& bx, bool pad = true) {
     if (B.mode == MODE_PHANTOM) {
         CurveAABB(&B.prims[size_t(i) * 16], B.per_point, B.radius, bx.mn, bx.mx);
     } else if (B.mode == MODE_LSS) {
@@ -121,6 +121,7 @@ void prim_box(const Blas& B, uint32_t i, Box& bx) {
             bx.mx[k] = std::max(a, std::max(b, c));
         }
     }
+    if (!pad) return;
     float p = pad_for(bx);
     for (int k = 0; k < 3; ++k) {
         bx.mn[k] -= p;
@@ -363,7 +364,9 @@ uint32_t trace_blas(const Blas& B, uint32_t inst_id, const Ray<float>& r, float 
             for (uint32_t i = N.left; i < N.left + N.count; ++i) {
                 uint32_t p = B.order[i];
                 if (cand) {
-                    if (slab(B.boxes[p], r, inv, tmin, tmax)) cand->push_back(p);
+                    Box exact;
+                    prim_box(B, p, exact, false);  // GenerateAABBs box, no pad
+                    if (slab(exact, r, inv, tmin, tmax)) cand->push_back(p);
                     continue;
                 }
                 Hit h{};
@@ -886,7 +889,7 @@ int orc_trace(orc_scene* sp, const float* rays, uint32_t n, void* hits, uint32_t
     return 0;
 }
 
-// N*_prim: number of primitives whose (padded) AABB overlaps [tmin, t_end] where
+// N*_prim: number of primitives whose exact (unpadded) box overlaps [tmin, t_end] where
 // t_end = hit t for hits, tmax for misses. Identity instances only.
 int orc_count_candidates(orc_scene* sp, const float* rays, const void* hits, uint32_t n, uint32_t* counts,
                          int nthreads) {

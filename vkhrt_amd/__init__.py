@@ -44,6 +44,8 @@ EXPORTS = [
     "vkhrt_render", "vkhrt_camera_look_at", "vkhrt_generate_strands", "vkhrt_generate_lines",
     "vkhrt_generate_curves", "vkhrt_curves_to_lines", "vkhrt_generate_lss", "vkhrt_generate_dots",
     "vkhrt_profiling_enable", "vkhrt_kernel_stats", "vkhrt_status_counters",
+    "vkhrt_generate_camera_rays", "vkhrt_generate_shadow_rays", "vkhrt_count_candidates",
+    "vkhrt_render_ray_counts",
 ]
 
 
@@ -120,6 +122,10 @@ def lib():
         "vkhrt_kernel_stats": ([P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(u32)], i32),
         "vkhrt_status_counters": ([P, ctypes.POINTER(u32)], i32),
+        "vkhrt_generate_camera_rays": ([P, ctypes.POINTER(Camera), u32, u32, f32, f32, P, P], i32),
+        "vkhrt_generate_shadow_rays": ([P, P, P, u32, P, f32, f32, P, P, P, P], i32),
+        "vkhrt_count_candidates": ([P, P, P, P, u32, P, P], i32),
+        "vkhrt_render_ray_counts": ([P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -211,6 +217,29 @@ class Context:
     def generate_dots(self, lines_ptr, count, pieces_ptr, constant_radius=-1.0, stream=None):
         self.check(lib().vkhrt_generate_dots(self.h, lines_ptr, count, constant_radius, pieces_ptr,
                                              stream), "generate_dots")
+
+    # auxiliary ray work
+    def generate_camera_rays(self, camera, width, height, rays_ptr, tmin=0.001, tmax=10000.0,
+                             stream=None):
+        self.check(lib().vkhrt_generate_camera_rays(self.h, ctypes.byref(camera), width, height, tmin,
+                                                    tmax, rays_ptr, stream), "generate_camera_rays")
+
+    def generate_shadow_rays(self, rays_ptr, hits_ptr, count, out_ptr, count_out_ptr,
+                             direction=(0.0, 1.0, 0.0), tmin=0.001, tmax=10000.0, source_ptr=None,
+                             stream=None):
+        self.check(lib().vkhrt_generate_shadow_rays(self.h, rays_ptr, hits_ptr, count, _f3(direction),
+                                                    tmin, tmax, out_ptr, source_ptr, count_out_ptr,
+                                                    stream), "generate_shadow_rays")
+
+    def count_candidates(self, tlas, rays_ptr, hits_ptr, count, counts_ptr, stream=None):
+        self.check(lib().vkhrt_count_candidates(self.h, tlas.h, rays_ptr, hits_ptr, count, counts_ptr,
+                                                stream), "count_candidates")
+
+    def render_ray_counts(self):
+        p, s = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(lib().vkhrt_render_ray_counts(self.h, ctypes.byref(p), ctypes.byref(s)),
+                   "render_ray_counts")
+        return p.value, s.value
 
     # profiling
     def profiling(self, enable):

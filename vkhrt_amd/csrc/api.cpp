@@ -39,6 +39,9 @@ struct vkhrt_context {
     uint32_t* queue = nullptr;
     uint32_t* counters = nullptr;
     size_t cap_pixels = 0;
+    uint64_t last_pixels = 0;
+    bool last_shadows = false;
+    hipStream_t last_stream = nullptr;
 };
 
 struct vkhrt_blas {
@@ -83,7 +86,8 @@ int hip_fail(vkhrt_context* c, hipError_t e, const char* what) {
     return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-hipStream_t pick(vkhrt_context* c, void* s) { return s ? reinterpret_cast<hipStream_t>(s) : c->stream; }
+// NULL selects HIP's null stream, as for any HIP API.
+hipStream_t pick(vkhrt_context*, void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 struct Timer {
     vkhrt_context* c;
@@ -552,7 +556,56 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
         c->pending.push_back(PendingTiming{"render_shadow", ev[2], ev[3]});
     }
     if (e != hipSuccess) return hip_fail(c, e, "render launch");
+    c->last_pixels = npix;
+    c->last_shadows = rp.shadows && !rp.debug_palette && !d->light_visibility;
+    c->last_stream = st;
     return VKHRT_SUCCESS;
+}
+
+int vkhrt_render_ray_counts(vkhrt_context* c, uint64_t* primary, uint64_t* shadow) {
+    if (!c || !primary || !shadow) return VKHRT_ERROR_INVALID_ARGUMENT;
+    *primary = c->last_pixels;
+    *shadow = 0;
+    if (c->last_pixels && c->last_shadows) {
+        uint32_t q = 0;
+        hipError_t e = hipStreamSynchronize(c->last_stream);
+        if (e == hipSuccess) e = hipMemcpy(&q, c->counters, 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(c, e, "render counts");
+        *shadow = q;
+    }
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_generate_camera_rays(vkhrt_context* c, const vkhrt_camera* cam, uint32_t width, uint32_t height, float tmin,
+                               float tmax, vkhrt_ray* rays, void* stream) {
+    if (!c || !cam || (uint64_t(width) * height && !rays)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    vk::Camera cm{};
+    std::memcpy(cm.view_inv, cam->view_inverse, 64);
+    std::memcpy(cm.proj_inv, cam->proj_inverse, 64);
+    cm.width = width;
+    cm.height = height;
+    hipError_t e = vk::launch_camera_rays(cm, tmin, tmax, reinterpret_cast<float*>(rays), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "camera rays");
+}
+
+int vkhrt_generate_shadow_rays(vkhrt_context* c, const vkhrt_ray* rays, const vkhrt_hit* hits, uint32_t count,
+                               const float direction[3], float tmin, float tmax, vkhrt_ray* out, uint32_t* source,
+                               uint32_t* count_out, void* stream) {
+    if (!c || !direction || !count_out || (count && (!rays || !hits || !out)))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    hipStream_t st = pick(c, stream);
+    hipError_t e = vk::launch_shadow_rays(reinterpret_cast<const float*>(rays), reinterpret_cast<const vk::HitRec*>(hits), count,
+                                   direction, tmin, tmax, reinterpret_cast<float*>(out), source, count_out, st);
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "shadow rays");
+}
+
+int vkhrt_count_candidates(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, const vkhrt_hit* hits,
+                           uint32_t count, uint32_t* counts, void* stream) {
+    if (!c || !t || (count && (!rays || !hits || !counts))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    vk::TraceParams tp{t->dev, t->count, 0u, c->status};
+    hipError_t e = vk::launch_count_candidates(tp, reinterpret_cast<const float*>(rays),
+                                               reinterpret_cast<const vk::HitRec*>(hits), count, counts, pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "count candidates");
 }
 
 int vkhrt_camera_look_at(const float eye[3], const float center[3], const float up[3], float tan_half_fovy,

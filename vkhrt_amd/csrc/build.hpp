@@ -65,6 +65,12 @@ hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t 
 hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hipStream_t st,
                          hipEvent_t* ev /*[4] or null*/);
 
+hipError_t launch_camera_rays(const Camera& cam, float tmin, float tmax, float* rays, hipStream_t st);
+hipError_t launch_shadow_rays(const float* rays, const HitRec* hits, uint32_t n, const float dir[3], float tmin,
+                              float tmax, float* out, uint32_t* src, uint32_t* counter, hipStream_t st);
+hipError_t launch_count_candidates(const TraceParams& tp, const float* rays, const HitRec* hits, uint32_t n,
+                                   uint32_t* counts, hipStream_t st);
+
 template <class G, int STACK>
 hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, hipStream_t st);
 template <class G, int STACK>

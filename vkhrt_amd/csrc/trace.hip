@@ -69,4 +69,205 @@ hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hi
     }
 }
 
+// ------------------------------------------------------------------ auxiliary kernels
+namespace {
+
+using trace_detail_pub::primary_ray_pub;
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ray_gen.rgen:18-28 as a standalone ray generator: one vkhrt_ray per launch id.
+__global__ void k_camera_rays(Camera cam, float tmin, float tmax, float4* __restrict__ rays) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= cam.width * cam.height) return;
+    f3 o, d;
+    primary_ray_pub(cam, pix % cam.width, pix / cam.width, o, d);
+    rays[size_t(pix) * 2] = make_float4(o.x, o.y, o.z, tmin);
+    rays[size_t(pix) * 2 + 1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+// Shadow rays toward `dir` from every hit: origin = o + t d (FP32, as the render kernels).
+// Compacted with wave-aggregated atomics; src[i] records the source ray of each output.
+__global__ void k_shadow_rays(const float4* __restrict__ rays, const HitRec* __restrict__ hits, uint32_t n, float3 dir,
+                              float tmin, float tmax, float4* __restrict__ out, uint32_t* __restrict__ src,
+                              uint32_t* __restrict__ counter) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0;
+    float t = 0.0f;
+    if (i < n) {
+        hit = hits[i].prim_id != 0xFFFFFFFFu;
+        r0 = rays[size_t(i) * 2];
+        r1 = rays[size_t(i) * 2 + 1];
+        t = hits[i].t;
+    }
+    const uint64_t m = __ballot(hit);
+    if (!m) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
+    base = __shfl(base, int(leader));
+    if (hit) {
+        const uint32_t k = base + __popcll(m & ((1ull << lane) - 1ull));
+        out[size_t(k) * 2] = make_float4(r0.x + t * r1.x, r0.y + t * r1.y, r0.z + t * r1.z, tmin);
+        out[size_t(k) * 2 + 1] = make_float4(dir.x, dir.y, dir.z, tmax);
+        if (src) src[k] = i;
+    }
+}
+
+// Exact GenerateAABBs-style box of leaf record `slot` (no pad). DOTS: triangle `tri`.
+__device__ __forceinline__ void exact_box(const BlasDev& B, uint32_t slot, int tri, float lo[3], float hi[3]) {
+    if (B.mode == kPhantom) {
+        const float4* c = B.prims + size_t(slot) * 4;
+        const float4 p0 = c[0], p1 = c[1], p2 = c[2], p3 = c[3];
+        float r = B.radius;
+        if (B.radius_mode == kRadiusPerPoint) r = gmax(gmax(p0.w, p1.w), gmax(p2.w, p3.w));
+        const float xs[4] = {p0.x, p1.x, p2.x, p3.x}, ys[4] = {p0.y, p1.y, p2.y, p3.y}, zs[4] = {p0.z, p1.z, p2.z, p3.z};
+        const float* a[3] = {xs, ys, zs};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = gmin(gmin(a[k][0], a[k][3]), gmin(a[k][1], a[k][2])) - r;
+            hi[k] = gmax(gmax(a[k][0], a[k][3]), gmax(a[k][1], a[k][2])) + r;
+        }
+    } else if (B.mode == kLss) {
+        const float4 a = B.prims[size_t(slot) * 2], b = B.prims[size_t(slot) * 2 + 1];
+        const float ra = fabsf(a.w), rb = b.w;
+        const float ax[3] = {a.x, a.y, a.z}, bx[3] = {b.x, b.y, b.z};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = gmin(ax[k] - ra, bx[k] - rb);
+            hi[k] = gmax(ax[k] + ra, bx[k] + rb);
+        }
+    } else {
+        const float4* pr = B.prims + size_t(slot) * 4;
+        float4 P[4] = {pr[0], pr[1], pr[2], pr[3]};
+        f3 v0, v1, v2;
+        dots_tri(P, tri >> 1, tri & 1, v0, v1, v2);
+        const float a[3] = {v0.x, v0.y, v0.z}, b[3] = {v1.x, v1.y, v1.z}, c[3] = {v2.x, v2.y, v2.z};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = gmin(a[k], gmin(b[k], c[k]));
+            hi[k] = gmax(a[k], gmax(b[k], c[k]));
+        }
+    }
+}
+
+// Slab test with explicit NaN handling (origin on a slab plane with a zero direction
+// component): same operation order as the oracle's candidate count.
+__device__ __forceinline__ bool slab_exact(const float lo[3], const float hi[3], f3 o, f3 inv, float tmin, float tmax) {
+    float t0 = tmin, t1 = tmax;
+    const float oo[3] = {o.x, o.y, o.z}, iv[3] = {inv.x, inv.y, inv.z};
+    for (int k = 0; k < 3; ++k) {
+        float a = (lo[k] - oo[k]) * iv[k];
+        float c = (hi[k] - oo[k]) * iv[k];
+        if (isnan(a) || isnan(c)) {
+            a = -INFINITY;
+            c = INFINITY;
+        }
+        const float l = c < a ? c : a, h = a < c ? c : a;
+        t0 = t0 < l ? l : t0;
+        const float hh = h * (1.0f + 4e-7f);
+        t1 = hh < t1 ? hh : t1;
+    }
+    return t0 <= t1;
+}
+
+// N*_prim: primitives whose exact box overlaps [tmin, t_end] (t_end = hit t, or tmax on a
+// miss), summed over instances. Visits the BVH without intersection tests.
+__global__ void k_count_candidates(TraceParams tp, const float4* __restrict__ rays, const HitRec* __restrict__ hits,
+                                   uint32_t n, uint32_t* __restrict__ counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = rays[size_t(i) * 2], r1 = rays[size_t(i) * 2 + 1];
+    const float tend = hits[i].prim_id != 0xFFFFFFFFu ? hits[i].t : r1.w;
+    uint32_t count = 0;
+    for (uint32_t in = 0; in < tp.ninst; ++in) {
+        const InstanceDev& I = tp.inst[in];
+        const BlasDev& B = I.blas;
+        if (B.nprims == 0) continue;
+        f3 o = f3{r0.x, r0.y, r0.z}, d = f3{r1.x, r1.y, r1.z};
+        if (!I.identity) {
+            o = trace_detail_pub::xform_pt_pub(I.inv, o);
+            d = trace_detail_pub::xform_dir_pub(I.inv, d);
+        }
+        const f3 inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        uint32_t stack[64];
+        int sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0) {
+            const uint32_t node = stack[--sp];
+            if (node >= B.nnodes) break;
+            const WideNode N = B.nodes[node];
+            const uint32_t imask = N.e_imask >> 24;
+            float sc[3];
+            for (int k = 0; k < 3; ++k) sc[k] = __uint_as_float(((N.e_imask >> (8 * k)) & 0xffu) << 23);
+            const float org[3] = {N.px, N.py, N.pz};
+            const uint32_t* ql[3] = {N.qlo_x, N.qlo_y, N.qlo_z};
+            const uint32_t* qh[3] = {N.qhi_x, N.qhi_y, N.qhi_z};
+            uint32_t rank = 0;
+            for (int s = 0; s < 8; ++s) {
+                const uint32_t meta = (N.meta[s >> 2] >> (8 * (s & 3))) & 0xffu;
+                if (!meta) continue;
+                float lo[3], hi[3];
+                for (int k = 0; k < 3; ++k) {
+                    const float a = float((ql[k][s >> 2] >> (8 * (s & 3))) & 0xffu);
+                    const float b = float((qh[k][s >> 2] >> (8 * (s & 3))) & 0xffu);
+                    const float m = fmaxf(fabsf(org[k]), fabsf(org[k] + 255.0f * sc[k]));
+                    lo[k] = org[k] + a * sc[k] - (1e-4f + 1e-6f * m);
+                    hi[k] = org[k] + b * sc[k] + (1e-4f + 1e-6f * m);
+                }
+                const bool isint = (meta & 0x80u) != 0;
+                const uint32_t child = N.child_base + rank;
+                if (isint) ++rank;
+                if (!slab_exact(lo, hi, o, inv, r0.w, tend)) continue;
+                if (isint) {
+                    if (sp < 64) stack[sp++] = child;
+                    else atomicAdd(&tp.status[0], 1u);
+                } else {
+                    const uint32_t cnt = meta >> 5, off = meta & 31u;
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        const uint32_t slot = N.prim_base + off + j;
+                        if (slot >= B.nprims) continue;
+                        const int ntri = B.mode == kDots ? 4 : 1;
+                        for (int tri = 0; tri < ntri; ++tri) {
+                            float plo[3], phi[3];
+                            exact_box(B, slot, tri, plo, phi);
+                            if (slab_exact(plo, phi, o, inv, r0.w, tend)) ++count;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    counts[i] = count;
+}
+
+}  // namespace
+
+hipError_t launch_camera_rays(const Camera& cam, float tmin, float tmax, float* rays, hipStream_t st) {
+    const uint32_t n = cam.width * cam.height;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_camera_rays, dim3((n + 255) / 256), dim3(256), 0, st, cam, tmin, tmax,
+                       reinterpret_cast<float4*>(rays));
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow_rays(const float* rays, const HitRec* hits, uint32_t n, const float dir[3], float tmin,
+                              float tmax, float* out, uint32_t* src, uint32_t* counter, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 1u);  // fill kernel, safe under capture
+    if (n == 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_shadow_rays, dim3((n + 255) / 256), dim3(256), 0, st, reinterpret_cast<const float4*>(rays), hits,
+                       n, make_float3(dir[0], dir[1], dir[2]), tmin, tmax, reinterpret_cast<float4*>(out), src, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_candidates(const TraceParams& tp, const float* rays, const HitRec* hits, uint32_t n,
+                                   uint32_t* counts, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count_candidates, dim3((n + 255) / 256), dim3(256), 0, st, tp,
+                       reinterpret_cast<const float4*>(rays), hits, n, counts);
+    return hipGetLastError();
+}
+
 }  // namespace vk

@@ -55,6 +55,11 @@ __device__ __forceinline__ bool better(float t, uint32_t inst, uint32_t pid, con
     return false;
 }
 
+// 1/d with zero components replaced by +-1e-20: the quantized slab form q*(s*inv) + (p-o)*inv
+// would otherwise produce inf - inf = NaN for axis-parallel rays and drop that slab.
+__device__ __forceinline__ float safe_rcp(float x) { return 1.0f / (fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x); }
+__device__ __forceinline__ f3 safe_inv(f3 d) { return f3{safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)}; }
+
 __device__ __forceinline__ float qbyte(uint32_t lo, uint32_t hi, int s) {
     uint32_t w = s < 4 ? lo : hi;
     return float((w >> (8 * (s & 3))) & 0xffu);
@@ -91,7 +96,7 @@ template <int MODE, int RMODE, bool POLISH, bool ANY, int STACK>
 __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmin, uint32_t inst, Best& best,
                                          uint2* lds, bool& done) {
     if (B.nprims == 0) return true;
-    const f3 inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    const f3 inv = safe_inv(d);
     const bool nx = signbit(d.x), ny = signbit(d.y), nz = signbit(d.z);
     const uint32_t oct = uint32_t(nx) | (uint32_t(ny) << 1) | (uint32_t(nz) << 2);
     RayFrame F;
@@ -264,7 +269,7 @@ __device__ __forceinline__ bool ray_box(f3 o, f3 d, const float* lo, const float
     const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float inv = 1.0f / dd[k];
+        float inv = safe_rcp(dd[k]);
         float a = (lo[k] - oo[k]) * inv, b = (hi[k] - oo[k]) * inv;
         t0 = fmaxf(t0, fminf(a, b));
         t1 = fminf(t1, fmaxf(a, b));
@@ -493,6 +498,15 @@ __global__ void k_zero(uint32_t* p, uint32_t n) {
 
 }  // namespace
 }  // namespace trace_detail
+
+// Helpers shared with the auxiliary kernels in trace.hip.
+namespace trace_detail_pub {
+__device__ __forceinline__ void primary_ray_pub(const Camera& c, uint32_t x, uint32_t y, f3& o, f3& d) {
+    trace_detail::primary_ray(c, x, y, o, d);
+}
+__device__ __forceinline__ f3 xform_pt_pub(const float* m, f3 p) { return trace_detail::xform_pt(m, p); }
+__device__ __forceinline__ f3 xform_dir_pub(const float* m, f3 p) { return trace_detail::xform_dir(m, p); }
+}  // namespace trace_detail_pub
 
 // Launchers for one geometry variant and stack depth.
 template <class G, int STACK>
