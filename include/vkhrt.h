@@ -157,7 +157,8 @@ int vkhrt_synchronize(vkhrt_context* ctx, void* stream);
 int vkhrt_abi_version(void);
 
 /* Options (autotuned parameters). */
-#define VKHRT_OPTION_STACK_KIND 1 /* LDS stack entries per lane: 0 -> 8, 1 -> 16, 2 -> 24 */
+#define VKHRT_OPTION_STACK_KIND 1 /* minimum LDS stack entries per lane: 0 -> 8, 1 -> 16, 2 -> 24,
+                                     3 -> 32; raised automatically to the deepest BVH */
 #define VKHRT_OPTION_LEAF_SIZE 2  /* default leaf size for builds (1..3) */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
 
@@ -222,6 +223,10 @@ int vkhrt_generate_shadow_rays(vkhrt_context* ctx, const vkhrt_ray* rays, const 
  * geometry_processor.cpp:422-436) overlaps [tmin, t_end], t_end = hit t or tmax on a miss. */
 int vkhrt_count_candidates(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* rays, const vkhrt_hit* hits,
                            uint32_t count, uint32_t* counts, void* stream);
+/* vkhrt_trace_rays plus per-ray traversal statistics: stats[2i] = 8-wide nodes visited,
+ * stats[2i+1] = primitive records tested (single-mode TLAS, no FP64 polish). */
+int vkhrt_trace_stats(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* rays, uint32_t count,
+                      vkhrt_hit* hits, uint32_t flags, uint32_t* stats, void* stream);
 /* Rays of the last vkhrt_render on this context (synchronizes its stream). */
 int vkhrt_render_ray_counts(vkhrt_context* ctx, uint64_t* primary, uint64_t* shadow);
 

@@ -45,7 +45,7 @@ EXPORTS = [
     "vkhrt_generate_curves", "vkhrt_curves_to_lines", "vkhrt_generate_lss", "vkhrt_generate_dots",
     "vkhrt_profiling_enable", "vkhrt_kernel_stats", "vkhrt_status_counters",
     "vkhrt_generate_camera_rays", "vkhrt_generate_shadow_rays", "vkhrt_count_candidates",
-    "vkhrt_render_ray_counts",
+    "vkhrt_render_ray_counts", "vkhrt_trace_stats",
 ]
 
 
@@ -126,6 +126,7 @@ def lib():
         "vkhrt_generate_shadow_rays": ([P, P, P, u32, P, f32, f32, P, P, P, P], i32),
         "vkhrt_count_candidates": ([P, P, P, P, u32, P, P], i32),
         "vkhrt_render_ray_counts": ([P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)], i32),
+        "vkhrt_trace_stats": ([P, P, P, u32, P, u32, P, P], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -234,6 +235,11 @@ class Context:
     def count_candidates(self, tlas, rays_ptr, hits_ptr, count, counts_ptr, stream=None):
         self.check(lib().vkhrt_count_candidates(self.h, tlas.h, rays_ptr, hits_ptr, count, counts_ptr,
                                                 stream), "count_candidates")
+
+    def trace_stats(self, tlas, rays_ptr, count, hits_ptr, stats_ptr, flags=0, stream=None):
+        """trace_rays with per-ray (nodes visited, primitives tested) into stats (2 x u32)."""
+        self.check(lib().vkhrt_trace_stats(self.h, tlas.h, rays_ptr, count, hits_ptr, flags, stats_ptr,
+                                           stream), "trace_stats")
 
     def render_ray_counts(self):
         p, s = ctypes.c_uint64(), ctypes.c_uint64()

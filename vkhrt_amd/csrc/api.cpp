@@ -63,9 +63,20 @@ struct vkhrt_tlas {
     vk::InstanceDev* dev = nullptr;
     uint32_t count = 0;
     int variant = vk::kVarMixed;  // kernel variant without FP64 polish
+    uint32_t max_depth = 0;       // deepest BLAS (8-wide levels)
 };
 
 namespace {
+// LDS stack entries per lane for a stack kind.
+constexpr uint32_t kStackEntries[4] = {8, 16, 24, 32};
+
+// Stack kind for a TLAS: the context's setting, raised until it holds the deepest tree.
+int stack_kind_for(const vkhrt_context* c, const vkhrt_tlas* t) {
+    int k = c->stack_kind;
+    while (k < 3 && kStackEntries[k] < t->max_depth) ++k;
+    return k;
+}
+
 // Kernel variant for a TLAS and the polish flag.
 int trace_variant(const vkhrt_tlas* t, bool polish) {
     int v = t->variant;
@@ -283,7 +294,7 @@ int vkhrt_synchronize(vkhrt_context* c, void* stream) {
 int vkhrt_set_option(vkhrt_context* c, int option, int value) {
     if (!c) return VKHRT_ERROR_INVALID_ARGUMENT;
     if (option == VKHRT_OPTION_STACK_KIND) {
-        if (value < 0 || value > 2) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "stack kind must be 0..2");
+        if (value < 0 || value > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "stack kind must be 0..3");
         c->stack_kind = value;
         return VKHRT_SUCCESS;
     }
@@ -477,6 +488,11 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
                      : b->radius_mode                 ? vk::kVarPhPoint
                                                       : vk::kVarPhConst;
             v = (v < 0 || v == vi) ? vi : vk::kVarMixed;
+            t->max_depth = std::max(t->max_depth, b->bvh.depth);
+        }
+        if (t->max_depth > kStackEntries[3]) {
+            vkhrt_tlas_destroy(t);
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "BVH deeper than the largest traversal stack (32 levels)");
         }
         t->variant = v < 0 ? vk::kVarPhConst : v;
     }
@@ -511,8 +527,9 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     vk::TraceParams tp{t->dev, t->count, flags, c->status};
     Timer tm(c, st, "trace_rays");
     hipError_t e = vk::launch_trace_rays(tp, reinterpret_cast<const float*>(rays), count,
-                                         reinterpret_cast<vk::HitRec*>(hits),
-                                         trace_variant(t, (flags & VKHRT_TRACE_FP64_POLISH) != 0), c->stack_kind, st);
+                                         reinterpret_cast<vk::HitRec*>(hits), nullptr,
+                                         trace_variant(t, (flags & VKHRT_TRACE_FP64_POLISH) != 0),
+                                         stack_kind_for(c, t), st);
     tm.stop();
     if (e != hipSuccess) return hip_fail(c, e, "trace launch");
     return VKHRT_SUCCESS;
@@ -549,8 +566,8 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (c->profiling)
         for (auto& x : ev) hipEventCreate(&x);
-    hipError_t e = vk::launch_render(rp, trace_variant(t, (d->flags & VKHRT_RENDER_FP64_POLISH) != 0), c->stack_kind,
-                                     st, c->profiling ? ev : nullptr);
+    hipError_t e = vk::launch_render(rp, trace_variant(t, (d->flags & VKHRT_RENDER_FP64_POLISH) != 0),
+                                     stack_kind_for(c, t), st, c->profiling ? ev : nullptr);
     if (c->profiling) {
         c->pending.push_back(PendingTiming{"render_primary", ev[0], ev[1]});
         c->pending.push_back(PendingTiming{"render_shadow", ev[2], ev[3]});
@@ -560,6 +577,19 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     c->last_shadows = rp.shadows && !rp.debug_palette && !d->light_visibility;
     c->last_stream = st;
     return VKHRT_SUCCESS;
+}
+
+int vkhrt_trace_stats(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits,
+                      uint32_t flags, uint32_t* stats, void* stream) {
+    if (!c || !t || (count && (!rays || !hits || !stats))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    if (flags & ~VKHRT_TRACE_TERMINATE_ON_FIRST_HIT)
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "trace_stats supports only the any-hit flag");
+    if (t->variant == vk::kVarMixed) return fail(c, VKHRT_ERROR_UNSUPPORTED, "trace_stats needs a single-mode TLAS");
+    vk::TraceParams tp{t->dev, t->count, flags, c->status};
+    hipError_t e = vk::launch_trace_rays(tp, reinterpret_cast<const float*>(rays), count,
+                                         reinterpret_cast<vk::HitRec*>(hits), reinterpret_cast<uint2*>(stats),
+                                         t->variant, stack_kind_for(c, t), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "trace stats");
 }
 
 int vkhrt_render_ray_counts(vkhrt_context* c, uint64_t* primary, uint64_t* shadow) {

@@ -60,8 +60,8 @@ enum TraceVariant {
     kVarMixed = 6
 };
 
-hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, int variant,
-                             int stack_kind, hipStream_t st);
+hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
+                             int variant, int stack_kind, hipStream_t st);
 hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hipStream_t st,
                          hipEvent_t* ev /*[4] or null*/);
 
@@ -72,7 +72,8 @@ hipError_t launch_count_candidates(const TraceParams& tp, const float* rays, con
                                    uint32_t* counts, hipStream_t st);
 
 template <class G, int STACK>
-hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, hipStream_t st);
+hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
+                                hipStream_t st);
 template <class G, int STACK>
 hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEvent_t* ev);
 

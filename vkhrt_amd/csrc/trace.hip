@@ -5,14 +5,17 @@ namespace vk {
 
 #define VKHRT_DECLARE_VARIANT(G)                                                                                   \
     extern template hipError_t launch_trace_variant<G, 8>(const TraceParams&, const float*, uint32_t, HitRec*,    \
-                                                          hipStream_t);                                           \
+                                                          uint2*, hipStream_t);                                   \
     extern template hipError_t launch_trace_variant<G, 16>(const TraceParams&, const float*, uint32_t, HitRec*,   \
-                                                           hipStream_t);                                          \
+                                                           uint2*, hipStream_t);                                  \
     extern template hipError_t launch_trace_variant<G, 24>(const TraceParams&, const float*, uint32_t, HitRec*,   \
-                                                           hipStream_t);                                          \
+                                                           uint2*, hipStream_t);                                  \
     extern template hipError_t launch_render_variant<G, 8>(const RenderParams&, hipStream_t, hipEvent_t*);        \
     extern template hipError_t launch_render_variant<G, 16>(const RenderParams&, hipStream_t, hipEvent_t*);       \
-    extern template hipError_t launch_render_variant<G, 24>(const RenderParams&, hipStream_t, hipEvent_t*);
+    extern template hipError_t launch_render_variant<G, 24>(const RenderParams&, hipStream_t, hipEvent_t*);       \
+    extern template hipError_t launch_trace_variant<G, 32>(const TraceParams&, const float*, uint32_t, HitRec*,   \
+                                                           uint2*, hipStream_t);                                  \
+    extern template hipError_t launch_render_variant<G, 32>(const RenderParams&, hipStream_t, hipEvent_t*);
 
 using PhConst = Geo<kPhantom, kRadiusConstant, false>;
 using PhConstPolish = Geo<kPhantom, kRadiusConstant, true>;
@@ -30,30 +33,32 @@ VKHRT_DECLARE_VARIANT(GeoMixed)
 
 namespace {
 template <class G>
-hipError_t trace_stack(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, int stack_kind,
-                       hipStream_t st) {
-    if (stack_kind == 0) return launch_trace_variant<G, 8>(tp, rays, n, hits, st);
-    if (stack_kind == 1) return launch_trace_variant<G, 16>(tp, rays, n, hits, st);
-    return launch_trace_variant<G, 24>(tp, rays, n, hits, st);
+hipError_t trace_stack(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
+                       int stack_kind, hipStream_t st) {
+    if (stack_kind == 0) return launch_trace_variant<G, 8>(tp, rays, n, hits, stats, st);
+    if (stack_kind == 1) return launch_trace_variant<G, 16>(tp, rays, n, hits, stats, st);
+    if (stack_kind == 2) return launch_trace_variant<G, 24>(tp, rays, n, hits, stats, st);
+    return launch_trace_variant<G, 32>(tp, rays, n, hits, stats, st);
 }
 template <class G>
 hipError_t render_stack(const RenderParams& rp, int stack_kind, hipStream_t st, hipEvent_t* ev) {
     if (stack_kind == 0) return launch_render_variant<G, 8>(rp, st, ev);
     if (stack_kind == 1) return launch_render_variant<G, 16>(rp, st, ev);
-    return launch_render_variant<G, 24>(rp, st, ev);
+    if (stack_kind == 2) return launch_render_variant<G, 24>(rp, st, ev);
+    return launch_render_variant<G, 32>(rp, st, ev);
 }
 }  // namespace
 
-hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, int variant,
-                             int stack_kind, hipStream_t st) {
+hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
+                             int variant, int stack_kind, hipStream_t st) {
     switch (variant) {
-        case kVarPhConst: return trace_stack<PhConst>(tp, rays, n, hits, stack_kind, st);
-        case kVarPhConstPolish: return trace_stack<PhConstPolish>(tp, rays, n, hits, stack_kind, st);
-        case kVarPhPoint: return trace_stack<PhPoint>(tp, rays, n, hits, stack_kind, st);
-        case kVarPhPointPolish: return trace_stack<PhPointPolish>(tp, rays, n, hits, stack_kind, st);
-        case kVarLss: return trace_stack<LssG>(tp, rays, n, hits, stack_kind, st);
-        case kVarDots: return trace_stack<DotsG>(tp, rays, n, hits, stack_kind, st);
-        default: return trace_stack<GeoMixed>(tp, rays, n, hits, stack_kind, st);
+        case kVarPhConst: return trace_stack<PhConst>(tp, rays, n, hits, stats, stack_kind, st);
+        case kVarPhConstPolish: return trace_stack<PhConstPolish>(tp, rays, n, hits, stats, stack_kind, st);
+        case kVarPhPoint: return trace_stack<PhPoint>(tp, rays, n, hits, stats, stack_kind, st);
+        case kVarPhPointPolish: return trace_stack<PhPointPolish>(tp, rays, n, hits, stats, stack_kind, st);
+        case kVarLss: return trace_stack<LssG>(tp, rays, n, hits, stats, stack_kind, st);
+        case kVarDots: return trace_stack<DotsG>(tp, rays, n, hits, stats, stack_kind, st);
+        default: return trace_stack<GeoMixed>(tp, rays, n, hits, stats, stack_kind, st);
     }
 }
 

@@ -4,7 +4,7 @@
 //
 // One ray per lane. The 8-wide BVH is walked front-to-back with node groups (child base,
 // internal-child mask, remaining hit bits permuted by the ray octant) kept on a short
-// per-lane stack in LDS, with a small private overflow area. Closest-hit semantics: the
+// per-lane stack in LDS (sized from the tree depth by the host). Closest-hit semantics: the
 // smallest t in [tMin, tMax]; equal t resolve to the lower (instance, primitive). Shadow
 // rays stop at the first accepted hit.
 //
@@ -38,7 +38,6 @@ namespace trace_detail {
 namespace {
 
 constexpr int kTraceBlock = 256;
-constexpr int kOverflow = 48;
 
 struct Best {
     float t;
@@ -65,36 +64,32 @@ __device__ __forceinline__ float qbyte(uint32_t lo, uint32_t hi, int s) {
     return float((w >> (8 * (s & 3))) & 0xffu);
 }
 
+// Per-lane stack of node groups in LDS (stride kTraceBlock entries). A group is pushed
+// only when descending one level, so a tree of depth D never needs more than D entries;
+// the host picks STACK >= depth and a push past STACK reports an error instead of spilling.
 template <int STACK>
 struct LaneStack {
-    uint2* lds;  // base for this lane, stride kTraceBlock entries
-    uint2 ovf[kOverflow];
+    uint2* lds;
     int sp;
     __device__ __forceinline__ bool push(uint2 g) {
-        if (sp < STACK) {
-            lds[sp * kTraceBlock] = g;
-        } else if (sp < STACK + kOverflow) {
-            ovf[sp - STACK] = g;
-        } else {
-            return false;
-        }
+        if (sp >= STACK) return false;
+        lds[sp * kTraceBlock] = g;
         ++sp;
         return true;
     }
-    __device__ __forceinline__ uint2 top() const { return sp <= STACK ? lds[(sp - 1) * kTraceBlock] : ovf[sp - 1 - STACK]; }
-    __device__ __forceinline__ void set_top(uint2 g) {
-        if (sp <= STACK)
-            lds[(sp - 1) * kTraceBlock] = g;
-        else
-            ovf[sp - 1 - STACK] = g;
-    }
+    __device__ __forceinline__ uint2 top() const { return lds[(sp - 1) * kTraceBlock]; }
 };
 
 // group: x = child_base, y = imask | (remaining hit bits in octant order) << 8
 // Traverses one BLAS. Returns false on stack overflow.
-template <int MODE, int RMODE, bool POLISH, bool ANY, int STACK>
+// Per-ray traversal statistics (nodes visited, primitive records tested).
+struct TravStats {
+    uint32_t nodes = 0, prims = 0;
+};
+
+template <int MODE, int RMODE, bool POLISH, bool ANY, int STACK, bool STATS = false>
 __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmin, uint32_t inst, Best& best,
-                                         uint2* lds, bool& done) {
+                                         uint2* lds, bool& done, TravStats* st = nullptr) {
     if (B.nprims == 0) return true;
     const f3 inv = safe_inv(d);
     const bool nx = signbit(d.x), ny = signbit(d.y), nz = signbit(d.z);
@@ -113,6 +108,7 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
     for (;;) {
         if (have_node) {
             if (node >= B.nnodes) return false;  // malformed structure: report, never read out of bounds
+            if (STATS) ++st->nodes;
             const uint4* np = reinterpret_cast<const uint4*>(B.nodes + node);
             const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
             const float sx = __uint_as_float((n0.w & 0xffu) << 23) * inv.x;
@@ -154,6 +150,7 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
                 const uint32_t k = __ffs(pmask) - 1;
                 pmask &= pmask - 1;
                 const uint32_t slot = pbase + k;
+                if (STATS) ++st->prims;
                 if (MODE == kPhantom) {
                     const float4* pr = B.prims + size_t(slot) * 4;
                     float4 cv[4] = {pr[0], pr[1], pr[2], pr[3]};
@@ -278,9 +275,9 @@ __device__ __forceinline__ bool ray_box(f3 o, f3 d, const float* lo, const float
 }
 
 // Traces one world-space ray through all instances. Fills `best`; returns false on overflow.
-template <class G, bool ANY, int STACK>
+template <class G, bool ANY, int STACK, bool STATS = false>
 __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, float tmin, float tmax, Best& best,
-                                            uint2* lds) {
+                                            uint2* lds, TravStats* st = nullptr) {
     best.t = tmax;
     best.inst = 0xFFFFFFFFu;
     best.pid = 0xFFFFFFFFu;
@@ -312,7 +309,8 @@ __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, f
                         : traverse<kPhantom, kRadiusConstant, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done);
             }
         } else {
-            r = traverse<G::kMode, G::kRMode, G::kPolish, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done);
+            r = traverse<G::kMode, G::kRMode, G::kPolish, ANY, STACK, STATS>(I.blas, oo, dd, tmin, i, best, lds, done,
+                                                                             st);
         }
         ok = ok && r;
         if (ANY && done) break;
@@ -369,9 +367,9 @@ __device__ __forceinline__ void finish_hit(const TraceParams& tp, f3 o, f3 d, co
     h.v = v;
 }
 
-template <class G, bool ANY, int STACK>
+template <class G, bool ANY, int STACK, bool STATS>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceParams tp, const float* __restrict__ rays, uint32_t n,
-                                                            HitRec* __restrict__ hits) {
+                                                            HitRec* __restrict__ hits, uint2* __restrict__ stats) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
@@ -379,7 +377,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceParams tp, cons
     const float4 r1 = reinterpret_cast<const float4*>(rays)[size_t(i) * 2 + 1];
     const f3 o = f3{r0.x, r0.y, r0.z}, d = f3{r1.x, r1.y, r1.z};
     Best best;
-    const bool ok = trace_world<G, ANY, STACK>(tp, o, d, r0.w, r1.w, best, stk + threadIdx.x);
+    TravStats ts;
+    const bool ok = trace_world<G, ANY, STACK, STATS>(tp, o, d, r0.w, r1.w, best, stk + threadIdx.x, &ts);
+    if (STATS) stats[i] = make_uint2(ts.nodes, ts.prims);
     if (!ok) atomicAdd(&tp.status[0], 1u);
     HitRec h;
     finish_hit(tp, o, d, best, h);
@@ -510,14 +510,26 @@ __device__ __forceinline__ f3 xform_dir_pub(const float* m, f3 p) { return trace
 
 // Launchers for one geometry variant and stack depth.
 template <class G, int STACK>
-hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, hipStream_t st) {
+hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
+                                hipStream_t st) {
     using namespace trace_detail;
     if (n == 0) return hipSuccess;
     const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
-    if (tp.flags & kTraceAnyHit)
-        hipLaunchKernelGGL((k_trace_rays<G, true, STACK>), grid, block, 0, st, tp, rays, n, hits);
+    const bool any = (tp.flags & kTraceAnyHit) != 0;
+    if (stats) {
+        if constexpr (!G::kMixed && !G::kPolish) {
+            if (any)
+                hipLaunchKernelGGL((k_trace_rays<G, true, STACK, true>), grid, block, 0, st, tp, rays, n, hits, stats);
+            else
+                hipLaunchKernelGGL((k_trace_rays<G, false, STACK, true>), grid, block, 0, st, tp, rays, n, hits, stats);
+            return hipGetLastError();
+        }
+        return hipErrorNotSupported;
+    }
+    if (any)
+        hipLaunchKernelGGL((k_trace_rays<G, true, STACK, false>), grid, block, 0, st, tp, rays, n, hits, stats);
     else
-        hipLaunchKernelGGL((k_trace_rays<G, false, STACK>), grid, block, 0, st, tp, rays, n, hits);
+        hipLaunchKernelGGL((k_trace_rays<G, false, STACK, false>), grid, block, 0, st, tp, rays, n, hits, stats);
     return hipGetLastError();
 }
 
@@ -539,14 +551,17 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
 
 // Explicit instantiation helpers for the trace_*.hip translation units.
 #define VKHRT_INSTANTIATE_TRACE(G)                                                                              \
-    template hipError_t launch_trace_variant<G, 8>(const TraceParams&, const float*, uint32_t, HitRec*,         \
+    template hipError_t launch_trace_variant<G, 8>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*, \
                                                    hipStream_t);                                                \
-    template hipError_t launch_trace_variant<G, 16>(const TraceParams&, const float*, uint32_t, HitRec*,        \
+    template hipError_t launch_trace_variant<G, 16>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
                                                     hipStream_t);                                               \
-    template hipError_t launch_trace_variant<G, 24>(const TraceParams&, const float*, uint32_t, HitRec*,        \
+    template hipError_t launch_trace_variant<G, 24>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
                                                     hipStream_t);                                               \
     template hipError_t launch_render_variant<G, 8>(const RenderParams&, hipStream_t, hipEvent_t*);             \
     template hipError_t launch_render_variant<G, 16>(const RenderParams&, hipStream_t, hipEvent_t*);            \
-    template hipError_t launch_render_variant<G, 24>(const RenderParams&, hipStream_t, hipEvent_t*);
+    template hipError_t launch_render_variant<G, 24>(const RenderParams&, hipStream_t, hipEvent_t*);            \
+    template hipError_t launch_trace_variant<G, 32>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
+                                                    hipStream_t);                                               \
+    template hipError_t launch_render_variant<G, 32>(const RenderParams&, hipStream_t, hipEvent_t*);
 
 }  // namespace vk
