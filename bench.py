@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--polish", type=int, default=1, help="FP64 polish of grazing roots (1) or off (0)")
     ap.add_argument("--stack-kind", type=int, default=1)
     ap.add_argument("--leaf-size", type=int, default=1)
+    ap.add_argument("--persistent", type=int, default=1)
+    ap.add_argument("--refill-min", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -70,6 +72,8 @@ def main():
     sp = stream.cuda_stream
     ctx = V.Context(0)
     ctx.set_option(V.OPTION_STACK_KIND, args.stack_kind)
+    ctx.set_option(V.OPTION_PERSISTENT, args.persistent)
+    ctx.set_option(V.OPTION_REFILL_MIN, args.refill_min)
 
     NS, SEG, W, H = args.strands, args.segments, args.width, args.height
     nseg_total = NS * SEG
@@ -156,7 +160,7 @@ def main():
             "rays_per_frame": rays, "primary_rays": n_primary, "shadow_rays": n_shadow,
             "bvh": {"width": 8, "leaf_size": args.leaf_size, "nodes": binfo["node_count"],
                     "depth": binfo["depth"], "build_ms": round(binfo["build_ms"], 3)},
-            "stack_kind": args.stack_kind,
+            "stack_kind": args.stack_kind, "persistent": bool(args.persistent), "refill_min": args.refill_min,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

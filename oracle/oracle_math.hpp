@@ -216,6 +216,7 @@ template <class T>
 struct RCI {
     bool isIntersection;
     T s, dt, dp, dc, sp;
+    T a, gscale;  // coefficient a ~ |axis|^2 (dist^2 - r^2) and |axis|^2 r^2 (grazing test)
 };
 
 // cone.glsl:21-62 RayConeIntersectRCC. With slant = 0 it reduces to ray vs infinite
@@ -235,6 +236,8 @@ inline RCI<T> RayConeIntersectRCC(const Cone<T>& cone) {
     T a = T(2) * drr * cdd + cxd * cxd - ddd * r2 + dp * cdz2;
     T det = b * b - a * c;
     RCI<T> r;
+    r.a = a;
+    r.gscale = ddd * r2;
     r.isIntersection = det > T(0);
     r.s = (b - (r.isIntersection ? std::sqrt(det) : T(0))) / c;
     r.dt = (r.s * cone.axis.z - cdd) / ddd;
@@ -290,10 +293,13 @@ struct PrhiOpts {
     bool fp64_polish = false;
 };
 
-// FP64 continuation of a side that ran all iterations without converging while still
-// near the root (grazing). Same update rule as the FP32 loop, evaluated in double on a
-// curve transformed with a double RCC matrix. Extension with no reference counterpart.
-// Trigger: FP32 side used all 8 iterations, final t in [0, 1] and |dt| < 0.05.
+// FP64 continuation of a side that ran all iterations without converging while grazing.
+// Same update rule as the FP32 loop, evaluated in double on a curve transformed with a
+// double RCC matrix. Extension with no reference counterpart.
+// Trigger: the FP32 side used all 8 iterations, its final t is in [0, 1], |dt| < 0.05 and
+// the ray grazes the tube at the last iterate: |a| <= 1e-2 |axis|^2 r^2, where the cone
+// coefficient a ~ |axis|^2 (d^2 - r^2) and d is the ray's distance to the cone axis (the
+// ray is within ~0.5% of the radius of tangent; there the FP32 iteration converges slowly).
 template <class T>
 struct PolishState {
     double t, told, dt1, dt2;
@@ -307,7 +313,7 @@ PrhiOut<T> Prhi(const Ray<T>& ray, const Curve<T>& curve, const T radii[4], cons
 template <class T>
 inline bool prhi_side_iterate(const Curve<T>& rcc, const T radii[4], int radius_mode, T const_r, T& t, T& told,
                               T& dt1, T& dt2, int max_iter, int first_i, int& iters, T& s_out, T& u_out,
-                              Cone<T>& cone_out, bool& exhausted, T& last_dt) {
+                              Cone<T>& cone_out, bool& exhausted, T& last_dt, bool& grazing) {
     exhausted = false;
     for (int i = first_i; i < first_i + max_iter; ++i) {
         Cone<T> cone;
@@ -322,6 +328,7 @@ inline bool prhi_side_iterate(const Curve<T>& rcc, const T radii[4], int radius_
         cone.axis = SampleCurveAxis(rcc, t);
         RCI<T> rci = RayConeIntersectRCC(cone);
         ++iters;
+        grazing = std::fabs(rci.a) <= T(1e-2f) * rci.gscale;
         if (rci.isIntersection && std::fabs(rci.dt) < T(5e-5f)) {
             s_out = rci.s;  // relative to cone.center.z
             u_out = t;
@@ -374,9 +381,9 @@ PrhiOut<T> Prhi(const Ray<T>& ray, const Curve<T>& curve, const T radii[4], cons
         T t = tStart, told = T(0), dt1 = T(0), dt2 = T(0);
         T s_rel = T(0), u = T(0), last_dt = T(0);
         Cone<T> cone{};
-        bool exhausted = false;
+        bool exhausted = false, grazing = false;
         bool conv = prhi_side_iterate(rcc, radii, o.radius_mode, const_r, t, told, dt1, dt2, 8, 0, out.iters, s_rel,
-                                      u, cone, exhausted, last_dt);
+                                      u, cone, exhausted, last_dt, grazing);
         if (conv) {
             T s = s_rel + cone.center.z;
             result = s;
@@ -391,7 +398,7 @@ PrhiOut<T> Prhi(const Ray<T>& ray, const Curve<T>& curve, const T radii[4], cons
             MakeOrthonormalBasis(e1, e2, e3);
             V3<T> nr = mk(-cone.center.x, -cone.center.y, s_rel);
             out.n_rcc = normalize(nr.x * e1 + nr.y * e2 + nr.z * e3);
-        } else if (o.fp64_polish && exhausted && t >= T(0) && t <= T(1) && std::fabs(last_dt) < T(0.05f)) {
+        } else if (o.fp64_polish && exhausted && t >= T(0) && t <= T(1) && std::fabs(last_dt) < T(0.05f) && grazing) {
             // FP64 polish of a grazing side (extension).
             Ray<double> rd{cvt<double>(ray.origin), cvt<double>(ray.direction)};
             Curve<double> cd{cvt<double>(curve.start), cvt<double>(curve.cp1), cvt<double>(curve.cp2),
@@ -402,10 +409,10 @@ PrhiOut<T> Prhi(const Ray<T>& ray, const Curve<T>& curve, const T radii[4], cons
             double td = double(t), toldd = double(told), dt1d = double(dt1), dt2d = double(dt2);
             double sd = 0, ud = 0, lastd = 0;
             Cone<double> coned{};
-            bool exd = false;
+            bool exd = false, grd = false;
             int it2 = 0;
             bool convd = prhi_side_iterate(rccd, rr, o.radius_mode, double(const_r), td, toldd, dt1d, dt2d, 8, 8,
-                                           it2, sd, ud, coned, exd, lastd);
+                                           it2, sd, ud, coned, exd, lastd, grd);
             out.iters += it2;
             if (convd) {
                 double s = sd + coned.center.z;

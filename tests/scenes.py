@@ -44,3 +44,39 @@ def targeted_rays(curves, n, seed, spread=0.06, dist=(5.0, 400.0)):
 
 def lines_from_curves(curves, pieces=4):
     return O.curves_to_lines(curves, pieces)
+
+
+def grazing_rays(curves, n, seed, per_point=True, eps=2e-3, radius=0.02):
+    """Rays tangent to the hair tube: through curve(t0) + r(t0)(1 + e) N with direction
+    perpendicular to N, e uniform in [-eps, eps], so the cone discriminant is near zero
+    (the FP64 polish trigger). Returns (n, 8) float32 rays with unit directions."""
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, curves.shape[0], n)
+    t = rng.uniform(0.05, 0.95, n)
+    c = curves[idx].reshape(n, 4, 4).astype(np.float64)
+    u = 1.0 - t
+    w = np.stack([u ** 3, 3 * u * u * t, 3 * u * t * t, t ** 3], axis=1)
+    dw = np.stack([-3 * u * u, 3 * (3 * t * t - 4 * t + 1), 3 * (2 - 3 * t) * t, 3 * t * t], axis=1)
+    p = np.einsum("nk,nkj->nj", w, c[:, :, :3])
+    tan = np.einsum("nk,nkj->nj", dw, c[:, :, :3])
+    tan /= np.linalg.norm(tan, axis=1, keepdims=True)
+    r = np.einsum("nk,nk->n", w, c[:, :, 3]) if per_point else np.full(n, radius)
+    a = rng.normal(size=(n, 3))
+    nrm = a - (a * tan).sum(1, keepdims=True) * tan
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    b = np.cross(tan, nrm)
+    ang = rng.uniform(-1.2, 1.2, n)[:, None]
+    d = np.cos(ang) * b + np.sin(ang) * tan
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    e = rng.uniform(-eps, eps, n)[:, None]
+    q = p + nrm * (r[:, None] * (1.0 + e))
+    L = rng.uniform(20.0, 300.0, n)[:, None]
+    o = q - d * L
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 0.001
+    dd = d.astype(np.float32)
+    dd = dd / np.sqrt((dd * dd).sum(1, keepdims=True))
+    rays[:, 4:7] = dd
+    rays[:, 7] = 10000.0
+    return rays

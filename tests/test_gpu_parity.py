@@ -172,3 +172,62 @@ def test_edge_cases(gpu_ctx, G, oracle, curves):
     ref = S.trace(rays)
     h = G.trace(gpu_ctx, tlas, rays)
     _compare_hits(h, ref, "edge")
+
+
+@pytest.mark.parametrize("per_point", [False, True])
+def test_phantom_fp64_polish_trace(gpu_ctx, G, oracle, curves, per_point):
+    """FP64 polish of grazing roots (deferred to a candidate pass on the GPU) equals the
+    oracle's in-place polish bit for bit, grazing-heavy ray set included."""
+    import scenes
+    import vkhrt_amd as V
+    rays = np.concatenate([scenes.targeted_rays(curves, 20000, seed=31 + per_point),
+                           scenes.grazing_rays(curves, 20000, seed=41 + per_point, per_point=per_point)])
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=per_point)
+    ref = S.trace(rays, flags=oracle.F_POLISH)
+    blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=per_point)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    h = G.trace(gpu_ctx, tlas, rays, flags=V.TRACE_FP64_POLISH)
+    _compare_hits(h, ref, f"polish per_point={per_point}")
+    assert np.array_equal(h["v"].view(np.uint32), ref["v"].view(np.uint32))  # polished flag
+    npol = int((ref["v"] == 1.0).sum())
+    print("polished roots:", npol)
+    assert npol > 50, "the grazing ray set must exercise the FP64 continuation"
+    hs = G.trace(gpu_ctx, tlas, rays, flags=V.TRACE_FP64_POLISH | V.TRACE_TERMINATE_ON_FIRST_HIT)
+    rs = S.trace(rays, flags=oracle.F_POLISH | oracle.F_ANY_HIT)
+    assert np.array_equal(hs["prim"] == 0xFFFFFFFF, rs["prim"] == 0xFFFFFFFF)
+
+
+def test_phantom_render_fp64_polish(gpu_ctx, G, oracle, curves):
+    import vkhrt_amd as V
+    W = H = 256
+    cam = V.camera_look_at((0.0, 50.0, 400.0), (0.0, 30.0, 0.0), fov_deg=60.0, aspect=1.0)
+    vi, pi = G.cam_arrays(cam)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    ref_img, ref_ids, _, _ = S.render(vi, pi, W, H, flags=oracle.F_POLISH, shadows=1)
+    blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    img, ids = G.render(gpu_ctx, tlas, cam, W, H, flags=V.RENDER_SHADOWS | V.RENDER_FP64_POLISH)
+    assert np.array_equal(ids, ref_ids)
+    assert np.array_equal(img[..., :3].view(np.uint32), ref_img.view(np.uint32))
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_persistent_and_plain_kernels_agree(gpu_ctx, G, oracle, curves, persistent):
+    import scenes
+    import vkhrt_amd as V
+    rays = scenes.targeted_rays(curves, 20000, seed=3)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    ref = S.trace(rays)
+    blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    gpu_ctx.set_option(V.OPTION_PERSISTENT, persistent)
+    gpu_ctx.set_option(V.OPTION_REFILL_MIN, 8)
+    try:
+        h = G.trace(gpu_ctx, tlas, rays)
+    finally:
+        gpu_ctx.set_option(V.OPTION_PERSISTENT, 1)
+        gpu_ctx.set_option(V.OPTION_REFILL_MIN, 1)
+    _compare_hits(h, ref, f"persistent={persistent}")

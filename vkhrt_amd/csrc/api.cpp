@@ -30,6 +30,8 @@ struct vkhrt_context {
     std::string err;
     int stack_kind = 1;
     int leaf_size = 1;
+    bool persistent = true;
+    uint32_t refill_min = 1;
     bool profiling = false;
     std::vector<PendingTiming> pending;
     std::map<std::string, std::pair<double, uint32_t>> stats;
@@ -42,6 +44,15 @@ struct vkhrt_context {
     uint64_t last_pixels = 0;
     bool last_shadows = false;
     hipStream_t last_stream = nullptr;
+    vk::HitRec* hitrec = nullptr;  // per pixel (deferred-polish render)
+    float* vis = nullptr;          // per pixel
+    // deferred FP64 polish
+    uint4* defer = nullptr;
+    vk::HitRec* cand_hits = nullptr;
+    unsigned long long* cand_keys = nullptr;
+    uint32_t defer_cap = 0;
+    unsigned long long* keys = nullptr;
+    size_t keys_cap = 0;
 };
 
 struct vkhrt_blas {
@@ -64,6 +75,7 @@ struct vkhrt_tlas {
     uint32_t count = 0;
     int variant = vk::kVarMixed;  // kernel variant without FP64 polish
     uint32_t max_depth = 0;       // deepest BLAS (8-wide levels)
+    bool polish_ok = true;        // merge keys can encode every (instance, primitive)
 };
 
 namespace {
@@ -220,14 +232,59 @@ int ensure_render_scratch(vkhrt_context* c, size_t npix) {
     if (npix <= c->cap_pixels) return VKHRT_SUCCESS;
     hipFree(c->hitbuf);
     hipFree(c->queue);
+    hipFree(c->hitrec);
+    hipFree(c->vis);
     c->hitbuf = nullptr;
     c->queue = nullptr;
+    c->hitrec = nullptr;
+    c->vis = nullptr;
     c->cap_pixels = 0;
     hipError_t e = hipMalloc(&c->hitbuf, npix * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(&c->queue, npix * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->hitrec, npix * sizeof(vk::HitRec));
+    if (e == hipSuccess) e = hipMalloc(&c->vis, npix * sizeof(float));
     if (e != hipSuccess) return hip_fail(c, e, "render scratch");
     c->cap_pixels = npix;
     return VKHRT_SUCCESS;
+}
+
+// Deferred-polish scratch for `nrays` rays: merge keys per ray and a candidate queue with
+// room for one candidate per ray (at least 64Ki).
+int ensure_polish_scratch(vkhrt_context* c, size_t nrays) {
+    hipError_t e = hipSuccess;
+    if (nrays > c->keys_cap) {
+        hipFree(c->keys);
+        c->keys = nullptr;
+        c->keys_cap = 0;
+        e = hipMalloc(&c->keys, nrays * sizeof(unsigned long long));
+        if (e != hipSuccess) return hip_fail(c, e, "polish keys");
+        c->keys_cap = nrays;
+    }
+    const size_t cap = std::max<size_t>(nrays, 65536);
+    if (cap > c->defer_cap) {
+        hipFree(c->defer);
+        hipFree(c->cand_hits);
+        hipFree(c->cand_keys);
+        c->defer = nullptr;
+        c->cand_hits = nullptr;
+        c->cand_keys = nullptr;
+        c->defer_cap = 0;
+        e = hipMalloc(&c->defer, cap * sizeof(uint4));
+        if (e == hipSuccess) e = hipMalloc(&c->cand_hits, cap * sizeof(vk::HitRec));
+        if (e == hipSuccess) e = hipMalloc(&c->cand_keys, cap * sizeof(unsigned long long));
+        if (e != hipSuccess) return hip_fail(c, e, "polish queue");
+        c->defer_cap = uint32_t(cap);
+    }
+    return VKHRT_SUCCESS;
+}
+
+void set_polish(vkhrt_context* c, vk::TraceParams& tp) {
+    tp.defer = c->defer;
+    tp.defer_count = c->counters + 4;
+    tp.defer_cap = c->defer_cap;
+    tp.keys = c->keys;
+    tp.cand_hits = c->cand_hits;
+    tp.cand_keys = c->cand_keys;
 }
 
 }  // namespace
@@ -248,7 +305,7 @@ int vkhrt_context_create(int device, vkhrt_context** out) {
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->status, 16);
-    if (e == hipSuccess) e = hipMalloc(&c->counters, 16);
+    if (e == hipSuccess) e = hipMalloc(&c->counters, 64);
     if (e == hipSuccess) e = hipMemset(c->status, 0, 16);
     if (e != hipSuccess) {
         vkhrt_context_destroy(c);
@@ -270,6 +327,12 @@ void vkhrt_context_destroy(vkhrt_context* c) {
     hipFree(c->counters);
     hipFree(c->hitbuf);
     hipFree(c->queue);
+    hipFree(c->hitrec);
+    hipFree(c->vis);
+    hipFree(c->defer);
+    hipFree(c->cand_hits);
+    hipFree(c->cand_keys);
+    hipFree(c->keys);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -283,10 +346,13 @@ int vkhrt_synchronize(vkhrt_context* c, void* stream) {
     uint32_t st[4];
     e = hipMemcpy(st, c->status, 16, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(c, e, "status readback");
-    if (st[0]) {
+    if (st[0] || st[1]) {
         hipMemset(c->status, 0, 16);
-        return fail(c, VKHRT_ERROR_STACK_OVERFLOW,
-                    "traversal stack overflow on " + std::to_string(st[0]) + " rays (results incomplete)");
+        if (st[0])
+            return fail(c, VKHRT_ERROR_STACK_OVERFLOW,
+                        "traversal stack overflow on " + std::to_string(st[0]) + " rays (results incomplete)");
+        return fail(c, VKHRT_ERROR_OUT_OF_MEMORY,
+                    "FP64 polish queue overflow: " + std::to_string(st[1]) + " candidates dropped (results incomplete)");
     }
     return VKHRT_SUCCESS;
 }
@@ -296,6 +362,16 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
     if (option == VKHRT_OPTION_STACK_KIND) {
         if (value < 0 || value > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "stack kind must be 0..3");
         c->stack_kind = value;
+        return VKHRT_SUCCESS;
+    }
+    if (option == VKHRT_OPTION_PERSISTENT) {
+        if (value < 0 || value > 1) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "persistent must be 0 or 1");
+        c->persistent = value != 0;
+        return VKHRT_SUCCESS;
+    }
+    if (option == VKHRT_OPTION_REFILL_MIN) {
+        if (value < 1 || value > 64) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "refill minimum must be 1..64");
+        c->refill_min = uint32_t(value);
         return VKHRT_SUCCESS;
     }
     if (option == VKHRT_OPTION_LEAF_SIZE) {
@@ -400,7 +476,8 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
     b->info.node_count = b->bvh.nnodes;
     b->info.depth = b->bvh.depth;
     b->info.device_bytes = uint64_t(b->bvh.nnodes) * sizeof(vk::WideNode) + uint64_t(d->count) * 4u +
-                           uint64_t(d->count) * (d->mode == VKHRT_GEOMETRY_LSS ? 32u : 64u);
+                           uint64_t(d->count) * (d->mode == VKHRT_GEOMETRY_LSS ? 32u : 64u) +
+                           (d->mode == VKHRT_GEOMETRY_PHANTOM_CURVES ? uint64_t(d->count) * 4u : 0u);
 out:
     if (own_input) hipFree(input);
     if (records && records != input) hipFree(records);
@@ -425,6 +502,7 @@ void vkhrt_blas_destroy(vkhrt_blas* b) {
     hipFree(b->bvh.nodes);
     hipFree(b->bvh.prim_ids);
     hipFree(b->bvh.prims);
+    hipFree(b->bvh.aux);
     delete b;
 }
 
@@ -439,6 +517,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         vk::InstanceDev& I = inst[i];
         std::memset(&I, 0, sizeof(I));
         I.blas.prims = b->bvh.prims;
+        I.blas.aux = b->bvh.aux;
         I.blas.prim_ids = b->bvh.prim_ids;
         I.blas.nodes = b->bvh.nodes;
         I.blas.nprims = b->records;
@@ -489,7 +568,9 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
                                                       : vk::kVarPhConst;
             v = (v < 0 || v == vi) ? vi : vk::kVarMixed;
             t->max_depth = std::max(t->max_depth, b->bvh.depth);
+            if (b->records >= (1u << 27)) t->polish_ok = false;
         }
+        if (count > 31) t->polish_ok = false;
         if (t->max_depth > kStackEntries[3]) {
             vkhrt_tlas_destroy(t);
             return fail(c, VKHRT_ERROR_UNSUPPORTED, "BVH deeper than the largest traversal stack (32 levels)");
@@ -524,7 +605,16 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     if (flags & ~(VKHRT_TRACE_TERMINATE_ON_FIRST_HIT | VKHRT_TRACE_FP64_POLISH))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown trace flags");
     hipStream_t st = pick(c, stream);
-    vk::TraceParams tp{t->dev, t->count, flags, c->status};
+    vk::TraceParams tp{t->dev, t->count, flags, c->status, c->persistent ? c->counters + 3 : nullptr, c->refill_min};
+    const bool polish = (flags & VKHRT_TRACE_FP64_POLISH) != 0;
+    if (polish) {
+        if (!t->polish_ok)
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "FP64 polish needs < 2^27 primitives per BLAS and <= 31 instances");
+        int rc = ensure_polish_scratch(c, count);
+        if (rc != VKHRT_SUCCESS) return rc;
+        set_polish(c, tp);
+        if (!tp.work) tp.work = c->counters + 3;  // the deferred pipeline runs on persistent kernels
+    }
     Timer tm(c, st, "trace_rays");
     hipError_t e = vk::launch_trace_rays(tp, reinterpret_cast<const float*>(rays), count,
                                          reinterpret_cast<vk::HitRec*>(hits), nullptr,
@@ -549,7 +639,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     hipStream_t st = pick(c, stream);
     vk::RenderParams rp{};
     rp.tp = vk::TraceParams{t->dev, t->count, (d->flags & VKHRT_RENDER_FP64_POLISH) ? uint32_t(vk::kTraceFp64Polish) : 0u,
-                            c->status};
+                            c->status, c->persistent ? c->counters + 1 : nullptr, c->refill_min};
     std::memcpy(rp.cam.view_inv, cam->view_inverse, 64);
     std::memcpy(rp.cam.proj_inv, cam->proj_inverse, 64);
     rp.cam.width = d->width;
@@ -563,6 +653,17 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     rp.queue = c->queue;
     rp.counters = c->counters;
     rp.light_vis = const_cast<float*>(d->light_visibility);
+    rp.hitrec = c->hitrec;
+    rp.vis = c->vis;
+    if (d->flags & VKHRT_RENDER_FP64_POLISH) {
+        if (!t->polish_ok || t->variant == vk::kVarMixed)
+            return fail(c, VKHRT_ERROR_UNSUPPORTED,
+                        "render FP64 polish needs a single-mode TLAS, < 2^27 primitives per BLAS, <= 31 instances");
+        rc = ensure_polish_scratch(c, npix);
+        if (rc != VKHRT_SUCCESS) return rc;
+        set_polish(c, rp.tp);
+        if (!rp.tp.work) rp.tp.work = c->counters + 1;
+    }
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (c->profiling)
         for (auto& x : ev) hipEventCreate(&x);
@@ -585,7 +686,7 @@ int vkhrt_trace_stats(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ra
     if (flags & ~VKHRT_TRACE_TERMINATE_ON_FIRST_HIT)
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "trace_stats supports only the any-hit flag");
     if (t->variant == vk::kVarMixed) return fail(c, VKHRT_ERROR_UNSUPPORTED, "trace_stats needs a single-mode TLAS");
-    vk::TraceParams tp{t->dev, t->count, flags, c->status};
+    vk::TraceParams tp{t->dev, t->count, flags, c->status, nullptr, 1u};
     hipError_t e = vk::launch_trace_rays(tp, reinterpret_cast<const float*>(rays), count,
                                          reinterpret_cast<vk::HitRec*>(hits), reinterpret_cast<uint2*>(stats),
                                          t->variant, stack_kind_for(c, t), pick(c, stream));
@@ -632,7 +733,7 @@ int vkhrt_generate_shadow_rays(vkhrt_context* c, const vkhrt_ray* rays, const vk
 int vkhrt_count_candidates(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, const vkhrt_hit* hits,
                            uint32_t count, uint32_t* counts, void* stream) {
     if (!c || !t || (count && (!rays || !hits || !counts))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
-    vk::TraceParams tp{t->dev, t->count, 0u, c->status};
+    vk::TraceParams tp{t->dev, t->count, 0u, c->status, nullptr, 1u};
     hipError_t e = vk::launch_count_candidates(tp, reinterpret_cast<const float*>(rays),
                                                reinterpret_cast<const vk::HitRec*>(hits), count, counts, pick(c, stream));
     return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "count candidates");

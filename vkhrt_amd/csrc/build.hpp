@@ -14,6 +14,7 @@ struct BuiltBvh {
     uint32_t nnodes = 0;
     uint32_t* prim_ids = nullptr;
     float4* prims = nullptr;  // primitive records in leaf order
+    float* aux = nullptr;     // phantom: early-out cylinder radius per leaf slot
     uint32_t depth = 0;       // wide levels
 };
 
@@ -26,6 +27,15 @@ struct TraceParams {
     uint32_t ninst;
     uint32_t flags;
     uint32_t* status;  // [0] stack overflow count
+    uint32_t* work;    // persistent kernels: global work counter (null = one thread per ray)
+    uint32_t refill_min;  // persistent kernels: idle lanes needed before a wave fetches new rays
+    // deferred FP64 polish (phantom POLISH variants): (ray, instance, slot) candidates
+    uint4* defer;
+    uint32_t* defer_count;
+    uint32_t defer_cap;
+    unsigned long long* keys;  // per ray (t bits << 32 | instance << 27 | primitive) for the merge
+    HitRec* cand_hits;         // per candidate
+    unsigned long long* cand_keys;
 };
 
 struct Camera {
@@ -46,6 +56,9 @@ struct RenderParams {
     uint32_t* queue;      // shadow queue (pixel indices)
     uint32_t* counters;   // [0] queue length
     float* light_vis;     // optional per-pixel visibility override (neural form), may be null
+    // deferred-polish pipeline scratch
+    HitRec* hitrec;       // per-pixel primary hit
+    float* vis;           // per-pixel shadow visibility
 };
 
 // Compile-time geometry variants of the traversal kernels. A TLAS whose instances share

@@ -541,6 +541,28 @@ __global__ void k_gather(const float4* __restrict__ in, const uint32_t* __restri
     out[i] = in[size_t(slots[s]) * rec + k];
 }
 
+// Early-out cylinder radius of each curve in leaf order (hair_intersection.rint:21-22):
+// CurveDistanceToCylinder(curve, SampleCurvePoint(curve, 0.5)) + radius, with the same IEEE
+// sequence as the intersector (no contraction), so the per-ray test is unchanged bit for bit.
+__global__ void k_phantom_rmax(const float4* __restrict__ prims, uint32_t n, int radius_mode, float radius,
+                               float* __restrict__ rmax) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = prims[size_t(i) * 4], b = prims[size_t(i) * 4 + 1], c = prims[size_t(i) * 4 + 2],
+                 d = prims[size_t(i) * 4 + 3];
+    const f3 p0 = f3{a.x, a.y, a.z}, p1 = f3{b.x, b.y, b.z}, p2 = f3{c.x, c.y, c.z}, p3 = f3{d.x, d.y, d.z};
+    float rpad = radius;
+    if (radius_mode == 1) rpad = gmax(gmax(a.w, b.w), gmax(c.w, d.w));
+    // SampleCurvePoint(curve, 0.5)
+    const float t = 0.5f, u = 1.0f - t, tt = t * t, uu = u * u, uuu = uu * u, ttt = tt * t;
+    const float w1 = 3.0f * uu * t, w2 = 3.0f * u * tt;
+    const f3 mid = f3{uuu * p0.x + w1 * p1.x + w2 * p2.x + ttt * p3.x, uuu * p0.y + w1 * p1.y + w2 * p2.y + ttt * p3.y,
+                      uuu * p0.z + w1 * p1.z + w2 * p2.z + ttt * p3.z};
+    float r = len3(cross3(sub(mid, p0), sub(mid, p3))) / len3(sub(p3, p0));
+    r += rpad;
+    rmax[i] = r;
+}
+
 __global__ void k_set_u32(uint32_t* p, uint32_t v, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -585,6 +607,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     WideNode* nodes = nullptr;
     uint32_t* slots = nullptr;
     float4* prims_sorted = nullptr;
+    float* aux = nullptr;
     hipError_t rc = hipSuccess;
     uint32_t nb = 0;
     uint32_t host_cnt[4] = {0, 0, 0, 0};
@@ -617,6 +640,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     VK_TRY(dalloc(&nodes, n));
     VK_TRY(dalloc(&slots, n));
     VK_TRY(dalloc(&prims_sorted, size_t(n) * rec));
+    if (mode == kPhantom) VK_TRY(dalloc(&aux, n));
 
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, cb);
     hipLaunchKernelGGL(k_prim_boxes, dim3(blocks(n)), dim3(kBlock), 0, st, prims_in, n, mode, radius_mode, radius, blo,
@@ -658,6 +682,9 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     VK_TRY(hipMemcpyAsync(cb_host, cb, 24, hipMemcpyDeviceToHost, st));
     hipLaunchKernelGGL(k_gather, dim3(blocks(uint64_t(n) * rec)), dim3(kBlock), 0, st, prims_in, slots, n, rec,
                        prims_sorted);
+    if (mode == kPhantom)
+        hipLaunchKernelGGL(k_phantom_rmax, dim3(blocks(n)), dim3(kBlock), 0, st, prims_sorted, n, radius_mode, radius,
+                           aux);
     VK_TRY(hipGetLastError());
     VK_TRY(hipStreamSynchronize(st));
     if (host_cnt[3] != n || host_cnt[2] > n) {
@@ -674,6 +701,8 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     out->nnodes = host_cnt[2];
     out->prim_ids = slots;
     out->prims = prims_sorted;
+    out->aux = aux;
+    aux = nullptr;
     nodes = nullptr;
     slots = nullptr;
     prims_sorted = nullptr;
@@ -702,6 +731,7 @@ done:
     hipFree(nodes);
     hipFree(slots);
     hipFree(prims_sorted);
+    hipFree(aux);
     return rc;
 }
 

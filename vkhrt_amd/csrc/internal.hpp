@@ -34,6 +34,7 @@ static_assert(sizeof(WideNode) == 80, "wide node is 80 bytes");
 
 struct BlasDev {
     const float4* prims;
+    const float* aux;  // phantom: precomputed early-out cylinder radius per leaf slot
     const uint32_t* prim_ids;
     const WideNode* nodes;
     uint32_t nprims;    // leaf primitives (curves / capsules / DOTS pieces)

@@ -80,7 +80,8 @@ __device__ __forceinline__ R bez_scalar_axis(R a, R b, R c, R d, R t) {
 
 // cone.glsl:21-62 for the quantities Prhi uses (isIntersection, s, dt).
 template <class R>
-__device__ __forceinline__ void cone_rcc(R cx, R cy, R ax, R ay, R az, R radius, R slant, bool& isect, R& s, R& dt) {
+__device__ __forceinline__ void cone_rcc(R cx, R cy, R ax, R ay, R az, R radius, R slant, bool& isect, R& s, R& dt,
+                                         bool& grazing) {
     R r2 = radius * radius;
     R drr = radius * slant;
     R ddd = ax * ax + ay * ay;
@@ -93,6 +94,7 @@ __device__ __forceinline__ void cone_rcc(R cx, R cy, R ax, R ay, R az, R radius,
     ddd += cdz2;
     R a = R(2) * drr * cdd + cxd * cxd - ddd * r2 + dp * cdz2;
     R det = b * b - a * c;
+    grazing = fabs(a) <= R(1e-2f) * (ddd * r2);
     isect = det > R(0);
     s = (b - (isect ? sqrt(det) : R(0))) / c;
     dt = (s * az - cdd) / ddd;
@@ -123,14 +125,21 @@ struct PhantomHit {
     float u;      // curve parameter at convergence
     float ncx, ncy, nsz;  // hit - curve(u) in the ray-centric frame
     bool polished;
+    bool deferred;  // kPolishDefer: this curve needs the FP64 continuation (evaluated later)
 };
+
+// FP64 polish handling inside Prhi: off, evaluated in place, or deferred (the FP32 pass
+// stops at the first side that would enter the FP64 continuation and flags the curve; a
+// later kernel re-runs the full Prhi with kPolishInline for flagged ray/curve pairs).
+enum PolishMode { kPolishOff = 0, kPolishInline = 1, kPolishDefer = 2 };
 
 // One side of the Prhi iteration (hair_intersection.rint:46-116) on an RCC-space curve.
 template <class V, class R>
 __device__ __forceinline__ int prhi_iterate(V q0, V q1, V q2, V q3, const R rr[4], int radius_mode, R const_r, R& t,
                                             R& told, R& dt1, R& dt2, int i0, int n, R& s_rel, R& cx, R& cy, R& cz,
                                             R& last_dt) {
-    // returns 1 converged, 0 left [0,1], 2 exhausted
+    // returns 1 converged, 0 left [0,1], 2 exhausted, 3 exhausted while grazing
+    bool last_grazing = false;
     for (int i = i0; i < i0 + n; ++i) {
         V c = bez_point(q0, q1, q2, q3, t);
         V ax = bez_axis(q0, q1, q2, q3, t);
@@ -142,9 +151,10 @@ __device__ __forceinline__ int prhi_iterate(V q0, V q1, V q2, V q3, const R rr[4
             radius = bez_scalar(rr[0], rr[1], rr[2], rr[3], t);
             slant = bez_scalar_axis(rr[0], rr[1], rr[2], rr[3], t);
         }
-        bool isect;
+        bool isect, grazing;
         R s, dt;
-        cone_rcc(c.x, c.y, ax.x, ax.y, ax.z, radius, slant, isect, s, dt);
+        cone_rcc(c.x, c.y, ax.x, ax.y, ax.z, radius, slant, isect, s, dt, grazing);
+        last_grazing = grazing;
         if (isect && fabs(dt) < R(5e-5f)) {
             s_rel = s;
             cx = c.x;
@@ -171,12 +181,14 @@ __device__ __forceinline__ int prhi_iterate(V q0, V q1, V q2, V q3, const R rr[4
         }
         if (t < R(0) || t > R(1)) return 0;
     }
-    return 2;
+    return last_grazing ? 3 : 2;
 }
 
 // FP64 continuation of a grazing side (extension, no reference counterpart): the FP32 side
-// used all 8 iterations, its final t is in [0, 1] and |dt| < 0.05. The same update rule
-// runs 8 more iterations in double on the curve transformed by a double RCC matrix.
+// used all 8 iterations, its final t is in [0, 1], |dt| < 0.05 and the ray grazes the tube
+// at the last iterate (|a| <= 1e-2 |axis|^2 r^2, a ~ |axis|^2 (d^2 - r^2) with d the ray's
+// distance to the cone axis). The same update rule runs 8 more iterations in double on the
+// curve transformed by a double RCC matrix.
 __device__ __forceinline__ bool prhi_polish_f64(f3 of, f3 df, const float4* cv, int radius_mode, float const_r,
                                                 float t32, float told32, float dt1_32, float dt2_32, PhantomHit& out) {
     d3 o = tod(of), d = tod(df);
@@ -196,7 +208,7 @@ __device__ __forceinline__ bool prhi_polish_f64(f3 of, f3 df, const float4* cv, 
     int st = prhi_iterate(q[0], q[1], q[2], q[3], rr, radius_mode, double(const_r), t, told, dt1, dt2, 8, 8, s_rel, cx,
                           cy, cz, last);
     if (st != 1) return false;
-    double s = s_rel + cz;
+    const double s = s_rel + cz;
     out.t = float(s);
     out.u = float(t);
     // normal in world space from the double frame, rounded once
@@ -212,17 +224,15 @@ __device__ __forceinline__ bool prhi_polish_f64(f3 of, f3 df, const float4* cv, 
 // hair_intersection.rint:15-130 Prhi. Returns true with out.t > 0 on an intersection.
 // RMODE (constant / per-point radius) and POLISH are compile-time so each traversal kernel
 // carries only the arithmetic it uses and no device-function calls.
-template <int RMODE, bool POLISH>
-__device__ __forceinline__ bool prhi(const RayFrame& F, const float4* cv, float const_r, PhantomHit& out) {
+// `rmax` is the curve's early-out cylinder radius (:21-22), precomputed per curve at build
+// time with the same arithmetic.
+template <int RMODE, int PMODE>
+__device__ __forceinline__ bool prhi(const RayFrame& F, const float4* cv, float rmax, float const_r, PhantomHit& out) {
+    out.deferred = false;
     constexpr int radius_mode = RMODE;
     const f3 p0 = f3{cv[0].x, cv[0].y, cv[0].z}, p1 = f3{cv[1].x, cv[1].y, cv[1].z};
     const f3 p2 = f3{cv[2].x, cv[2].y, cv[2].z}, p3 = f3{cv[3].x, cv[3].y, cv[3].z};
-    float rpad = const_r;
-    if (radius_mode == kRadiusPerPoint) rpad = gmax(gmax(cv[0].w, cv[1].w), gmax(cv[2].w, cv[3].w));
-    // early out (:21-33)
-    f3 mid = bez_point(p0, p1, p2, p3, 0.5f);
-    float rmax = len3(cross3(sub(mid, p0), sub(mid, p3))) / len3(sub(p3, p0));
-    rmax += rpad;
+    // early out (:24-33)
     if (!ray_cylinder(F.o, F.d, p0, p3, rmax)) return false;
     // ray-centric curve (:38-39)
     f3 q0 = rcc_xform(F.M, p0), q1 = rcc_xform(F.M, p1), q2 = rcc_xform(F.M, p2), q3 = rcc_xform(F.M, p3);
@@ -245,10 +255,15 @@ __device__ __forceinline__ bool prhi(const RayFrame& F, const float4* cv, float 
                 out.polished = false;
                 return true;
             }
-        } else if (POLISH && st == 2 && t >= 0.0f && t <= 1.0f && fabsf(last) < 0.05f) {
+        } else if (PMODE != kPolishOff && st == 3 && t >= 0.0f && t <= 1.0f && fabsf(last) < 0.05f) {
+            if (PMODE == kPolishDefer) {
+                out.deferred = true;
+                return false;
+            }
             PhantomHit ph;
             if (prhi_polish_f64(F.o, F.d, cv, radius_mode, const_r, t, told, dt1, dt2, ph) && ph.t > 0.0f) {
                 out = ph;
+                out.deferred = false;
                 return true;
             }
         }

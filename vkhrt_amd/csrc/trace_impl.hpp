@@ -48,6 +48,16 @@ struct Best {
     uint32_t extra;    // polished flag / DOTS triangle sub-index
 };
 
+// Merge key of a hit: t (positive float, so its bits order like the value), then instance
+// and primitive (ties resolve to the lower pair). Requires primitive < 2^27, instance < 31.
+__device__ __forceinline__ unsigned long long pack_key(float t, uint32_t inst, uint32_t pid) {
+    return (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | ((inst << 27) | pid);
+}
+__device__ __forceinline__ unsigned long long best_key(const Best& b) {
+    return (static_cast<unsigned long long>(__float_as_uint(b.t)) << 32) |
+           (b.inst == 0xFFFFFFFFu ? 0xFFFFFFFFu : ((b.inst << 27) | b.pid));
+}
+
 __device__ __forceinline__ bool better(float t, uint32_t inst, uint32_t pid, const Best& B) {
     if (t < B.t) return true;
     if (t == B.t) return inst < B.inst || (inst == B.inst && pid < B.pid);
@@ -64,22 +74,9 @@ __device__ __forceinline__ float qbyte(uint32_t lo, uint32_t hi, int s) {
     return float((w >> (8 * (s & 3))) & 0xffu);
 }
 
-// Per-lane stack of node groups in LDS (stride kTraceBlock entries). A group is pushed
-// only when descending one level, so a tree of depth D never needs more than D entries;
-// the host picks STACK >= depth and a push past STACK reports an error instead of spilling.
-template <int STACK>
-struct LaneStack {
-    uint2* lds;
-    int sp;
-    __device__ __forceinline__ bool push(uint2 g) {
-        if (sp >= STACK) return false;
-        lds[sp * kTraceBlock] = g;
-        ++sp;
-        return true;
-    }
-    __device__ __forceinline__ uint2 top() const { return lds[(sp - 1) * kTraceBlock]; }
-};
-
+// Per-lane stack of node groups in LDS (stride kTraceBlock entries). A group is pushed only
+// when descending one level, so a tree of depth D never needs more than D entries; the host
+// picks STACK >= depth and a push past STACK reports an error instead of spilling.
 // group: x = child_base, y = imask | (remaining hit bits in octant order) << 8
 // Traverses one BLAS. Returns false on stack overflow.
 // Per-ray traversal statistics (nodes visited, primitive records tested).
@@ -87,28 +84,41 @@ struct TravStats {
     uint32_t nodes = 0, prims = 0;
 };
 
+enum StepResult { kStepMore = 0, kStepDone = 1, kStepHit = 2, kStepError = 3 };
+
+// Resumable traversal of one BLAS by one lane: begin() sets up the ray, each step() visits
+// one 8-wide node (and tests its leaf primitives) and selects the next node. Used whole by
+// traverse() and one step per iteration by the persistent kernels.
 template <int MODE, int RMODE, bool POLISH, bool ANY, int STACK, bool STATS = false>
-__device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmin, uint32_t inst, Best& best,
-                                         uint2* lds, bool& done, TravStats* st = nullptr) {
-    if (B.nprims == 0) return true;
-    const f3 inv = safe_inv(d);
-    const bool nx = signbit(d.x), ny = signbit(d.y), nz = signbit(d.z);
-    const uint32_t oct = uint32_t(nx) | (uint32_t(ny) << 1) | (uint32_t(nz) << 2);
+struct Walker {
+    f3 o, d, inv;
+    uint32_t oct;
     RayFrame F;
     TriRay TR;
-    if (MODE == kPhantom) make_frame(o, d, F);
-    if (MODE == kDots) TR = make_tri_ray(d);
+    uint2 cur;  // current group: x = child_base, y = imask | remaining hit bits (octant order) << 8
+    uint32_t node;
+    int sp;
+    bool have_node;
 
-    LaneStack<STACK> S;
-    S.lds = lds;
-    S.sp = 0;
-    uint2 cur = make_uint2(0u, 0u);  // current group (empty)
-    uint32_t node = 0;
-    bool have_node = true;
-    for (;;) {
+    __device__ __forceinline__ void begin(f3 o_, f3 d_) {
+        o = o_;
+        d = d_;
+        inv = safe_inv(d);
+        oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
+        if (MODE == kPhantom) make_frame(o, d, F);
+        if (MODE == kDots) TR = make_tri_ray(d);
+        cur = make_uint2(0u, 0u);
+        node = 0;
+        sp = 0;
+        have_node = true;
+    }
+
+    __device__ __forceinline__ int step(const BlasDev& B, float tmin, uint32_t inst, Best& best, uint2* lds,
+                                        TravStats* st, const TraceParams& tp, uint32_t ray_id) {
         if (have_node) {
-            if (node >= B.nnodes) return false;  // malformed structure: report, never read out of bounds
+            if (node >= B.nnodes) return kStepError;  // malformed structure: report, never read out of bounds
             if (STATS) ++st->nodes;
+            const bool nx = oct & 1u, ny = oct & 2u, nz = oct & 4u;
             const uint4* np = reinterpret_cast<const uint4*>(B.nodes + node);
             const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
             const float sx = __uint_as_float((n0.w & 0xffu) << 23) * inv.x;
@@ -119,13 +129,13 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
             const float oz = (__uint_as_float(n0.z) - o.z) * inv.z;
             const uint32_t imask = n0.w >> 24;
             // near/far plane bytes per axis by ray direction sign
-            const uint32_t nxl = nx ? n3.z : n2.x, nxh = nx ? n3.w : n2.y;  // near x
-            const uint32_t fxl = nx ? n2.x : n3.z, fxh = nx ? n2.y : n3.w;  // far x
+            const uint32_t nxl = nx ? n3.z : n2.x, nxh = nx ? n3.w : n2.y;
+            const uint32_t fxl = nx ? n2.x : n3.z, fxh = nx ? n2.y : n3.w;
             const uint32_t nyl = ny ? n4.x : n2.z, nyh = ny ? n4.y : n2.w;
             const uint32_t fyl = ny ? n2.z : n4.x, fyh = ny ? n2.w : n4.y;
             const uint32_t nzl = nz ? n4.z : n3.x, nzh = nz ? n4.w : n3.y;
             const uint32_t fzl = nz ? n3.x : n4.z, fzh = nz ? n3.y : n4.w;
-            const float tcur = ANY ? best.t : best.t;
+            const float tcur = best.t;
             uint32_t hits = 0, pmask = 0;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
@@ -145,7 +155,7 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
             }
             // leaf primitives of this node
             const uint32_t pbase = n1.y;
-            if (pmask && pbase + (32u - __clz(pmask)) > B.nprims) return false;
+            if (pmask && pbase + (32u - __clz(pmask)) > B.nprims) return kStepError;
             while (pmask) {
                 const uint32_t k = __ffs(pmask) - 1;
                 pmask &= pmask - 1;
@@ -155,7 +165,16 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
                     const float4* pr = B.prims + size_t(slot) * 4;
                     float4 cv[4] = {pr[0], pr[1], pr[2], pr[3]};
                     PhantomHit ph;
-                    if (prhi<RMODE, POLISH>(F, cv, B.radius, ph) && ph.t >= tmin && ph.t <= best.t) {
+                    const bool hit = prhi<RMODE, POLISH ? kPolishDefer : kPolishOff>(F, cv, B.aux[slot], B.radius, ph);
+                    if (POLISH && ph.deferred) {
+                        // this curve needs the FP64 continuation: queue (ray, instance, slot)
+                        const uint32_t q = atomicAdd(tp.defer_count, 1u);
+                        if (q < tp.defer_cap)
+                            tp.defer[q] = make_uint4(ray_id, inst, slot, 0u);
+                        else
+                            atomicAdd(&tp.status[1], 1u);
+                    }
+                    if (hit && ph.t >= tmin && ph.t <= best.t) {
                         const uint32_t pid = B.prim_ids[slot];
                         if (better(ph.t, inst, pid, best)) {
                             best.t = ph.t;
@@ -167,10 +186,7 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
                             best.b = ph.ncy;
                             best.c = ph.nsz;
                             best.extra = ph.polished ? 1u : 0u;
-                            if (ANY) {
-                                done = true;
-                                return true;
-                            }
+                            if (ANY) return kStepHit;
                         }
                     }
                 } else if (MODE == kLss) {
@@ -190,10 +206,7 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
                             best.b = lh.n.y;
                             best.c = lh.n.z;
                             best.extra = 0;
-                            if (ANY) {
-                                done = true;
-                                return true;
-                            }
+                            if (ANY) return kStepHit;
                         }
                     }
                 } else {
@@ -216,10 +229,7 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
                                 best.u = b1;
                                 best.a = b2;
                                 best.extra = uint32_t(tri);
-                                if (ANY) {
-                                    done = true;
-                                    return true;
-                                }
+                                if (ANY) return kStepHit;
                             }
                         }
                     }
@@ -229,15 +239,17 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
             if (hits) {
                 const uint2 g = make_uint2(n1.x, imask | (hits << 8));
                 if (cur.y >> 8) {
-                    if (!S.push(cur)) return false;
+                    if (sp >= STACK) return kStepError;
+                    lds[sp * kTraceBlock] = cur;
+                    ++sp;
                 }
                 cur = g;
             }
         }
         if ((cur.y >> 8) == 0) {
-            if (S.sp == 0) break;
-            cur = S.top();
-            --S.sp;
+            if (sp == 0) return kStepDone;
+            --sp;
+            cur = lds[sp * kTraceBlock];
         }
         // next child of the current group
         uint32_t hb = cur.y >> 8;
@@ -248,8 +260,24 @@ __device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmi
         node = cur.x + __popc(im & ((1u << slot) - 1u));
         cur.y = im | (hb << 8);
         have_node = true;
+        return kStepMore;
     }
-    return true;
+};
+
+// Whole traversal of one BLAS. Returns false on a malformed structure or stack overflow.
+template <int MODE, int RMODE, bool POLISH, bool ANY, int STACK, bool STATS = false>
+__device__ __forceinline__ bool traverse(const BlasDev& B, f3 o, f3 d, float tmin, uint32_t inst, Best& best,
+                                         uint2* lds, bool& done, const TraceParams& tp, uint32_t ray_id,
+                                         TravStats* st = nullptr) {
+    if (B.nprims == 0) return true;
+    Walker<MODE, RMODE, POLISH, ANY, STACK, STATS> W;
+    W.begin(o, d);
+    for (;;) {
+        const int r = W.step(B, tmin, inst, best, lds, st, tp, ray_id);
+        if (r == kStepMore) continue;
+        if (r == kStepHit) done = true;
+        return r != kStepError;
+    }
 }
 
 __device__ __forceinline__ f3 xform_pt(const float* m /*col-major 4x4*/, f3 p) {
@@ -277,7 +305,7 @@ __device__ __forceinline__ bool ray_box(f3 o, f3 d, const float* lo, const float
 // Traces one world-space ray through all instances. Fills `best`; returns false on overflow.
 template <class G, bool ANY, int STACK, bool STATS = false>
 __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, float tmin, float tmax, Best& best,
-                                            uint2* lds, TravStats* st = nullptr) {
+                                            uint2* lds, uint32_t ray_id, TravStats* st = nullptr) {
     best.t = tmax;
     best.inst = 0xFFFFFFFFu;
     best.pid = 0xFFFFFFFFu;
@@ -298,19 +326,19 @@ __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, f
         if constexpr (G::kMixed) {
             const bool pol = (tp.flags & kTraceFp64Polish) != 0;
             if (I.blas.mode == kLss) {
-                r = traverse<kLss, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done);
+                r = traverse<kLss, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
             } else if (I.blas.mode == kDots) {
-                r = traverse<kDots, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done);
+                r = traverse<kDots, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
             } else if (I.blas.radius_mode == kRadiusPerPoint) {
-                r = pol ? traverse<kPhantom, kRadiusPerPoint, true, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done)
-                        : traverse<kPhantom, kRadiusPerPoint, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done);
+                r = pol ? traverse<kPhantom, kRadiusPerPoint, true, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id)
+                        : traverse<kPhantom, kRadiusPerPoint, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
             } else {
-                r = pol ? traverse<kPhantom, kRadiusConstant, true, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done)
-                        : traverse<kPhantom, kRadiusConstant, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done);
+                r = pol ? traverse<kPhantom, kRadiusConstant, true, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id)
+                        : traverse<kPhantom, kRadiusConstant, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
             }
         } else {
             r = traverse<G::kMode, G::kRMode, G::kPolish, ANY, STACK, STATS>(I.blas, oo, dd, tmin, i, best, lds, done,
-                                                                             st);
+                                                                             tp, ray_id, st);
         }
         ok = ok && r;
         if (ANY && done) break;
@@ -378,8 +406,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceParams tp, cons
     const f3 o = f3{r0.x, r0.y, r0.z}, d = f3{r1.x, r1.y, r1.z};
     Best best;
     TravStats ts;
-    const bool ok = trace_world<G, ANY, STACK, STATS>(tp, o, d, r0.w, r1.w, best, stk + threadIdx.x, &ts);
+    const bool ok = trace_world<G, ANY, STACK, STATS>(tp, o, d, r0.w, r1.w, best, stk + threadIdx.x, i, &ts);
     if (STATS) stats[i] = make_uint2(ts.nodes, ts.prims);
+    if (tp.keys) tp.keys[i] = best_key(best);
     if (!ok) atomicAdd(&tp.status[0], 1u);
     HitRec h;
     finish_hit(tp, o, d, best, h);
@@ -437,7 +466,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary(RenderParams rp)
         f3 o, d;
         primary_ray(rp.cam, x, y, o, d);
         Best best;
-        bool ok = trace_world<G, false, STACK>(rp.tp, o, d, 0.001f, 10000.0f, best, stk + threadIdx.x);
+        bool ok = trace_world<G, false, STACK>(rp.tp, o, d, 0.001f, 10000.0f, best, stk + threadIdx.x, pix);
         if (!ok) atomicAdd(&rp.tp.status[0], 1u);
         HitRec h;
         finish_hit(rp.tp, o, d, best, h);
@@ -484,11 +513,384 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow(RenderParams rp) 
     } else {
         const f3 hp = add(o, scl(hb.w, d));
         Best best;
-        bool ok = trace_world<G, true, STACK>(rp.tp, hp, f3{0.0f, 1.0f, 0.0f}, 0.001f, 10000.0f, best, stk + threadIdx.x);
+        bool ok = trace_world<G, true, STACK>(rp.tp, hp, f3{0.0f, 1.0f, 0.0f}, 0.001f, 10000.0f, best, stk + threadIdx.x,
+                                              qi);
         if (!ok) atomicAdd(&rp.tp.status[0], 1u);
         vis = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
     }
     store_px(rp, pix, shade(f3{hb.x, hb.y, hb.z}, vis));
+}
+
+// ------------------------------------------------------------------ persistent lanes
+// Persistent wavefronts with per-lane refill: every iteration each active lane performs one
+// walker step; lanes whose ray finished take new work from a global counter (one atomic per
+// wave per refill), so a wavefront is not held back by its slowest ray. src(idx, o, d, tmin,
+// tmax) produces work item idx; snk(idx, o, d, best) consumes its result (called by the
+// lanes that finish in the same iteration, so it may use wave ballots).
+template <class G, bool ANY, int STACK, class Src, class Snk>
+__device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t total, uint32_t* counter, uint2* lds,
+                                                 Src&& src, Snk&& snk) {
+    static_assert(!G::kMixed, "persistent kernels serve single-mode TLASes");
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    Walker<G::kMode, G::kRMode, G::kPolish, ANY, STACK> W;
+    Best best;
+    f3 ow = f3{0, 0, 0}, dw = f3{0, 0, 0};
+    float tmin = 0.0f, tmax = 0.0f;
+    uint32_t idx = 0, inst = 0;
+    bool active = false, need_inst = false;
+    bool exhausted = false;  // wave-uniform
+    for (;;) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            if (idle && (__popcll(idle) >= tp.refill_min || idle == __ballot(true))) {
+                const uint32_t n = __popcll(idle);
+                const uint32_t leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(counter, n);
+                base = __shfl(base, int(leader));
+                if (base + n >= total) exhausted = true;
+                if (!active) {
+                    const uint32_t r = base + __popcll(idle & lt);
+                    if (r < total) {
+                        idx = r;
+                        src(idx, ow, dw, tmin, tmax);
+                        best.t = tmax;
+                        best.inst = 0xFFFFFFFFu;
+                        best.pid = 0xFFFFFFFFu;
+                        best.slot = 0xFFFFFFFFu;
+                        best.u = best.a = best.b = best.c = 0.0f;
+                        best.extra = 0;
+                        inst = 0;
+                        need_inst = true;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__ballot(active)) break;
+        if (active) {
+            bool finish = false;
+            if (need_inst) {
+                // next instance whose world box the ray overlaps (top_level_acceleration_structure.cpp:23-43)
+                while (inst < tp.ninst) {
+                    const InstanceDev& I = tp.inst[inst];
+                    if (I.blas.nprims != 0 && ray_box(ow, dw, I.lo, I.hi, tmin, best.t)) break;
+                    ++inst;
+                }
+                if (inst >= tp.ninst) {
+                    finish = true;
+                } else {
+                    const InstanceDev& I = tp.inst[inst];
+                    f3 oo = ow, dd = dw;
+                    if (!I.identity) {
+                        oo = xform_pt(I.inv, ow);
+                        dd = xform_dir(I.inv, dw);
+                    }
+                    W.begin(oo, dd);
+                    need_inst = false;
+                }
+            } else {
+                const int r = W.step(tp.inst[inst].blas, tmin, inst, best, lds, nullptr, tp, idx);
+                if (r == kStepError) {
+                    atomicAdd(&tp.status[0], 1u);
+                    finish = true;
+                } else if (r == kStepHit) {
+                    finish = true;
+                } else if (r == kStepDone) {
+                    ++inst;
+                    need_inst = true;
+                }
+            }
+            if (finish) {
+                snk(idx, ow, dw, best);
+                active = false;
+            }
+        }
+    }
+}
+
+template <class G, bool ANY, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TraceParams tp, const float* __restrict__ rays,
+                                                                       uint32_t n, HitRec* __restrict__ hits,
+                                                                       uint32_t* __restrict__ counter) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    persistent_lanes<G, ANY, STACK>(
+        tp, n, counter, stk + threadIdx.x,
+        [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
+            const float4 r0 = reinterpret_cast<const float4*>(rays)[size_t(i) * 2];
+            const float4 r1 = reinterpret_cast<const float4*>(rays)[size_t(i) * 2 + 1];
+            o = f3{r0.x, r0.y, r0.z};
+            d = f3{r1.x, r1.y, r1.z};
+            tmin = r0.w;
+            tmax = r1.w;
+        },
+        [&](uint32_t i, f3 o, f3 d, const Best& best) {
+            HitRec h;
+            finish_hit(tp, o, d, best, h);
+            hits[i] = h;
+            if (G::kPolish) tp.keys[i] = best_key(best);
+        });
+}
+
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_render_primary_persistent(RenderParams rp) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t npix = rp.cam.width * rp.cam.height;
+    persistent_lanes<G, false, STACK>(
+        rp.tp, npix, rp.counters + 1, stk + threadIdx.x,
+        [&](uint32_t pix, f3& o, f3& d, float& tmin, float& tmax) {
+            primary_ray(rp.cam, pix % rp.cam.width, pix / rp.cam.width, o, d);
+            tmin = 0.001f;
+            tmax = 10000.0f;
+        },
+        [&](uint32_t pix, f3 o, f3 d, const Best& best) {
+            HitRec h;
+            finish_hit(rp.tp, o, d, best, h);
+            bool hit = h.prim_id != 0xFFFFFFFFu;
+            if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
+            const f3 n = f3{h.n[0], h.n[1], h.n[2]};
+            if (!hit) {
+                store_px(rp, pix, f3{0.0f, 0.0f, 0.0f});
+            } else if (rp.debug_palette) {
+                store_px(rp, pix, debug_color(h.prim_id));
+                hit = false;
+            } else if (!rp.shadows) {
+                store_px(rp, pix, shade(n, 1.0f));
+                hit = false;
+            } else {
+                rp.hitbuf[pix] = make_float4(n.x, n.y, n.z, h.t);
+            }
+            // append to the shadow queue, aggregated over the lanes finishing now
+            const uint64_t m = __ballot(hit);
+            if (m) {
+                const uint32_t lane = threadIdx.x & 63u;
+                const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&rp.counters[0], uint32_t(__popcll(m)));
+                base = __shfl(base, int(leader));
+                if (hit) rp.queue[base + __popcll(m & ((1ull << lane) - 1ull))] = pix;
+            }
+        });
+}
+
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_render_shadow_persistent(RenderParams rp) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t total = rp.counters[0];
+    persistent_lanes<G, true, STACK>(
+        rp.tp, total, rp.counters + 2, stk + threadIdx.x,
+        [&](uint32_t qi, f3& o, f3& d, float& tmin, float& tmax) {
+            const uint32_t pix = rp.queue[qi];
+            f3 po, pd;
+            primary_ray(rp.cam, pix % rp.cam.width, pix / rp.cam.width, po, pd);
+            const float4 hb = rp.hitbuf[pix];
+            o = add(po, scl(hb.w, pd));
+            d = f3{0.0f, 1.0f, 0.0f};
+            tmin = 0.001f;
+            tmax = 10000.0f;
+        },
+        [&](uint32_t qi, f3 o, f3 d, const Best& best) {
+            const uint32_t pix = rp.queue[qi];
+            const float4 hb = rp.hitbuf[pix];
+            const float vis = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
+            store_px(rp, pix, shade(f3{hb.x, hb.y, hb.z}, vis));
+        });
+}
+
+// ------------------------------------------------------------------ deferred FP64 polish
+
+// Ray sources for candidate evaluation: the ray id stored with a candidate -> world ray.
+struct SrcRays {
+    const float4* rays;
+    __device__ __forceinline__ void get(uint32_t i, f3& o, f3& d, float& tmin, float& tmax) const {
+        const float4 r0 = rays[size_t(i) * 2], r1 = rays[size_t(i) * 2 + 1];
+        o = f3{r0.x, r0.y, r0.z};
+        d = f3{r1.x, r1.y, r1.z};
+        tmin = r0.w;
+        tmax = r1.w;
+    }
+};
+struct SrcCamera {
+    Camera cam;
+    __device__ __forceinline__ void get(uint32_t pix, f3& o, f3& d, float& tmin, float& tmax) const {
+        primary_ray(cam, pix % cam.width, pix / cam.width, o, d);
+        tmin = 0.001f;
+        tmax = 10000.0f;
+    }
+};
+struct SrcShadow {
+    Camera cam;
+    const uint32_t* queue;
+    const float4* hitbuf;
+    __device__ __forceinline__ void get(uint32_t qi, f3& o, f3& d, float& tmin, float& tmax) const {
+        const uint32_t pix = queue[qi];
+        f3 po, pd;
+        primary_ray(cam, pix % cam.width, pix / cam.width, po, pd);
+        o = add(po, scl(hitbuf[pix].w, pd));
+        d = f3{0.0f, 1.0f, 0.0f};
+        tmin = 0.001f;
+        tmax = 10000.0f;
+    }
+};
+
+// Full Prhi (FP32 sides and the FP64 continuation) for one queued (ray, instance, slot).
+__device__ __forceinline__ bool eval_candidate(const TraceParams& tp, uint4 c, f3 ow, f3 dw, float tmin, float tmax,
+                                               HitRec& h, unsigned long long& key) {
+    const InstanceDev& I = tp.inst[c.y];
+    f3 oo = ow, dd = dw;
+    if (!I.identity) {
+        oo = xform_pt(I.inv, ow);
+        dd = xform_dir(I.inv, dw);
+    }
+    RayFrame F;
+    make_frame(oo, dd, F);
+    const float4* pr = I.blas.prims + size_t(c.z) * 4;
+    float4 cv[4] = {pr[0], pr[1], pr[2], pr[3]};
+    PhantomHit ph;
+    const bool hit = I.blas.radius_mode == kRadiusPerPoint
+                         ? prhi<kRadiusPerPoint, kPolishInline>(F, cv, I.blas.aux[c.z], I.blas.radius, ph)
+                         : prhi<kRadiusConstant, kPolishInline>(F, cv, I.blas.aux[c.z], I.blas.radius, ph);
+    if (!hit || !(ph.t >= tmin && ph.t <= tmax)) return false;
+    const uint32_t pid = I.blas.prim_ids[c.z];
+    key = pack_key(ph.t, c.y, pid);
+    f3 n = phantom_normal(dd, ph);
+    if (!I.identity) {
+        const float* x = I.xf;
+        n = nrm3(f3{x[0] * n.x + x[1] * n.y + x[2] * n.z, x[4] * n.x + x[5] * n.y + x[6] * n.z,
+                    x[8] * n.x + x[9] * n.y + x[10] * n.z});
+    }
+    h.t = ph.t;
+    h.prim_id = pid;
+    h.inst_id = c.y;
+    h.u = ph.u;
+    h.n[0] = n.x;
+    h.n[1] = n.y;
+    h.n[2] = n.z;
+    h.v = ph.polished ? 1.0f : 0.0f;
+    return true;
+}
+
+// Closest-hit merge: the candidate's key competes with the ray's FP32 result by atomicMin.
+template <class Src>
+__global__ void k_polish_closest(TraceParams tp, Src src, HitRec* __restrict__ cand_hits,
+                                 unsigned long long* __restrict__ cand_keys) {
+    const uint32_t n = min(*tp.defer_count, tp.defer_cap);
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < n; ci += gridDim.x * blockDim.x) {
+        const uint4 c = tp.defer[ci];
+        f3 o, d;
+        float tmin, tmax;
+        src.get(c.x, o, d, tmin, tmax);
+        HitRec h;
+        unsigned long long key = ~0ull;
+        if (eval_candidate(tp, c, o, d, tmin, tmax, h, key)) {
+            atomicMin(&tp.keys[c.x], key);
+            cand_hits[ci] = h;
+        }
+        cand_keys[ci] = key;
+    }
+}
+
+// The candidate that won the merge writes its hit record.
+__global__ void k_polish_fixup(TraceParams tp, const HitRec* __restrict__ cand_hits,
+                               const unsigned long long* __restrict__ cand_keys, HitRec* __restrict__ out) {
+    const uint32_t n = min(*tp.defer_count, tp.defer_cap);
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < n; ci += gridDim.x * blockDim.x) {
+        const unsigned long long k = cand_keys[ci];
+        const uint4 c = tp.defer[ci];
+        if (k != ~0ull && k == tp.keys[c.x]) out[c.x] = cand_hits[ci];
+    }
+}
+
+// Shadow rays: any polished hit in range occludes.
+__global__ void k_polish_shadow(TraceParams tp, SrcShadow src, float* __restrict__ vis) {
+    const uint32_t n = min(*tp.defer_count, tp.defer_cap);
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < n; ci += gridDim.x * blockDim.x) {
+        const uint4 c = tp.defer[ci];
+        f3 o, d;
+        float tmin, tmax;
+        src.get(c.x, o, d, tmin, tmax);
+        HitRec h;
+        unsigned long long key;
+        if (eval_candidate(tp, c, o, d, tmin, tmax, h, key)) vis[src.queue[c.x]] = 0.0f;
+    }
+}
+
+// Deferred-polish render, pass 1: primary rays (FP32) -> per-pixel hit record and merge key.
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderParams rp) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t npix = rp.cam.width * rp.cam.height;
+    persistent_lanes<G, false, STACK>(
+        rp.tp, npix, rp.counters + 1, stk + threadIdx.x,
+        [&](uint32_t pix, f3& o, f3& d, float& tmin, float& tmax) {
+            primary_ray(rp.cam, pix % rp.cam.width, pix / rp.cam.width, o, d);
+            tmin = 0.001f;
+            tmax = 10000.0f;
+        },
+        [&](uint32_t pix, f3 o, f3 d, const Best& best) {
+            HitRec h;
+            finish_hit(rp.tp, o, d, best, h);
+            rp.hitrec[pix] = h;
+            rp.tp.keys[pix] = best_key(best);
+        });
+}
+
+// Pass 2 (after the polish merge): shade misses / palette / unshadowed pixels, queue the rest.
+__global__ void k_render_classify(RenderParams rp) {
+    const uint32_t npix = rp.cam.width * rp.cam.height;
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (pix < npix) {
+        const HitRec h = rp.hitrec[pix];
+        hit = h.prim_id != 0xFFFFFFFFu;
+        if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
+        const f3 n = f3{h.n[0], h.n[1], h.n[2]};
+        if (!hit) {
+            store_px(rp, pix, f3{0.0f, 0.0f, 0.0f});
+        } else if (rp.debug_palette) {
+            store_px(rp, pix, debug_color(h.prim_id));
+            hit = false;
+        } else if (!rp.shadows) {
+            store_px(rp, pix, shade(n, 1.0f));
+            hit = false;
+        } else {
+            rp.hitbuf[pix] = make_float4(n.x, n.y, n.z, h.t);
+        }
+    }
+    const uint64_t m = __ballot(hit);
+    if (m) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&rp.counters[0], uint32_t(__popcll(m)));
+        base = __shfl(base, int(leader));
+        if (hit) rp.queue[base + __popcll(m & ((1ull << lane) - 1ull))] = pix;
+    }
+}
+
+// Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParams rp) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t total = rp.counters[0];
+    const SrcShadow src{rp.cam, rp.queue, rp.hitbuf};
+    persistent_lanes<G, true, STACK>(
+        rp.tp, total, rp.counters + 2, stk + threadIdx.x,
+        [&](uint32_t qi, f3& o, f3& d, float& tmin, float& tmax) { src.get(qi, o, d, tmin, tmax); },
+        [&](uint32_t qi, f3 o, f3 d, const Best& best) {
+            rp.vis[rp.queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
+        });
+}
+
+// Shades the queued hit pixels with a per-pixel visibility (neural form, or the deferred
+// pipeline's shadow result).
+__global__ void k_render_apply_visibility(RenderParams rp, const float* __restrict__ vis) {
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= rp.counters[0]) return;
+    const uint32_t pix = rp.queue[qi];
+    const float4 hb = rp.hitbuf[pix];
+    store_px(rp, pix, shade(f3{hb.x, hb.y, hb.z}, vis[pix]));
 }
 
 __global__ void k_zero(uint32_t* p, uint32_t n) {
@@ -508,6 +910,24 @@ __device__ __forceinline__ f3 xform_pt_pub(const float* m, f3 p) { return trace_
 __device__ __forceinline__ f3 xform_dir_pub(const float* m, f3 p) { return trace_detail::xform_dir(m, p); }
 }  // namespace trace_detail_pub
 
+// Grid of a persistent kernel: enough 256-thread blocks to fill every CU at the occupancy
+// the LDS stack allows (4 wavefronts per block, at most 8 blocks per CU).
+template <class K>
+inline dim3 persistent_grid(K kernel) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel),
+                                                 trace_detail::kTraceBlock, 0);
+    if (per_cu < 1) per_cu = 1;
+    if (cus < 1) cus = 1;
+    return dim3(uint32_t(cus * per_cu));
+}
+
+template <class G, int STACK>
+hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, dim3 grid, dim3 block,
+                             bool any, hipStream_t st);
+
 // Launchers for one geometry variant and stack depth.
 template <class G, int STACK>
 hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
@@ -526,6 +946,37 @@ hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32
         }
         return hipErrorNotSupported;
     }
+    if (tp.defer) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
+    hipError_t e = launch_trace_pass<G, STACK>(tp, rays, n, hits, grid, block, any, st);
+    if (e == hipSuccess && tp.defer) {
+        const dim3 pgrid(1024);
+        hipLaunchKernelGGL(k_polish_closest<SrcRays>, pgrid, block, 0, st, tp,
+                           SrcRays{reinterpret_cast<const float4*>(rays)}, tp.cand_hits, tp.cand_keys);
+        hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, hits);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+template <class G, int STACK>
+hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, dim3 grid, dim3 block,
+                             bool any, hipStream_t st) {
+    using namespace trace_detail;
+    uint2* stats = nullptr;
+    if constexpr (!G::kMixed) {
+        if (tp.work) {
+            hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.work, 1u);
+            if (any)
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK>),
+                                   persistent_grid(k_trace_rays_persistent<G, true, STACK>), block, 0, st, tp, rays, n,
+                                   hits, tp.work);
+            else
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK>),
+                                   persistent_grid(k_trace_rays_persistent<G, false, STACK>), block, 0, st, tp, rays, n,
+                                   hits, tp.work);
+            return hipGetLastError();
+        }
+    }
     if (any)
         hipLaunchKernelGGL((k_trace_rays<G, true, STACK, false>), grid, block, 0, st, tp, rays, n, hits, stats);
     else
@@ -539,12 +990,59 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
     const uint32_t npix = rp.cam.width * rp.cam.height;
     if (npix == 0) return hipSuccess;
     const dim3 grid((npix + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, rp.counters, 4u);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, rp.counters, 8u);
+    if constexpr (!G::kMixed) {
+        if (rp.tp.defer) {  // deferred FP64 polish pipeline
+            const TraceParams& tp = rp.tp;
+            const dim3 pgrid(1024);
+            if (ev) hipEventRecord(ev[0], st);
+            hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>),
+                               block, 0, st, rp);
+            hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam}, tp.cand_hits,
+                               tp.cand_keys);
+            hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
+            hipLaunchKernelGGL(k_render_classify, grid, block, 0, st, rp);
+            if (ev) hipEventRecord(ev[1], st);
+            if (ev) hipEventRecord(ev[2], st);
+            if (rp.shadows && !rp.debug_palette) {
+                if (rp.light_vis) {
+                    hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.light_vis);
+                } else {
+                    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
+                    hipLaunchKernelGGL((k_render_shadow_defer<G, STACK>),
+                                       persistent_grid(k_render_shadow_defer<G, STACK>), block, 0, st, rp);
+                    hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp, SrcShadow{rp.cam, rp.queue, rp.hitbuf},
+                                       rp.vis);
+                    hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.vis);
+                }
+            }
+            if (ev) hipEventRecord(ev[3], st);
+            return hipGetLastError();
+        }
+    }
+    bool persistent = false;
+    if constexpr (!G::kMixed) persistent = rp.tp.work != nullptr;
     if (ev) hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL((k_render_primary<G, STACK>), grid, block, 0, st, rp);
+    if constexpr (!G::kMixed) {
+        if (persistent)
+            hipLaunchKernelGGL((k_render_primary_persistent<G, STACK>),
+                               persistent_grid(k_render_primary_persistent<G, STACK>), block, 0, st, rp);
+    }
+    if (!persistent) hipLaunchKernelGGL((k_render_primary<G, STACK>), grid, block, 0, st, rp);
     if (ev) hipEventRecord(ev[1], st);
     if (ev) hipEventRecord(ev[2], st);
-    if (rp.shadows && !rp.debug_palette) hipLaunchKernelGGL((k_render_shadow<G, STACK>), grid, block, 0, st, rp);
+    if (rp.shadows && !rp.debug_palette) {
+        if (rp.light_vis) {
+            hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.light_vis);
+        } else {
+            if constexpr (!G::kMixed) {
+                if (persistent)
+                    hipLaunchKernelGGL((k_render_shadow_persistent<G, STACK>),
+                                       persistent_grid(k_render_shadow_persistent<G, STACK>), block, 0, st, rp);
+            }
+            if (!persistent) hipLaunchKernelGGL((k_render_shadow<G, STACK>), grid, block, 0, st, rp);
+        }
+    }
     if (ev) hipEventRecord(ev[3], st);
     return hipGetLastError();
 }
