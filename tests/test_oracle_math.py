@@ -1,0 +1,429 @@
+"""Pins for the oracle's restatement of the reference GLSL (shaders/*.glsl, *.rint).
+
+Each function is checked against something independent of its own formula: closed forms
+(textbook ray/cylinder, ray/capsule and ray/triangle cases), invariants (orthonormality,
+A * inverse(A) = I), FP64 de Casteljau and finite differences, and brute-force root finding.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+RNG = np.random.default_rng(1234)
+
+
+def f32(*v):
+    return np.array(v, np.float32)
+
+
+def ray(o, d, tmin=0.001, tmax=10000.0):
+    return np.array(list(o) + [tmin] + list(d) + [tmax], np.float32)
+
+
+def curve(points, radii=(0.02, 0.02, 0.02, 0.02)):
+    c = np.zeros(16, np.float32)
+    for k in range(4):
+        c[k * 4:k * 4 + 3] = points[k]
+        c[k * 4 + 3] = radii[k]
+    return c
+
+
+# ------------------------------------------------------------------ ray.glsl
+def test_inverse_matches_identity():
+    L = O.lib()
+    for _ in range(50):
+        m = RNG.normal(size=(4, 4)).astype(np.float32)
+        m += 4 * np.eye(4, dtype=np.float32)  # well conditioned
+        inv = np.zeros(16, np.float32)
+        # column-major storage: m_col[c*4 + r] = A[r][c]
+        L.orc_inverse4(O.ptr(np.ascontiguousarray(m.T.reshape(16))), O.ptr(inv))
+        A = m.astype(np.float64)
+        B = inv.reshape(4, 4).T.astype(np.float64)
+        assert np.allclose(A @ B, np.eye(4), atol=2e-5)
+
+
+def test_basis_orthonormal_and_rcc_matrix():
+    L = O.lib()
+    for _ in range(200):
+        w = RNG.normal(size=3)
+        w /= np.linalg.norm(w)
+        w = w.astype(np.float32)
+        u = np.zeros(3, np.float32)
+        v = np.zeros(3, np.float32)
+        L.orc_basis(O.ptr(w), O.ptr(u), O.ptr(v))
+        assert abs(np.linalg.norm(u) - 1) < 1e-6 and abs(np.linalg.norm(v) - 1) < 1e-6
+        assert abs(u @ v) < 1e-6 and abs(v @ w) < 1e-6 and abs(u @ w) < 1e-6
+        assert np.allclose(np.cross(u, v), w, atol=1e-5) or np.allclose(np.cross(u, v), -w, atol=1e-5)
+    for _ in range(100):
+        o = RNG.uniform(-300, 300, 3)
+        d = RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        r = ray(o, d)
+        M = np.zeros(16, np.float64)
+        L.orc_rcc_matrix(O.ptr(r), 0, M.ctypes.data_as(O.ctypes.c_void_p))
+        Mm = M.reshape(4, 4).T
+        o4 = np.r_[r[:3].astype(np.float64), 1.0]
+        assert np.allclose((Mm @ o4)[:3], 0.0, atol=1e-3)  # origin -> 0
+        p = np.r_[r[:3].astype(np.float64) + 50.0 * r[4:7].astype(np.float64), 1.0]
+        assert np.allclose((Mm @ p)[:3], [0.0, 0.0, 50.0], atol=2e-3)  # along the ray -> +z
+
+
+# ------------------------------------------------------------------ curve.glsl
+def de_casteljau(P, t):
+    P = [np.asarray(p, np.float64) for p in P]
+    while len(P) > 1:
+        P = [(1 - t) * P[i] + t * P[i + 1] for i in range(len(P) - 1)]
+    return P[0]
+
+
+def test_sample_curve_point_and_axis():
+    L = O.lib()
+    for _ in range(50):
+        pts = RNG.uniform(-100, 100, (4, 3))
+        c = curve(pts)
+        for t in (0.0, 0.1, 0.5, 0.77, 1.0):
+            p = np.zeros(3)
+            a = np.zeros(3)
+            L.orc_sample_curve(O.ptr(c), t, 1, p.ctypes.data_as(O.ctypes.c_void_p),
+                               a.ctypes.data_as(O.ctypes.c_void_p))
+            ref = de_casteljau(c.reshape(4, 4)[:, :3], t)
+            assert np.allclose(p, ref, atol=1e-9)
+            h = 1e-6
+            lo, hi = max(0.0, t - h), min(1.0, t + h)
+            fd = (de_casteljau(c.reshape(4, 4)[:, :3], hi) - de_casteljau(c.reshape(4, 4)[:, :3], lo)) / (hi - lo)
+            assert np.allclose(a, fd, rtol=1e-4, atol=1e-4)
+
+
+# ------------------------------------------------------------------ cylinder.glsl
+@pytest.mark.parametrize("o,d,expect", [
+    ((5, 0, 0.5), (-1, 0, 0), True),      # through the body
+    ((5, 0, 1.5), (-1, 0, 0), False),     # above the top cap plane
+    ((0.5, 0, 5), (0.01, 0, -1), True),   # through the top cap
+    ((1.5, 0, 5), (0.01, 0, -1), False),  # nearly parallel to the axis, outside
+    ((5, 5, 0.5), (-1, 0, 0), False),     # passes beside
+    ((0.2, 0.1, -3), (0, 0.01, 1), True), # through the bottom cap from below
+    ((5, 0.99, 0.5), (-1, 0, 0), True),   # almost tangent, inside radius
+])
+def test_ray_cylinder_textbook(o, d, expect):
+    L = O.lib()
+    d = np.asarray(d, np.float64)
+    r = ray(o, d / np.linalg.norm(d))
+    assert L.orc_cylinder(O.ptr(r), O.ptr(f32(0, 0, 0)), O.ptr(f32(0, 0, 1)), 1.0) == int(expect)
+
+
+def test_ray_cylinder_axis_parallel_is_degenerate():
+    """cylinder.glsl divides by k2 = |ba|^2 - (ba.rd)^2, zero for a ray exactly parallel to the
+    axis: the reference then reports no intersection even through the caps."""
+    L = O.lib()
+    r = ray((0.5, 0, 5), (0, 0, -1))
+    assert L.orc_cylinder(O.ptr(r), O.ptr(f32(0, 0, 0)), O.ptr(f32(0, 0, 1)), 1.0) == 0
+
+
+# ------------------------------------------------------------------ cone.glsl
+def test_ray_cone_rcc_cylinder_reduction():
+    """slant = 0: ray (z axis) vs infinite cylinder of radius r about the line through center
+    with direction axis. det > 0 iff the line distance < r; s = near root - center.z."""
+    L = O.lib()
+    out = np.zeros(6, np.float32)
+    for _ in range(500):
+        c = RNG.uniform(-0.1, 0.1, 3)
+        c[2] = RNG.uniform(5, 50)
+        a = RNG.normal(size=3)
+        a[2] *= 0.3
+        r = 0.05
+        L.orc_cone(O.ptr(c.astype(np.float32)), r, O.ptr(a.astype(np.float32)), 0.0, O.ptr(out))
+        cc, aa = c.astype(np.float32).astype(np.float64), a.astype(np.float32).astype(np.float64)
+        ez = np.array([0.0, 0, 1])
+        n = np.cross(ez, aa)
+        dist = abs(cc @ n) / np.linalg.norm(n)
+        if abs(dist - r) < 1e-4:
+            continue
+        assert bool(out[0]) == (dist < r)
+        if dist < r:
+            # |(0,0,z) - c - ((0,0,z)-c).a_hat a_hat| = r  -> quadratic in z
+            ah = aa / np.linalg.norm(aa)
+            # p(z) = z*ez - cc ; perp = p - (p.ah) ah ; |perp|^2 = r^2
+            A = 1 - ah[2] ** 2
+            B = -2 * (cc[2] - (cc @ ah) * ah[2])
+            C = cc @ cc - (cc @ ah) ** 2 - r * r
+            z = (-B - math.sqrt(B * B - 4 * A * C)) / (2 * A)
+            assert abs((out[1] + cc[2]) - z) < 1e-3 * max(1.0, abs(z))
+
+
+# ------------------------------------------------------------------ Prhi
+def test_prhi_straight_curve_equals_cylinder():
+    """A straight Bezier (control points evenly spaced on a segment) is a capped cylinder of
+    radius 0.02: the phantom root must equal the analytic ray/infinite-cylinder entry t and
+    the normal must be radial."""
+    n_checked = 0
+    for _ in range(300):
+        a = RNG.uniform(-50, 50, 3)
+        dirc = RNG.normal(size=3)
+        dirc /= np.linalg.norm(dirc)
+        L_ = RNG.uniform(0.5, 3.0)
+        pts = [a + dirc * L_ * k / 3.0 for k in range(4)]
+        c = curve(pts)
+        # ray aimed at a point at distance < r from the axis, mid-segment
+        s = RNG.uniform(0.3, 0.7)
+        perp = np.cross(dirc, RNG.normal(size=3))
+        perp /= np.linalg.norm(perp)
+        q = a + dirc * L_ * s + perp * RNG.uniform(0.0, 0.015)
+        dr = RNG.normal(size=3)
+        dr -= (dr @ dirc) * dirc * 0.5
+        dr /= np.linalg.norm(dr)
+        o = q - dr * RNG.uniform(10, 100)
+        r = ray(o, dr)
+        out = O.prhi(r, c)
+        if out[0] <= 0:
+            continue
+        cpts = c.reshape(4, 4)[:, :3].astype(np.float64)
+        ax = cpts[3] - cpts[0]
+        ah = ax / np.linalg.norm(ax)
+        oo, dd = r[:3].astype(np.float64), r[4:7].astype(np.float64)
+        dd /= np.linalg.norm(dd)
+        w = oo - cpts[0]
+        A = 1 - (dd @ ah) ** 2
+        B = 2 * (w @ dd - (w @ ah) * (dd @ ah))
+        C = w @ w - (w @ ah) ** 2 - 0.02 ** 2
+        disc = B * B - 4 * A * C
+        assert disc > 0
+        t = (-B - math.sqrt(disc)) / (2 * A)
+        assert abs(out[0] - t) <= 2e-4 * max(1.0, t)
+        hp = oo + t * dd
+        radial = hp - cpts[0] - ((hp - cpts[0]) @ ah) * ah
+        radial /= np.linalg.norm(radial)
+        assert np.dot(out[5:8], radial) > 0.999
+        n_checked += 1
+    assert n_checked > 150
+
+
+def test_prhi_curved_vs_bruteforce():
+    """Hair segments from the generator (the method's domain: gently curved): the hit/miss
+    decision and the entry t agree with FP64 brute force on |ray(s) - curve(t)| = r away from
+    grazing (closest approach not within 10% of r). The phantom method is approximate
+    (two starting points, 8 iterations, cull by the midpoint's chord distance), so up to 1%
+    of decisions may differ; t agrees to 1e-3 where both hit. Rays within 18 degrees of the
+    chord (running along the fibre) are excluded: the method misses many of those by design."""
+    import scenes
+    curves = scenes.c0_curves(200, 16, seed=99)
+    rng = np.random.default_rng(5)
+    checked = wrong = 0
+    ts = np.linspace(0, 1, 2001)
+    for _ in range(600):
+        c = curves[rng.integers(0, curves.shape[0])].copy()
+        c[3::4] = 0.02
+        cp = c.reshape(4, 4)[:, :3].astype(np.float64)
+        u = 1 - ts
+        P = (u ** 3)[:, None] * cp[0] + (3 * u * u * ts)[:, None] * cp[1] + \
+            (3 * u * ts * ts)[:, None] * cp[2] + (ts ** 3)[:, None] * cp[3]
+        target = P[rng.integers(200, 1800)] + rng.normal(size=3) * 0.015
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        r = ray(target - d * 30.0, d)
+        out = O.prhi(r, c)
+        oo, dd = r[:3].astype(np.float64), r[4:7].astype(np.float64)
+        dd /= np.linalg.norm(dd)
+        s = (P - oo) @ dd
+        dist = np.linalg.norm(P - oo - s[:, None] * dd, axis=1)
+        dmin = dist.min()
+        if abs(dmin - 0.02) < 0.002:
+            continue  # grazing: left to the FP64 polish tests
+        if abs(dd @ (cp[3] - cp[0])) > 0.95 * np.linalg.norm(cp[3] - cp[0]):
+            continue
+        checked += 1
+        if (out[0] > 0) != (dmin < 0.02):
+            wrong += 1
+            continue
+        if out[0] > 0:
+            inside = dist < 0.02
+            t_entry = np.min(s[inside] - np.sqrt(0.02 ** 2 - dist[inside] ** 2))
+            assert abs(out[0] - t_entry) < 1e-3, (out[0], t_entry)
+    print(f"phantom vs brute force: {wrong}/{checked} decisions differ")
+    assert checked > 400 and wrong <= 0.01 * checked
+
+
+@pytest.mark.parametrize("dist", [(5.0, 10.0), (50.0, 60.0), (390.0, 400.0)])
+def test_prhi_fp32_vs_fp64_accuracy(dist):
+    """FP32 intersector vs the same intersector in FP64 on the C0 hair, by ray length.
+    Hit distance meets 1e-5 relative error at the 99th percentile. The shading normal does
+    not meet 1e-4 rad: its error is set by FP32 rounding of the ray-centric transform (a
+    generic 4x4 inverse of the ray frame, world coordinates ~100) relative to the 0.02
+    radius. Measured median 1.1e-4 .. 1.9e-4 rad, p99 1.1e-3 .. 2.3e-3 rad; the bounds below
+    state that."""
+    import scenes
+    curves = scenes.c0_curves()
+    rays = scenes.targeted_rays(curves, 3000, seed=2, dist=dist)
+    S = O.Scene()
+    S.add(O.MODE_PHANTOM, curves, per_point=True)
+    h = S.trace(rays)
+    rel, ang = [], []
+    for i in np.nonzero(h["prim"] != 0xFFFFFFFF)[0][:800]:
+        r64 = O.prhi_f64(rays[i], curves[h["prim"][i]], per_point=True)
+        if r64[0] <= 0:
+            continue
+        rel.append(abs(float(h["t"][i]) - r64[0]) / r64[0])
+        n32 = h["n"][i].astype(np.float64)
+        n64 = r64[5:8]
+        ang.append(math.acos(min(1.0, abs(n32 @ n64) / np.linalg.norm(n32) / np.linalg.norm(n64))))
+    rel, ang = np.array(rel), np.array(ang)
+    print(f"ray length {dist}: t rel err median {np.median(rel):.2g} p99 {np.percentile(rel, 99):.2g}; "
+          f"normal angle median {np.median(ang):.2g} p99 {np.percentile(ang, 99):.2g} rad")
+    assert len(rel) > 500
+    assert np.percentile(rel, 99) <= 1e-5
+    assert np.median(ang) <= 2.5e-4 and np.percentile(ang, 99) <= 3e-3
+
+
+# ------------------------------------------------------------------ LSS / triangles
+def test_lss_capsule_analytic():
+    out = np.zeros(6, np.float64)
+    L = O.lib()
+    cap = np.array([0, 0, 0, 0.5, 4, 0, 0, 0.5], np.float32)
+    # perpendicular ray into the middle of the body: entry at distance 10 - 0.5
+    r = ray((2, 0, 10), (0, 0, -1))
+    L.orc_lss(O.ptr(r), O.ptr(cap), 1, 1, out.ctypes.data_as(O.ctypes.c_void_p))
+    assert out[0] == 1 and abs(out[1] - 9.5) < 1e-9 and abs(out[2] - 0.5) < 1e-9
+    assert np.allclose(out[3:6], [0, 0, 1], atol=1e-9)
+    # along the axis into the start cap sphere: entry at 10 - 0.5
+    r = ray((-10, 0, 0), (1, 0, 0))
+    L.orc_lss(O.ptr(r), O.ptr(cap), 1, 1, out.ctypes.data_as(O.ctypes.c_void_p))
+    assert out[0] == 1 and abs(out[1] - 9.5) < 1e-9 and out[2] == 0.0
+    # chained: start cap belongs to the previous piece -> entering through it is no hit
+    L.orc_lss(O.ptr(r), O.ptr(cap), 0, 1, out.ctypes.data_as(O.ctypes.c_void_p))
+    assert out[0] == 0
+    # tapered cone: ra = 1, rb = 0.5 over length 4; perpendicular ray at x = 2 grazes the
+    # lateral surface at height r(u) / cos(theta)
+    cap = np.array([0, 0, 0, 1.0, 4, 0, 0, 0.5], np.float32)
+    r = ray((2, 0, 10), (0, 0, -1))
+    L.orc_lss(O.ptr(r), O.ptr(cap), 1, 1, out.ctypes.data_as(O.ctypes.c_void_p))
+    sin_t = 0.5 / 4.0
+    cos_t = math.sqrt(1 - sin_t ** 2)
+    # lateral line: n.(y, x) = ra with n = (sin_t, cos_t): x = (ra - sin_t * y) / cos_t
+    z = (1.0 - sin_t * 2.0) / cos_t
+    assert out[0] == 1 and abs(out[1] - (10 - z)) < 1e-9
+    # miss
+    r = ray((2, 5, 10), (0, 0, -1))
+    L.orc_lss(O.ptr(r), O.ptr(cap), 1, 1, out.ctypes.data_as(O.ctypes.c_void_p))
+    assert out[0] == 0
+
+
+def test_triangle_watertight():
+    L = O.lib()
+    out = np.zeros(4, np.float32)
+    v = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32)
+    r = ray((0.2, 0.3, 5), (0, 0, -1))
+    L.orc_triangle(O.ptr(r), O.ptr(v), O.ptr(out))
+    assert out[0] == 1 and abs(out[1] - 5) < 1e-6 and abs(out[2] - 0.2) < 1e-6 and abs(out[3] - 0.3) < 1e-6
+    r = ray((0.2, 0.3, -5), (0, 0, 1))  # back side: no facing cull
+    L.orc_triangle(O.ptr(r), O.ptr(v), O.ptr(out))
+    assert out[0] == 1
+    r = ray((0.8, 0.8, 5), (0, 0, -1))
+    L.orc_triangle(O.ptr(r), O.ptr(v), O.ptr(out))
+    assert out[0] == 0
+    # watertight: a ray through the shared edge of two triangles hits exactly one or both,
+    # never neither
+    v2 = np.array([1, 0, 0, 1, 1, 0, 0, 1, 0], np.float32)
+    for k in range(200):
+        x = RNG.uniform(0.01, 0.99)
+        r = ray((x, 1 - x, 5), (0, 0, -1))
+        a = np.zeros(4, np.float32)
+        b = np.zeros(4, np.float32)
+        L.orc_triangle(O.ptr(r), O.ptr(v), O.ptr(a))
+        L.orc_triangle(O.ptr(r), O.ptr(v2), O.ptr(b))
+        assert a[0] == 1 or b[0] == 1
+
+
+def test_perp_stark_and_dots_pieces():
+    L = O.lib()
+    for _ in range(200):
+        u = RNG.normal(size=3)
+        u /= np.linalg.norm(u)
+        u = u.astype(np.float32)
+        v = np.zeros(3, np.float32)
+        L.orc_perp_stark(O.ptr(u), O.ptr(v))
+        assert abs(np.linalg.norm(v) - 1) < 1e-6 and abs(u @ v) < 1e-6
+    lines = np.array([[0, 0, 0, 0.04, 1.5, 0.3, 0.2, 0.03]], np.float32)
+    pieces = O.generate_dots(lines)
+    tris = O.dots_triangles(pieces)
+    s, t = pieces[0, 8:11], pieces[0, 12:15]
+    assert abs(s @ t) < 1e-6  # the two quads are orthogonal
+    axis = lines[0, 4:7] - lines[0, :3]
+    for q in range(2):  # each quad contains the segment axis: the axis ends are on its plane
+        a, b, c = tris[q * 2]
+        n = np.cross(b - a, c - a)
+        assert abs(n @ (lines[0, :3] - a)) < 1e-5 and abs(n @ (lines[0, 4:7] - a)) < 1e-5
+        assert abs(n @ axis) < 1e-5
+
+
+# ------------------------------------------------------------------ geometry processing
+def test_generate_curves_c1_continuity_and_straight_lines():
+    verts = O.generate_strands(5, 20, 16)
+    lines = O.strands_to_lines(verts)
+    cv = O.generate_curves(lines).reshape(-1, 4, 4)
+    for s in range(20):
+        for k in range(15):
+            a, b = cv[s * 16 + k], cv[s * 16 + k + 1]
+            assert np.array_equal(a[3], b[0])  # C0 at the joint (exact)
+            d1 = a[3, :3] - a[2, :3]
+            d2 = b[1, :3] - b[0, :3]
+            assert np.allclose(d1, d2, atol=1e-5)  # C1: equal tangents (tension 1)
+    # strand ends duplicate the end line: cp1 = p1 + (p2 - p1)/6 for the first segment
+    first = cv[0]
+    assert np.allclose(first[1, :3], first[0, :3] + (first[3, :3] - first[0, :3]) / 6.0, atol=1e-5)
+    # straight polyline -> evenly spaced collinear control points
+    pts = np.array([[0, 0, 0], [1, 2, 3], [2, 4, 6], [3, 6, 9]], np.float32)
+    ln = np.zeros((3, 8), np.float32)
+    for k in range(3):
+        ln[k, :3], ln[k, 4:7] = pts[k], pts[k + 1]
+    c = O.generate_curves(ln).reshape(3, 4, 4)[1]
+    assert np.allclose(c[1, :3], pts[1] + (pts[2] - pts[1]) / 3, atol=1e-5)
+    assert np.allclose(c[2, :3], pts[1] + 2 * (pts[2] - pts[1]) / 3, atol=1e-5)
+
+
+def test_generator_properties():
+    v = O.generate_strands(777, 2000, 32)
+    roots = v[:, 0, :3].astype(np.float64)
+    assert np.allclose(np.linalg.norm(roots, axis=1), 100.0, atol=1e-3)
+    assert (roots[:, 1] >= 0).all()
+    steps = np.linalg.norm(np.diff(v[:, :, :3].astype(np.float64), axis=1), axis=2)
+    assert np.allclose(steps, 1.5, atol=1e-4)
+    assert np.allclose(v[:, 0, 3], 0.04) and np.allclose(v[:, -1, 3], 0.01)
+    assert len(np.unique(v[:, 0, :3].view(np.dtype((np.void, 12))))) == 2000  # no repeated strands
+    assert np.array_equal(v, O.generate_strands(777, 2000, 32))  # deterministic
+    # hemisphere coverage: roughly uniform in y (uniform on the sphere => y uniform)
+    hist, _ = np.histogram(roots[:, 1] / 100.0, bins=4, range=(0, 1))
+    assert hist.min() > 0.8 * hist.mean()
+
+
+def test_shade_and_palette():
+    L = O.lib()
+    rgb = np.zeros(3, np.float32)
+    L.orc_shade(O.ptr(f32(0, 1, 0)), 1.0, O.ptr(rgb))
+    assert np.allclose(rgb, [0.7, 0.5, 0.4])  # |n.l| = 1
+    L.orc_shade(O.ptr(f32(1, 0, 0)), 1.0, O.ptr(rgb))
+    assert np.allclose(rgb, [0.3, 0.3, 0.3])  # |n.l| = 0: ambient only
+    L.orc_shade(O.ptr(f32(0, -1, 0)), 0.0, O.ptr(rgb))
+    assert np.allclose(rgb, [0.3, 0.3, 0.3])  # shadowed
+    pal = [[1.0, 0.0, 0.3], [0.8, 0.2, 0.3], [0.6, 0.4, 0.3], [0.4, 0.6, 0.3], [0.2, 0.8, 0.3], [0.0, 1.0, 0.3]]
+    for p in range(13):
+        L.orc_debug_color(p, O.ptr(rgb))
+        assert np.allclose(rgb, pal[p % 6])
+
+
+def test_camera_center_ray_hits_target():
+    W, H = 64, 36
+    vi, pi = O.camera((0, 60, 300), (0, 40, 0), aspect=W / H)
+    rays = O.raygen(vi, pi, W, H)
+    # rays through the 4 central pixels straddle the view direction
+    d = rays.reshape(H, W, 8)[H // 2 - 1:H // 2 + 1, W // 2 - 1:W // 2 + 1, 4:7].reshape(-1, 3).mean(0)
+    f = np.array([0, -20, -300.0])
+    f /= np.linalg.norm(f)
+    assert np.dot(d / np.linalg.norm(d), f) > 0.9999
+    assert np.allclose(rays[:, :3], [0, 60, 300], atol=1e-3)
+    assert np.allclose(np.linalg.norm(rays[:, 4:7], axis=1), 1.0, atol=1e-6)
+    # vertical field of view 60 degrees: top row vs bottom row angle ~ 60 * (H-1)/H
+    top = rays.reshape(H, W, 8)[0, W // 2, 4:7]
+    bot = rays.reshape(H, W, 8)[-1, W // 2, 4:7]
+    ang = math.degrees(math.acos(np.clip(top @ bot, -1, 1)))
+    assert abs(ang - 60.0 * (H - 1) / H) < 0.5
