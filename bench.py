@@ -53,7 +53,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run a plain float32 copy kernel over MIB MiB three times and exit "
+                         "(the known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration << 20)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -169,6 +175,20 @@ def main():
         out["parity_sample"] = parity
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def copy_calibration(nbytes):
+    """A unit-scale elementwise copy (reads nbytes, writes nbytes per launch), 4x larger than the
+    256 MB last-level cache so both sides go to HBM."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}), flush=True)
 
 
 def roofline_block(ctx, tlas, cam, W, H, polish, kavg, sp, torch, np, V):

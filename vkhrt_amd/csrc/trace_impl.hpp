@@ -995,28 +995,35 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
         if (rp.tp.defer) {  // deferred FP64 polish pipeline
             const TraceParams& tp = rp.tp;
             const dim3 pgrid(1024);
+            // the events bracket the traversal kernels alone (the per-kernel timings)
             if (ev) hipEventRecord(ev[0], st);
             hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>),
                                block, 0, st, rp);
+            if (ev) hipEventRecord(ev[1], st);
             hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam}, tp.cand_hits,
                                tp.cand_keys);
             hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
             hipLaunchKernelGGL(k_render_classify, grid, block, 0, st, rp);
-            if (ev) hipEventRecord(ev[1], st);
-            if (ev) hipEventRecord(ev[2], st);
+            bool shadow_timed = false;
             if (rp.shadows && !rp.debug_palette) {
                 if (rp.light_vis) {
                     hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.light_vis);
                 } else {
                     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
+                    if (ev) hipEventRecord(ev[2], st);
                     hipLaunchKernelGGL((k_render_shadow_defer<G, STACK>),
                                        persistent_grid(k_render_shadow_defer<G, STACK>), block, 0, st, rp);
+                    if (ev) hipEventRecord(ev[3], st);
+                    shadow_timed = true;
                     hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp, SrcShadow{rp.cam, rp.queue, rp.hitbuf},
                                        rp.vis);
                     hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.vis);
                 }
             }
-            if (ev) hipEventRecord(ev[3], st);
+            if (ev && !shadow_timed) {
+                hipEventRecord(ev[2], st);
+                hipEventRecord(ev[3], st);
+            }
             return hipGetLastError();
         }
     }
