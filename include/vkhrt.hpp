@@ -1,0 +1,197 @@
+/*
+ * C++ host interface over the vkhrt C-ABI, mirroring the reference's classes:
+ *   BLASType, BLASInput, BottomLevelAccelerationStructure
+ *       (include/bottom_level_acceleration_structure.hpp:12-50)
+ *   TopLevelAccelerationStructure (include/top_level_acceleration_structure.hpp:9-23)
+ *   VulkanContext -> vkhrt::Context (one device, one stream per call)
+ * The Vulkan geometry / build-range structures of BLASInput become a primitive pointer and a
+ * count; the type/geometry pairing follows Renderer::InitializeBLAS
+ * (source/renderer.cpp:543-692): eHair = phantom curves (AABBs + hair_intersection.rint),
+ * eMesh = LSS capsules or DOTS triangles, eVoxels is not on this path.
+ * Errors: the reference aborts through VkCheckResult (source/vk_common.cpp:5-15); here a failed
+ * call throws vkhrt::Error carrying the vkhrt_status code and vkhrt_last_error().
+ * Header-only; link with libvkhrt.so. No HIP headers are needed: streams are void*.
+ */
+#ifndef VKHRT_HPP
+#define VKHRT_HPP
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "vkhrt.h"
+
+namespace vkhrt {
+
+inline const char* StatusName(int code) {
+    switch (code) {
+        case VKHRT_SUCCESS: return "SUCCESS";
+        case VKHRT_ERROR_INVALID_ARGUMENT: return "INVALID_ARGUMENT";
+        case VKHRT_ERROR_OUT_OF_MEMORY: return "OUT_OF_MEMORY";
+        case VKHRT_ERROR_DEVICE: return "DEVICE";
+        case VKHRT_ERROR_UNSUPPORTED: return "UNSUPPORTED";
+        case VKHRT_ERROR_STACK_OVERFLOW: return "STACK_OVERFLOW";
+        case VKHRT_ERROR_NO_DEVICE: return "NO_DEVICE";
+        default: return "UNKNOWN";
+    }
+}
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& what)
+        : std::runtime_error(what + " [" + StatusName(code) + "]"), _code(code) {}
+    int Code() const noexcept { return _code; }
+
+private:
+    int _code;
+};
+
+class Context {
+public:
+    explicit Context(int device = 0) {
+        int rc = vkhrt_context_create(device, &_handle);
+        if (rc != VKHRT_SUCCESS) throw Error(rc, "vkhrt_context_create");
+    }
+    ~Context() { vkhrt_context_destroy(_handle); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+
+    vkhrt_context* Handle() const noexcept { return _handle; }
+    void Check(int rc, const char* what) const {
+        if (rc != VKHRT_SUCCESS) throw Error(rc, std::string(what) + ": " + vkhrt_last_error(_handle));
+    }
+    void SetOption(int option, int value) { Check(vkhrt_set_option(_handle, option, value), "vkhrt_set_option"); }
+    void Synchronize(void* stream = nullptr) { Check(vkhrt_synchronize(_handle, stream), "vkhrt_synchronize"); }
+
+private:
+    vkhrt_context* _handle = nullptr;
+};
+
+// include/bottom_level_acceleration_structure.hpp:12-17
+enum class BLASType : uint8_t {
+    eMesh,
+    eHair,
+    eVoxels,
+};
+
+// include/bottom_level_acceleration_structure.hpp:19-29. `transform` is a column-major 4x4
+// (glm::mat4); `primitives` holds `count` records in the layout of vkhrt_blas_desc.
+struct BLASInput {
+    BLASType type = BLASType::eHair;
+    std::array<float, 16> transform{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    vkhrt_geometry_mode geometry = VKHRT_GEOMETRY_PHANTOM_CURVES;
+    const float* primitives = nullptr;
+    uint32_t count = 0;
+    float radius = 0.02f;  // hair_intersection.rint:18
+    bool perPointRadius = false;
+    bool hostPointers = false;
+    uint32_t materialIndex = 0;
+    uint32_t leafSize = 0;
+};
+
+class BottomLevelAccelerationStructure {
+public:
+    BottomLevelAccelerationStructure(const BLASInput& input, const std::shared_ptr<Context>& context,
+                                     void* stream = nullptr)
+        : _type(input.type), _transform(input.transform), _context(context) {
+        const bool hair = input.geometry == VKHRT_GEOMETRY_PHANTOM_CURVES;
+        if (input.type == BLASType::eVoxels)
+            throw Error(VKHRT_ERROR_UNSUPPORTED, "BottomLevelAccelerationStructure: voxel BLAS is not on this path");
+        if ((input.type == BLASType::eHair) != hair)
+            throw Error(VKHRT_ERROR_INVALID_ARGUMENT,
+                        "BottomLevelAccelerationStructure: eHair pairs with phantom curves, eMesh with LSS / DOTS");
+        if (!context) throw Error(VKHRT_ERROR_INVALID_ARGUMENT, "BottomLevelAccelerationStructure: no context");
+        vkhrt_blas_desc d{};
+        d.mode = uint32_t(input.geometry);
+        d.flags = VKHRT_BLAS_HAS_TRANSFORM | (input.perPointRadius ? VKHRT_BLAS_PER_POINT_RADIUS : 0u) |
+                  (input.hostPointers ? VKHRT_BLAS_HOST_POINTERS : 0u);
+        d.primitives = input.primitives;
+        d.count = input.count;
+        d.radius = input.radius;
+        d.material_index = input.materialIndex;
+        d.leaf_size = input.leafSize;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) d.transform[r * 4 + c] = input.transform[c * 4 + r];
+        _context->Check(vkhrt_blas_build(_context->Handle(), &d, stream, &_blas), "vkhrt_blas_build");
+    }
+    ~BottomLevelAccelerationStructure() { vkhrt_blas_destroy(_blas); }
+    BottomLevelAccelerationStructure(BottomLevelAccelerationStructure&& other) noexcept
+        : _blas(std::exchange(other._blas, nullptr)),
+          _type(other._type),
+          _transform(other._transform),
+          _context(std::move(other._context)) {}
+    BottomLevelAccelerationStructure& operator=(BottomLevelAccelerationStructure&&) = delete;
+    BottomLevelAccelerationStructure(const BottomLevelAccelerationStructure&) = delete;
+    BottomLevelAccelerationStructure& operator=(const BottomLevelAccelerationStructure&) = delete;
+
+    const vkhrt_blas* Structure() const noexcept { return _blas; }
+    BLASType Type() const noexcept { return _type; }
+    const std::array<float, 16>& Transform() const noexcept { return _transform; }
+    vkhrt_blas_info Info() const {
+        vkhrt_blas_info info{};
+        _context->Check(vkhrt_blas_info_get(_blas, &info), "vkhrt_blas_info_get");
+        return info;
+    }
+
+private:
+    vkhrt_blas* _blas = nullptr;
+    BLASType _type;
+    std::array<float, 16> _transform;
+    std::shared_ptr<Context> _context;
+};
+
+class TopLevelAccelerationStructure {
+public:
+    // Instance i = blases[i] (customIndex = i, mask 0xFF), source/top_level_acceleration_structure.cpp:23-43.
+    TopLevelAccelerationStructure(const std::vector<BottomLevelAccelerationStructure>& blases,
+                                  const std::shared_ptr<Context>& context, void* stream = nullptr)
+        : _context(context) {
+        if (!context) throw Error(VKHRT_ERROR_INVALID_ARGUMENT, "TopLevelAccelerationStructure: no context");
+        std::vector<const vkhrt_blas*> handles;
+        handles.reserve(blases.size());
+        for (const auto& b : blases) handles.push_back(b.Structure());
+        _context->Check(vkhrt_tlas_build(_context->Handle(), handles.data(), uint32_t(handles.size()), stream, &_tlas),
+                        "vkhrt_tlas_build");
+    }
+    ~TopLevelAccelerationStructure() { vkhrt_tlas_destroy(_tlas); }
+    TopLevelAccelerationStructure(const TopLevelAccelerationStructure&) = delete;
+    TopLevelAccelerationStructure& operator=(const TopLevelAccelerationStructure&) = delete;
+    TopLevelAccelerationStructure(TopLevelAccelerationStructure&&) = delete;
+    TopLevelAccelerationStructure& operator=(TopLevelAccelerationStructure&&) = delete;
+
+    const vkhrt_tlas* Structure() const noexcept { return _tlas; }
+
+    // traceRayEXT over `count` device ray records (shaders/ray_gen.rgen:26-45).
+    void TraceRays(const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits, uint32_t flags = 0,
+                   void* stream = nullptr) const {
+        _context->Check(vkhrt_trace_rays(_context->Handle(), _tlas, rays, count, hits, flags, stream),
+                        "vkhrt_trace_rays");
+    }
+    // traceRaysKHR(W, H, 1) with the closest-hit shading (source/renderer.cpp:156-166).
+    void Render(const vkhrt_camera& camera, const vkhrt_render_desc& desc, void* stream = nullptr) const {
+        _context->Check(vkhrt_render(_context->Handle(), _tlas, &camera, &desc, stream), "vkhrt_render");
+    }
+
+private:
+    vkhrt_tlas* _tlas = nullptr;
+    std::shared_ptr<Context> _context;
+};
+
+// FlyCamera view/projection (source/fly_camera.cpp:25-35), inverted as in
+// Renderer::UpdateCameraResource (source/renderer.cpp:189-195).
+inline vkhrt_camera CameraLookAt(const std::array<float, 3>& eye, const std::array<float, 3>& center,
+                                 float tanHalfFovy, float aspect, const std::array<float, 3>& up = {0, 1, 0},
+                                 float znear = 0.1f, float zfar = 1000.0f) {
+    vkhrt_camera cam{};
+    int rc = vkhrt_camera_look_at(eye.data(), center.data(), up.data(), tanHalfFovy, aspect, znear, zfar, &cam);
+    if (rc != VKHRT_SUCCESS) throw Error(rc, "vkhrt_camera_look_at");
+    return cam;
+}
+
+}  // namespace vkhrt
+
+#endif /* VKHRT_HPP */

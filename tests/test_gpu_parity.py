@@ -231,3 +231,32 @@ def test_persistent_and_plain_kernels_agree(gpu_ctx, G, oracle, curves, persiste
         gpu_ctx.set_option(V.OPTION_PERSISTENT, 1)
         gpu_ctx.set_option(V.OPTION_REFILL_MIN, 1)
     _compare_hits(h, ref, f"persistent={persistent}")
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_trace(tmp_path):
+    """The C++ mirror (include/vkhrt.hpp) builds 3 translated instances from host pointers and
+    traces through vkhrt_trace_rays; hits equal the oracle's bit for bit."""
+    import subprocess
+    import oracle_lib as O
+    import scenes
+    from test_host import cpp_test_binary
+    curves = scenes.c0_curves(200, 16, seed=31)
+    rays = scenes.targeted_rays(curves, 3000, seed=32)
+    rays[1000:2000, 0] += 10.0  # aim a third of the rays at instance 1
+    cpath, rpath, hpath = tmp_path / "c.bin", tmp_path / "r.bin", tmp_path / "h.bin"
+    curves.astype(np.float32).tofile(cpath)
+    rays.astype(np.float32).tofile(rpath)
+    print("[cpp_mirror_trace] 3 instances x", curves.shape[0], "curves,", rays.shape[0], "rays", flush=True)
+    r = subprocess.run([cpp_test_binary(), "trace", str(cpath), str(curves.shape[0]), str(rpath),
+                        str(rays.shape[0]), str(hpath), "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(hpath, O.HIT_DTYPE)
+    S = O.Scene()
+    for i in range(3):
+        xf = np.array([[1.0, 0, 0, 10.0 * i], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+        S.add(O.MODE_PHANTOM, curves, per_point=True, transform=xf)
+    ref = S.trace(rays)
+    assert (ref["prim"] != 0xFFFFFFFF).mean() > 0.2
+    assert np.array_equal(got["prim"], ref["prim"]) and np.array_equal(got["inst"], ref["inst"])
+    assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))

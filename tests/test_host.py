@@ -176,3 +176,26 @@ def test_candidate_count_matches_slab_brute_force():
     # boxes that touch the interval only at its end point can round either way
     assert np.abs(cnt.astype(np.int64) - ref).max() <= 1
     assert (cnt == ref).mean() > 0.98
+
+
+# ------------------------------------------------------------------ C++ mirror of the reference classes
+CPP_TEST = os.path.join(ROOT, "tests", "cpp", "host_api_test")
+
+
+def cpp_test_binary():
+    import subprocess
+    if not os.path.exists(CPP_TEST):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")], check=True)
+    return CPP_TEST
+
+
+def test_cpp_mirror_without_device():
+    """include/vkhrt.hpp: Context -> NO_DEVICE, BLASType/geometry pairing errors, camera."""
+    import subprocess
+    r = subprocess.run([cpp_test_binary(), "nodevice"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[-1] == "ok"
+    cam = np.array([float.fromhex(x) for x in lines[0].split()], np.float32)
+    vi, pi = O.camera((0, 60, 300), (0, 40, 0), tan_half_fovy=np.float32(0.57735026), aspect=16.0 / 9.0)
+    assert np.array_equal(cam[:16], vi) and np.array_equal(cam[16:], pi)
