@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--stack-kind", type=int, default=1)
     ap.add_argument("--leaf-size", type=int, default=1)
     ap.add_argument("--persistent", type=int, default=1)
-    ap.add_argument("--refill-min", type=int, default=1)
+    ap.add_argument("--refill-min", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")

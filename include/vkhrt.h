@@ -157,13 +157,13 @@ int vkhrt_synchronize(vkhrt_context* ctx, void* stream);
 int vkhrt_abi_version(void);
 
 /* Options (autotuned parameters). */
-#define VKHRT_OPTION_STACK_KIND 1 /* minimum LDS stack entries per lane: 0 -> 8, 1 -> 16, 2 -> 24,
+#define VKHRT_OPTION_STACK_KIND 1 /* minimum LDS stack entries per lane: 0 -> 8, 1 -> 12, 2 -> 16,
                                      3 -> 32; raised automatically to the deepest BVH */
 #define VKHRT_OPTION_LEAF_SIZE 2  /* default leaf size for builds (1..3) */
 #define VKHRT_OPTION_PERSISTENT 3 /* 1 (default): persistent wavefronts with per-lane ray refill;
                                      0: one thread per ray */
 #define VKHRT_OPTION_REFILL_MIN 4 /* persistent: idle lanes (1..64) a wavefront waits for before
-                                     fetching new rays together */
+                                     fetching new rays together (default 32) */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
 
 /* ---- acceleration structures */

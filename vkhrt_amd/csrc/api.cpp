@@ -31,7 +31,7 @@ struct vkhrt_context {
     int stack_kind = 1;
     int leaf_size = 1;
     bool persistent = true;
-    uint32_t refill_min = 1;
+    uint32_t refill_min = 32;
     bool profiling = false;
     std::vector<PendingTiming> pending;
     std::map<std::string, std::pair<double, uint32_t>> stats;
@@ -75,12 +75,12 @@ struct vkhrt_tlas {
     uint32_t count = 0;
     int variant = vk::kVarMixed;  // kernel variant without FP64 polish
     uint32_t max_depth = 0;       // deepest BLAS (8-wide levels)
-    bool polish_ok = true;        // merge keys can encode every (instance, primitive)
+    bool polish_ok = true;        // merge keys and compacted-queue entries can encode every (instance, slot)
 };
 
 namespace {
 // LDS stack entries per lane for a stack kind.
-constexpr uint32_t kStackEntries[4] = {8, 16, 24, 32};
+constexpr uint32_t kStackEntries[4] = {8, 12, 16, 32};
 
 // Stack kind for a TLAS: the context's setting, raised until it holds the deepest tree.
 int stack_kind_for(const vkhrt_context* c, const vkhrt_tlas* t) {
@@ -88,6 +88,8 @@ int stack_kind_for(const vkhrt_context* c, const vkhrt_tlas* t) {
     while (k < 3 && kStackEntries[k] < t->max_depth) ++k;
     return k;
 }
+
+bool is_phantom(const vkhrt_tlas* t) { return t->variant == vk::kVarPhConst || t->variant == vk::kVarPhPoint; }
 
 // Kernel variant for a TLAS and the polish flag.
 int trace_variant(const vkhrt_tlas* t, bool polish) {
@@ -568,7 +570,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
                                                       : vk::kVarPhConst;
             v = (v < 0 || v == vi) ? vi : vk::kVarMixed;
             t->max_depth = std::max(t->max_depth, b->bvh.depth);
-            if (b->records >= (1u << 27)) t->polish_ok = false;
+            if (b->records >= (1u << 26)) t->polish_ok = false;
         }
         if (count > 31) t->polish_ok = false;
         if (t->max_depth > kStackEntries[3]) {
@@ -605,11 +607,13 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     if (flags & ~(VKHRT_TRACE_TERMINATE_ON_FIRST_HIT | VKHRT_TRACE_FP64_POLISH))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown trace flags");
     hipStream_t st = pick(c, stream);
-    vk::TraceParams tp{t->dev, t->count, flags, c->status, c->persistent ? c->counters + 3 : nullptr, c->refill_min};
+    // persistent phantom kernels compact intersection work through keys that need polish_ok
+    const bool persistent = c->persistent && (t->polish_ok || !is_phantom(t));
+    vk::TraceParams tp{t->dev, t->count, flags, c->status, persistent ? c->counters + 3 : nullptr, c->refill_min};
     const bool polish = (flags & VKHRT_TRACE_FP64_POLISH) != 0;
     if (polish) {
         if (!t->polish_ok)
-            return fail(c, VKHRT_ERROR_UNSUPPORTED, "FP64 polish needs < 2^27 primitives per BLAS and <= 31 instances");
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "FP64 polish needs < 2^26 primitives per BLAS and <= 31 instances");
         int rc = ensure_polish_scratch(c, count);
         if (rc != VKHRT_SUCCESS) return rc;
         set_polish(c, tp);
@@ -639,7 +643,8 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     hipStream_t st = pick(c, stream);
     vk::RenderParams rp{};
     rp.tp = vk::TraceParams{t->dev, t->count, (d->flags & VKHRT_RENDER_FP64_POLISH) ? uint32_t(vk::kTraceFp64Polish) : 0u,
-                            c->status, c->persistent ? c->counters + 1 : nullptr, c->refill_min};
+                            c->status, c->persistent && (t->polish_ok || !is_phantom(t)) ? c->counters + 1 : nullptr,
+                            c->refill_min};
     std::memcpy(rp.cam.view_inv, cam->view_inverse, 64);
     std::memcpy(rp.cam.proj_inv, cam->proj_inverse, 64);
     rp.cam.width = d->width;
@@ -658,7 +663,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     if (d->flags & VKHRT_RENDER_FP64_POLISH) {
         if (!t->polish_ok || t->variant == vk::kVarMixed)
             return fail(c, VKHRT_ERROR_UNSUPPORTED,
-                        "render FP64 polish needs a single-mode TLAS, < 2^27 primitives per BLAS, <= 31 instances");
+                        "render FP64 polish needs a single-mode TLAS, < 2^26 primitives per BLAS, <= 31 instances");
         rc = ensure_polish_scratch(c, npix);
         if (rc != VKHRT_SUCCESS) return rc;
         set_polish(c, rp.tp);

@@ -225,15 +225,15 @@ __device__ __forceinline__ bool prhi_polish_f64(f3 of, f3 df, const float4* cv, 
 // RMODE (constant / per-point radius) and POLISH are compile-time so each traversal kernel
 // carries only the arithmetic it uses and no device-function calls.
 // `rmax` is the curve's early-out cylinder radius (:21-22), precomputed per curve at build
-// time with the same arithmetic.
-template <int RMODE, int PMODE>
+// time with the same arithmetic. CULL = false skips that test for callers that ran it already.
+template <int RMODE, int PMODE, bool CULL = true>
 __device__ __forceinline__ bool prhi(const RayFrame& F, const float4* cv, float rmax, float const_r, PhantomHit& out) {
     out.deferred = false;
     constexpr int radius_mode = RMODE;
     const f3 p0 = f3{cv[0].x, cv[0].y, cv[0].z}, p1 = f3{cv[1].x, cv[1].y, cv[1].z};
     const f3 p2 = f3{cv[2].x, cv[2].y, cv[2].z}, p3 = f3{cv[3].x, cv[3].y, cv[3].z};
     // early out (:24-33)
-    if (!ray_cylinder(F.o, F.d, p0, p3, rmax)) return false;
+    if (CULL && !ray_cylinder(F.o, F.d, p0, p3, rmax)) return false;
     // ray-centric curve (:38-39)
     f3 q0 = rcc_xform(F.M, p0), q1 = rcc_xform(F.M, p1), q2 = rcc_xform(F.M, p2), q3 = rcc_xform(F.M, p3);
     // start end (:42-44): dot(normalize(q3 - q0), (0,0,1)) > 0

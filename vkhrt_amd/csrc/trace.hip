@@ -8,11 +8,11 @@ namespace vk {
                                                           uint2*, hipStream_t);                                   \
     extern template hipError_t launch_trace_variant<G, 16>(const TraceParams&, const float*, uint32_t, HitRec*,   \
                                                            uint2*, hipStream_t);                                  \
-    extern template hipError_t launch_trace_variant<G, 24>(const TraceParams&, const float*, uint32_t, HitRec*,   \
+    extern template hipError_t launch_trace_variant<G, 12>(const TraceParams&, const float*, uint32_t, HitRec*,   \
                                                            uint2*, hipStream_t);                                  \
     extern template hipError_t launch_render_variant<G, 8>(const RenderParams&, hipStream_t, hipEvent_t*);        \
     extern template hipError_t launch_render_variant<G, 16>(const RenderParams&, hipStream_t, hipEvent_t*);       \
-    extern template hipError_t launch_render_variant<G, 24>(const RenderParams&, hipStream_t, hipEvent_t*);       \
+    extern template hipError_t launch_render_variant<G, 12>(const RenderParams&, hipStream_t, hipEvent_t*);       \
     extern template hipError_t launch_trace_variant<G, 32>(const TraceParams&, const float*, uint32_t, HitRec*,   \
                                                            uint2*, hipStream_t);                                  \
     extern template hipError_t launch_render_variant<G, 32>(const RenderParams&, hipStream_t, hipEvent_t*);
@@ -36,15 +36,15 @@ template <class G>
 hipError_t trace_stack(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
                        int stack_kind, hipStream_t st) {
     if (stack_kind == 0) return launch_trace_variant<G, 8>(tp, rays, n, hits, stats, st);
-    if (stack_kind == 1) return launch_trace_variant<G, 16>(tp, rays, n, hits, stats, st);
-    if (stack_kind == 2) return launch_trace_variant<G, 24>(tp, rays, n, hits, stats, st);
+    if (stack_kind == 1) return launch_trace_variant<G, 12>(tp, rays, n, hits, stats, st);
+    if (stack_kind == 2) return launch_trace_variant<G, 16>(tp, rays, n, hits, stats, st);
     return launch_trace_variant<G, 32>(tp, rays, n, hits, stats, st);
 }
 template <class G>
 hipError_t render_stack(const RenderParams& rp, int stack_kind, hipStream_t st, hipEvent_t* ev) {
     if (stack_kind == 0) return launch_render_variant<G, 8>(rp, st, ev);
-    if (stack_kind == 1) return launch_render_variant<G, 16>(rp, st, ev);
-    if (stack_kind == 2) return launch_render_variant<G, 24>(rp, st, ev);
+    if (stack_kind == 1) return launch_render_variant<G, 12>(rp, st, ev);
+    if (stack_kind == 2) return launch_render_variant<G, 16>(rp, st, ev);
     return launch_render_variant<G, 32>(rp, st, ev);
 }
 }  // namespace

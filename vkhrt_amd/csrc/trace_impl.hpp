@@ -89,6 +89,50 @@ enum StepResult { kStepMore = 0, kStepDone = 1, kStepHit = 2, kStepError = 3 };
 // Resumable traversal of one BLAS by one lane: begin() sets up the ray, each step() visits
 // one 8-wide node (and tests its leaf primitives) and selects the next node. Used whole by
 // traverse() and one step per iteration by the persistent kernels.
+// Tests the 8 children of one node against [tmin, tcur]: `hits` gets the internal children
+// hit (bit j ^ octant for slot j, so lowest-bit-first is near-to-far), `pmask` the leaf
+// records hit (relative to the node's prim base).
+__device__ __forceinline__ void node_visit(const WideNode* nodes, uint32_t node, f3 o, f3 inv, uint32_t oct,
+                                           float tmin, float tcur, uint32_t& hits, uint32_t& pmask,
+                                           uint32_t& child_base, uint32_t& imask, uint32_t& pbase) {
+    const bool nx = oct & 1u, ny = oct & 2u, nz = oct & 4u;
+    const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+    const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+    const float sx = __uint_as_float((n0.w & 0xffu) << 23) * inv.x;
+    const float sy = __uint_as_float(((n0.w >> 8) & 0xffu) << 23) * inv.y;
+    const float sz = __uint_as_float(((n0.w >> 16) & 0xffu) << 23) * inv.z;
+    const float ox = (__uint_as_float(n0.x) - o.x) * inv.x;
+    const float oy = (__uint_as_float(n0.y) - o.y) * inv.y;
+    const float oz = (__uint_as_float(n0.z) - o.z) * inv.z;
+    imask = n0.w >> 24;
+    child_base = n1.x;
+    pbase = n1.y;
+    // near/far plane bytes per axis by ray direction sign
+    const uint32_t nxl = nx ? n3.z : n2.x, nxh = nx ? n3.w : n2.y;
+    const uint32_t fxl = nx ? n2.x : n3.z, fxh = nx ? n2.y : n3.w;
+    const uint32_t nyl = ny ? n4.x : n2.z, nyh = ny ? n4.y : n2.w;
+    const uint32_t fyl = ny ? n2.z : n4.x, fyh = ny ? n2.w : n4.y;
+    const uint32_t nzl = nz ? n4.z : n3.x, nzh = nz ? n4.w : n3.y;
+    const uint32_t fzl = nz ? n3.x : n4.z, fzh = nz ? n3.y : n4.w;
+    hits = 0;
+    pmask = 0;
+    // The slab test only has to be conservative (the quantized boxes round outward and tf is
+    // widened by 8 ulp), so it uses fused multiply-adds; branch-free mask updates.
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const uint32_t meta = ((s < 4 ? n1.z : n1.w) >> (8 * (s & 3))) & 0xffu;
+        const float t0x = fmaf(qbyte(nxl, nxh, s), sx, ox), t1x = fmaf(qbyte(fxl, fxh, s), sx, ox);
+        const float t0y = fmaf(qbyte(nyl, nyh, s), sy, oy), t1y = fmaf(qbyte(fyl, fyh, s), sy, oy);
+        const float t0z = fmaf(qbyte(nzl, nzh, s), sz, oz), t1z = fmaf(qbyte(fzl, fzh, s), sz, oz);
+        const float tn = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
+        const float tf = fminf(fminf(t1x, t1y), fminf(t1z, tcur)) * 1.00000095f;
+        const bool hit = meta != 0 && tn <= tf;
+        const bool internal = (meta & 0x80u) != 0u;
+        hits |= (hit && internal) ? (1u << (uint32_t(s) ^ oct)) : 0u;
+        pmask |= (hit && !internal) ? (((1u << (meta >> 5)) - 1u) << (meta & 31u)) : 0u;
+    }
+}
+
 template <int MODE, int RMODE, bool POLISH, bool ANY, int STACK, bool STATS = false>
 struct Walker {
     f3 o, d, inv;
@@ -118,43 +162,9 @@ struct Walker {
         if (have_node) {
             if (node >= B.nnodes) return kStepError;  // malformed structure: report, never read out of bounds
             if (STATS) ++st->nodes;
-            const bool nx = oct & 1u, ny = oct & 2u, nz = oct & 4u;
-            const uint4* np = reinterpret_cast<const uint4*>(B.nodes + node);
-            const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
-            const float sx = __uint_as_float((n0.w & 0xffu) << 23) * inv.x;
-            const float sy = __uint_as_float(((n0.w >> 8) & 0xffu) << 23) * inv.y;
-            const float sz = __uint_as_float(((n0.w >> 16) & 0xffu) << 23) * inv.z;
-            const float ox = (__uint_as_float(n0.x) - o.x) * inv.x;
-            const float oy = (__uint_as_float(n0.y) - o.y) * inv.y;
-            const float oz = (__uint_as_float(n0.z) - o.z) * inv.z;
-            const uint32_t imask = n0.w >> 24;
-            // near/far plane bytes per axis by ray direction sign
-            const uint32_t nxl = nx ? n3.z : n2.x, nxh = nx ? n3.w : n2.y;
-            const uint32_t fxl = nx ? n2.x : n3.z, fxh = nx ? n2.y : n3.w;
-            const uint32_t nyl = ny ? n4.x : n2.z, nyh = ny ? n4.y : n2.w;
-            const uint32_t fyl = ny ? n2.z : n4.x, fyh = ny ? n2.w : n4.y;
-            const uint32_t nzl = nz ? n4.z : n3.x, nzh = nz ? n4.w : n3.y;
-            const uint32_t fzl = nz ? n3.x : n4.z, fzh = nz ? n3.y : n4.w;
-            const float tcur = best.t;
-            uint32_t hits = 0, pmask = 0;
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const uint32_t meta = ((s < 4 ? n1.z : n1.w) >> (8 * (s & 3))) & 0xffu;
-                float t0x = qbyte(nxl, nxh, s) * sx + ox, t1x = qbyte(fxl, fxh, s) * sx + ox;
-                float t0y = qbyte(nyl, nyh, s) * sy + oy, t1y = qbyte(fyl, fyh, s) * sy + oy;
-                float t0z = qbyte(nzl, nzh, s) * sz + oz, t1z = qbyte(fzl, fzh, s) * sz + oz;
-                float tn = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
-                float tf = fminf(fminf(t1x, t1y), fminf(t1z, tcur)) * 1.00000095f;
-                bool hit = meta != 0 && tn <= tf;
-                if (hit) {
-                    if (meta & 0x80u)
-                        hits |= 1u << (uint32_t(s) ^ oct);
-                    else
-                        pmask |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
-                }
-            }
+            uint32_t hits, pmask, child_base, imask, pbase;
+            node_visit(B.nodes, node, o, inv, oct, tmin, best.t, hits, pmask, child_base, imask, pbase);
             // leaf primitives of this node
-            const uint32_t pbase = n1.y;
             if (pmask && pbase + (32u - __clz(pmask)) > B.nprims) return kStepError;
             while (pmask) {
                 const uint32_t k = __ffs(pmask) - 1;
@@ -237,7 +247,7 @@ struct Walker {
             }
             have_node = false;
             if (hits) {
-                const uint2 g = make_uint2(n1.x, imask | (hits << 8));
+                const uint2 g = make_uint2(child_base, imask | (hits << 8));
                 if (cur.y >> 8) {
                     if (sp >= STACK) return kStepError;
                     lds[sp * kTraceBlock] = cur;
@@ -528,8 +538,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow(RenderParams rp) 
 // tmax) produces work item idx; snk(idx, o, d, best) consumes its result (called by the
 // lanes that finish in the same iteration, so it may use wave ballots).
 template <class G, bool ANY, int STACK, class Src, class Snk>
-__device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t total, uint32_t* counter, uint2* lds,
-                                                 Src&& src, Snk&& snk) {
+__device__ __forceinline__ void persistent_generic(const TraceParams& tp, uint32_t total, uint32_t* counter,
+                                                   uint2* lds, Src&& src, Snk&& snk) {
     static_assert(!G::kMixed, "persistent kernels serve single-mode TLASes");
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -610,6 +620,298 @@ __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t
     }
 }
 
+// ------------------------------------------------------------------ compacted phantom lanes
+// Phantom persistent loop with wave-level compaction of the intersection work. Lanes run one
+// traversal step per iteration: a node test, or one leaf record through the capped-cylinder
+// early-out (hair_intersection.rint:24-33). A (lane, leaf slot) pair that survives the
+// early-out is appended to a per-wave queue in LDS. When the queue holds 64 pairs, or no
+// lane can traverse, every lane of the wave takes one pair and runs the rest of Prhi on the
+// owner's ray (read with cross-lane permutes), so the iteration loop runs on full
+// wavefronts instead of on the few lanes that happen to sit at a leaf. Results merge into the
+// owner's LDS slot by a 64-bit atomicMin on (t, instance, primitive): that minimum is unique,
+// so the closest hit and its tie rule do not depend on the processing order. A lane does not
+// move to its next instance, finish or take a new ray while pairs of its ray are queued.
+constexpr uint32_t kCompactQueue = 128;    // per wave: < 64 left after a round + <= 64 appended per step
+constexpr uint32_t kCompactSlotBits = 26;  // queue entry = lane << 26 | leaf slot
+constexpr uint32_t kNoHitLow = 0xFFFFFFFFu;
+constexpr uint32_t kCompactIterationCap = 1u << 24;
+
+template <int STACK>
+struct CWalker {
+    f3 o, d, inv;
+    uint32_t oct;
+    uint2 cur;  // current group: x = child_base, y = imask | remaining hit bits (octant order) << 8
+    uint32_t node, pbase, pmask;
+    int sp;
+    bool have_node;
+
+    __device__ __forceinline__ void begin(f3 o_, f3 d_) {
+        o = o_;
+        d = d_;
+        inv = safe_inv(d);
+        oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
+        cur = make_uint2(0u, 0u);
+        node = 0;
+        pbase = 0;
+        pmask = 0;
+        sp = 0;
+        have_node = true;
+    }
+
+    // One step: one pending leaf record through the early-out, or one node test; then, when
+    // no record of the node is pending, the next node is selected. `cand` receives the leaf
+    // slot that passed the early-out (0xFFFFFFFF if none); it is valid with any return value.
+    __device__ __forceinline__ int step(const BlasDev& B, float tmin, float tcur, uint2* lds, uint32_t& cand) {
+        cand = 0xFFFFFFFFu;
+        if (pmask) {
+            const uint32_t slot = pbase + uint32_t(__ffs(pmask) - 1);
+            pmask &= pmask - 1;
+            const float4* pr = B.prims + size_t(slot) * 4;
+            const float4 a = pr[0], b = pr[3];
+            if (ray_cylinder(o, d, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, B.aux[slot])) cand = slot;
+        } else if (have_node) {
+            if (node >= B.nnodes) return kStepError;  // malformed structure: report, never read out of bounds
+            uint32_t hits, pm, child_base, imask, pb;
+            node_visit(B.nodes, node, o, inv, oct, tmin, tcur, hits, pm, child_base, imask, pb);
+            if (pm && pb + (32u - __clz(pm)) > B.nprims) return kStepError;
+            pbase = pb;
+            pmask = pm;
+            have_node = false;
+            if (hits) {
+                const uint2 g = make_uint2(child_base, imask | (hits << 8));
+                if (cur.y >> 8) {
+                    if (sp >= STACK) return kStepError;
+                    lds[sp * kTraceBlock] = cur;
+                    ++sp;
+                }
+                cur = g;
+            }
+        }
+        if (pmask || have_node) return kStepMore;
+        if ((cur.y >> 8) == 0) {
+            if (sp == 0) return kStepDone;
+            --sp;
+            cur = lds[sp * kTraceBlock];
+        }
+        uint32_t hb = cur.y >> 8;
+        const uint32_t j = __ffs(hb) - 1;
+        hb &= hb - 1;
+        const uint32_t slot = j ^ oct;
+        const uint32_t im = cur.y & 0xffu;
+        node = cur.x + __popc(im & ((1u << slot) - 1u));
+        cur.y = im | (hb << 8);
+        have_node = true;
+        return kStepMore;
+    }
+};
+
+template <class G, bool ANY, int STACK, class Src, class Snk>
+__device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32_t total, uint32_t* counter,
+                                                   uint2* lds, Src&& src, Snk&& snk) {
+    static_assert(G::kMode == kPhantom, "compacted lanes serve the phantom intersector");
+    __shared__ uint32_t s_queue[(kTraceBlock / 64) * kCompactQueue];
+    __shared__ unsigned long long s_key[kTraceBlock];  // best (t, instance, primitive) of the lane's ray
+    __shared__ float4 s_pay[kTraceBlock];              // its u and ray-centric normal vector
+    __shared__ uint32_t s_pol[kTraceBlock];            // its polished flag
+    __shared__ uint32_t s_cnt[kTraceBlock];            // pairs of the lane's ray still queued
+    enum { kIdle = 0, kInst = 1, kTrav = 2, kWait = 3 };
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wbase = threadIdx.x & ~63u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t* queue = s_queue + (threadIdx.x >> 6) * kCompactQueue;
+    CWalker<STACK> W;
+    W.o = W.d = f3{0.0f, 0.0f, 0.0f};
+    f3 ow = f3{0, 0, 0}, dw = f3{0, 0, 0};
+    float tmin = 0.0f, tmax = 0.0f, tcur = 0.0f;
+    uint32_t idx = 0, inst = 0;
+    int state = kIdle;
+    bool ray_done = false;
+    bool exhausted = false;  // wave-uniform
+    uint32_t qn = 0;         // wave-uniform queue length
+    // Every wave leaves the loop: a normal frame takes ~10^4 iterations per wave; past the cap
+    // the wave reports an error (status[0]) and stops, leaving its rays unwritten.
+    for (uint32_t guard = 0;; ++guard) {
+        if (guard >= kCompactIterationCap) {
+            if (lane == 0) atomicAdd(&tp.status[0], 1u);
+            break;
+        }
+        // (a) a waiting lane whose pairs are all merged moves to its next instance or finishes
+        if (state == kWait && s_cnt[threadIdx.x] == 0u) {
+            if (ray_done) {
+                state = kInst;
+                inst = tp.ninst;  // finish below
+            } else {
+                ++inst;
+                state = kInst;
+            }
+        }
+        // (b) refill idle lanes from the global counter
+        if (!exhausted) {
+            const uint64_t idle = __ballot(state == kIdle);
+            if (idle && (__popcll(idle) >= tp.refill_min || idle == __ballot(true))) {
+                const uint32_t n = __popcll(idle);
+                const uint32_t leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(counter, n);
+                base = __shfl(base, int(leader));
+                if (base + n >= total) exhausted = true;
+                if (state == kIdle) {
+                    const uint32_t r = base + __popcll(idle & lt);
+                    if (r < total) {
+                        idx = r;
+                        src(idx, ow, dw, tmin, tmax);
+                        tcur = tmax;
+                        s_key[threadIdx.x] =
+                            (static_cast<unsigned long long>(__float_as_uint(tmax)) << 32) | kNoHitLow;
+                        s_pol[threadIdx.x] = 0u;
+                        s_cnt[threadIdx.x] = 0u;
+                        inst = 0;
+                        ray_done = false;
+                        state = kInst;
+                    }
+                }
+            }
+        }
+        // (c) next instance whose world box the ray overlaps (top_level_acceleration_structure.cpp:23-43)
+        if (state == kInst) {
+            while (inst < tp.ninst) {
+                const InstanceDev& I = tp.inst[inst];
+                if (I.blas.nprims != 0 && ray_box(ow, dw, I.lo, I.hi, tmin, tcur)) break;
+                ++inst;
+            }
+            if (inst >= tp.ninst) {
+                const unsigned long long kk = s_key[threadIdx.x];
+                const uint32_t lo = uint32_t(kk);
+                Best best;
+                best.slot = 0xFFFFFFFFu;
+                if (lo == kNoHitLow) {
+                    best.t = tmax;
+                    best.inst = best.pid = 0xFFFFFFFFu;
+                    best.u = best.a = best.b = best.c = 0.0f;
+                    best.extra = 0u;
+                } else {
+                    const float4 pay = s_pay[threadIdx.x];
+                    best.t = __uint_as_float(uint32_t(kk >> 32));
+                    best.inst = lo >> 27;
+                    best.pid = lo & ((1u << 27) - 1u);
+                    best.u = pay.x;
+                    best.a = pay.y;
+                    best.b = pay.z;
+                    best.c = pay.w;
+                    best.extra = s_pol[threadIdx.x];
+                }
+                snk(idx, ow, dw, best);
+                state = kIdle;
+            } else {
+                const InstanceDev& I = tp.inst[inst];
+                f3 oo = ow, dd = dw;
+                if (!I.identity) {
+                    oo = xform_pt(I.inv, ow);
+                    dd = xform_dir(I.inv, dw);
+                }
+                W.begin(oo, dd);
+                state = kTrav;
+            }
+        }
+        // lanes that finished in (c) take new rays in the next iteration: leave only when the
+        // counter is exhausted and no lane holds a ray (then no pair is queued either)
+        if (!__ballot(state != kIdle)) {
+            if (exhausted) break;
+            continue;
+        }
+        // (d) intersection round on up to 64 queued pairs, one per lane
+        if (qn >= 64u || (qn != 0u && !__ballot(state == kTrav))) {
+            const uint32_t m = qn < 64u ? qn : 64u;
+            const uint32_t e = lane < m ? queue[qn - m + lane] : 0u;
+            const int owner = int(e >> kCompactSlotBits);
+            const uint32_t slot = e & ((1u << kCompactSlotBits) - 1u);
+            const f3 po = f3{__shfl(W.o.x, owner), __shfl(W.o.y, owner), __shfl(W.o.z, owner)};
+            const f3 pd = f3{__shfl(W.d.x, owner), __shfl(W.d.y, owner), __shfl(W.d.z, owner)};
+            const float ptmin = __shfl(tmin, owner), ptmax = __shfl(tmax, owner);
+            const uint32_t pinst = __shfl(inst, owner), pidx = __shfl(idx, owner);
+            const uint32_t os = wbase + uint32_t(owner);
+            unsigned long long k = ~0ull;
+            PhantomHit ph;
+            bool hit = false;
+            if (lane < m) {
+                const bool skip = ANY && uint32_t(s_key[os]) != kNoHitLow;  // already occluded
+                if (!skip) {
+                    const BlasDev& B = tp.inst[pinst].blas;
+                    RayFrame F;
+                    make_frame(po, pd, F);
+                    const float4* pr = B.prims + size_t(slot) * 4;
+                    float4 cv[4] = {pr[0], pr[1], pr[2], pr[3]};
+                    hit = prhi<G::kRMode, G::kPolish ? kPolishDefer : kPolishOff, false>(F, cv, 0.0f, B.radius, ph);
+                    if (G::kPolish && ph.deferred) {
+                        // this curve needs the FP64 continuation: queue (ray, instance, slot)
+                        const uint32_t q = atomicAdd(tp.defer_count, 1u);
+                        if (q < tp.defer_cap)
+                            tp.defer[q] = make_uint4(pidx, pinst, slot, 0u);
+                        else
+                            atomicAdd(&tp.status[1], 1u);
+                    }
+                    hit = hit && ph.t >= ptmin && ph.t <= ptmax;
+                    if (hit) {
+                        k = pack_key(ph.t, pinst, B.prim_ids[slot]);
+                        atomicMin(&s_key[os], k);
+                    }
+                }
+                atomicSub(&s_cnt[os], 1u);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (hit && s_key[os] == k) {  // the unique winner for this owner writes the payload
+                s_pay[os] = make_float4(ph.u, ph.ncx, ph.ncy, ph.nsz);
+                s_pol[os] = ph.polished ? 1u : 0u;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            qn -= m;
+            if (state != kIdle) {
+                const unsigned long long kk = s_key[threadIdx.x];
+                tcur = __uint_as_float(uint32_t(kk >> 32));
+                if (ANY && uint32_t(kk) != kNoHitLow && !ray_done) {
+                    ray_done = true;
+                    state = kWait;
+                }
+            }
+            continue;
+        }
+        // (e) one traversal step per traversing lane; survivors of the early-out are queued
+        uint32_t cand = 0xFFFFFFFFu;
+        if (state == kTrav) {
+            const int r = W.step(tp.inst[inst].blas, tmin, tcur, lds, cand);
+            if (r == kStepError) {
+                atomicAdd(&tp.status[0], 1u);
+                ray_done = true;
+                state = kWait;
+            } else if (r == kStepDone) {
+                state = kWait;
+            }
+        }
+        const bool app = cand != 0xFFFFFFFFu;
+        const uint64_t am = __ballot(app);
+        if (app) {
+            queue[qn + __popcll(am & lt)] = (lane << kCompactSlotBits) | cand;
+            atomicAdd(&s_cnt[threadIdx.x], 1u);
+        }
+        qn += __popcll(am);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Persistent loop for one geometry variant: compacted intersection rounds for phantom curves
+// (the expensive, iterative test), the per-lane loop for LSS and DOTS.
+template <class G, bool ANY, int STACK, class Src, class Snk>
+__device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t total, uint32_t* counter, uint2* lds,
+                                                 Src&& src, Snk&& snk) {
+    if constexpr (G::kMode == kPhantom)
+        persistent_compact<G, ANY, STACK>(tp, total, counter, lds, src, snk);
+    else
+        persistent_generic<G, ANY, STACK>(tp, total, counter, lds, src, snk);
+}
+
 template <class G, bool ANY, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TraceParams tp, const float* __restrict__ rays,
                                                                        uint32_t n, HitRec* __restrict__ hits,
@@ -630,71 +932,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TracePara
             finish_hit(tp, o, d, best, h);
             hits[i] = h;
             if (G::kPolish) tp.keys[i] = best_key(best);
-        });
-}
-
-template <class G, int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_render_primary_persistent(RenderParams rp) {
-    __shared__ uint2 stk[STACK * kTraceBlock];
-    const uint32_t npix = rp.cam.width * rp.cam.height;
-    persistent_lanes<G, false, STACK>(
-        rp.tp, npix, rp.counters + 1, stk + threadIdx.x,
-        [&](uint32_t pix, f3& o, f3& d, float& tmin, float& tmax) {
-            primary_ray(rp.cam, pix % rp.cam.width, pix / rp.cam.width, o, d);
-            tmin = 0.001f;
-            tmax = 10000.0f;
-        },
-        [&](uint32_t pix, f3 o, f3 d, const Best& best) {
-            HitRec h;
-            finish_hit(rp.tp, o, d, best, h);
-            bool hit = h.prim_id != 0xFFFFFFFFu;
-            if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
-            const f3 n = f3{h.n[0], h.n[1], h.n[2]};
-            if (!hit) {
-                store_px(rp, pix, f3{0.0f, 0.0f, 0.0f});
-            } else if (rp.debug_palette) {
-                store_px(rp, pix, debug_color(h.prim_id));
-                hit = false;
-            } else if (!rp.shadows) {
-                store_px(rp, pix, shade(n, 1.0f));
-                hit = false;
-            } else {
-                rp.hitbuf[pix] = make_float4(n.x, n.y, n.z, h.t);
-            }
-            // append to the shadow queue, aggregated over the lanes finishing now
-            const uint64_t m = __ballot(hit);
-            if (m) {
-                const uint32_t lane = threadIdx.x & 63u;
-                const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&rp.counters[0], uint32_t(__popcll(m)));
-                base = __shfl(base, int(leader));
-                if (hit) rp.queue[base + __popcll(m & ((1ull << lane) - 1ull))] = pix;
-            }
-        });
-}
-
-template <class G, int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_render_shadow_persistent(RenderParams rp) {
-    __shared__ uint2 stk[STACK * kTraceBlock];
-    const uint32_t total = rp.counters[0];
-    persistent_lanes<G, true, STACK>(
-        rp.tp, total, rp.counters + 2, stk + threadIdx.x,
-        [&](uint32_t qi, f3& o, f3& d, float& tmin, float& tmax) {
-            const uint32_t pix = rp.queue[qi];
-            f3 po, pd;
-            primary_ray(rp.cam, pix % rp.cam.width, pix / rp.cam.width, po, pd);
-            const float4 hb = rp.hitbuf[pix];
-            o = add(po, scl(hb.w, pd));
-            d = f3{0.0f, 1.0f, 0.0f};
-            tmin = 0.001f;
-            tmax = 10000.0f;
-        },
-        [&](uint32_t qi, f3 o, f3 d, const Best& best) {
-            const uint32_t pix = rp.queue[qi];
-            const float4 hb = rp.hitbuf[pix];
-            const float vis = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
-            store_px(rp, pix, shade(f3{hb.x, hb.y, hb.z}, vis));
         });
 }
 
@@ -816,7 +1053,8 @@ __global__ void k_polish_shadow(TraceParams tp, SrcShadow src, float* __restrict
     }
 }
 
-// Deferred-polish render, pass 1: primary rays (FP32) -> per-pixel hit record and merge key.
+// Persistent render, pass 1: primary rays (FP32) -> per-pixel hit record (and the merge key
+// when grazing roots are deferred to the FP64 polish).
 template <class G, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderParams rp) {
     __shared__ uint2 stk[STACK * kTraceBlock];
@@ -832,11 +1070,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderPara
             HitRec h;
             finish_hit(rp.tp, o, d, best, h);
             rp.hitrec[pix] = h;
-            rp.tp.keys[pix] = best_key(best);
+            if (G::kPolish) rp.tp.keys[pix] = best_key(best);
         });
 }
 
-// Pass 2 (after the polish merge): shade misses / palette / unshadowed pixels, queue the rest.
+// Pass 2 (after the optional polish merge): shade misses / palette / unshadowed pixels, queue
+// the rest for shadow rays.
 __global__ void k_render_classify(RenderParams rp) {
     const uint32_t npix = rp.cam.width * rp.cam.height;
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -992,7 +1231,10 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
     const dim3 grid((npix + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, rp.counters, 8u);
     if constexpr (!G::kMixed) {
-        if (rp.tp.defer) {  // deferred FP64 polish pipeline
+        if (rp.tp.work) {
+            // Persistent pipeline: the traversal kernels only record hits (their sinks stay
+            // small because a lane finishing any ray makes the whole wave run the sink);
+            // shading, the shadow queue and the optional deferred FP64 polish run as passes.
             const TraceParams& tp = rp.tp;
             const dim3 pgrid(1024);
             // the events bracket the traversal kernels alone (the per-kernel timings)
@@ -1000,23 +1242,26 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
             hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>),
                                block, 0, st, rp);
             if (ev) hipEventRecord(ev[1], st);
-            hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam}, tp.cand_hits,
-                               tp.cand_keys);
-            hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
+            if (tp.defer) {
+                hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam},
+                                   tp.cand_hits, tp.cand_keys);
+                hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
+            }
             hipLaunchKernelGGL(k_render_classify, grid, block, 0, st, rp);
             bool shadow_timed = false;
             if (rp.shadows && !rp.debug_palette) {
                 if (rp.light_vis) {
                     hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.light_vis);
                 } else {
-                    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
+                    if (tp.defer) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
                     if (ev) hipEventRecord(ev[2], st);
                     hipLaunchKernelGGL((k_render_shadow_defer<G, STACK>),
                                        persistent_grid(k_render_shadow_defer<G, STACK>), block, 0, st, rp);
                     if (ev) hipEventRecord(ev[3], st);
                     shadow_timed = true;
-                    hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp, SrcShadow{rp.cam, rp.queue, rp.hitbuf},
-                                       rp.vis);
+                    if (tp.defer)
+                        hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp,
+                                           SrcShadow{rp.cam, rp.queue, rp.hitbuf}, rp.vis);
                     hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.vis);
                 }
             }
@@ -1027,28 +1272,16 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
             return hipGetLastError();
         }
     }
-    bool persistent = false;
-    if constexpr (!G::kMixed) persistent = rp.tp.work != nullptr;
+    // one thread per pixel (mixed-mode TLASes, or persistence switched off)
     if (ev) hipEventRecord(ev[0], st);
-    if constexpr (!G::kMixed) {
-        if (persistent)
-            hipLaunchKernelGGL((k_render_primary_persistent<G, STACK>),
-                               persistent_grid(k_render_primary_persistent<G, STACK>), block, 0, st, rp);
-    }
-    if (!persistent) hipLaunchKernelGGL((k_render_primary<G, STACK>), grid, block, 0, st, rp);
+    hipLaunchKernelGGL((k_render_primary<G, STACK>), grid, block, 0, st, rp);
     if (ev) hipEventRecord(ev[1], st);
     if (ev) hipEventRecord(ev[2], st);
     if (rp.shadows && !rp.debug_palette) {
-        if (rp.light_vis) {
+        if (rp.light_vis)
             hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.light_vis);
-        } else {
-            if constexpr (!G::kMixed) {
-                if (persistent)
-                    hipLaunchKernelGGL((k_render_shadow_persistent<G, STACK>),
-                                       persistent_grid(k_render_shadow_persistent<G, STACK>), block, 0, st, rp);
-            }
-            if (!persistent) hipLaunchKernelGGL((k_render_shadow<G, STACK>), grid, block, 0, st, rp);
-        }
+        else
+            hipLaunchKernelGGL((k_render_shadow<G, STACK>), grid, block, 0, st, rp);
     }
     if (ev) hipEventRecord(ev[3], st);
     return hipGetLastError();
@@ -1060,11 +1293,11 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                                    hipStream_t);                                                \
     template hipError_t launch_trace_variant<G, 16>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
                                                     hipStream_t);                                               \
-    template hipError_t launch_trace_variant<G, 24>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
+    template hipError_t launch_trace_variant<G, 12>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
                                                     hipStream_t);                                               \
     template hipError_t launch_render_variant<G, 8>(const RenderParams&, hipStream_t, hipEvent_t*);             \
     template hipError_t launch_render_variant<G, 16>(const RenderParams&, hipStream_t, hipEvent_t*);            \
-    template hipError_t launch_render_variant<G, 24>(const RenderParams&, hipStream_t, hipEvent_t*);            \
+    template hipError_t launch_render_variant<G, 12>(const RenderParams&, hipStream_t, hipEvent_t*);            \
     template hipError_t launch_trace_variant<G, 32>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
                                                     hipStream_t);                                               \
     template hipError_t launch_render_variant<G, 32>(const RenderParams&, hipStream_t, hipEvent_t*);
