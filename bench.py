@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=1)
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--refill-min", type=int, default=32)
+    ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -80,6 +81,7 @@ def main():
     ctx.set_option(V.OPTION_STACK_KIND, args.stack_kind)
     ctx.set_option(V.OPTION_PERSISTENT, args.persistent)
     ctx.set_option(V.OPTION_REFILL_MIN, args.refill_min)
+    ctx.set_option(V.OPTION_RAY_ORDER, args.ray_order)
 
     NS, SEG, W, H = args.strands, args.segments, args.width, args.height
     nseg_total = NS * SEG
@@ -167,6 +169,7 @@ def main():
             "bvh": {"width": 8, "leaf_size": args.leaf_size, "nodes": binfo["node_count"],
                     "depth": binfo["depth"], "build_ms": round(binfo["build_ms"], 3)},
             "stack_kind": args.stack_kind, "persistent": bool(args.persistent), "refill_min": args.refill_min,
+            "ray_order": args.ray_order,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -274,18 +274,23 @@ def test_persistent_many_rays_per_lane(gpu_ctx, G, oracle, curves, any_hit):
         _compare_hits(pers[sample], ref, "persistent many rays")
 
 
-@pytest.mark.parametrize("polish", [False, True])
-def test_persistent_render_1024(gpu_ctx, G, oracle, curves, polish):
-    """A 1024x1024 render (16 pixels per resident lane) with shadows: persistent and
-    one-thread-per-ray kernels give the same image; primitive ids equal the oracle's."""
+@pytest.mark.parametrize("polish,order", [(False, 0), (True, 0), (False, 1), (True, 1)])
+def test_persistent_render_1024(gpu_ctx, G, oracle, curves, polish, order):
+    """A 1000x1000 render (about 5 pixels per resident lane; 8x8 tiles cross the image edge)
+    with shadows, row-major or tiled work order: persistent and one-thread-per-ray kernels
+    give the same image; primitive ids equal the oracle's."""
     import vkhrt_amd as V
     blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
     tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
-    W = H = 1024
+    W = H = 1000
     cam = V.camera_look_at((0.0, 50.0, 400.0), (0.0, 30.0, 0.0), aspect=1.0)
     flags = V.RENDER_SHADOWS | (V.RENDER_FP64_POLISH if polish else 0)
-    print(f"[persistent_render_1024] polish={polish}", flush=True)
-    img, ids = G.render(gpu_ctx, tlas, cam, W, H, flags)
+    print(f"[persistent_render_1024] polish={polish} order={order}", flush=True)
+    gpu_ctx.set_option(V.OPTION_RAY_ORDER, order)
+    try:
+        img, ids = G.render(gpu_ctx, tlas, cam, W, H, flags)
+    finally:
+        gpu_ctx.set_option(V.OPTION_RAY_ORDER, 0)
     if not polish:
         gpu_ctx.set_option(V.OPTION_PERSISTENT, 0)
         try:
@@ -298,7 +303,7 @@ def test_persistent_render_1024(gpu_ctx, G, oracle, curves, polish):
     vi, pi = G.cam_arrays(cam)
     rimg, rids, _, _ = S.render(vi, pi, W, H, oracle.F_POLISH if polish else 0, shadows=1, pixel_stride=7)
     sel = rids != 0xFFFFFFFF
-    assert (ids != 0xFFFFFFFF).sum() > 2000
+    assert (ids != 0xFFFFFFFF).sum() > 1500
     # the oracle traced every 7th pixel; the rest of its id image is the miss value
     stride_mask = (np.arange(W * H).reshape(H, W) % 7) == 0
     assert np.array_equal(ids[stride_mask], rids[stride_mask])

@@ -34,6 +34,7 @@ OPTION_STACK_KIND = 1
 OPTION_LEAF_SIZE = 2
 OPTION_PERSISTENT = 3
 OPTION_REFILL_MIN = 4
+OPTION_RAY_ORDER = 5
 
 STATUS = {0: "SUCCESS", -1: "INVALID_ARGUMENT", -2: "OUT_OF_MEMORY", -3: "DEVICE",
           -4: "UNSUPPORTED", -5: "STACK_OVERFLOW", -6: "NO_DEVICE"}

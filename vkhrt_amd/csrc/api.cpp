@@ -32,6 +32,7 @@ struct vkhrt_context {
     int leaf_size = 1;
     bool persistent = true;
     uint32_t refill_min = 32;
+    uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles
     bool profiling = false;
     std::vector<PendingTiming> pending;
     std::map<std::string, std::pair<double, uint32_t>> stats;
@@ -87,6 +88,16 @@ int stack_kind_for(const vkhrt_context* c, const vkhrt_tlas* t) {
     int k = c->stack_kind;
     while (k < 3 && kStackEntries[k] < t->max_depth) ++k;
     return k;
+}
+
+// Work items of a frame and the constants of the work item -> pixel division.
+void set_work_order(vk::Camera& cm, uint32_t order) {
+    cm.order = order;
+    cm.tiles_x = (cm.width + 7u) / 8u;
+    const uint32_t tiles_y = (cm.height + 7u) / 8u;
+    cm.work = order ? cm.tiles_x * tiles_y * 64u : cm.width * cm.height;
+    cm.inv_w = cm.width ? 1.0f / float(cm.width) : 0.0f;
+    cm.inv_tiles_x = cm.tiles_x ? 1.0f / float(cm.tiles_x) : 0.0f;
 }
 
 bool is_phantom(const vkhrt_tlas* t) { return t->variant == vk::kVarPhConst || t->variant == vk::kVarPhPoint; }
@@ -376,6 +387,11 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
         c->refill_min = uint32_t(value);
         return VKHRT_SUCCESS;
     }
+    if (option == VKHRT_OPTION_RAY_ORDER) {
+        if (value < 0 || value > 1) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "ray order must be 0 or 1");
+        c->ray_order = uint32_t(value);
+        return VKHRT_SUCCESS;
+    }
     if (option == VKHRT_OPTION_LEAF_SIZE) {
         if (value < 1 || value > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "leaf size must be 1..3");
         c->leaf_size = value;
@@ -649,6 +665,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     std::memcpy(rp.cam.proj_inv, cam->proj_inverse, 64);
     rp.cam.width = d->width;
     rp.cam.height = d->height;
+    set_work_order(rp.cam, c->ray_order);
     rp.shadows = (d->flags & VKHRT_RENDER_SHADOWS) ? 1u : 0u;
     rp.out_format = (d->flags & VKHRT_RENDER_RGBA8) ? 1u : 0u;
     rp.debug_palette = (d->flags & VKHRT_RENDER_DEBUG_PALETTE) ? 1u : 0u;
@@ -720,6 +737,7 @@ int vkhrt_generate_camera_rays(vkhrt_context* c, const vkhrt_camera* cam, uint32
     std::memcpy(cm.proj_inv, cam->proj_inverse, 64);
     cm.width = width;
     cm.height = height;
+    set_work_order(cm, 0);
     hipError_t e = vk::launch_camera_rays(cm, tmin, tmax, reinterpret_cast<float*>(rays), pick(c, stream));
     return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "camera rays");
 }

@@ -42,6 +42,10 @@ struct Camera {
     float view_inv[16];
     float proj_inv[16];
     uint32_t width, height;
+    uint32_t order;    // work item -> pixel: 0 row-major, 1 8x8 tiles (one tile per wavefront)
+    uint32_t tiles_x;  // ceil(width / 8)
+    uint32_t work;     // work items of a frame (order 1: whole tiles, some outside the image)
+    float inv_w, inv_tiles_x;  // 1 / width, 1 / tiles_x for the index division
 };
 
 struct RenderParams {

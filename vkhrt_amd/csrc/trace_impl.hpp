@@ -425,6 +425,30 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceParams tp, cons
     hits[i] = h;
 }
 
+// n / d for n < 2^24 from a float reciprocal, corrected to the exact quotient.
+__device__ __forceinline__ uint32_t udiv_small(uint32_t n, uint32_t d, float inv_d) {
+    uint32_t q = uint32_t(float(n) * inv_d);
+    if (q * d > n)
+        --q;
+    else if ((q + 1u) * d <= n)
+        ++q;
+    return q;
+}
+
+// Work item r of a frame -> pixel (x, y); false for tile slots outside the image.
+__device__ __forceinline__ bool work_pixel(const Camera& c, uint32_t r, uint32_t& x, uint32_t& y) {
+    if (c.order == 0) {
+        y = r < (1u << 24) ? udiv_small(r, c.width, c.inv_w) : r / c.width;
+        x = r - y * c.width;
+        return y < c.height;
+    }
+    const uint32_t tile = r >> 6, within = r & 63u;
+    const uint32_t ty = tile < (1u << 24) ? udiv_small(tile, c.tiles_x, c.inv_tiles_x) : tile / c.tiles_x;
+    x = (tile - ty * c.tiles_x) * 8u + (within & 7u);
+    y = ty * 8u + (within >> 3);
+    return x < c.width && y < c.height;
+}
+
 // ray_gen.rgen:18-24 with GLSL mat4*vec4 order.
 __device__ __forceinline__ void primary_ray(const Camera& c, uint32_t x, uint32_t y, f3& o, f3& d) {
     const float pcx = float(x) + 0.5f, pcy = float(y) + 0.5f;
@@ -534,9 +558,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow(RenderParams rp) 
 // ------------------------------------------------------------------ persistent lanes
 // Persistent wavefronts with per-lane refill: every iteration each active lane performs one
 // walker step; lanes whose ray finished take new work from a global counter (one atomic per
-// wave per refill), so a wavefront is not held back by its slowest ray. src(idx, o, d, tmin,
-// tmax) produces work item idx; snk(idx, o, d, best) consumes its result (called by the
-// lanes that finish in the same iteration, so it may use wave ballots).
+// wave per refill), so a wavefront is not held back by its slowest ray. src(r, id, o, d, tmin,
+// tmax) produces work item r and names its ray `id` (the index results and deferred-polish
+// entries use); snk(id, o, d, best) consumes the result (called by the lanes that finish in
+// the same iteration, so it may use wave ballots).
 template <class G, bool ANY, int STACK, class Src, class Snk>
 __device__ __forceinline__ void persistent_generic(const TraceParams& tp, uint32_t total, uint32_t* counter,
                                                    uint2* lds, Src&& src, Snk&& snk) {
@@ -563,8 +588,7 @@ __device__ __forceinline__ void persistent_generic(const TraceParams& tp, uint32
                 if (!active) {
                     const uint32_t r = base + __popcll(idle & lt);
                     if (r < total) {
-                        idx = r;
-                        src(idx, ow, dw, tmin, tmax);
+                        src(r, idx, ow, dw, tmin, tmax);
                         best.t = tmax;
                         best.inst = 0xFFFFFFFFu;
                         best.pid = 0xFFFFFFFFu;
@@ -758,8 +782,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
                 if (state == kIdle) {
                     const uint32_t r = base + __popcll(idle & lt);
                     if (r < total) {
-                        idx = r;
-                        src(idx, ow, dw, tmin, tmax);
+                        src(r, idx, ow, dw, tmin, tmax);
                         tcur = tmax;
                         s_key[threadIdx.x] =
                             (static_cast<unsigned long long>(__float_as_uint(tmax)) << 32) | kNoHitLow;
@@ -919,7 +942,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TracePara
     __shared__ uint2 stk[STACK * kTraceBlock];
     persistent_lanes<G, ANY, STACK>(
         tp, n, counter, stk + threadIdx.x,
-        [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
+        [&](uint32_t i, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
+            id = i;
             const float4 r0 = reinterpret_cast<const float4*>(rays)[size_t(i) * 2];
             const float4 r1 = reinterpret_cast<const float4*>(rays)[size_t(i) * 2 + 1];
             o = f3{r0.x, r0.y, r0.z};
@@ -1058,15 +1082,24 @@ __global__ void k_polish_shadow(TraceParams tp, SrcShadow src, float* __restrict
 template <class G, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderParams rp) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    const uint32_t npix = rp.cam.width * rp.cam.height;
     persistent_lanes<G, false, STACK>(
-        rp.tp, npix, rp.counters + 1, stk + threadIdx.x,
-        [&](uint32_t pix, f3& o, f3& d, float& tmin, float& tmax) {
-            primary_ray(rp.cam, pix % rp.cam.width, pix / rp.cam.width, o, d);
-            tmin = 0.001f;
-            tmax = 10000.0f;
+        rp.tp, rp.cam.work, rp.counters + 1, stk + threadIdx.x,
+        [&](uint32_t r, uint32_t& pix, f3& o, f3& d, float& tmin, float& tmax) {
+            uint32_t x, y;
+            if (work_pixel(rp.cam, r, x, y)) {
+                pix = y * rp.cam.width + x;
+                primary_ray(rp.cam, x, y, o, d);
+                tmin = 0.001f;
+                tmax = 10000.0f;
+            } else {  // tile slot outside the image: an empty interval, no result
+                pix = 0xFFFFFFFFu;
+                o = d = f3{0.0f, 0.0f, 1.0f};
+                tmin = 1.0f;
+                tmax = 0.0f;
+            }
         },
         [&](uint32_t pix, f3 o, f3 d, const Best& best) {
+            if (pix == 0xFFFFFFFFu) return;
             HitRec h;
             finish_hit(rp.tp, o, d, best, h);
             rp.hitrec[pix] = h;
@@ -1076,11 +1109,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderPara
 
 // Pass 2 (after the optional polish merge): shade misses / palette / unshadowed pixels, queue
 // the rest for shadow rays.
+// Walks the pixels in the primary pass's work order, so the shadow queue keeps its coherence.
 __global__ void k_render_classify(RenderParams rp) {
-    const uint32_t npix = rp.cam.width * rp.cam.height;
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t x = 0, y = 0;
+    const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
+    const uint32_t pix = y * rp.cam.width + x;
     bool hit = false;
-    if (pix < npix) {
+    if (valid) {
         const HitRec h = rp.hitrec[pix];
         hit = h.prim_id != 0xFFFFFFFFu;
         if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
@@ -1116,7 +1152,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
     const SrcShadow src{rp.cam, rp.queue, rp.hitbuf};
     persistent_lanes<G, true, STACK>(
         rp.tp, total, rp.counters + 2, stk + threadIdx.x,
-        [&](uint32_t qi, f3& o, f3& d, float& tmin, float& tmax) { src.get(qi, o, d, tmin, tmax); },
+        [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
+            id = qi;
+            src.get(qi, o, d, tmin, tmax);
+        },
         [&](uint32_t qi, f3 o, f3 d, const Best& best) {
             rp.vis[rp.queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
         });
@@ -1247,7 +1286,8 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                    tp.cand_hits, tp.cand_keys);
                 hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
             }
-            hipLaunchKernelGGL(k_render_classify, grid, block, 0, st, rp);
+            hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kTraceBlock - 1) / kTraceBlock), block, 0, st,
+                               rp);
             bool shadow_timed = false;
             if (rp.shadows && !rp.debug_palette) {
                 if (rp.light_vis) {
