@@ -38,11 +38,12 @@
 extern "C" {
 #endif
 
-#define VKHRT_ABI_VERSION 1
+#define VKHRT_ABI_VERSION 2
 
 typedef struct vkhrt_context vkhrt_context;
 typedef struct vkhrt_blas vkhrt_blas;
 typedef struct vkhrt_tlas vkhrt_tlas;
+typedef struct vkhrt_shadow_mlp vkhrt_shadow_mlp;
 
 typedef enum vkhrt_status {
     VKHRT_SUCCESS = 0,
@@ -138,6 +139,9 @@ typedef struct vkhrt_camera {
 #define VKHRT_RENDER_DEBUG_PALETTE (1u << 1) /* debug.glsl palette by primitive id */
 #define VKHRT_RENDER_RGBA8 (1u << 2)         /* image is RGBA8 (else RGBA32F) */
 #define VKHRT_RENDER_FP64_POLISH (1u << 3)
+#define VKHRT_RENDER_NEURAL_SHADOWS (1u << 4) /* shadow visibility from `shadow_mlp`; rays whose
+                                                 predicted variance p (1 - p) exceeds
+                                                 `shadow_variance_threshold` trace exactly */
 
 typedef struct vkhrt_render_desc {
     uint32_t width;
@@ -146,7 +150,9 @@ typedef struct vkhrt_render_desc {
     void* image;                    /* device, width * height pixels */
     uint32_t* primitive_ids;        /* device, optional (0xFFFFFFFF = miss) */
     const float* light_visibility;  /* device, optional per-pixel light visibility in [0, 1]
-                                       replacing the shadow ray (neural shadow form) */
+                                       replacing the shadow ray */
+    const vkhrt_shadow_mlp* shadow_mlp; /* with VKHRT_RENDER_NEURAL_SHADOWS */
+    float shadow_variance_threshold;    /* 0 = every ray exact, 0.25 = none */
 } vkhrt_render_desc;
 
 /* ---- context */
@@ -235,6 +241,54 @@ int vkhrt_trace_stats(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ra
                       vkhrt_hit* hits, uint32_t flags, uint32_t* stats, void* stream);
 /* Rays of the last vkhrt_render on this context (synchronizes its stream). */
 int vkhrt_render_ray_counts(vkhrt_context* ctx, uint64_t* primary, uint64_t* shadow);
+
+/* Rays of the last vkhrt_render: primary rays, shadow queries (hit pixels that needed a light
+ * visibility) and shadow rays traced exactly (all queries in the exact form, the fallback in
+ * the neural form). Synchronizes the render's stream. */
+typedef struct vkhrt_render_stats {
+    uint64_t primary_rays;
+    uint64_t shadow_queries;
+    uint64_t shadow_rays_traced;
+} vkhrt_render_stats;
+int vkhrt_render_stats_get(vkhrt_context* ctx, vkhrt_render_stats* out);
+
+/* ---- neural shadow visibility (extension; no reference counterpart)
+ * A small MLP predicts the light visibility of a shadow ray from its origin and direction:
+ * (p normalized by the scene box, d) -> 6 x 8 frequencies x {sin, cos} = 96 features ->
+ * 4 x (64, ReLU) -> sigmoid. Trained with binary cross-entropy on shadow rays the exact
+ * tracer labels (vkhrt_shadow_mlp_collect); BF16 MFMA forward/backward, FP32 master weights,
+ * Adam. Samples are 8 floats (p.xyz, 0, d.xyz, 0); targets 1 = lit, 0 = occluded. */
+#define VKHRT_SHADOW_MLP_PARAMS 18753 /* W1[64][96] b1 W2[64][64] b2 W3 b3 W4 b4 w5[64] b5 */
+typedef struct vkhrt_shadow_mlp_desc {
+    float scene_min[3]; /* box the positions are normalized by (e.g. the TLAS bounds) */
+    float scene_max[3];
+    uint64_t seed;      /* weight initialization */
+} vkhrt_shadow_mlp_desc;
+typedef struct vkhrt_shadow_mlp_train_desc {
+    uint32_t batch;       /* samples per Adam step (e.g. 65536) */
+    uint32_t epochs;
+    float learning_rate;  /* e.g. 3e-3 */
+    uint64_t seed;        /* epoch orders */
+} vkhrt_shadow_mlp_train_desc;
+int vkhrt_shadow_mlp_create(vkhrt_context* ctx, const vkhrt_shadow_mlp_desc* desc, vkhrt_shadow_mlp** out);
+void vkhrt_shadow_mlp_destroy(vkhrt_shadow_mlp* mlp);
+/* FP32 parameters in the flat layout (host buffers; synchronous). */
+int vkhrt_shadow_mlp_get_params(vkhrt_context* ctx, const vkhrt_shadow_mlp* mlp, float* params);
+int vkhrt_shadow_mlp_set_params(vkhrt_context* ctx, vkhrt_shadow_mlp* mlp, const float* params);
+/* Traces `count` shadow rays (any hit) and writes training samples and visibility targets. */
+int vkhrt_shadow_mlp_collect(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* shadow_rays, uint32_t count,
+                             float* samples, float* targets, void* stream);
+/* Adam on the mean binary cross-entropy; `mean_loss` (host, optional) receives the mean loss
+ * of the last epoch (synchronizes the stream). */
+int vkhrt_shadow_mlp_train(vkhrt_context* ctx, vkhrt_shadow_mlp* mlp, const float* samples, const float* targets,
+                           uint32_t count, const vkhrt_shadow_mlp_train_desc* desc, float* mean_loss, void* stream);
+/* Gradient of the mean binary cross-entropy over all `count` samples (device, PARAMS floats),
+ * no update; `mean_loss` (host, optional). */
+int vkhrt_shadow_mlp_gradient(vkhrt_context* ctx, vkhrt_shadow_mlp* mlp, const float* samples, const float* targets,
+                              uint32_t count, float* gradient, float* mean_loss, void* stream);
+/* Predicted visibility p per sample (device). */
+int vkhrt_shadow_mlp_infer(vkhrt_context* ctx, const vkhrt_shadow_mlp* mlp, const float* samples, uint32_t count,
+                           float* visibility, void* stream);
 
 /* ---- profiling: per-kernel HIP-event timing on the launch stream */
 int vkhrt_profiling_enable(vkhrt_context* ctx, int enable);

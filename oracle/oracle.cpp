@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "oracle_geom.hpp"
+#include "oracle_mlp.hpp"
 
 using namespace orc;
 
@@ -966,5 +967,51 @@ long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_i
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     return nrays.load();
 }
+
+// ---- neural shadow MLP (oracle_mlp.hpp). samples: n x 8 floats (p.xyz, 0, d.xyz, 0).
+// fp64 = 1: exact double function without bf16 rounding (finite-difference checks);
+// fp64 = 0: FP32 with the GPU kernels' bf16 roundings.
+int orc_mlp_forward(const float* params, const float center[3], const float inv_half[3], const float* samples,
+                    uint32_t n, int fp64, float* out_p) {
+    if (fp64) {
+        std::vector<double> P(params, params + kMlpParams);
+        for (uint32_t s = 0; s < n; ++s) {
+            MlpRef<double, false>::Acts a;
+            MlpRef<double, false>::forward(P, samples + size_t(s) * 8, center, inv_half, a);
+            out_p[s] = float(a.p);
+        }
+    } else {
+        std::vector<float> P(params, params + kMlpParams);
+        for (uint32_t s = 0; s < n; ++s) {
+            MlpRef<float, true>::Acts a;
+            MlpRef<float, true>::forward(P, samples + size_t(s) * 8, center, inv_half, a);
+            out_p[s] = a.p;
+        }
+    }
+    return 0;
+}
+
+// Gradient of the mean binary cross-entropy; params / grad in the flat layout. fp64 = 1
+// takes and returns doubles.
+int orc_mlp_grad(const void* params, const float center[3], const float inv_half[3], const float* samples,
+                 const float* targets, uint32_t n, int fp64, void* grad, double* loss) {
+    if (n == 0) return -1;
+    if (fp64) {
+        const double* pp = static_cast<const double*>(params);
+        std::vector<double> P(pp, pp + kMlpParams), g;
+        const double l = MlpRef<double, false>::grad(P, samples, targets, n, center, inv_half, g);
+        std::memcpy(grad, g.data(), sizeof(double) * kMlpParams);
+        if (loss) *loss = l;
+    } else {
+        const float* pp = static_cast<const float*>(params);
+        std::vector<float> P(pp, pp + kMlpParams), g;
+        const double l = MlpRef<float, true>::grad(P, samples, targets, n, center, inv_half, g);
+        std::memcpy(grad, g.data(), sizeof(float) * kMlpParams);
+        if (loss) *loss = l;
+    }
+    return 0;
+}
+
+int orc_mlp_param_count(void) { return kMlpParams; }
 
 }  // extern "C"

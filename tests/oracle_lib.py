@@ -69,6 +69,9 @@ def lib():
         "orc_trace": ([P, P, u32, P, u32, i32], i32),
         "orc_count_candidates": ([P, P, P, u32, P, i32], i32),
         "orc_render": ([P, P, P, u32, u32, u32, i32, P, P, i32, u32, P], ctypes.c_longlong),
+        "orc_mlp_forward": ([P, P, P, P, u32, i32, P], i32),
+        "orc_mlp_grad": ([P, P, P, P, P, u32, i32, P, ctypes.POINTER(ctypes.c_double)], i32),
+        "orc_mlp_param_count": ([], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -215,3 +218,39 @@ def prhi_f64(ray, curve, per_point=False, radius=0.02):
     out = np.zeros(10, np.float64)
     lib().orc_prhi_f64(ptr(f32(ray)), ptr(f32(curve)), int(per_point), radius, ptr(out))
     return out
+
+
+# ---- neural shadow MLP
+MLP_PARAMS = 18753
+MLP_OFF = {"W1": 0, "b1": 6144, "W2": 6208, "b2": 10304, "W3": 10368, "b3": 14464, "W4": 14528,
+           "b4": 18624, "w5": 18688, "b5": 18752}
+
+
+def mlp_forward(params, center, inv_half, samples, fp64=False):
+    samples = f32(samples)
+    out = np.zeros(samples.shape[0], np.float32)
+    lib().orc_mlp_forward(ptr(f32(params)), ptr(f32(center)), ptr(f32(inv_half)), ptr(samples),
+                          samples.shape[0], int(fp64), ptr(out))
+    return out
+
+
+def mlp_grad(params, center, inv_half, samples, targets, fp64=False):
+    samples = f32(samples)
+    dt = np.float64 if fp64 else np.float32
+    P = np.ascontiguousarray(params, dt)
+    g = np.zeros(MLP_PARAMS, dt)
+    loss = ctypes.c_double(0)
+    lib().orc_mlp_grad(P.ctypes.data_as(ctypes.c_void_p), ptr(f32(center)), ptr(f32(inv_half)), ptr(samples),
+                       ptr(f32(targets)), samples.shape[0], int(fp64), g.ctypes.data_as(ctypes.c_void_p),
+                       ctypes.byref(loss))
+    return g, loss.value
+
+
+def mlp_init(seed):
+    """He-uniform weights, zero biases (test parameters)."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros(MLP_PARAMS, np.float32)
+    for name, fan_in, size in [("W1", 96, 6144), ("W2", 64, 4096), ("W3", 64, 4096), ("W4", 64, 4096), ("w5", 64, 64)]:
+        lim = np.sqrt(6.0 / fan_in)
+        P[MLP_OFF[name]:MLP_OFF[name] + size] = rng.uniform(-lim, lim, size)
+    return P

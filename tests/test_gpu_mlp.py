@@ -1,0 +1,202 @@
+"""GPU tests of the neural shadow-visibility MLP (BF16 MFMA kernels) against the oracle
+(oracle/oracle_mlp.hpp), and of the neural shadow form of the render against the exact one."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+SCENE_MIN = (-150.0, -110.0, -150.0)
+SCENE_MAX = (150.0, 190.0, 150.0)
+
+
+@pytest.fixture(scope="module")
+def O():
+    import oracle_lib
+    return oracle_lib
+
+
+def _box(O):
+    center = np.array([(a + b) / 2 for a, b in zip(SCENE_MIN, SCENE_MAX)], np.float32)
+    inv_half = np.array([2.0 / (b - a) for a, b in zip(SCENE_MIN, SCENE_MAX)], np.float32)
+    return center, inv_half
+
+
+def _samples(n, seed):
+    rng = np.random.default_rng(seed)
+    s = np.zeros((n, 8), np.float32)
+    s[:, :3] = rng.uniform(SCENE_MIN, SCENE_MAX, (n, 3))
+    d = rng.normal(size=(n, 3))
+    s[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    return s
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_forward_matches_oracle(gpu_ctx, O):
+    import vkhrt_amd as V
+    mlp = V.ShadowMLP(gpu_ctx, SCENE_MIN, SCENE_MAX, seed=3)
+    P = O.mlp_init(11)
+    P[O.MLP_OFF["b1"]:O.MLP_OFF["b1"] + 64] = 0.1
+    P[O.MLP_OFF["b5"]] = -0.3
+    mlp.set_params(P)
+    assert np.array_equal(np.array(mlp.get_params(), np.float32), P)
+    smp = _samples(5000, 12)
+    ds = _dev(smp)
+    out = torch.empty(5000, dtype=torch.float32, device="cuda")
+    print("[mlp forward] 5000 samples", flush=True)
+    mlp.infer(ds.data_ptr(), 5000, out.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    center, inv_half = _box(O)
+    ref = O.mlp_forward(P, center, inv_half, smp)
+    exact = O.mlp_forward(P, center, inv_half, smp, fp64=True)
+    err = np.abs(got - ref)
+    print(f"forward |gpu - oracle(bf16)| max {err.max():.3g} mean {err.mean():.3g}; "
+          f"|gpu - fp64| max {np.abs(got - exact).max():.3g}")
+    assert err.max() < 1e-2 and err.mean() < 1e-3
+    assert np.abs(got - exact).max() < 0.05
+
+
+def test_gradient_matches_oracle(gpu_ctx, O):
+    import vkhrt_amd as V
+    mlp = V.ShadowMLP(gpu_ctx, SCENE_MIN, SCENE_MAX, seed=4)
+    P = O.mlp_init(21)
+    P[O.MLP_OFF["b1"]:O.MLP_OFF["b1"] + 64] = 0.05
+    mlp.set_params(P)
+    n = 1000  # not a multiple of 64: exercises the partial chunk
+    smp = _samples(n, 22)
+    tgt = (np.random.default_rng(23).random(n) < 0.4).astype(np.float32)
+    ds, dt = _dev(smp), _dev(tgt)
+    grad = torch.zeros(V.SHADOW_MLP_PARAMS, dtype=torch.float32, device="cuda")
+    print("[mlp gradient] 1000 samples", flush=True)
+    loss = mlp.gradient(ds.data_ptr(), dt.data_ptr(), n, grad.data_ptr())
+    torch.cuda.synchronize()
+    g = grad.cpu().numpy()
+    center, inv_half = _box(O)
+    gref, lref = O.mlp_grad(P, center, inv_half, smp, tgt)
+    assert abs(loss - lref) < 1e-3 * max(1.0, abs(lref)), (loss, lref)
+    rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print(f"gradient relative L2 error vs oracle {rel:.3g}; loss {loss:.6f} vs {lref:.6f}")
+    assert rel < 2e-2
+    for name, size in [("W1", 6144), ("W2", 4096), ("W3", 4096), ("W4", 4096), ("w5", 64)]:
+        a = g[O.MLP_OFF[name]:O.MLP_OFF[name] + size]
+        b = gref[O.MLP_OFF[name]:O.MLP_OFF[name] + size]
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+        assert cos > 0.999, (name, cos)
+    # the same parameters are untouched by a gradient evaluation
+    assert np.array_equal(np.array(mlp.get_params(), np.float32), P)
+
+
+def test_training_learns_a_half_space(gpu_ctx, O):
+    """Visibility = [y_normalized > 0.2]: Adam on the device drives the loss down and the
+    classifier right on held-out samples."""
+    import vkhrt_amd as V
+    mlp = V.ShadowMLP(gpu_ctx, SCENE_MIN, SCENE_MAX, seed=5)
+    n = 1 << 16
+    smp = _samples(n, 31)
+    center, inv_half = _box(O)
+    y = ((smp[:, 1] - center[1]) * inv_half[1] > 0.2).astype(np.float32)
+    ds, dt = _dev(smp), _dev(y)
+    print("[mlp train] half space", flush=True)
+    l0 = mlp.train(ds.data_ptr(), dt.data_ptr(), n, batch=8192, epochs=1, learning_rate=3e-3)
+    l1 = mlp.train(ds.data_ptr(), dt.data_ptr(), n, batch=8192, epochs=15, learning_rate=3e-3)
+    hold = _samples(20000, 32)
+    yh = ((hold[:, 1] - center[1]) * inv_half[1] > 0.2)
+    out = torch.empty(20000, dtype=torch.float32, device="cuda")
+    mlp.infer(_dev(hold).data_ptr(), 20000, out.data_ptr())
+    torch.cuda.synchronize()
+    acc = float(((out.cpu().numpy() > 0.5) == yh).mean())
+    print(f"loss after 1 epoch {l0:.4f}, after 16 {l1:.4f}; held-out accuracy {acc:.4f}")
+    assert l1 < 0.5 * l0 and l1 < 0.1
+    assert acc > 0.97
+
+
+def _hair_tlas(ctx, nstrands, seed):
+    import gpu_util as G
+    import oracle_lib as O
+    import vkhrt_amd as V
+    curves = O.hair_curves(seed, nstrands, 16)
+    blas = G.build_blas(ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
+    return blas, V.TopLevelAccelerationStructure(ctx, [blas])
+
+
+def _shadow_samples(ctx, tlas, mlp, eyes, W, H):
+    """Training data from the exact tracer: primary hits of each view -> shadow ray toward
+    the light -> traced visibility."""
+    import vkhrt_amd as V
+    samples, targets = [], []
+    for eye in eyes:
+        cam = V.camera_look_at(eye, (0.0, 40.0, 0.0), aspect=W / H)
+        n = W * H
+        rays = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+        hits = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+        ctx.generate_camera_rays(cam, W, H, rays.data_ptr())
+        ctx.trace_rays(tlas, rays.data_ptr(), n, hits.data_ptr())
+        srays = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+        cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+        ctx.generate_shadow_rays(rays.data_ptr(), hits.data_ptr(), n, srays.data_ptr(), cnt.data_ptr())
+        torch.cuda.synchronize()
+        m = int(cnt.item())
+        s = torch.empty(m * 8, dtype=torch.float32, device="cuda")
+        t = torch.empty(m, dtype=torch.float32, device="cuda")
+        mlp.collect(tlas, srays.data_ptr(), m, s.data_ptr(), t.data_ptr())
+        samples.append(s.view(m, 8))
+        targets.append(t)
+    torch.cuda.synchronize()
+    return torch.cat(samples).contiguous(), torch.cat(targets).contiguous()
+
+
+def test_neural_shadow_render(gpu_ctx):
+    """Trained on shadow rays the exact tracer labels from 8 views, the MLP renders a held-out
+    view at PSNR >= 40 dB against the exact image while tracing only the uncertain shadow
+    rays; an untrained network at the same threshold does worse. Threshold 0 reproduces the
+    exact image bit for bit; 0.25 traces no shadow ray."""
+    import gpu_util as G
+    import vkhrt_amd as V
+    ctx = gpu_ctx
+    blas, tlas = _hair_tlas(ctx, 20000, 77)
+    mlp = V.ShadowMLP(ctx, SCENE_MIN, SCENE_MAX, seed=9)
+    untrained = V.ShadowMLP(ctx, SCENE_MIN, SCENE_MAX, seed=9)
+    eyes = [(300 * math.sin(a), 60.0 + 80 * math.cos(3 * a), 300 * math.cos(a))
+            for a in np.linspace(0, 2 * math.pi, 8, endpoint=False)]
+    print("[neural render] collecting", flush=True)
+    smp, tgt = _shadow_samples(ctx, tlas, mlp, eyes, 384, 384)
+    n = smp.shape[0]
+    print(f"[neural render] training on {n} shadow rays ({float(tgt.mean()):.3f} lit)", flush=True)
+    loss = mlp.train(smp.data_ptr(), tgt.data_ptr(), n, batch=16384, epochs=30, learning_rate=3e-3)
+    W = H = 512
+    cam = V.camera_look_at((120.0, 90.0, 330.0), (0.0, 40.0, 0.0), aspect=1.0)  # not a training view
+    exact, ids = G.render(ctx, tlas, cam, W, H, V.RENDER_SHADOWS)
+    _, queries, traced = ctx.render_stats()
+
+    def neural(net, thr):
+        img = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+        ctx.render(tlas, cam, W, H, img.data_ptr(), flags=V.RENDER_SHADOWS | V.RENDER_NEURAL_SHADOWS,
+                   shadow_mlp=net, variance_threshold=thr, stream=G.stream_ptr())
+        torch.cuda.synchronize()
+        _, q, tr = ctx.render_stats()
+        return img.cpu().numpy().reshape(H, W, 4), q, tr
+
+    same, q0, t0 = neural(mlp, 0.0)
+    assert np.array_equal(same, exact) and t0 == q0 == queries
+    _, _, tnone = neural(mlp, 0.25)
+    assert tnone == 0
+    thr = 0.05
+    img, q, tr = neural(mlp, thr)
+    uimg, _, utr = neural(untrained, thr)
+    p_trained = G.psnr(img[..., :3], exact[..., :3])
+    p_untrained = G.psnr(uimg[..., :3], exact[..., :3])
+    p_all, _, _ = neural(mlp, 0.25)
+    p_nofallback = G.psnr(p_all[..., :3], exact[..., :3])
+    print(f"train loss {loss:.4f}; shadow queries {q}: traced exactly {tr} ({tr / q:.3f}) trained, "
+          f"{utr} untrained; PSNR trained {p_trained:.2f} dB, untrained {p_untrained:.2f} dB, "
+          f"trained without fallback {p_nofallback:.2f} dB")
+    assert p_trained >= 40.0
+    assert tr < q  # the network answers part of the queries
+    assert tr < utr or p_trained > p_untrained + 3.0

@@ -29,6 +29,9 @@ RENDER_SHADOWS = 1 << 0
 RENDER_DEBUG_PALETTE = 1 << 1
 RENDER_RGBA8 = 1 << 2
 RENDER_FP64_POLISH = 1 << 3
+RENDER_NEURAL_SHADOWS = 1 << 4
+
+SHADOW_MLP_PARAMS = 18753
 
 OPTION_STACK_KIND = 1
 OPTION_LEAF_SIZE = 2
@@ -48,7 +51,10 @@ EXPORTS = [
     "vkhrt_generate_curves", "vkhrt_curves_to_lines", "vkhrt_generate_lss", "vkhrt_generate_dots",
     "vkhrt_profiling_enable", "vkhrt_kernel_stats", "vkhrt_status_counters",
     "vkhrt_generate_camera_rays", "vkhrt_generate_shadow_rays", "vkhrt_count_candidates",
-    "vkhrt_render_ray_counts", "vkhrt_trace_stats",
+    "vkhrt_render_ray_counts", "vkhrt_trace_stats", "vkhrt_render_stats_get",
+    "vkhrt_shadow_mlp_create", "vkhrt_shadow_mlp_destroy", "vkhrt_shadow_mlp_get_params",
+    "vkhrt_shadow_mlp_set_params", "vkhrt_shadow_mlp_collect", "vkhrt_shadow_mlp_train",
+    "vkhrt_shadow_mlp_gradient", "vkhrt_shadow_mlp_infer",
 ]
 
 
@@ -79,7 +85,23 @@ class Camera(ctypes.Structure):
 class RenderDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("flags", ctypes.c_uint32),
                 ("image", ctypes.c_void_p), ("primitive_ids", ctypes.c_void_p),
-                ("light_visibility", ctypes.c_void_p)]
+                ("light_visibility", ctypes.c_void_p), ("shadow_mlp", ctypes.c_void_p),
+                ("shadow_variance_threshold", ctypes.c_float)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("primary_rays", ctypes.c_uint64), ("shadow_queries", ctypes.c_uint64),
+                ("shadow_rays_traced", ctypes.c_uint64)]
+
+
+class ShadowMlpDesc(ctypes.Structure):
+    _fields_ = [("scene_min", ctypes.c_float * 3), ("scene_max", ctypes.c_float * 3),
+                ("seed", ctypes.c_uint64)]
+
+
+class ShadowMlpTrainDesc(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_uint32), ("epochs", ctypes.c_uint32),
+                ("learning_rate", ctypes.c_float), ("seed", ctypes.c_uint64)]
 
 
 class StrandParams(ctypes.Structure):
@@ -130,6 +152,16 @@ def lib():
         "vkhrt_count_candidates": ([P, P, P, P, u32, P, P], i32),
         "vkhrt_render_ray_counts": ([P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)], i32),
         "vkhrt_trace_stats": ([P, P, P, u32, P, u32, P, P], i32),
+        "vkhrt_render_stats_get": ([P, ctypes.POINTER(RenderStats)], i32),
+        "vkhrt_shadow_mlp_create": ([P, ctypes.POINTER(ShadowMlpDesc), ctypes.POINTER(P)], i32),
+        "vkhrt_shadow_mlp_destroy": ([P], None),
+        "vkhrt_shadow_mlp_get_params": ([P, P, P], i32),
+        "vkhrt_shadow_mlp_set_params": ([P, P, P], i32),
+        "vkhrt_shadow_mlp_collect": ([P, P, P, u32, P, P, P], i32),
+        "vkhrt_shadow_mlp_train": ([P, P, P, P, u32, ctypes.POINTER(ShadowMlpTrainDesc),
+                                    ctypes.POINTER(f32), P], i32),
+        "vkhrt_shadow_mlp_gradient": ([P, P, P, P, u32, P, ctypes.POINTER(f32), P], i32),
+        "vkhrt_shadow_mlp_infer": ([P, P, P, u32, P, P], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -190,9 +222,13 @@ class Context:
                    "trace_rays")
 
     def render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS,
-               prim_ids_ptr=None, light_visibility_ptr=None, stream=None):
-        """traceRaysKHR(W, H, 1) with ray_gen.rgen and the closest-hit shaders."""
-        d = RenderDesc(width, height, flags, image_ptr, prim_ids_ptr, light_visibility_ptr)
+               prim_ids_ptr=None, light_visibility_ptr=None, stream=None, shadow_mlp=None,
+               variance_threshold=0.0):
+        """traceRaysKHR(W, H, 1) with ray_gen.rgen and the closest-hit shaders. With
+        RENDER_NEURAL_SHADOWS, `shadow_mlp` answers shadow queries whose predicted variance
+        p (1 - p) is at most `variance_threshold`; the rest trace exactly."""
+        d = RenderDesc(width, height, flags, image_ptr, prim_ids_ptr, light_visibility_ptr,
+                       shadow_mlp.h if shadow_mlp is not None else None, variance_threshold)
         self.check(lib().vkhrt_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), stream),
                    "render")
 
@@ -243,6 +279,12 @@ class Context:
         """trace_rays with per-ray (nodes visited, primitives tested) into stats (2 x u32)."""
         self.check(lib().vkhrt_trace_stats(self.h, tlas.h, rays_ptr, count, hits_ptr, flags, stats_ptr,
                                            stream), "trace_stats")
+
+    def render_stats(self):
+        """(primary rays, shadow queries, shadow rays traced exactly) of the last render."""
+        st = RenderStats()
+        self.check(lib().vkhrt_render_stats_get(self.h, ctypes.byref(st)), "render_stats")
+        return st.primary_rays, st.shadow_queries, st.shadow_rays_traced
 
     def render_ray_counts(self):
         p, s = ctypes.c_uint64(), ctypes.c_uint64()
@@ -317,3 +359,55 @@ class TopLevelAccelerationStructure:
 
     def __del__(self):
         self.close()
+
+
+class ShadowMLP:
+    """Neural shadow-visibility MLP (BF16 MFMA training and inference on the device)."""
+
+    def __init__(self, ctx, scene_min, scene_max, seed=1):
+        self.ctx = ctx
+        d = ShadowMlpDesc((ctypes.c_float * 3)(*scene_min), (ctypes.c_float * 3)(*scene_max), seed)
+        h = ctypes.c_void_p()
+        ctx.check(lib().vkhrt_shadow_mlp_create(ctx.h, ctypes.byref(d), ctypes.byref(h)), "shadow_mlp_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vkhrt_shadow_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get_params(self):
+        buf = (ctypes.c_float * SHADOW_MLP_PARAMS)()
+        self.ctx.check(lib().vkhrt_shadow_mlp_get_params(self.ctx.h, self.h, buf), "shadow_mlp_get_params")
+        return list(buf)
+
+    def set_params(self, params):
+        buf = (ctypes.c_float * SHADOW_MLP_PARAMS)(*[float(x) for x in params])
+        self.ctx.check(lib().vkhrt_shadow_mlp_set_params(self.ctx.h, self.h, buf), "shadow_mlp_set_params")
+
+    def collect(self, tlas, shadow_rays_ptr, count, samples_ptr, targets_ptr, stream=None):
+        self.ctx.check(lib().vkhrt_shadow_mlp_collect(self.ctx.h, tlas.h, shadow_rays_ptr, count, samples_ptr,
+                                                      targets_ptr, stream), "shadow_mlp_collect")
+
+    def train(self, samples_ptr, targets_ptr, count, batch=65536, epochs=1, learning_rate=3e-3, seed=1,
+              stream=None):
+        d = ShadowMlpTrainDesc(batch, epochs, learning_rate, seed)
+        loss = ctypes.c_float()
+        self.ctx.check(lib().vkhrt_shadow_mlp_train(self.ctx.h, self.h, samples_ptr, targets_ptr, count,
+                                                    ctypes.byref(d), ctypes.byref(loss), stream),
+                       "shadow_mlp_train")
+        return loss.value
+
+    def gradient(self, samples_ptr, targets_ptr, count, grad_ptr, stream=None):
+        loss = ctypes.c_float()
+        self.ctx.check(lib().vkhrt_shadow_mlp_gradient(self.ctx.h, self.h, samples_ptr, targets_ptr, count,
+                                                       grad_ptr, ctypes.byref(loss), stream),
+                       "shadow_mlp_gradient")
+        return loss.value
+
+    def infer(self, samples_ptr, count, out_ptr, stream=None):
+        self.ctx.check(lib().vkhrt_shadow_mlp_infer(self.ctx.h, self.h, samples_ptr, count, out_ptr, stream),
+                       "shadow_mlp_infer")

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <new>
 #include <string>
 #include <vector>
@@ -14,6 +15,7 @@
 #include "build.hpp"
 #include "geometry.hpp"
 #include "internal.hpp"
+#include "mlp.hpp"
 
 namespace {
 
@@ -54,6 +56,20 @@ struct vkhrt_context {
     uint32_t defer_cap = 0;
     unsigned long long* keys = nullptr;
     size_t keys_cap = 0;
+    // neural shadow form
+    float4* origins = nullptr;
+    uint32_t* fallback = nullptr;
+    size_t cap_neural = 0;
+    bool last_neural = false;
+};
+
+struct vkhrt_shadow_mlp {
+    vkhrt_context* ctx = nullptr;
+    vk::MlpDev dev{};
+    float* partial = nullptr;  // [blocks][params] gradient partials
+    uint32_t partial_blocks = 0;
+    float* loss = nullptr;     // device scalar
+    uint32_t step = 0;         // Adam steps taken
 };
 
 struct vkhrt_blas {
@@ -291,6 +307,29 @@ int ensure_polish_scratch(vkhrt_context* c, size_t nrays) {
     return VKHRT_SUCCESS;
 }
 
+int ensure_neural_scratch(vkhrt_context* c, size_t npix) {
+    if (npix <= c->cap_neural) return VKHRT_SUCCESS;
+    hipFree(c->origins);
+    hipFree(c->fallback);
+    c->origins = nullptr;
+    c->fallback = nullptr;
+    c->cap_neural = 0;
+    hipError_t e = hipMalloc(&c->origins, npix * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&c->fallback, npix * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_fail(c, e, "neural shadow scratch");
+    c->cap_neural = npix;
+    return VKHRT_SUCCESS;
+}
+
+const float kZeroF = 0.0f;
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 void set_polish(vkhrt_context* c, vk::TraceParams& tp) {
     tp.defer = c->defer;
     tp.defer_count = c->counters + 4;
@@ -346,6 +385,8 @@ void vkhrt_context_destroy(vkhrt_context* c) {
     hipFree(c->cand_hits);
     hipFree(c->cand_keys);
     hipFree(c->keys);
+    hipFree(c->origins);
+    hipFree(c->fallback);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -651,7 +692,8 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     if (!d->image) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null image");
     if (d->width == 0 || d->height == 0 || uint64_t(d->width) * d->height > (1ull << 28))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad image size");
-    const uint32_t known = VKHRT_RENDER_SHADOWS | VKHRT_RENDER_DEBUG_PALETTE | VKHRT_RENDER_RGBA8 | VKHRT_RENDER_FP64_POLISH;
+    const uint32_t known = VKHRT_RENDER_SHADOWS | VKHRT_RENDER_DEBUG_PALETTE | VKHRT_RENDER_RGBA8 |
+                           VKHRT_RENDER_FP64_POLISH | VKHRT_RENDER_NEURAL_SHADOWS;
     if (d->flags & ~known) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown render flags");
     const size_t npix = size_t(d->width) * d->height;
     int rc = ensure_render_scratch(c, npix);
@@ -686,6 +728,22 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
         set_polish(c, rp.tp);
         if (!rp.tp.work) rp.tp.work = c->counters + 1;
     }
+    const bool neural = (d->flags & VKHRT_RENDER_NEURAL_SHADOWS) && rp.shadows && !rp.debug_palette &&
+                        !d->light_visibility;
+    if (neural) {
+        if (!d->shadow_mlp) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "neural shadows need shadow_mlp");
+        if (d->shadow_mlp->ctx != c) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "shadow_mlp of another context");
+        if (!rp.tp.work || t->variant == vk::kVarMixed)
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "neural shadows need the persistent kernels and a single-mode TLAS");
+        if (!(d->shadow_variance_threshold >= 0.0f))
+            return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "variance threshold must be >= 0");
+        rc = ensure_neural_scratch(c, npix);
+        if (rc != VKHRT_SUCCESS) return rc;
+        rp.mlp = &d->shadow_mlp->dev;
+        rp.mlp_threshold = d->shadow_variance_threshold;
+        rp.origins = c->origins;
+        rp.fallback_queue = c->fallback;
+    }
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (c->profiling)
         for (auto& x : ev) hipEventCreate(&x);
@@ -698,7 +756,22 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     if (e != hipSuccess) return hip_fail(c, e, "render launch");
     c->last_pixels = npix;
     c->last_shadows = rp.shadows && !rp.debug_palette && !d->light_visibility;
+    c->last_neural = neural;
     c->last_stream = st;
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_render_stats_get(vkhrt_context* c, vkhrt_render_stats* out) {
+    if (!c || !out) return VKHRT_ERROR_INVALID_ARGUMENT;
+    *out = vkhrt_render_stats{c->last_pixels, 0, 0};
+    if (c->last_pixels && c->last_shadows) {
+        uint32_t q[6] = {0, 0, 0, 0, 0, 0};
+        hipError_t e = hipStreamSynchronize(c->last_stream);
+        if (e == hipSuccess) e = hipMemcpy(q, c->counters, sizeof(q), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(c, e, "render stats");
+        out->shadow_queries = q[0];
+        out->shadow_rays_traced = c->last_neural ? q[5] : q[0];
+    }
     return VKHRT_SUCCESS;
 }
 
@@ -722,7 +795,7 @@ int vkhrt_render_ray_counts(vkhrt_context* c, uint64_t* primary, uint64_t* shado
     if (c->last_pixels && c->last_shadows) {
         uint32_t q = 0;
         hipError_t e = hipStreamSynchronize(c->last_stream);
-        if (e == hipSuccess) e = hipMemcpy(&q, c->counters, 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(&q, c->counters + (c->last_neural ? 5 : 0), 4, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(c, e, "render counts");
         *shadow = q;
     }
@@ -844,6 +917,192 @@ int vkhrt_generate_dots(vkhrt_context* c, const float* lines, uint32_t n, float 
 }
 
 // ---- profiling
+// ---- neural shadow visibility
+int vkhrt_shadow_mlp_create(vkhrt_context* c, const vkhrt_shadow_mlp_desc* d, vkhrt_shadow_mlp** out) {
+    if (!c || !d || !out) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    float center[3], inv_half[3];
+    for (int i = 0; i < 3; ++i) {
+        const float lo = d->scene_min[i], hi = d->scene_max[i];
+        if (!(hi > lo)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "empty scene box");
+        center[i] = 0.5f * (lo + hi);
+        inv_half[i] = 2.0f / (hi - lo);
+    }
+    hipError_t e = vk::mlp_enable_train_lds();
+    if (e != hipSuccess) return hip_fail(c, e, "MLP training kernel LDS");
+    vkhrt_shadow_mlp* m = new (std::nothrow) vkhrt_shadow_mlp();
+    if (!m) return VKHRT_ERROR_OUT_OF_MEMORY;
+    m->ctx = c;
+    std::memcpy(m->dev.center, center, sizeof(center));
+    std::memcpy(m->dev.inv_half, inv_half, sizeof(inv_half));
+    const size_t P = vk::kMlpParamCount;
+    e = hipMalloc(&m->dev.master, P * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->dev.adam_m, P * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->dev.adam_v, P * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->dev.packed, 120 * 256 * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(&m->loss, sizeof(float));
+    if (e != hipSuccess) {
+        vkhrt_shadow_mlp_destroy(m);
+        return hip_fail(c, e, "MLP buffers");
+    }
+    // He-uniform weights, zero biases
+    std::vector<float> w(P, 0.0f);
+    uint64_t st = d->seed * 0xD1B54A32D192ED03ull + 0x5DEECE66Dull;
+    auto fill = [&](int off, int fan_in, int count) {
+        const float lim = std::sqrt(6.0f / float(fan_in));
+        for (int i = 0; i < count; ++i)
+            w[off + i] = lim * (2.0f * float(splitmix(st) >> 40) * (1.0f / 16777216.0f) - 1.0f);
+    };
+    fill(0, 96, 64 * 96);
+    fill(6208, 64, 64 * 64);
+    fill(10368, 64, 64 * 64);
+    fill(14528, 64, 64 * 64);
+    fill(18688, 64, 64);
+    int rc = vkhrt_shadow_mlp_set_params(c, m, w.data());
+    if (rc != VKHRT_SUCCESS) {
+        vkhrt_shadow_mlp_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return VKHRT_SUCCESS;
+}
+
+void vkhrt_shadow_mlp_destroy(vkhrt_shadow_mlp* m) {
+    if (!m) return;
+    hipFree(m->dev.master);
+    hipFree(m->dev.adam_m);
+    hipFree(m->dev.adam_v);
+    hipFree(m->dev.packed);
+    hipFree(m->partial);
+    hipFree(m->loss);
+    delete m;
+}
+
+int vkhrt_shadow_mlp_get_params(vkhrt_context* c, const vkhrt_shadow_mlp* m, float* params) {
+    if (!c || !m || !params) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(params, m->dev.master, vk::kMlpParamCount * sizeof(float), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "MLP params");
+}
+
+// Sets the FP32 parameters, clears the Adam state and rebuilds the bf16 fragments.
+int vkhrt_shadow_mlp_set_params(vkhrt_context* c, vkhrt_shadow_mlp* m, const float* params) {
+    if (!c || !m || !params) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    const size_t bytes = vk::kMlpParamCount * sizeof(float);
+    std::vector<float> zero(vk::kMlpParamCount, 0.0f);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(m->dev.master, params, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->dev.adam_m, zero.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->dev.adam_v, zero.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = vk::launch_mlp_pack(m->dev, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    m->step = 0;
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "MLP params");
+}
+
+int vkhrt_shadow_mlp_collect(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count,
+                             float* samples, float* targets, void* stream) {
+    if (!c || !t || (count && (!rays || !samples || !targets))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    if (count == 0) return VKHRT_SUCCESS;
+    hipStream_t st = pick(c, stream);
+    vk::HitRec* hits = nullptr;
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&hits), size_t(count) * sizeof(vk::HitRec), st);
+    if (e != hipSuccess) return hip_fail(c, e, "collect scratch");
+    int rc = vkhrt_trace_rays(c, t, rays, count, reinterpret_cast<vkhrt_hit*>(hits), VKHRT_TRACE_TERMINATE_ON_FIRST_HIT,
+                              stream);
+    if (rc == VKHRT_SUCCESS) {
+        e = vk::launch_mlp_collect(reinterpret_cast<const float*>(rays), hits, count, samples, targets, st);
+        if (e != hipSuccess) rc = hip_fail(c, e, "collect");
+    }
+    hipFreeAsync(hits, st);
+    return rc;
+}
+
+namespace {
+// One pass of gradient steps over `count` samples (or the full-batch gradient when `grad`).
+int mlp_run(vkhrt_context* c, vkhrt_shadow_mlp* m, const float* samples, const float* targets, uint32_t count,
+            uint32_t batch, uint32_t epochs, float lr, uint64_t seed, float* grad, float* mean_loss, hipStream_t st) {
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t step_n = std::min(batch, count);
+    const uint32_t blocks = std::max(1u, std::min<uint32_t>(uint32_t(std::max(cus, 1)), (step_n + 63u) / 64u));
+    if (blocks > m->partial_blocks) {
+        hipFree(m->partial);
+        m->partial = nullptr;
+        m->partial_blocks = 0;
+        hipError_t e = hipMalloc(&m->partial, size_t(blocks) * vk::kMlpParamCount * sizeof(float));
+        if (e != hipSuccess) return hip_fail(c, e, "MLP partials");
+        m->partial_blocks = blocks;
+    }
+    uint64_t rs = seed * 0x9E3779B97F4A7C15ull + 1;
+    hipError_t e = hipSuccess;
+    for (uint32_t ep = 0; ep < epochs && e == hipSuccess; ++ep) {
+        vk::MlpTrainParams tp{};
+        tp.samples = reinterpret_cast<const float4*>(samples);
+        tp.targets = targets;
+        tp.total = count;
+        tp.partial = m->partial;
+        tp.loss = m->loss;
+        // epoch order i -> (i mul + add) mod count with gcd(mul, count) = 1; identity for the gradient
+        tp.perm_mul = 1;
+        tp.perm_add = 0;
+        if (!grad && count > 1) {
+            uint64_t mul = (splitmix(rs) % count) | 1u;
+            while (std::gcd<uint64_t, uint64_t>(mul, count) != 1) mul += 2;
+            tp.perm_mul = mul % count;
+            tp.perm_add = splitmix(rs) % count;
+        }
+        e = hipMemcpyAsync(m->loss, &kZeroF, sizeof(float), hipMemcpyHostToDevice, st);
+        for (uint32_t first = 0; first < count && e == hipSuccess; first += step_n) {
+            tp.first = first;
+            tp.n = std::min(step_n, count - first);
+            const uint32_t nb = std::max(1u, std::min(blocks, (tp.n + 63u) / 64u));
+            vk::AdamParams ap{lr, 0.9f, 0.999f, 1e-8f, 1.0f, 1.0f};
+            if (!grad) {
+                ++m->step;
+                ap.bias1 = 1.0f - std::pow(0.9f, float(m->step));
+                ap.bias2 = 1.0f - std::pow(0.999f, float(m->step));
+            }
+            e = vk::launch_mlp_train_step(m->dev, tp, nb, grad, ap, st);
+            if (grad) break;
+        }
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "MLP training");
+    if (mean_loss) {
+        float l = 0.0f;
+        e = hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(c, e, "MLP loss");
+        *mean_loss = l / float(count);
+    }
+    return VKHRT_SUCCESS;
+}
+}  // namespace
+
+int vkhrt_shadow_mlp_train(vkhrt_context* c, vkhrt_shadow_mlp* m, const float* samples, const float* targets,
+                           uint32_t count, const vkhrt_shadow_mlp_train_desc* d, float* mean_loss, void* stream) {
+    if (!c || !m || !d || (count && (!samples || !targets))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    if (count == 0 || d->epochs == 0) return VKHRT_SUCCESS;
+    if (d->batch == 0 || !(d->learning_rate > 0.0f)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad training setup");
+    return mlp_run(c, m, samples, targets, count, d->batch, d->epochs, d->learning_rate, d->seed, nullptr, mean_loss,
+                   pick(c, stream));
+}
+
+int vkhrt_shadow_mlp_gradient(vkhrt_context* c, vkhrt_shadow_mlp* m, const float* samples, const float* targets,
+                              uint32_t count, float* gradient, float* mean_loss, void* stream) {
+    if (!c || !m || !gradient || !count || !samples || !targets)
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    return mlp_run(c, m, samples, targets, count, count, 1, 0.0f, 0, gradient, mean_loss, pick(c, stream));
+}
+
+int vkhrt_shadow_mlp_infer(vkhrt_context* c, const vkhrt_shadow_mlp* m, const float* samples, uint32_t count,
+                           float* visibility, void* stream) {
+    if (!c || !m || (count && (!samples || !visibility))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    hipError_t e = vk::launch_mlp_infer(m->dev, samples, count, visibility, pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "MLP inference");
+}
+
 int vkhrt_profiling_enable(vkhrt_context* c, int enable) {
     if (!c) return VKHRT_ERROR_INVALID_ARGUMENT;
     resolve_timings(c);

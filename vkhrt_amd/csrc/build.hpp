@@ -48,6 +48,8 @@ struct Camera {
     float inv_w, inv_tiles_x;  // 1 / width, 1 / tiles_x for the index division
 };
 
+struct MlpDev;
+
 struct RenderParams {
     TraceParams tp;
     Camera cam;
@@ -63,6 +65,11 @@ struct RenderParams {
     // deferred-polish pipeline scratch
     HitRec* hitrec;       // per-pixel primary hit
     float* vis;           // per-pixel shadow visibility
+    // neural shadow form (optional): the MLP gates the shadow queue, uncertain pixels fall back
+    const MlpDev* mlp;         // host-side descriptor (read by the launch code only), null = exact
+    float mlp_threshold;       // exact fallback when p (1 - p) > threshold
+    float4* origins;           // shadow-ray origin per queue entry
+    uint32_t* fallback_queue;  // pixels left to the exact shadow ray ([5] of counters = length)
 };
 
 // Compile-time geometry variants of the traversal kernels. A TLAS whose instances share

@@ -1,0 +1,467 @@
+// Neural shadow-visibility MLP on gfx950: BF16 MFMA forward, backward and fused inference with
+// FP32 accumulation and FP32 master weights (Adam). No reference counterpart: the reference
+// has no shadows; this is the learned form of the shadow ray of the hair path.
+//
+// Network: (p, d) -> p normalized by the scene box -> 6 dims x 8 frequencies x {sin, cos}
+// (pi 2^k x) = 96 features -> 4 x (64, ReLU) -> logit -> sigmoid = visibility. Parameters,
+// FP32, flat: W1[64][96] b1 W2[64][64] b2 W3 b3 W4 b4 w5[64] b5 (18,753 values).
+//
+// MFMA mapping (v_mfma_f32_16x16x16_bf16): a layer computes D = W X^T for 16 outputs x 16
+// samples. Lane l = 16 g + c holds D[4 g + j][c] (j = 0..3), i.e. outputs 4g..4g+3 of sample
+// c, which is exactly the B operand slice (K rows 4g..4g+3, column c) of the next layer. The
+// forward pass and the back-propagated deltas (dX = W^T dZ) therefore chain from layer to
+// layer in registers. Weight gradients (dW = dZ X^T, K = samples) need the transposed
+// operands and are staged through LDS.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp.hpp"
+
+namespace vk {
+namespace mlp_detail {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kEnc = 96, kHid = 64;
+constexpr int kOffW1 = 0, kOffB1 = 6144, kOffW2 = 6208, kOffB2 = 10304, kOffW3 = 10368, kOffB3 = 14464;
+constexpr int kOffW4 = 14528, kOffB4 = 18624, kOffW5 = 18688, kOffB5 = 18752;
+// packed bf16 fragments: [frag][64 lanes][4]; forward W1 (4x6 frags), W2..W4 (4x4 each), then
+// the transposed W2..W4 (4x4 each) for back-propagation
+constexpr int kFragW1 = 0, kFragW2 = 24, kFragW3 = 40, kFragW4 = 56, kFragT2 = 72, kFragT3 = 88, kFragT4 = 104;
+constexpr int kFragsFwd = 72, kFragsAll = 120;
+constexpr int kSmall = 4 * kHid + kHid + 1;  // biases b1..b4, w5, b5 (FP32)
+
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ float bf16_val(float x) { return __uint_as_float(bf16_bits(x) << 16); }
+__device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
+    s4 v;
+    v.x = short(bf16_bits(a));
+    v.y = short(bf16_bits(b));
+    v.z = short(bf16_bits(c));
+    v.w = short(bf16_bits(d));
+    return v;
+}
+__device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// Source index in the flat parameters of packed element (frag, lane, j).
+__device__ __forceinline__ int packed_source(int frag, int lane, int j) {
+    const int g = lane >> 4, c = lane & 15;
+    if (frag < kFragW2) {  // W1 [m][t]
+        const int m = frag / 6, t = frag % 6;
+        return kOffW1 + (16 * m + c) * kEnc + 16 * t + 4 * g + j;
+    }
+    if (frag < kFragT2) {  // W2..W4 [m][t]
+        const int L = (frag - kFragW2) / 16, q = (frag - kFragW2) % 16, m = q / 4, t = q % 4;
+        const int off = L == 0 ? kOffW2 : (L == 1 ? kOffW3 : kOffW4);
+        return off + (16 * m + c) * kHid + 16 * t + 4 * g + j;
+    }
+    // transposed W2..W4 [t][m]: row = input 16 t + c, k = output 16 m + 4 g + j
+    const int L = (frag - kFragT2) / 16, q = (frag - kFragT2) % 16, t = q / 4, m = q % 4;
+    const int off = L == 0 ? kOffW2 : (L == 1 ? kOffW3 : kOffW4);
+    return off + (16 * m + 4 * g + j) * kHid + 16 * t + c;
+}
+
+__global__ void k_pack(const float* __restrict__ master, uint16_t* __restrict__ packed) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;  // element
+    if (e >= kFragsAll * 256) return;
+    const int frag = e >> 8, lane = (e >> 2) & 63, j = e & 3;
+    packed[e] = uint16_t(bf16_bits(master[packed_source(frag, lane, j)]));
+}
+
+// Feature f = 16 t + 4 g + j of the lane's sample for input dim t: k = 2 g + j / 2,
+// sin for even j, cos for odd j (feature (dim * 8 + k) * 2 + s).
+__device__ __forceinline__ void encode(const float x[6], int g, s4 enc[6]) {
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const float v0 = ldexpf(x[t], 2 * g), v1 = ldexpf(x[t], 2 * g + 1);
+        enc[t] = pack4(sinpif(v0), cospif(v0), sinpif(v1), cospif(v1));
+    }
+}
+
+__device__ __forceinline__ void load_sample(const float4* s, float4 c, float4 ih, float x[6]) {
+    const float4 a = s[0], b = s[1];
+    x[0] = (a.x - c.x) * ih.x;
+    x[1] = (a.y - c.y) * ih.y;
+    x[2] = (a.z - c.z) * ih.z;
+    x[3] = b.x;
+    x[4] = b.y;
+    x[5] = b.z;
+}
+
+// Forward of 16 samples (lane l: sample l & 15); h[L][m] = post-ReLU FP32 outputs 16 m + 4 g + j.
+__device__ __forceinline__ float forward16(const s4* __restrict__ W, const float* __restrict__ small, const s4 enc[6],
+                                           int lane, f4 h[4][4]) {
+    const int g = lane >> 4;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 6; ++t) acc = mfma(W[(kFragW1 + m * 6 + t) * 64 + lane], enc[t], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h[0][m][j] = fmaxf(acc[j] + small[16 * m + 4 * g + j], 0.0f);
+    }
+#pragma unroll
+    for (int L = 1; L < 4; ++L) {
+        s4 xin[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) xin[t] = pack4(h[L - 1][t][0], h[L - 1][t][1], h[L - 1][t][2], h[L - 1][t][3]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc = mfma(W[(kFragW2 + (L - 1) * 16 + m * 4 + t) * 64 + lane], xin[t], acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[L][m][j] = fmaxf(acc[j] + small[kHid * L + 16 * m + 4 * g + j], 0.0f);
+        }
+    }
+    float z = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) z += small[4 * kHid + 16 * m + 4 * g + j] * h[3][m][j];
+    z += __shfl_xor(z, 16);
+    z += __shfl_xor(z, 32);
+    return z + small[4 * kHid + kHid];
+}
+
+__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+// Stages the forward fragments and the FP32 small parameters into LDS (all threads).
+__device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ packed, const float* __restrict__ master,
+                                              s4* lw, float* lsmall, int nfrags) {
+    const s4* src = reinterpret_cast<const s4*>(packed);
+    for (int i = threadIdx.x; i < nfrags * 64; i += blockDim.x) lw[i] = src[i];
+    for (int i = threadIdx.x; i < kSmall; i += blockDim.x) {
+        int s;
+        if (i < 4 * kHid) {
+            const int L = i / kHid, o = i % kHid;
+            s = (L == 0 ? kOffB1 : L == 1 ? kOffB2 : L == 2 ? kOffB3 : kOffB4) + o;
+        } else {
+            s = kOffW5 + (i - 4 * kHid);
+        }
+        lsmall[i] = master[s];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_infer(MlpDev m, const float4* __restrict__ samples, uint32_t n,
+                                               float* __restrict__ out) {
+    __shared__ s4 lw[kFragsFwd * 64];
+    __shared__ float lsmall[kSmall];
+    stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
+    const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
+    for (uint32_t base = wave * 16; base < n; base += nwaves * 16) {
+        const uint32_t s = base + uint32_t(lane & 15);
+        float x[6] = {0, 0, 0, 0, 0, 0};
+        if (s < n) load_sample(samples + size_t(s) * 2, c, ih, x);
+        s4 enc[6];
+        encode(x, lane >> 4, enc);
+        f4 h[4][4];
+        const float z = forward16(lw, lsmall, enc, lane, h);
+        if (s < n && lane < 16) out[s] = sigmoid(z);
+    }
+}
+
+// Render form: for each queued shadow pixel, the MLP's visibility p at (hit point, light
+// direction). Confident predictions (p (1 - p) <= threshold) are written as the pixel's
+// visibility; the rest are appended to the exact-fallback queue.
+__global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
+    __shared__ s4 lw[kFragsFwd * 64];
+    __shared__ float lsmall[kSmall];
+    stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = *sp.count;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
+    const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
+    for (uint32_t base = wave * 16; base < n; base += nwaves * 16) {
+        const uint32_t q = base + uint32_t(lane & 15);
+        float x[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t pix = 0;
+        if (q < n) {
+            pix = sp.queue[q];
+            const float4 o = sp.origins[q];
+            x[0] = (o.x - c.x) * ih.x;
+            x[1] = (o.y - c.y) * ih.y;
+            x[2] = (o.z - c.z) * ih.z;
+            x[3] = sp.dir[0];
+            x[4] = sp.dir[1];
+            x[5] = sp.dir[2];
+        }
+        s4 enc[6];
+        encode(x, lane >> 4, enc);
+        f4 h[4][4];
+        const float p = sigmoid(forward16(lw, lsmall, enc, lane, h));
+        const bool own = q < n && lane < 16;
+        const bool fallback = own && p * (1.0f - p) > sp.threshold;
+        if (own && !fallback) sp.vis[pix] = p;
+        const uint64_t fm = __ballot(fallback);
+        if (fm) {
+            const uint32_t leader = __ffsll((unsigned long long)fm) - 1;
+            uint32_t b = 0;
+            if (lane == int(leader)) b = atomicAdd(sp.fallback_count, uint32_t(__popcll(fm)));
+            b = __shfl(b, int(leader));
+            if (fallback) sp.fallback_queue[b + __popcll(fm & ((1ull << lane) - 1ull))] = pix;
+        }
+    }
+}
+
+// Forward + backward of the batch; each block accumulates its share of the gradient and
+// writes it to partial[block][param]. Wave w of a 64-sample chunk runs 16 samples through the
+// network and back, staging bf16 inputs X_L and deltas dZ_L per layer in LDS; then wave w
+// accumulates dW of layer w + 1 over the chunk's 64 samples with MFMAs (K = samples).
+struct TrainLds {
+    alignas(16) s4 w[kFragsAll * 64];       // 61,440 B
+    alignas(16) float small[kSmall];
+    alignas(16) uint16_t x1[kEnc * 64];     // [feature][sample]
+    alignas(16) uint16_t x[3][kHid * 64];   // h1, h2, h3 as inputs of layers 2..4
+    alignas(16) uint16_t dz[4][kHid * 64];  // [output][sample]
+    alignas(16) float db[4 * kHid + kHid + 1];  // bias, w5 and b5 gradients
+    float loss;
+};
+
+__global__ __launch_bounds__(256) void k_train(MlpDev m, MlpTrainParams tp) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    TrainLds& S = *reinterpret_cast<TrainLds*>(lds_raw);
+    {
+        const s4* src = reinterpret_cast<const s4*>(m.packed);
+        for (int i = threadIdx.x; i < kFragsAll * 64; i += blockDim.x) S.w[i] = src[i];
+        for (int i = threadIdx.x; i < kSmall; i += blockDim.x) {
+            int s;
+            if (i < 4 * kHid) {
+                const int L = i / kHid, o = i % kHid;
+                s = (L == 0 ? kOffB1 : L == 1 ? kOffB2 : L == 2 ? kOffB3 : kOffB4) + o;
+            } else {
+                s = kOffW5 + (i - 4 * kHid);
+            }
+            S.small[i] = m.master[s];
+        }
+        for (int i = threadIdx.x; i < 4 * kHid + kHid + 1; i += blockDim.x) S.db[i] = 0.0f;
+        if (threadIdx.x == 0) S.loss = 0.0f;
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, cs = lane & 15;
+    const int col = 16 * w + cs;  // this lane's sample column within the chunk
+    const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
+    const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
+    const float inv_b = 1.0f / float(tp.n);
+    // dW accumulators of this wave's layer: D[o = 16 mo + 4 g + j][i = 16 ti + cs]
+    f4 acc[24];
+#pragma unroll
+    for (int i = 0; i < 24; ++i) acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t nchunks = (tp.n + 63u) / 64u;
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint32_t s = chunk * 64u + uint32_t(col);
+        const bool valid = s < tp.n;
+        float x[6] = {0, 0, 0, 0, 0, 0};
+        float y = 0.0f;
+        if (valid) {
+            // epoch order: sample (first + s) of the permutation i -> (i mul + add) mod total
+            const uint32_t src = uint32_t(((uint64_t(tp.first) + s) * tp.perm_mul + tp.perm_add) % tp.total);
+            load_sample(tp.samples + size_t(src) * 2, c, ih, x);
+            y = tp.targets[src];
+        }
+        s4 enc[6];
+        encode(x, g, enc);
+        f4 h[4][4];
+        const float z = forward16(S.w, S.small, enc, lane, h);
+        const float p = sigmoid(z);
+        const float dz5 = valid ? (p - y) * inv_b : 0.0f;
+        if (valid && g == 0) {
+            const float lp = z >= 0.0f ? -log1pf(expf(-z)) : z - log1pf(expf(z));
+            const float lq = z >= 0.0f ? -z - log1pf(expf(-z)) : -log1pf(expf(z));
+            atomicAdd(&S.loss, -(y * lp + (1.0f - y) * lq));
+        }
+        // stage inputs: X_1 = features, X_2..4 = bf16(h1..h3)
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            S.x1[(16 * t + 4 * g + 0) * 64 + col] = uint16_t(enc[t].x);
+            S.x1[(16 * t + 4 * g + 1) * 64 + col] = uint16_t(enc[t].y);
+            S.x1[(16 * t + 4 * g + 2) * 64 + col] = uint16_t(enc[t].z);
+            S.x1[(16 * t + 4 * g + 3) * 64 + col] = uint16_t(enc[t].w);
+        }
+#pragma unroll
+        for (int L = 0; L < 3; ++L)
+#pragma unroll
+            for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    S.x[L][(16 * mm + 4 * g + j) * 64 + col] = uint16_t(bf16_bits(h[L][mm][j]));
+        // output layer gradients and the delta entering layer 4
+        f4 dh[4];
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = 16 * mm + 4 * g + j;
+                if (dz5 != 0.0f) atomicAdd(&S.db[4 * kHid + k], dz5 * h[3][mm][j]);
+                dh[mm][j] = dz5 * S.small[4 * kHid + k];
+            }
+        if (g == 0 && dz5 != 0.0f) atomicAdd(&S.db[5 * kHid], dz5);
+        // back-propagation through layers 4..1 (0-based L = 3..0)
+#pragma unroll
+        for (int L = 3; L >= 0; --L) {
+            s4 dzb[4];
+#pragma unroll
+            for (int mm = 0; mm < 4; ++mm) {
+                float d[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    d[j] = bf16_val(h[L][mm][j] > 0.0f ? dh[mm][j] : 0.0f);
+                    const int o = 16 * mm + 4 * g + j;
+                    S.dz[L][o * 64 + col] = uint16_t(bf16_bits(d[j]));
+                    if (d[j] != 0.0f) atomicAdd(&S.db[kHid * L + o], d[j]);
+                }
+                dzb[mm] = pack4(d[0], d[1], d[2], d[3]);
+            }
+            if (L > 0) {
+                const int tb = L == 1 ? kFragT2 : (L == 2 ? kFragT3 : kFragT4);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    f4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int mm = 0; mm < 4; ++mm) a = mfma(S.w[(tb + t * 4 + mm) * 64 + lane], dzb[mm], a);
+                    dh[t] = a;
+                }
+            }
+        }
+        __syncthreads();
+        // dW of layer w + 1 over the chunk: A = dZ tile (rows = outputs), B = X tile (cols = inputs)
+        const uint16_t* dz = S.dz[w];
+        const uint16_t* xs = w == 0 ? S.x1 : S.x[w - 1];
+        const int ntiles = w == 0 ? 6 : 4;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+#pragma unroll
+            for (int mo = 0; mo < 4; ++mo) {
+                const s4 a = *reinterpret_cast<const s4*>(dz + (16 * mo + cs) * 64 + 16 * ks + 4 * g);
+#pragma unroll
+                for (int ti = 0; ti < 6; ++ti) {
+                    if (ti < ntiles) {
+                        const s4 b = *reinterpret_cast<const s4*>(xs + (16 * ti + cs) * 64 + 16 * ks + 4 * g);
+                        acc[mo * 6 + ti] = mfma(a, b, acc[mo * 6 + ti]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // write this block's partial gradient
+    float* part = tp.partial + size_t(blockIdx.x) * kMlpParamCount;
+    const int in_dim = w == 0 ? kEnc : kHid;
+    const int offw = w == 0 ? kOffW1 : (w == 1 ? kOffW2 : (w == 2 ? kOffW3 : kOffW4));
+    const int ntiles = w == 0 ? 6 : 4;
+#pragma unroll
+    for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+        for (int ti = 0; ti < 6; ++ti)
+            if (ti < ntiles)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    part[offw + (16 * mo + 4 * g + j) * in_dim + 16 * ti + cs] = acc[mo * 6 + ti][j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 4 * kHid + kHid + 1; i += blockDim.x) {
+        int d;
+        if (i < 4 * kHid) {
+            const int L = i / kHid, o = i % kHid;
+            d = (L == 0 ? kOffB1 : L == 1 ? kOffB2 : L == 2 ? kOffB3 : kOffB4) + o;
+        } else if (i < 5 * kHid) {
+            d = kOffW5 + (i - 4 * kHid);
+        } else {
+            d = kOffB5;
+        }
+        part[d] = S.db[i];
+    }
+    if (threadIdx.x == 0) atomicAdd(tp.loss, S.loss);
+}
+
+// Sums the block partials; either returns the gradient or applies one Adam step.
+__global__ void k_reduce_adam(MlpDev m, const float* __restrict__ partial, uint32_t nblocks, float* grad_out,
+                              AdamParams ap) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParamCount) return;
+    float g = 0.0f;
+    for (uint32_t b = 0; b < nblocks; ++b) g += partial[size_t(b) * kMlpParamCount + p];
+    if (grad_out) {
+        grad_out[p] = g;
+        return;
+    }
+    const float m1 = ap.beta1 * m.adam_m[p] + (1.0f - ap.beta1) * g;
+    const float m2 = ap.beta2 * m.adam_v[p] + (1.0f - ap.beta2) * g * g;
+    m.adam_m[p] = m1;
+    m.adam_v[p] = m2;
+    const float mh = m1 / ap.bias1, vh = m2 / ap.bias2;
+    m.master[p] -= ap.lr * mh / (sqrtf(vh) + ap.eps);
+}
+
+// Training samples from traced shadow rays: (origin, 0, direction, 0), target 1 = unoccluded.
+__global__ void k_collect(const float4* __restrict__ rays, const HitRec* __restrict__ hits, uint32_t n,
+                          float4* __restrict__ samples, float* __restrict__ targets) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = rays[size_t(i) * 2], b = rays[size_t(i) * 2 + 1];
+    samples[size_t(i) * 2] = make_float4(a.x, a.y, a.z, 0.0f);
+    samples[size_t(i) * 2 + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+    targets[i] = hits[i].prim_id == 0xFFFFFFFFu ? 1.0f : 0.0f;
+}
+
+}  // namespace
+}  // namespace mlp_detail
+
+size_t mlp_train_lds_bytes() { return sizeof(mlp_detail::TrainLds); }
+
+hipError_t launch_mlp_pack(const MlpDev& m, hipStream_t st) {
+    hipLaunchKernelGGL(mlp_detail::k_pack, dim3((mlp_detail::kFragsAll * 256 + 255) / 256), dim3(256), 0, st, m.master,
+                       m.packed);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer(const MlpDev& m, const float* samples, uint32_t n, float* out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t groups = (n + 15u) / 16u;
+    const uint32_t blocks = std::min<uint32_t>((groups + 3u) / 4u, 2048u);
+    hipLaunchKernelGGL(mlp_detail::k_infer, dim3(blocks), dim3(256), 0, st, m,
+                       reinterpret_cast<const float4*>(samples), n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_shadow_gate(const MlpDev& m, const MlpShadowParams& sp, uint32_t max_count, hipStream_t st) {
+    if (max_count == 0) return hipSuccess;
+    const uint32_t groups = (max_count + 15u) / 16u;
+    const uint32_t blocks = std::min<uint32_t>((groups + 3u) / 4u, 2048u);
+    hipLaunchKernelGGL(mlp_detail::k_shadow_gate, dim3(blocks), dim3(256), 0, st, m, sp);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_train_step(const MlpDev& m, const MlpTrainParams& tp, uint32_t nblocks, float* grad_out,
+                                 const AdamParams& ap, hipStream_t st) {
+    hipLaunchKernelGGL(mlp_detail::k_train, dim3(nblocks), dim3(256), mlp_train_lds_bytes(), st, m, tp);
+    hipLaunchKernelGGL(mlp_detail::k_reduce_adam, dim3((kMlpParamCount + 255) / 256), dim3(256), 0, st, m, tp.partial,
+                       nblocks, grad_out, ap);
+    if (!grad_out) return launch_mlp_pack(m, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_collect(const float* rays, const HitRec* hits, uint32_t n, float* samples, float* targets,
+                              hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_detail::k_collect, dim3((n + 255) / 256), dim3(256), 0, st,
+                       reinterpret_cast<const float4*>(rays), hits, n, reinterpret_cast<float4*>(samples), targets);
+    return hipGetLastError();
+}
+
+hipError_t mlp_enable_train_lds() {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(mlp_detail::k_train),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(mlp_train_lds_bytes()));
+}
+
+}  // namespace vk

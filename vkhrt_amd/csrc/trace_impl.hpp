@@ -17,6 +17,7 @@
 
 #include "build.hpp"
 #include "intersect.hpp"
+#include "mlp.hpp"
 
 namespace vk {
 
@@ -1146,19 +1147,31 @@ __global__ void k_render_classify(RenderParams rp) {
 
 // Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
 template <class G, int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParams rp) {
+__global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParams rp, const uint32_t* queue,
+                                                                     const uint32_t* count, uint32_t* work) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    const uint32_t total = rp.counters[0];
-    const SrcShadow src{rp.cam, rp.queue, rp.hitbuf};
+    const uint32_t total = *count;
+    const SrcShadow src{rp.cam, queue, rp.hitbuf};
     persistent_lanes<G, true, STACK>(
-        rp.tp, total, rp.counters + 2, stk + threadIdx.x,
+        rp.tp, total, work, stk + threadIdx.x,
         [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
             id = qi;
             src.get(qi, o, d, tmin, tmax);
         },
         [&](uint32_t qi, f3 o, f3 d, const Best& best) {
-            rp.vis[rp.queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
+            rp.vis[queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
         });
+}
+
+// Shadow-ray origins of the queued pixels (the MLP's positions).
+__global__ void k_shadow_origins(RenderParams rp) {
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= rp.counters[0]) return;
+    const SrcShadow src{rp.cam, rp.queue, rp.hitbuf};
+    f3 o, d;
+    float tmin, tmax;
+    src.get(qi, o, d, tmin, tmax);
+    rp.origins[qi] = make_float4(o.x, o.y, o.z, 0.0f);
 }
 
 // Shades the queued hit pixels with a per-pixel visibility (neural form, or the deferred
@@ -1293,15 +1306,29 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                 if (rp.light_vis) {
                     hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.light_vis);
                 } else {
+                    // exact: every queued pixel traces its shadow ray; neural: the MLP answers the
+                    // confident ones and the rest (fallback queue, counters[5]) trace it
+                    const uint32_t* squeue = rp.queue;
+                    const uint32_t* scount = rp.counters;
                     if (tp.defer) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
                     if (ev) hipEventRecord(ev[2], st);
+                    if (rp.mlp) {
+                        hipLaunchKernelGGL(k_shadow_origins, grid, block, 0, st, rp);
+                        MlpShadowParams sp{rp.queue, rp.counters, rp.origins, {0.0f, 1.0f, 0.0f}, rp.mlp_threshold,
+                                           rp.vis, rp.fallback_queue, rp.counters + 5};
+                        const hipError_t e = launch_mlp_shadow_gate(*rp.mlp, sp, npix, st);
+                        if (e != hipSuccess) return e;
+                        squeue = rp.fallback_queue;
+                        scount = rp.counters + 5;
+                    }
                     hipLaunchKernelGGL((k_render_shadow_defer<G, STACK>),
-                                       persistent_grid(k_render_shadow_defer<G, STACK>), block, 0, st, rp);
+                                       persistent_grid(k_render_shadow_defer<G, STACK>), block, 0, st, rp, squeue,
+                                       scount, rp.counters + 2);
                     if (ev) hipEventRecord(ev[3], st);
                     shadow_timed = true;
                     if (tp.defer)
-                        hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp,
-                                           SrcShadow{rp.cam, rp.queue, rp.hitbuf}, rp.vis);
+                        hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp, SrcShadow{rp.cam, squeue, rp.hitbuf},
+                                           rp.vis);
                     hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.vis);
                 }
             }
