@@ -49,11 +49,14 @@ def main():
     ap.add_argument("--stack-kind", type=int, default=1)
     ap.add_argument("--leaf-size", type=int, default=1)
     ap.add_argument("--persistent", type=int, default=1)
-    ap.add_argument("--refill-min", type=int, default=32)
+    ap.add_argument("--refill-min", type=int, default=16)
     ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-neural", action="store_true", help="skip the neural shadow form")
+    ap.add_argument("--neural-views", type=int, default=32, help="training views for the shadow MLP")
+    ap.add_argument("--neural-epochs", type=int, default=80)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain float32 copy kernel over MIB MiB three times and exit "
                          "(the known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -142,6 +145,10 @@ def main():
     if not args.no_roofline:
         roofline = roofline_block(ctx, tlas, cam, W, H, args.polish, kavg, sp, torch, np, V)
 
+    neural = None
+    if not args.no_neural:
+        neural = neural_block(args, ctx, tlas, cam, W, H, flags, ms, sp, stream, torch, np, V, curves, nseg_total)
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -173,11 +180,128 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "neural": neural,
     }
     if parity is not None:
         out["parity_sample"] = parity
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def neural_block(args, ctx, tlas, cam, W, H, flags, exact_ms, sp, stream, torch, np, V, curves, nseg):
+    """Neural shadow form: the MLP trained on shadow rays the exact tracer labels from
+    `neural_views` cameras around the hair (the bench camera is not among them), then frames
+    rendered with the variance-gated MLP. The threshold is the largest of a fixed list whose
+    image stays at PSNR >= 40 dB against the exact image of the same frame."""
+    cv = curves.view(-1, 4, 4)[:, :, :3].reshape(-1, 3)
+    lo = cv.min(0).values.tolist()
+    hi = cv.max(0).values.tolist()
+    pad = [0.05 * (b - a) + 1.0 for a, b in zip(lo, hi)]
+    mlp = V.ShadowMLP(ctx, [a - p for a, p in zip(lo, pad)], [b + p for b, p in zip(hi, pad)], seed=1)
+    TW, TH = 960, 540
+    n = TW * TH
+    rays = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+    hits = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+    srays = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    samples, targets = [], []
+    t0 = time.perf_counter()
+    for k in range(args.neural_views):
+        a = 2 * math.pi * (k + 0.5) / args.neural_views
+        eye = (320 * math.sin(a), 40.0 + 120 * math.sin(3 * a + 0.7), 320 * math.cos(a))
+        vc = V.camera_look_at(eye, (0.0, 40.0, 0.0), fov_deg=60.0, aspect=TW / TH)
+        ctx.generate_camera_rays(vc, TW, TH, rays.data_ptr(), stream=sp)
+        ctx.trace_rays(tlas, rays.data_ptr(), n, hits.data_ptr(), 0, sp)
+        cnt.zero_()
+        ctx.generate_shadow_rays(rays.data_ptr(), hits.data_ptr(), n, srays.data_ptr(), cnt.data_ptr(), stream=sp)
+        torch.cuda.synchronize()
+        m = int(cnt.item())
+        smp = torch.empty(m * 8, dtype=torch.float32, device="cuda")
+        tgt = torch.empty(m, dtype=torch.float32, device="cuda")
+        mlp.collect(tlas, srays.data_ptr(), m, smp.data_ptr(), tgt.data_ptr(), stream=sp)
+        samples.append(smp)
+        targets.append(tgt)
+    smp = torch.cat(samples).contiguous()
+    tgt = torch.cat(targets).contiguous()
+    torch.cuda.synchronize()
+    collect_s = time.perf_counter() - t0
+    ns = tgt.numel()
+    del rays, hits, srays, samples, targets
+    t0 = time.perf_counter()
+    loss = mlp.train(smp.data_ptr(), tgt.data_ptr(), ns, batch=65536, epochs=args.neural_epochs,
+                     learning_rate=3e-3, stream=sp)
+    train_s = time.perf_counter() - t0
+    train_tflops = 3 * 36992 * ns * args.neural_epochs / train_s / 1e12  # forward + backward ~ 3x forward
+    log(f"[bench] neural: {ns} shadow samples from {args.neural_views} views ({collect_s:.2f} s), "
+        f"{args.neural_epochs} epochs in {train_s:.2f} s, loss {loss:.4f}")
+
+    # exact and neural float images of the bench frame for PSNR
+    fimg_exact = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    ctx.render(tlas, cam, W, H, fimg_exact.data_ptr(), flags=flags & ~V.RENDER_RGBA8, stream=sp)
+    torch.cuda.synchronize()
+    exact_img = fimg_exact.view(H, W, 4)[..., :3]
+    fimg = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+
+    def psnr_at(thr):
+        ctx.render(tlas, cam, W, H, fimg.data_ptr(), flags=(flags & ~V.RENDER_RGBA8) | V.RENDER_NEURAL_SHADOWS,
+                   shadow_mlp=mlp, variance_threshold=thr, stream=sp)
+        torch.cuda.synchronize()
+        mse = float(((fimg.view(H, W, 4)[..., :3] - exact_img) ** 2).mean())
+        _, q, tr = ctx.render_stats()
+        return (float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)), q, tr
+
+    sweep = []
+    chosen = None
+    for thr in (0.25, 0.2, 0.15, 0.1, 0.05, 0.02, 0.0):
+        p, q, tr = psnr_at(thr)
+        sweep.append({"threshold": thr, "psnr_db": round(p, 2) if math.isfinite(p) else None,
+                      "fallback_fraction": round(tr / max(q, 1), 4)})
+        if p >= 40.0 and chosen is None:
+            chosen = (thr, p, q, tr)
+    thr, p, q, tr = chosen
+    image = torch.empty(W * H, dtype=torch.int32, device="cuda")
+    nflags = flags | V.RENDER_NEURAL_SHADOWS
+
+    def frame():
+        ctx.render(tlas, cam, W, H, image.data_ptr(), flags=nflags, shadow_mlp=mlp, variance_threshold=thr, stream=sp)
+
+    for _ in range(args.warmup):
+        frame()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        frame()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    prim, queries, traced = ctx.render_stats()
+    ctx.profiling(True)
+    for _ in range(args.steps):
+        frame()
+    torch.cuda.synchronize()
+    shadow_stage = ctx.kernel_stats("render_shadow")
+    ctx.profiling(False)
+    stage_ms = shadow_stage[0] / max(shadow_stage[1], 1)
+    log(f"[bench] neural: threshold {thr} PSNR {p:.2f} dB, fallback {traced}/{queries}, {ms:.3f} ms/frame "
+        f"(exact {exact_ms:.3f}), shadow stage {stage_ms:.3f} ms")
+    mlp.close()
+    return {
+        "ms_per_step": round(ms, 4),
+        "value": round((prim + queries) / (ms * 1e-3) / 1e6, 3),
+        "unit": "Mrays/s (primary rays + shadow queries)",
+        "frame_speedup_vs_exact": round(exact_ms / ms, 3),
+        "psnr_db_vs_exact": round(p, 2) if math.isfinite(p) else None,
+        "variance_threshold": thr,
+        "shadow_queries": queries,
+        "shadow_rays_traced": traced,
+        "shadow_stage_ms": round(stage_ms, 4),
+        "threshold_sweep": sweep,
+        "train": {"samples": ns, "views": args.neural_views, "epochs": args.neural_epochs, "batch": 65536,
+                  "seconds": round(train_s, 3), "collect_seconds": round(collect_s, 3),
+                  "final_loss": round(loss, 5), "approx_tflops": round(train_tflops, 2)},
+        "held_out": "bench camera is not among the training views",
+    }
 
 
 def copy_calibration(nbytes):

@@ -169,7 +169,7 @@ int vkhrt_abi_version(void);
 #define VKHRT_OPTION_PERSISTENT 3 /* 1 (default): persistent wavefronts with per-lane ray refill;
                                      0: one thread per ray */
 #define VKHRT_OPTION_REFILL_MIN 4 /* persistent: idle lanes (1..64) a wavefront waits for before
-                                     fetching new rays together (default 32) */
+                                     fetching new rays together (default 16) */
 #define VKHRT_OPTION_RAY_ORDER 5   /* vkhrt_render work order: 0 row-major (default), 1 8x8 pixel
                                      tiles, one tile per wavefront */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);

@@ -229,7 +229,7 @@ def test_persistent_and_plain_kernels_agree(gpu_ctx, G, oracle, curves, persiste
         h = G.trace(gpu_ctx, tlas, rays)
     finally:
         gpu_ctx.set_option(V.OPTION_PERSISTENT, 1)
-        gpu_ctx.set_option(V.OPTION_REFILL_MIN, 32)
+        gpu_ctx.set_option(V.OPTION_REFILL_MIN, 16)
     _compare_hits(h, ref, f"persistent={persistent}")
 
 

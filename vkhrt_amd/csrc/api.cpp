@@ -33,7 +33,7 @@ struct vkhrt_context {
     int stack_kind = 1;
     int leaf_size = 1;
     bool persistent = true;
-    uint32_t refill_min = 32;
+    uint32_t refill_min = 16;
     uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles
     bool profiling = false;
     std::vector<PendingTiming> pending;
@@ -49,6 +49,7 @@ struct vkhrt_context {
     hipStream_t last_stream = nullptr;
     vk::HitRec* hitrec = nullptr;  // per pixel (deferred-polish render)
     float* vis = nullptr;          // per pixel
+    uint32_t* live = nullptr;      // per pixel: primary rays that enter an instance box
     // deferred FP64 polish
     uint4* defer = nullptr;
     vk::HitRec* cand_hits = nullptr;
@@ -263,6 +264,8 @@ int ensure_render_scratch(vkhrt_context* c, size_t npix) {
     hipFree(c->queue);
     hipFree(c->hitrec);
     hipFree(c->vis);
+    hipFree(c->live);
+    c->live = nullptr;
     c->hitbuf = nullptr;
     c->queue = nullptr;
     c->hitrec = nullptr;
@@ -272,6 +275,7 @@ int ensure_render_scratch(vkhrt_context* c, size_t npix) {
     if (e == hipSuccess) e = hipMalloc(&c->queue, npix * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&c->hitrec, npix * sizeof(vk::HitRec));
     if (e == hipSuccess) e = hipMalloc(&c->vis, npix * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&c->live, npix * sizeof(uint32_t));
     if (e != hipSuccess) return hip_fail(c, e, "render scratch");
     c->cap_pixels = npix;
     return VKHRT_SUCCESS;
@@ -381,6 +385,7 @@ void vkhrt_context_destroy(vkhrt_context* c) {
     hipFree(c->queue);
     hipFree(c->hitrec);
     hipFree(c->vis);
+    hipFree(c->live);
     hipFree(c->defer);
     hipFree(c->cand_hits);
     hipFree(c->cand_keys);
@@ -719,6 +724,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     rp.light_vis = const_cast<float*>(d->light_visibility);
     rp.hitrec = c->hitrec;
     rp.vis = c->vis;
+    rp.live = c->live;
     if (d->flags & VKHRT_RENDER_FP64_POLISH) {
         if (!t->polish_ok || t->variant == vk::kVarMixed)
             return fail(c, VKHRT_ERROR_UNSUPPORTED,

@@ -70,6 +70,8 @@ struct RenderParams {
     float mlp_threshold;       // exact fallback when p (1 - p) > threshold
     float4* origins;           // shadow-ray origin per queue entry
     uint32_t* fallback_queue;  // pixels left to the exact shadow ray ([5] of counters = length)
+    uint32_t* live;            // persistent pipeline: pixels whose primary ray enters an instance box
+                               // ([6] of counters = length); the rest are recorded as misses up front
 };
 
 // Compile-time geometry variants of the traversal kernels. A TLAS whose instances share

@@ -667,7 +667,7 @@ struct CWalker {
     uint32_t oct;
     uint2 cur;  // current group: x = child_base, y = imask | remaining hit bits (octant order) << 8
     uint32_t node, pbase, pmask;
-    int sp;
+    int sp, sb;  // the stack is a deque: push / pop at sp, other lanes take entries at sb
     bool have_node;
 
     __device__ __forceinline__ void begin(f3 o_, f3 d_) {
@@ -679,8 +679,22 @@ struct CWalker {
         node = 0;
         pbase = 0;
         pmask = 0;
-        sp = 0;
+        sp = sb = 0;
         have_node = true;
+    }
+
+    // Continues another lane's ray from one of its stacked node groups.
+    __device__ __forceinline__ void adopt(f3 o_, f3 d_, f3 inv_, uint32_t oct_, uint2 group) {
+        o = o_;
+        d = d_;
+        inv = inv_;
+        oct = oct_;
+        cur = group;
+        node = 0;
+        pbase = 0;
+        pmask = 0;
+        sp = sb = 0;
+        have_node = false;
     }
 
     // One step: one pending leaf record through the early-out, or one node test; then, when
@@ -714,7 +728,7 @@ struct CWalker {
         }
         if (pmask || have_node) return kStepMore;
         if ((cur.y >> 8) == 0) {
-            if (sp == 0) return kStepDone;
+            if (sp == sb) return kStepDone;
             --sp;
             cur = lds[sp * kTraceBlock];
         }
@@ -739,7 +753,11 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
     __shared__ float4 s_pay[kTraceBlock];              // its u and ray-centric normal vector
     __shared__ uint32_t s_pol[kTraceBlock];            // its polished flag
     __shared__ uint32_t s_cnt[kTraceBlock];            // pairs of the lane's ray still queued
-    enum { kIdle = 0, kInst = 1, kTrav = 2, kWait = 3 };
+    __shared__ uint32_t s_walk[kTraceBlock];           // other lanes walking subtrees of the lane's ray
+    __shared__ uint32_t s_xfer[kTraceBlock];           // donor lane by rank (subtree hand-over)
+    // kTrav: walking its own ray; kHelp: walking a subtree of the ray of lane `home`; kWait: own
+    // walk done, waiting for queued pairs and helpers of its ray
+    enum { kIdle = 0, kInst = 1, kTrav = 2, kWait = 3, kHelp = 4 };
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wbase = threadIdx.x & ~63u;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -748,7 +766,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
     W.o = W.d = f3{0.0f, 0.0f, 0.0f};
     f3 ow = f3{0, 0, 0}, dw = f3{0, 0, 0};
     float tmin = 0.0f, tmax = 0.0f, tcur = 0.0f;
-    uint32_t idx = 0, inst = 0;
+    uint32_t idx = 0, inst = 0, home = lane;
     int state = kIdle;
     bool ray_done = false;
     bool exhausted = false;  // wave-uniform
@@ -761,7 +779,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
             break;
         }
         // (a) a waiting lane whose pairs are all merged moves to its next instance or finishes
-        if (state == kWait && s_cnt[threadIdx.x] == 0u) {
+        if (state == kWait && s_cnt[threadIdx.x] == 0u && s_walk[threadIdx.x] == 0u) {
             if (ray_done) {
                 state = kInst;
                 inst = tp.ninst;  // finish below
@@ -789,6 +807,8 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
                             (static_cast<unsigned long long>(__float_as_uint(tmax)) << 32) | kNoHitLow;
                         s_pol[threadIdx.x] = 0u;
                         s_cnt[threadIdx.x] = 0u;
+                        s_walk[threadIdx.x] = 0u;
+                        home = lane;
                         inst = 0;
                         ray_done = false;
                         state = kInst;
@@ -837,6 +857,42 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
                 state = kTrav;
             }
         }
+        // (c') once the counter is exhausted, idle lanes take the bottom (largest) stacked node
+        // group of a walking lane and continue that ray from there, so the last long rays of
+        // the frame are walked by many lanes instead of one
+        if (exhausted) {
+            const bool give = (state == kTrav || state == kHelp) && W.sp > W.sb;
+            const uint64_t dm = __ballot(give);
+            const uint64_t im = __ballot(state == kIdle);
+            if (dm && im) {
+                const uint32_t n = min(uint32_t(__popcll(dm)), uint32_t(__popcll(im)));
+                const uint32_t kd = __popcll(dm & lt), ki = __popcll(im & lt);
+                if (give && kd < n) s_xfer[wbase + kd] = lane;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                const bool take = state == kIdle && ki < n;
+                const int from = take ? int(s_xfer[wbase + ki]) : int(lane);
+                const f3 go = f3{__shfl(W.o.x, from), __shfl(W.o.y, from), __shfl(W.o.z, from)};
+                const f3 gd = f3{__shfl(W.d.x, from), __shfl(W.d.y, from), __shfl(W.d.z, from)};
+                const f3 gi = f3{__shfl(W.inv.x, from), __shfl(W.inv.y, from), __shfl(W.inv.z, from)};
+                const uint32_t goct = __shfl(W.oct, from), gsb = uint32_t(__shfl(W.sb, from));
+                const uint32_t ghome = __shfl(home, from), ginst = __shfl(inst, from);
+                const float gtmin = __shfl(tmin, from);
+                if (take) {
+                    const uint2 grp = lds[(from - int(lane)) + int(gsb) * kTraceBlock];
+                    W.adopt(go, gd, gi, goct, grp);
+                    home = ghome;
+                    inst = ginst;
+                    tmin = gtmin;
+                    tcur = __uint_as_float(uint32_t(s_key[wbase + home] >> 32));
+                    atomicAdd(&s_walk[wbase + home], 1u);
+                    state = kHelp;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (give && kd < n) ++W.sb;  // the group now belongs to the helper
+            }
+        }
         // lanes that finished in (c) take new rays in the next iteration: leave only when the
         // counter is exhausted and no lane holds a ray (then no pair is queued either)
         if (!__ballot(state != kIdle)) {
@@ -844,7 +900,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
             continue;
         }
         // (d) intersection round on up to 64 queued pairs, one per lane
-        if (qn >= 64u || (qn != 0u && !__ballot(state == kTrav))) {
+        if (qn >= 64u || (qn != 0u && !__ballot(state == kTrav || state == kHelp))) {
             const uint32_t m = qn < 64u ? qn : 64u;
             const uint32_t e = lane < m ? queue[qn - m + lane] : 0u;
             const int owner = int(e >> kCompactSlotBits);
@@ -892,9 +948,9 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
             __builtin_amdgcn_wave_barrier();
             qn -= m;
             if (state != kIdle) {
-                const unsigned long long kk = s_key[threadIdx.x];
+                const unsigned long long kk = s_key[wbase + home];
                 tcur = __uint_as_float(uint32_t(kk >> 32));
-                if (ANY && uint32_t(kk) != kNoHitLow && !ray_done) {
+                if (ANY && state != kHelp && uint32_t(kk) != kNoHitLow && !ray_done) {
                     ray_done = true;
                     state = kWait;
                 }
@@ -903,10 +959,18 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
         }
         // (e) one traversal step per traversing lane; survivors of the early-out are queued
         uint32_t cand = 0xFFFFFFFFu;
-        if (state == kTrav) {
+        if (state == kHelp && ANY && uint32_t(s_key[wbase + home]) != kNoHitLow) {
+            atomicSub(&s_walk[wbase + home], 1u);  // the ray is occluded already
+            state = kIdle;
+        } else if (state == kTrav || state == kHelp) {
             const int r = W.step(tp.inst[inst].blas, tmin, tcur, lds, cand);
-            if (r == kStepError) {
-                atomicAdd(&tp.status[0], 1u);
+            if (r == kStepError) atomicAdd(&tp.status[0], 1u);
+            if (state == kHelp) {
+                if (r == kStepError || r == kStepDone) {
+                    atomicSub(&s_walk[wbase + home], 1u);
+                    state = kIdle;
+                }
+            } else if (r == kStepError) {
                 ray_done = true;
                 state = kWait;
             } else if (r == kStepDone) {
@@ -916,8 +980,8 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
         const bool app = cand != 0xFFFFFFFFu;
         const uint64_t am = __ballot(app);
         if (app) {
-            queue[qn + __popcll(am & lt)] = (lane << kCompactSlotBits) | cand;
-            atomicAdd(&s_cnt[threadIdx.x], 1u);
+            queue[qn + __popcll(am & lt)] = (home << kCompactSlotBits) | cand;
+            atomicAdd(&s_cnt[wbase + home], 1u);
         }
         qn += __popcll(am);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -1080,27 +1144,56 @@ __global__ void k_polish_shadow(TraceParams tp, SrcShadow src, float* __restrict
 
 // Persistent render, pass 1: primary rays (FP32) -> per-pixel hit record (and the merge key
 // when grazing roots are deferred to the FP64 polish).
+// Persistent render, pass 0: primary rays that miss every instance box are recorded as misses
+// here; the others are appended (in work order within each wavefront) to the live queue, so
+// the persistent lanes only take rays that enter the scene.
+__global__ void k_render_live(RenderParams rp) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t x = 0, y = 0;
+    const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
+    const uint32_t pix = y * rp.cam.width + x;
+    bool live = false;
+    if (valid) {
+        f3 o, d;
+        primary_ray(rp.cam, x, y, o, d);
+        for (uint32_t i = 0; i < rp.tp.ninst && !live; ++i) {
+            const InstanceDev& I = rp.tp.inst[i];
+            live = I.blas.nprims != 0 && ray_box(o, d, I.lo, I.hi, 0.001f, 10000.0f);
+        }
+        if (!live) {
+            HitRec h;
+            h.t = 0.0f;
+            h.prim_id = h.inst_id = 0xFFFFFFFFu;
+            h.u = h.v = 0.0f;
+            h.n[0] = h.n[1] = h.n[2] = 0.0f;
+            rp.hitrec[pix] = h;
+            if (rp.tp.keys) rp.tp.keys[pix] = (static_cast<unsigned long long>(__float_as_uint(10000.0f)) << 32) | 0xFFFFFFFFull;
+        }
+    }
+    const uint64_t m = __ballot(live);
+    if (m) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&rp.counters[6], uint32_t(__popcll(m)));
+        base = __shfl(base, int(leader));
+        if (live) rp.live[base + __popcll(m & ((1ull << lane) - 1ull))] = pix;
+    }
+}
+
 template <class G, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderParams rp) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     persistent_lanes<G, false, STACK>(
-        rp.tp, rp.cam.work, rp.counters + 1, stk + threadIdx.x,
+        rp.tp, rp.counters[6], rp.counters + 1, stk + threadIdx.x,
         [&](uint32_t r, uint32_t& pix, f3& o, f3& d, float& tmin, float& tmax) {
-            uint32_t x, y;
-            if (work_pixel(rp.cam, r, x, y)) {
-                pix = y * rp.cam.width + x;
-                primary_ray(rp.cam, x, y, o, d);
-                tmin = 0.001f;
-                tmax = 10000.0f;
-            } else {  // tile slot outside the image: an empty interval, no result
-                pix = 0xFFFFFFFFu;
-                o = d = f3{0.0f, 0.0f, 1.0f};
-                tmin = 1.0f;
-                tmax = 0.0f;
-            }
+            pix = rp.live[r];
+            const uint32_t y = pix < (1u << 24) ? udiv_small(pix, rp.cam.width, rp.cam.inv_w) : pix / rp.cam.width;
+            primary_ray(rp.cam, pix - y * rp.cam.width, y, o, d);
+            tmin = 0.001f;
+            tmax = 10000.0f;
         },
         [&](uint32_t pix, f3 o, f3 d, const Best& best) {
-            if (pix == 0xFFFFFFFFu) return;
             HitRec h;
             finish_hit(rp.tp, o, d, best, h);
             rp.hitrec[pix] = h;
@@ -1289,6 +1382,7 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
             // shading, the shadow queue and the optional deferred FP64 polish run as passes.
             const TraceParams& tp = rp.tp;
             const dim3 pgrid(1024);
+            hipLaunchKernelGGL(k_render_live, dim3((rp.cam.work + kTraceBlock - 1) / kTraceBlock), block, 0, st, rp);
             // the events bracket the traversal kernels alone (the per-kernel timings)
             if (ev) hipEventRecord(ev[0], st);
             hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>),
