@@ -177,8 +177,11 @@ __global__ __launch_bounds__(256) void k_infer(MlpDev m, const float4* __restric
 __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
     __shared__ s4 lw[kFragsFwd * 64];
     __shared__ float lsmall[kSmall];
+    __shared__ uint32_t s_fb[4][128];  // per-wave fallback pixels, flushed 64+ at a time
     stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
     const int lane = threadIdx.x & 63;
+    uint32_t* fb = s_fb[threadIdx.x >> 6];
+    uint32_t fbn = 0;  // wave-uniform
     const uint32_t n = *sp.count;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
     const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
@@ -205,13 +208,21 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
         const bool fallback = own && p * (1.0f - p) > sp.threshold;
         if (own && !fallback) sp.vis[pix] = p;
         const uint64_t fm = __ballot(fallback);
-        if (fm) {
-            const uint32_t leader = __ffsll((unsigned long long)fm) - 1;
+        if (fallback) fb[fbn + __popcll(fm & ((1ull << lane) - 1ull))] = pix;
+        fbn += uint32_t(__popcll(fm));  // <= 16 per pass: at most 63 + 16 < 128 before a flush
+        if (fbn >= 64u) {
             uint32_t b = 0;
-            if (lane == int(leader)) b = atomicAdd(sp.fallback_count, uint32_t(__popcll(fm)));
-            b = __shfl(b, int(leader));
-            if (fallback) sp.fallback_queue[b + __popcll(fm & ((1ull << lane) - 1ull))] = pix;
+            if (lane == 0) b = atomicAdd(sp.fallback_count, fbn);
+            b = __shfl(b, 0);
+            for (uint32_t i = uint32_t(lane); i < fbn; i += 64u) sp.fallback_queue[b + i] = fb[i];
+            fbn = 0;
         }
+    }
+    if (fbn) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(sp.fallback_count, fbn);
+        b = __shfl(b, 0);
+        for (uint32_t i = uint32_t(lane); i < fbn; i += 64u) sp.fallback_queue[b + i] = fb[i];
     }
 }
 
@@ -436,8 +447,13 @@ hipError_t launch_mlp_infer(const MlpDev& m, const float* samples, uint32_t n, f
 
 hipError_t launch_mlp_shadow_gate(const MlpDev& m, const MlpShadowParams& sp, uint32_t max_count, hipStream_t st) {
     if (max_count == 0) return hipSuccess;
+    // one block per CU: each wavefront walks many 16-sample groups, so its fallback buffer
+    // reaches the queue with few atomics
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const uint32_t groups = (max_count + 15u) / 16u;
-    const uint32_t blocks = std::min<uint32_t>((groups + 3u) / 4u, 2048u);
+    const uint32_t blocks = std::max(1u, std::min<uint32_t>((groups + 3u) / 4u, uint32_t(std::max(cus, 1))));
     hipLaunchKernelGGL(mlp_detail::k_shadow_gate, dim3(blocks), dim3(256), 0, st, m, sp);
     return hipGetLastError();
 }

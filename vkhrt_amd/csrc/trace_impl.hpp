@@ -1144,10 +1144,37 @@ __global__ void k_polish_shadow(TraceParams tp, SrcShadow src, float* __restrict
 
 // Persistent render, pass 1: primary rays (FP32) -> per-pixel hit record (and the merge key
 // when grazing roots are deferred to the FP64 polish).
+// Ordered append from a whole block with one atomic: every thread whose `flag` is set gets a
+// slot; slots follow thread order within the block (one counter update per block instead of
+// one per wavefront keeps the single queue counter from serializing large grids). Called by
+// all threads of the block.
+constexpr int kAppendBlock = 1024;
+__device__ __forceinline__ uint32_t block_append(bool flag, uint32_t* counter) {
+    __shared__ uint32_t s_wave[kAppendBlock / 64 + 1];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(flag);
+    if (lane == 0) s_wave[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t nw = (blockDim.x + 63u) / 64u;
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t c = s_wave[w];
+            s_wave[w] = sum;
+            sum += c;
+        }
+        s_wave[kAppendBlock / 64] = sum ? atomicAdd(counter, sum) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_wave[kAppendBlock / 64] + s_wave[wave] + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+    __syncthreads();
+    return slot;
+}
+
 // Persistent render, pass 0: primary rays that miss every instance box are recorded as misses
 // here; the others are appended (in work order within each wavefront) to the live queue, so
 // the persistent lanes only take rays that enter the scene.
-__global__ void k_render_live(RenderParams rp) {
+__global__ __launch_bounds__(kAppendBlock) void k_render_live(RenderParams rp) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t x = 0, y = 0;
     const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
@@ -1170,15 +1197,8 @@ __global__ void k_render_live(RenderParams rp) {
             if (rp.tp.keys) rp.tp.keys[pix] = (static_cast<unsigned long long>(__float_as_uint(10000.0f)) << 32) | 0xFFFFFFFFull;
         }
     }
-    const uint64_t m = __ballot(live);
-    if (m) {
-        const uint32_t lane = threadIdx.x & 63u;
-        const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&rp.counters[6], uint32_t(__popcll(m)));
-        base = __shfl(base, int(leader));
-        if (live) rp.live[base + __popcll(m & ((1ull << lane) - 1ull))] = pix;
-    }
+    const uint32_t slot = block_append(live, &rp.counters[6]);
+    if (live) rp.live[slot] = pix;
 }
 
 template <class G, int STACK>
@@ -1204,7 +1224,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderPara
 // Pass 2 (after the optional polish merge): shade misses / palette / unshadowed pixels, queue
 // the rest for shadow rays.
 // Walks the pixels in the primary pass's work order, so the shadow queue keeps its coherence.
-__global__ void k_render_classify(RenderParams rp) {
+__global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams rp) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t x = 0, y = 0;
     const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
@@ -1227,15 +1247,8 @@ __global__ void k_render_classify(RenderParams rp) {
             rp.hitbuf[pix] = make_float4(n.x, n.y, n.z, h.t);
         }
     }
-    const uint64_t m = __ballot(hit);
-    if (m) {
-        const uint32_t lane = threadIdx.x & 63u;
-        const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&rp.counters[0], uint32_t(__popcll(m)));
-        base = __shfl(base, int(leader));
-        if (hit) rp.queue[base + __popcll(m & ((1ull << lane) - 1ull))] = pix;
-    }
+    const uint32_t slot = block_append(hit, &rp.counters[0]);
+    if (hit) rp.queue[slot] = pix;
 }
 
 // Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
@@ -1382,7 +1395,8 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
             // shading, the shadow queue and the optional deferred FP64 polish run as passes.
             const TraceParams& tp = rp.tp;
             const dim3 pgrid(1024);
-            hipLaunchKernelGGL(k_render_live, dim3((rp.cam.work + kTraceBlock - 1) / kTraceBlock), block, 0, st, rp);
+            hipLaunchKernelGGL(k_render_live, dim3((rp.cam.work + kAppendBlock - 1) / kAppendBlock), dim3(kAppendBlock),
+                               0, st, rp);
             // the events bracket the traversal kernels alone (the per-kernel timings)
             if (ev) hipEventRecord(ev[0], st);
             hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>),
@@ -1393,8 +1407,8 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                    tp.cand_hits, tp.cand_keys);
                 hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
             }
-            hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kTraceBlock - 1) / kTraceBlock), block, 0, st,
-                               rp);
+            hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kAppendBlock - 1) / kAppendBlock),
+                               dim3(kAppendBlock), 0, st, rp);
             bool shadow_timed = false;
             if (rp.shadows && !rp.debug_palette) {
                 if (rp.light_vis) {
