@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural shadow form")
+    ap.add_argument("--no-modes", action="store_true", help="skip the LSS and DOTS forms of the frame")
     ap.add_argument("--neural-views", type=int, default=32, help="training views for the shadow MLP")
     ap.add_argument("--neural-epochs", type=int, default=80)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -145,6 +146,10 @@ def main():
     if not args.no_roofline:
         roofline = roofline_block(ctx, tlas, cam, W, H, args.polish, kavg, sp, torch, np, V)
 
+    modes = None
+    if not args.no_modes:
+        modes = geometry_modes_block(args, ctx, tlas, cam, W, H, ms, sp, stream, torch, V, curves, nseg_total)
+
     neural = None
     if not args.no_neural:
         neural = neural_block(args, ctx, tlas, cam, W, H, flags, ms, sp, stream, torch, np, V, curves, nseg_total)
@@ -181,6 +186,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "neural": neural,
+        "geometry_modes": modes,
     }
     if parity is not None:
         out["parity_sample"] = parity
@@ -302,6 +308,74 @@ def neural_block(args, ctx, tlas, cam, W, H, flags, exact_ms, sp, stream, torch,
                   "final_loss": round(loss, 5), "approx_tflops": round(train_tflops, 2)},
         "held_out": "bench camera is not among the training views",
     }
+
+
+def geometry_modes_block(args, ctx, tlas, cam, W, H, phantom_ms, sp, stream, torch, V, curves, nseg):
+    """The same hair and frame in the other two geometry modes: every cubic segment split into
+    4 linear pieces (12.8M at 3.2M segments), traced as Linear Swept Spheres (one capsule per
+    piece) and as DOTS (2 orthogonal quads = 4 triangles per piece). Reported per mode: build
+    time, frame time and Mrays/s of the exact primary + shadow frame, and the PSNR of its float
+    image against the phantom image of the same frame (the image error the mode trades)."""
+    pieces = nseg * 4
+    pl = torch.empty(pieces * 8, dtype=torch.float32, device="cuda")
+    ctx.curves_to_lines(curves.data_ptr(), nseg, 4, pl.data_ptr(), stream=sp)
+    fimg = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    ctx.render(tlas, cam, W, H, fimg.data_ptr(), flags=V.RENDER_SHADOWS | V.RENDER_FP64_POLISH, stream=sp)
+    torch.cuda.synchronize()
+    phantom_img = fimg.view(H, W, 4)[..., :3].clone()
+    image = torch.empty(W * H, dtype=torch.int32, device="cuda")
+    out = {}
+    for mode, name in ((V.GEOMETRY_LSS, "lss"), (V.GEOMETRY_DOTS, "dots")):
+        if mode == V.GEOMETRY_LSS:
+            prims = torch.empty(pieces * 8, dtype=torch.float32, device="cuda")
+            ctx.generate_lss(pl.data_ptr(), pieces, prims.data_ptr(), stream=sp)
+        else:
+            prims = pl  # the DOTS BLAS takes the lines and builds its quads on the device
+        blas = V.BottomLevelAccelerationStructure(ctx, mode, prims.data_ptr(), pieces,
+                                                  leaf_size=args.leaf_size, stream=sp)
+        mtlas = V.TopLevelAccelerationStructure(ctx, [blas], stream=sp)
+        torch.cuda.synchronize()
+        info = blas.info()
+
+        def frame():
+            ctx.render(mtlas, cam, W, H, image.data_ptr(), flags=V.RENDER_SHADOWS | V.RENDER_RGBA8, stream=sp)
+
+        for _ in range(args.warmup):
+            frame()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            frame()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        ctx.synchronize(sp)  # raises on traversal stack overflow
+        n_primary, n_shadow = ctx.render_ray_counts()
+        fimg.zero_()
+        ctx.render(mtlas, cam, W, H, fimg.data_ptr(), flags=V.RENDER_SHADOWS, stream=sp)
+        torch.cuda.synchronize()
+        mse = float(((fimg.view(H, W, 4)[..., :3] - phantom_img) ** 2).mean())
+        psnr = float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)
+        mrays = (n_primary + n_shadow) / (ms * 1e-3) / 1e6
+        log(f"[bench] {name}: {pieces} pieces, build {info['build_ms']:.1f} ms, depth {info['depth']}, "
+            f"{ms:.3f} ms/frame, {mrays:.1f} Mrays/s, PSNR vs phantom {psnr:.2f} dB")
+        out[name] = {
+            "pieces": pieces,
+            "triangles": pieces * 4 if mode == V.GEOMETRY_DOTS else None,
+            "ms_per_step": round(ms, 4),
+            "value": round(mrays, 3),
+            "unit": "Mrays/s",
+            "frame_time_vs_phantom": round(ms / phantom_ms, 3),
+            "primary_rays": n_primary, "shadow_rays": n_shadow,
+            "psnr_db_vs_phantom": round(psnr, 2) if math.isfinite(psnr) else None,
+            "bvh": {"nodes": info["node_count"], "depth": info["depth"], "build_ms": round(info["build_ms"], 3),
+                    "device_bytes": info["device_bytes"]},
+        }
+        mtlas.close()
+        blas.close()
+        del prims
+    return out
 
 
 def copy_calibration(nbytes):

@@ -196,10 +196,12 @@ class Context:
         if rc:
             raise VkhrtError(rc, "context_create")
         self.h = h
+        # bound now: module globals may already be cleared when __del__ runs at exit
+        self._destroy = lib().vkhrt_context_destroy
 
     def close(self):
         if getattr(self, "h", None):
-            lib().vkhrt_context_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -328,6 +330,7 @@ class BottomLevelAccelerationStructure:
         h = ctypes.c_void_p()
         ctx.check(lib().vkhrt_blas_build(ctx.h, ctypes.byref(d), stream, ctypes.byref(h)), "blas_build")
         self.ctx, self.h, self.mode = ctx, h, mode
+        self._destroy = lib().vkhrt_blas_destroy
 
     def info(self):
         i = BlasInfo()
@@ -336,7 +339,7 @@ class BottomLevelAccelerationStructure:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().vkhrt_blas_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -351,10 +354,11 @@ class TopLevelAccelerationStructure:
         h = ctypes.c_void_p()
         ctx.check(lib().vkhrt_tlas_build(ctx.h, arr, len(blases), stream, ctypes.byref(h)), "tlas_build")
         self.ctx, self.h, self.blases = ctx, h, list(blases)
+        self._destroy = lib().vkhrt_tlas_destroy
 
     def close(self):
         if getattr(self, "h", None):
-            lib().vkhrt_tlas_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -370,10 +374,11 @@ class ShadowMLP:
         h = ctypes.c_void_p()
         ctx.check(lib().vkhrt_shadow_mlp_create(ctx.h, ctypes.byref(d), ctypes.byref(h)), "shadow_mlp_create")
         self.h = h
+        self._destroy = lib().vkhrt_shadow_mlp_destroy
 
     def close(self):
         if getattr(self, "h", None):
-            lib().vkhrt_shadow_mlp_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
