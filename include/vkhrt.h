@@ -252,6 +252,57 @@ typedef struct vkhrt_render_stats {
 } vkhrt_render_stats;
 int vkhrt_render_stats_get(vkhrt_context* ctx, vkhrt_render_stats* out);
 
+/* ---- batched path tracing (extension; no reference counterpart: the reference renders one
+ * view per traceRaysKHR, source/renderer.cpp:156-166)
+ * Every pixel of every view is one path through the TLAS; all views go through the same
+ * launches (one persistent traversal launch per depth for the whole batch, over a global
+ * path queue). Path rule: light from (0, 1, 0) with radiance 1; at a hit with normal n
+ * (facing normal nf = n turned against the ray), albedo a = material_albedo[instance
+ * material] * Shade(n) (shading.glsl:1-10 with full light); light contribution
+ * T * a * (nf . (0,1,0)) when positive and its shadow ray (origin = hit, tmin 0.001) is
+ * unoccluded; up to `bounces` cosine-distributed bounce rays about nf with throughput
+ * T * a; a ray that leaves the scene adds T * environment. Random numbers are a fixed
+ * function of (seed, path, depth), so a frame is deterministic. */
+#define VKHRT_BATCH_MAX_BOUNCES 8
+#define VKHRT_BATCH_MAX_MATERIALS 16 /* material indices above 15 use entry 15 */
+#define VKHRT_BATCH_RGBA8 (1u << 2)          /* images are RGBA8 (else RGBA32F radiance) */
+#define VKHRT_BATCH_FP64_POLISH (1u << 3)    /* FP64 Newton polish of grazing phantom roots */
+#define VKHRT_BATCH_SORT_MATERIALS (1u << 5) /* shade each depth in material order */
+#define VKHRT_BATCH_GRAPH (1u << 6)          /* run the frame as a captured HIP graph; the graph
+                                                is captured on the first call and replayed while
+                                                the TLAS, total pixel count, bounces, flags and
+                                                options stay the same (cameras, view sizes,
+                                                output buffers, seed and colours may change) */
+typedef struct vkhrt_view {
+    vkhrt_camera camera;
+    uint32_t width;
+    uint32_t height;
+    void* image;             /* device, width * height pixels */
+    uint32_t* primitive_ids; /* device, optional: primary hit primitive per pixel */
+} vkhrt_view;
+typedef struct vkhrt_batch_desc {
+    uint32_t flags;
+    uint32_t bounces;                /* 0..VKHRT_BATCH_MAX_BOUNCES (0 = direct light only) */
+    uint64_t seed;                   /* sampling seed (e.g. the frame number) */
+    float environment[3];            /* radiance of rays leaving the scene */
+    const float* material_albedo;    /* host, optional: material_count x 3 tints (default 1) */
+    uint32_t material_count;         /* <= VKHRT_BATCH_MAX_MATERIALS */
+} vkhrt_batch_desc;
+/* Renders `view_count` views (at most 65536, 2^31 pixels in total) of `tlas`. The TLAS must
+ * hold instances of one geometry mode. */
+int vkhrt_render_batch(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_view* views, uint32_t view_count,
+                       const vkhrt_batch_desc* desc, void* stream);
+/* Rays of the last vkhrt_render_batch (synchronizes its stream): one primary ray per path,
+ * the bounce rays and the shadow rays traced. */
+typedef struct vkhrt_batch_stats {
+    uint64_t paths;
+    uint64_t bounce_rays;
+    uint64_t shadow_rays;
+    uint32_t graph_captures; /* graphs captured on this context so far */
+    uint32_t reserved;
+} vkhrt_batch_stats;
+int vkhrt_batch_stats_get(vkhrt_context* ctx, vkhrt_batch_stats* out);
+
 /* ---- neural shadow visibility (extension; no reference counterpart)
  * A small MLP predicts the light visibility of a shadow ray from its origin and direction:
  * (p normalized by the scene box, d) -> 6 x 8 frequencies x {sin, cos} = 96 features ->
@@ -292,7 +343,7 @@ int vkhrt_shadow_mlp_infer(vkhrt_context* ctx, const vkhrt_shadow_mlp* mlp, cons
 
 /* ---- profiling: per-kernel HIP-event timing on the launch stream */
 int vkhrt_profiling_enable(vkhrt_context* ctx, int enable);
-/* name: "render_primary", "render_shadow", "trace_rays", "blas_build". */
+/* name: "render_primary", "render_shadow", "render_batch", "trace_rays", "blas_build". */
 int vkhrt_kernel_stats(vkhrt_context* ctx, const char* name, double* total_ms, uint32_t* launches);
 int vkhrt_status_counters(vkhrt_context* ctx, uint32_t* stack_overflows);
 

@@ -23,6 +23,7 @@
 
 #include "oracle_geom.hpp"
 #include "oracle_mlp.hpp"
+#include "oracle_path.hpp"
 
 using namespace orc;
 
@@ -89,6 +90,7 @@ struct Blas {
 struct Instance {
     std::shared_ptr<Blas> blas;
     bool identity = true;
+    uint32_t material = 0;  // batched path tracing: albedo tint index
     float xf[12];     // world-from-object, row-major 3x4
     M4<float> inv;    // object-from-world
 };
@@ -867,6 +869,13 @@ int orc_scene_add_instance(orc_scene* sp, int mode, const float* prims, uint32_t
     return int(S.inst.size() - 1);
 }
 
+int orc_scene_set_material(orc_scene* sp, uint32_t inst, uint32_t material) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    if (inst >= S.inst.size()) return -1;
+    S.inst[inst].material = material;
+    return 0;
+}
+
 int orc_scene_stats(orc_scene* sp, uint32_t inst, uint32_t* out /*nprims, nnodes*/) {
     Scene& S = *reinterpret_cast<Scene*>(sp);
     if (inst >= S.inst.size()) return -1;
@@ -966,6 +975,106 @@ long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_i
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     return nrays.load();
+}
+
+// ---- batched path tracing (oracle_path.hpp). Views: cams = n x 32 floats (view inverse,
+// projection inverse, column-major), sizes = n x 2 (width, height); paths are the pixels of
+// the views in order, row-major within a view. albedo: 16 x 3 tints. rgb: 3 floats per path,
+// prim_ids: primary hit per path (0xFFFFFFFF = miss). Only every `path_stride`-th path is
+// computed (the others are left untouched). counts[0] += bounce rays, counts[1] += shadow rays.
+long long orc_render_batch(orc_scene* sp, const float* cams, const uint32_t* sizes, uint32_t nviews, uint32_t bounces,
+                           uint64_t seed, const float env[3], const float* albedo, uint32_t flags, float* rgb,
+                           uint32_t* prim_ids, unsigned long long* counts, int nthreads, uint32_t path_stride,
+                           double* seconds) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    std::vector<uint64_t> first(nviews + 1, 0);
+    for (uint32_t v = 0; v < nviews; ++v) first[v + 1] = first[v] + uint64_t(sizes[2 * v]) * sizes[2 * v + 1];
+    const uint64_t total = first[nviews];
+    if (path_stride == 0) path_stride = 1;
+    const uint32_t nsample = uint32_t((total + path_stride - 1) / path_stride);
+    std::atomic<long long> nrays{0};
+    std::atomic<unsigned long long> nbounce{0}, nshadow{0};
+    const uint32_t tflags = flags & (F_REF_NORMAL | F_POLISH);
+    auto t0 = std::chrono::steady_clock::now();
+    parallel_for(nsample, nthreads, [&](uint32_t si) {
+        const uint64_t p = uint64_t(si) * path_stride;
+        const uint32_t v = uint32_t(std::upper_bound(first.begin(), first.end(), p) - first.begin()) - 1;
+        const uint32_t W = sizes[2 * v], H = sizes[2 * v + 1];
+        const uint32_t local = uint32_t(p - first[v]);
+        M4<float> Vi, Pi;
+        std::memcpy(Vi.m, cams + size_t(v) * 32, 64);
+        std::memcpy(Pi.m, cams + size_t(v) * 32 + 16, 64);
+        V3<float> o, d;
+        RayGen(Vi, Pi, local % W, local / W, W, H, o, d);
+        V3<float> T = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
+        uint32_t prim0 = 0xFFFFFFFFu;
+        long long nr = 0;
+        unsigned long long nb = 0, ns = 0;
+        for (uint32_t b = 0; b <= bounces; ++b) {
+            Hit h;
+            ++nr;
+            const bool hit = trace_scene<float>(S, Ray<float>{o, d}, 0.001f, 10000.0f, tflags, h);
+            if (b == 0 && hit) prim0 = h.prim;
+            if (!hit) {
+                L = mk(L.x + T.x * env[0], L.y + T.y * env[1], L.z + T.z * env[2]);
+                break;
+            }
+            const V3<float> x = o + h.t * d;
+            const V3<float> n = mk(h.n[0], h.n[1], h.n[2]);
+            const V3<float> nf = dot(n, d) > 0.0f ? mk(-n.x, -n.y, -n.z) : n;
+            const V3<float> s0 = Shade(n, 1.0f);
+            const float* tint = albedo + 3 * std::min<uint32_t>(S.inst[h.inst].material, 15u);
+            const V3<float> a = mk(tint[0] * s0.x, tint[1] * s0.y, tint[2] * s0.z);
+            const float c = nf.y;
+            if (c > 0.0f) {
+                const V3<float> nee = mk(T.x * a.x * c, T.y * a.y * c, T.z * a.z * c);
+                Hit sh;
+                ++nr;
+                ++ns;
+                if (!trace_scene<float>(S, Ray<float>{x, mk(0.0f, 1.0f, 0.0f)}, 0.001f, 10000.0f,
+                                        F_ANY_HIT | (flags & F_POLISH), sh))
+                    L = mk(L.x + nee.x, L.y + nee.y, L.z + nee.z);
+            }
+            if (b == bounces) break;
+            d = path_cosine_dir(nf, seed, uint32_t(p), b);
+            T = mk(T.x * a.x, T.y * a.y, T.z * a.z);
+            o = x;
+            ++nb;
+        }
+        rgb[p * 3 + 0] = L.x;
+        rgb[p * 3 + 1] = L.y;
+        rgb[p * 3 + 2] = L.z;
+        if (prim_ids) prim_ids[p] = prim0;
+        nrays += nr;
+        nbounce += nb;
+        nshadow += ns;
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (counts) {
+        counts[0] += nbounce.load();
+        counts[1] += nshadow.load();
+    }
+    return nrays.load();
+}
+
+float orc_path_rand(uint64_t seed, uint32_t p, uint32_t depth, uint32_t k) { return path_rand(seed, p, depth, k); }
+
+int orc_path_basis(const float* n, float* t1, float* t2) {
+    V3<float> a, b;
+    path_basis(mk(n[0], n[1], n[2]), a, b);
+    t1[0] = a.x, t1[1] = a.y, t1[2] = a.z;
+    t2[0] = b.x, t2[1] = b.y, t2[2] = b.z;
+    return 0;
+}
+
+// Directions for paths p = p0 .. p0 + count - 1 at one depth (count x 3 floats).
+int orc_path_cosine_dirs(const float* n, uint64_t seed, uint32_t p0, uint32_t count, uint32_t depth, float* out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const V3<float> w = path_cosine_dir(mk(n[0], n[1], n[2]), seed, p0 + i, depth);
+        out[3 * i] = w.x, out[3 * i + 1] = w.y, out[3 * i + 2] = w.z;
+    }
+    return 0;
 }
 
 // ---- neural shadow MLP (oracle_mlp.hpp). samples: n x 8 floats (p.xyz, 0, d.xyz, 0).

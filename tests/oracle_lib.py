@@ -69,6 +69,11 @@ def lib():
         "orc_trace": ([P, P, u32, P, u32, i32], i32),
         "orc_count_candidates": ([P, P, P, u32, P, i32], i32),
         "orc_render": ([P, P, P, u32, u32, u32, i32, P, P, i32, u32, P], ctypes.c_longlong),
+        "orc_scene_set_material": ([P, u32, u32], i32),
+        "orc_render_batch": ([P, P, P, u32, u32, u64, P, P, u32, P, P, P, i32, u32, P], ctypes.c_longlong),
+        "orc_path_rand": ([u64, u32, u32, u32], f32),
+        "orc_path_basis": ([P, P, P], i32),
+        "orc_path_cosine_dirs": ([P, u64, u32, u32, u32, P], i32),
         "orc_mlp_forward": ([P, P, P, P, u32, i32, P], i32),
         "orc_mlp_grad": ([P, P, P, P, P, u32, i32, P, ctypes.POINTER(ctypes.c_double)], i32),
         "orc_mlp_param_count": ([], i32),
@@ -197,6 +202,34 @@ class Scene:
         lib().orc_count_candidates(self.h, ptr(rays), ptr(hits), rays.shape[0], ptr(counts), nthreads)
         return counts
 
+    def set_material(self, inst, material):
+        assert lib().orc_scene_set_material(self.h, inst, material) == 0
+
+    def render_batch(self, views, bounces, seed, env=(0.0, 0.0, 0.0), albedo=None, flags=0, nthreads=0,
+                     path_stride=1):
+        """Batched path tracing: views = [(view_inv, proj_inv, W, H)]. Returns (per-view float
+        images (H, W, 3), per-view primary ids, bounce rays, shadow rays, seconds)."""
+        cams = np.concatenate([np.concatenate([f32(vi).reshape(16), f32(pi).reshape(16)]) for vi, pi, _, _ in views])
+        sizes = np.array([[w, h] for _, _, w, h in views], np.uint32)
+        total = int((sizes[:, 0].astype(np.int64) * sizes[:, 1]).sum())
+        tints = np.ones((16, 3), np.float32)
+        if albedo is not None:
+            a = f32(albedo).reshape(-1, 3)
+            tints[:a.shape[0]] = a
+        rgb = np.zeros((total, 3), np.float32)
+        ids = np.full(total, 0xFFFFFFFF, np.uint32)
+        counts = np.zeros(2, np.uint64)
+        secs = ctypes.c_double(0)
+        lib().orc_render_batch(self.h, ptr(cams), ptr(sizes), len(views), bounces, seed, ptr(f32(env)), ptr(tints),
+                               flags, ptr(rgb), ptr(ids), ptr(counts), nthreads, path_stride, ctypes.byref(secs))
+        imgs, pids, o = [], [], 0
+        for w, h in sizes:
+            n = int(w) * int(h)
+            imgs.append(rgb[o:o + n].reshape(int(h), int(w), 3))
+            pids.append(ids[o:o + n].reshape(int(h), int(w)))
+            o += n
+        return imgs, pids, int(counts[0]), int(counts[1]), secs.value
+
     def render(self, view_inv, proj_inv, W, H, flags=0, shadows=1, nthreads=0, pixel_stride=1):
         img = np.zeros((H, W, 3), np.float32)
         ids = np.full((H, W), 0xFFFFFFFF, np.uint32)
@@ -254,3 +287,20 @@ def mlp_init(seed):
         lim = np.sqrt(6.0 / fan_in)
         P[MLP_OFF[name]:MLP_OFF[name] + size] = rng.uniform(-lim, lim, size)
     return P
+
+
+def path_rand(seed, p, depth, k):
+    return lib().orc_path_rand(seed, p, depth, k)
+
+
+def path_basis(n):
+    t1 = np.zeros(3, np.float32)
+    t2 = np.zeros(3, np.float32)
+    lib().orc_path_basis(ptr(f32(n)), ptr(t1), ptr(t2))
+    return t1, t2
+
+
+def path_cosine_dirs(n, seed, p0, count, depth=0):
+    out = np.zeros((count, 3), np.float32)
+    lib().orc_path_cosine_dirs(ptr(f32(n)), seed, p0, count, depth, ptr(out))
+    return out

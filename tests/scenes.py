@@ -11,6 +11,18 @@ def c0_curves(nstrands=1000, nseg=16, seed=C0_SEED):
     return O.hair_curves(seed, nstrands, nseg)
 
 
+def dense_ball_curves(nstrands=1200, nseg=8, seed=77, sphere_radius=12.0):
+    """The seeded generator on a small sphere: hair dense enough that small views (tens of
+    pixels) see hundreds of hits and bounce rays hit other fibres."""
+    v = O.generate_strands(seed, nstrands, nseg, sphere_radius=sphere_radius)
+    return O.generate_curves(O.strands_to_lines(v))
+
+
+DENSE_VIEWS = [((0.0, 20.0, 45.0), (0.0, 8.0, 0.0), 0.577, (48, 40)),
+               ((30.0, 25.0, 30.0), (0.0, 8.0, 0.0), 0.4, (40, 32)),
+               ((-35.0, 10.0, 20.0), (5.0, 6.0, 0.0), 0.5, (56, 24))]
+
+
 def c0_camera(oracle_cam=True, W=256, H=256):
     return O.camera((0.0, 50.0, 400.0), (0.0, 30.0, 0.0), aspect=W / H)
 

@@ -39,6 +39,13 @@ OPTION_PERSISTENT = 3
 OPTION_REFILL_MIN = 4
 OPTION_RAY_ORDER = 5
 
+BATCH_MAX_BOUNCES = 8
+BATCH_MAX_MATERIALS = 16
+BATCH_RGBA8 = 1 << 2
+BATCH_FP64_POLISH = 1 << 3
+BATCH_SORT_MATERIALS = 1 << 5
+BATCH_GRAPH = 1 << 6
+
 STATUS = {0: "SUCCESS", -1: "INVALID_ARGUMENT", -2: "OUT_OF_MEMORY", -3: "DEVICE",
           -4: "UNSUPPORTED", -5: "STACK_OVERFLOW", -6: "NO_DEVICE"}
 
@@ -54,7 +61,7 @@ EXPORTS = [
     "vkhrt_render_ray_counts", "vkhrt_trace_stats", "vkhrt_render_stats_get",
     "vkhrt_shadow_mlp_create", "vkhrt_shadow_mlp_destroy", "vkhrt_shadow_mlp_get_params",
     "vkhrt_shadow_mlp_set_params", "vkhrt_shadow_mlp_collect", "vkhrt_shadow_mlp_train",
-    "vkhrt_shadow_mlp_gradient", "vkhrt_shadow_mlp_infer",
+    "vkhrt_shadow_mlp_gradient", "vkhrt_shadow_mlp_infer", "vkhrt_render_batch", "vkhrt_batch_stats_get",
 ]
 
 
@@ -92,6 +99,22 @@ class RenderDesc(ctypes.Structure):
 class RenderStats(ctypes.Structure):
     _fields_ = [("primary_rays", ctypes.c_uint64), ("shadow_queries", ctypes.c_uint64),
                 ("shadow_rays_traced", ctypes.c_uint64)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("camera", Camera), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("image", ctypes.c_void_p), ("primitive_ids", ctypes.c_void_p)]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint32), ("bounces", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("environment", ctypes.c_float * 3), ("material_albedo", ctypes.c_void_p),
+                ("material_count", ctypes.c_uint32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("paths", ctypes.c_uint64), ("bounce_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("graph_captures", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class ShadowMlpDesc(ctypes.Structure):
@@ -162,6 +185,8 @@ def lib():
                                     ctypes.POINTER(f32), P], i32),
         "vkhrt_shadow_mlp_gradient": ([P, P, P, P, u32, P, ctypes.POINTER(f32), P], i32),
         "vkhrt_shadow_mlp_infer": ([P, P, P, u32, P, P], i32),
+        "vkhrt_render_batch": ([P, P, ctypes.POINTER(View), u32, ctypes.POINTER(BatchDesc), P], i32),
+        "vkhrt_batch_stats_get": ([P, ctypes.POINTER(BatchStats)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -233,6 +258,35 @@ class Context:
                        shadow_mlp.h if shadow_mlp is not None else None, variance_threshold)
         self.check(lib().vkhrt_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), stream),
                    "render")
+
+    def render_batch(self, tlas, views, bounces=4, seed=0, environment=(0.0, 0.0, 0.0), flags=0,
+                     material_albedo=None, stream=None):
+        """Batched path tracing of many views of `tlas` in one pipeline. views: list of
+        (camera, width, height, image_ptr[, primitive_ids_ptr]); material_albedo: list of
+        (r, g, b) tints indexed by BLAS material index."""
+        arr = (View * len(views))()
+        for i, v in enumerate(views):
+            cam, w, h, img = v[:4]
+            arr[i].camera = cam
+            arr[i].width, arr[i].height, arr[i].image = w, h, img
+            arr[i].primitive_ids = v[4] if len(v) > 4 else None
+        d = BatchDesc()
+        d.flags, d.bounces, d.seed = flags, bounces, seed
+        for k in range(3):
+            d.environment[k] = float(environment[k])
+        tints = None
+        if material_albedo:
+            tints = (ctypes.c_float * (3 * len(material_albedo)))(*[float(x) for t in material_albedo for x in t])
+            d.material_albedo = ctypes.cast(tints, ctypes.c_void_p)
+            d.material_count = len(material_albedo)
+        self.check(lib().vkhrt_render_batch(self.h, tlas.h, arr, len(views), ctypes.byref(d), stream),
+                   "render_batch")
+
+    def batch_stats(self):
+        """(paths, bounce rays, shadow rays, graph captures) of the last render_batch."""
+        st = BatchStats()
+        self.check(lib().vkhrt_batch_stats_get(self.h, ctypes.byref(st)), "batch_stats")
+        return st.paths, st.bounce_rays, st.shadow_rays, st.graph_captures
 
     # geometry front-end
     def generate_strands(self, seed, nstrands, nseg, vertices_ptr, params=None, stream=None):

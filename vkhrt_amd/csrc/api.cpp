@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <numeric>
 #include <new>
@@ -62,6 +63,40 @@ struct vkhrt_context {
     uint32_t* fallback = nullptr;
     size_t cap_neural = 0;
     bool last_neural = false;
+    // batched path tracing: per-path state, queues, counters and its own polish arrays
+    // (a captured graph holds these addresses; `batch_gen` changes whenever they move)
+    float4* b_ray = nullptr;
+    float4* b_thr = nullptr;
+    float4* b_rad = nullptr;
+    float4* b_nee = nullptr;
+    vk::HitRec* b_hits = nullptr;
+    uint32_t* b_q0 = nullptr;
+    uint32_t* b_q1 = nullptr;
+    uint32_t* b_order = nullptr;
+    uint32_t* b_squeue = nullptr;
+    uint32_t* b_counters = nullptr;
+    unsigned long long* b_keys = nullptr;
+    uint4* b_defer = nullptr;
+    vk::HitRec* b_cand_hits = nullptr;
+    unsigned long long* b_cand_keys = nullptr;
+    size_t b_cap = 0;
+    vk::BatchView* b_views = nullptr;  // device view table
+    uint32_t b_views_cap = 0;
+    vk::BatchFrame* b_frame = nullptr;
+    uint8_t* b_host = nullptr;         // pinned staging of the view table + frame values
+    size_t b_host_cap = 0;
+    hipEvent_t b_upload = nullptr;     // last upload out of the staging buffer
+    uint64_t batch_gen = 0;
+    // captured frame graph and what it was captured for
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    uint32_t graph_captures = 0;
+    // last batch (stats)
+    uint64_t last_batch_paths = 0;
+    uint32_t last_batch_bounces = 0;
+    hipStream_t last_batch_stream = nullptr;
 };
 
 struct vkhrt_shadow_mlp {
@@ -94,6 +129,7 @@ struct vkhrt_tlas {
     int variant = vk::kVarMixed;  // kernel variant without FP64 polish
     uint32_t max_depth = 0;       // deepest BLAS (8-wide levels)
     bool polish_ok = true;        // merge keys and compacted-queue entries can encode every (instance, slot)
+    uint64_t serial = 0;          // unique per TLAS built in this process (graph reuse key)
 };
 
 namespace {
@@ -325,6 +361,78 @@ int ensure_neural_scratch(vkhrt_context* c, size_t npix) {
     return VKHRT_SUCCESS;
 }
 
+std::atomic<uint64_t> g_tlas_serial{0};
+
+void free_batch_scratch(vkhrt_context* c) {
+    void* bufs[] = {c->b_ray,     c->b_thr,    c->b_rad,    c->b_nee,   c->b_hits,        c->b_q0,          c->b_q1,
+                    c->b_order,   c->b_squeue, c->b_counters, c->b_keys, c->b_defer, c->b_cand_hits, c->b_cand_keys};
+    for (void* b : bufs) hipFree(b);
+    c->b_ray = c->b_thr = c->b_rad = c->b_nee = nullptr;
+    c->b_hits = c->b_cand_hits = nullptr;
+    c->b_q0 = c->b_q1 = c->b_order = c->b_squeue = c->b_counters = nullptr;
+    c->b_keys = c->b_cand_keys = nullptr;
+    c->b_defer = nullptr;
+    c->b_cap = 0;
+}
+
+// Per-path batch state for `paths` paths (the polish candidate queue holds one candidate per
+// path per depth, at least 64Ki).
+int ensure_batch_scratch(vkhrt_context* c, size_t paths) {
+    if (paths <= c->b_cap && c->b_counters) return VKHRT_SUCCESS;
+    free_batch_scratch(c);
+    ++c->batch_gen;
+    const size_t cand = std::max<size_t>(paths, 65536);
+    hipError_t e = hipMalloc(&c->b_ray, paths * 2 * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&c->b_thr, paths * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&c->b_rad, paths * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&c->b_nee, paths * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&c->b_hits, paths * sizeof(vk::HitRec));
+    if (e == hipSuccess) e = hipMalloc(&c->b_q0, paths * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->b_q1, paths * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->b_order, paths * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->b_squeue, paths * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->b_counters, vk::kBatchCounterWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->b_keys, paths * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&c->b_defer, cand * sizeof(uint4));
+    if (e == hipSuccess) e = hipMalloc(&c->b_cand_hits, cand * sizeof(vk::HitRec));
+    if (e == hipSuccess) e = hipMalloc(&c->b_cand_keys, cand * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        free_batch_scratch(c);
+        return hip_fail(c, e, "batch scratch");
+    }
+    c->b_cap = paths;
+    return VKHRT_SUCCESS;
+}
+
+// Device view table (+ frame values) and its pinned staging buffer.
+int ensure_batch_tables(vkhrt_context* c, uint32_t nviews) {
+    if (!c->b_frame) {
+        hipError_t e = hipMalloc(&c->b_frame, sizeof(vk::BatchFrame));
+        if (e == hipSuccess && !c->b_upload) e = hipEventCreateWithFlags(&c->b_upload, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(c, e, "batch frame");
+    }
+    if (nviews > c->b_views_cap) {
+        hipFree(c->b_views);
+        c->b_views = nullptr;
+        c->b_views_cap = 0;
+        ++c->batch_gen;
+        hipError_t e = hipMalloc(&c->b_views, size_t(nviews) * sizeof(vk::BatchView));
+        if (e != hipSuccess) return hip_fail(c, e, "batch views");
+        c->b_views_cap = nviews;
+    }
+    const size_t need = size_t(nviews) * sizeof(vk::BatchView) + sizeof(vk::BatchFrame);
+    if (need > c->b_host_cap) {
+        if (c->b_upload) hipEventSynchronize(c->b_upload);
+        hipHostFree(c->b_host);
+        c->b_host = nullptr;
+        c->b_host_cap = 0;
+        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&c->b_host), need, hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(c, e, "batch staging");
+        c->b_host_cap = need;
+    }
+    return VKHRT_SUCCESS;
+}
+
 const float kZeroF = 0.0f;
 
 uint64_t splitmix(uint64_t& s) {
@@ -392,6 +500,15 @@ void vkhrt_context_destroy(vkhrt_context* c) {
     hipFree(c->keys);
     hipFree(c->origins);
     hipFree(c->fallback);
+    free_batch_scratch(c);
+    hipFree(c->b_views);
+    hipFree(c->b_frame);
+    if (c->b_upload) hipEventSynchronize(c->b_upload);
+    if (c->b_upload) hipEventDestroy(c->b_upload);
+    hipHostFree(c->b_host);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    if (c->capture_stream) hipStreamDestroy(c->capture_stream);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -589,6 +706,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         I.blas.mode = b->mode;
         I.blas.radius_mode = b->radius_mode;
         I.blas.radius = b->radius;
+        I.material = b->material_index;
         const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
         const float* xf = b->has_xf ? b->xf : ident;
         std::memcpy(I.xf, xf, sizeof(I.xf));
@@ -622,6 +740,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
     if (!t) return fail(c, VKHRT_ERROR_OUT_OF_MEMORY, "tlas");
     t->ctx = c;
     t->count = count;
+    t->serial = ++g_tlas_serial;
     {
         int v = -1;
         for (uint32_t i = 0; i < count; ++i) {
@@ -777,6 +896,156 @@ int vkhrt_render_stats_get(vkhrt_context* c, vkhrt_render_stats* out) {
         if (e != hipSuccess) return hip_fail(c, e, "render stats");
         out->shadow_queries = q[0];
         out->shadow_rays_traced = c->last_neural ? q[5] : q[0];
+    }
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* views, uint32_t view_count,
+                       const vkhrt_batch_desc* d, void* stream) {
+    if (!c || !t || !d || !views) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (view_count == 0 || view_count > 65536) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "view count must be 1..65536");
+    const uint32_t known = VKHRT_BATCH_RGBA8 | VKHRT_BATCH_FP64_POLISH | VKHRT_BATCH_SORT_MATERIALS | VKHRT_BATCH_GRAPH;
+    if (d->flags & ~known) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown batch flags");
+    if (d->bounces > VKHRT_BATCH_MAX_BOUNCES) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bounces above the maximum");
+    if (d->material_count > VKHRT_BATCH_MAX_MATERIALS || (d->material_count && !d->material_albedo))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad material table");
+    if (t->variant == vk::kVarMixed)
+        return fail(c, VKHRT_ERROR_UNSUPPORTED, "batched path tracing needs a TLAS of one geometry mode");
+    if (is_phantom(t) && !t->polish_ok)
+        return fail(c, VKHRT_ERROR_UNSUPPORTED,
+                    "batched phantom tracing needs < 2^26 primitives per BLAS and <= 31 instances");
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < view_count; ++i) {
+        if (!views[i].image) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null view image");
+        if (views[i].width == 0 || views[i].height == 0)
+            return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "empty view");
+        total += uint64_t(views[i].width) * views[i].height;
+        if (total > (1ull << 31)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "more than 2^31 pixels in a batch");
+    }
+    int rc = ensure_batch_scratch(c, total);
+    if (rc == VKHRT_SUCCESS) rc = ensure_batch_tables(c, view_count);
+    if (rc != VKHRT_SUCCESS) return rc;
+    hipStream_t st = pick(c, stream);
+
+    // view table and frame values, staged in pinned memory once the previous upload is done
+    hipError_t e = hipEventSynchronize(c->b_upload);
+    if (e != hipSuccess) return hip_fail(c, e, "batch staging");
+    auto* hv = reinterpret_cast<vk::BatchView*>(c->b_host);
+    uint32_t first = 0;
+    for (uint32_t i = 0; i < view_count; ++i) {
+        vk::BatchView& v = hv[i];
+        std::memset(&v, 0, sizeof(v));
+        std::memcpy(v.cam.view_inv, views[i].camera.view_inverse, 64);
+        std::memcpy(v.cam.proj_inv, views[i].camera.proj_inverse, 64);
+        v.cam.width = views[i].width;
+        v.cam.height = views[i].height;
+        set_work_order(v.cam, 0);
+        v.first = first;
+        v.image = views[i].image;
+        v.prim_out = views[i].primitive_ids;
+        first += views[i].width * views[i].height;
+    }
+    auto* hf = reinterpret_cast<vk::BatchFrame*>(c->b_host + size_t(view_count) * sizeof(vk::BatchView));
+    std::memset(hf, 0, sizeof(*hf));
+    hf->seed = d->seed;
+    for (int k = 0; k < 3; ++k) hf->env[k] = d->environment[k];
+    hf->nviews = view_count;
+    hf->out_format = (d->flags & VKHRT_BATCH_RGBA8) ? 1u : 0u;
+    for (uint32_t m = 0; m < vk::kBatchMaxMaterials; ++m)
+        for (int k = 0; k < 3; ++k) hf->albedo[m][k] = m < d->material_count ? d->material_albedo[m * 3 + k] : 1.0f;
+    e = hipMemcpyAsync(c->b_views, hv, size_t(view_count) * sizeof(vk::BatchView), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->b_frame, hf, sizeof(*hf), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(c->b_upload, st);
+    if (e != hipSuccess) return hip_fail(c, e, "batch upload");
+
+    const bool polish = (d->flags & VKHRT_BATCH_FP64_POLISH) && is_phantom(t);
+    vk::BatchParams bp{};
+    bp.tp = vk::TraceParams{t->dev, t->count, polish ? uint32_t(vk::kTraceFp64Polish) : 0u, c->status,
+                            c->b_counters, c->refill_min};
+    if (polish) {
+        bp.tp.defer = c->b_defer;
+        bp.tp.defer_cap = uint32_t(std::max<size_t>(c->b_cap, 65536));
+        bp.tp.keys = c->b_keys;
+        bp.tp.cand_hits = c->b_cand_hits;
+        bp.tp.cand_keys = c->b_cand_keys;
+    }
+    bp.views = c->b_views;
+    bp.frame = c->b_frame;
+    bp.total = uint32_t(total);
+    bp.bounces = d->bounces;
+    bp.sort = (d->flags & VKHRT_BATCH_SORT_MATERIALS) ? 1u : 0u;
+    bp.ray = c->b_ray;
+    bp.thr = c->b_thr;
+    bp.rad = c->b_rad;
+    bp.nee = c->b_nee;
+    bp.hits = c->b_hits;
+    bp.q[0] = c->b_q0;
+    bp.q[1] = c->b_q1;
+    bp.order = c->b_order;
+    bp.squeue = c->b_squeue;
+    bp.counters = c->b_counters;
+    const int variant = trace_variant(t, polish), stack = stack_kind_for(c, t);
+
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (c->profiling) {
+        for (auto& x : ev) hipEventCreate(&x);
+        hipEventRecord(ev[0], st);
+    }
+    if (d->flags & VKHRT_BATCH_GRAPH) {
+        const std::vector<uint64_t> key = {t->serial,  uint64_t(variant), uint64_t(stack), total, d->bounces,
+                                           bp.sort,    polish,            c->refill_min,   c->batch_gen};
+        if (!c->graph_exec || c->graph_key != key) {
+            if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+            if (c->graph) hipGraphDestroy(c->graph);
+            c->graph_exec = nullptr;
+            c->graph = nullptr;
+            c->graph_key.clear();
+            if (!c->capture_stream) e = hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeRelaxed);
+            if (e != hipSuccess) return hip_fail(c, e, "batch graph capture");
+            const hipError_t le = vk::launch_batch(bp, variant, stack, c->capture_stream);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(c->capture_stream, &g);
+            if (le != hipSuccess || e != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                return hip_fail(c, le != hipSuccess ? le : e, "batch graph capture");
+            }
+            e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                hipGraphDestroy(g);
+                c->graph_exec = nullptr;
+                return hip_fail(c, e, "batch graph instantiate");
+            }
+            c->graph = g;
+            c->graph_key = key;
+            ++c->graph_captures;
+        }
+        e = hipGraphLaunch(c->graph_exec, st);
+    } else {
+        e = vk::launch_batch(bp, variant, stack, st);
+    }
+    if (c->profiling) {
+        hipEventRecord(ev[1], st);
+        c->pending.push_back(PendingTiming{"render_batch", ev[0], ev[1]});
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "batch launch");
+    c->last_batch_paths = total;
+    c->last_batch_bounces = d->bounces;
+    c->last_batch_stream = st;
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_batch_stats_get(vkhrt_context* c, vkhrt_batch_stats* out) {
+    if (!c || !out) return VKHRT_ERROR_INVALID_ARGUMENT;
+    *out = vkhrt_batch_stats{c->last_batch_paths, 0, 0, c->graph_captures, 0};
+    if (!c->last_batch_paths) return VKHRT_SUCCESS;
+    uint32_t w[vk::kBatchDepthWords * (vk::kBatchMaxBounces + 2)];
+    hipError_t e = hipStreamSynchronize(c->last_batch_stream);
+    if (e == hipSuccess) e = hipMemcpy(w, c->b_counters, sizeof(w), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "batch stats");
+    for (uint32_t b = 0; b <= c->last_batch_bounces; ++b) {
+        if (b > 0) out->bounce_rays += w[b * vk::kBatchDepthWords + vk::kBcQueue];
+        out->shadow_rays += w[b * vk::kBatchDepthWords + vk::kBcShadow];
     }
     return VKHRT_SUCCESS;
 }

@@ -86,8 +86,68 @@ enum TraceVariant {
     kVarMixed = 6
 };
 
+// ---- batched path tracing: many views of one TLAS in one wavefront pipeline
+constexpr uint32_t kBatchMaxBounces = 8;
+constexpr uint32_t kBatchMaxMaterials = 16;
+// Counter words (zeroed at the start of a frame): per depth b = 0..kBatchMaxBounces, 8 words
+// from kBatchDepthWords * b; the material histogram and cursors of depth b (64 words) from
+// kBatchMatBase + 64 * b.
+constexpr uint32_t kBatchDepthWords = 8;
+enum BatchCounter {
+    kBcQueue = 0,        // paths whose ray of depth b is traced (depth 0: those entering an instance box)
+    kBcWork = 1,         // persistent work counter of that trace
+    kBcShadow = 2,       // shadow rays of depth b
+    kBcShadowWork = 3,
+    kBcDefer = 4,        // deferred FP64 polish candidates of the closest-hit trace
+    kBcDeferShadow = 5,  // ... of the shadow trace
+};
+constexpr uint32_t kBatchMatBase = 128;
+constexpr uint32_t kBatchCounterWords = kBatchMatBase + 64 * (kBatchMaxBounces + 1);
+
+// One view of a batch (device table): its camera (width/height set, row-major work order),
+// its first path index and its output buffers.
+struct BatchView {
+    Camera cam;
+    uint32_t first;
+    uint32_t pad_;
+    void* image;
+    uint32_t* prim_out;  // optional: primary hit primitive per pixel
+};
+
+// Values of one frame that the kernels read from device memory, so that a captured graph
+// replays with new ones.
+struct BatchFrame {
+    unsigned long long seed;
+    float env[3];      // radiance of rays that leave the scene
+    uint32_t nviews;
+    uint32_t out_format;  // 0: RGBA32F, 1: RGBA8
+    uint32_t pad_;
+    float albedo[kBatchMaxMaterials][3];  // per-material tint of the albedo
+};
+
+// Kernel arguments: buffers (fixed while a captured graph is reused) and shape.
+struct BatchParams {
+    TraceParams tp;
+    const BatchView* views;
+    const BatchFrame* frame;
+    uint32_t total;    // paths (pixels of all views)
+    uint32_t bounces;  // indirect bounces per path
+    uint32_t sort;     // material-sorted shading
+    uint32_t pad_;
+    float4* ray;       // per path, 2 x float4: (origin, tmin), (direction, tmax)
+    float4* thr;       // per path: throughput rgb
+    float4* rad;       // per path: radiance rgb
+    float4* nee;       // per path: light contribution of the pending shadow ray, w = its visibility
+    HitRec* hits;      // per queue slot of the current depth
+    uint32_t* q[2];    // path queues of even and odd depths
+    uint32_t* order;   // queue slots in material order
+    uint32_t* squeue;  // paths with a shadow ray at the current depth
+    uint32_t* counters;  // kBatchCounterWords
+};
+
 hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
                              int variant, int stack_kind, hipStream_t st);
+hipError_t launch_batch(const BatchParams& bp, int variant, int stack_kind, hipStream_t st);
 hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hipStream_t st,
                          hipEvent_t* ev /*[4] or null*/);
 
@@ -102,5 +162,7 @@ hipError_t launch_trace_variant(const TraceParams& tp, const float* rays, uint32
                                 hipStream_t st);
 template <class G, int STACK>
 hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEvent_t* ev);
+template <class G, int STACK>
+hipError_t launch_batch_variant(const BatchParams& bp, hipStream_t st);
 
 }  // namespace vk

@@ -50,7 +50,7 @@ struct InstanceDev {
     float inv[16];     // object-from-world, column-major 4x4
     float lo[3], hi[3];  // world bounds
     uint32_t identity;
-    uint32_t pad;
+    uint32_t material;  // BLAS material index
 };
 
 // Hit record, 32 bytes (matches vkhrt_hit in include/vkhrt.h).

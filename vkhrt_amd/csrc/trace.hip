@@ -15,7 +15,11 @@ namespace vk {
     extern template hipError_t launch_render_variant<G, 12>(const RenderParams&, hipStream_t, hipEvent_t*);       \
     extern template hipError_t launch_trace_variant<G, 32>(const TraceParams&, const float*, uint32_t, HitRec*,   \
                                                            uint2*, hipStream_t);                                  \
-    extern template hipError_t launch_render_variant<G, 32>(const RenderParams&, hipStream_t, hipEvent_t*);
+    extern template hipError_t launch_render_variant<G, 32>(const RenderParams&, hipStream_t, hipEvent_t*);        \
+    extern template hipError_t launch_batch_variant<G, 8>(const BatchParams&, hipStream_t);                       \
+    extern template hipError_t launch_batch_variant<G, 12>(const BatchParams&, hipStream_t);                      \
+    extern template hipError_t launch_batch_variant<G, 16>(const BatchParams&, hipStream_t);                      \
+    extern template hipError_t launch_batch_variant<G, 32>(const BatchParams&, hipStream_t);
 
 using PhConst = Geo<kPhantom, kRadiusConstant, false>;
 using PhConstPolish = Geo<kPhantom, kRadiusConstant, true>;
@@ -47,7 +51,26 @@ hipError_t render_stack(const RenderParams& rp, int stack_kind, hipStream_t st, 
     if (stack_kind == 2) return launch_render_variant<G, 16>(rp, st, ev);
     return launch_render_variant<G, 32>(rp, st, ev);
 }
+template <class G>
+hipError_t batch_stack(const BatchParams& bp, int stack_kind, hipStream_t st) {
+    if (stack_kind == 0) return launch_batch_variant<G, 8>(bp, st);
+    if (stack_kind == 1) return launch_batch_variant<G, 12>(bp, st);
+    if (stack_kind == 2) return launch_batch_variant<G, 16>(bp, st);
+    return launch_batch_variant<G, 32>(bp, st);
+}
 }  // namespace
+
+hipError_t launch_batch(const BatchParams& bp, int variant, int stack_kind, hipStream_t st) {
+    switch (variant) {
+        case kVarPhConst: return batch_stack<PhConst>(bp, stack_kind, st);
+        case kVarPhConstPolish: return batch_stack<PhConstPolish>(bp, stack_kind, st);
+        case kVarPhPoint: return batch_stack<PhPoint>(bp, stack_kind, st);
+        case kVarPhPointPolish: return batch_stack<PhPointPolish>(bp, stack_kind, st);
+        case kVarLss: return batch_stack<LssG>(bp, stack_kind, st);
+        case kVarDots: return batch_stack<DotsG>(bp, stack_kind, st);
+        default: return hipErrorNotSupported;
+    }
+}
 
 hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
                              int variant, int stack_kind, hipStream_t st) {

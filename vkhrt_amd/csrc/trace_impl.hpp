@@ -481,13 +481,17 @@ __device__ __forceinline__ f3 debug_color(uint32_t prim) {
     return f3{r[idx], g[idx], 0.3f};
 }
 
-__device__ __forceinline__ void store_px(const RenderParams& rp, uint32_t pix, f3 c) {
-    if (rp.out_format == 0) {
-        reinterpret_cast<float4*>(rp.image)[pix] = make_float4(c.x, c.y, c.z, 1.0f);
+// RGBA32F (alpha 1) or RGBA8 (clamped to [0, 1], rounded) pixel store.
+__device__ __forceinline__ void store_rgba(void* image, uint32_t format, uint32_t pix, f3 c) {
+    if (format == 0) {
+        reinterpret_cast<float4*>(image)[pix] = make_float4(c.x, c.y, c.z, 1.0f);
     } else {
         auto q = [](float v) { return uint32_t(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f); };
-        reinterpret_cast<uint32_t*>(rp.image)[pix] = q(c.x) | (q(c.y) << 8) | (q(c.z) << 16) | (255u << 24);
+        reinterpret_cast<uint32_t*>(image)[pix] = q(c.x) | (q(c.y) << 8) | (q(c.z) << 16) | (255u << 24);
     }
+}
+__device__ __forceinline__ void store_px(const RenderParams& rp, uint32_t pix, f3 c) {
+    store_rgba(rp.image, rp.out_format, pix, c);
 }
 
 template <class G, int STACK>
@@ -1462,8 +1466,18 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
     return hipGetLastError();
 }
 
+}  // namespace vk
+
+#include "batch_impl.hpp"
+
+namespace vk {
+
 // Explicit instantiation helpers for the trace_*.hip translation units.
 #define VKHRT_INSTANTIATE_TRACE(G)                                                                              \
+    template hipError_t launch_batch_variant<G, 8>(const BatchParams&, hipStream_t);                            \
+    template hipError_t launch_batch_variant<G, 12>(const BatchParams&, hipStream_t);                           \
+    template hipError_t launch_batch_variant<G, 16>(const BatchParams&, hipStream_t);                           \
+    template hipError_t launch_batch_variant<G, 32>(const BatchParams&, hipStream_t);                           \
     template hipError_t launch_trace_variant<G, 8>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*, \
                                                    hipStream_t);                                                \
     template hipError_t launch_trace_variant<G, 16>(const TraceParams&, const float*, uint32_t, HitRec*, uint2*,\
