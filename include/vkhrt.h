@@ -170,6 +170,10 @@ int vkhrt_abi_version(void);
                                      0: one thread per ray */
 #define VKHRT_OPTION_REFILL_MIN 4 /* persistent: idle lanes (1..64) a wavefront waits for before
                                      fetching new rays together (default 16) */
+#define VKHRT_OPTION_BLOCKS_PER_CU 6 /* persistent traversal grids: 256-thread blocks (4 wavefronts)
+                                       per CU, 0 (default) = as many as the LDS stack allows */
+#define VKHRT_OPTION_BVH_WIDTH 7     /* children per node for later builds, 4..8 (default 8; nodes
+                                       keep the 8-slot layout) */
 #define VKHRT_OPTION_RAY_ORDER 5   /* vkhrt_render work order: 0 row-major (default), 1 8x8 pixel
                                      tiles, one tile per wavefront */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
@@ -268,6 +272,9 @@ int vkhrt_render_stats_get(vkhrt_context* ctx, vkhrt_render_stats* out);
 #define VKHRT_BATCH_RGBA8 (1u << 2)          /* images are RGBA8 (else RGBA32F radiance) */
 #define VKHRT_BATCH_FP64_POLISH (1u << 3)    /* FP64 Newton polish of grazing phantom roots */
 #define VKHRT_BATCH_SORT_MATERIALS (1u << 5) /* shade each depth in material order */
+#define VKHRT_BATCH_SORT_RAYS (1u << 7)      /* reorder each bounce depth's rays by direction octant
+                                                and origin cell (Morton order in the scene box)
+                                                before tracing */
 #define VKHRT_BATCH_GRAPH (1u << 6)          /* run the frame as a captured HIP graph; the graph
                                                 is captured on the first call and replayed while
                                                 the TLAS, total pixel count, bounces, flags and
@@ -302,6 +309,40 @@ typedef struct vkhrt_batch_stats {
     uint32_t reserved;
 } vkhrt_batch_stats;
 int vkhrt_batch_stats_get(vkhrt_context* ctx, vkhrt_batch_stats* out);
+
+/* ---- autotuner (extension; the reference's driver builds and traces with fixed settings)
+ * Times the caller's own workload under candidate settings (coordinate descent from the
+ * current ones: each knob is swept with the others fixed, twice) and leaves the fastest in
+ * the context. Each trial runs the workload `repeats` times (>= 1, after one warm-up) and
+ * scores the median, timed with HIP events on `stream`. The workload's output buffers are
+ * overwritten. Knobs: stack kind, persistent blocks per CU (waves per CU / 4), refill
+ * minimum, and per workload: the render's ray order, the batch's sort flags, the BLAS's width
+ * and leaf size. Results never depend on the knobs, only the time does. */
+typedef struct vkhrt_tuning {
+    int32_t stack_kind;
+    int32_t blocks_per_cu;
+    int32_t refill_min;
+    int32_t ray_order;
+    int32_t bvh_width;
+    int32_t leaf_size;
+    uint32_t batch_flags;  /* vkhrt_autotune_batch: the chosen VKHRT_BATCH_SORT_* flags */
+    uint32_t trials;
+    float baseline_ms;     /* the workload with the starting settings */
+    float best_ms;         /* ... with the chosen ones (<= baseline_ms) */
+} vkhrt_tuning;
+/* vkhrt_render(tlas, camera, desc) as the workload; sets stack kind, blocks per CU, refill
+ * minimum and ray order. */
+int vkhrt_autotune_render(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_camera* camera,
+                          const vkhrt_render_desc* desc, uint32_t repeats, vkhrt_tuning* out, void* stream);
+/* vkhrt_render_batch as the workload; sets stack kind, blocks per CU and refill minimum and
+ * reports the best sort flags in out->batch_flags (desc->flags' sort bits are the start). */
+int vkhrt_autotune_batch(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_view* views, uint32_t view_count,
+                         const vkhrt_batch_desc* desc, uint32_t repeats, vkhrt_tuning* out, void* stream);
+/* Builds `blas` (with its TLAS) for each BVH width in {4, 6, 8} and leaf size in {1, 2, 3},
+ * renders `desc` from `camera` on each and sets the fastest width and leaf size as the
+ * context's options for later builds (desc.leaf_size of `blas` is ignored here). */
+int vkhrt_autotune_blas(vkhrt_context* ctx, const vkhrt_blas_desc* blas, const vkhrt_camera* camera,
+                        const vkhrt_render_desc* desc, uint32_t repeats, vkhrt_tuning* out, void* stream);
 
 /* ---- neural shadow visibility (extension; no reference counterpart)
  * A small MLP predicts the light visibility of a shadow ray from its origin and direction:

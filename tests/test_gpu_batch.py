@@ -82,13 +82,15 @@ def _assert_same(gi, gp, ri, rp, what):
         assert np.all(a[..., 3] == 1.0)
 
 
-@pytest.mark.parametrize("polish,sort", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("polish,sort", [(0, 0), (0, 1), (1, 0), (1, 1), (1, 2), (0, 3)])
 def test_batch_matches_oracle(gpu_ctx, oracle, scene, polish, sort):
+    """sort: bit 0 material-sorted shading, bit 1 ray-sorted bounce queues."""
     import scenes
     import vkhrt_amd as V
     S, tlas = scene
     views = _views(oracle, scenes.DENSE_VIEWS)
-    flags = (V.BATCH_FP64_POLISH if polish else 0) | (V.BATCH_SORT_MATERIALS if sort else 0)
+    flags = ((V.BATCH_FP64_POLISH if polish else 0) | (V.BATCH_SORT_MATERIALS if sort & 1 else 0) |
+             (V.BATCH_SORT_RAYS if sort & 2 else 0))
     print(f"[batch parity] polish {polish} sort {sort}: 3 views, 3 bounces", flush=True)
     gi, gp, (paths, nb, ns, _) = _gpu_batch(gpu_ctx, tlas, views, 3, 21, flags)
     ri, rp, rnb, rns, _ = S.render_batch(views, bounces=3, seed=21, env=ENV, albedo=TINTS,
@@ -166,8 +168,8 @@ def test_batch_lss_dots(gpu_ctx, oracle, mode):
 
 def test_batch_many_views_sampled(gpu_ctx, oracle):
     """A larger batch (16 views of uneven sizes over two hair models of uneven size, 4 bounces):
-    material sort on/off and graph/eager agree bitwise on every pixel; the oracle checks every
-    13th path."""
+    unsorted, material-sorted, and ray-sorted + graph frames agree bitwise on every pixel; the
+    oracle checks every 13th path."""
     import gpu_util as G
     import scenes
     import vkhrt_amd as V
@@ -197,7 +199,8 @@ def test_batch_many_views_sampled(gpu_ctx, oracle):
     print("[batch many views] 16 views, 4 bounces", flush=True)
     a, ap, (paths, nb, ns, _) = _gpu_batch(gpu_ctx, tlas, views, 4, 77, V.BATCH_FP64_POLISH)
     b, bp, stb = _gpu_batch(gpu_ctx, tlas, views, 4, 77, V.BATCH_FP64_POLISH | V.BATCH_SORT_MATERIALS)
-    c, cp, stc = _gpu_batch(gpu_ctx, tlas, views, 4, 77, V.BATCH_FP64_POLISH | V.BATCH_SORT_MATERIALS | V.BATCH_GRAPH)
+    c, cp, stc = _gpu_batch(gpu_ctx, tlas, views, 4, 77,
+                            V.BATCH_FP64_POLISH | V.BATCH_SORT_MATERIALS | V.BATCH_SORT_RAYS | V.BATCH_GRAPH)
     assert stb[1:3] == stc[1:3] == (nb, ns)
     for x, y, z, px, py, pz in zip(a, b, c, ap, bp, cp):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32)) and np.array_equal(x.view(np.uint32), z.view(np.uint32))

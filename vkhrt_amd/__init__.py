@@ -38,6 +38,8 @@ OPTION_LEAF_SIZE = 2
 OPTION_PERSISTENT = 3
 OPTION_REFILL_MIN = 4
 OPTION_RAY_ORDER = 5
+OPTION_BLOCKS_PER_CU = 6
+OPTION_BVH_WIDTH = 7
 
 BATCH_MAX_BOUNCES = 8
 BATCH_MAX_MATERIALS = 16
@@ -45,6 +47,7 @@ BATCH_RGBA8 = 1 << 2
 BATCH_FP64_POLISH = 1 << 3
 BATCH_SORT_MATERIALS = 1 << 5
 BATCH_GRAPH = 1 << 6
+BATCH_SORT_RAYS = 1 << 7
 
 STATUS = {0: "SUCCESS", -1: "INVALID_ARGUMENT", -2: "OUT_OF_MEMORY", -3: "DEVICE",
           -4: "UNSUPPORTED", -5: "STACK_OVERFLOW", -6: "NO_DEVICE"}
@@ -62,6 +65,7 @@ EXPORTS = [
     "vkhrt_shadow_mlp_create", "vkhrt_shadow_mlp_destroy", "vkhrt_shadow_mlp_get_params",
     "vkhrt_shadow_mlp_set_params", "vkhrt_shadow_mlp_collect", "vkhrt_shadow_mlp_train",
     "vkhrt_shadow_mlp_gradient", "vkhrt_shadow_mlp_infer", "vkhrt_render_batch", "vkhrt_batch_stats_get",
+    "vkhrt_autotune_render", "vkhrt_autotune_batch", "vkhrt_autotune_blas",
 ]
 
 
@@ -115,6 +119,16 @@ class BatchDesc(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("paths", ctypes.c_uint64), ("bounce_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("graph_captures", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("stack_kind", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("refill_min", ctypes.c_int32),
+                ("ray_order", ctypes.c_int32), ("bvh_width", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
+                ("batch_flags", ctypes.c_uint32), ("trials", ctypes.c_uint32), ("baseline_ms", ctypes.c_float),
+                ("best_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ShadowMlpDesc(ctypes.Structure):
@@ -187,6 +201,12 @@ def lib():
         "vkhrt_shadow_mlp_infer": ([P, P, P, u32, P, P], i32),
         "vkhrt_render_batch": ([P, P, ctypes.POINTER(View), u32, ctypes.POINTER(BatchDesc), P], i32),
         "vkhrt_batch_stats_get": ([P, ctypes.POINTER(BatchStats)], i32),
+        "vkhrt_autotune_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderDesc), u32,
+                                   ctypes.POINTER(Tuning), P], i32),
+        "vkhrt_autotune_batch": ([P, P, ctypes.POINTER(View), u32, ctypes.POINTER(BatchDesc), u32,
+                                  ctypes.POINTER(Tuning), P], i32),
+        "vkhrt_autotune_blas": ([P, ctypes.POINTER(BlasDesc), ctypes.POINTER(Camera), ctypes.POINTER(RenderDesc), u32,
+                                 ctypes.POINTER(Tuning), P], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -259,11 +279,8 @@ class Context:
         self.check(lib().vkhrt_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), stream),
                    "render")
 
-    def render_batch(self, tlas, views, bounces=4, seed=0, environment=(0.0, 0.0, 0.0), flags=0,
-                     material_albedo=None, stream=None):
-        """Batched path tracing of many views of `tlas` in one pipeline. views: list of
-        (camera, width, height, image_ptr[, primitive_ids_ptr]); material_albedo: list of
-        (r, g, b) tints indexed by BLAS material index."""
+    @staticmethod
+    def _batch_args(views, bounces, seed, environment, flags, material_albedo):
         arr = (View * len(views))()
         for i, v in enumerate(views):
             cam, w, h, img = v[:4]
@@ -279,8 +296,42 @@ class Context:
             tints = (ctypes.c_float * (3 * len(material_albedo)))(*[float(x) for t in material_albedo for x in t])
             d.material_albedo = ctypes.cast(tints, ctypes.c_void_p)
             d.material_count = len(material_albedo)
+        return arr, d, tints
+
+    def render_batch(self, tlas, views, bounces=4, seed=0, environment=(0.0, 0.0, 0.0), flags=0,
+                     material_albedo=None, stream=None):
+        """Batched path tracing of many views of `tlas` in one pipeline. views: list of
+        (camera, width, height, image_ptr[, primitive_ids_ptr]); material_albedo: list of
+        (r, g, b) tints indexed by BLAS material index."""
+        arr, d, _keep = self._batch_args(views, bounces, seed, environment, flags, material_albedo)
         self.check(lib().vkhrt_render_batch(self.h, tlas.h, arr, len(views), ctypes.byref(d), stream),
                    "render_batch")
+
+    # autotuner: times the given workload under candidate settings, keeps the fastest
+    def autotune_render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS, repeats=3, stream=None):
+        d = RenderDesc(width, height, flags, image_ptr, None, None, None, 0.0)
+        t = Tuning()
+        self.check(lib().vkhrt_autotune_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), repeats,
+                                               ctypes.byref(t), stream), "autotune_render")
+        return t.as_dict()
+
+    def autotune_batch(self, tlas, views, bounces=4, seed=0, environment=(0.0, 0.0, 0.0), flags=0,
+                       material_albedo=None, repeats=3, stream=None):
+        arr, d, _keep = self._batch_args(views, bounces, seed, environment, flags, material_albedo)
+        t = Tuning()
+        self.check(lib().vkhrt_autotune_batch(self.h, tlas.h, arr, len(views), ctypes.byref(d), repeats,
+                                              ctypes.byref(t), stream), "autotune_batch")
+        return t.as_dict()
+
+    def autotune_blas(self, mode, primitives_ptr, count, camera, width, height, image_ptr, blas_flags=0,
+                      radius=0.02, render_flags=RENDER_SHADOWS, repeats=3, stream=None):
+        b = BlasDesc()
+        b.mode, b.flags, b.primitives, b.count, b.radius = mode, blas_flags, primitives_ptr, count, radius
+        d = RenderDesc(width, height, render_flags, image_ptr, None, None, None, 0.0)
+        t = Tuning()
+        self.check(lib().vkhrt_autotune_blas(self.h, ctypes.byref(b), ctypes.byref(camera), ctypes.byref(d), repeats,
+                                             ctypes.byref(t), stream), "autotune_blas")
+        return t.as_dict()
 
     def batch_stats(self):
         """(paths, bounce rays, shadow rays, graph captures) of the last render_batch."""

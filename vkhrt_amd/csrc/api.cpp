@@ -33,9 +33,11 @@ struct vkhrt_context {
     std::string err;
     int stack_kind = 1;
     int leaf_size = 1;
+    int bvh_width = 8;
     bool persistent = true;
     uint32_t refill_min = 16;
     uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles
+    uint32_t blocks_per_cu = 0;  // persistent grids: 256-thread blocks per CU (0 = occupancy limit)
     bool profiling = false;
     std::vector<PendingTiming> pending;
     std::map<std::string, std::pair<double, uint32_t>> stats;
@@ -74,6 +76,8 @@ struct vkhrt_context {
     uint32_t* b_q1 = nullptr;
     uint32_t* b_order = nullptr;
     uint32_t* b_squeue = nullptr;
+    uint32_t* b_qsort = nullptr;
+    uint32_t* b_bins = nullptr;
     uint32_t* b_counters = nullptr;
     unsigned long long* b_keys = nullptr;
     uint4* b_defer = nullptr;
@@ -130,6 +134,7 @@ struct vkhrt_tlas {
     uint32_t max_depth = 0;       // deepest BLAS (8-wide levels)
     bool polish_ok = true;        // merge keys and compacted-queue entries can encode every (instance, slot)
     uint64_t serial = 0;          // unique per TLAS built in this process (graph reuse key)
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // world bounds of all instances
 };
 
 namespace {
@@ -364,12 +369,13 @@ int ensure_neural_scratch(vkhrt_context* c, size_t npix) {
 std::atomic<uint64_t> g_tlas_serial{0};
 
 void free_batch_scratch(vkhrt_context* c) {
-    void* bufs[] = {c->b_ray,     c->b_thr,    c->b_rad,    c->b_nee,   c->b_hits,        c->b_q0,          c->b_q1,
-                    c->b_order,   c->b_squeue, c->b_counters, c->b_keys, c->b_defer, c->b_cand_hits, c->b_cand_keys};
+    void* bufs[] = {c->b_ray,    c->b_thr,    c->b_rad,   c->b_nee,      c->b_hits,  c->b_q0,
+                    c->b_q1,     c->b_order,  c->b_squeue, c->b_qsort,   c->b_bins,  c->b_counters,
+                    c->b_keys,   c->b_defer,  c->b_cand_hits, c->b_cand_keys};
     for (void* b : bufs) hipFree(b);
     c->b_ray = c->b_thr = c->b_rad = c->b_nee = nullptr;
     c->b_hits = c->b_cand_hits = nullptr;
-    c->b_q0 = c->b_q1 = c->b_order = c->b_squeue = c->b_counters = nullptr;
+    c->b_q0 = c->b_q1 = c->b_order = c->b_squeue = c->b_qsort = c->b_bins = c->b_counters = nullptr;
     c->b_keys = c->b_cand_keys = nullptr;
     c->b_defer = nullptr;
     c->b_cap = 0;
@@ -391,6 +397,8 @@ int ensure_batch_scratch(vkhrt_context* c, size_t paths) {
     if (e == hipSuccess) e = hipMalloc(&c->b_q1, paths * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&c->b_order, paths * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&c->b_squeue, paths * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->b_qsort, paths * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->b_bins, 2 * vk::kRayBins * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&c->b_counters, vk::kBatchCounterWords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&c->b_keys, paths * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&c->b_defer, cand * sizeof(uint4));
@@ -555,6 +563,16 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
         c->ray_order = uint32_t(value);
         return VKHRT_SUCCESS;
     }
+    if (option == VKHRT_OPTION_BLOCKS_PER_CU) {
+        if (value < 0 || value > 32) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "blocks per CU must be 0..32");
+        c->blocks_per_cu = uint32_t(value);
+        return VKHRT_SUCCESS;
+    }
+    if (option == VKHRT_OPTION_BVH_WIDTH) {
+        if (value < 4 || value > 8) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "BVH width must be 4..8");
+        c->bvh_width = value;
+        return VKHRT_SUCCESS;
+    }
     if (option == VKHRT_OPTION_LEAF_SIZE) {
         if (value < 1 || value > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "leaf size must be 1..3");
         c->leaf_size = value;
@@ -624,7 +642,7 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
         {
             Timer tm(c, st, "blas_build");
             e = vk::build_bvh(int(d->mode), records, d->count, int(b->radius_mode), d->radius,
-                              d->leaf_size ? int(d->leaf_size) : c->leaf_size, st, &b->bvh, err);
+                              d->leaf_size ? int(d->leaf_size) : c->leaf_size, c->bvh_width, st, &b->bvh, err);
         }
         if (e != hipSuccess) {
             rc = fail(c, e == hipErrorOutOfMemory ? VKHRT_ERROR_OUT_OF_MEMORY : VKHRT_ERROR_DEVICE, "bvh build: " + err);
@@ -741,6 +759,18 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
     t->ctx = c;
     t->count = count;
     t->serial = ++g_tlas_serial;
+    for (int k = 0; k < 3; ++k) {
+        t->lo[k] = INFINITY;
+        t->hi[k] = -INFINITY;
+    }
+    for (const vk::InstanceDev& I : inst)
+        for (int k = 0; k < 3; ++k)
+            if (I.lo[k] <= I.hi[k]) {
+                t->lo[k] = std::min(t->lo[k], I.lo[k]);
+                t->hi[k] = std::max(t->hi[k], I.hi[k]);
+            }
+    for (int k = 0; k < 3; ++k)
+        if (!(t->lo[k] <= t->hi[k])) t->lo[k] = t->hi[k] = 0.0f;
     {
         int v = -1;
         for (uint32_t i = 0; i < count; ++i) {
@@ -791,6 +821,7 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     // persistent phantom kernels compact intersection work through keys that need polish_ok
     const bool persistent = c->persistent && (t->polish_ok || !is_phantom(t));
     vk::TraceParams tp{t->dev, t->count, flags, c->status, persistent ? c->counters + 3 : nullptr, c->refill_min};
+    tp.blocks_per_cu = c->blocks_per_cu;
     const bool polish = (flags & VKHRT_TRACE_FP64_POLISH) != 0;
     if (polish) {
         if (!t->polish_ok)
@@ -827,6 +858,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     rp.tp = vk::TraceParams{t->dev, t->count, (d->flags & VKHRT_RENDER_FP64_POLISH) ? uint32_t(vk::kTraceFp64Polish) : 0u,
                             c->status, c->persistent && (t->polish_ok || !is_phantom(t)) ? c->counters + 1 : nullptr,
                             c->refill_min};
+    rp.tp.blocks_per_cu = c->blocks_per_cu;
     std::memcpy(rp.cam.view_inv, cam->view_inverse, 64);
     std::memcpy(rp.cam.proj_inv, cam->proj_inverse, 64);
     rp.cam.width = d->width;
@@ -904,7 +936,8 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
                        const vkhrt_batch_desc* d, void* stream) {
     if (!c || !t || !d || !views) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
     if (view_count == 0 || view_count > 65536) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "view count must be 1..65536");
-    const uint32_t known = VKHRT_BATCH_RGBA8 | VKHRT_BATCH_FP64_POLISH | VKHRT_BATCH_SORT_MATERIALS | VKHRT_BATCH_GRAPH;
+    const uint32_t known = VKHRT_BATCH_RGBA8 | VKHRT_BATCH_FP64_POLISH | VKHRT_BATCH_SORT_MATERIALS |
+                           VKHRT_BATCH_SORT_RAYS | VKHRT_BATCH_GRAPH;
     if (d->flags & ~known) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown batch flags");
     if (d->bounces > VKHRT_BATCH_MAX_BOUNCES) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bounces above the maximum");
     if (d->material_count > VKHRT_BATCH_MAX_MATERIALS || (d->material_count && !d->material_albedo))
@@ -962,6 +995,7 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     vk::BatchParams bp{};
     bp.tp = vk::TraceParams{t->dev, t->count, polish ? uint32_t(vk::kTraceFp64Polish) : 0u, c->status,
                             c->b_counters, c->refill_min};
+    bp.tp.blocks_per_cu = c->blocks_per_cu;
     if (polish) {
         bp.tp.defer = c->b_defer;
         bp.tp.defer_cap = uint32_t(std::max<size_t>(c->b_cap, 65536));
@@ -973,7 +1007,31 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     bp.frame = c->b_frame;
     bp.total = uint32_t(total);
     bp.bounces = d->bounces;
-    bp.sort = (d->flags & VKHRT_BATCH_SORT_MATERIALS) ? 1u : 0u;
+    bp.sort = ((d->flags & VKHRT_BATCH_SORT_MATERIALS) ? uint32_t(vk::kBatchSortMaterial) : 0u) |
+              ((d->flags & VKHRT_BATCH_SORT_RAYS) ? uint32_t(vk::kBatchSortRays) : 0u);
+    bp.qsort = c->b_qsort;
+    bp.bins = c->b_bins;
+    // cell grid of the ray sort: 9 bits over the scene box, split toward the longest cell edge
+    for (int k = 0; k < 3; ++k) {
+        bp.scene_lo[k] = t->lo[k];
+        bp.cell_bits[k] = 0;
+    }
+    for (int i = 0; i < 9; ++i) {
+        int best = 0;
+        float edge = -1.0f;
+        for (int k = 0; k < 3; ++k) {
+            const float e = std::ldexp(std::max(t->hi[k] - t->lo[k], 0.0f), -int(bp.cell_bits[k]));
+            if (e > edge) {
+                edge = e;
+                best = k;
+            }
+        }
+        ++bp.cell_bits[best];
+    }
+    for (int k = 0; k < 3; ++k) {
+        const float ext = t->hi[k] - t->lo[k];
+        bp.cell_scale[k] = ext > 0.0f ? std::ldexp(1.0f, int(bp.cell_bits[k])) / ext : 0.0f;
+    }
     bp.ray = c->b_ray;
     bp.thr = c->b_thr;
     bp.rad = c->b_rad;
@@ -992,8 +1050,8 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
         hipEventRecord(ev[0], st);
     }
     if (d->flags & VKHRT_BATCH_GRAPH) {
-        const std::vector<uint64_t> key = {t->serial,  uint64_t(variant), uint64_t(stack), total, d->bounces,
-                                           bp.sort,    polish,            c->refill_min,   c->batch_gen};
+        const std::vector<uint64_t> key = {t->serial, uint64_t(variant), uint64_t(stack), total,         d->bounces,
+                                           bp.sort,   polish,            c->refill_min,   c->blocks_per_cu, c->batch_gen};
         if (!c->graph_exec || c->graph_key != key) {
             if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
             if (c->graph) hipGraphDestroy(c->graph);
@@ -1047,6 +1105,218 @@ int vkhrt_batch_stats_get(vkhrt_context* c, vkhrt_batch_stats* out) {
         if (b > 0) out->bounce_rays += w[b * vk::kBatchDepthWords + vk::kBcQueue];
         out->shadow_rays += w[b * vk::kBatchDepthWords + vk::kBcShadow];
     }
+    return VKHRT_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Median time (ms) of `repeats` runs of `work` on `st` after one warm-up run.
+template <class F>
+int time_workload(vkhrt_context* c, hipStream_t st, uint32_t repeats, F&& work, float& ms) {
+    int rc = work();
+    if (rc != VKHRT_SUCCESS) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    std::vector<float> t;
+    for (uint32_t i = 0; i < repeats && e == hipSuccess && rc == VKHRT_SUCCESS; ++i) {
+        e = hipEventRecord(a, st);
+        rc = work();
+        if (e == hipSuccess) e = hipEventRecord(b, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float x = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&x, a, b);
+        t.push_back(x);
+    }
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    if (rc != VKHRT_SUCCESS) return rc;
+    if (e != hipSuccess) return hip_fail(c, e, "autotune timing");
+    std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+    ms = t[t.size() / 2];
+    return VKHRT_SUCCESS;
+}
+
+struct Knob {
+    int* value;
+    std::vector<int> candidates;
+};
+
+// Coordinate descent: two sweeps over the knobs, each value tried with the others fixed;
+// `apply` pushes the knob values into the context before a trial. On a failed trial the
+// starting values are restored.
+template <class Apply, class F>
+int coordinate_descent(vkhrt_context* c, hipStream_t st, uint32_t repeats, std::vector<Knob>& knobs, Apply&& apply,
+                       F&& work, vkhrt_tuning* out) {
+    std::vector<int> start;
+    for (const Knob& k : knobs) start.push_back(*k.value);
+    auto restore = [&] {
+        for (size_t i = 0; i < knobs.size(); ++i) *knobs[i].value = start[i];
+        apply();
+    };
+    apply();
+    float best = 0.0f;
+    int rc = time_workload(c, st, repeats, work, best);
+    if (rc != VKHRT_SUCCESS) {
+        restore();
+        return rc;
+    }
+    out->baseline_ms = best;
+    out->trials = 1;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (Knob& k : knobs) {
+            const int keep = *k.value;
+            int chosen = keep;
+            for (int v : k.candidates) {
+                if (v == keep) continue;
+                *k.value = v;
+                apply();
+                float ms = 0.0f;
+                rc = time_workload(c, st, repeats, work, ms);
+                if (rc != VKHRT_SUCCESS) {
+                    restore();
+                    return rc;
+                }
+                ++out->trials;
+                if (ms < best) {
+                    best = ms;
+                    chosen = v;
+                }
+            }
+            *k.value = chosen;
+            apply();
+        }
+    }
+    out->best_ms = best;
+    return VKHRT_SUCCESS;
+}
+
+void fill_tuning(const vkhrt_context* c, vkhrt_tuning* out) {
+    out->stack_kind = c->stack_kind;
+    out->blocks_per_cu = int32_t(c->blocks_per_cu);
+    out->refill_min = int32_t(c->refill_min);
+    out->ray_order = int32_t(c->ray_order);
+    out->bvh_width = c->bvh_width;
+    out->leaf_size = c->leaf_size;
+}
+
+// Stack kinds that can hold the TLAS's deepest tree (smaller ones are raised to the same).
+std::vector<int> stack_candidates(const vkhrt_tlas* t) {
+    std::vector<int> v;
+    for (int k = 0; k < 4; ++k)
+        if (kStackEntries[k] >= t->max_depth || k == 3) v.push_back(k);
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vkhrt_autotune_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam, const vkhrt_render_desc* d,
+                          uint32_t repeats, vkhrt_tuning* out, void* stream) {
+    if (!c || !t || !cam || !d || !out) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (repeats < 1 || repeats > 100) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "repeats must be 1..100");
+    *out = vkhrt_tuning{};
+    hipStream_t st = pick(c, stream);
+    int stack = c->stack_kind, bpc = int(c->blocks_per_cu), refill = int(c->refill_min), order = int(c->ray_order);
+    std::vector<Knob> knobs = {{&stack, stack_candidates(t)},
+                               {&bpc, {0, 6, 4, 3, 2}},
+                               {&refill, {4, 8, 16, 32, 48}},
+                               {&order, {0, 1}}};
+    auto apply = [&] {
+        c->stack_kind = stack;
+        c->blocks_per_cu = uint32_t(bpc);
+        c->refill_min = uint32_t(refill);
+        c->ray_order = uint32_t(order);
+    };
+    const int rc = coordinate_descent(c, st, repeats, knobs, apply, [&] { return vkhrt_render(c, t, cam, d, stream); },
+                                      out);
+    fill_tuning(c, out);
+    return rc;
+}
+
+int vkhrt_autotune_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* views, uint32_t view_count,
+                         const vkhrt_batch_desc* d, uint32_t repeats, vkhrt_tuning* out, void* stream) {
+    if (!c || !t || !views || !d || !out) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (repeats < 1 || repeats > 100) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "repeats must be 1..100");
+    *out = vkhrt_tuning{};
+    hipStream_t st = pick(c, stream);
+    const uint32_t sort_bits = VKHRT_BATCH_SORT_MATERIALS | VKHRT_BATCH_SORT_RAYS;
+    vkhrt_batch_desc dd = *d;
+    int stack = c->stack_kind, bpc = int(c->blocks_per_cu), refill = int(c->refill_min);
+    int sort = int(d->flags & sort_bits);
+    std::vector<Knob> knobs = {{&stack, stack_candidates(t)},
+                               {&bpc, {0, 6, 4, 3, 2}},
+                               {&refill, {4, 8, 16, 32, 48}},
+                               {&sort, {0, int(VKHRT_BATCH_SORT_RAYS), int(VKHRT_BATCH_SORT_MATERIALS), int(sort_bits)}}};
+    auto apply = [&] {
+        c->stack_kind = stack;
+        c->blocks_per_cu = uint32_t(bpc);
+        c->refill_min = uint32_t(refill);
+        dd.flags = (d->flags & ~sort_bits) | uint32_t(sort);
+    };
+    const int rc = coordinate_descent(
+        c, st, repeats, knobs, apply, [&] { return vkhrt_render_batch(c, t, views, view_count, &dd, stream); }, out);
+    fill_tuning(c, out);
+    out->batch_flags = uint32_t(sort);
+    return rc;
+}
+
+int vkhrt_autotune_blas(vkhrt_context* c, const vkhrt_blas_desc* bd, const vkhrt_camera* cam,
+                        const vkhrt_render_desc* d, uint32_t repeats, vkhrt_tuning* out, void* stream) {
+    if (!c || !bd || !cam || !d || !out) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (repeats < 1 || repeats > 100) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "repeats must be 1..100");
+    *out = vkhrt_tuning{};
+    hipStream_t st = pick(c, stream);
+    const int width0 = c->bvh_width, leaf0 = c->leaf_size;
+    // one trial: build with (width, leaf), render, time
+    auto trial = [&](int width, int leaf, float& ms) {
+        c->bvh_width = width;
+        vkhrt_blas_desc b = *bd;
+        b.leaf_size = uint32_t(leaf);
+        vkhrt_blas* blas = nullptr;
+        vkhrt_tlas* tlas = nullptr;
+        int rc = vkhrt_blas_build(c, &b, stream, &blas);
+        if (rc == VKHRT_SUCCESS) rc = vkhrt_tlas_build(c, &blas, 1, stream, &tlas);
+        if (rc == VKHRT_SUCCESS)
+            rc = time_workload(c, st, repeats, [&] { return vkhrt_render(c, tlas, cam, d, stream); }, ms);
+        if (rc == VKHRT_SUCCESS) rc = vkhrt_synchronize(c, stream);
+        if (tlas) vkhrt_tlas_destroy(tlas);
+        if (blas) vkhrt_blas_destroy(blas);
+        return rc;
+    };
+    float best = 0.0f;
+    int rc = trial(width0, leaf0, best);
+    if (rc != VKHRT_SUCCESS) {
+        c->bvh_width = width0;
+        return rc;
+    }
+    out->baseline_ms = best;
+    out->trials = 1;
+    int bw = width0, bl = leaf0;
+    for (int width : {4, 6, 8})
+        for (int leaf : {1, 2, 3}) {
+            if (width == width0 && leaf == leaf0) continue;
+            float ms = 0.0f;
+            rc = trial(width, leaf, ms);
+            if (rc != VKHRT_SUCCESS) {
+                c->bvh_width = width0;
+                c->leaf_size = leaf0;
+                return rc;
+            }
+            ++out->trials;
+            if (ms < best) {
+                best = ms;
+                bw = width;
+                bl = leaf;
+            }
+        }
+    c->bvh_width = bw;
+    c->leaf_size = bl;
+    out->best_ms = best;
+    fill_tuning(c, out);
     return VKHRT_SUCCESS;
 }
 

@@ -86,6 +86,95 @@ __device__ __forceinline__ uint32_t find_view(const BatchView* v, uint32_t nview
     return lo;
 }
 
+// Queue of depth b: the ray-sorted copy for bounce depths when rays are sorted.
+__host__ __device__ __forceinline__ uint32_t* depth_queue(const BatchParams& bp, uint32_t b) {
+    return ((bp.sort & kBatchSortRays) && b > 0) ? bp.qsort : bp.q[b & 1u];
+}
+
+// Sort bin of a ray: direction octant, then the Morton code of its origin's cell.
+__device__ __forceinline__ uint32_t ray_bin(const BatchParams& bp, float4 r0, float4 r1) {
+    const uint32_t oct = (r1.x < 0.0f ? 1u : 0u) | (r1.y < 0.0f ? 2u : 0u) | (r1.z < 0.0f ? 4u : 0u);
+    const float o[3] = {r0.x, r0.y, r0.z};
+    uint32_t c[3];
+    for (int k = 0; k < 3; ++k) {
+        const float u = (o[k] - bp.scene_lo[k]) * bp.cell_scale[k];
+        const uint32_t top = (1u << bp.cell_bits[k]) - 1u;
+        c[k] = u > 0.0f ? min(uint32_t(u), top) : 0u;
+    }
+    uint32_t code = 0;
+    for (int bit = 8; bit >= 0; --bit)
+        for (int k = 0; k < 3; ++k)
+            if (uint32_t(bit) < bp.cell_bits[k]) code = (code << 1) | ((c[k] >> bit) & 1u);
+    return (oct << 9) | code;
+}
+
+// Ray sort of the depth-b queue (b >= 1), pass 1: bin histogram.
+__global__ __launch_bounds__(kAppendBlock) void k_batch_ray_hist(BatchParams bp, uint32_t b) {
+    __shared__ uint32_t s_h[kRayBins];
+    for (uint32_t i = threadIdx.x; i < kRayBins; i += blockDim.x) s_h[i] = 0u;
+    __syncthreads();
+    const uint32_t n = bp.counters[bc(b, kBcQueue)];
+    const uint32_t* q = bp.q[b & 1u];
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const size_t p = q[s];
+        atomicAdd(&s_h[ray_bin(bp, bp.ray[2 * p], bp.ray[2 * p + 1])], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kRayBins; i += blockDim.x)
+        if (s_h[i]) atomicAdd(&bp.bins[i], s_h[i]);
+}
+
+// Pass 2 (one block): histogram -> exclusive bin offsets, in place.
+__global__ __launch_bounds__(kAppendBlock) void k_batch_ray_scan(BatchParams bp) {
+    __shared__ uint32_t s_part[kAppendBlock];
+    constexpr uint32_t per = kRayBins / kAppendBlock;
+    uint32_t v[per], sum = 0;
+    for (uint32_t j = 0; j < per; ++j) {
+        v[j] = bp.bins[threadIdx.x * per + j];
+        sum += v[j];
+    }
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < kAppendBlock; off <<= 1) {  // inclusive scan of the thread sums
+        const uint32_t add = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = s_part[threadIdx.x] - sum;
+    for (uint32_t j = 0; j < per; ++j) {
+        bp.bins[threadIdx.x * per + j] = run;
+        run += v[j];
+    }
+}
+
+// Pass 3: scatter the queue into bin order (bin offset + the block's reservation in the bin
+// + rank within the block).
+__global__ __launch_bounds__(kAppendBlock) void k_batch_ray_scatter(BatchParams bp, uint32_t b) {
+    __shared__ uint32_t s_cnt[kRayBins];
+    __shared__ uint32_t s_base[kRayBins];
+    const uint32_t n = bp.counters[bc(b, kBcQueue)];
+    const uint32_t* q = bp.q[b & 1u];
+    uint32_t* cursor = bp.bins + kRayBins;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < kRayBins; i += blockDim.x) s_cnt[i] = 0u;
+        __syncthreads();
+        const uint32_t s = base + threadIdx.x;
+        uint32_t bin = 0, rank = 0, p = 0;
+        if (s < n) {
+            p = q[s];
+            bin = ray_bin(bp, bp.ray[2 * size_t(p)], bp.ray[2 * size_t(p) + 1]);
+            rank = atomicAdd(&s_cnt[bin], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kRayBins; i += blockDim.x)
+            s_base[i] = s_cnt[i] ? atomicAdd(&cursor[i], s_cnt[i]) : 0u;
+        __syncthreads();
+        if (s < n) bp.qsort[bp.bins[bin] + s_base[bin] + rank] = p;
+        __syncthreads();
+    }
+}
+
 // Primary ray of every path; paths that miss every instance box add the environment and end.
 __global__ __launch_bounds__(kAppendBlock) void k_batch_primary(BatchParams bp) {
     const BatchFrame& F = *bp.frame;
@@ -122,7 +211,7 @@ __global__ __launch_bounds__(kAppendBlock) void k_batch_primary(BatchParams bp) 
 template <class G, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_batch_trace(BatchParams bp, uint32_t b) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    const uint32_t* q = bp.q[b & 1u];
+    const uint32_t* q = depth_queue(bp, b);
     persistent_lanes<G, false, STACK>(
         bp.tp, bp.counters[bc(b, kBcQueue)], bp.counters + bc(b, kBcWork), stk + threadIdx.x,
         [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
@@ -236,7 +325,7 @@ __global__ __launch_bounds__(kAppendBlock) void k_batch_material_scatter(BatchPa
 __global__ __launch_bounds__(kAppendBlock) void k_batch_shade(BatchParams bp, uint32_t b) {
     const BatchFrame& F = *bp.frame;
     const uint32_t n = bp.counters[bc(b, kBcQueue)];
-    const uint32_t* q = bp.q[b & 1u];
+    const uint32_t* q = depth_queue(bp, b);
     uint32_t* qn = bp.q[(b + 1u) & 1u];
     const bool more = b < bp.bounces;
     // the trip count is uniform per block (block_append needs every thread)
@@ -245,7 +334,7 @@ __global__ __launch_bounds__(kAppendBlock) void k_batch_shade(BatchParams bp, ui
         bool shadow = false, bounce = false;
         uint32_t p = 0;
         if (s < n) {
-            const uint32_t qi = bp.sort ? bp.order[s] : s;
+            const uint32_t qi = (bp.sort & kBatchSortMaterial) ? bp.order[s] : s;
             p = q[qi];
             const HitRec h = bp.hits[qi];
             const float4 T = bp.thr[p];
@@ -350,21 +439,29 @@ hipError_t launch_batch_variant(const BatchParams& bp0, hipStream_t st) {
         hipLaunchKernelGGL(k_batch_primary, all, ab, 0, st, bp0);
         for (uint32_t b = 0; b <= bp0.bounces; ++b) {
             BatchParams bp = bp0;
+            if ((bp.sort & kBatchSortRays) && b > 0) {
+                hipLaunchKernelGGL(k_zero, dim3(2 * kRayBins / 256), dim3(256), 0, st, bp.bins, 2 * kRayBins);
+                hipLaunchKernelGGL(k_batch_ray_hist, strided, ab, 0, st, bp, b);
+                hipLaunchKernelGGL(k_batch_ray_scan, dim3(1), ab, 0, st, bp);
+                hipLaunchKernelGGL(k_batch_ray_scatter, strided, ab, 0, st, bp, b);
+            }
             bp.tp.defer_count = bp0.counters + bc(b, kBcDefer);
-            hipLaunchKernelGGL((k_batch_trace<G, STACK>), persistent_grid(k_batch_trace<G, STACK>), tb, 0, st, bp, b);
+            hipLaunchKernelGGL((k_batch_trace<G, STACK>), persistent_grid(k_batch_trace<G, STACK>, bp.tp.blocks_per_cu),
+                               tb, 0, st, bp, b);
             if (bp.tp.defer) {
-                const SrcBatch src{bp.q[b & 1u], bp.ray};
+                const SrcBatch src{depth_queue(bp, b), bp.ray};
                 hipLaunchKernelGGL(k_polish_closest<SrcBatch>, pgrid, tb, 0, st, bp.tp, src, bp.tp.cand_hits,
                                    bp.tp.cand_keys);
                 hipLaunchKernelGGL(k_polish_fixup, pgrid, tb, 0, st, bp.tp, bp.tp.cand_hits, bp.tp.cand_keys, bp.hits);
             }
-            if (bp.sort) {
+            if (bp.sort & kBatchSortMaterial) {
                 hipLaunchKernelGGL(k_batch_material_hist, strided, ab, 0, st, bp, b);
                 hipLaunchKernelGGL(k_batch_material_scatter, strided, ab, 0, st, bp, b);
             }
             hipLaunchKernelGGL(k_batch_shade, strided, ab, 0, st, bp, b);
             bp.tp.defer_count = bp0.counters + bc(b, kBcDeferShadow);
-            hipLaunchKernelGGL((k_batch_shadow<G, STACK>), persistent_grid(k_batch_shadow<G, STACK>), tb, 0, st, bp, b);
+            hipLaunchKernelGGL((k_batch_shadow<G, STACK>), persistent_grid(k_batch_shadow<G, STACK>, bp.tp.blocks_per_cu),
+                               tb, 0, st, bp, b);
             if (bp.tp.defer)
                 hipLaunchKernelGGL(k_batch_polish_shadow, pgrid, tb, 0, st, bp.tp, SrcBatchShadow{bp.squeue, bp.ray},
                                    bp.nee);

@@ -19,8 +19,9 @@ struct BuiltBvh {
 };
 
 // Builds the 8-wide BVH over `n` primitive records (device pointer) on stream `st`.
+// `width` (2..8) caps the children per wide node.
 hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mode, float radius, int leaf_max,
-                     hipStream_t st, BuiltBvh* out, std::string& err);
+                     int width, hipStream_t st, BuiltBvh* out, std::string& err);
 
 struct TraceParams {
     const InstanceDev* inst;
@@ -36,6 +37,7 @@ struct TraceParams {
     unsigned long long* keys;  // per ray (t bits << 32 | instance << 27 | primitive) for the merge
     HitRec* cand_hits;         // per candidate
     unsigned long long* cand_keys;
+    uint32_t blocks_per_cu;    // persistent grids: 256-thread blocks per CU, 0 = as many as fit
 };
 
 struct Camera {
@@ -132,7 +134,7 @@ struct BatchParams {
     const BatchFrame* frame;
     uint32_t total;    // paths (pixels of all views)
     uint32_t bounces;  // indirect bounces per path
-    uint32_t sort;     // material-sorted shading
+    uint32_t sort;     // kBatchSortMaterial | kBatchSortRays
     uint32_t pad_;
     float4* ray;       // per path, 2 x float4: (origin, tmin), (direction, tmax)
     float4* thr;       // per path: throughput rgb
@@ -143,7 +145,16 @@ struct BatchParams {
     uint32_t* order;   // queue slots in material order
     uint32_t* squeue;  // paths with a shadow ray at the current depth
     uint32_t* counters;  // kBatchCounterWords
+    // ray sort of the bounce queues: bin = direction octant << 9 | Morton cell of the origin in
+    // the scene box (cell_bits per axis, 9 in total)
+    uint32_t* qsort;     // depth >= 1 queue in bin order
+    uint32_t* bins;      // kRayBins histogram (then bin offsets) + kRayBins cursors
+    float scene_lo[3];
+    float cell_scale[3];  // cells per unit length
+    uint32_t cell_bits[3];
 };
+enum BatchSort { kBatchSortMaterial = 1, kBatchSortRays = 2 };
+constexpr uint32_t kRayBins = 4096;
 
 hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
                              int variant, int stack_kind, hipStream_t st);

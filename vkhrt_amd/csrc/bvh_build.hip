@@ -10,7 +10,7 @@
 //   4. Karras (HPG 2012) binary radix tree, one thread per internal node
 //   5. bottom-up box refit, second-arriving thread continues (acq_rel atomics)
 //   6. top-down collapse to 8-wide nodes, one level per launch: each binary subtree is
-//      opened greedily by largest surface area until 8 children; subtrees with at most
+//      opened greedily by largest surface area until `width` (<= 8) children; subtrees with at most
 //      `leaf_max` primitives become leaf children. Child slots are ordered per ray octant;
 //      child boxes are quantized to 8 bits against power-of-two scales (outward rounding)
 //   7. gather primitive records into leaf order
@@ -345,7 +345,8 @@ __device__ __forceinline__ uint32_t fexp_for(float extent) {
 
 __global__ void k_collapse(const Task* __restrict__ tin, const uint32_t* __restrict__ ntin, Task* __restrict__ tout,
                            uint32_t* __restrict__ ntout, uint32_t* __restrict__ node_counter,
-                           uint32_t* __restrict__ prim_counter, int n, int leaf_max, const uint2* __restrict__ child,
+                           uint32_t* __restrict__ prim_counter, int n, int leaf_max, int width,
+                           const uint2* __restrict__ child,
                            const uint2* __restrict__ range, const uint32_t* __restrict__ vals,
                            const float4* __restrict__ blo, const float4* __restrict__ bhi, const float* __restrict__ nlo,
                            const float* __restrict__ nhi, WideNode* __restrict__ nodes,
@@ -388,7 +389,7 @@ __global__ void k_collapse(const Task* __restrict__ tin, const uint32_t* __restr
         load(1, c.y);
         ne = 2;
     }
-    while (ne < 8) {
+    while (ne < width) {
         int best = -1;
         float ba = -1.0f;
         for (int e = 0; e < ne; ++e) {
@@ -593,7 +594,7 @@ inline uint32_t blocks(uint64_t n, int b = kBlock) { return uint32_t((n + b - 1)
     } while (0)
 
 hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mode, float radius, int leaf_max,
-                     hipStream_t st, BuiltBvh* out, std::string& err) {
+                     int width, hipStream_t st, BuiltBvh* out, std::string& err) {
     const int rec = mode == kLss ? 2 : 4;
     float4 *blo = nullptr, *bhi = nullptr;
     uint32_t* cb = nullptr;
@@ -617,6 +618,8 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     if (n == 0) return hipSuccess;
     if (leaf_max < 1) leaf_max = 1;
     if (leaf_max > 3) leaf_max = 3;
+    if (width < 2) width = 2;
+    if (width > 8) width = 8;
 
     VK_TRY(dalloc(&blo, n));
     VK_TRY(dalloc(&bhi, n));
@@ -671,7 +674,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
         uint32_t* nout = cnt + ((level & 1) ^ 1);
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, nout, 0u, 1u);
         hipLaunchKernelGGL(k_collapse, dim3(blocks(cur, 64)), dim3(64), 0, st, (level & 1) ? tb : ta, nin,
-                           (level & 1) ? ta : tb, nout, cnt + 2, cnt + 3, int(n), leaf_max, child, range, v0, blo, bhi,
+                           (level & 1) ? ta : tb, nout, cnt + 2, cnt + 3, int(n), leaf_max, width, child, range, v0, blo, bhi,
                            nlo, nhi, nodes, slots);
         VK_TRY(hipGetLastError());
         VK_TRY(hipMemcpyAsync(&cur, nout, 4, hipMemcpyDeviceToHost, st));

@@ -1313,14 +1313,16 @@ __device__ __forceinline__ f3 xform_dir_pub(const float* m, f3 p) { return trace
 
 // Grid of a persistent kernel: enough 256-thread blocks to fill every CU at the occupancy
 // the LDS stack allows (4 wavefronts per block, at most 8 blocks per CU).
+// `blocks_per_cu` (> 0) caps the blocks per CU below that occupancy limit.
 template <class K>
-inline dim3 persistent_grid(K kernel) {
+inline dim3 persistent_grid(K kernel, uint32_t blocks_per_cu = 0) {
     int dev = 0, cus = 0, per_cu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel),
                                                  trace_detail::kTraceBlock, 0);
     if (per_cu < 1) per_cu = 1;
+    if (blocks_per_cu > 0 && int(blocks_per_cu) < per_cu) per_cu = int(blocks_per_cu);
     if (cus < 1) cus = 1;
     return dim3(uint32_t(cus * per_cu));
 }
@@ -1369,11 +1371,11 @@ hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t 
             hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.work, 1u);
             if (any)
                 hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK>),
-                                   persistent_grid(k_trace_rays_persistent<G, true, STACK>), block, 0, st, tp, rays, n,
+                                   persistent_grid(k_trace_rays_persistent<G, true, STACK>, tp.blocks_per_cu), block, 0, st, tp, rays, n,
                                    hits, tp.work);
             else
                 hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK>),
-                                   persistent_grid(k_trace_rays_persistent<G, false, STACK>), block, 0, st, tp, rays, n,
+                                   persistent_grid(k_trace_rays_persistent<G, false, STACK>, tp.blocks_per_cu), block, 0, st, tp, rays, n,
                                    hits, tp.work);
             return hipGetLastError();
         }
@@ -1403,7 +1405,7 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                0, st, rp);
             // the events bracket the traversal kernels alone (the per-kernel timings)
             if (ev) hipEventRecord(ev[0], st);
-            hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>),
+            hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>, tp.blocks_per_cu),
                                block, 0, st, rp);
             if (ev) hipEventRecord(ev[1], st);
             if (tp.defer) {
@@ -1434,7 +1436,7 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                         scount = rp.counters + 5;
                     }
                     hipLaunchKernelGGL((k_render_shadow_defer<G, STACK>),
-                                       persistent_grid(k_render_shadow_defer<G, STACK>), block, 0, st, rp, squeue,
+                                       persistent_grid(k_render_shadow_defer<G, STACK>, tp.blocks_per_cu), block, 0, st, rp, squeue,
                                        scount, rp.counters + 2);
                     if (ev) hipEventRecord(ev[3], st);
                     shadow_timed = true;
