@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--no-neural", action="store_true", help="skip the neural shadow form")
     ap.add_argument("--no-modes", action="store_true", help="skip the LSS and DOTS forms of the frame")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view path-traced form")
+    ap.add_argument("--autotune", type=int, default=1, help="pick BVH width, leaf size, stack kind, blocks per CU, "
+                    "refill minimum and ray order with the autotuner before timing (1) or use the flags (0)")
     ap.add_argument("--neural-views", type=int, default=32, help="training views for the shadow MLP")
     ap.add_argument("--neural-epochs", type=int, default=80)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -109,6 +111,24 @@ def main():
     cam = V.camera_look_at(CAMERA_EYE, CAMERA_TARGET, fov_deg=60.0, aspect=W / H)
     image = torch.empty(W * H, dtype=torch.int32, device="cuda")
     flags = V.RENDER_SHADOWS | V.RENDER_RGBA8 | (V.RENDER_FP64_POLISH if args.polish else 0)
+    tuning = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tb = ctx.autotune_blas(V.GEOMETRY_PHANTOM_CURVES, curves.data_ptr(), nseg_total, cam, W, H, image.data_ptr(),
+                               blas_flags=V.BLAS_PER_POINT_RADIUS, render_flags=flags, repeats=3, stream=sp)
+        tlas.close()
+        blas.close()
+        blas = V.BottomLevelAccelerationStructure(ctx, V.GEOMETRY_PHANTOM_CURVES, curves.data_ptr(),
+                                                  nseg_total, flags=V.BLAS_PER_POINT_RADIUS, stream=sp)
+        tlas = V.TopLevelAccelerationStructure(ctx, [blas], stream=sp)
+        tr = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags, repeats=5, stream=sp)
+        torch.cuda.synchronize()
+        binfo = blas.info()
+        tuning = {"bvh_width": tb["bvh_width"], "leaf_size": tb["leaf_size"], "stack_kind": tr["stack_kind"],
+                  "blocks_per_cu": tr["blocks_per_cu"], "refill_min": tr["refill_min"], "ray_order": tr["ray_order"],
+                  "trials": tb["trials"] + tr["trials"], "seconds": round(time.perf_counter() - t0, 2),
+                  "start_ms": round(tb["baseline_ms"], 4), "tuned_ms": round(tr["best_ms"], 4)}
+        log(f"[bench] autotuned: {tuning}")
 
     def frame():
         ctx.render(tlas, cam, W, H, image.data_ptr(), flags=flags, stream=sp)
@@ -183,10 +203,13 @@ def main():
             "width": W, "height": H, "geometry": "phantom_cubic_bezier",
             "radius": "per-point taper 0.04->0.01", "fp64_polish": bool(args.polish),
             "rays_per_frame": rays, "primary_rays": n_primary, "shadow_rays": n_shadow,
-            "bvh": {"width": 8, "leaf_size": args.leaf_size, "nodes": binfo["node_count"],
+            "bvh": {"width": tuning["bvh_width"] if tuning else 8,
+                    "leaf_size": tuning["leaf_size"] if tuning else args.leaf_size, "nodes": binfo["node_count"],
                     "depth": binfo["depth"], "build_ms": round(binfo["build_ms"], 3)},
-            "stack_kind": args.stack_kind, "persistent": bool(args.persistent), "refill_min": args.refill_min,
-            "ray_order": args.ray_order,
+            "stack_kind": tuning["stack_kind"] if tuning else args.stack_kind, "persistent": bool(args.persistent),
+            "refill_min": tuning["refill_min"] if tuning else args.refill_min,
+            "ray_order": tuning["ray_order"] if tuning else args.ray_order,
+            "autotune": tuning,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -516,10 +539,18 @@ def batch_block(args, ctx, sp, stream, torch, V):
         return e0.elapsed_time(e1) / frames, nb, nshadow, caps
 
     frames = max(2, min(args.steps, 5))
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tuned = ctx.autotune_batch(tlas, views, bounces=4, seed=0, environment=(0.25, 0.3, 0.35),
+                                   flags=base | V.BATCH_GRAPH, material_albedo=BATCH_TINTS, repeats=1, stream=sp)
+        tuned["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"[bench] batch autotuned: {tuned}")
     variants = {}
-    for name, fl in (("graph_raysort", base | V.BATCH_GRAPH | V.BATCH_SORT_RAYS),
+    named = [("autotuned", base | V.BATCH_GRAPH | tuned["batch_flags"])] if tuned else []
+    for name, fl in named + [("graph_raysort", base | V.BATCH_GRAPH | V.BATCH_SORT_RAYS),
                      ("graph_raysort_material", base | V.BATCH_GRAPH | V.BATCH_SORT_RAYS | V.BATCH_SORT_MATERIALS),
-                     ("graph", base | V.BATCH_GRAPH), ("eager", base)):
+                     ("graph", base | V.BATCH_GRAPH), ("eager", base)]:
         ms, nb, nshadow, caps = timed(fl, frames)
         rays = paths + nb + nshadow
         variants[name] = {"ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1),
@@ -536,7 +567,7 @@ def batch_block(args, ctx, sp, stream, torch, V):
         "build_seconds": round(build_s, 3),
         "value": variants[best]["mrays_per_s"], "unit": "Mrays/s (primary + bounce + shadow rays)",
         "ms_per_frame": variants[best]["ms_per_frame"], "best": best, "variants": variants,
-        "fp64_polish": True,
+        "fp64_polish": True, "autotune": tuned,
     }
 
 

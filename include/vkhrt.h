@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define VKHRT_ABI_VERSION 2
+#define VKHRT_ABI_VERSION 3
 
 typedef struct vkhrt_context vkhrt_context;
 typedef struct vkhrt_blas vkhrt_blas;
@@ -143,6 +143,18 @@ typedef struct vkhrt_camera {
                                                  predicted variance p (1 - p) exceeds
                                                  `shadow_variance_threshold` trace exactly */
 
+/* Area light (extension; the reference lights with the fixed direction of shading.glsl:4): the
+ * parallelogram center + a edge_u + b edge_v, a, b in [-1, 1]. Each hit traces `samples`
+ * shadow rays toward points of it fixed by (seed, pixel, sample) (tmax = distance to the
+ * point); the pixel's visibility is their mean. */
+typedef struct vkhrt_area_light {
+    float center[3];
+    float edge_u[3];
+    float edge_v[3];
+    uint32_t samples; /* 1..16 */
+    uint64_t seed;
+} vkhrt_area_light;
+
 typedef struct vkhrt_render_desc {
     uint32_t width;
     uint32_t height;
@@ -153,6 +165,9 @@ typedef struct vkhrt_render_desc {
                                        replacing the shadow ray */
     const vkhrt_shadow_mlp* shadow_mlp; /* with VKHRT_RENDER_NEURAL_SHADOWS */
     float shadow_variance_threshold;    /* 0 = every ray exact, 0.25 = none */
+    const vkhrt_area_light* area_light; /* host, optional: shadow rays toward an area light instead
+                                           of the direction (0, 1, 0); needs a TLAS of one
+                                           geometry mode */
 } vkhrt_render_desc;
 
 /* ---- context */
@@ -367,6 +382,13 @@ void vkhrt_shadow_mlp_destroy(vkhrt_shadow_mlp* mlp);
 /* FP32 parameters in the flat layout (host buffers; synchronous). */
 int vkhrt_shadow_mlp_get_params(vkhrt_context* ctx, const vkhrt_shadow_mlp* mlp, float* params);
 int vkhrt_shadow_mlp_set_params(vkhrt_context* ctx, vkhrt_shadow_mlp* mlp, const float* params);
+/* Training rays from the hair surface: `count` shadow rays from points drawn on the TLAS's
+ * phantom curves (record uniform over all instances' records, curve parameter and angle about
+ * the fibre uniform, radius as the intersector's) toward a point of `light` (one per ray,
+ * fixed by (seed, i); light->samples is ignored) or, with `light` NULL, along (0, 1, 0).
+ * Label them with vkhrt_shadow_mlp_collect. */
+int vkhrt_generate_surface_shadow_rays(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_area_light* light,
+                                       uint32_t count, uint64_t seed, vkhrt_ray* rays, void* stream);
 /* Traces `count` shadow rays (any hit) and writes training samples and visibility targets. */
 int vkhrt_shadow_mlp_collect(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* shadow_rays, uint32_t count,
                              float* samples, float* targets, void* stream);

@@ -930,9 +930,13 @@ int orc_count_candidates(orc_scene* sp, const float* rays, const void* hits, uin
 // pixel_stride > 1 traces every stride-th pixel (row-major index) for bounded CPU timing.
 // image: W*H*3 floats (untouched for skipped pixels); prim_ids: W*H (0xFFFFFFFF miss) or NULL.
 // Returns the number of rays traced; *seconds gets the wall time of the tracing loop.
+// area_light (optional, 9 floats: center, edge u, edge v): `light_samples` shadow rays per hit
+// toward center + a u + b v, a = 2 path_rand(seed, pixel, s, 0) - 1, b likewise with k = 1,
+// tmax = distance; the visibility is their mean (sum, then / samples). NULL: one ray along
+// (0, 1, 0).
 long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_inv[16], uint32_t W, uint32_t H,
                      uint32_t flags, int shadows, float* image, uint32_t* prim_ids, int nthreads, uint32_t pixel_stride,
-                     double* seconds) {
+                     double* seconds, const float* area_light, uint32_t light_samples, uint64_t light_seed) {
     Scene& S = *reinterpret_cast<Scene*>(sp);
     M4<float> Vi, Pi;
     std::memcpy(Vi.m, view_inv, 64);
@@ -953,12 +957,28 @@ long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_i
         float rgb[3] = {0.0f, 0.0f, 0.0f};
         if (hit) {
             float vis = 1.0f;
-            if (shadows) {
+            if (shadows && !area_light) {
                 V3<float> hp = o + h.t * d;
                 Ray<float> sr{hp, mk(0.0f, 1.0f, 0.0f)};
                 Hit sh;
                 ++nr;
                 if (trace_scene<float>(S, sr, 0.001f, 10000.0f, F_ANY_HIT | (flags & F_POLISH), sh)) vis = 0.0f;
+            } else if (shadows) {
+                const V3<float> hp = o + h.t * d;
+                const float* L = area_light;
+                float sum = 0.0f;
+                for (uint32_t k = 0; k < light_samples; ++k) {
+                    const float a = 2.0f * path_rand(light_seed, pix, k, 0) - 1.0f;
+                    const float b = 2.0f * path_rand(light_seed, pix, k, 1) - 1.0f;
+                    const V3<float> y = mk(L[0] + a * L[3] + b * L[6], L[1] + a * L[4] + b * L[7], L[2] + a * L[5] + b * L[8]);
+                    const V3<float> w = y - hp;
+                    Hit sh;
+                    ++nr;
+                    const bool occ = trace_scene<float>(S, Ray<float>{hp, normalize(w)}, 0.001f, std::sqrt(dot(w, w)),
+                                                        F_ANY_HIT | (flags & F_POLISH), sh);
+                    sum = k == 0 ? (occ ? 0.0f : 1.0f) : sum + (occ ? 0.0f : 1.0f);
+                }
+                vis = light_samples > 1 ? sum / float(light_samples) : sum;
             }
             V3<float> c = Shade(mk(h.n[0], h.n[1], h.n[2]), vis);
             rgb[0] = c.x;

@@ -65,7 +65,7 @@ EXPORTS = [
     "vkhrt_shadow_mlp_create", "vkhrt_shadow_mlp_destroy", "vkhrt_shadow_mlp_get_params",
     "vkhrt_shadow_mlp_set_params", "vkhrt_shadow_mlp_collect", "vkhrt_shadow_mlp_train",
     "vkhrt_shadow_mlp_gradient", "vkhrt_shadow_mlp_infer", "vkhrt_render_batch", "vkhrt_batch_stats_get",
-    "vkhrt_autotune_render", "vkhrt_autotune_batch", "vkhrt_autotune_blas",
+    "vkhrt_autotune_render", "vkhrt_autotune_batch", "vkhrt_autotune_blas", "vkhrt_generate_surface_shadow_rays",
 ]
 
 
@@ -93,11 +93,25 @@ class Camera(ctypes.Structure):
     _fields_ = [("view_inverse", ctypes.c_float * 16), ("proj_inverse", ctypes.c_float * 16)]
 
 
+class AreaLight(ctypes.Structure):
+    _fields_ = [("center", ctypes.c_float * 3), ("edge_u", ctypes.c_float * 3), ("edge_v", ctypes.c_float * 3),
+                ("samples", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
+def area_light(center, edge_u, edge_v, samples=2, seed=0):
+    """Parallelogram light center + a edge_u + b edge_v, a, b in [-1, 1]."""
+    L = AreaLight()
+    for k in range(3):
+        L.center[k], L.edge_u[k], L.edge_v[k] = float(center[k]), float(edge_u[k]), float(edge_v[k])
+    L.samples, L.seed = samples, seed
+    return L
+
+
 class RenderDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("flags", ctypes.c_uint32),
                 ("image", ctypes.c_void_p), ("primitive_ids", ctypes.c_void_p),
                 ("light_visibility", ctypes.c_void_p), ("shadow_mlp", ctypes.c_void_p),
-                ("shadow_variance_threshold", ctypes.c_float)]
+                ("shadow_variance_threshold", ctypes.c_float), ("area_light", ctypes.POINTER(AreaLight))]
 
 
 class RenderStats(ctypes.Structure):
@@ -200,6 +214,7 @@ def lib():
         "vkhrt_shadow_mlp_gradient": ([P, P, P, P, u32, P, ctypes.POINTER(f32), P], i32),
         "vkhrt_shadow_mlp_infer": ([P, P, P, u32, P, P], i32),
         "vkhrt_render_batch": ([P, P, ctypes.POINTER(View), u32, ctypes.POINTER(BatchDesc), P], i32),
+        "vkhrt_generate_surface_shadow_rays": ([P, P, ctypes.POINTER(AreaLight), u32, ctypes.c_uint64, P, P], i32),
         "vkhrt_batch_stats_get": ([P, ctypes.POINTER(BatchStats)], i32),
         "vkhrt_autotune_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderDesc), u32,
                                    ctypes.POINTER(Tuning), P], i32),
@@ -270,12 +285,14 @@ class Context:
 
     def render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS,
                prim_ids_ptr=None, light_visibility_ptr=None, stream=None, shadow_mlp=None,
-               variance_threshold=0.0):
+               variance_threshold=0.0, area_light=None):
         """traceRaysKHR(W, H, 1) with ray_gen.rgen and the closest-hit shaders. With
         RENDER_NEURAL_SHADOWS, `shadow_mlp` answers shadow queries whose predicted variance
-        p (1 - p) is at most `variance_threshold`; the rest trace exactly."""
+        p (1 - p) is at most `variance_threshold`; the rest trace exactly. `area_light`
+        (an AreaLight) replaces the directional light with light samples per hit."""
         d = RenderDesc(width, height, flags, image_ptr, prim_ids_ptr, light_visibility_ptr,
-                       shadow_mlp.h if shadow_mlp is not None else None, variance_threshold)
+                       shadow_mlp.h if shadow_mlp is not None else None, variance_threshold,
+                       ctypes.pointer(area_light) if area_light is not None else None)
         self.check(lib().vkhrt_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), stream),
                    "render")
 
@@ -377,6 +394,13 @@ class Context:
         self.check(lib().vkhrt_generate_shadow_rays(self.h, rays_ptr, hits_ptr, count, _f3(direction),
                                                     tmin, tmax, out_ptr, source_ptr, count_out_ptr,
                                                     stream), "generate_shadow_rays")
+
+    def generate_surface_shadow_rays(self, tlas, count, seed, rays_ptr, light=None, stream=None):
+        """Shadow rays from points drawn on the TLAS's hair surfaces toward `light` (AreaLight or
+        None for the direction (0, 1, 0))."""
+        self.check(lib().vkhrt_generate_surface_shadow_rays(self.h, tlas.h, ctypes.pointer(light) if light else None,
+                                                            count, seed, rays_ptr, stream),
+                   "generate_surface_shadow_rays")
 
     def count_candidates(self, tlas, rays_ptr, hits_ptr, count, counts_ptr, stream=None):
         self.check(lib().vkhrt_count_candidates(self.h, tlas.h, rays_ptr, hits_ptr, count, counts_ptr,

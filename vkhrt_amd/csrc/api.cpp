@@ -62,6 +62,7 @@ struct vkhrt_context {
     size_t keys_cap = 0;
     // neural shadow form
     float4* origins = nullptr;
+    float4* dirs = nullptr;
     uint32_t* fallback = nullptr;
     size_t cap_neural = 0;
     bool last_neural = false;
@@ -135,6 +136,8 @@ struct vkhrt_tlas {
     bool polish_ok = true;        // merge keys and compacted-queue entries can encode every (instance, slot)
     uint64_t serial = 0;          // unique per TLAS built in this process (graph reuse key)
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // world bounds of all instances
+    std::vector<uint32_t> records;  // leaf records per instance
+    std::vector<uint32_t> modes;    // geometry mode per instance
 };
 
 namespace {
@@ -356,11 +359,14 @@ int ensure_neural_scratch(vkhrt_context* c, size_t npix) {
     if (npix <= c->cap_neural) return VKHRT_SUCCESS;
     hipFree(c->origins);
     hipFree(c->fallback);
+    hipFree(c->dirs);
     c->origins = nullptr;
     c->fallback = nullptr;
+    c->dirs = nullptr;
     c->cap_neural = 0;
     hipError_t e = hipMalloc(&c->origins, npix * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(&c->fallback, npix * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->dirs, npix * sizeof(float4));
     if (e != hipSuccess) return hip_fail(c, e, "neural shadow scratch");
     c->cap_neural = npix;
     return VKHRT_SUCCESS;
@@ -508,6 +514,7 @@ void vkhrt_context_destroy(vkhrt_context* c) {
     hipFree(c->keys);
     hipFree(c->origins);
     hipFree(c->fallback);
+    hipFree(c->dirs);
     free_batch_scratch(c);
     hipFree(c->b_views);
     hipFree(c->b_frame);
@@ -759,6 +766,10 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
     t->ctx = c;
     t->count = count;
     t->serial = ++g_tlas_serial;
+    for (uint32_t i = 0; i < count; ++i) {
+        t->records.push_back(blases[i]->records);
+        t->modes.push_back(blases[i]->mode);
+    }
     for (int k = 0; k < 3; ++k) {
         t->lo[k] = INFINITY;
         t->hi[k] = -INFINITY;
@@ -851,10 +862,26 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
                            VKHRT_RENDER_FP64_POLISH | VKHRT_RENDER_NEURAL_SHADOWS;
     if (d->flags & ~known) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown render flags");
     const size_t npix = size_t(d->width) * d->height;
-    int rc = ensure_render_scratch(c, npix);
+    const vkhrt_area_light* al = d->light_visibility ? nullptr : d->area_light;
+    const uint32_t S = al ? al->samples : 1u;
+    if (al) {
+        if (S < 1 || S > 16) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "area light samples must be 1..16");
+        if (t->variant == vk::kVarMixed || !c->persistent || (is_phantom(t) && !t->polish_ok))
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "area lights need the persistent kernels and a single-mode TLAS");
+        if (npix * S > (size_t(1) << 31)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "too many shadow rays");
+    }
+    int rc = ensure_render_scratch(c, npix * S);
     if (rc != VKHRT_SUCCESS) return rc;
     hipStream_t st = pick(c, stream);
     vk::RenderParams rp{};
+    rp.light.samples = S;
+    if (al) {
+        rp.light.enabled = 1;
+        std::memcpy(rp.light.c, al->center, 12);
+        std::memcpy(rp.light.u, al->edge_u, 12);
+        std::memcpy(rp.light.v, al->edge_v, 12);
+        rp.light.seed = al->seed;
+    }
     rp.tp = vk::TraceParams{t->dev, t->count, (d->flags & VKHRT_RENDER_FP64_POLISH) ? uint32_t(vk::kTraceFp64Polish) : 0u,
                             c->status, c->persistent && (t->polish_ok || !is_phantom(t)) ? c->counters + 1 : nullptr,
                             c->refill_min};
@@ -880,7 +907,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
         if (!t->polish_ok || t->variant == vk::kVarMixed)
             return fail(c, VKHRT_ERROR_UNSUPPORTED,
                         "render FP64 polish needs a single-mode TLAS, < 2^26 primitives per BLAS, <= 31 instances");
-        rc = ensure_polish_scratch(c, npix);
+        rc = ensure_polish_scratch(c, npix * S);
         if (rc != VKHRT_SUCCESS) return rc;
         set_polish(c, rp.tp);
         if (!rp.tp.work) rp.tp.work = c->counters + 1;
@@ -894,9 +921,10 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
             return fail(c, VKHRT_ERROR_UNSUPPORTED, "neural shadows need the persistent kernels and a single-mode TLAS");
         if (!(d->shadow_variance_threshold >= 0.0f))
             return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "variance threshold must be >= 0");
-        rc = ensure_neural_scratch(c, npix);
+        rc = ensure_neural_scratch(c, npix * S);
         if (rc != VKHRT_SUCCESS) return rc;
         rp.mlp = &d->shadow_mlp->dev;
+        if (al) rp.dirs = c->dirs;
         rp.mlp_threshold = d->shadow_variance_threshold;
         rp.origins = c->origins;
         rp.fallback_queue = c->fallback;
@@ -916,6 +944,36 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     c->last_neural = neural;
     c->last_stream = st;
     return VKHRT_SUCCESS;
+}
+
+int vkhrt_generate_surface_shadow_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_area_light* light,
+                                       uint32_t count, uint64_t seed, vkhrt_ray* rays, void* stream) {
+    if (!c || !t || (count && !rays)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (t->count == 0 || t->count > vk::kSurfaceMaxInstances)
+        return fail(c, VKHRT_ERROR_UNSUPPORTED, "surface sampling needs 1..64 instances");
+    vk::SurfaceParams sp{};
+    sp.inst = t->dev;
+    sp.ninst = t->count;
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < t->count; ++i) {
+        if (t->modes[i] != VKHRT_GEOMETRY_PHANTOM_CURVES)
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "surface sampling needs phantom-curve instances");
+        sp.prefix[i] = uint32_t(total);
+        total += t->records[i];
+    }
+    if (total == 0 || total >= (1ull << 32)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "no curve records");
+    sp.prefix[t->count] = uint32_t(total);
+    sp.light.samples = 1;
+    sp.light.seed = seed;
+    if (light) {
+        sp.light.enabled = 1;
+        std::memcpy(sp.light.c, light->center, 12);
+        std::memcpy(sp.light.u, light->edge_u, 12);
+        std::memcpy(sp.light.v, light->edge_v, 12);
+    }
+    sp.seed = seed;
+    const hipError_t e = vk::launch_surface_shadow_rays(sp, count, reinterpret_cast<float*>(rays), pick(c, stream));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "surface shadow rays");
 }
 
 int vkhrt_render_stats_get(vkhrt_context* c, vkhrt_render_stats* out) {

@@ -30,26 +30,11 @@ namespace {
 
 __host__ __device__ __forceinline__ uint32_t bc(uint32_t depth, uint32_t word) { return depth * kBatchDepthWords + word; }
 
-// SplitMix64 output function (Steele, Lea, Flood, OOPSLA 2014).
-__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-// Uniform sample in [0, 1) with 24 random bits: number k of depth b of path p.
-__device__ __forceinline__ float path_rand(unsigned long long seed, uint32_t p, uint32_t b, uint32_t k) {
-    const unsigned long long key =
-        (static_cast<unsigned long long>(p) << 24) | (static_cast<unsigned long long>(b) << 16) | k;
-    return float(uint32_t(mix64((seed ^ key) + 0x9E3779B97F4A7C15ull) >> 40)) * 0x1p-24f;
-}
-
 constexpr uint32_t kDiskTries = 16;
 
 // Cosine-distributed unit direction about the unit normal n: a uniform point of the unit
 // disk by rejection (at most kDiskTries pairs; the centre if all fail) lifted to the
-// hemisphere (Malley's method), in the orthonormal basis of Duff et al., "Building an
-// Orthonormal Basis, Revisited", JCGT 6(1), 2017.
+// hemisphere (Malley's method), in the orthonormal basis of onb_duff.
 __device__ __forceinline__ f3 cosine_dir(f3 n, unsigned long long seed, uint32_t p, uint32_t b) {
     float x = 0.0f, y = 0.0f, r2 = 0.0f;
     for (uint32_t k = 0; k < kDiskTries; ++k) {
@@ -64,11 +49,8 @@ __device__ __forceinline__ f3 cosine_dir(f3 n, unsigned long long seed, uint32_t
         }
     }
     const float z = sqrtf(1.0f - r2);
-    const float sg = copysignf(1.0f, n.z);
-    const float a = -1.0f / (sg + n.z);
-    const float bb = n.x * n.y * a;
-    const f3 t1{1.0f + sg * n.x * n.x * a, sg * bb, -sg * n.x};
-    const f3 t2{bb, sg + n.y * n.y * a, -n.y};
+    f3 t1, t2;
+    onb_duff(n, t1, t2);
     const f3 w{(x * t1.x + y * t2.x) + z * n.x, (x * t1.y + y * t2.y) + z * n.y, (x * t1.z + y * t2.z) + z * n.z};
     return nrm3(w);
 }

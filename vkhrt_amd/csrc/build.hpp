@@ -52,6 +52,16 @@ struct Camera {
 
 struct MlpDev;
 
+// Light of the render's shadow rays: the directional light toward (0, 1, 0) (one ray per
+// hit), or a parallelogram c + a u + b v (a, b in [-1, 1]) sampled `samples` times per hit at
+// points fixed by (seed, pixel, sample).
+struct AreaLight {
+    float c[3], u[3], v[3];
+    uint32_t samples;  // shadow rays per hit pixel (1 with the directional light)
+    uint32_t enabled;  // 0: directional
+    unsigned long long seed;
+};
+
 struct RenderParams {
     TraceParams tp;
     Camera cam;
@@ -74,6 +84,8 @@ struct RenderParams {
     uint32_t* fallback_queue;  // pixels left to the exact shadow ray ([5] of counters = length)
     uint32_t* live;            // persistent pipeline: pixels whose primary ray enters an instance box
                                // ([6] of counters = length); the rest are recorded as misses up front
+    AreaLight light;           // shadow queue entries are ids pixel * light.samples + sample
+    float4* dirs;              // neural form with an area light: shadow-ray direction per queue entry
 };
 
 // Compile-time geometry variants of the traversal kernels. A TLAS whose instances share
@@ -158,6 +170,20 @@ constexpr uint32_t kRayBins = 4096;
 
 hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, uint2* stats,
                              int variant, int stack_kind, hipStream_t st);
+
+// Shadow rays from points on phantom-curve surfaces: record g uniform over all instances'
+// records (prefix[i] = records before instance i, prefix[ninst] = total), curve parameter and
+// angle about the fibre uniform; toward a point of `light` (or along (0, 1, 0)).
+constexpr uint32_t kSurfaceMaxInstances = 64;
+struct SurfaceParams {
+    const InstanceDev* inst;
+    uint32_t ninst;
+    uint32_t prefix[kSurfaceMaxInstances + 1];
+    AreaLight light;
+    unsigned long long seed;
+};
+hipError_t launch_surface_shadow_rays(const SurfaceParams& sp, uint32_t count, float* rays, hipStream_t st);
+
 hipError_t launch_batch(const BatchParams& bp, int variant, int stack_kind, hipStream_t st);
 hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hipStream_t st,
                          hipEvent_t* ev /*[4] or null*/);

@@ -196,9 +196,16 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
             x[0] = (o.x - c.x) * ih.x;
             x[1] = (o.y - c.y) * ih.y;
             x[2] = (o.z - c.z) * ih.z;
-            x[3] = sp.dir[0];
-            x[4] = sp.dir[1];
-            x[5] = sp.dir[2];
+            if (sp.dirs) {
+                const float4 d = sp.dirs[q];
+                x[3] = d.x;
+                x[4] = d.y;
+                x[5] = d.z;
+            } else {
+                x[3] = sp.dir[0];
+                x[4] = sp.dir[1];
+                x[5] = sp.dir[2];
+            }
         }
         s4 enc[6];
         encode(x, lane >> 4, enc);

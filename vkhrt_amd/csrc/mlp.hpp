@@ -36,14 +36,15 @@ struct AdamParams {
 };
 
 struct MlpShadowParams {
-    const uint32_t* queue;    // shadow pixels
+    const uint32_t* queue;    // shadow query ids (pixel * light samples + sample); vis is indexed by id
     const uint32_t* count;    // their number (device)
     const float4* origins;    // shadow-ray origins per queue entry
     float dir[3];             // light direction
     float threshold;          // exact fallback when p (1 - p) > threshold
     float* vis;               // per-pixel visibility
-    uint32_t* fallback_queue; // pixels left to the exact shadow ray
+    uint32_t* fallback_queue; // ids left to the exact shadow ray
     uint32_t* fallback_count;
+    const float4* dirs;       // per queue entry directions (area light), null = `dir` for all
 };
 
 size_t mlp_train_lds_bytes();

@@ -117,6 +117,44 @@ __global__ void k_camera_rays(Camera cam, float tmin, float tmax, float4* __rest
     rays[size_t(pix) * 2 + 1] = make_float4(d.x, d.y, d.z, tmax);
 }
 
+// Training rays of the shadow MLP: a point on a phantom fibre's surface (record, curve
+// parameter t and angle about the axis uniform; radius as the intersector's per-point model)
+// toward a light point. Random numbers: key (sample i, slot 1) for the surface, the light's
+// own (i, 0) pair, and one 64-bit draw for the record.
+__global__ void k_surface_shadow_rays(SurfaceParams sp, uint32_t count, float4* __restrict__ rays) {
+    using namespace trace_detail;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const unsigned long long r = mix64((sp.seed ^ (static_cast<unsigned long long>(i) << 24)) + 0x9E3779B97F4A7C15ull +
+                                       0x632BE59BD9B4E019ull);
+    const uint32_t g = uint32_t(__umul64hi(r, sp.prefix[sp.ninst]));
+    uint32_t k = 0;
+    while (k + 1 < sp.ninst && sp.prefix[k + 1] <= g) ++k;
+    const InstanceDev& I = sp.inst[k];
+    const float4* cv = I.blas.prims + size_t(g - sp.prefix[k]) * 4;
+    const float4 c0 = cv[0], c1 = cv[1], c2 = cv[2], c3 = cv[3];
+    const f3 p0{c0.x, c0.y, c0.z}, p1{c1.x, c1.y, c1.z}, p2{c2.x, c2.y, c2.z}, p3{c3.x, c3.y, c3.z};
+    const float t = path_rand(sp.seed, i, 1, 0);
+    const float turn = path_rand(sp.seed, i, 1, 1);
+    const f3 P = bez_point(p0, p1, p2, p3, t);
+    const float rad = I.blas.radius_mode == kRadiusPerPoint ? bez_scalar(c0.w, c1.w, c2.w, c3.w, t) : I.blas.radius;
+    f3 t1, t2;
+    onb_duff(nrm3(bez_axis(p0, p1, p2, p3, t)), t1, t2);
+    float sn, cs;
+    sincospif(2.0f * turn, &sn, &cs);
+    f3 x = add(P, scl(rad, add(scl(cs, t1), scl(sn, t2))));
+    if (!I.identity) {
+        const float* m = I.xf;
+        x = f3{m[0] * x.x + m[1] * x.y + m[2] * x.z + m[3], m[4] * x.x + m[5] * x.y + m[6] * x.z + m[7],
+               m[8] * x.x + m[9] * x.y + m[10] * x.z + m[11]};
+    }
+    f3 d;
+    float tmax;
+    light_ray(sp.light, i, 0, x, d, tmax);
+    rays[size_t(i) * 2] = make_float4(x.x, x.y, x.z, 0.001f);
+    rays[size_t(i) * 2 + 1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
 // Shadow rays toward `dir` from every hit: origin = o + t d (FP32, as the render kernels).
 // Compacted with wave-aggregated atomics; src[i] records the source ray of each output.
 __global__ void k_shadow_rays(const float4* __restrict__ rays, const HitRec* __restrict__ hits, uint32_t n, float3 dir,
@@ -272,6 +310,13 @@ __global__ void k_count_candidates(TraceParams tp, const float4* __restrict__ ra
 }
 
 }  // namespace
+
+hipError_t launch_surface_shadow_rays(const SurfaceParams& sp, uint32_t count, float* rays, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_surface_shadow_rays, dim3((count + 255) / 256), dim3(256), 0, st, sp, count,
+                       reinterpret_cast<float4*>(rays));
+    return hipGetLastError();
+}
 
 hipError_t launch_camera_rays(const Camera& cam, float tmin, float tmax, float* rays, hipStream_t st) {
     const uint32_t n = cam.width * cam.height;

@@ -357,6 +357,46 @@ __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, f
     return ok;
 }
 
+// SplitMix64 output function (Steele, Lea, Flood, OOPSLA 2014).
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Uniform sample in [0, 1) with 24 random bits: number k of slot b of item p (a path's depth,
+// a hit's light sample).
+__device__ __forceinline__ float path_rand(unsigned long long seed, uint32_t p, uint32_t b, uint32_t k) {
+    const unsigned long long key =
+        (static_cast<unsigned long long>(p) << 24) | (static_cast<unsigned long long>(b) << 16) | k;
+    return float(uint32_t(mix64((seed ^ key) + 0x9E3779B97F4A7C15ull) >> 40)) * 0x1p-24f;
+}
+
+// Tangents t1, t2 of the unit vector n (Duff et al., "Building an Orthonormal Basis,
+// Revisited", JCGT 6(1), 2017).
+__device__ __forceinline__ void onb_duff(f3 n, f3& t1, f3& t2) {
+    const float sg = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sg + n.z);
+    const float bb = n.x * n.y * a;
+    t1 = f3{1.0f + sg * n.x * n.x * a, sg * bb, -sg * n.x};
+    t2 = f3{bb, sg + n.y * n.y * a, -n.y};
+}
+
+// Shadow ray of light sample s of pixel `pix` from the hit point o: direction and tmax.
+__device__ __forceinline__ void light_ray(const AreaLight& L, uint32_t pix, uint32_t s, f3 o, f3& d, float& tmax) {
+    if (!L.enabled) {
+        d = f3{0.0f, 1.0f, 0.0f};
+        tmax = 10000.0f;
+        return;
+    }
+    const float a = 2.0f * path_rand(L.seed, pix, s, 0) - 1.0f;
+    const float b = 2.0f * path_rand(L.seed, pix, s, 1) - 1.0f;
+    const f3 y{L.c[0] + a * L.u[0] + b * L.v[0], L.c[1] + a * L.u[1] + b * L.v[1], L.c[2] + a * L.u[2] + b * L.v[2]};
+    const f3 w = sub(y, o);
+    d = nrm3(w);
+    tmax = sqrtf(dot3(w, w));
+}
+
 // Resolves the hit record of `best` (normal in world space).
 __device__ __forceinline__ void finish_hit(const TraceParams& tp, f3 o, f3 d, const Best& best, HitRec& h) {
     if (best.inst == 0xFFFFFFFFu) {
@@ -1049,18 +1089,20 @@ struct SrcCamera {
         tmax = 10000.0f;
     }
 };
+// Shadow queue entries are ids pixel * samples + light sample.
 struct SrcShadow {
     Camera cam;
     const uint32_t* queue;
     const float4* hitbuf;
+    AreaLight light;
     __device__ __forceinline__ void get(uint32_t qi, f3& o, f3& d, float& tmin, float& tmax) const {
-        const uint32_t pix = queue[qi];
+        const uint32_t id = queue[qi];
+        const uint32_t pix = light.samples == 1 ? id : id / light.samples;
         f3 po, pd;
         primary_ray(cam, pix % cam.width, pix / cam.width, po, pd);
         o = add(po, scl(hitbuf[pix].w, pd));
-        d = f3{0.0f, 1.0f, 0.0f};
+        light_ray(light, pix, id - pix * light.samples, o, d, tmax);
         tmin = 0.001f;
-        tmax = 10000.0f;
     }
 };
 
@@ -1152,12 +1194,13 @@ __global__ void k_polish_shadow(TraceParams tp, SrcShadow src, float* __restrict
 // slot; slots follow thread order within the block (one counter update per block instead of
 // one per wavefront keeps the single queue counter from serializing large grids). Called by
 // all threads of the block.
+// With m > 1 each flagged thread gets m consecutive slots (the returned one is the first).
 constexpr int kAppendBlock = 1024;
-__device__ __forceinline__ uint32_t block_append(bool flag, uint32_t* counter) {
+__device__ __forceinline__ uint32_t block_append(bool flag, uint32_t* counter, uint32_t m = 1) {
     __shared__ uint32_t s_wave[kAppendBlock / 64 + 1];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint64_t m = __ballot(flag);
-    if (lane == 0) s_wave[wave] = uint32_t(__popcll(m));
+    const uint64_t bal = __ballot(flag);
+    if (lane == 0) s_wave[wave] = uint32_t(__popcll(bal));
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t nw = (blockDim.x + 63u) / 64u;
@@ -1167,10 +1210,10 @@ __device__ __forceinline__ uint32_t block_append(bool flag, uint32_t* counter) {
             s_wave[w] = sum;
             sum += c;
         }
-        s_wave[kAppendBlock / 64] = sum ? atomicAdd(counter, sum) : 0u;
+        s_wave[kAppendBlock / 64] = sum ? atomicAdd(counter, sum * m) : 0u;
     }
     __syncthreads();
-    const uint32_t slot = s_wave[kAppendBlock / 64] + s_wave[wave] + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+    const uint32_t slot = s_wave[kAppendBlock / 64] + (s_wave[wave] + uint32_t(__popcll(bal & ((1ull << lane) - 1ull)))) * m;
     __syncthreads();
     return slot;
 }
@@ -1251,8 +1294,10 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams r
             rp.hitbuf[pix] = make_float4(n.x, n.y, n.z, h.t);
         }
     }
-    const uint32_t slot = block_append(hit, &rp.counters[0]);
-    if (hit) rp.queue[slot] = pix;
+    const uint32_t S = rp.light.samples;
+    const uint32_t slot = block_append(hit, &rp.counters[0], S);
+    if (hit)
+        for (uint32_t k = 0; k < S; ++k) rp.queue[slot + k] = pix * S + k;
 }
 
 // Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
@@ -1261,7 +1306,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
                                                                      const uint32_t* count, uint32_t* work) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t total = *count;
-    const SrcShadow src{rp.cam, queue, rp.hitbuf};
+    const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
     persistent_lanes<G, true, STACK>(
         rp.tp, total, work, stk + threadIdx.x,
         [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
@@ -1273,25 +1318,33 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
         });
 }
 
-// Shadow-ray origins of the queued pixels (the MLP's positions).
+// Shadow-ray origins (and, with an area light, directions) of the queued ids: the MLP's inputs.
 __global__ void k_shadow_origins(RenderParams rp) {
     const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= rp.counters[0]) return;
-    const SrcShadow src{rp.cam, rp.queue, rp.hitbuf};
+    const SrcShadow src{rp.cam, rp.queue, rp.hitbuf, rp.light};
     f3 o, d;
     float tmin, tmax;
     src.get(qi, o, d, tmin, tmax);
     rp.origins[qi] = make_float4(o.x, o.y, o.z, 0.0f);
+    if (rp.dirs) rp.dirs[qi] = make_float4(d.x, d.y, d.z, 0.0f);
 }
 
-// Shades the queued hit pixels with a per-pixel visibility (neural form, or the deferred
-// pipeline's shadow result).
+// Shades the queued hit pixels with their light visibility: the mean over the pixel's light
+// samples (ids pixel * S + s; the entry of sample 0 shades the pixel). A caller's per-pixel
+// visibility (light_visibility) comes with S = 1.
 __global__ void k_render_apply_visibility(RenderParams rp, const float* __restrict__ vis) {
     const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= rp.counters[0]) return;
-    const uint32_t pix = rp.queue[qi];
+    const uint32_t S = rp.light.samples;
+    const uint32_t id = rp.queue[qi];
+    const uint32_t pix = S == 1 ? id : id / S;
+    if (id != pix * S) return;
+    float v = vis[id];
+    for (uint32_t k = 1; k < S; ++k) v = v + vis[id + k];
+    if (S > 1) v = v / float(S);
     const float4 hb = rp.hitbuf[pix];
-    store_px(rp, pix, shade(f3{hb.x, hb.y, hb.z}, vis[pix]));
+    store_px(rp, pix, shade(f3{hb.x, hb.y, hb.z}, v));
 }
 
 __global__ void k_zero(uint32_t* p, uint32_t n) {
@@ -1393,6 +1446,8 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
     const uint32_t npix = rp.cam.width * rp.cam.height;
     if (npix == 0) return hipSuccess;
     const dim3 grid((npix + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
+    // the shadow queue holds up to npix * samples ids
+    const dim3 sgrid(uint32_t((uint64_t(npix) * rp.light.samples + kTraceBlock - 1) / kTraceBlock));
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, rp.counters, 8u);
     if constexpr (!G::kMixed) {
         if (rp.tp.work) {
@@ -1427,10 +1482,10 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                     if (tp.defer) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
                     if (ev) hipEventRecord(ev[2], st);
                     if (rp.mlp) {
-                        hipLaunchKernelGGL(k_shadow_origins, grid, block, 0, st, rp);
+                        hipLaunchKernelGGL(k_shadow_origins, sgrid, block, 0, st, rp);
                         MlpShadowParams sp{rp.queue, rp.counters, rp.origins, {0.0f, 1.0f, 0.0f}, rp.mlp_threshold,
-                                           rp.vis, rp.fallback_queue, rp.counters + 5};
-                        const hipError_t e = launch_mlp_shadow_gate(*rp.mlp, sp, npix, st);
+                                           rp.vis, rp.fallback_queue, rp.counters + 5, rp.dirs};
+                        const hipError_t e = launch_mlp_shadow_gate(*rp.mlp, sp, npix * rp.light.samples, st);
                         if (e != hipSuccess) return e;
                         squeue = rp.fallback_queue;
                         scount = rp.counters + 5;
@@ -1441,9 +1496,9 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                     if (ev) hipEventRecord(ev[3], st);
                     shadow_timed = true;
                     if (tp.defer)
-                        hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp, SrcShadow{rp.cam, squeue, rp.hitbuf},
-                                           rp.vis);
-                    hipLaunchKernelGGL(k_render_apply_visibility, grid, block, 0, st, rp, rp.vis);
+                        hipLaunchKernelGGL(k_polish_shadow, pgrid, block, 0, st, tp,
+                                           SrcShadow{rp.cam, squeue, rp.hitbuf, rp.light}, rp.vis);
+                    hipLaunchKernelGGL(k_render_apply_visibility, sgrid, block, 0, st, rp, rp.vis);
                 }
             }
             if (ev && !shadow_timed) {
