@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--no-neural", action="store_true", help="skip the neural shadow form")
     ap.add_argument("--no-modes", action="store_true", help="skip the LSS and DOTS forms of the frame")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view path-traced form")
+    ap.add_argument("--no-area", action="store_true", help="skip the 16M-segment area-light neural form")
     ap.add_argument("--autotune", type=int, default=1, help="pick BVH width, leaf size, stack kind, blocks per CU, "
                     "refill minimum and ray order with the autotuner before timing (1) or use the flags (0)")
     ap.add_argument("--neural-views", type=int, default=32, help="training views for the shadow MLP")
@@ -179,6 +180,10 @@ def main():
     if not args.no_batch:
         batch = batch_block(args, ctx, sp, stream, torch, V)
 
+    area = None
+    if not args.no_area:
+        area = neural_area_block(args, ctx, sp, stream, torch, np, V)
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -216,6 +221,7 @@ def main():
         "neural": neural,
         "geometry_modes": modes,
         "batch": batch,
+        "neural_area_light": area,
     }
     if parity is not None:
         out["parity_sample"] = parity
@@ -568,6 +574,135 @@ def batch_block(args, ctx, sp, stream, torch, V):
         "value": variants[best]["mrays_per_s"], "unit": "Mrays/s (primary + bounce + shadow rays)",
         "ms_per_frame": variants[best]["ms_per_frame"], "best": best, "variants": variants,
         "fp64_polish": True, "autotune": tuned,
+    }
+
+
+AREA_LIGHT = ((0.0, 260.0, 0.0), (100.0, 0.0, 0.0), (0.0, 0.0, 100.0))  # center, edge u, edge v
+
+
+def neural_area_block(args, ctx, sp, stream, torch, np, V):
+    """Neural shadow form on the 500k-strand x 32-segment model (16M segments) at 1920x1080
+    with 2 area-light samples per hit. The MLP is trained (Adam, BF16 MFMA forward/backward) on
+    2^24 shadow rays from points drawn uniformly on the strand surfaces toward light points,
+    labelled by the exact tracer; judged on 2^20 held-out rays (another seed); then the frame
+    is rendered with the variance-gated MLP (exact fallback above the threshold) against the
+    exact frame."""
+    ns, nseg, seed = 500000, 32, 104
+    n = ns * nseg
+    verts = torch.empty(ns * (nseg + 1) * 4, dtype=torch.float32, device="cuda")
+    lines = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+    curves = torch.empty(n * 16, dtype=torch.float32, device="cuda")
+    ctx.generate_strands(seed, ns, nseg, verts.data_ptr(), stream=sp)
+    ctx.generate_lines(verts.data_ptr(), ns, nseg, lines.data_ptr(), stream=sp)
+    ctx.generate_curves(lines.data_ptr(), n, curves.data_ptr(), stream=sp)
+    del verts, lines
+    blas = V.BottomLevelAccelerationStructure(ctx, V.GEOMETRY_PHANTOM_CURVES, curves.data_ptr(), n,
+                                              flags=V.BLAS_PER_POINT_RADIUS, stream=sp)
+    tlas = V.TopLevelAccelerationStructure(ctx, [blas], stream=sp)
+    cv = curves.view(-1, 4, 4)[:, :, :3].reshape(-1, 3)
+    lo, hi = cv.min(0).values.tolist(), cv.max(0).values.tolist()
+    del cv
+    torch.cuda.synchronize()
+    W, H = args.width, args.height
+    cam = V.camera_look_at(CAMERA_EYE, CAMERA_TARGET, fov_deg=60.0, aspect=W / H)
+    light = V.area_light(*AREA_LIGHT, samples=2, seed=7)
+    flags = V.RENDER_SHADOWS | V.RENDER_RGBA8 | (V.RENDER_FP64_POLISH if args.polish else 0)
+    image = torch.empty(W * H, dtype=torch.int32, device="cuda")
+
+    def timed(**kw):
+        nflags = flags | (V.RENDER_NEURAL_SHADOWS if kw else 0)
+        for _ in range(args.warmup):
+            ctx.render(tlas, cam, W, H, image.data_ptr(), flags=nflags, stream=sp, area_light=light, **kw)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            ctx.render(tlas, cam, W, H, image.data_ptr(), flags=nflags, stream=sp, area_light=light, **kw)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ctx.synchronize(sp)
+        return e0.elapsed_time(e1) / args.steps, ctx.render_stats()
+
+    exact_ms, (prim, queries, _) = timed()
+    log(f"[bench] area light: 16M segments, exact {exact_ms:.3f} ms/frame, {prim} primary + {queries} shadow rays "
+        f"-> {(prim + queries) / (exact_ms * 1e-3) / 1e6:.1f} Mrays/s")
+
+    pad = [0.02 * (b - a) + 1.0 for a, b in zip(lo, hi)]
+    mlp = V.ShadowMLP(ctx, [a - p for a, p in zip(lo, pad)], [b + p for b, p in zip(hi, pad)], seed=1)
+    ntrain, nheld = 1 << 24, 1 << 20
+
+    def data(count, s):
+        rays = torch.empty(count * 8, dtype=torch.float32, device="cuda")
+        ctx.generate_surface_shadow_rays(tlas, count, s, rays.data_ptr(), light=light, stream=sp)
+        smp = torch.empty(count * 8, dtype=torch.float32, device="cuda")
+        tgt = torch.empty(count, dtype=torch.float32, device="cuda")
+        mlp.collect(tlas, rays.data_ptr(), count, smp.data_ptr(), tgt.data_ptr(), stream=sp)
+        torch.cuda.synchronize()
+        return smp, tgt
+
+    t0 = time.perf_counter()
+    smp, tgt = data(ntrain, 1)
+    collect_s = time.perf_counter() - t0
+    epochs = 24
+    t0 = time.perf_counter()
+    loss = mlp.train(smp.data_ptr(), tgt.data_ptr(), ntrain, batch=65536, epochs=epochs, learning_rate=3e-3, stream=sp)
+    train_s = time.perf_counter() - t0
+    del smp, tgt
+    hs, ht = data(nheld, 2)
+    p = torch.empty(nheld, dtype=torch.float32, device="cuda")
+    mlp.infer(hs.data_ptr(), nheld, p.data_ptr(), stream=sp)
+    torch.cuda.synchronize()
+    y = ht.double()
+    pd = p.double().clamp(1e-7, 1 - 1e-7)
+    held = {"rays": nheld, "lit_fraction": round(float(y.mean()), 4),
+            "bce": round(float(-(y * pd.log() + (1 - y) * (1 - pd).log()).mean()), 4),
+            "accuracy": round(float(((pd > 0.5).double() == y).double().mean()), 4),
+            "mean_abs_error": round(float((pd - y).abs().mean()), 4)}
+    var = pd * (1 - pd)
+    for thr in (0.05, 0.15):
+        conf = var <= thr
+        held[f"confident_fraction_at_{thr}"] = round(float(conf.double().mean()), 4)
+        held[f"error_with_fallback_at_{thr}"] = round(float(((pd - y).abs() * conf).mean()), 4)
+    del hs, ht, p
+    log(f"[bench] area light: trained on 2^24 surface rays ({collect_s:.2f} s to trace) for {epochs} epochs in "
+        f"{train_s:.2f} s, loss {loss:.4f}; held-out {held}")
+
+    fimg = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    fexact = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    ctx.render(tlas, cam, W, H, fexact.data_ptr(), flags=flags & ~V.RENDER_RGBA8, stream=sp, area_light=light)
+    torch.cuda.synchronize()
+    sweep, chosen = [], None
+    for thr in (0.25, 0.2, 0.15, 0.1, 0.05, 0.02, 0.0):
+        ctx.render(tlas, cam, W, H, fimg.data_ptr(), flags=(flags & ~V.RENDER_RGBA8) | V.RENDER_NEURAL_SHADOWS,
+                   shadow_mlp=mlp, variance_threshold=thr, stream=sp, area_light=light)
+        torch.cuda.synchronize()
+        mse = float(((fimg.view(H, W, 4)[..., :3] - fexact.view(H, W, 4)[..., :3]) ** 2).mean())
+        psnr = float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)
+        _, q, tr = ctx.render_stats()
+        sweep.append({"threshold": thr, "psnr_db": round(psnr, 2) if math.isfinite(psnr) else None,
+                      "fallback_fraction": round(tr / max(q, 1), 4)})
+        if psnr >= 40.0 and chosen is None:
+            chosen = (thr, psnr)
+    thr, psnr = chosen
+    neural_ms, (_, q, traced) = timed(shadow_mlp=mlp, variance_threshold=thr)
+    log(f"[bench] area light neural: threshold {thr} PSNR {psnr:.2f} dB, fallback {traced}/{q}, {neural_ms:.3f} "
+        f"ms/frame (exact {exact_ms:.3f}) -> {exact_ms / neural_ms:.3f}x")
+    mlp.close()
+    tlas.close()
+    blas.close()
+    return {
+        "workload": "phantom_500k_strands_x32_segments_1920x1080_area_light_2_samples",
+        "segments": n, "light_samples": 2,
+        "exact": {"ms_per_step": round(exact_ms, 4), "primary_rays": prim, "shadow_rays": queries,
+                  "mrays_per_s": round((prim + queries) / (exact_ms * 1e-3) / 1e6, 1)},
+        "neural": {"ms_per_step": round(neural_ms, 4), "variance_threshold": thr,
+                   "psnr_db_vs_exact": round(psnr, 2) if math.isfinite(psnr) else None,
+                   "shadow_rays_traced": traced, "shadow_queries": q,
+                   "frame_speedup_vs_exact": round(exact_ms / neural_ms, 3), "threshold_sweep": sweep},
+        "train": {"samples": ntrain, "epochs": epochs, "batch": 65536, "seconds": round(train_s, 2),
+                  "trace_seconds": round(collect_s, 2), "final_loss": round(loss, 5),
+                  "approx_tflops": round(3 * 36992 * ntrain * epochs / train_s / 1e12, 2)},
+        "held_out": held,
     }
 
 
