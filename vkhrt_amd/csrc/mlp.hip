@@ -454,13 +454,18 @@ hipError_t launch_mlp_infer(const MlpDev& m, const float* samples, uint32_t n, f
 
 hipError_t launch_mlp_shadow_gate(const MlpDev& m, const MlpShadowParams& sp, uint32_t max_count, hipStream_t st) {
     if (max_count == 0) return hipSuccess;
-    // one block per CU: each wavefront walks many 16-sample groups, so its fallback buffer
-    // reaches the queue with few atomics
-    int dev = 0, cus = 0;
+    // as many resident blocks as the LDS weights allow (latency hiding for the per-sample loads
+    // and the encoding); each wavefront still walks many 16-sample groups, so its fallback
+    // buffer reaches the queue with few atomics
+    int dev = 0, cus = 0, per_cu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(mlp_detail::k_shadow_gate), 256,
+                                                 0);
+    per_cu = std::max(1, std::min(per_cu, 4));
     const uint32_t groups = (max_count + 15u) / 16u;
-    const uint32_t blocks = std::max(1u, std::min<uint32_t>((groups + 3u) / 4u, uint32_t(std::max(cus, 1))));
+    const uint32_t blocks =
+        std::max(1u, std::min<uint32_t>((groups + 3u) / 4u, uint32_t(std::max(cus, 1) * per_cu)));
     hipLaunchKernelGGL(mlp_detail::k_shadow_gate, dim3(blocks), dim3(256), 0, st, m, sp);
     return hipGetLastError();
 }

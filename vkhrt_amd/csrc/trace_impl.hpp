@@ -1038,6 +1038,8 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
 template <class G, bool ANY, int STACK, class Src, class Snk>
 __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t total, uint32_t* counter, uint2* lds,
                                                  Src&& src, Snk&& snk) {
+    // an empty queue (uniform over the grid) ends the launch without touching the counter
+    if (total == 0) return;
     if constexpr (G::kMode == kPhantom)
         persistent_compact<G, ANY, STACK>(tp, total, counter, lds, src, snk);
     else
