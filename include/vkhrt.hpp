@@ -175,6 +175,37 @@ public:
     void Render(const vkhrt_camera& camera, const vkhrt_render_desc& desc, void* stream = nullptr) const {
         _context->Check(vkhrt_render(_context->Handle(), _tlas, &camera, &desc, stream), "vkhrt_render");
     }
+    // Extensions (no reference counterpart): batched path tracing of many views, its stats, the
+    // autotuner on a render or batch workload, and MLP training rays from the hair surface.
+    void RenderBatch(const std::vector<vkhrt_view>& views, const vkhrt_batch_desc& desc, void* stream = nullptr) const {
+        _context->Check(vkhrt_render_batch(_context->Handle(), _tlas, views.data(), uint32_t(views.size()), &desc, stream),
+                        "vkhrt_render_batch");
+    }
+    vkhrt_batch_stats BatchStats() const {
+        vkhrt_batch_stats st{};
+        _context->Check(vkhrt_batch_stats_get(_context->Handle(), &st), "vkhrt_batch_stats_get");
+        return st;
+    }
+    vkhrt_tuning AutotuneRender(const vkhrt_camera& camera, const vkhrt_render_desc& desc, uint32_t repeats = 3,
+                                void* stream = nullptr) const {
+        vkhrt_tuning t{};
+        _context->Check(vkhrt_autotune_render(_context->Handle(), _tlas, &camera, &desc, repeats, &t, stream),
+                        "vkhrt_autotune_render");
+        return t;
+    }
+    vkhrt_tuning AutotuneBatch(const std::vector<vkhrt_view>& views, const vkhrt_batch_desc& desc,
+                               uint32_t repeats = 3, void* stream = nullptr) const {
+        vkhrt_tuning t{};
+        _context->Check(vkhrt_autotune_batch(_context->Handle(), _tlas, views.data(), uint32_t(views.size()), &desc,
+                                             repeats, &t, stream),
+                        "vkhrt_autotune_batch");
+        return t;
+    }
+    void SurfaceShadowRays(const vkhrt_area_light* light, uint32_t count, uint64_t seed, vkhrt_ray* rays,
+                           void* stream = nullptr) const {
+        _context->Check(vkhrt_generate_surface_shadow_rays(_context->Handle(), _tlas, light, count, seed, rays, stream),
+                        "vkhrt_generate_surface_shadow_rays");
+    }
 
 private:
     vkhrt_tlas* _tlas = nullptr;

@@ -3,6 +3,7 @@
 //       error behaviour without a GPU: Context -> NO_DEVICE, BLASType / geometry pairing,
 //       CameraLookAt; prints "ok".
 //   host_api_test trace <curves.bin> <n> <rays.bin> <m> <hits.bin> [instances]
+//   host_api_test batch <curves.bin> <n> <cams.bin> <nviews> <W1xH1,...> <bounces> <seed> <out.bin>
 //       builds `instances` phantom BLASes (per-point radius, host pointers) with transforms
 //       x -> x + 10 * i, one TLAS, traces m rays (closest hit) and writes m vkhrt_hit records.
 #include <hip/hip_runtime_api.h>
@@ -94,9 +95,74 @@ static int trace(int argc, char** argv) {
     return 0;
 }
 
+// Batched path tracing through the C++ mirror: views from cams.bin (nviews x 32 floats: view
+// inverse, projection inverse) with sizes "W1xH1,W2xH2,...", RGBA32F images concatenated to out.
+static int batch(int argc, char** argv) {
+    if (argc < 9) return 2;
+    const size_t n = std::strtoul(argv[3], nullptr, 10), nviews = std::strtoul(argv[5], nullptr, 10);
+    std::vector<float> curves = read_floats(argv[2], n * 16), cams = read_floats(argv[4], nviews * 32);
+    std::vector<std::pair<uint32_t, uint32_t>> sizes;
+    for (const char* p = argv[6]; *p;) {
+        char* e = nullptr;
+        const uint32_t w = uint32_t(std::strtoul(p, &e, 10));
+        const uint32_t h = uint32_t(std::strtoul(e + 1, &e, 10));
+        sizes.emplace_back(w, h);
+        p = *e ? e + 1 : e;
+    }
+    if (sizes.size() != nviews) return 2;
+    auto ctx = std::make_shared<vkhrt::Context>(0);
+    std::vector<vkhrt::BottomLevelAccelerationStructure> blases;
+    vkhrt::BLASInput in;
+    in.type = vkhrt::BLASType::eHair;
+    in.geometry = VKHRT_GEOMETRY_PHANTOM_CURVES;
+    in.primitives = curves.data();
+    in.count = uint32_t(n);
+    in.perPointRadius = true;
+    in.hostPointers = true;
+    blases.emplace_back(in, ctx);
+    vkhrt::TopLevelAccelerationStructure tlas(blases, ctx);
+    std::vector<vkhrt_view> views(nviews);
+    size_t total = 0;
+    for (size_t v = 0; v < nviews; ++v) total += size_t(sizes[v].first) * sizes[v].second;
+    void* dimg = nullptr;
+    if (hipMalloc(&dimg, total * 16) != hipSuccess) return 3;
+    size_t off = 0;
+    for (size_t v = 0; v < nviews; ++v) {
+        std::memcpy(views[v].camera.view_inverse, &cams[v * 32], 64);
+        std::memcpy(views[v].camera.proj_inverse, &cams[v * 32 + 16], 64);
+        views[v].width = sizes[v].first;
+        views[v].height = sizes[v].second;
+        views[v].image = static_cast<char*>(dimg) + off * 16;
+        views[v].primitive_ids = nullptr;
+        off += size_t(sizes[v].first) * sizes[v].second;
+    }
+    vkhrt_batch_desc d{};
+    d.flags = VKHRT_BATCH_FP64_POLISH | VKHRT_BATCH_GRAPH;
+    d.bounces = uint32_t(std::atoi(argv[7]));
+    d.seed = std::strtoull(argv[8], nullptr, 10);
+    d.environment[0] = 0.2f;
+    d.environment[1] = 0.3f;
+    d.environment[2] = 0.4f;
+    tlas.RenderBatch(views, d);
+    ctx->Synchronize();
+    const vkhrt_batch_stats st = tlas.BatchStats();
+    std::vector<float> img(total * 4);
+    if (hipMemcpy(img.data(), dimg, total * 16, hipMemcpyDeviceToHost) != hipSuccess || hipFree(dimg) != hipSuccess)
+        return 3;
+    FILE* f = std::fopen(argv[9], "wb");
+    if (!f || std::fwrite(img.data(), 4, img.size(), f) != img.size()) return 4;
+    std::fclose(f);
+    std::printf("batch: %llu paths, %llu bounce rays, %llu shadow rays\n", (unsigned long long)st.paths,
+                (unsigned long long)st.bounce_rays, (unsigned long long)st.shadow_rays);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::strcmp(argv[1], "nodevice") == 0) return no_device();
     if (argc >= 2 && std::strcmp(argv[1], "trace") == 0) return trace(argc, argv);
-    std::fprintf(stderr, "usage: host_api_test nodevice | trace curves n rays m hits [instances]\n");
+    if (argc >= 2 && std::strcmp(argv[1], "batch") == 0) return batch(argc, argv);
+    std::fprintf(stderr,
+                 "usage: host_api_test nodevice | trace curves n rays m hits [instances]\n"
+                 "       | batch curves n cams nviews WxH,... bounces seed out\n");
     return 2;
 }

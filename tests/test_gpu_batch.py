@@ -244,3 +244,29 @@ def test_batch_errors(gpu_ctx, oracle, scene):
     with pytest.raises(V.VkhrtError) as e:
         _gpu_batch(gpu_ctx, mixed, views, 1, 1)
     assert e.value.code == -4
+
+
+def test_cpp_mirror_batch(tmp_path, oracle):
+    """The C++ mirror (TopLevelAccelerationStructure::RenderBatch, graph mode, FP64 polish) renders
+    the same images as the oracle, bit for bit."""
+    import subprocess
+    import scenes
+    from test_host import cpp_test_binary
+    curves = scenes.dense_ball_curves(nstrands=900, nseg=8)
+    views = _views(oracle, scenes.DENSE_VIEWS[:2])
+    cams = np.concatenate([np.concatenate([vi, pi]) for vi, pi, _, _ in views]).astype(np.float32)
+    cpath, vpath, opath = tmp_path / "c.bin", tmp_path / "v.bin", tmp_path / "o.bin"
+    curves.astype(np.float32).tofile(cpath)
+    cams.tofile(vpath)
+    sizes = ",".join(f"{w}x{h}" for _, _, w, h in views)
+    print("[cpp_mirror_batch] 2 views, 3 bounces", flush=True)
+    r = subprocess.run([cpp_test_binary(), "batch", str(cpath), str(curves.shape[0]), str(vpath), "2", sizes, "3",
+                        "41", str(opath)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(opath, np.float32).reshape(-1, 4)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    ri, _, nb, ns, _ = S.render_batch(views, bounces=3, seed=41, env=ENV, flags=oracle.F_POLISH)
+    ref = np.concatenate([x.reshape(-1, 3) for x in ri])
+    assert f"{nb} bounce rays, {ns} shadow rays" in r.stdout
+    assert np.array_equal(got[:, :3].view(np.uint32), ref.view(np.uint32))
