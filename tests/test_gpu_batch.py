@@ -270,3 +270,28 @@ def test_cpp_mirror_batch(tmp_path, oracle):
     ref = np.concatenate([x.reshape(-1, 3) for x in ri])
     assert f"{nb} bounce rays, {ns} shadow rays" in r.stdout
     assert np.array_equal(got[:, :3].view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("bounces", [0, 8])
+def test_batch_edge_views(gpu_ctx, oracle, scene, bounces):
+    """1x1 and odd-sized views, a view that sees no hair (every path adds the environment), the
+    bounce extremes, and a repeated call with the same seed: all bit for bit equal to the oracle
+    and to the first call."""
+    import scenes
+    S, tlas = scene
+    specs = [(scenes.DENSE_VIEWS[0][0], scenes.DENSE_VIEWS[0][1], 0.577, (1, 1)),
+             (scenes.DENSE_VIEWS[1][0], scenes.DENSE_VIEWS[1][1], 0.4, (7, 3)),
+             ((0.0, 20.0, 45.0), (0.0, 20.0, 500.0), 0.3, (16, 9)),  # looking away from the hair
+             (scenes.DENSE_VIEWS[2][0], scenes.DENSE_VIEWS[2][1], 0.5, (33, 17))]
+    views = _views(oracle, specs)
+    print(f"[batch edge views] bounces {bounces}", flush=True)
+    gi, gp, (paths, nb, ns, _) = _gpu_batch(gpu_ctx, tlas, views, bounces, 9)
+    gi2, gp2, st2 = _gpu_batch(gpu_ctx, tlas, views, bounces, 9)
+    ri, rp, rnb, rns, _ = S.render_batch(views, bounces=bounces, seed=9, env=ENV, albedo=TINTS)
+    assert paths == 1 + 21 + 144 + 561 and (nb, ns) == (rnb, rns) == st2[1:3]
+    _assert_same(gi, gp, ri, rp, f"edge views, bounces {bounces}")
+    assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(gi, gi2))
+    away = gi[2][..., :3]
+    assert np.all(away == np.float32(ENV)) and np.all(gp[2] == 0xFFFFFFFF)
+    if bounces == 0:
+        assert nb == 0

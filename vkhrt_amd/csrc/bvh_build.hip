@@ -8,7 +8,7 @@
 //   3. stable LSD radix sort of (key, index), 8 bits per pass: LDS histogram, one-block
 //      scan, LDS-ranked stable scatter (wave ballots rank equal digits)
 //   4. Karras (HPG 2012) binary radix tree, one thread per internal node
-//   5. bottom-up box refit, second-arriving thread continues (acq_rel atomics)
+//   5. bottom-up box refit in level-synchronous rounds (one launch per tree level)
 //   6. top-down collapse to 8-wide nodes, one level per launch: each binary subtree is
 //      opened greedily by largest surface area until `width` (<= 8) children; subtrees with at most
 //      `leaf_max` primitives become leaf children. Child slots are ordered per ray octant;
@@ -273,13 +273,6 @@ __global__ void k_karras(const uint64_t* __restrict__ keys, int n, uint2* __rest
     if (i == 0) parent_int[0] = 0xFFFFFFFFu;
 }
 
-__device__ __forceinline__ float ld_acq(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel(float* p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ void child_box(uint32_t ref, const float4* blo, const float4* bhi, const uint32_t* vals,
                                           const float* nlo, const float* nhi, float lo[3], float hi[3]) {
     if (ref & kLeafFlag) {
@@ -293,33 +286,34 @@ __device__ __forceinline__ void child_box(uint32_t ref, const float4* blo, const
         hi[2] = b.z;
     } else {
         for (int k = 0; k < 3; ++k) {
-            lo[k] = ld_acq(&nlo[ref * 3 + k]);
-            hi[k] = ld_acq(&nhi[ref * 3 + k]);
+            lo[k] = nlo[ref * 3 + k];
+            hi[k] = nhi[ref * 3 + k];
         }
     }
 }
 
-__global__ void k_refit(int n, const uint2* __restrict__ child, const uint32_t* __restrict__ parent_int,
-                        const uint32_t* __restrict__ parent_leaf, const uint32_t* __restrict__ vals,
-                        const float4* __restrict__ blo, const float4* __restrict__ bhi, float* nlo, float* nhi,
-                        uint32_t* counters) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    uint32_t node = parent_leaf[j];
-    while (node != 0xFFFFFFFFu) {
-        uint32_t prev = __hip_atomic_fetch_add(&counters[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;  // the other child is not done yet
-        uint2 c = child[node];
-        float alo[3], ahi[3], blo3[3], bhi3[3];
-        child_box(c.x, blo, bhi, vals, nlo, nhi, alo, ahi);
-        child_box(c.y, blo, bhi, vals, nlo, nhi, blo3, bhi3);
-        for (int k = 0; k < 3; ++k) {
-            st_rel(&nlo[node * 3 + k], fminf(alo[k], blo3[k]));
-            st_rel(&nhi[node * 3 + k], fmaxf(ahi[k], bhi3[k]));
-        }
-        __threadfence();
-        node = parent_int[node];
+// Level-synchronous refit, one round per launch: an internal node whose children were
+// finished in earlier rounds (leaves always are) gets its box and records the round. Rounds =
+// tree height. The kernel boundary makes the children's boxes visible, so the rounds need no
+// atomics or fences (a per-node climb needs agent-scope release/acquire pairs, which write back
+// and invalidate L2 across the XCDs at every step).
+__global__ void k_refit_round(int nint, const uint2* __restrict__ child, const uint32_t* __restrict__ vals,
+                              const float4* __restrict__ blo, const float4* __restrict__ bhi, float* __restrict__ nlo,
+                              float* __restrict__ nhi, uint32_t* __restrict__ done, uint32_t round) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nint || done[i]) return;
+    const uint2 c = child[i];
+    const bool ra = (c.x & kLeafFlag) || (done[c.x] != 0u && done[c.x] < round);
+    const bool rb = (c.y & kLeafFlag) || (done[c.y] != 0u && done[c.y] < round);
+    if (!(ra && rb)) return;
+    float alo[3], ahi[3], blo3[3], bhi3[3];
+    child_box(c.x, blo, bhi, vals, nlo, nhi, alo, ahi);
+    child_box(c.y, blo, bhi, vals, nlo, nhi, blo3, bhi3);
+    for (int k = 0; k < 3; ++k) {
+        nlo[i * 3 + k] = fminf(alo[k], blo3[k]);
+        nhi[i * 3 + k] = fmaxf(ahi[k], bhi3[k]);
     }
+    done[i] = round;
 }
 
 // ------------------------------------------------------------------ collapse to 8-wide
@@ -662,8 +656,21 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     if (n >= 2) {
         hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1)), dim3(kBlock), 0, st, k0, int(n), child, range, par_i, par_l);
         hipLaunchKernelGGL(k_set_u32, dim3(blocks(n)), dim3(kBlock), 0, st, counters, 0u, n);
-        hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, st, int(n), child, par_i, par_l, v0, blo, bhi,
-                           nlo, nhi, counters);
+        // rounds in groups of 8; the root is the last node to finish
+        uint32_t root_done = 0;
+        for (uint32_t round = 1; !root_done; round += 8) {
+            if (round > 8192) {
+                err = "BVH refit did not reach the root";
+                rc = hipErrorUnknown;
+                goto fail;
+            }
+            for (uint32_t r = round; r < round + 8; ++r)
+                hipLaunchKernelGGL(k_refit_round, dim3(blocks(n - 1)), dim3(kBlock), 0, st, int(n - 1), child, v0, blo,
+                                   bhi, nlo, nhi, counters, r);
+            VK_TRY(hipGetLastError());
+            VK_TRY(hipMemcpyAsync(&root_done, counters, 4, hipMemcpyDeviceToHost, st));
+            VK_TRY(hipStreamSynchronize(st));
+        }
     }
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, cnt, 0u, 4u);
     hipLaunchKernelGGL(k_root_task, dim3(1), dim3(1), 0, st, ta, cnt + 0, cnt + 2, n >= 2 ? 0u : kLeafFlag);
