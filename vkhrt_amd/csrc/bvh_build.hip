@@ -54,13 +54,16 @@ __device__ __forceinline__ float box_pad(float lo[3], float hi[3]) {
     return 1e-4f + 4e-7f * m;
 }
 
-__global__ void k_prim_boxes(const float4* __restrict__ prims, uint32_t n, int mode, int radius_mode, float radius,
-                             float4* __restrict__ blo, float4* __restrict__ bhi, uint32_t* __restrict__ cbounds) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Grid-stride over the records (a fixed grid), so the centroid bounds take one set of atomics
+// per block: per-wave atomics on the same 6 words serialized large builds.
+__global__ __launch_bounds__(kBlock) void k_prim_boxes(const float4* __restrict__ prims, uint32_t n, int mode,
+                                                       int radius_mode, float radius, float4* __restrict__ blo,
+                                                       float4* __restrict__ bhi, uint32_t* __restrict__ cbounds) {
+    float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     float c[3] = {0, 0, 0};
-    bool valid = i < n;
-    if (valid) {
+    {
         if (mode == kPhantom) {
             float4 p[4];
             for (int k = 0; k < 4; ++k) p[k] = prims[size_t(i) * 4 + k];
@@ -110,14 +113,34 @@ __global__ void k_prim_boxes(const float4* __restrict__ prims, uint32_t n, int m
         blo[i] = make_float4(lo[0], lo[1], lo[2], 0.0f);
         bhi[i] = make_float4(hi[0], hi[1], hi[2], 0.0f);
     }
-    // centroid bounds: wave reduction then one atomic per wave and axis
     for (int k = 0; k < 3; ++k) {
-        float mn = valid ? c[k] : INFINITY, mx = valid ? c[k] : -INFINITY;
+        cmn[k] = fminf(cmn[k], c[k]);
+        cmx[k] = fmaxf(cmx[k], c[k]);
+    }
+    }
+    // centroid bounds: wave reduction, block reduction in LDS, one atomic per block and word
+    __shared__ float s_mn[kBlock / 64][3], s_mx[kBlock / 64][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 3; ++k) {
+        float mn = cmn[k], mx = cmx[k];
         for (int off = 32; off > 0; off >>= 1) {
             mn = fminf(mn, __shfl_xor(mn, off));
             mx = fmaxf(mx, __shfl_xor(mx, off));
         }
-        if ((threadIdx.x & 63) == 0) {
+        if (lane == 0) {
+            s_mn[wave][k] = mn;
+            s_mx[wave][k] = mx;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        float mn = s_mn[0][k], mx = s_mx[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            mn = fminf(mn, s_mn[w][k]);
+            mx = fmaxf(mx, s_mx[w][k]);
+        }
+        if (mn <= mx) {
             atomicMin(&cbounds[k], f2ord(mn));
             atomicMax(&cbounds[3 + k], f2ord(mx));
         }
@@ -191,6 +214,52 @@ __global__ void k_scan_single(uint32_t* __restrict__ data, uint32_t count) {
         data[i] = run;
         run += v;
     }
+}
+
+// Exclusive prefix sum over `count` words in three launches: tile sums (4096 words per block,
+// coalesced uint4 loads), one block scans the sums, then each block scans its tile from its
+// offset.
+constexpr uint32_t kScanTile = 4096;
+__device__ __forceinline__ uint4 load4(const uint32_t* d, uint32_t i, uint32_t count) {
+    if (i + 3 < count && (i & 3u) == 0u) return *reinterpret_cast<const uint4*>(d + i);
+    return make_uint4(i < count ? d[i] : 0u, i + 1 < count ? d[i + 1] : 0u, i + 2 < count ? d[i + 2] : 0u,
+                      i + 3 < count ? d[i + 3] : 0u);
+}
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* __restrict__ data, uint32_t count,
+                                                     uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_w[16];
+    const uint4 v = load4(data, blockIdx.x * kScanTile + threadIdx.x * 4u, count);
+    uint32_t t = v.x + v.y + v.z + v.w;
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < 16; ++w) sum += s_w[w];
+        sums[blockIdx.x] = sum;
+    }
+}
+__global__ __launch_bounds__(1024) void k_scan_apply(uint32_t* __restrict__ data, uint32_t count,
+                                                     const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_part[1024];
+    const uint32_t i = blockIdx.x * kScanTile + threadIdx.x * 4u;
+    const uint4 v = load4(data, i, count);
+    const uint32_t t = v.x + v.y + v.z + v.w;
+    s_part[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = sums[blockIdx.x] + s_part[threadIdx.x] - t;
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (uint32_t k = 0; k < 4; ++k)
+        if (i + k < count) {
+            data[i + k] = run;
+            run += w[k];
+        }
 }
 
 __global__ void k_rs_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
@@ -593,7 +662,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     float4 *blo = nullptr, *bhi = nullptr;
     uint32_t* cb = nullptr;
     uint64_t *k0 = nullptr, *k1 = nullptr;
-    uint32_t *v0 = nullptr, *v1 = nullptr, *hist = nullptr;
+    uint32_t *v0 = nullptr, *v1 = nullptr, *hist = nullptr, *scan_sums = nullptr;
     uint2 *child = nullptr, *range = nullptr;
     uint32_t *par_i = nullptr, *par_l = nullptr, *counters = nullptr;
     float *nlo = nullptr, *nhi = nullptr;
@@ -624,6 +693,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     VK_TRY(dalloc(&v1, n));
     nb = blocks(n, kRsTile);
     VK_TRY(dalloc(&hist, size_t(nb) * 256));
+    VK_TRY(dalloc(&scan_sums, (size_t(nb) * 256 + kScanTile - 1) / kScanTile));
     VK_TRY(dalloc(&child, n));
     VK_TRY(dalloc(&range, n));
     VK_TRY(dalloc(&par_i, n));
@@ -640,14 +710,19 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     if (mode == kPhantom) VK_TRY(dalloc(&aux, n));
 
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, cb);
-    hipLaunchKernelGGL(k_prim_boxes, dim3(blocks(n)), dim3(kBlock), 0, st, prims_in, n, mode, radius_mode, radius, blo,
-                       bhi, cb);
+    hipLaunchKernelGGL(k_prim_boxes, dim3(std::min<uint32_t>(blocks(n), 2048u)), dim3(kBlock), 0, st, prims_in, n, mode,
+                       radius_mode, radius, blo, bhi, cb);
     hipLaunchKernelGGL(k_morton, dim3(blocks(n)), dim3(kBlock), 0, st, blo, bhi, n, cb, k0, v0);
     VK_TRY(hipGetLastError());
     for (int pass = 0; pass < 8; ++pass) {
         int shift = pass * 8;
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kBlock), 0, st, k0, n, shift, hist, nb);
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, hist, nb * 256u);
+        {
+            const uint32_t count = nb * 256u, tiles = (count + kScanTile - 1) / kScanTile;
+            hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(1024), 0, st, hist, count, scan_sums);
+            hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, scan_sums, tiles);
+            hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(1024), 0, st, hist, count, scan_sums);
+        }
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kBlock), 0, st, k0, v0, k1, v1, n, shift, hist, nb);
         std::swap(k0, k1);
         std::swap(v0, v1);
@@ -728,6 +803,7 @@ done:
     hipFree(v0);
     hipFree(v1);
     hipFree(hist);
+    hipFree(scan_sums);
     hipFree(child);
     hipFree(range);
     hipFree(par_i);
