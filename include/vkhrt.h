@@ -61,7 +61,8 @@ typedef enum vkhrt_status {
 typedef enum vkhrt_geometry_mode {
     VKHRT_GEOMETRY_PHANTOM_CURVES = 0,
     VKHRT_GEOMETRY_LSS = 1,
-    VKHRT_GEOMETRY_DOTS = 2
+    VKHRT_GEOMETRY_DOTS = 2,
+    VKHRT_GEOMETRY_TRIANGLES = 3 /* indexed triangle mesh (BLASType::eMesh, renderer.cpp:543-582) */
 } vkhrt_geometry_mode;
 
 /* vkhrt_blas_desc.flags */
@@ -77,6 +78,10 @@ typedef enum vkhrt_geometry_mode {
  *                   (renderer.cpp:664-674): a start cap equal to the previous end is shared
  *   DOTS:           8 floats = line (start.xyz, r0, end.xyz, r1); 2 orthogonal quads
  *                   (4 triangles) per line are generated; primitive id = 4 * line + triangle
+ *   TRIANGLES:      `primitives` holds vertex_count vertices of 8 floats (position, normal,
+ *                   texCoord: the Vertex of model.hpp:44-49) and `indices` 3 u32 per triangle;
+ *                   count = triangles; the hit reports the interpolated vertex normal
+ *                   (triangle_closest_hit.rchit:31-36) and barycentrics u, v
  * radius: constant radius used when VKHRT_BLAS_PER_POINT_RADIUS is not set (reference 0.02).
  * transform: row-major 3x4 world-from-object (BLASInput.transform, Node::GetWorldMatrix). */
 typedef struct vkhrt_blas_desc {
@@ -88,6 +93,8 @@ typedef struct vkhrt_blas_desc {
     uint32_t material_index;
     uint32_t leaf_size; /* primitives per leaf child, 1..3; 0 = default (1) */
     float transform[12];
+    const uint32_t* indices; /* TRIANGLES: 3 x count vertex indices (u32) */
+    uint32_t vertex_count;   /* TRIANGLES: vertices in `primitives` */
 } vkhrt_blas_desc;
 
 typedef struct vkhrt_blas_info {

@@ -91,6 +91,8 @@ struct BLASInput {
     bool hostPointers = false;
     uint32_t materialIndex = 0;
     uint32_t leafSize = 0;
+    const uint32_t* indices = nullptr;  // VKHRT_GEOMETRY_TRIANGLES: 3 per triangle (count = triangles)
+    uint32_t vertexCount = 0;           // VKHRT_GEOMETRY_TRIANGLES: vertices in `primitives`
 };
 
 class BottomLevelAccelerationStructure {
@@ -103,7 +105,8 @@ public:
             throw Error(VKHRT_ERROR_UNSUPPORTED, "BottomLevelAccelerationStructure: voxel BLAS is not on this path");
         if ((input.type == BLASType::eHair) != hair)
             throw Error(VKHRT_ERROR_INVALID_ARGUMENT,
-                        "BottomLevelAccelerationStructure: eHair pairs with phantom curves, eMesh with LSS / DOTS");
+                        "BottomLevelAccelerationStructure: eHair pairs with phantom curves, eMesh with LSS / DOTS / "
+                        "triangles");
         if (!context) throw Error(VKHRT_ERROR_INVALID_ARGUMENT, "BottomLevelAccelerationStructure: no context");
         vkhrt_blas_desc d{};
         d.mode = uint32_t(input.geometry);
@@ -114,6 +117,8 @@ public:
         d.radius = input.radius;
         d.material_index = input.materialIndex;
         d.leaf_size = input.leafSize;
+        d.indices = input.indices;
+        d.vertex_count = input.vertexCount;
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 4; ++c) d.transform[r * 4 + c] = input.transform[c * 4 + r];
         _context->Check(vkhrt_blas_build(_context->Handle(), &d, stream, &_blas), "vkhrt_blas_build");

@@ -29,7 +29,7 @@ using namespace orc;
 
 namespace {
 
-enum { MODE_PHANTOM = 0, MODE_LSS = 1, MODE_DOTS = 2 };
+enum { MODE_PHANTOM = 0, MODE_LSS = 1, MODE_DOTS = 2, MODE_TRIS = 3 };
 enum {
     F_ANY_HIT = 1u << 0,
     F_BRUTE = 1u << 1,
@@ -81,6 +81,7 @@ struct Blas {
     std::vector<float> prims;        // phantom: 16/curve, LSS: 8/capsule, DOTS: 8/line
     std::vector<uint8_t> cap_a;      // LSS chained caps
     std::vector<DotsPiece> pieces;   // DOTS
+    std::vector<float> tri, tnrm;    // triangle mesh: 3 vertex positions / normals per triangle
     uint32_t nprims = 0;             // primitive count (DOTS: triangles)
     std::vector<Box> boxes;
     std::vector<uint32_t> order;
@@ -113,6 +114,12 @@ void prim_box(const Blas& B, uint32_t i, Box& bx, bool pad = true) {
         for (int k = 0; k < 3; ++k) {
             bx.mn[k] = std::min(c[k] - c[3], c[4 + k] - c[7]);
             bx.mx[k] = std::max(c[k] + c[3], c[4 + k] + c[7]);
+        }
+    } else if (B.mode == MODE_TRIS) {
+        const float* v = &B.tri[size_t(i) * 9];
+        for (int k = 0; k < 3; ++k) {
+            bx.mn[k] = std::min(v[k], std::min(v[3 + k], v[6 + k]));
+            bx.mx[k] = std::max(v[k], std::max(v[3 + k], v[6 + k]));
         }
     } else {
         const DotsPiece& p = B.pieces[i / 4];
@@ -292,6 +299,24 @@ bool test_prim(const Blas& B, uint32_t p, const Ray<float>& ray, float tmin, flo
         h.n[2] = float(out.n.z);
         h.v = 0.0f;
         return true;
+    } else if (B.mode == MODE_TRIS) {
+        // triangle_closest_hit.rchit:31-36: the interpolated vertex normal
+        const float* v = &B.tri[size_t(p) * 9];
+        const float* nv = &B.tnrm[size_t(p) * 9];
+        const V3<float> v0 = mk(v[0], v[1], v[2]), v1 = mk(v[3], v[4], v[5]), v2 = mk(v[6], v[7], v[8]);
+        TriOut<float> out;
+        if (!IntersectTriangle(ray.origin, ray.direction, v0, v1, v2, tmin, tmax, out)) return false;
+        h.t = out.t;
+        h.u = out.b1;
+        h.v = out.b2;
+        const float b0 = 1.0f - out.b1 - out.b2;
+        const V3<float> s = mk(nv[0], nv[1], nv[2]) * b0 + mk(nv[3], nv[4], nv[5]) * out.b1 +
+                            mk(nv[6], nv[7], nv[8]) * out.b2;
+        const V3<float> n = dot(s, s) > 0.0f ? normalize(s) : normalize(cross(v1 - v0, v2 - v0));
+        h.n[0] = n.x;
+        h.n[1] = n.y;
+        h.n[2] = n.z;
+        return true;
     } else {
         const DotsPiece& pc = B.pieces[p / 4];
         V3<float> v0, v1, v2;
@@ -465,6 +490,26 @@ void parallel_for(uint32_t n, int nthreads, F f) {
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; ++t) th.emplace_back(work);
     for (auto& t : th) t.join();
+}
+
+// Appends an instance of B with an optional row-major 3x4 world-from-object transform.
+int add_instance(Scene& S, const std::shared_ptr<Blas>& B, const float* transform) {
+    Instance I;
+    I.blas = B;
+    I.identity = true;
+    if (transform) {
+        std::memcpy(I.xf, transform, 48);
+        const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        I.identity = std::memcmp(transform, id, 48) == 0;
+        M4<float> M{};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) M.m[c][r] = transform[r * 4 + c];
+        M.m[0][3] = M.m[1][3] = M.m[2][3] = 0.0f;
+        M.m[3][3] = 1.0f;
+        I.inv = inverse(M);
+    }
+    S.inst.push_back(I);
+    return int(S.inst.size() - 1);
 }
 
 inline Ray<float> load_ray(const float* r, float& tmin, float& tmax) {
@@ -851,22 +896,29 @@ int orc_scene_add_instance(orc_scene* sp, int mode, const float* prims, uint32_t
         return -1;
     }
     build_blas(*B, leaf_size > 0 ? leaf_size : 4);
-    Instance I;
-    I.blas = B;
-    I.identity = true;
-    if (transform) {
-        std::memcpy(I.xf, transform, 48);
-        const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-        I.identity = std::memcmp(transform, id, 48) == 0;
-        M4<float> M{};
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 4; ++c) M.m[c][r] = transform[r * 4 + c];
-        M.m[0][3] = M.m[1][3] = M.m[2][3] = 0.0f;
-        M.m[3][3] = 1.0f;
-        I.inv = inverse(M);
-    }
-    S.inst.push_back(I);
-    return int(S.inst.size() - 1);
+    return add_instance(S, B, transform);
+}
+
+// Indexed triangle mesh: verts = nverts x 8 floats (position, normal, texCoord), indices =
+// 3 x ntri. Returns the instance index, or -1 when an index is not below nverts.
+int orc_scene_add_mesh(orc_scene* sp, const float* verts, uint32_t nverts, const uint32_t* indices, uint32_t ntri,
+                       const float* transform, int leaf_size) {
+    auto B = std::make_shared<Blas>();
+    B->mode = MODE_TRIS;
+    B->nprims = ntri;
+    B->tri.resize(size_t(ntri) * 9);
+    B->tnrm.resize(size_t(ntri) * 9);
+    for (uint32_t t = 0; t < ntri; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = indices[size_t(t) * 3 + k];
+            if (v >= nverts) return -1;
+            for (int c = 0; c < 3; ++c) {
+                B->tri[size_t(t) * 9 + k * 3 + c] = verts[size_t(v) * 8 + c];
+                B->tnrm[size_t(t) * 9 + k * 3 + c] = verts[size_t(v) * 8 + 3 + c];
+            }
+        }
+    build_blas(*B, leaf_size > 0 ? leaf_size : 4);
+    return add_instance(*reinterpret_cast<Scene*>(sp), B, transform);
 }
 
 int orc_scene_set_material(orc_scene* sp, uint32_t inst, uint32_t material) {

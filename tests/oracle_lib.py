@@ -70,6 +70,7 @@ def lib():
         "orc_count_candidates": ([P, P, P, u32, P, i32], i32),
         "orc_render": ([P, P, P, u32, u32, u32, i32, P, P, i32, u32, P, P, u32, u64], ctypes.c_longlong),
         "orc_scene_set_material": ([P, u32, u32], i32),
+        "orc_scene_add_mesh": ([P, P, u32, P, u32, P, i32], i32),
         "orc_render_batch": ([P, P, P, u32, u32, u64, P, P, u32, P, P, P, i32, u32, P], ctypes.c_longlong),
         "orc_path_rand": ([u64, u32, u32, u32], f32),
         "orc_path_basis": ([P, P, P], i32),
@@ -201,6 +202,15 @@ class Scene:
         counts = np.zeros(rays.shape[0], np.uint32)
         lib().orc_count_candidates(self.h, ptr(rays), ptr(hits), rays.shape[0], ptr(counts), nthreads)
         return counts
+
+    def add_mesh(self, verts, indices, transform=None, leaf_size=4):
+        """verts: (nv, 8) position, normal, texCoord; indices: (nt, 3) u32."""
+        verts = f32(verts)
+        idx = np.ascontiguousarray(indices, dtype=np.uint32)
+        xf = None if transform is None else f32(np.asarray(transform).reshape(12))
+        r = lib().orc_scene_add_mesh(self.h, ptr(verts), verts.shape[0], ptr(idx), idx.shape[0], ptr(xf), leaf_size)
+        assert r >= 0, "mesh index out of range"
+        return r
 
     def set_material(self, inst, material):
         assert lib().orc_scene_set_material(self.h, inst, material) == 0

@@ -92,3 +92,25 @@ def grazing_rays(curves, n, seed, per_point=True, eps=2e-3, radius=0.02):
     rays[:, 4:7] = dd
     rays[:, 7] = 10000.0
     return rays
+
+
+def uv_sphere(radius=10.0, nlat=24, nlon=48, center=(0.0, 0.0, 0.0)):
+    """Latitude-longitude sphere mesh: vertices (position, radial normal, texCoord) as 8 floats,
+    and u32 triangle indices (the poles are fans)."""
+    c = np.asarray(center, np.float64)
+    verts = []
+    for i in range(nlat + 1):
+        th = np.pi * i / nlat
+        for j in range(nlon + 1):
+            ph = 2 * np.pi * j / nlon
+            n = np.array([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)])
+            verts.append([*(c + radius * n), *n, j / nlon, i / nlat])
+    idx = []
+    for i in range(nlat):
+        for j in range(nlon):
+            a, b = i * (nlon + 1) + j, (i + 1) * (nlon + 1) + j
+            if i != 0:
+                idx.append([a, b, a + 1])
+            if i != nlat - 1:
+                idx.append([a + 1, b, b + 1])
+    return np.asarray(verts, np.float32), np.asarray(idx, np.uint32)

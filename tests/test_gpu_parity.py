@@ -338,3 +338,86 @@ def test_cpp_mirror_trace(tmp_path):
     assert (ref["prim"] != 0xFFFFFFFF).mean() > 0.2
     assert np.array_equal(got["prim"], ref["prim"]) and np.array_equal(got["inst"], ref["inst"])
     assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))
+
+
+def _mesh_blas(ctx, G, verts, idx, transform=None, host=False):
+    import vkhrt_amd as V
+    if host:
+        return V.BottomLevelAccelerationStructure(ctx, V.GEOMETRY_TRIANGLES, verts.ctypes.data, idx.shape[0],
+                                                  flags=V.BLAS_HOST_POINTERS, transform=transform,
+                                                  indices_ptr=idx.ctypes.data, vertex_count=verts.shape[0])
+    dv, di = G.dev(verts), G.dev(idx)
+    b = V.BottomLevelAccelerationStructure(ctx, V.GEOMETRY_TRIANGLES, dv.data_ptr(), idx.shape[0], transform=transform,
+                                           indices_ptr=di.data_ptr(), vertex_count=verts.shape[0],
+                                           stream=G.stream_ptr())
+    torch.cuda.synchronize()
+    return b
+
+
+@pytest.mark.parametrize("host", [False, True])
+def test_mesh_trace_matches_oracle(gpu_ctx, G, oracle, curves, host):
+    """Triangle meshes (BLASType::eMesh): a mesh-only TLAS (the persistent mesh kernels) and a
+    mesh + hair TLAS (mixed kernels) trace bit for bit like the oracle, interpolated normals
+    included."""
+    import scenes
+    import vkhrt_amd as V
+    verts, idx = scenes.uv_sphere(30.0, 32, 64, (0.0, 40.0, 0.0))
+    xf = np.array([[0.0, -1, 0, 5], [1, 0, 0, 10], [0, 0, 1, -20]], np.float32)
+    rng = np.random.default_rng(6)
+    o = rng.uniform(-150, 150, (20000, 3))
+    d = rng.uniform(-40, 40, (20000, 3)) + np.array([0, 40, 0]) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, np.full((20000, 1), 0.001), d, np.full((20000, 1), 1e4)], 1).astype(np.float32)
+    print(f"[mesh trace] host pointers {host}", flush=True)
+    m0 = _mesh_blas(gpu_ctx, G, verts, idx, host=host)
+    m1 = _mesh_blas(gpu_ctx, G, verts, idx, transform=xf.reshape(12), host=host)
+    tl = V.TopLevelAccelerationStructure(gpu_ctx, [m0, m1])
+    S = oracle.Scene()
+    S.add_mesh(verts, idx)
+    S.add_mesh(verts, idx, transform=xf)
+    _compare_hits(G.trace(gpu_ctx, tl, rays), S.trace(rays), "mesh")
+    g_any = G.trace(gpu_ctx, tl, rays, V.TRACE_TERMINATE_ON_FIRST_HIT)
+    r_any = S.trace(rays, flags=oracle.F_ANY_HIT)
+    assert np.array_equal(g_any["prim"] != 0xFFFFFFFF, r_any["prim"] != 0xFFFFFFFF)
+    hair = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
+    mixed = V.TopLevelAccelerationStructure(gpu_ctx, [hair, m1])
+    S2 = oracle.Scene()
+    S2.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    S2.add_mesh(verts, idx, transform=xf)
+    _compare_hits(G.trace(gpu_ctx, mixed, rays), S2.trace(rays), "mesh + hair")
+
+
+def test_mesh_render_and_batch_match_oracle(gpu_ctx, G, oracle):
+    import scenes
+    import vkhrt_amd as V
+    verts, idx = scenes.uv_sphere(8.0, 24, 48, (0.0, 8.0, 0.0))
+    blas = _mesh_blas(gpu_ctx, G, verts, idx)
+    tl = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    S = oracle.Scene()
+    S.add_mesh(verts, idx)
+    W, H = 96, 64
+    vi, pi = oracle.camera((0.0, 20.0, 45.0), (0.0, 8.0, 0.0), aspect=W / H)
+    cam = V.Camera()
+    for k in range(16):
+        cam.view_inverse[k], cam.proj_inverse[k] = float(vi[k]), float(pi[k])
+    img, ids = G.render(gpu_ctx, tl, cam, W, H)
+    ref, rids, _, _ = S.render(vi, pi, W, H, shadows=1)
+    assert (rids != 0xFFFFFFFF).sum() > 250
+    assert np.array_equal(ids, rids) and np.array_equal(img[..., :3].view(np.uint32), ref.view(np.uint32))
+    out = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    gpu_ctx.render_batch(tl, [(cam, W, H, out.data_ptr())], bounces=2, seed=3, environment=(0.2, 0.3, 0.4),
+                         stream=G.stream_ptr())
+    torch.cuda.synchronize()
+    ri, _, _, _, _ = S.render_batch([(vi, pi, W, H)], bounces=2, seed=3, env=(0.2, 0.3, 0.4))
+    got = out.cpu().numpy().reshape(H, W, 4)[..., :3]
+    assert np.array_equal(got.view(np.uint32), ri[0].view(np.uint32))
+
+
+def test_mesh_rejects_out_of_range_indices(gpu_ctx, G):
+    import vkhrt_amd as V
+    verts = np.zeros((3, 8), np.float32)
+    verts[1, 0] = verts[2, 1] = 1.0
+    idx = np.array([[0, 1, 5]], np.uint32)
+    with pytest.raises(V.VkhrtError) as e:
+        _mesh_blas(gpu_ctx, G, verts, idx)
+    assert e.value.code == -1

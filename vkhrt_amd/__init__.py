@@ -16,7 +16,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvkhrt.so")
 
-GEOMETRY_PHANTOM_CURVES, GEOMETRY_LSS, GEOMETRY_DOTS = 0, 1, 2
+GEOMETRY_PHANTOM_CURVES, GEOMETRY_LSS, GEOMETRY_DOTS, GEOMETRY_TRIANGLES = 0, 1, 2, 3
 
 BLAS_HOST_POINTERS = 1 << 0
 BLAS_PER_POINT_RADIUS = 1 << 1
@@ -79,7 +79,8 @@ class BlasDesc(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("flags", ctypes.c_uint32),
                 ("primitives", ctypes.c_void_p), ("count", ctypes.c_uint32),
                 ("radius", ctypes.c_float), ("material_index", ctypes.c_uint32),
-                ("leaf_size", ctypes.c_uint32), ("transform", ctypes.c_float * 12)]
+                ("leaf_size", ctypes.c_uint32), ("transform", ctypes.c_float * 12),
+                ("indices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint32)]
 
 
 class BlasInfo(ctypes.Structure):
@@ -448,10 +449,13 @@ class BottomLevelAccelerationStructure:
     """
 
     def __init__(self, ctx, mode, primitives_ptr, count, radius=0.02, flags=0, transform=None,
-                 leaf_size=0, material_index=0, stream=None):
+                 leaf_size=0, material_index=0, stream=None, indices_ptr=None, vertex_count=0):
+        """GEOMETRY_TRIANGLES: `primitives_ptr` holds `vertex_count` vertices (position, normal,
+        texCoord = 8 floats), `indices_ptr` 3 u32 per triangle, `count` triangles."""
         d = BlasDesc()
         d.mode, d.flags, d.primitives, d.count = mode, flags, primitives_ptr, count
         d.radius, d.material_index, d.leaf_size = radius, material_index, leaf_size
+        d.indices, d.vertex_count = indices_ptr, vertex_count
         if transform is not None:
             d.flags |= BLAS_HAS_TRANSFORM
             for i, v in enumerate(list(transform)[:12]):

@@ -115,6 +115,7 @@ struct vkhrt_shadow_mlp {
 
 struct vkhrt_blas {
     vkhrt_context* ctx = nullptr;
+    float4* normals = nullptr;  // triangle meshes: 3 vertex normals per triangle
     uint32_t mode = 0;
     uint32_t radius_mode = 0;
     float radius = 0.02f;
@@ -591,8 +592,11 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
 int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, vkhrt_blas** out) {
     if (!c || !d || !out) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (d->mode > VKHRT_GEOMETRY_DOTS) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown geometry mode");
+    if (d->mode > VKHRT_GEOMETRY_TRIANGLES) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown geometry mode");
     if (d->count && !d->primitives) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null primitives");
+    const bool mesh = d->mode == VKHRT_GEOMETRY_TRIANGLES;
+    if (mesh && d->count && (!d->indices || d->vertex_count == 0))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "a triangle mesh needs indices and vertices");
     if (d->count > (1u << 28)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "too many primitives");
     if (d->leaf_size > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "leaf size must be 0..3");
     const bool per_point = (d->flags & VKHRT_BLAS_PER_POINT_RADIUS) != 0;
@@ -604,9 +608,12 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
     hipEventCreate(&e1);
     hipEventRecord(e0, st);
     const uint32_t in_floats = d->mode == VKHRT_GEOMETRY_PHANTOM_CURVES ? 16u : 8u;
-    const size_t in_bytes = size_t(d->count) * in_floats * sizeof(float);
+    const size_t in_bytes = (mesh ? size_t(d->vertex_count) : size_t(d->count)) * in_floats * sizeof(float);
+    const size_t idx_bytes = mesh ? size_t(d->count) * 3 * sizeof(uint32_t) : 0;
     float4* input = nullptr;
     float4* records = nullptr;
+    uint32_t* indices = nullptr;
+    uint32_t* bad = nullptr;
     bool own_input = false;
     vkhrt_blas* b = new (std::nothrow) vkhrt_blas();
     if (!b) return fail(c, VKHRT_ERROR_OUT_OF_MEMORY, "blas");
@@ -625,9 +632,12 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
         if (d->flags & VKHRT_BLAS_HOST_POINTERS) {
             e = hipMalloc(&input, in_bytes);
             if (e == hipSuccess) e = hipMemcpyAsync(input, d->primitives, in_bytes, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess && mesh) e = hipMalloc(&indices, idx_bytes);
+            if (e == hipSuccess && mesh) e = hipMemcpyAsync(indices, d->indices, idx_bytes, hipMemcpyHostToDevice, st);
             own_input = true;
         } else {
             input = reinterpret_cast<float4*>(const_cast<float*>(d->primitives));
+            indices = const_cast<uint32_t*>(d->indices);
         }
         if (e != hipSuccess) {
             rc = hip_fail(c, e, "blas upload");
@@ -638,6 +648,22 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
         } else if (d->mode == VKHRT_GEOMETRY_LSS) {
             e = hipMalloc(&records, size_t(d->count) * 2 * sizeof(float4));
             if (e == hipSuccess) e = vk::lss_chain(input, d->count, records, st);
+        } else if (mesh) {
+            uint32_t nbad = 0;
+            e = hipMalloc(&records, size_t(d->count) * 3 * sizeof(float4));
+            if (e == hipSuccess) e = hipMalloc(&b->normals, size_t(d->count) * 3 * sizeof(float4));
+            if (e == hipSuccess) e = hipMalloc(&bad, sizeof(uint32_t));
+            if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(uint32_t), st);
+            if (e == hipSuccess)
+                e = vk::gen_tri_records(reinterpret_cast<const float*>(input), d->vertex_count, indices, d->count,
+                                        records, b->normals, bad, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(&nbad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e == hipSuccess && nbad) {
+                rc = fail(c, VKHRT_ERROR_INVALID_ARGUMENT,
+                          std::to_string(nbad) + " triangle indices are not below the vertex count");
+                goto out;
+            }
         } else {
             e = hipMalloc(&records, size_t(d->count) * 4 * sizeof(float4));
             if (e == hipSuccess) e = vk::gen_dots(input, d->count, per_point ? -1.0f : d->radius, records, st);
@@ -682,10 +708,14 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
     b->info.node_count = b->bvh.nnodes;
     b->info.depth = b->bvh.depth;
     b->info.device_bytes = uint64_t(b->bvh.nnodes) * sizeof(vk::WideNode) + uint64_t(d->count) * 4u +
-                           uint64_t(d->count) * (d->mode == VKHRT_GEOMETRY_LSS ? 32u : 64u) +
+                           uint64_t(d->count) * (d->mode == VKHRT_GEOMETRY_LSS ? 32u : mesh ? 96u : 64u) +
                            (d->mode == VKHRT_GEOMETRY_PHANTOM_CURVES ? uint64_t(d->count) * 4u : 0u);
 out:
-    if (own_input) hipFree(input);
+    if (own_input) {
+        hipFree(input);
+        hipFree(indices);
+    }
+    hipFree(bad);
     if (records && records != input) hipFree(records);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -709,6 +739,7 @@ void vkhrt_blas_destroy(vkhrt_blas* b) {
     hipFree(b->bvh.prim_ids);
     hipFree(b->bvh.prims);
     hipFree(b->bvh.aux);
+    hipFree(b->normals);
     delete b;
 }
 
@@ -731,6 +762,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         I.blas.mode = b->mode;
         I.blas.radius_mode = b->radius_mode;
         I.blas.radius = b->radius;
+        I.blas.normals = b->normals;
         I.material = b->material_index;
         const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
         const float* xf = b->has_xf ? b->xf : ident;
@@ -786,10 +818,11 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         int v = -1;
         for (uint32_t i = 0; i < count; ++i) {
             const vkhrt_blas* b = blases[i];
-            int vi = b->mode == VKHRT_GEOMETRY_LSS    ? vk::kVarLss
-                     : b->mode == VKHRT_GEOMETRY_DOTS ? vk::kVarDots
-                     : b->radius_mode                 ? vk::kVarPhPoint
-                                                      : vk::kVarPhConst;
+            int vi = b->mode == VKHRT_GEOMETRY_LSS         ? vk::kVarLss
+                     : b->mode == VKHRT_GEOMETRY_DOTS      ? vk::kVarDots
+                     : b->mode == VKHRT_GEOMETRY_TRIANGLES ? vk::kVarTris
+                     : b->radius_mode                      ? vk::kVarPhPoint
+                                                           : vk::kVarPhConst;
             v = (v < 0 || v == vi) ? vi : vk::kVarMixed;
             t->max_depth = std::max(t->max_depth, b->bvh.depth);
             if (b->records >= (1u << 26)) t->polish_ok = false;

@@ -97,7 +97,8 @@ enum TraceVariant {
     kVarPhPointPolish = 3,
     kVarLss = 4,
     kVarDots = 5,
-    kVarMixed = 6
+    kVarMixed = 6,
+    kVarTris = 7
 };
 
 // ---- batched path tracing: many views of one TLAS in one wavefront pipeline

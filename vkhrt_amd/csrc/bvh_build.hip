@@ -87,6 +87,15 @@ __global__ __launch_bounds__(kBlock) void k_prim_boxes(const float4* __restrict_
             hi[0] = fmaxf(a.x + ra, b.x + rb);
             hi[1] = fmaxf(a.y + ra, b.y + rb);
             hi[2] = fmaxf(a.z + ra, b.z + rb);
+        } else if (mode == kTriangles) {
+            for (int v = 0; v < 3; ++v) {
+                const float4 q = prims[size_t(i) * 3 + v];
+                const float a[3] = {q.x, q.y, q.z};
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = fminf(lo[k], a[k]);
+                    hi[k] = fmaxf(hi[k], a[k]);
+                }
+            }
         } else {
             float4 P[4];
             for (int k = 0; k < 4; ++k) P[k] = prims[size_t(i) * 4 + k];
@@ -658,7 +667,7 @@ inline uint32_t blocks(uint64_t n, int b = kBlock) { return uint32_t((n + b - 1)
 
 hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mode, float radius, int leaf_max,
                      int width, hipStream_t st, BuiltBvh* out, std::string& err) {
-    const int rec = mode == kLss ? 2 : 4;
+    const int rec = mode == kLss ? 2 : mode == kTriangles ? 3 : 4;
     float4 *blo = nullptr, *bhi = nullptr;
     uint32_t* cb = nullptr;
     uint64_t *k0 = nullptr, *k1 = nullptr;

@@ -192,7 +192,36 @@ __global__ void k_lss_chain(const float4* __restrict__ in, uint32_t n, float4* _
 
 inline dim3 grid(uint64_t n) { return dim3(uint32_t((n + kBlock - 1) / kBlock)); }
 
+// Indexed mesh -> triangle records: 3 x float4 vertex positions per triangle and its 3 vertex
+// normals (the reference's Vertex is position, normal, texCoord = 8 floats; Indices are u32
+// triples, triangle_closest_hit.rchit:19-29). An index past the vertex count is replaced by 0
+// and counted in `bad`, so no read leaves the vertex buffer.
+__global__ void k_tri_records(const float* __restrict__ verts, uint32_t nverts, const uint32_t* __restrict__ idx,
+                              uint32_t ntri, float4* __restrict__ recs, float4* __restrict__ normals,
+                              uint32_t* __restrict__ bad) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t v = idx[size_t(t) * 3 + k];
+        if (v >= nverts) {
+            atomicAdd(bad, 1u);
+            v = 0;
+        }
+        const float* p = verts + size_t(v) * 8;
+        recs[size_t(t) * 3 + k] = make_float4(p[0], p[1], p[2], 0.0f);
+        normals[size_t(t) * 3 + k] = make_float4(p[3], p[4], p[5], 0.0f);
+    }
+}
+
 }  // namespace
+
+hipError_t gen_tri_records(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri, float4* recs,
+                           float4* normals, uint32_t* bad, hipStream_t st) {
+    if (ntri == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tri_records, dim3((ntri + 255) / 256), dim3(256), 0, st, verts, nverts, idx, ntri, recs, normals,
+                       bad);
+    return hipGetLastError();
+}
 
 hipError_t gen_strands(uint64_t seed, uint32_t nstrands, uint32_t nseg, const StrandParams& hp, float4* out,
                        hipStream_t st) {

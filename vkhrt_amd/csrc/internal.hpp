@@ -5,6 +5,7 @@
 //                 phantom curve : 4 x float4 (control point xyz, radius)      64 B
 //                 LSS capsule   : 2 x float4 (a xyz, +-ra ; b xyz, rb)        32 B
 //                 DOTS piece    : 4 x float4 (start,r0 ; end,r1 ; s ; t)      64 B
+//                 triangle      : 3 x float4 (vertex positions)               48 B
 //   prim_ids  : original primitive index per leaf slot (u32)
 //   nodes     : 8-wide BVH nodes with 8-bit quantized child boxes, 80 B each
 #pragma once
@@ -13,7 +14,7 @@
 
 namespace vk {
 
-enum GeometryMode { kPhantom = 0, kLss = 1, kDots = 2 };
+enum GeometryMode { kPhantom = 0, kLss = 1, kDots = 2, kTriangles = 3 };
 
 // 8-wide node. Child slot s has a box quantized to 8 bits per axis relative to the node
 // origin with power-of-two scales; internal children are stored contiguously from
@@ -37,6 +38,7 @@ struct BlasDev {
     const float* aux;  // phantom: precomputed early-out cylinder radius per leaf slot
     const uint32_t* prim_ids;
     const WideNode* nodes;
+    const float4* normals;  // triangles: 3 vertex normals per original triangle (shading normal)
     uint32_t nprims;    // leaf primitives (curves / capsules / DOTS pieces)
     uint32_t nnodes;
     uint32_t mode;

@@ -27,12 +27,14 @@ using PhPoint = Geo<kPhantom, kRadiusPerPoint, false>;
 using PhPointPolish = Geo<kPhantom, kRadiusPerPoint, true>;
 using LssG = Geo<kLss, 0, false>;
 using DotsG = Geo<kDots, 0, false>;
+using TrisG = Geo<kTriangles, 0, false>;
 VKHRT_DECLARE_VARIANT(PhConst)
 VKHRT_DECLARE_VARIANT(PhConstPolish)
 VKHRT_DECLARE_VARIANT(PhPoint)
 VKHRT_DECLARE_VARIANT(PhPointPolish)
 VKHRT_DECLARE_VARIANT(LssG)
 VKHRT_DECLARE_VARIANT(DotsG)
+VKHRT_DECLARE_VARIANT(TrisG)
 VKHRT_DECLARE_VARIANT(GeoMixed)
 
 namespace {
@@ -68,6 +70,7 @@ hipError_t launch_batch(const BatchParams& bp, int variant, int stack_kind, hipS
         case kVarPhPointPolish: return batch_stack<PhPointPolish>(bp, stack_kind, st);
         case kVarLss: return batch_stack<LssG>(bp, stack_kind, st);
         case kVarDots: return batch_stack<DotsG>(bp, stack_kind, st);
+        case kVarTris: return batch_stack<TrisG>(bp, stack_kind, st);
         default: return hipErrorNotSupported;
     }
 }
@@ -81,6 +84,7 @@ hipError_t launch_trace_rays(const TraceParams& tp, const float* rays, uint32_t 
         case kVarPhPointPolish: return trace_stack<PhPointPolish>(tp, rays, n, hits, stats, stack_kind, st);
         case kVarLss: return trace_stack<LssG>(tp, rays, n, hits, stats, stack_kind, st);
         case kVarDots: return trace_stack<DotsG>(tp, rays, n, hits, stats, stack_kind, st);
+        case kVarTris: return trace_stack<TrisG>(tp, rays, n, hits, stats, stack_kind, st);
         default: return trace_stack<GeoMixed>(tp, rays, n, hits, stats, stack_kind, st);
     }
 }
@@ -93,6 +97,7 @@ hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hi
         case kVarPhPointPolish: return render_stack<PhPointPolish>(rp, stack_kind, st, ev);
         case kVarLss: return render_stack<LssG>(rp, stack_kind, st, ev);
         case kVarDots: return render_stack<DotsG>(rp, stack_kind, st, ev);
+        case kVarTris: return render_stack<TrisG>(rp, stack_kind, st, ev);
         default: return render_stack<GeoMixed>(rp, stack_kind, st, ev);
     }
 }
@@ -205,6 +210,14 @@ __device__ __forceinline__ void exact_box(const BlasDev& B, uint32_t slot, int t
         for (int k = 0; k < 3; ++k) {
             lo[k] = gmin(ax[k] - ra, bx[k] - rb);
             hi[k] = gmax(ax[k] + ra, bx[k] + rb);
+        }
+    } else if (B.mode == kTriangles) {
+        const float4* pr = B.prims + size_t(slot) * 3;
+        const float4 v[3] = {pr[0], pr[1], pr[2]};
+        const float a[3] = {v[0].x, v[0].y, v[0].z}, b[3] = {v[1].x, v[1].y, v[1].z}, c[3] = {v[2].x, v[2].y, v[2].z};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = gmin(a[k], gmin(b[k], c[k]));
+            hi[k] = gmax(a[k], gmax(b[k], c[k]));
         }
     } else {
         const float4* pr = B.prims + size_t(slot) * 4;

@@ -151,7 +151,7 @@ struct Walker {
         inv = safe_inv(d);
         oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
         if (MODE == kPhantom) make_frame(o, d, F);
-        if (MODE == kDots) TR = make_tri_ray(d);
+        if (MODE == kDots || MODE == kTriangles) TR = make_tri_ray(d);
         cur = make_uint2(0u, 0u);
         node = 0;
         sp = 0;
@@ -216,6 +216,24 @@ struct Walker {
                             best.a = lh.n.x;
                             best.b = lh.n.y;
                             best.c = lh.n.z;
+                            best.extra = 0;
+                            if (ANY) return kStepHit;
+                        }
+                    }
+                } else if (MODE == kTriangles) {
+                    const float4* pr = B.prims + size_t(slot) * 3;
+                    const float4 a = pr[0], b = pr[1], c = pr[2];
+                    float t, b1, b2;
+                    if (tri_intersect(TR, o, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, tmin, best.t, t,
+                                      b1, b2)) {
+                        const uint32_t pid = B.prim_ids[slot];
+                        if (better(t, inst, pid, best)) {
+                            best.t = t;
+                            best.inst = inst;
+                            best.pid = pid;
+                            best.slot = slot;
+                            best.u = b1;
+                            best.a = b2;
                             best.extra = 0;
                             if (ANY) return kStepHit;
                         }
@@ -340,6 +358,8 @@ __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, f
                 r = traverse<kLss, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
             } else if (I.blas.mode == kDots) {
                 r = traverse<kDots, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
+            } else if (I.blas.mode == kTriangles) {
+                r = traverse<kTriangles, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
             } else if (I.blas.radius_mode == kRadiusPerPoint) {
                 r = pol ? traverse<kPhantom, kRadiusPerPoint, true, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id)
                         : traverse<kPhantom, kRadiusPerPoint, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
@@ -423,6 +443,22 @@ __device__ __forceinline__ void finish_hit(const TraceParams& tp, f3 o, f3 d, co
         v = best.extra ? 1.0f : 0.0f;
     } else if (I.blas.mode == kLss) {
         n = f3{best.a, best.b, best.c};
+    } else if (I.blas.mode == kTriangles) {
+        // triangle_closest_hit.rchit:31-36: normalize(n0 b0 + n1 b1 + n2 b2), b = (1 - u - v, u, v);
+        // the geometric normal when the vertex normals cancel
+        const float4* vn = I.blas.normals + size_t(best.pid) * 3;
+        const float4 n0 = vn[0], n1 = vn[1], n2 = vn[2];
+        const float b0 = 1.0f - best.u - best.a;
+        const f3 s{n0.x * b0 + n1.x * best.u + n2.x * best.a, n0.y * b0 + n1.y * best.u + n2.y * best.a,
+                   n0.z * b0 + n1.z * best.u + n2.z * best.a};
+        if (dot3(s, s) > 0.0f) {
+            n = nrm3(s);
+        } else {
+            const float4* pr = I.blas.prims + size_t(best.slot) * 3;
+            const float4 a = pr[0], b = pr[1], c = pr[2];
+            n = nrm3(cross3(f3{b.x - a.x, b.y - a.y, b.z - a.z}, f3{c.x - a.x, c.y - a.y, c.z - a.z}));
+        }
+        v = best.a;
     } else {
         const float4* pr = I.blas.prims + size_t(best.slot) * 4;
         float4 P[4] = {pr[0], pr[1], pr[2], pr[3]};
