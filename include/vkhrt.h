@@ -162,6 +162,17 @@ typedef struct vkhrt_area_light {
     uint64_t seed;
 } vkhrt_area_light;
 
+/* Environment map of the miss shader (miss.rmiss:17-38): an equirectangular RGBA32F image
+ * (row-major, `width` x `height` texels, device memory) sampled bilinearly with repeat
+ * addressing at DirectionToUV(normalize(-direction)), then 1 - exp(-c exposure) and the 1/2.2
+ * gamma. Without one, primary misses are black. */
+typedef struct vkhrt_environment_map {
+    const float* texels;
+    uint32_t width;
+    uint32_t height;
+    float exposure; /* 1 in the reference */
+} vkhrt_environment_map;
+
 typedef struct vkhrt_render_desc {
     uint32_t width;
     uint32_t height;
@@ -175,6 +186,7 @@ typedef struct vkhrt_render_desc {
     const vkhrt_area_light* area_light; /* host, optional: shadow rays toward an area light instead
                                            of the direction (0, 1, 0); needs a TLAS of one
                                            geometry mode */
+    const vkhrt_environment_map* environment; /* host, optional: the miss colour of primary rays */
 } vkhrt_render_desc;
 
 /* ---- context */

@@ -982,13 +982,38 @@ int orc_count_candidates(orc_scene* sp, const float* rays, const void* hits, uin
 // pixel_stride > 1 traces every stride-th pixel (row-major index) for bounded CPU timing.
 // image: W*H*3 floats (untouched for skipped pixels); prim_ids: W*H (0xFFFFFFFF miss) or NULL.
 // Returns the number of rays traced; *seconds gets the wall time of the tracing loop.
+// miss.rmiss:17-38. DirectionToUV of the inverted direction: gamma = asin(v.y),
+// theta = atan(v.x, -v.z), uv = (theta / 2pi, gamma / pi) + 0.5; the texture() lookup of the
+// reference's default sampler (linear filter, repeat addressing, texel centres at
+// (i + 0.5) / size, one mip level); then 1 - exp(-c exposure) and pow(c, 1 / 2.2).
+static void EnvironmentColor(const float* env, uint32_t ew, uint32_t eh, float exposure, V3<float> dir, float rgb[3]) {
+    const V3<float> v = normalize(mk(-dir.x, -dir.y, -dir.z));
+    const float gamma = std::asin(v.y);
+    const float theta = std::atan2(v.x, -v.z);
+    const float u = theta * 0.3183098861837f * 0.5f + 0.5f;
+    const float t = gamma * 0.3183098861837f + 0.5f;
+    const float x = u * float(ew) - 0.5f, y = t * float(eh) - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float ax = x - fx, ay = y - fy;
+    auto wrap = [](long i, long n) { long r = i % n; return r < 0 ? r + n : r; };
+    const long x0 = wrap(long(fx), ew), x1 = wrap(long(fx) + 1, ew), y0 = wrap(long(fy), eh), y1 = wrap(long(fy) + 1, eh);
+    auto tex = [&](long xx, long yy, int c) { return env[(size_t(yy) * ew + size_t(xx)) * 4 + c]; };
+    auto lerp = [](float a, float b, float s) { return a + (b - a) * s; };
+    for (int c = 0; c < 3; ++c) {
+        const float val = lerp(lerp(tex(x0, y0, c), tex(x1, y0, c), ax), lerp(tex(x0, y1, c), tex(x1, y1, c), ax), ay);
+        rgb[c] = std::pow(1.0f - std::exp(-val * exposure), 1.0f / 2.2f);
+    }
+}
+
 // area_light (optional, 9 floats: center, edge u, edge v): `light_samples` shadow rays per hit
 // toward center + a u + b v, a = 2 path_rand(seed, pixel, s, 0) - 1, b likewise with k = 1,
 // tmax = distance; the visibility is their mean (sum, then / samples). NULL: one ray along
 // (0, 1, 0).
+// env (optional): an equirectangular RGBA32F map of ew x eh texels, the miss colour (miss.rmiss).
 long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_inv[16], uint32_t W, uint32_t H,
                      uint32_t flags, int shadows, float* image, uint32_t* prim_ids, int nthreads, uint32_t pixel_stride,
-                     double* seconds, const float* area_light, uint32_t light_samples, uint64_t light_seed) {
+                     double* seconds, const float* area_light, uint32_t light_samples, uint64_t light_seed,
+                     const float* env, uint32_t ew, uint32_t eh, float exposure) {
     Scene& S = *reinterpret_cast<Scene*>(sp);
     M4<float> Vi, Pi;
     std::memcpy(Vi.m, view_inv, 64);
@@ -1007,6 +1032,7 @@ long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_i
         long long nr = 1;
         bool hit = trace_scene<float>(S, r, 0.001f, 10000.0f, flags & (F_REF_NORMAL | F_POLISH), h);
         float rgb[3] = {0.0f, 0.0f, 0.0f};
+        if (!hit && env) EnvironmentColor(env, ew, eh, exposure, d, rgb);
         if (hit) {
             float vis = 1.0f;
             if (shadows && !area_light) {

@@ -68,7 +68,8 @@ def lib():
         "orc_scene_stats": ([P, u32, P], i32),
         "orc_trace": ([P, P, u32, P, u32, i32], i32),
         "orc_count_candidates": ([P, P, P, u32, P, i32], i32),
-        "orc_render": ([P, P, P, u32, u32, u32, i32, P, P, i32, u32, P, P, u32, u64], ctypes.c_longlong),
+        "orc_render": ([P, P, P, u32, u32, u32, i32, P, P, i32, u32, P, P, u32, u64, P, u32, u32, f32],
+                       ctypes.c_longlong),
         "orc_scene_set_material": ([P, u32, u32], i32),
         "orc_scene_add_mesh": ([P, P, u32, P, u32, P, i32], i32),
         "orc_render_batch": ([P, P, P, u32, u32, u64, P, P, u32, P, P, P, i32, u32, P], ctypes.c_longlong),
@@ -241,15 +242,18 @@ class Scene:
         return imgs, pids, int(counts[0]), int(counts[1]), secs.value
 
     def render(self, view_inv, proj_inv, W, H, flags=0, shadows=1, nthreads=0, pixel_stride=1, area_light=None,
-               light_samples=1, light_seed=0):
-        """area_light: (center, edge_u, edge_v) of a parallelogram light, light_samples rays per hit."""
+               light_samples=1, light_seed=0, env=None, exposure=1.0):
+        """area_light: (center, edge_u, edge_v) of a parallelogram light, light_samples rays per hit.
+        env: (h, w, 4) float32 equirectangular map coloring the misses."""
         img = np.zeros((H, W, 3), np.float32)
         ids = np.full((H, W), 0xFFFFFFFF, np.uint32)
         secs = ctypes.c_double(0)
         light = None if area_light is None else f32(np.asarray(area_light, np.float32).reshape(9))
         nrays = lib().orc_render(self.h, ptr(f32(view_inv)), ptr(f32(proj_inv)), W, H, flags,
                                  shadows, ptr(img), ptr(ids), nthreads, pixel_stride,
-                                 ctypes.byref(secs), ptr(light), light_samples, light_seed)
+                                 ctypes.byref(secs), ptr(light), light_samples, light_seed,
+                                 ptr(None if env is None else f32(env)), 0 if env is None else env.shape[1],
+                                 0 if env is None else env.shape[0], exposure)
         return img, ids, int(nrays), secs.value
 
 

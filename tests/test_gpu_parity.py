@@ -421,3 +421,50 @@ def test_mesh_rejects_out_of_range_indices(gpu_ctx, G):
     with pytest.raises(V.VkhrtError) as e:
         _mesh_blas(gpu_ctx, G, verts, idx)
     assert e.value.code == -1
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_environment_map_render(gpu_ctx, G, oracle, mixed):
+    """miss.rmiss with an environment map: hit pixels bit for bit, missed pixels within 1e-4 of the
+    oracle, median 1e-6 (asin / atan2 / exp / pow come from different math libraries); the
+    persistent render and the mixed-TLAS render."""
+    import scenes
+    import vkhrt_amd as V
+    curves = scenes.dense_ball_curves(nstrands=1500)
+    rng = np.random.default_rng(12)
+    ew, eh = 96, 48
+    xx, yy = np.meshgrid(np.linspace(0, 6.28, ew), np.linspace(0, 3.14, eh))
+    env = np.stack([1 + np.sin(xx), 0.5 + 0.5 * np.cos(2 * yy), rng.uniform(0, 3, (eh, ew)), np.ones((eh, ew))],
+                   -1).astype(np.float32)
+    denv = G.dev(env)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    blases = [G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)]
+    if mixed:
+        verts, idx = scenes.uv_sphere(4.0, 12, 24, (12.0, 4.0, 0.0))
+        S.add_mesh(verts, idx)
+        blases.append(_mesh_blas(gpu_ctx, G, verts, idx))
+    tl = V.TopLevelAccelerationStructure(gpu_ctx, blases)
+    W, H = 128, 96
+    vi, pi = oracle.camera((0.0, 20.0, 45.0), (0.0, 8.0, 0.0), aspect=W / H)
+    cam = V.Camera()
+    for k in range(16):
+        cam.view_inverse[k], cam.proj_inverse[k] = float(vi[k]), float(pi[k])
+    em = V.EnvironmentMap(denv.data_ptr(), ew, eh, 1.0)
+    img = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    ids = torch.zeros(W * H, dtype=torch.int32, device="cuda")
+    print(f"[environment map] mixed {mixed}", flush=True)
+    gpu_ctx.render(tl, cam, W, H, img.data_ptr(), prim_ids_ptr=ids.data_ptr(), stream=G.stream_ptr(), environment=em)
+    torch.cuda.synchronize()
+    got = img.cpu().numpy().reshape(H, W, 4)[..., :3]
+    gid = ids.cpu().numpy().view(np.uint32).reshape(H, W)
+    ref, rid, _, _ = S.render(vi, pi, W, H, shadows=1, env=env)
+    hit = rid != 0xFFFFFFFF
+    assert np.array_equal(gid, rid) and 200 < hit.sum() < W * H - 200
+    assert np.array_equal(got[hit].view(np.uint32), ref[hit].view(np.uint32))
+    # a 1-ulp difference of asin / atan2 moves the lookup by ~1e-7 rad; the map's steepest channel
+    # (uniform noise per texel) changes by up to ~3 per texel, i.e. ~5e-5 per 1e-7 rad at 96 texels
+    # per turn, hence the bound on the maximum; the median difference is at rounding level
+    diff = np.abs(got[~hit] - ref[~hit])
+    assert diff.max() < 1e-4 and np.median(diff) < 1e-6
+    assert np.ptp(ref[~hit][:, 2]) > 0.1  # the map varies across the missed pixels

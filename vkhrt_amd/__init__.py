@@ -108,11 +108,17 @@ def area_light(center, edge_u, edge_v, samples=2, seed=0):
     return L
 
 
+class EnvironmentMap(ctypes.Structure):
+    _fields_ = [("texels", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("exposure", ctypes.c_float)]
+
+
 class RenderDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("flags", ctypes.c_uint32),
                 ("image", ctypes.c_void_p), ("primitive_ids", ctypes.c_void_p),
                 ("light_visibility", ctypes.c_void_p), ("shadow_mlp", ctypes.c_void_p),
-                ("shadow_variance_threshold", ctypes.c_float), ("area_light", ctypes.POINTER(AreaLight))]
+                ("shadow_variance_threshold", ctypes.c_float), ("area_light", ctypes.POINTER(AreaLight)),
+                ("environment", ctypes.POINTER(EnvironmentMap))]
 
 
 class RenderStats(ctypes.Structure):
@@ -286,14 +292,16 @@ class Context:
 
     def render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS,
                prim_ids_ptr=None, light_visibility_ptr=None, stream=None, shadow_mlp=None,
-               variance_threshold=0.0, area_light=None):
+               variance_threshold=0.0, area_light=None, environment=None):
         """traceRaysKHR(W, H, 1) with ray_gen.rgen and the closest-hit shaders. With
         RENDER_NEURAL_SHADOWS, `shadow_mlp` answers shadow queries whose predicted variance
         p (1 - p) is at most `variance_threshold`; the rest trace exactly. `area_light`
-        (an AreaLight) replaces the directional light with light samples per hit."""
+        (an AreaLight) replaces the directional light with light samples per hit. `environment`
+        (an EnvironmentMap over a device RGBA32F image) colours the primary misses."""
         d = RenderDesc(width, height, flags, image_ptr, prim_ids_ptr, light_visibility_ptr,
                        shadow_mlp.h if shadow_mlp is not None else None, variance_threshold,
-                       ctypes.pointer(area_light) if area_light is not None else None)
+                       ctypes.pointer(area_light) if area_light is not None else None,
+                       ctypes.pointer(environment) if environment is not None else None)
         self.check(lib().vkhrt_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), stream),
                    "render")
 

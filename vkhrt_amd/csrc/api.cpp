@@ -907,6 +907,12 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     if (rc != VKHRT_SUCCESS) return rc;
     hipStream_t st = pick(c, stream);
     vk::RenderParams rp{};
+    if (d->environment) {
+        const vkhrt_environment_map* em = d->environment;
+        if (!em->texels || em->width == 0 || em->height == 0 || !(em->exposure >= 0.0f))
+            return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad environment map");
+        rp.env = vk::EnvMap{reinterpret_cast<const float4*>(em->texels), em->width, em->height, em->exposure};
+    }
     rp.light.samples = S;
     if (al) {
         rp.light.enabled = 1;

@@ -52,6 +52,14 @@ struct Camera {
 
 struct MlpDev;
 
+// Equirectangular RGBA32F environment map of the miss shader (miss.rmiss), sampled bilinearly
+// with repeat addressing like the reference's default sampler; texels == null: misses are black.
+struct EnvMap {
+    const float4* texels;
+    uint32_t width, height;
+    float exposure;
+};
+
 // Light of the render's shadow rays: the directional light toward (0, 1, 0) (one ray per
 // hit), or a parallelogram c + a u + b v (a, b in [-1, 1]) sampled `samples` times per hit at
 // points fixed by (seed, pixel, sample).
@@ -85,6 +93,7 @@ struct RenderParams {
     uint32_t* live;            // persistent pipeline: pixels whose primary ray enters an instance box
                                // ([6] of counters = length); the rest are recorded as misses up front
     AreaLight light;           // shadow queue entries are ids pixel * light.samples + sample
+    EnvMap env;                // miss colour
     float4* dirs;              // neural form with an area light: shadow-ray direction per queue entry
 };
 

@@ -557,6 +557,35 @@ __device__ __forceinline__ f3 debug_color(uint32_t prim) {
     return f3{r[idx], g[idx], 0.3f};
 }
 
+// miss.rmiss:17-38: the inverted ray direction -> equirectangular uv (DirectionToUV), a
+// bilinear RGBA32F lookup with repeat addressing (texel centres at (i + 0.5) / size), then
+// 1 - exp(-c exposure) and the 1/2.2 gamma.
+__device__ __forceinline__ f3 env_color(const EnvMap& E, f3 d) {
+    const f3 v = nrm3(f3{-d.x, -d.y, -d.z});
+    const float gamma = asinf(v.y);
+    const float theta = atan2f(v.x, -v.z);
+    const float u = theta * 0.3183098861837f * 0.5f + 0.5f;
+    const float t = gamma * 0.3183098861837f + 0.5f;
+    const float x = u * float(E.width) - 0.5f, y = t * float(E.height) - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float ax = x - fx, ay = y - fy;
+    auto wrap = [](int i, int n) {
+        const int r = i % n;
+        return r < 0 ? r + n : r;
+    };
+    const int w = int(E.width), h = int(E.height);
+    const int x0 = wrap(int(fx), w), x1 = wrap(int(fx) + 1, w), y0 = wrap(int(fy), h), y1 = wrap(int(fy) + 1, h);
+    const float4 c00 = E.texels[size_t(y0) * w + x0], c10 = E.texels[size_t(y0) * w + x1];
+    const float4 c01 = E.texels[size_t(y1) * w + x0], c11 = E.texels[size_t(y1) * w + x1];
+    auto lerp = [](float a, float b, float s) { return a + (b - a) * s; };
+    const float r = lerp(lerp(c00.x, c10.x, ax), lerp(c01.x, c11.x, ax), ay);
+    const float g = lerp(lerp(c00.y, c10.y, ax), lerp(c01.y, c11.y, ax), ay);
+    const float b = lerp(lerp(c00.z, c10.z, ax), lerp(c01.z, c11.z, ax), ay);
+    const float k = 1.0f / 2.2f;
+    return f3{powf(1.0f - expf(-r * E.exposure), k), powf(1.0f - expf(-g * E.exposure), k),
+              powf(1.0f - expf(-b * E.exposure), k)};
+}
+
 // RGBA32F (alpha 1) or RGBA8 (clamped to [0, 1], rounded) pixel store.
 __device__ __forceinline__ void store_rgba(void* image, uint32_t format, uint32_t pix, f3 c) {
     if (format == 0) {
@@ -589,7 +618,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary(RenderParams rp)
         if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
         const f3 n = f3{h.n[0], h.n[1], h.n[2]};
         if (!hit) {
-            store_px(rp, pix, f3{0.0f, 0.0f, 0.0f});
+            f3 c{0.0f, 0.0f, 0.0f};
+            if (rp.env.texels) {
+                f3 o, d;
+                primary_ray(rp.cam, x, y, o, d);
+                c = env_color(rp.env, d);
+            }
+            store_px(rp, pix, c);
         } else if (rp.debug_palette) {
             store_px(rp, pix, debug_color(h.prim_id));
             hit = false;
@@ -1321,7 +1356,13 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams r
         if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
         const f3 n = f3{h.n[0], h.n[1], h.n[2]};
         if (!hit) {
-            store_px(rp, pix, f3{0.0f, 0.0f, 0.0f});
+            f3 c{0.0f, 0.0f, 0.0f};
+            if (rp.env.texels) {
+                f3 o, d;
+                primary_ray(rp.cam, x, y, o, d);
+                c = env_color(rp.env, d);
+            }
+            store_px(rp, pix, c);
         } else if (rp.debug_palette) {
             store_px(rp, pix, debug_color(h.prim_id));
             hit = false;
