@@ -329,8 +329,9 @@ typedef struct vkhrt_batch_desc {
     const float* material_albedo;    /* host, optional: material_count x 3 tints (default 1) */
     uint32_t material_count;         /* <= VKHRT_BATCH_MAX_MATERIALS */
 } vkhrt_batch_desc;
-/* Renders `view_count` views (at most 65536, 2^31 pixels in total) of `tlas`. The TLAS must
- * hold instances of one geometry mode. */
+/* Renders `view_count` views (at most 65536, 2^31 pixels in total) of `tlas`. A TLAS of one
+ * geometry mode runs the persistent traversal kernels; a mixed one (e.g. hair and triangle
+ * meshes from one asset) one lane per path and no FP64 polish. */
 int vkhrt_render_batch(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_view* views, uint32_t view_count,
                        const vkhrt_batch_desc* desc, void* stream);
 /* Rays of the last vkhrt_render_batch (synchronizes its stream): one primary ray per path,
@@ -422,6 +423,112 @@ int vkhrt_shadow_mlp_gradient(vkhrt_context* ctx, vkhrt_shadow_mlp* mlp, const f
 /* Predicted visibility p per sample (device). */
 int vkhrt_shadow_mlp_infer(vkhrt_context* ctx, const vkhrt_shadow_mlp* mlp, const float* samples, uint32_t count,
                            float* visibility, void* stream);
+
+/* ---- polyline / triangle assets (glTF 2.0)
+ * Replaces ModelLoader::LoadFromFile (source/resources/model/model_loader.cpp:274-291: assimp
+ * 6.0.1 ReadFile with Triangulate | FlipUVs | GenNormals | GenBoundingBoxes), LoadModel
+ * (:293-312), ProcessMesh (:139-206), ProcessNodes (:208-264), ProcessMaterial factors
+ * (:60-137) and Node::GetWorldMatrix (source/resources/model/model.cpp:6-18).
+ * Reads .gltf (external buffers or base64 data: URIs) and .glb. Each mesh primitive is one
+ * mesh, in file order; LINES / LINE_LOOP / LINE_STRIP become line meshes (index pairs),
+ * TRIANGLES / TRIANGLE_STRIP / TRIANGLE_FAN triangle meshes; POINTS and sparse accessors are
+ * UNSUPPORTED. Indices are mesh-local (the reference adds a mesh's firstVertex both in
+ * ProcessMesh :157 and in GenerateLines geometry_processor.cpp:61; it is applied once here).
+ * Nodes are listed in pre-order from the root; several scene roots get an identity root
+ * "ROOT" (as assimp does). Triangle meshes without normals get face normals written per vertex,
+ * the last face of a vertex winning (aiProcess_GenNormals); texCoords are the file's (the
+ * importer's V flip and FlipUVs cancel). Loading is host work: no context, no device. */
+typedef struct vkhrt_asset vkhrt_asset;
+
+#define VKHRT_PRIMITIVE_TRIANGLES 0 /* Mesh::PrimitiveType::eTriangles */
+#define VKHRT_PRIMITIVE_LINES 1     /* Mesh::PrimitiveType::eLines */
+
+typedef struct vkhrt_asset_info {
+    uint32_t mesh_count;     /* mesh primitives */
+    uint32_t node_count;
+    uint32_t material_count;
+    uint32_t instance_count; /* (node, mesh) pairs = BLASes made by vkhrt_asset_build */
+    uint32_t line_mesh_count;
+    uint32_t triangle_mesh_count;
+} vkhrt_asset_info;
+
+/* Mesh (model.hpp Mesh + its slice of the vertex and index buffers). Host pointers owned by
+ * the asset. */
+typedef struct vkhrt_asset_mesh {
+    uint32_t primitive;      /* VKHRT_PRIMITIVE_* */
+    int32_t material;        /* material index, -1 = none */
+    uint32_t vertex_count;
+    uint32_t index_count;    /* 2 per line, 3 per triangle */
+    const float* vertices;   /* vertex_count x 8 floats: position, normal, texCoord (Vertex) */
+    const uint32_t* indices; /* mesh-local, each < vertex_count */
+    float bounds_min[3];     /* boundingBox of all vertices (aiProcess_GenBoundingBoxes) */
+    float bounds_max[3];
+    const char* name;
+} vkhrt_asset_mesh;
+
+/* Node (model.hpp Node): column-major 4x4 matrices as glm::mat4. */
+typedef struct vkhrt_asset_node {
+    int32_t parent;         /* node index, -1 for the root */
+    uint32_t mesh_count;
+    const uint32_t* meshes; /* host, owned by the asset */
+    float local[16];        /* localMatrix */
+    float world[16];        /* GetWorldMatrix(): parents multiplied in from the nearest */
+    const char* name;
+} vkhrt_asset_node;
+
+/* Material factors (MaterialCreation; textures are not read). glTF defaults when absent. */
+typedef struct vkhrt_asset_material {
+    float base_color[4]; /* pbrMetallicRoughness.baseColorFactor */
+    float metallic;
+    float roughness;
+    float emissive[3];
+    float pad;
+} vkhrt_asset_material;
+
+/* One BLAS input: node's world matrix as a row-major 3x4 (BLASInput.transform). */
+typedef struct vkhrt_asset_instance {
+    uint32_t node;
+    uint32_t mesh;
+    float transform[12];
+} vkhrt_asset_instance;
+
+int vkhrt_asset_load(const char* path, vkhrt_asset** out);
+/* `data` holds a .glb or the text of a .gltf; relative buffer URIs resolve against base_dir
+ * (NULL: only data: URIs and the GLB binary chunk are allowed). */
+int vkhrt_asset_load_memory(const void* data, size_t size, const char* base_dir, vkhrt_asset** out);
+void vkhrt_asset_destroy(vkhrt_asset* asset);
+/* Message of the last failed vkhrt_asset_* call on this thread. */
+const char* vkhrt_asset_last_error(void);
+int vkhrt_asset_info_get(const vkhrt_asset* asset, vkhrt_asset_info* out);
+int vkhrt_asset_mesh_get(const vkhrt_asset* asset, uint32_t index, vkhrt_asset_mesh* out);
+int vkhrt_asset_node_get(const vkhrt_asset* asset, uint32_t index, vkhrt_asset_node* out);
+int vkhrt_asset_material_get(const vkhrt_asset* asset, uint32_t index, vkhrt_asset_material* out);
+int vkhrt_asset_instance_get(const vkhrt_asset* asset, uint32_t index, vkhrt_asset_instance* out);
+
+/* ProcessHairCurves / ProcessHairLSS / ProcessHairDOTS (geometry_processor.cpp:604-688,740-779)
+ * and Renderer::InitializeBLAS (renderer.cpp:694-728) on the device. */
+typedef struct vkhrt_asset_build_desc {
+    uint32_t hair_mode;        /* line meshes: VKHRT_GEOMETRY_PHANTOM_CURVES, _LSS or _DOTS */
+    float radius;              /* fibre radius (0.02 in the reference) */
+    float tension;             /* GenerateCurves tension (1) */
+    uint32_t leaf_size;        /* 0 = the context's default */
+    uint32_t default_material; /* material_index of meshes without a material */
+} vkhrt_asset_build_desc;
+/* One BLAS per instance (vkhrt_asset_instance_get order), with the instance's transform and
+ * its mesh's material. Line meshes: GenerateLines (mesh-local indices) -> GenerateCurves ->
+ * phantom curves, or -> capsules (LSS) / lines (DOTS); triangle meshes as
+ * VKHRT_GEOMETRY_TRIANGLES. Each mesh is generated once and shared by its instances; every
+ * BLAS owns its records (the reference points every hair BLAS at the start of the curve
+ * buffer, renderer.cpp:611). blases == NULL with capacity 0 only sets *count; otherwise
+ * capacity must cover the instances. On failure no BLAS is left. */
+int vkhrt_asset_build(vkhrt_context* ctx, const vkhrt_asset* asset, const vkhrt_asset_build_desc* desc, void* stream,
+                      vkhrt_blas** blases, uint32_t capacity, uint32_t* count);
+/* GenerateLines (geometry_processor.cpp:45-67) for an indexed line mesh on the device: line i
+ * = positions of vertices[indices[2i]] and vertices[indices[2i+1]] (8-float vertices,
+ * mesh-local indices), radius `radius` at both ends; `count` lines. INVALID_ARGUMENT when an
+ * index is not below vertex_count (synchronizes the stream to check). */
+int vkhrt_generate_mesh_lines(vkhrt_context* ctx, const float* vertices, uint32_t vertex_count,
+                              const uint32_t* indices, uint32_t count, float radius, float* lines, void* stream);
 
 /* ---- profiling: per-kernel HIP-event timing on the launch stream */
 int vkhrt_profiling_enable(vkhrt_context* ctx, int enable);

@@ -576,6 +576,44 @@ int orc_generate_curves(const float* lines, uint32_t n, float* curves) {
     return 0;
 }
 
+// GenerateLines (geometry_processor.cpp:45-67) over an indexed line mesh whose indices are
+// mesh-local: line i = (position[idx[2i]], radius, position[idx[2i+1]], radius), vertices of 8
+// floats (Mesh::Vertex, model.hpp:44-49). The reference adds the mesh's firstVertex to indices
+// that ProcessMesh (model_loader.cpp:157) already offset; here it is applied once. Returns -1
+// (and writes nothing further) at the first index not below nv.
+int orc_mesh_lines(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t n, float radius, float* lines) {
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t v = idx[size_t(i) * 2 + k];
+            if (v >= nv) return -1;
+            float* L = lines + size_t(i) * 8 + k * 4;
+            for (int c = 0; c < 3; ++c) L[c] = verts[size_t(v) * 8 + c];
+            L[3] = radius;
+        }
+    return 0;
+}
+
+// Node::GetWorldMatrix (source/resources/model/model.cpp:6-18): matrix = localMatrix, then
+// matrix = parent->localMatrix * matrix for each ancestor from the nearest one, as glm::mat4
+// products in float (column j of A * B = A * B[j], summed k = 0..3 in order). `locals` and
+// `out` hold n column-major 4x4 matrices; parents[i] < 0 marks a root. -1 on a parent cycle.
+int orc_world_matrix(const float* locals, const int32_t* parents, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        M4<float> m;
+        std::memcpy(m.m, locals + size_t(i) * 16, 64);
+        uint32_t steps = 0;
+        for (int32_t p = parents[i]; p >= 0; p = parents[p]) {
+            if (uint32_t(p) >= n || ++steps > n) return -1;
+            M4<float> P, r;
+            std::memcpy(P.m, locals + size_t(p) * 16, 64);
+            for (int j = 0; j < 4; ++j) mul4(P, m.m[j], r.m[j]);
+            m = r;
+        }
+        std::memcpy(out + size_t(i) * 16, m.m, 64);
+    }
+    return 0;
+}
+
 int orc_curves_to_lines(const float* curves, uint32_t n, uint32_t pieces, float* lines) {
     std::vector<float> c(curves, curves + size_t(n) * 16);
     std::vector<LineR> L;

@@ -65,3 +65,16 @@ def psnr(a, b):
 
 def cam_arrays(cam):
     return np.array(cam.view_inverse, np.float32), np.array(cam.proj_inverse, np.float32)
+
+
+def compare_hits(h, ref, what):
+    """Hit records agree bit for bit: hit/miss, (instance, primitive), t, u and normal."""
+    miss_g = h["prim"] == 0xFFFFFFFF
+    miss_r = ref["prim"] == 0xFFFFFFFF
+    assert (miss_g == miss_r).all(), f"{what}: hit/miss differs on {(miss_g != miss_r).sum()} rays"
+    agree = (h["prim"] == ref["prim"]) & (h["inst"] == ref["inst"])
+    assert agree.all(), f"{what}: (instance, primitive) differs on {(~agree).sum()} rays"
+    hit = ~miss_r
+    for k in ("t", "u", "n"):
+        assert np.array_equal(h[k][hit].view(np.uint32), ref[k][hit].view(np.uint32)), f"{what}: {k} differs"
+    return int(hit.sum())

@@ -42,6 +42,8 @@ def lib():
         "orc_generate_strands": ([u64, u32, u32, f32, P, i32], i32),
         "orc_strands_to_lines": ([P, u32, u32, P], i32),
         "orc_generate_curves": ([P, u32, P], i32),
+        "orc_mesh_lines": ([P, u32, P, u32, f32, P], i32),
+        "orc_world_matrix": ([P, P, u32, P], i32),
         "orc_curves_to_lines": ([P, u32, u32, P], i32),
         "orc_generate_lss": ([P, u32, f32, P], i32),
         "orc_generate_dots": ([P, u32, f32, P], i32),
@@ -114,6 +116,28 @@ def generate_curves(lines):
     lines = f32(lines)
     out = np.empty((lines.shape[0], 16), np.float32)
     lib().orc_generate_curves(ptr(lines), lines.shape[0], ptr(out))
+    return out
+
+
+def mesh_lines(verts, indices, radius=0.02):
+    """GenerateLines of an indexed line mesh: (n, 8) lines from 8-float vertices and mesh-local
+    index pairs; raises on an index past the vertices."""
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 8)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    n = idx.size // 2
+    out = np.zeros((n, 8), np.float32)
+    if lib().orc_mesh_lines(ptr(verts), verts.shape[0], ptr(idx), n, radius, ptr(out)) != 0:
+        raise ValueError("line index past the vertex count")
+    return out
+
+
+def world_matrices(locals_, parents):
+    """Node::GetWorldMatrix for every node: (n, 16) column-major float32."""
+    loc = np.ascontiguousarray(locals_, np.float32).reshape(-1, 16)
+    par = np.ascontiguousarray(parents, np.int32)
+    out = np.zeros_like(loc)
+    if lib().orc_world_matrix(ptr(loc), ptr(par), loc.shape[0], ptr(out)) != 0:
+        raise ValueError("parent cycle")
     return out
 
 

@@ -242,8 +242,45 @@ def test_batch_errors(gpu_ctx, oracle, scene):
     b_lss = G.build_blas(gpu_ctx, V.GEOMETRY_LSS, caps)
     mixed = V.TopLevelAccelerationStructure(gpu_ctx, [b_ph, b_lss])
     with pytest.raises(V.VkhrtError) as e:
-        _gpu_batch(gpu_ctx, mixed, views, 1, 1)
+        _gpu_batch(gpu_ctx, mixed, views, 1, 1, V.BATCH_FP64_POLISH)
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("flags", [0, 3], ids=["plain", "sorted+graph"])
+def test_mixed_tlas_batch_matches_oracle(gpu_ctx, oracle, flags):
+    """Phantom hair, LSS capsules and a triangle mesh in one TLAS (the per-instance dispatch,
+    one lane per path) against the oracle, with material tints per instance."""
+    import gpu_util as G
+    import scenes
+    import vkhrt_amd as V
+    curves = scenes.dense_ball_curves(nstrands=1500, nseg=8)
+    caps = oracle.generate_lss(scenes.lines_from_curves(curves[::2], 2))
+    verts, tri = scenes.uv_sphere(4.0, 16, 32, (0.0, 22.0, 0.0))
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    S.add(oracle.MODE_LSS, caps, transform=XF)
+    S.add_mesh(verts, tri)
+    S.set_material(1, 1)
+    S.set_material(2, 1)
+    b_ph = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
+    d_caps = G.dev(caps)
+    b_lss = V.BottomLevelAccelerationStructure(gpu_ctx, V.GEOMETRY_LSS, d_caps.data_ptr(), caps.shape[0],
+                                               transform=XF.reshape(12), material_index=1, stream=G.stream_ptr())
+    dv, di = G.dev(verts), G.dev(tri)
+    b_tri = V.BottomLevelAccelerationStructure(gpu_ctx, V.GEOMETRY_TRIANGLES, dv.data_ptr(), tri.shape[0],
+                                               indices_ptr=di.data_ptr(), vertex_count=verts.shape[0],
+                                               material_index=1, stream=G.stream_ptr())
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [b_ph, b_lss, b_tri], stream=G.stream_ptr())
+    torch.cuda.synchronize()
+    views = _views(oracle, scenes.DENSE_VIEWS)
+    gflags = (V.BATCH_SORT_MATERIALS | V.BATCH_SORT_RAYS | V.BATCH_GRAPH) if flags else 0
+    print(f"[mixed batch] flags {gflags}: 3 views, 3 bounces", flush=True)
+    for seed in (4, 5):
+        gi, gp, (_, nb, ns, _) = _gpu_batch(gpu_ctx, tlas, views, 3, seed, gflags)
+        ri, rp, rnb, rns, _ = S.render_batch(views, bounces=3, seed=seed, env=ENV, albedo=TINTS)
+        assert (nb, ns) == (rnb, rns) and nb > 500 and ns > 500
+        _assert_same(gi, gp, ri, rp, f"mixed batch seed {seed}")
+    tlas.close()
 
 
 def test_cpp_mirror_batch(tmp_path, oracle):

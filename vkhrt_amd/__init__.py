@@ -49,6 +49,8 @@ BATCH_SORT_MATERIALS = 1 << 5
 BATCH_GRAPH = 1 << 6
 BATCH_SORT_RAYS = 1 << 7
 
+PRIMITIVE_TRIANGLES, PRIMITIVE_LINES = 0, 1
+
 STATUS = {0: "SUCCESS", -1: "INVALID_ARGUMENT", -2: "OUT_OF_MEMORY", -3: "DEVICE",
           -4: "UNSUPPORTED", -5: "STACK_OVERFLOW", -6: "NO_DEVICE"}
 
@@ -66,6 +68,9 @@ EXPORTS = [
     "vkhrt_shadow_mlp_set_params", "vkhrt_shadow_mlp_collect", "vkhrt_shadow_mlp_train",
     "vkhrt_shadow_mlp_gradient", "vkhrt_shadow_mlp_infer", "vkhrt_render_batch", "vkhrt_batch_stats_get",
     "vkhrt_autotune_render", "vkhrt_autotune_batch", "vkhrt_autotune_blas", "vkhrt_generate_surface_shadow_rays",
+    "vkhrt_asset_load", "vkhrt_asset_load_memory", "vkhrt_asset_destroy", "vkhrt_asset_last_error",
+    "vkhrt_asset_info_get", "vkhrt_asset_mesh_get", "vkhrt_asset_node_get", "vkhrt_asset_material_get",
+    "vkhrt_asset_instance_get", "vkhrt_asset_build", "vkhrt_generate_mesh_lines",
 ]
 
 
@@ -167,6 +172,37 @@ class StrandParams(ctypes.Structure):
                 ("sphere_radius", "step", "gravity", "curl", "r_root", "r_tip")]
 
 
+class AssetInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("mesh_count", "node_count", "material_count", "instance_count",
+                                               "line_mesh_count", "triangle_mesh_count")]
+
+
+class AssetMesh(ctypes.Structure):
+    _fields_ = [("primitive", ctypes.c_uint32), ("material", ctypes.c_int32), ("vertex_count", ctypes.c_uint32),
+                ("index_count", ctypes.c_uint32), ("vertices", ctypes.POINTER(ctypes.c_float)),
+                ("indices", ctypes.POINTER(ctypes.c_uint32)), ("bounds_min", ctypes.c_float * 3),
+                ("bounds_max", ctypes.c_float * 3), ("name", ctypes.c_char_p)]
+
+
+class AssetNode(ctypes.Structure):
+    _fields_ = [("parent", ctypes.c_int32), ("mesh_count", ctypes.c_uint32), ("meshes", ctypes.POINTER(ctypes.c_uint32)),
+                ("local", ctypes.c_float * 16), ("world", ctypes.c_float * 16), ("name", ctypes.c_char_p)]
+
+
+class AssetMaterial(ctypes.Structure):
+    _fields_ = [("base_color", ctypes.c_float * 4), ("metallic", ctypes.c_float), ("roughness", ctypes.c_float),
+                ("emissive", ctypes.c_float * 3), ("pad", ctypes.c_float)]
+
+
+class AssetInstance(ctypes.Structure):
+    _fields_ = [("node", ctypes.c_uint32), ("mesh", ctypes.c_uint32), ("transform", ctypes.c_float * 12)]
+
+
+class AssetBuildDesc(ctypes.Structure):
+    _fields_ = [("hair_mode", ctypes.c_uint32), ("radius", ctypes.c_float), ("tension", ctypes.c_float),
+                ("leaf_size", ctypes.c_uint32), ("default_material", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -229,6 +265,18 @@ def lib():
                                   ctypes.POINTER(Tuning), P], i32),
         "vkhrt_autotune_blas": ([P, ctypes.POINTER(BlasDesc), ctypes.POINTER(Camera), ctypes.POINTER(RenderDesc), u32,
                                  ctypes.POINTER(Tuning), P], i32),
+        "vkhrt_asset_load": ([ctypes.c_char_p, ctypes.POINTER(P)], i32),
+        "vkhrt_asset_load_memory": ([P, ctypes.c_size_t, ctypes.c_char_p, ctypes.POINTER(P)], i32),
+        "vkhrt_asset_destroy": ([P], None),
+        "vkhrt_asset_last_error": ([], ctypes.c_char_p),
+        "vkhrt_asset_info_get": ([P, ctypes.POINTER(AssetInfo)], i32),
+        "vkhrt_asset_mesh_get": ([P, u32, ctypes.POINTER(AssetMesh)], i32),
+        "vkhrt_asset_node_get": ([P, u32, ctypes.POINTER(AssetNode)], i32),
+        "vkhrt_asset_material_get": ([P, u32, ctypes.POINTER(AssetMaterial)], i32),
+        "vkhrt_asset_instance_get": ([P, u32, ctypes.POINTER(AssetInstance)], i32),
+        "vkhrt_asset_build": ([P, P, ctypes.POINTER(AssetBuildDesc), P, ctypes.POINTER(P), u32,
+                               ctypes.POINTER(u32)], i32),
+        "vkhrt_generate_mesh_lines": ([P, P, u32, P, u32, f32, P, P], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -391,6 +439,12 @@ class Context:
         self.check(lib().vkhrt_generate_dots(self.h, lines_ptr, count, constant_radius, pieces_ptr,
                                              stream), "generate_dots")
 
+    def generate_mesh_lines(self, vertices_ptr, vertex_count, indices_ptr, count, lines_ptr, radius=0.02,
+                            stream=None):
+        """GenerateLines of an indexed line mesh (8-float vertices, mesh-local index pairs)."""
+        self.check(lib().vkhrt_generate_mesh_lines(self.h, vertices_ptr, vertex_count, indices_ptr, count, radius,
+                                                   lines_ptr, stream), "generate_mesh_lines")
+
     # auxiliary ray work
     def generate_camera_rays(self, camera, width, height, rays_ptr, tmin=0.001, tmax=10000.0,
                              stream=None):
@@ -473,10 +527,122 @@ class BottomLevelAccelerationStructure:
         self.ctx, self.h, self.mode = ctx, h, mode
         self._destroy = lib().vkhrt_blas_destroy
 
+    @classmethod
+    def _adopt(cls, ctx, handle, mode):
+        """Wraps a vkhrt_blas* made by another entry point (it is destroyed with this object)."""
+        obj = cls.__new__(cls)
+        obj.ctx, obj.h, obj.mode = ctx, handle, mode
+        obj._destroy = lib().vkhrt_blas_destroy
+        return obj
+
     def info(self):
         i = BlasInfo()
         lib().vkhrt_blas_info_get(self.h, ctypes.byref(i))
         return {k: getattr(i, k) for k, _ in BlasInfo._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Asset:
+    """A glTF 2.0 polyline / triangle asset: ModelLoader::LoadFromFile
+    (source/resources/model/model_loader.cpp:274-312) and the SceneGraph it builds, read by the
+    library's own glTF reader on the host. Meshes are mesh primitives in file order, nodes are
+    in pre-order with ``world`` = Node::GetWorldMatrix (model.cpp:6-18), column-major.
+
+    ``Asset(path)`` reads a .gltf / .glb file; ``Asset(data=bytes, base_dir=...)`` reads one
+    from memory. Errors raise VkhrtError with the loader's message."""
+
+    def __init__(self, path=None, data=None, base_dir=None):
+        h = ctypes.c_void_p()
+        if path is not None:
+            rc = lib().vkhrt_asset_load(os.fsencode(path), ctypes.byref(h))
+        else:
+            buf = bytes(data)
+            rc = lib().vkhrt_asset_load_memory(buf, len(buf), os.fsencode(base_dir) if base_dir else None,
+                                               ctypes.byref(h))
+        if rc:
+            raise VkhrtError(rc, f"asset_load: {lib().vkhrt_asset_last_error().decode()}")
+        self.h = h
+        self._destroy = lib().vkhrt_asset_destroy
+
+    def _get(self, fn, i, struct):
+        out = struct()
+        rc = fn(self.h, i, ctypes.byref(out))
+        if rc:
+            raise VkhrtError(rc, lib().vkhrt_asset_last_error().decode())
+        return out
+
+    def info(self):
+        i = AssetInfo()
+        lib().vkhrt_asset_info_get(self.h, ctypes.byref(i))
+        return {k: getattr(i, k) for k, _ in AssetInfo._fields_}
+
+    def mesh(self, i):
+        """dict: primitive, material, vertices (n, 8) float32 (position, normal, texCoord),
+        indices (uint32, mesh-local), bounds_min, bounds_max, name; arrays are copies."""
+        import numpy as np
+        m = self._get(lib().vkhrt_asset_mesh_get, i, AssetMesh)
+        verts = (np.ctypeslib.as_array(m.vertices, (m.vertex_count * 8,)).reshape(-1, 8).copy()
+                 if m.vertex_count else np.zeros((0, 8), np.float32))
+        idx = (np.ctypeslib.as_array(m.indices, (m.index_count,)).copy() if m.index_count
+               else np.zeros(0, np.uint32))
+        return {"primitive": m.primitive, "material": m.material, "vertices": verts, "indices": idx,
+                "bounds_min": np.array(m.bounds_min[:], np.float32), "bounds_max": np.array(m.bounds_max[:], np.float32),
+                "name": (m.name or b"").decode()}
+
+    def node(self, i):
+        """dict: parent, meshes, local and world (16 floats, column-major), name."""
+        import numpy as np
+        n = self._get(lib().vkhrt_asset_node_get, i, AssetNode)
+        return {"parent": n.parent, "meshes": [n.meshes[k] for k in range(n.mesh_count)],
+                "local": np.array(n.local[:], np.float32), "world": np.array(n.world[:], np.float32),
+                "name": (n.name or b"").decode()}
+
+    def material(self, i):
+        m = self._get(lib().vkhrt_asset_material_get, i, AssetMaterial)
+        return {"base_color": list(m.base_color), "metallic": m.metallic, "roughness": m.roughness,
+                "emissive": list(m.emissive)}
+
+    def instance(self, i):
+        """dict: node, mesh, transform (row-major 3x4 world-from-object, 12 floats)."""
+        import numpy as np
+        t = self._get(lib().vkhrt_asset_instance_get, i, AssetInstance)
+        return {"node": t.node, "mesh": t.mesh, "transform": np.array(t.transform[:], np.float32)}
+
+    @property
+    def meshes(self):
+        return [self.mesh(i) for i in range(self.info()["mesh_count"])]
+
+    @property
+    def nodes(self):
+        return [self.node(i) for i in range(self.info()["node_count"])]
+
+    @property
+    def instances(self):
+        return [self.instance(i) for i in range(self.info()["instance_count"])]
+
+    def build(self, ctx, hair_mode=GEOMETRY_PHANTOM_CURVES, radius=0.02, tension=1.0, leaf_size=0,
+              default_material=0, stream=None):
+        """One BLAS per instance (Renderer::InitializeBLAS, renderer.cpp:694-728), built on the
+        device; pass the list to TopLevelAccelerationStructure."""
+        d = AssetBuildDesc(hair_mode, radius, tension, leaf_size, default_material)
+        n = self.info()["instance_count"]
+        arr = (ctypes.c_void_p * max(1, n))()
+        cnt = ctypes.c_uint32()
+        ctx.check(lib().vkhrt_asset_build(ctx.h, self.h, ctypes.byref(d), stream, arr, n, ctypes.byref(cnt)),
+                  "asset_build")
+        out = []
+        for i in range(cnt.value):
+            inst = self.instance(i)
+            mode = hair_mode if self.mesh(inst["mesh"])["primitive"] == PRIMITIVE_LINES else GEOMETRY_TRIANGLES
+            out.append(BottomLevelAccelerationStructure._adopt(ctx, ctypes.c_void_p(arr[i]), mode))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

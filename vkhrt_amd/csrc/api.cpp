@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "vkhrt.h"
+#include "asset.hpp"
 #include "build.hpp"
 #include "geometry.hpp"
 #include "internal.hpp"
@@ -1039,8 +1040,8 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     if (d->bounces > VKHRT_BATCH_MAX_BOUNCES) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bounces above the maximum");
     if (d->material_count > VKHRT_BATCH_MAX_MATERIALS || (d->material_count && !d->material_albedo))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad material table");
-    if (t->variant == vk::kVarMixed)
-        return fail(c, VKHRT_ERROR_UNSUPPORTED, "batched path tracing needs a TLAS of one geometry mode");
+    if (t->variant == vk::kVarMixed && (d->flags & VKHRT_BATCH_FP64_POLISH))
+        return fail(c, VKHRT_ERROR_UNSUPPORTED, "batched FP64 polish needs a TLAS of one geometry mode");
     if (is_phantom(t) && !t->polish_ok)
         return fail(c, VKHRT_ERROR_UNSUPPORTED,
                     "batched phantom tracing needs < 2^26 primitives per BLAS and <= 31 instances");
@@ -1556,6 +1557,127 @@ int vkhrt_generate_dots(vkhrt_context* c, const float* lines, uint32_t n, float 
     hipError_t e = vk::gen_dots(reinterpret_cast<const float4*>(lines), n, constant_radius,
                                 reinterpret_cast<float4*>(pieces), pick(c, stream));
     return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "generate dots");
+}
+int vkhrt_generate_mesh_lines(vkhrt_context* c, const float* v, uint32_t nv, const uint32_t* idx, uint32_t n,
+                              float radius, float* lines, void* stream) {
+    if (!c || (n && (!v || !idx || !lines))) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad argument");
+    if (n == 0) return VKHRT_SUCCESS;
+    if (nv == 0) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "line indices given without vertices");
+    hipSetDevice(c->device);
+    hipStream_t st = pick(c, stream);
+    uint32_t* bad = nullptr;
+    uint32_t nbad = 0;
+    hipError_t e = hipMalloc(&bad, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(uint32_t), st);
+    if (e == hipSuccess) e = vk::gen_mesh_lines(v, nv, idx, n, radius, reinterpret_cast<float4*>(lines), bad, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&nbad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(bad);
+    if (e != hipSuccess) return hip_fail(c, e, "generate mesh lines");
+    if (nbad) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, std::to_string(nbad) + " line indices are not below the vertex count");
+    return VKHRT_SUCCESS;
+}
+
+// ---- assets
+int vkhrt_asset_build(vkhrt_context* c, const vkhrt_asset* a, const vkhrt_asset_build_desc* d, void* stream,
+                      vkhrt_blas** blases, uint32_t capacity, uint32_t* count) {
+    if (!c || !a || !d) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    const uint32_t n = uint32_t(a->instances.size());
+    if (count) *count = n;
+    if (!blases) return capacity == 0 ? VKHRT_SUCCESS : fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null blases");
+    if (capacity < n)
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT,
+                    "capacity " + std::to_string(capacity) + " is below the asset's " + std::to_string(n) + " instances");
+    if (d->hair_mode > VKHRT_GEOMETRY_DOTS)
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "hair mode must be phantom curves, LSS or DOTS");
+    if (!(d->radius >= 0.0f) || !std::isfinite(d->radius))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "radius must be finite and >= 0");
+    if (!std::isfinite(d->tension)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "tension must be finite");
+    if (d->leaf_size > 3) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "leaf size must be 0..3");
+    for (uint32_t i = 0; i < n; ++i) blases[i] = nullptr;
+    hipSetDevice(c->device);
+    hipStream_t st = pick(c, stream);
+    // device records per mesh, generated at its first instance and shared by the others
+    struct Gen {
+        float* verts = nullptr;
+        uint32_t* idx = nullptr;
+        float* lines = nullptr;
+        float* recs = nullptr;
+        uint32_t count = 0;
+        bool done = false;
+    };
+    std::vector<Gen> gen(a->meshes.size());
+    int rc = VKHRT_SUCCESS;
+    for (uint32_t i = 0; i < n && rc == VKHRT_SUCCESS; ++i) {
+        const vk::AssetInstance& I = a->instances[i];
+        const vk::AssetMesh& M = a->meshes[I.mesh];
+        const bool hair = M.primitive == VKHRT_PRIMITIVE_LINES;
+        const uint32_t nv = uint32_t(M.vertices.size() / 8), ni = uint32_t(M.indices.size());
+        Gen& g = gen[I.mesh];
+        if (!g.done) {
+            g.done = true;
+            g.count = hair ? ni / 2 : ni / 3;
+            if (g.count) {
+                hipError_t e = hipMalloc(&g.verts, M.vertices.size() * sizeof(float));
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(g.verts, M.vertices.data(), M.vertices.size() * sizeof(float),
+                                       hipMemcpyHostToDevice, st);
+                if (e == hipSuccess) e = hipMalloc(&g.idx, size_t(ni) * sizeof(uint32_t));
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(g.idx, M.indices.data(), size_t(ni) * sizeof(uint32_t), hipMemcpyHostToDevice, st);
+                if (e == hipSuccess && hair) e = hipMalloc(&g.lines, size_t(g.count) * 8 * sizeof(float));
+                if (e != hipSuccess) {
+                    rc = hip_fail(c, e, "asset upload");
+                    break;
+                }
+                if (!hair) {
+                    g.recs = g.verts;
+                } else {
+                    rc = vkhrt_generate_mesh_lines(c, g.verts, nv, g.idx, g.count, d->radius, g.lines, stream);
+                    if (rc == VKHRT_SUCCESS && d->hair_mode == VKHRT_GEOMETRY_DOTS) {
+                        g.recs = g.lines;
+                    } else if (rc == VKHRT_SUCCESS) {
+                        const bool curves = d->hair_mode == VKHRT_GEOMETRY_PHANTOM_CURVES;
+                        e = hipMalloc(&g.recs, size_t(g.count) * (curves ? 16 : 8) * sizeof(float));
+                        if (e != hipSuccess) rc = hip_fail(c, e, "asset records");
+                        else if (curves) rc = vkhrt_generate_curves(c, g.lines, g.count, d->tension, g.recs, stream);
+                        else rc = vkhrt_generate_lss(c, g.lines, g.count, d->radius, g.recs, stream);
+                    }
+                }
+                if (rc != VKHRT_SUCCESS) break;
+            }
+        }
+        vkhrt_blas_desc bd{};
+        bd.mode = hair ? d->hair_mode : uint32_t(VKHRT_GEOMETRY_TRIANGLES);
+        bd.flags = VKHRT_BLAS_HAS_TRANSFORM;
+        bd.primitives = g.recs;
+        bd.count = g.count;
+        bd.radius = d->radius;
+        bd.material_index = M.material >= 0 ? uint32_t(M.material) : d->default_material;
+        bd.leaf_size = d->leaf_size;
+        const float* W = a->nodes[I.node].world;
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 4; ++k) bd.transform[r * 4 + k] = W[k * 4 + r];
+        if (!hair) {
+            bd.indices = g.idx;
+            bd.vertex_count = nv;
+        }
+        rc = vkhrt_blas_build(c, &bd, stream, &blases[i]);
+        if (rc != VKHRT_SUCCESS) c->err = "asset instance " + std::to_string(i) + ": " + c->err;
+    }
+    hipStreamSynchronize(st);
+    for (Gen& g : gen) {
+        if (g.recs != g.verts && g.recs != g.lines) hipFree(g.recs);
+        hipFree(g.verts);
+        hipFree(g.idx);
+        hipFree(g.lines);
+    }
+    if (rc != VKHRT_SUCCESS)
+        for (uint32_t i = 0; i < n; ++i) {
+            vkhrt_blas_destroy(blases[i]);
+            blases[i] = nullptr;
+        }
+    return rc;
 }
 
 // ---- profiling

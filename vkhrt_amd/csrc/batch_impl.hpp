@@ -213,6 +213,24 @@ __global__ __launch_bounds__(kTraceBlock) void k_batch_trace(BatchParams bp, uin
         });
 }
 
+// Mixed-mode TLASes (per-instance geometry dispatch): one lane per queue slot, the queue length
+// read on the device; the grid covers every path.
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_batch_trace_flat(BatchParams bp, uint32_t b) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t qi = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (qi >= bp.counters[bc(b, kBcQueue)]) return;
+    const size_t p = depth_queue(bp, b)[qi];
+    const float4 r0 = bp.ray[2 * p], r1 = bp.ray[2 * p + 1];
+    const f3 o{r0.x, r0.y, r0.z}, d{r1.x, r1.y, r1.z};
+    Best best;
+    if (!trace_world<G, false, STACK>(bp.tp, o, d, r0.w, r1.w, best, stk + threadIdx.x, qi))
+        atomicAdd(&bp.tp.status[0], 1u);
+    HitRec h;
+    finish_hit(bp.tp, o, d, best, h);
+    bp.hits[qi] = h;
+}
+
 // Candidate sources of the polish kernels: queue slot -> world ray.
 struct SrcBatch {
     const uint32_t* q;
@@ -376,6 +394,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_batch_shadow(BatchParams bp, ui
         });
 }
 
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_batch_shadow_flat(BatchParams bp, uint32_t b) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t qi = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (qi >= bp.counters[bc(b, kBcShadow)]) return;
+    f3 o, d;
+    float tmin, tmax;
+    SrcBatchShadow{bp.squeue, bp.ray}.get(qi, o, d, tmin, tmax);
+    Best best;
+    if (!trace_world<G, true, STACK>(bp.tp, o, d, tmin, tmax, best, stk + threadIdx.x, qi))
+        atomicAdd(&bp.tp.status[0], 1u);
+    bp.nee[bp.squeue[qi]].w = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
+}
+
 // Adds the light contribution of the unoccluded shadow rays of depth b.
 __global__ void k_batch_apply(BatchParams bp, uint32_t b) {
     const uint32_t n = bp.counters[bc(b, kBcShadow)];
@@ -408,9 +440,7 @@ __global__ void k_batch_resolve(BatchParams bp) {
 template <class G, int STACK>
 hipError_t launch_batch_variant(const BatchParams& bp0, hipStream_t st) {
     using namespace trace_detail;
-    if constexpr (G::kMixed) {
-        return hipErrorNotSupported;
-    } else {
+    {
         const uint32_t total = bp0.total;
         if (total == 0) return hipSuccess;
         const dim3 ab(kAppendBlock), tb(kTraceBlock);
@@ -428,8 +458,12 @@ hipError_t launch_batch_variant(const BatchParams& bp0, hipStream_t st) {
                 hipLaunchKernelGGL(k_batch_ray_scatter, strided, ab, 0, st, bp, b);
             }
             bp.tp.defer_count = bp0.counters + bc(b, kBcDefer);
-            hipLaunchKernelGGL((k_batch_trace<G, STACK>), persistent_grid(k_batch_trace<G, STACK>, bp.tp.blocks_per_cu),
-                               tb, 0, st, bp, b);
+            if constexpr (G::kMixed)
+                hipLaunchKernelGGL((k_batch_trace_flat<G, STACK>), dim3((total + kTraceBlock - 1) / kTraceBlock), tb, 0,
+                                   st, bp, b);
+            else
+                hipLaunchKernelGGL((k_batch_trace<G, STACK>),
+                                   persistent_grid(k_batch_trace<G, STACK>, bp.tp.blocks_per_cu), tb, 0, st, bp, b);
             if (bp.tp.defer) {
                 const SrcBatch src{depth_queue(bp, b), bp.ray};
                 hipLaunchKernelGGL(k_polish_closest<SrcBatch>, pgrid, tb, 0, st, bp.tp, src, bp.tp.cand_hits,
@@ -442,8 +476,12 @@ hipError_t launch_batch_variant(const BatchParams& bp0, hipStream_t st) {
             }
             hipLaunchKernelGGL(k_batch_shade, strided, ab, 0, st, bp, b);
             bp.tp.defer_count = bp0.counters + bc(b, kBcDeferShadow);
-            hipLaunchKernelGGL((k_batch_shadow<G, STACK>), persistent_grid(k_batch_shadow<G, STACK>, bp.tp.blocks_per_cu),
-                               tb, 0, st, bp, b);
+            if constexpr (G::kMixed)
+                hipLaunchKernelGGL((k_batch_shadow_flat<G, STACK>), dim3((total + kTraceBlock - 1) / kTraceBlock), tb, 0,
+                                   st, bp, b);
+            else
+                hipLaunchKernelGGL((k_batch_shadow<G, STACK>),
+                                   persistent_grid(k_batch_shadow<G, STACK>, bp.tp.blocks_per_cu), tb, 0, st, bp, b);
             if (bp.tp.defer)
                 hipLaunchKernelGGL(k_batch_polish_shadow, pgrid, tb, 0, st, bp.tp, SrcBatchShadow{bp.squeue, bp.ray},
                                    bp.nee);

@@ -213,7 +213,35 @@ __global__ void k_tri_records(const float* __restrict__ verts, uint32_t nverts, 
     }
 }
 
+// GenerateLines (geometry_processor.cpp:45-67) for an indexed line mesh: line i joins the
+// positions of vertices idx[2i] and idx[2i + 1], both indices mesh-local (the reference's
+// firstVertex is added once, not again as at :61-62), with radius r at both ends. An index past
+// the vertex count is replaced by 0 and counted in `bad`.
+__global__ void k_mesh_lines(const float* __restrict__ verts, uint32_t nverts, const uint32_t* __restrict__ idx,
+                             uint32_t n, float r, float4* __restrict__ lines, uint32_t* __restrict__ bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t miss = 0;
+    for (int k = 0; k < 2; ++k) {
+        uint32_t v = idx[size_t(i) * 2 + k];
+        if (v >= nverts) {
+            ++miss;
+            v = 0;
+        }
+        const float* p = verts + size_t(v) * 8;
+        lines[size_t(i) * 2 + k] = make_float4(p[0], p[1], p[2], r);
+    }
+    if (miss) atomicAdd(bad, miss);
+}
+
 }  // namespace
+
+hipError_t gen_mesh_lines(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t n, float radius,
+                          float4* lines, uint32_t* bad, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mesh_lines, grid(n), dim3(kBlock), 0, st, verts, nverts, idx, n, radius, lines, bad);
+    return hipGetLastError();
+}
 
 hipError_t gen_tri_records(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri, float4* recs,
                            float4* normals, uint32_t* bad, hipStream_t st) {

@@ -71,7 +71,7 @@ hipError_t launch_batch(const BatchParams& bp, int variant, int stack_kind, hipS
         case kVarLss: return batch_stack<LssG>(bp, stack_kind, st);
         case kVarDots: return batch_stack<DotsG>(bp, stack_kind, st);
         case kVarTris: return batch_stack<TrisG>(bp, stack_kind, st);
-        default: return hipErrorNotSupported;
+        default: return batch_stack<GeoMixed>(bp, stack_kind, st);
     }
 }
 
