@@ -3,6 +3,8 @@
  *   BLASType, BLASInput, BottomLevelAccelerationStructure
  *       (include/bottom_level_acceleration_structure.hpp:12-50)
  *   TopLevelAccelerationStructure (include/top_level_acceleration_structure.hpp:9-23)
+ *   ModelLoader, Model, SceneGraph, Node, Mesh (include/resources/model/model_loader.hpp:12-30,
+ *       include/resources/model/model.hpp:6-111) over the vkhrt_asset_* loader
  *   VulkanContext -> vkhrt::Context (one device, one stream per call)
  * The Vulkan geometry / build-range structures of BLASInput become a primitive pointer and a
  * count; the type/geometry pairing follows Renderer::InitializeBLAS
@@ -15,6 +17,7 @@
 #ifndef VKHRT_HPP
 #define VKHRT_HPP
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <memory>
@@ -123,6 +126,12 @@ public:
             for (int c = 0; c < 4; ++c) d.transform[r * 4 + c] = input.transform[c * 4 + r];
         _context->Check(vkhrt_blas_build(_context->Handle(), &d, stream, &_blas), "vkhrt_blas_build");
     }
+    // Takes ownership of a BLAS made by another entry point (e.g. vkhrt_asset_build).
+    static BottomLevelAccelerationStructure Adopt(vkhrt_blas* blas, BLASType type,
+                                                  const std::array<float, 16>& transform,
+                                                  const std::shared_ptr<Context>& context) {
+        return BottomLevelAccelerationStructure(blas, type, transform, context);
+    }
     ~BottomLevelAccelerationStructure() { vkhrt_blas_destroy(_blas); }
     BottomLevelAccelerationStructure(BottomLevelAccelerationStructure&& other) noexcept
         : _blas(std::exchange(other._blas, nullptr)),
@@ -143,6 +152,10 @@ public:
     }
 
 private:
+    BottomLevelAccelerationStructure(vkhrt_blas* blas, BLASType type, const std::array<float, 16>& transform,
+                                     const std::shared_ptr<Context>& context)
+        : _blas(blas), _type(type), _transform(transform), _context(context) {}
+
     vkhrt_blas* _blas = nullptr;
     BLASType _type;
     std::array<float, 16> _transform;
@@ -215,6 +228,140 @@ public:
 private:
     vkhrt_tlas* _tlas = nullptr;
     std::shared_ptr<Context> _context;
+};
+
+// ---- models: include/resources/model/model.hpp (Node :6-16, AABB :18-22, Mesh :42-66,
+// SceneGraph :98-111) and ModelLoader (model_loader.hpp:12-30). Each mesh keeps its own
+// vertex and index arrays (mesh-local indices), a node names its parent by index, and the
+// loader throws Error where the reference logs and returns nullptr (model_loader.cpp:280-284).
+struct AABB {
+    std::array<float, 3> min{};
+    std::array<float, 3> max{};
+};
+
+struct Node {
+    std::string name;
+    int32_t parent = -1;               // index into SceneGraph::nodes, -1 for the root
+    std::array<float, 16> localMatrix{};  // column-major (glm::mat4)
+    std::array<float, 16> worldMatrix{};  // parents applied from the nearest (model.cpp:6-18)
+    std::vector<uint32_t> meshes;
+    std::array<float, 16> GetWorldMatrix() const { return worldMatrix; }
+};
+
+struct Mesh {
+    enum class PrimitiveType : uint8_t {
+        eTriangles,
+        eLines,
+    };
+    PrimitiveType primitiveType = PrimitiveType::eTriangles;
+    uint32_t vertexCount = 0;
+    uint32_t indexCount = 0;
+    const float* vertices = nullptr;    // vertexCount x Vertex (position, normal, texCoord), host
+    const uint32_t* indices = nullptr;  // mesh-local, host
+    int32_t material = -1;              // index into SceneGraph::materials, -1 = none
+    AABB boundingBox;
+    std::string name;
+    uint32_t GetIndicesPerFaceNum() const { return primitiveType == PrimitiveType::eLines ? 2u : 3u; }
+};
+
+struct SceneGraph {
+    std::vector<Node> nodes;  // pre-order, nodes[0] is the root
+    std::vector<Mesh> meshes;
+    std::vector<vkhrt_asset_material> materials;
+};
+
+class Model {
+public:
+    explicit Model(vkhrt_asset* asset) : _asset(asset), sceneGraph(std::make_shared<SceneGraph>()) {
+        vkhrt_asset_info info{};
+        vkhrt_asset_info_get(_asset, &info);
+        for (uint32_t i = 0; i < info.mesh_count; ++i) {
+            vkhrt_asset_mesh m{};
+            vkhrt_asset_mesh_get(_asset, i, &m);
+            Mesh& M = sceneGraph->meshes.emplace_back();
+            M.primitiveType = m.primitive == VKHRT_PRIMITIVE_LINES ? Mesh::PrimitiveType::eLines
+                                                                   : Mesh::PrimitiveType::eTriangles;
+            M.vertexCount = m.vertex_count;
+            M.indexCount = m.index_count;
+            M.vertices = m.vertices;
+            M.indices = m.indices;
+            M.material = m.material;
+            for (int k = 0; k < 3; ++k) {
+                M.boundingBox.min[k] = m.bounds_min[k];
+                M.boundingBox.max[k] = m.bounds_max[k];
+            }
+            M.name = m.name ? m.name : "";
+        }
+        for (uint32_t i = 0; i < info.node_count; ++i) {
+            vkhrt_asset_node n{};
+            vkhrt_asset_node_get(_asset, i, &n);
+            Node& N = sceneGraph->nodes.emplace_back();
+            N.name = n.name ? n.name : "";
+            N.parent = n.parent;
+            std::copy(n.local, n.local + 16, N.localMatrix.begin());
+            std::copy(n.world, n.world + 16, N.worldMatrix.begin());
+            N.meshes.assign(n.meshes, n.meshes + n.mesh_count);
+        }
+        for (uint32_t i = 0; i < info.material_count; ++i) {
+            vkhrt_asset_material m{};
+            vkhrt_asset_material_get(_asset, i, &m);
+            sceneGraph->materials.push_back(m);
+        }
+    }
+    ~Model() { vkhrt_asset_destroy(_asset); }
+    Model(const Model&) = delete;
+    Model& operator=(const Model&) = delete;
+
+    const vkhrt_asset* Handle() const noexcept { return _asset; }
+
+    // ProcessHairCurves / ProcessHairLSS / ProcessHairDOTS on the device and
+    // Renderer::InitializeBLAS for this model (renderer.cpp:694-728): one BLAS per (node, mesh)
+    // in node pre-order, line meshes as `hairMode`, triangle meshes as eMesh triangles.
+    std::vector<BottomLevelAccelerationStructure> BuildBLASes(const std::shared_ptr<Context>& context,
+                                                              vkhrt_geometry_mode hairMode = VKHRT_GEOMETRY_PHANTOM_CURVES,
+                                                              float radius = 0.02f, float tension = 1.0f,
+                                                              uint32_t defaultMaterial = 0, void* stream = nullptr) const {
+        if (!context) throw Error(VKHRT_ERROR_INVALID_ARGUMENT, "Model::BuildBLASes: no context");
+        vkhrt_asset_build_desc d{};
+        d.hair_mode = uint32_t(hairMode);
+        d.radius = radius;
+        d.tension = tension;
+        d.default_material = defaultMaterial;
+        uint32_t n = 0;
+        context->Check(vkhrt_asset_build(context->Handle(), _asset, &d, stream, nullptr, 0, &n), "vkhrt_asset_build");
+        std::vector<vkhrt_blas*> raw(n, nullptr);
+        context->Check(vkhrt_asset_build(context->Handle(), _asset, &d, stream, raw.data(), n, &n), "vkhrt_asset_build");
+        std::vector<BottomLevelAccelerationStructure> out;
+        out.reserve(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            vkhrt_asset_instance inst{};
+            vkhrt_asset_instance_get(_asset, i, &inst);
+            const Mesh& M = sceneGraph->meshes[inst.mesh];
+            const BLASType type = M.primitiveType == Mesh::PrimitiveType::eLines && hairMode == VKHRT_GEOMETRY_PHANTOM_CURVES
+                                      ? BLASType::eHair
+                                      : BLASType::eMesh;
+            out.push_back(BottomLevelAccelerationStructure::Adopt(raw[i], type, sceneGraph->nodes[inst.node].worldMatrix,
+                                                                  context));
+        }
+        return out;
+    }
+
+private:
+    vkhrt_asset* _asset = nullptr;
+
+public:
+    std::shared_ptr<SceneGraph> sceneGraph;
+};
+
+class ModelLoader {
+public:
+    // .gltf (external buffers or data: URIs) or .glb.
+    std::shared_ptr<Model> LoadFromFile(const std::string& path) const {
+        vkhrt_asset* a = nullptr;
+        int rc = vkhrt_asset_load(path.c_str(), &a);
+        if (rc != VKHRT_SUCCESS) throw Error(rc, "ModelLoader::LoadFromFile: " + std::string(vkhrt_asset_last_error()));
+        return std::make_shared<Model>(a);
+    }
 };
 
 // FlyCamera view/projection (source/fly_camera.cpp:25-35), inverted as in

@@ -4,6 +4,11 @@
 //       CameraLookAt; prints "ok".
 //   host_api_test trace <curves.bin> <n> <rays.bin> <m> <hits.bin> [instances]
 //   host_api_test batch <curves.bin> <n> <cams.bin> <nviews> <W1xH1,...> <bounces> <seed> <out.bin>
+//   host_api_test assetinfo <file.gltf|.glb>
+//       ModelLoader::LoadFromFile without a device: prints the scene graph (meshes, nodes with
+//       world matrices as hex floats, instances) or the loader's error.
+//   host_api_test asset <file> <hair_mode> <rays.bin> <m> <hits.bin>
+//       Model::BuildBLASes + TLAS + closest-hit trace of m rays; writes m vkhrt_hit records.
 //       builds `instances` phantom BLASes (per-point radius, host pointers) with transforms
 //       x -> x + 10 * i, one TLAS, traces m rays (closest hit) and writes m vkhrt_hit records.
 #include <hip/hip_runtime_api.h>
@@ -157,10 +162,64 @@ static int batch(int argc, char** argv) {
     return 0;
 }
 
+static int asset_info(const char* path) {
+    std::shared_ptr<vkhrt::Model> model;
+    try {
+        model = vkhrt::ModelLoader().LoadFromFile(path);
+    } catch (const vkhrt::Error& e) {
+        std::printf("error %d %s\n", e.Code(), e.what());
+        return 0;
+    }
+    const vkhrt::SceneGraph& g = *model->sceneGraph;
+    for (const vkhrt::Mesh& m : g.meshes) {
+        unsigned long long sum = 0;  // order-sensitive digest of the index array
+        for (uint32_t i = 0; i < m.indexCount; ++i) sum = sum * 1000003ull + m.indices[i];
+        std::printf("mesh %d %u %u %d %llu %a %a\n", m.primitiveType == vkhrt::Mesh::PrimitiveType::eLines ? 1 : 0,
+                    m.vertexCount, m.indexCount, m.material, sum, double(m.boundingBox.min[0]),
+                    double(m.boundingBox.max[2]));
+    }
+    for (const vkhrt::Node& n : g.nodes) {
+        std::printf("node %d %zu", n.parent, n.meshes.size());
+        const std::array<float, 16> w = n.GetWorldMatrix();
+        for (float x : w) std::printf(" %a", double(x));
+        std::printf("\n");
+    }
+    std::printf("materials %zu\nok\n", g.materials.size());
+    return 0;
+}
+
+static int asset_trace(int argc, char** argv) {
+    if (argc < 7) return 2;
+    const int mode = std::atoi(argv[3]);
+    const size_t m = std::strtoul(argv[5], nullptr, 10);
+    std::vector<float> rays = read_floats(argv[4], m * 8);
+    auto ctx = std::make_shared<vkhrt::Context>(0);
+    auto model = vkhrt::ModelLoader().LoadFromFile(argv[2]);
+    std::vector<vkhrt::BottomLevelAccelerationStructure> blases =
+        model->BuildBLASes(ctx, vkhrt_geometry_mode(mode), 0.02f, 1.0f, 0);
+    vkhrt::TopLevelAccelerationStructure tlas(blases, ctx);
+    void *drays = nullptr, *dhits = nullptr;
+    if (hipMalloc(&drays, m * sizeof(vkhrt_ray)) != hipSuccess || hipMalloc(&dhits, m * sizeof(vkhrt_hit)) != hipSuccess)
+        return 3;
+    if (hipMemcpy(drays, rays.data(), m * sizeof(vkhrt_ray), hipMemcpyHostToDevice) != hipSuccess) return 3;
+    tlas.TraceRays(static_cast<const vkhrt_ray*>(drays), uint32_t(m), static_cast<vkhrt_hit*>(dhits));
+    ctx->Synchronize();
+    std::vector<vkhrt_hit> hits(m);
+    if (hipMemcpy(hits.data(), dhits, m * sizeof(vkhrt_hit), hipMemcpyDeviceToHost) != hipSuccess) return 3;
+    if (hipFree(drays) != hipSuccess || hipFree(dhits) != hipSuccess) return 3;
+    FILE* f = std::fopen(argv[6], "wb");
+    if (!f || std::fwrite(hits.data(), sizeof(vkhrt_hit), m, f) != m) return 4;
+    std::fclose(f);
+    std::printf("traced %zu rays against %zu BLASes of %s\n", m, blases.size(), argv[2]);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::strcmp(argv[1], "nodevice") == 0) return no_device();
     if (argc >= 2 && std::strcmp(argv[1], "trace") == 0) return trace(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "batch") == 0) return batch(argc, argv);
+    if (argc >= 3 && std::strcmp(argv[1], "assetinfo") == 0) return asset_info(argv[2]);
+    if (argc >= 2 && std::strcmp(argv[1], "asset") == 0) return asset_trace(argc, argv);
     std::fprintf(stderr,
                  "usage: host_api_test nodevice | trace curves n rays m hits [instances]\n"
                  "       | batch curves n cams nviews WxH,... bounces seed out\n");

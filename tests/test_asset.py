@@ -407,3 +407,39 @@ def test_mesh_lines_oracle_and_the_reference_offset():
     assert twice.max() >= allpos.shape[0] or not np.array_equal(allpos[twice[0::2]], p1[i1[0::2]])
     with pytest.raises(ValueError):
         O.mesh_lines(m1["vertices"], np.array([0, m1["vertices"].shape[0]], np.uint32))
+
+
+def test_cpp_model_loader_mirror(tmp_path):
+    """include/vkhrt.hpp ModelLoader / Model / SceneGraph (host only) report the same scene graph
+    as the C-ABI: meshes, index arrays, bounds, node parents and world matrices bit for bit."""
+    import subprocess
+    from test_host import cpp_test_binary
+    rng = np.random.default_rng(11)
+    g = GW.Gltf()
+    _hierarchy(g, rng)
+    g.material(base_color=[1.0, 0.5, 0.5, 1.0])
+    p = tmp_path / "h.glb"
+    p.write_bytes(g.glb())
+    r = subprocess.run([cpp_test_binary(), "assetinfo", str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[-1] == "ok" and lines[-2] == "materials 1"
+    a = V.Asset(str(p))
+    meshes = [ln.split() for ln in lines if ln.startswith("mesh ")]
+    nodes = [ln.split() for ln in lines if ln.startswith("node ")]
+    assert len(meshes) == a.info()["mesh_count"] and len(nodes) == a.info()["node_count"]
+    for fields, i in zip(meshes, range(len(meshes))):
+        m = a.mesh(i)
+        digest = 0
+        for x in m["indices"].tolist():
+            digest = (digest * 1000003 + x) % (1 << 64)
+        assert fields[1:6] == [str(m["primitive"]), str(m["vertices"].shape[0]), str(m["indices"].size),
+                               str(m["material"]), str(digest)]
+        assert float.fromhex(fields[6]) == m["bounds_min"][0] and float.fromhex(fields[7]) == m["bounds_max"][2]
+    for fields, n in zip(nodes, a.nodes):
+        assert int(fields[1]) == n["parent"] and int(fields[2]) == len(n["meshes"])
+        w = np.array([float.fromhex(x) for x in fields[3:]], np.float32)
+        assert np.array_equal(w.view(np.uint32), n["world"].view(np.uint32))
+    r = subprocess.run([cpp_test_binary(), "assetinfo", str(tmp_path / "none.glb")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0 and r.stdout.startswith("error -1 ") and "cannot read" in r.stdout

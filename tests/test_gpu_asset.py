@@ -25,11 +25,15 @@ def _quat(axis, angle):
 
 
 def _asset(oracle, with_mesh=False):
+    import vkhrt_amd as V
+    return V.Asset(data=_gltf(oracle, with_mesh).glb())
+
+
+def _gltf(oracle, with_mesh=False):
     """ROOT -> a (head: two line primitives) -> b (tail, rotated and scaled); ROOT -> c (tail
     again, translated); optionally ROOT -> d (a triangle ball)."""
     import gltf_writer as GW
     import scenes
-    import vkhrt_amd as V
     g = GW.Gltf()
     v0 = oracle.generate_strands(21, 900, 8, sphere_radius=12.0)
     v1 = oracle.generate_strands(22, 500, 6, sphere_radius=8.0)
@@ -49,7 +53,7 @@ def _asset(oracle, with_mesh=False):
         ball = g.mesh([{"positions": sv[:, :3], "normals": sv[:, 3:6], "indices": si.reshape(-1), "material": m1}])
         roots.append(g.node(mesh=ball, translation=[0.0, 25.0, 0.0]))
     g.scene(roots)
-    return V.Asset(data=g.glb())
+    return g
 
 
 def _oracle_scene(oracle, asset, mode):
@@ -177,3 +181,27 @@ def test_asset_build_arguments(gpu_ctx, oracle):
     d.hair_mode, d.radius = V.GEOMETRY_LSS, float("nan")
     assert L.vkhrt_asset_build(gpu_ctx.h, asset.h, ctypes.byref(d), None, arr, 4, ctypes.byref(cnt)) == -1
     assert all(p is None for p in arr)
+
+
+def test_cpp_model_loader_trace(gpu_ctx, oracle, tmp_path):
+    """The C++ mirror (ModelLoader::LoadFromFile -> Model::BuildBLASes -> TLAS -> TraceRays)
+    traces the asset file like the oracle, bit for bit."""
+    import subprocess
+
+    import gpu_util as G
+    import scenes
+    from test_host import cpp_test_binary
+    g = _gltf(oracle)
+    path = tmp_path / "hair.gltf"
+    g.write_gltf(str(path), bin_name="hair.bin")
+    asset = _asset(oracle)
+    S, wc = _oracle_scene(oracle, asset, 0)
+    rays = scenes.targeted_rays(wc, 20000, seed=32, dist=(5.0, 120.0))
+    rpath, hpath = tmp_path / "rays.bin", tmp_path / "hits.bin"
+    rays.tofile(rpath)
+    print("[asset] C++ mirror trace", flush=True)
+    r = subprocess.run([cpp_test_binary(), "asset", str(path), "0", str(rpath), str(rays.shape[0]), str(hpath)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    h = np.fromfile(hpath, dtype=G.HIT_DTYPE)
+    assert G.compare_hits(h, S.trace(rays), "C++ asset") > 5000
