@@ -21,6 +21,9 @@
  *   ProcessHairCurves / GenerateCurves / GenerateLinearSweptSpheres /
  *   GenerateDisjointOrthogonalTriangleStrips source/resources/model/geometry_processor.cpp:
  *   123-156, 221-298, 604-654 -> vkhrt_generate_* (run on the GPU)
+ *   ModelLoader::LoadFromFile source/resources/model/model_loader.cpp:274-312, Node::GetWorldMatrix
+ *   source/resources/model/model.cpp:6-18 -> vkhrt_asset_* (host glTF 2.0 reader);
+ *   ProcessHair* + Renderer::InitializeBLAS (renderer.cpp:694-728) -> vkhrt_asset_build
  *
  * Conventions: all functions return a vkhrt_status (0 = success, negative = error, no
  * aborts, no exceptions across the ABI); vkhrt_last_error() describes the last failure of
