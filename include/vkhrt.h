@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define VKHRT_ABI_VERSION 3
+#define VKHRT_ABI_VERSION 4
 
 typedef struct vkhrt_context vkhrt_context;
 typedef struct vkhrt_blas vkhrt_blas;
@@ -65,13 +65,16 @@ typedef enum vkhrt_geometry_mode {
     VKHRT_GEOMETRY_PHANTOM_CURVES = 0,
     VKHRT_GEOMETRY_LSS = 1,
     VKHRT_GEOMETRY_DOTS = 2,
-    VKHRT_GEOMETRY_TRIANGLES = 3 /* indexed triangle mesh (BLASType::eMesh, renderer.cpp:543-582) */
+    VKHRT_GEOMETRY_TRIANGLES = 3, /* indexed triangle mesh (BLASType::eMesh, renderer.cpp:543-582) */
+    VKHRT_GEOMETRY_VOXELS = 4     /* voxelized hair (BLASType::eVoxels, renderer.cpp:620-651) */
 } vkhrt_geometry_mode;
 
 /* vkhrt_blas_desc.flags */
 #define VKHRT_BLAS_HOST_POINTERS (1u << 0)    /* `primitives` is host memory */
 #define VKHRT_BLAS_PER_POINT_RADIUS (1u << 1) /* use per-point radii (.w) instead of `radius` */
 #define VKHRT_BLAS_HAS_TRANSFORM (1u << 2)    /* `transform` is valid (else identity) */
+#define VKHRT_BLAS_HAS_BOUNDS (1u << 3)       /* VOXELS: grid over bounds_min..bounds_max (else the lines' box) */
+#define VKHRT_BLAS_VOXEL_REFERENCE_SWEEP (1u << 4) /* VOXELS: the reference's fill loop, see below */
 
 /* Mirrors BLASInput (bottom_level_acceleration_structure.hpp:19-29).
  * primitives, `count` records of:
@@ -85,6 +88,16 @@ typedef enum vkhrt_geometry_mode {
  *                   texCoord: the Vertex of model.hpp:44-49) and `indices` 3 u32 per triangle;
  *                   count = triangles; the hit reports the interpolated vertex normal
  *                   (triangle_closest_hit.rchit:31-36) and barycentrics u, v
+ *   VOXELS:         8 floats = line (start.xyz, -, end.xyz, -), voxelized as capsules of radius
+ *                   `radius` into a grid of `voxel_size` cells over the bounds
+ *                   (GenerateVoxelMesh, geometry_processor.cpp:515-602; the bounds' extent is
+ *                   rounded up to whole cells, the resolution truncated as there). By default
+ *                   every cell of each sweep box is filled; with VKHRT_BLAS_VOXEL_REFERENCE_SWEEP
+ *                   only the cells at the box's minimum x, with the index wrapping like the
+ *                   reference's (:585-593). `voxel_lod` L traces the grid OR-reduced L times
+ *                   (cells of voxel_size * 2^L). A hit is the first filled cell the ray enters,
+ *                   at its entry t, primitive id = the cell's linear index x + y rx + z rx ry
+ *                   at that level, normal = the face crossed
  * radius: constant radius used when VKHRT_BLAS_PER_POINT_RADIUS is not set (reference 0.02).
  * transform: row-major 3x4 world-from-object (BLASInput.transform, Node::GetWorldMatrix). */
 typedef struct vkhrt_blas_desc {
@@ -98,12 +111,16 @@ typedef struct vkhrt_blas_desc {
     float transform[12];
     const uint32_t* indices; /* TRIANGLES: 3 x count vertex indices (u32) */
     uint32_t vertex_count;   /* TRIANGLES: vertices in `primitives` */
+    float voxel_size;        /* VOXELS: cell edge (0.1 in the reference) */
+    uint32_t voxel_lod;      /* VOXELS: level of detail traced, 0..15 */
+    float bounds_min[3];     /* VOXELS with VKHRT_BLAS_HAS_BOUNDS: the mesh bounds */
+    float bounds_max[3];
 } vkhrt_blas_desc;
 
 typedef struct vkhrt_blas_info {
     uint32_t mode;
     uint32_t primitive_count; /* reported primitives (DOTS: triangles) */
-    uint32_t leaf_records;    /* curves / capsules / DOTS pieces */
+    uint32_t leaf_records;    /* curves / capsules / DOTS pieces / filled voxels */
     uint32_t node_count;      /* 8-wide nodes */
     uint32_t depth;           /* 8-wide levels */
     uint64_t device_bytes;
@@ -219,6 +236,17 @@ int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
 int vkhrt_blas_build(vkhrt_context* ctx, const vkhrt_blas_desc* desc, void* stream, vkhrt_blas** out);
 int vkhrt_blas_info_get(const vkhrt_blas* blas, vkhrt_blas_info* out);
 void vkhrt_blas_destroy(vkhrt_blas* blas);
+/* Voxel BLAS: the traced grid (cells per axis, origin, cell edge, occupancy words). */
+typedef struct vkhrt_voxel_grid {
+    int32_t res[3];
+    float origin[3];
+    float cell;
+    uint64_t words; /* u32 words of occupancy, (res product + 31) / 32 */
+} vkhrt_voxel_grid;
+int vkhrt_blas_voxel_grid_get(const vkhrt_blas* blas, vkhrt_voxel_grid* out);
+/* Copies `words` occupancy words (bit i of word w = cell 32 w + i, x fastest) to host memory;
+ * synchronous. */
+int vkhrt_blas_voxels_read(vkhrt_context* ctx, const vkhrt_blas* blas, uint32_t* words, uint64_t count);
 /* Instance i = blases[i] with its transform; customIndex = i. */
 int vkhrt_tlas_build(vkhrt_context* ctx, const vkhrt_blas* const* blases, uint32_t count, void* stream,
                      vkhrt_tlas** out);
@@ -511,15 +539,19 @@ int vkhrt_asset_instance_get(const vkhrt_asset* asset, uint32_t index, vkhrt_ass
 /* ProcessHairCurves / ProcessHairLSS / ProcessHairDOTS (geometry_processor.cpp:604-688,740-779)
  * and Renderer::InitializeBLAS (renderer.cpp:694-728) on the device. */
 typedef struct vkhrt_asset_build_desc {
-    uint32_t hair_mode;        /* line meshes: VKHRT_GEOMETRY_PHANTOM_CURVES, _LSS or _DOTS */
+    uint32_t hair_mode;        /* line meshes: VKHRT_GEOMETRY_PHANTOM_CURVES, _LSS, _DOTS or _VOXELS */
     float radius;              /* fibre radius (0.02 in the reference) */
     float tension;             /* GenerateCurves tension (1) */
     uint32_t leaf_size;        /* 0 = the context's default */
     uint32_t default_material; /* material_index of meshes without a material */
+    float voxel_size;          /* VOXELS: cell edge (ProcessHairVoxels uses 0.1, :712) */
+    uint32_t voxel_lod;        /* VOXELS: level of detail */
+    uint32_t voxel_flags;      /* VOXELS: VKHRT_BLAS_VOXEL_REFERENCE_SWEEP or 0 */
 } vkhrt_asset_build_desc;
 /* One BLAS per instance (vkhrt_asset_instance_get order), with the instance's transform and
  * its mesh's material. Line meshes: GenerateLines (mesh-local indices) -> GenerateCurves ->
- * phantom curves, or -> capsules (LSS) / lines (DOTS); triangle meshes as
+ * phantom curves, or -> capsules (LSS) / lines (DOTS) / a voxel grid over the mesh bounds
+ * (VOXELS, ProcessHairVoxels :690-737); triangle meshes as
  * VKHRT_GEOMETRY_TRIANGLES. Each mesh is generated once and shared by its instances; every
  * BLAS owns its records (the reference points every hair BLAS at the start of the curve
  * buffer, renderer.cpp:611). blases == NULL with capacity 0 only sets *count; otherwise

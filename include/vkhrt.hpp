@@ -96,6 +96,11 @@ struct BLASInput {
     uint32_t leafSize = 0;
     const uint32_t* indices = nullptr;  // VKHRT_GEOMETRY_TRIANGLES: 3 per triangle (count = triangles)
     uint32_t vertexCount = 0;           // VKHRT_GEOMETRY_TRIANGLES: vertices in `primitives`
+    float voxelSize = 0.1f;             // VKHRT_GEOMETRY_VOXELS (eVoxels): cell edge
+    uint32_t voxelLod = 0;              // VKHRT_GEOMETRY_VOXELS: level of detail traced
+    bool voxelReferenceSweep = false;   // VKHRT_GEOMETRY_VOXELS: the reference's fill loop
+    bool hasBounds = false;             // VKHRT_GEOMETRY_VOXELS: grid over boundsMin..boundsMax
+    std::array<float, 3> boundsMin{}, boundsMax{};
 };
 
 class BottomLevelAccelerationStructure {
@@ -104,12 +109,11 @@ public:
                                      void* stream = nullptr)
         : _type(input.type), _transform(input.transform), _context(context) {
         const bool hair = input.geometry == VKHRT_GEOMETRY_PHANTOM_CURVES;
-        if (input.type == BLASType::eVoxels)
-            throw Error(VKHRT_ERROR_UNSUPPORTED, "BottomLevelAccelerationStructure: voxel BLAS is not on this path");
-        if ((input.type == BLASType::eHair) != hair)
+        const bool voxels = input.geometry == VKHRT_GEOMETRY_VOXELS;
+        if ((input.type == BLASType::eHair) != hair || (input.type == BLASType::eVoxels) != voxels)
             throw Error(VKHRT_ERROR_INVALID_ARGUMENT,
-                        "BottomLevelAccelerationStructure: eHair pairs with phantom curves, eMesh with LSS / DOTS / "
-                        "triangles");
+                        "BottomLevelAccelerationStructure: eHair pairs with phantom curves, eVoxels with voxels, "
+                        "eMesh with LSS / DOTS / triangles");
         if (!context) throw Error(VKHRT_ERROR_INVALID_ARGUMENT, "BottomLevelAccelerationStructure: no context");
         vkhrt_blas_desc d{};
         d.mode = uint32_t(input.geometry);
@@ -122,6 +126,14 @@ public:
         d.leaf_size = input.leafSize;
         d.indices = input.indices;
         d.vertex_count = input.vertexCount;
+        d.voxel_size = input.voxelSize;
+        d.voxel_lod = input.voxelLod;
+        d.flags |= (input.voxelReferenceSweep ? VKHRT_BLAS_VOXEL_REFERENCE_SWEEP : 0u) |
+                   (input.hasBounds ? VKHRT_BLAS_HAS_BOUNDS : 0u);
+        for (int k = 0; k < 3; ++k) {
+            d.bounds_min[k] = input.boundsMin[k];
+            d.bounds_max[k] = input.boundsMax[k];
+        }
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 4; ++c) d.transform[r * 4 + c] = input.transform[c * 4 + r];
         _context->Check(vkhrt_blas_build(_context->Handle(), &d, stream, &_blas), "vkhrt_blas_build");
@@ -320,13 +332,16 @@ public:
     std::vector<BottomLevelAccelerationStructure> BuildBLASes(const std::shared_ptr<Context>& context,
                                                               vkhrt_geometry_mode hairMode = VKHRT_GEOMETRY_PHANTOM_CURVES,
                                                               float radius = 0.02f, float tension = 1.0f,
-                                                              uint32_t defaultMaterial = 0, void* stream = nullptr) const {
+                                                              uint32_t defaultMaterial = 0, void* stream = nullptr,
+                                                              float voxelSize = 0.1f, uint32_t voxelLod = 0) const {
         if (!context) throw Error(VKHRT_ERROR_INVALID_ARGUMENT, "Model::BuildBLASes: no context");
         vkhrt_asset_build_desc d{};
         d.hair_mode = uint32_t(hairMode);
         d.radius = radius;
         d.tension = tension;
         d.default_material = defaultMaterial;
+        d.voxel_size = voxelSize;
+        d.voxel_lod = voxelLod;
         uint32_t n = 0;
         context->Check(vkhrt_asset_build(context->Handle(), _asset, &d, stream, nullptr, 0, &n), "vkhrt_asset_build");
         std::vector<vkhrt_blas*> raw(n, nullptr);
@@ -337,9 +352,10 @@ public:
             vkhrt_asset_instance inst{};
             vkhrt_asset_instance_get(_asset, i, &inst);
             const Mesh& M = sceneGraph->meshes[inst.mesh];
-            const BLASType type = M.primitiveType == Mesh::PrimitiveType::eLines && hairMode == VKHRT_GEOMETRY_PHANTOM_CURVES
-                                      ? BLASType::eHair
-                                      : BLASType::eMesh;
+            const bool lines = M.primitiveType == Mesh::PrimitiveType::eLines;
+            const BLASType type = lines && hairMode == VKHRT_GEOMETRY_PHANTOM_CURVES ? BLASType::eHair
+                                  : lines && hairMode == VKHRT_GEOMETRY_VOXELS       ? BLASType::eVoxels
+                                                                                     : BLASType::eMesh;
             out.push_back(BottomLevelAccelerationStructure::Adopt(raw[i], type, sceneGraph->nodes[inst.node].worldMatrix,
                                                                   context));
         }

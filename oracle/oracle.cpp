@@ -24,12 +24,13 @@
 #include "oracle_geom.hpp"
 #include "oracle_mlp.hpp"
 #include "oracle_path.hpp"
+#include "oracle_voxel.hpp"
 
 using namespace orc;
 
 namespace {
 
-enum { MODE_PHANTOM = 0, MODE_LSS = 1, MODE_DOTS = 2, MODE_TRIS = 3 };
+enum { MODE_PHANTOM = 0, MODE_LSS = 1, MODE_DOTS = 2, MODE_TRIS = 3, MODE_VOXELS = 4 };
 enum {
     F_ANY_HIT = 1u << 0,
     F_BRUTE = 1u << 1,
@@ -82,6 +83,10 @@ struct Blas {
     std::vector<uint8_t> cap_a;      // LSS chained caps
     std::vector<DotsPiece> pieces;   // DOTS
     std::vector<float> tri, tnrm;    // triangle mesh: 3 vertex positions / normals per triangle
+    std::vector<uint32_t> vbits;     // voxels: occupancy bits (x fastest)
+    float vorigin[3] = {0, 0, 0};
+    float vcell = 0.0f;
+    int32_t vres[3] = {0, 0, 0};
     uint32_t nprims = 0;             // primitive count (DOTS: triangles)
     std::vector<Box> boxes;
     std::vector<uint32_t> order;
@@ -364,6 +369,22 @@ uint32_t trace_blas(const Blas& B, uint32_t inst_id, const Ray<float>& r, float 
     uint32_t tests = 0;
     if (B.nprims == 0) return 0;
     const bool any = (flags & F_ANY_HIT) != 0;
+    if (B.mode == MODE_VOXELS) {
+        const float o[3] = {r.origin.x, r.origin.y, r.origin.z}, d[3] = {r.direction.x, r.direction.y, r.direction.z};
+        VoxelHit vh{};
+        const bool hit = (flags & F_BRUTE) ? VoxelBrute(B.vbits, B.vorigin, B.vcell, B.vres, o, d, tmin, tmax, vh)
+                                           : VoxelWalk(B.vbits, B.vorigin, B.vcell, B.vres, o, d, tmin, tmax, vh);
+        if (hit && (!found || better(vh.t, inst_id, vh.cell, best))) {
+            best = Hit{};
+            best.t = vh.t;
+            best.prim = vh.cell;
+            best.inst = inst_id;
+            std::memcpy(best.n, vh.n, sizeof(best.n));
+            found = true;
+            if (!any) tmax = vh.t;
+        }
+        return 1;
+    }
     if (flags & F_BRUTE) {
         for (uint32_t p = 0; p < B.nprims; ++p) {
             Hit h{};
@@ -612,6 +633,47 @@ int orc_world_matrix(const float* locals, const int32_t* parents, uint32_t n, fl
         std::memcpy(out + size_t(i) * 16, m.m, 64);
     }
     return 0;
+}
+
+// Voxel grid of GenerateVoxelMesh over [lo, hi] (or the lines' end points when lo is null),
+// filled from `lines` (n x 8 floats) and OR-reduced `lod` times. res_out: 3 cells per axis,
+// geo_out: origin xyz, cell edge. bits: (res product + 31) / 32 words, or null to query the
+// size. Returns the word count, -1 for an invalid grid, -2 when a line visits > 65536 cells.
+long long orc_voxelize(const float* lines, uint32_t n, const float* lo, const float* hi, float radius, float size,
+                       int reference_sweep, uint32_t lod, uint32_t* bits, int32_t* res_out, float* geo_out) {
+    float blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+    if (lo && hi) {
+        std::memcpy(blo, lo, sizeof(blo));
+        std::memcpy(bhi, hi, sizeof(bhi));
+    } else if (n) {
+        for (int k = 0; k < 3; ++k) {
+            blo[k] = INFINITY;
+            bhi[k] = -INFINITY;
+        }
+        for (uint32_t i = 0; i < 2 * n; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const float c = lines[size_t(i) * 4 + k];
+                if (c < blo[k]) blo[k] = c;
+                if (c > bhi[k]) bhi[k] = c;
+            }
+    }
+    VGrid g;
+    if (!VoxelGridFor(blo, bhi, size, g)) return -1;
+    std::vector<uint32_t> v((size_t(g.count) + 31) / 32, 0u);
+    if (!Voxelize(lines, n, g, radius, reference_sweep != 0, v)) return -2;
+    int32_t res[3] = {g.res[0], g.res[1], g.res[2]};
+    float cell = g.size;
+    for (uint32_t l = 0; l < lod && g.count; ++l) {
+        int32_t cr[3];
+        v = Downsample(v, res, cr);
+        std::memcpy(res, cr, sizeof(res));
+        cell = cell * 2.0f;
+    }
+    std::memcpy(res_out, res, sizeof(res));
+    for (int k = 0; k < 3; ++k) geo_out[k] = g.origin[k];
+    geo_out[3] = cell;
+    if (bits) std::memcpy(bits, v.data(), v.size() * sizeof(uint32_t));
+    return (long long)v.size();
 }
 
 int orc_curves_to_lines(const float* curves, uint32_t n, uint32_t pieces, float* lines) {
@@ -957,6 +1019,23 @@ int orc_scene_add_mesh(orc_scene* sp, const float* verts, uint32_t nverts, const
         }
     build_blas(*B, leaf_size > 0 ? leaf_size : 4);
     return add_instance(*reinterpret_cast<Scene*>(sp), B, transform);
+}
+
+// A voxel instance over an occupancy grid (bits: res product bits, x fastest).
+int orc_scene_add_voxels(orc_scene* sp, const uint32_t* bits, const int32_t* res, const float* origin, float cell,
+                         const float* transform) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    auto B = std::make_shared<Blas>();
+    B->mode = MODE_VOXELS;
+    const uint64_t count = uint64_t(res[0]) * uint64_t(res[1]) * uint64_t(res[2]);
+    B->vbits.assign(bits, bits + (count + 31) / 32);
+    std::memcpy(B->vres, res, sizeof(B->vres));
+    std::memcpy(B->vorigin, origin, sizeof(B->vorigin));
+    B->vcell = cell;
+    uint32_t filled = 0;
+    for (uint32_t w : B->vbits) filled += uint32_t(__builtin_popcount(w));
+    B->nprims = filled;
+    return add_instance(S, B, transform);
 }
 
 int orc_scene_set_material(orc_scene* sp, uint32_t inst, uint32_t material) {

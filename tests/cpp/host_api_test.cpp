@@ -1,6 +1,7 @@
 // Drives the C++ mirror of the reference's BLAS / TLAS classes (include/vkhrt.hpp).
 //   host_api_test nodevice
-//       error behaviour without a GPU: Context -> NO_DEVICE, BLASType / geometry pairing,
+//       error behaviour without a GPU: Context -> NO_DEVICE, BLASType / geometry pairing
+//       (eVoxels with VKHRT_GEOMETRY_VOXELS),
 //       CameraLookAt; prints "ok".
 //   host_api_test trace <curves.bin> <n> <rays.bin> <m> <hits.bin> [instances]
 //   host_api_test batch <curves.bin> <n> <cams.bin> <nviews> <W1xH1,...> <bounces> <seed> <out.bin>
@@ -45,9 +46,15 @@ static int expect_error(int code, void (*fn)()) {
 static int no_device() {
     int bad = 0;
     bad += expect_error(VKHRT_ERROR_NO_DEVICE, [] { vkhrt::Context ctx(0); });
-    bad += expect_error(VKHRT_ERROR_UNSUPPORTED, [] {
+    bad += expect_error(VKHRT_ERROR_INVALID_ARGUMENT, [] {
+        vkhrt::BLASInput in;
+        in.type = vkhrt::BLASType::eVoxels;  // voxels need VKHRT_GEOMETRY_VOXELS
+        vkhrt::BottomLevelAccelerationStructure b(in, nullptr);
+    });
+    bad += expect_error(VKHRT_ERROR_INVALID_ARGUMENT, [] {
         vkhrt::BLASInput in;
         in.type = vkhrt::BLASType::eVoxels;
+        in.geometry = VKHRT_GEOMETRY_VOXELS;  // paired, but no context
         vkhrt::BottomLevelAccelerationStructure b(in, nullptr);
     });
     bad += expect_error(VKHRT_ERROR_INVALID_ARGUMENT, [] {

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "build", "liboracle.so")
 
-MODE_PHANTOM, MODE_LSS, MODE_DOTS = 0, 1, 2
+MODE_PHANTOM, MODE_LSS, MODE_DOTS, MODE_VOXELS = 0, 1, 2, 4
 F_ANY_HIT, F_BRUTE, F_FP64, F_POLISH, F_REF_NORMAL = 1, 2, 4, 8, 16
 
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("u", "<f4"),
@@ -44,6 +44,8 @@ def lib():
         "orc_generate_curves": ([P, u32, P], i32),
         "orc_mesh_lines": ([P, u32, P, u32, f32, P], i32),
         "orc_world_matrix": ([P, P, u32, P], i32),
+        "orc_voxelize": ([P, u32, P, P, f32, f32, i32, u32, P, P, P], ctypes.c_longlong),
+        "orc_scene_add_voxels": ([P, P, P, P, f32, P], i32),
         "orc_curves_to_lines": ([P, u32, u32, P], i32),
         "orc_generate_lss": ([P, u32, f32, P], i32),
         "orc_generate_dots": ([P, u32, f32, P], i32),
@@ -141,6 +143,23 @@ def world_matrices(locals_, parents):
     return out
 
 
+def voxelize(lines, voxel_size, radius=0.02, bounds=None, reference_sweep=False, lod=0):
+    """GenerateVoxelMesh (+ lod OR-reductions): (bits uint32 words, res (3,), origin (3,), cell)."""
+    lines = np.ascontiguousarray(lines, np.float32).reshape(-1, 8)
+    lo = hi = None
+    if bounds is not None:
+        lo, hi = f32(bounds[0]), f32(bounds[1])
+    res = np.zeros(3, np.int32)
+    geo = np.zeros(4, np.float32)
+    args = (ptr(lines), lines.shape[0], ptr(lo), ptr(hi), radius, voxel_size, int(reference_sweep), lod)
+    nw = lib().orc_voxelize(*args, None, ptr(res), ptr(geo))
+    if nw < 0:
+        raise ValueError("invalid voxel grid" if nw == -1 else "a line covers more than 65536 cells")
+    bits = np.zeros(max(nw, 1), np.uint32)
+    lib().orc_voxelize(*args, ptr(bits), ptr(res), ptr(geo))
+    return bits[:nw], res, geo[:3].copy(), float(geo[3])
+
+
 def curves_to_lines(curves, pieces):
     curves = f32(curves)
     out = np.empty((curves.shape[0] * pieces, 8), np.float32)
@@ -235,6 +254,16 @@ class Scene:
         xf = None if transform is None else f32(np.asarray(transform).reshape(12))
         r = lib().orc_scene_add_mesh(self.h, ptr(verts), verts.shape[0], ptr(idx), idx.shape[0], ptr(xf), leaf_size)
         assert r >= 0, "mesh index out of range"
+        return r
+
+    def add_voxels(self, bits, res, origin, cell, transform=None):
+        bits = np.ascontiguousarray(bits, np.uint32)
+        if bits.size == 0:
+            bits = np.zeros(1, np.uint32)
+        xf = None if transform is None else f32(np.asarray(transform).reshape(12))
+        r = lib().orc_scene_add_voxels(self.h, ptr(bits), ptr(np.ascontiguousarray(res, np.int32)), ptr(f32(origin)),
+                                       cell, ptr(xf))
+        assert r >= 0
         return r
 
     def set_material(self, inst, material):

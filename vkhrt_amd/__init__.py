@@ -16,11 +16,13 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvkhrt.so")
 
-GEOMETRY_PHANTOM_CURVES, GEOMETRY_LSS, GEOMETRY_DOTS, GEOMETRY_TRIANGLES = 0, 1, 2, 3
+GEOMETRY_PHANTOM_CURVES, GEOMETRY_LSS, GEOMETRY_DOTS, GEOMETRY_TRIANGLES, GEOMETRY_VOXELS = 0, 1, 2, 3, 4
 
 BLAS_HOST_POINTERS = 1 << 0
 BLAS_PER_POINT_RADIUS = 1 << 1
 BLAS_HAS_TRANSFORM = 1 << 2
+BLAS_HAS_BOUNDS = 1 << 3
+BLAS_VOXEL_REFERENCE_SWEEP = 1 << 4
 
 TRACE_TERMINATE_ON_FIRST_HIT = 1 << 0
 TRACE_FP64_POLISH = 1 << 3
@@ -70,7 +72,8 @@ EXPORTS = [
     "vkhrt_autotune_render", "vkhrt_autotune_batch", "vkhrt_autotune_blas", "vkhrt_generate_surface_shadow_rays",
     "vkhrt_asset_load", "vkhrt_asset_load_memory", "vkhrt_asset_destroy", "vkhrt_asset_last_error",
     "vkhrt_asset_info_get", "vkhrt_asset_mesh_get", "vkhrt_asset_node_get", "vkhrt_asset_material_get",
-    "vkhrt_asset_instance_get", "vkhrt_asset_build", "vkhrt_generate_mesh_lines",
+    "vkhrt_asset_instance_get", "vkhrt_asset_build", "vkhrt_generate_mesh_lines", "vkhrt_blas_voxel_grid_get",
+    "vkhrt_blas_voxels_read",
 ]
 
 
@@ -85,7 +88,9 @@ class BlasDesc(ctypes.Structure):
                 ("primitives", ctypes.c_void_p), ("count", ctypes.c_uint32),
                 ("radius", ctypes.c_float), ("material_index", ctypes.c_uint32),
                 ("leaf_size", ctypes.c_uint32), ("transform", ctypes.c_float * 12),
-                ("indices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint32)]
+                ("indices", ctypes.c_void_p), ("vertex_count", ctypes.c_uint32),
+                ("voxel_size", ctypes.c_float), ("voxel_lod", ctypes.c_uint32),
+                ("bounds_min", ctypes.c_float * 3), ("bounds_max", ctypes.c_float * 3)]
 
 
 class BlasInfo(ctypes.Structure):
@@ -200,7 +205,13 @@ class AssetInstance(ctypes.Structure):
 
 class AssetBuildDesc(ctypes.Structure):
     _fields_ = [("hair_mode", ctypes.c_uint32), ("radius", ctypes.c_float), ("tension", ctypes.c_float),
-                ("leaf_size", ctypes.c_uint32), ("default_material", ctypes.c_uint32)]
+                ("leaf_size", ctypes.c_uint32), ("default_material", ctypes.c_uint32),
+                ("voxel_size", ctypes.c_float), ("voxel_lod", ctypes.c_uint32), ("voxel_flags", ctypes.c_uint32)]
+
+
+class VoxelGrid(ctypes.Structure):
+    _fields_ = [("res", ctypes.c_int32 * 3), ("origin", ctypes.c_float * 3), ("cell", ctypes.c_float),
+                ("words", ctypes.c_uint64)]
 
 
 _lib = None
@@ -277,6 +288,8 @@ def lib():
         "vkhrt_asset_build": ([P, P, ctypes.POINTER(AssetBuildDesc), P, ctypes.POINTER(P), u32,
                                ctypes.POINTER(u32)], i32),
         "vkhrt_generate_mesh_lines": ([P, P, u32, P, u32, f32, P, P], i32),
+        "vkhrt_blas_voxel_grid_get": ([P, ctypes.POINTER(VoxelGrid)], i32),
+        "vkhrt_blas_voxels_read": ([P, P, P, ctypes.c_uint64], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -511,13 +524,22 @@ class BottomLevelAccelerationStructure:
     """
 
     def __init__(self, ctx, mode, primitives_ptr, count, radius=0.02, flags=0, transform=None,
-                 leaf_size=0, material_index=0, stream=None, indices_ptr=None, vertex_count=0):
+                 leaf_size=0, material_index=0, stream=None, indices_ptr=None, vertex_count=0, voxel_size=0.1,
+                 voxel_lod=0, bounds=None):
         """GEOMETRY_TRIANGLES: `primitives_ptr` holds `vertex_count` vertices (position, normal,
-        texCoord = 8 floats), `indices_ptr` 3 u32 per triangle, `count` triangles."""
+        texCoord = 8 floats), `indices_ptr` 3 u32 per triangle, `count` triangles.
+        GEOMETRY_VOXELS: `count` lines (8 floats) voxelized as capsules of `radius` into cells of
+        `voxel_size` over `bounds` ((min xyz, max xyz); default: the lines' box), traced at
+        level of detail `voxel_lod`."""
         d = BlasDesc()
         d.mode, d.flags, d.primitives, d.count = mode, flags, primitives_ptr, count
         d.radius, d.material_index, d.leaf_size = radius, material_index, leaf_size
         d.indices, d.vertex_count = indices_ptr, vertex_count
+        d.voxel_size, d.voxel_lod = voxel_size, voxel_lod
+        if bounds is not None:
+            d.flags |= BLAS_HAS_BOUNDS
+            for k in range(3):
+                d.bounds_min[k], d.bounds_max[k] = float(bounds[0][k]), float(bounds[1][k])
         if transform is not None:
             d.flags |= BLAS_HAS_TRANSFORM
             for i, v in enumerate(list(transform)[:12]):
@@ -534,6 +556,19 @@ class BottomLevelAccelerationStructure:
         obj.ctx, obj.h, obj.mode = ctx, handle, mode
         obj._destroy = lib().vkhrt_blas_destroy
         return obj
+
+    def voxel_grid(self):
+        """Voxel BLAS: (occupancy as a bool array [z, y, x], origin (3,), cell edge)."""
+        import numpy as np
+        g = VoxelGrid()
+        if lib().vkhrt_blas_voxel_grid_get(self.h, ctypes.byref(g)) != 0:
+            raise VkhrtError(-1, "not a voxel BLAS")
+        words = np.zeros(max(1, g.words), np.uint32)
+        self.ctx.check(lib().vkhrt_blas_voxels_read(self.ctx.h, self.h, words.ctypes.data, g.words), "voxels_read")
+        res = [g.res[0], g.res[1], g.res[2]]
+        n = res[0] * res[1] * res[2]
+        cells = np.unpackbits(words.view(np.uint8), bitorder="little")[:n].astype(bool).reshape(res[2], res[1], res[0])
+        return cells, np.array(g.origin[:], np.float32), g.cell
 
     def info(self):
         i = BlasInfo()
@@ -628,10 +663,10 @@ class Asset:
         return [self.instance(i) for i in range(self.info()["instance_count"])]
 
     def build(self, ctx, hair_mode=GEOMETRY_PHANTOM_CURVES, radius=0.02, tension=1.0, leaf_size=0,
-              default_material=0, stream=None):
+              default_material=0, stream=None, voxel_size=0.1, voxel_lod=0, voxel_flags=0):
         """One BLAS per instance (Renderer::InitializeBLAS, renderer.cpp:694-728), built on the
         device; pass the list to TopLevelAccelerationStructure."""
-        d = AssetBuildDesc(hair_mode, radius, tension, leaf_size, default_material)
+        d = AssetBuildDesc(hair_mode, radius, tension, leaf_size, default_material, voxel_size, voxel_lod, voxel_flags)
         n = self.info()["instance_count"]
         arr = (ctypes.c_void_p * max(1, n))()
         cnt = ctypes.c_uint32()

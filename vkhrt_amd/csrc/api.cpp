@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <atomic>
@@ -18,6 +19,7 @@
 #include "geometry.hpp"
 #include "internal.hpp"
 #include "mlp.hpp"
+#include "voxel.hpp"
 
 namespace {
 
@@ -127,6 +129,10 @@ struct vkhrt_blas {
     uint32_t records = 0;
     float lo[3], hi[3];  // object-space bounds
     vkhrt_blas_info info{};
+    uint32_t* voxels = nullptr;  // voxel BLAS: occupancy bits of the traced level
+    float vorigin[3] = {0, 0, 0};
+    float vcell = 0.0f;
+    int32_t vres[3] = {0, 0, 0};
 };
 
 struct vkhrt_tlas {
@@ -590,10 +596,140 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
     return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown option");
 }
 
+namespace {
+
+// Voxel BLAS (BLASType::eVoxels): ProcessHairVoxels' grid and fill (geometry_processor.cpp:
+// 690-737, GenerateVoxelMesh :515-602) on the device, then `voxel_lod` OR-reductions.
+int build_voxels(vkhrt_context* c, const vkhrt_blas_desc* d, hipStream_t st, vkhrt_blas* b) {
+    if (!(d->voxel_size > 0.0f) || !std::isfinite(d->voxel_size))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "voxel size must be finite and > 0");
+    if (!(d->radius >= 0.0f) || !std::isfinite(d->radius))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "voxel radius must be finite and >= 0");
+    if (d->voxel_lod > 15) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "voxel level of detail must be 0..15");
+    if (d->flags & VKHRT_BLAS_PER_POINT_RADIUS)
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "voxels take one radius, not per-point radii");
+    const size_t line_bytes = size_t(d->count) * 8 * sizeof(float);
+    float4* lines = nullptr;
+    bool own = false;
+    uint32_t* bits = nullptr;
+    uint32_t* scratch = nullptr;  // [0] overflow, [1] filled count, [2..7] bounds keys
+    int rc = VKHRT_SUCCESS;
+    hipError_t e = hipMalloc(&scratch, 8 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(scratch, 0, 8 * sizeof(uint32_t), st);
+    if (e == hipSuccess && d->count) {
+        if (d->flags & VKHRT_BLAS_HOST_POINTERS) {
+            e = hipMalloc(&lines, line_bytes);
+            if (e == hipSuccess) e = hipMemcpyAsync(lines, d->primitives, line_bytes, hipMemcpyHostToDevice, st);
+            own = true;
+        } else {
+            lines = reinterpret_cast<float4*>(const_cast<float*>(d->primitives));
+        }
+    }
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (e == hipSuccess) {
+        if (d->flags & VKHRT_BLAS_HAS_BOUNDS) {
+            std::memcpy(lo, d->bounds_min, sizeof(lo));
+            std::memcpy(hi, d->bounds_max, sizeof(hi));
+        } else if (d->count) {
+            int32_t keys[6];
+            e = vk::voxel_line_bounds(lines, d->count, reinterpret_cast<int32_t*>(scratch + 2), st);
+            if (e == hipSuccess) e = hipMemcpyAsync(keys, scratch + 2, sizeof(keys), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e == hipSuccess) vk::voxel_decode_bounds(keys, lo, hi);
+        }
+    }
+    vk::VoxelGrid g{};
+    int32_t res[3] = {0, 0, 0};
+    float cell = d->voxel_size;
+    uint32_t filled = 0;
+    if (e != hipSuccess) {
+        rc = hip_fail(c, e, "voxel input");
+        goto done;
+    }
+    if (!vk::voxel_grid_params(lo, hi, d->voxel_size, g)) {
+        rc = fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "voxel grid: bounds not finite, or 2^31 cells or more");
+        goto done;
+    }
+    std::memcpy(res, g.res, sizeof(res));
+    if (g.count) {
+        uint32_t overflow = 0;
+        const size_t words = (size_t(g.count) + 31) / 32;
+        e = hipMalloc(&bits, words * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemsetAsync(bits, 0, words * sizeof(uint32_t), st);
+        if (e == hipSuccess)
+            e = vk::voxelize_lines(lines, d->count, g, d->radius, (d->flags & VKHRT_BLAS_VOXEL_REFERENCE_SWEEP) != 0,
+                                   bits, scratch, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&overflow, scratch, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "voxelize");
+            goto done;
+        }
+        if (overflow) {
+            rc = fail(c, VKHRT_ERROR_INVALID_ARGUMENT,
+                      std::to_string(overflow) + " lines cover more than 65536 voxel cells each (voxel size too small)");
+            goto done;
+        }
+        for (uint32_t l = 0; l < d->voxel_lod; ++l) {
+            const int32_t cres[3] = {(res[0] + 1) / 2, (res[1] + 1) / 2, (res[2] + 1) / 2};
+            const size_t cwords = (size_t(cres[0]) * size_t(cres[1]) * size_t(cres[2]) + 31) / 32;
+            uint32_t* coarse = nullptr;
+            e = hipMalloc(&coarse, cwords * sizeof(uint32_t));
+            if (e == hipSuccess) e = vk::voxel_downsample(bits, res, coarse, cres, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                hipFree(coarse);
+                rc = hip_fail(c, e, "voxel level of detail");
+                goto done;
+            }
+            hipFree(bits);
+            bits = coarse;
+            std::memcpy(res, cres, sizeof(res));
+            cell = cell * 2.0f;
+        }
+        const uint32_t nwords = uint32_t((uint64_t(res[0]) * uint64_t(res[1]) * uint64_t(res[2]) + 31) / 32);
+        e = vk::voxel_count(bits, nwords, scratch + 1, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&filled, scratch + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "voxel count");
+            goto done;
+        }
+        b->info.device_bytes = uint64_t(nwords) * 4u;
+    }
+    b->voxels = bits;
+    bits = nullptr;
+    b->records = filled;
+    b->vcell = cell;
+    std::memcpy(b->vres, res, sizeof(res));
+    std::memcpy(b->vorigin, g.origin, sizeof(b->vorigin));
+    {
+        // object bounds: the grid box, padded like the BVH boxes
+        float m = 0.0f;
+        for (int k = 0; k < 3; ++k) {
+            b->lo[k] = g.origin[k];
+            b->hi[k] = g.origin[k] + float(res[k]) * cell;
+            m = std::max(m, std::max(std::fabs(b->lo[k]), std::fabs(b->hi[k])));
+        }
+        for (int k = 0; k < 3; ++k) {
+            b->lo[k] -= 1e-4f + 4e-7f * m;
+            b->hi[k] += 1e-4f + 4e-7f * m;
+        }
+    }
+done:
+    hipStreamSynchronize(st);
+    if (own) hipFree(lines);
+    hipFree(bits);
+    hipFree(scratch);
+    return rc;
+}
+
+}  // namespace
+
 int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, vkhrt_blas** out) {
     if (!c || !d || !out) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    if (d->mode > VKHRT_GEOMETRY_TRIANGLES) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown geometry mode");
+    if (d->mode > VKHRT_GEOMETRY_VOXELS) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown geometry mode");
     if (d->count && !d->primitives) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null primitives");
     const bool mesh = d->mode == VKHRT_GEOMETRY_TRIANGLES;
     if (mesh && d->count && (!d->indices || d->vertex_count == 0))
@@ -604,6 +740,27 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
     if (!per_point && !(d->radius >= 0.0f)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "radius must be >= 0");
     hipSetDevice(c->device);
     hipStream_t st = pick(c, stream);
+    if (d->mode == VKHRT_GEOMETRY_VOXELS) {
+        vkhrt_blas* vb = new (std::nothrow) vkhrt_blas();
+        if (!vb) return fail(c, VKHRT_ERROR_OUT_OF_MEMORY, "blas");
+        vb->ctx = c;
+        vb->mode = d->mode;
+        vb->radius = d->radius;
+        vb->material_index = d->material_index;
+        vb->has_xf = (d->flags & VKHRT_BLAS_HAS_TRANSFORM) != 0;
+        std::memcpy(vb->xf, d->transform, sizeof(vb->xf));
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = build_voxels(c, d, st, vb);
+        if (rc != VKHRT_SUCCESS) {
+            vkhrt_blas_destroy(vb);
+            return rc;
+        }
+        vb->info.mode = d->mode;
+        vb->info.primitive_count = vb->info.leaf_records = vb->records;
+        vb->info.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        *out = vb;
+        return VKHRT_SUCCESS;
+    }
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -728,6 +885,26 @@ out:
     return VKHRT_SUCCESS;
 }
 
+int vkhrt_blas_voxel_grid_get(const vkhrt_blas* b, vkhrt_voxel_grid* out) {
+    if (!b || !out || b->mode != VKHRT_GEOMETRY_VOXELS) return VKHRT_ERROR_INVALID_ARGUMENT;
+    std::memcpy(out->res, b->vres, sizeof(out->res));
+    std::memcpy(out->origin, b->vorigin, sizeof(out->origin));
+    out->cell = b->vcell;
+    out->words = (uint64_t(b->vres[0]) * uint64_t(b->vres[1]) * uint64_t(b->vres[2]) + 31) / 32;
+    return VKHRT_SUCCESS;
+}
+
+int vkhrt_blas_voxels_read(vkhrt_context* c, const vkhrt_blas* b, uint32_t* words, uint64_t count) {
+    if (!c || !b || (count && !words)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
+    if (b->mode != VKHRT_GEOMETRY_VOXELS) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "not a voxel BLAS");
+    const uint64_t have = (uint64_t(b->vres[0]) * uint64_t(b->vres[1]) * uint64_t(b->vres[2]) + 31) / 32;
+    if (count > have) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "more words than the grid holds");
+    if (count == 0) return VKHRT_SUCCESS;
+    hipSetDevice(c->device);
+    hipError_t e = hipMemcpy(words, b->voxels, count * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "voxel readback");
+}
+
 int vkhrt_blas_info_get(const vkhrt_blas* b, vkhrt_blas_info* out) {
     if (!b || !out) return VKHRT_ERROR_INVALID_ARGUMENT;
     *out = b->info;
@@ -741,6 +918,7 @@ void vkhrt_blas_destroy(vkhrt_blas* b) {
     hipFree(b->bvh.prims);
     hipFree(b->bvh.aux);
     hipFree(b->normals);
+    hipFree(b->voxels);
     delete b;
 }
 
@@ -764,6 +942,10 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         I.blas.radius_mode = b->radius_mode;
         I.blas.radius = b->radius;
         I.blas.normals = b->normals;
+        I.blas.voxels = b->voxels;
+        std::memcpy(I.blas.vorigin, b->vorigin, sizeof(I.blas.vorigin));
+        I.blas.vcell = b->vcell;
+        std::memcpy(I.blas.vres, b->vres, sizeof(I.blas.vres));
         I.material = b->material_index;
         const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
         const float* xf = b->has_xf ? b->xf : ident;
@@ -819,7 +1001,8 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         int v = -1;
         for (uint32_t i = 0; i < count; ++i) {
             const vkhrt_blas* b = blases[i];
-            int vi = b->mode == VKHRT_GEOMETRY_LSS         ? vk::kVarLss
+            int vi = b->mode == VKHRT_GEOMETRY_VOXELS      ? vk::kVarMixed  // per-instance dispatch walks the grid
+                     : b->mode == VKHRT_GEOMETRY_LSS       ? vk::kVarLss
                      : b->mode == VKHRT_GEOMETRY_DOTS      ? vk::kVarDots
                      : b->mode == VKHRT_GEOMETRY_TRIANGLES ? vk::kVarTris
                      : b->radius_mode                      ? vk::kVarPhPoint
@@ -1588,8 +1771,10 @@ int vkhrt_asset_build(vkhrt_context* c, const vkhrt_asset* a, const vkhrt_asset_
     if (capacity < n)
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT,
                     "capacity " + std::to_string(capacity) + " is below the asset's " + std::to_string(n) + " instances");
-    if (d->hair_mode > VKHRT_GEOMETRY_DOTS)
-        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "hair mode must be phantom curves, LSS or DOTS");
+    if (d->hair_mode > VKHRT_GEOMETRY_VOXELS || d->hair_mode == VKHRT_GEOMETRY_TRIANGLES)
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "hair mode must be phantom curves, LSS, DOTS or voxels");
+    if (d->voxel_flags & ~uint32_t(VKHRT_BLAS_VOXEL_REFERENCE_SWEEP))
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown voxel flags");
     if (!(d->radius >= 0.0f) || !std::isfinite(d->radius))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "radius must be finite and >= 0");
     if (!std::isfinite(d->tension)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "tension must be finite");
@@ -1634,7 +1819,8 @@ int vkhrt_asset_build(vkhrt_context* c, const vkhrt_asset* a, const vkhrt_asset_
                     g.recs = g.verts;
                 } else {
                     rc = vkhrt_generate_mesh_lines(c, g.verts, nv, g.idx, g.count, d->radius, g.lines, stream);
-                    if (rc == VKHRT_SUCCESS && d->hair_mode == VKHRT_GEOMETRY_DOTS) {
+                    if (rc == VKHRT_SUCCESS &&
+                        (d->hair_mode == VKHRT_GEOMETRY_DOTS || d->hair_mode == VKHRT_GEOMETRY_VOXELS)) {
                         g.recs = g.lines;
                     } else if (rc == VKHRT_SUCCESS) {
                         const bool curves = d->hair_mode == VKHRT_GEOMETRY_PHANTOM_CURVES;
@@ -1661,6 +1847,13 @@ int vkhrt_asset_build(vkhrt_context* c, const vkhrt_asset* a, const vkhrt_asset_
         if (!hair) {
             bd.indices = g.idx;
             bd.vertex_count = nv;
+        } else if (d->hair_mode == VKHRT_GEOMETRY_VOXELS) {
+            // ProcessHairVoxels: the grid over the mesh's bounding box (geometry_processor.cpp:716)
+            bd.flags |= VKHRT_BLAS_HAS_BOUNDS | d->voxel_flags;
+            bd.voxel_size = d->voxel_size;
+            bd.voxel_lod = d->voxel_lod;
+            std::memcpy(bd.bounds_min, M.lo, sizeof(bd.bounds_min));
+            std::memcpy(bd.bounds_max, M.hi, sizeof(bd.bounds_max));
         }
         rc = vkhrt_blas_build(c, &bd, stream, &blases[i]);
         if (rc != VKHRT_SUCCESS) c->err = "asset instance " + std::to_string(i) + ": " + c->err;

@@ -6,6 +6,7 @@
 //                 LSS capsule   : 2 x float4 (a xyz, +-ra ; b xyz, rb)        32 B
 //                 DOTS piece    : 4 x float4 (start,r0 ; end,r1 ; s ; t)      64 B
 //                 triangle      : 3 x float4 (vertex positions)               48 B
+//   voxels    : voxel BLAS only, one occupancy bit per cell (x fastest) in u32 words; no nodes
 //   prim_ids  : original primitive index per leaf slot (u32)
 //   nodes     : 8-wide BVH nodes with 8-bit quantized child boxes, 80 B each
 #pragma once
@@ -14,7 +15,7 @@
 
 namespace vk {
 
-enum GeometryMode { kPhantom = 0, kLss = 1, kDots = 2, kTriangles = 3 };
+enum GeometryMode { kPhantom = 0, kLss = 1, kDots = 2, kTriangles = 3, kVoxels = 4 };
 
 // 8-wide node. Child slot s has a box quantized to 8 bits per axis relative to the node
 // origin with power-of-two scales; internal children are stored contiguously from
@@ -44,6 +45,10 @@ struct BlasDev {
     uint32_t mode;
     uint32_t radius_mode;  // phantom: 0 constant radius, 1 per control point
     float radius;          // phantom constant radius
+    const uint32_t* voxels;  // voxels: occupancy bits of the traced level of detail
+    float vorigin[3];        // voxels: grid origin (object space)
+    float vcell;             // voxels: cell edge at the traced level
+    int32_t vres[3];         // voxels: cells per axis at the traced level
 };
 
 struct InstanceDev {

@@ -332,6 +332,89 @@ __device__ __forceinline__ bool ray_box(f3 o, f3 d, const float* lo, const float
 }
 
 // Traces one world-space ray through all instances. Fills `best`; returns false on overflow.
+// Voxel instances (BLASType::eVoxels): a 3D DDA (Amanatides, Woo, Eurographics 1987) over the
+// occupancy grid from the ray's entry into the grid box. The hit is the first filled cell the
+// walk enters within [tmin, tcur], at its entry t (never before the box entry), with the normal
+// of the face crossed; a ray that starts inside a filled cell takes the face across its major
+// axis. Cell boundaries are origin + i * size in float, like the reference's voxel boxes
+// (GetVoxelWorldPosition, geometry_processor.cpp:438-449); the primitive id is the cell's
+// linear index (its AABB index there, gl_PrimitiveID in voxel_closest_hit.rchit:20).
+template <bool ANY>
+__device__ __forceinline__ void voxel_walk(const BlasDev& B, f3 o, f3 d, float tmin, uint32_t inst, Best& best,
+                                           bool& done) {
+    const int R[3] = {B.vres[0], B.vres[1], B.vres[2]};
+    if (R[0] <= 0 || R[1] <= 0 || R[2] <= 0) return;
+    const float O[3] = {B.vorigin[0], B.vorigin[1], B.vorigin[2]};
+    const float c = B.vcell;
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    float inv[3];
+    float t0 = tmin, t1 = best.t;
+    int ea = -1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float hi = O[k] + float(R[k]) * c;
+        if (dd[k] == 0.0f) {
+            if (oo[k] < O[k] || oo[k] > hi) return;
+            inv[k] = 0.0f;
+            continue;
+        }
+        inv[k] = 1.0f / dd[k];
+        const float ta = (O[k] - oo[k]) * inv[k], tb = (hi - oo[k]) * inv[k];
+        const float lo = ta < tb ? ta : tb, up = ta < tb ? tb : ta;
+        if (lo > t0) {
+            t0 = lo;
+            ea = k;
+        }
+        if (up < t1) t1 = up;
+    }
+    if (!(t0 <= t1)) return;
+    int i[3], step[3];
+    float tn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float f = floorf((oo[k] + dd[k] * t0 - O[k]) / c);
+        i[k] = f < 0.0f ? 0 : f > float(R[k] - 1) ? R[k] - 1 : int(f);
+        step[k] = dd[k] > 0.0f ? 1 : dd[k] < 0.0f ? -1 : 0;
+        tn[k] = step[k] == 0 ? INFINITY : (O[k] + float(i[k] + (step[k] > 0 ? 1 : 0)) * c - oo[k]) * inv[k];
+    }
+    const uint32_t rx = uint32_t(R[0]), ry = uint32_t(R[1]);
+    float te = t0;
+    for (int it = 0, lim = R[0] + R[1] + R[2] + 3; it < lim; ++it) {
+        const uint32_t idx = uint32_t(i[0]) + uint32_t(i[1]) * rx + uint32_t(i[2]) * (rx * ry);
+        if ((B.voxels[idx >> 5] >> (idx & 31u)) & 1u) {
+            if (te < best.t || (te == best.t && (inst < best.inst || (inst == best.inst && idx < best.pid)))) {
+                int ax = ea;
+                if (ax < 0) {
+                    ax = 0;
+                    if (fabsf(dd[1]) > fabsf(dd[ax])) ax = 1;
+                    if (fabsf(dd[2]) > fabsf(dd[ax])) ax = 2;
+                }
+                const float sgn = dd[ax] > 0.0f ? -1.0f : 1.0f;
+                best.t = te;
+                best.inst = inst;
+                best.pid = idx;
+                best.slot = idx;
+                best.u = 0.0f;
+                best.a = ax == 0 ? sgn : 0.0f;
+                best.b = ax == 1 ? sgn : 0.0f;
+                best.c = ax == 2 ? sgn : 0.0f;
+                best.extra = 0;
+                if (ANY) done = true;
+            }
+            return;
+        }
+        int a = 0;
+        if (tn[1] < tn[a]) a = 1;
+        if (tn[2] < tn[a]) a = 2;
+        if (!(tn[a] <= t1)) return;
+        te = tn[a] > t0 ? tn[a] : t0;
+        ea = a;
+        i[a] += step[a];
+        if (i[a] < 0 || i[a] >= R[a]) return;
+        tn[a] = (O[a] + float(i[a] + (step[a] > 0 ? 1 : 0)) * c - oo[a]) * inv[a];
+    }
+}
+
 template <class G, bool ANY, int STACK, bool STATS = false>
 __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, float tmin, float tmax, Best& best,
                                             uint2* lds, uint32_t ray_id, TravStats* st = nullptr) {
@@ -360,6 +443,9 @@ __device__ __forceinline__ bool trace_world(const TraceParams& tp, f3 o, f3 d, f
                 r = traverse<kDots, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
             } else if (I.blas.mode == kTriangles) {
                 r = traverse<kTriangles, 0, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
+            } else if (I.blas.mode == kVoxels) {
+                voxel_walk<ANY>(I.blas, oo, dd, tmin, i, best, done);
+                r = true;
             } else if (I.blas.radius_mode == kRadiusPerPoint) {
                 r = pol ? traverse<kPhantom, kRadiusPerPoint, true, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id)
                         : traverse<kPhantom, kRadiusPerPoint, false, ANY, STACK>(I.blas, oo, dd, tmin, i, best, lds, done, tp, ray_id);
@@ -441,7 +527,7 @@ __device__ __forceinline__ void finish_hit(const TraceParams& tp, f3 o, f3 d, co
         ph.polished = best.extra != 0;
         n = phantom_normal(dd, ph);
         v = best.extra ? 1.0f : 0.0f;
-    } else if (I.blas.mode == kLss) {
+    } else if (I.blas.mode == kLss || I.blas.mode == kVoxels) {
         n = f3{best.a, best.b, best.c};
     } else if (I.blas.mode == kTriangles) {
         // triangle_closest_hit.rchit:31-36: normalize(n0 b0 + n1 b1 + n2 b2), b = (1 - u - v, u, v);
