@@ -55,7 +55,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural shadow form")
-    ap.add_argument("--no-modes", action="store_true", help="skip the LSS and DOTS forms of the frame")
+    ap.add_argument("--no-modes", action="store_true", help="skip the LSS, DOTS and voxel forms of the frame")
+    ap.add_argument("--voxel-size", type=float, default=0.5, help="cell edge of the voxel form")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view path-traced form")
     ap.add_argument("--no-area", action="store_true", help="skip the 16M-segment area-light neural form")
     ap.add_argument("--autotune", type=int, default=1, help="pick BVH width, leaf size, stack kind, blocks per CU, "
@@ -346,11 +347,13 @@ def neural_block(args, ctx, tlas, cam, W, H, flags, exact_ms, sp, stream, torch,
 
 
 def geometry_modes_block(args, ctx, tlas, cam, W, H, phantom_ms, sp, stream, torch, V, curves, nseg):
-    """The same hair and frame in the other two geometry modes: every cubic segment split into
+    """The same hair and frame in the other geometry modes: every cubic segment split into
     4 linear pieces (12.8M at 3.2M segments), traced as Linear Swept Spheres (one capsule per
-    piece) and as DOTS (2 orthogonal quads = 4 triangles per piece). Reported per mode: build
-    time, frame time and Mrays/s of the exact primary + shadow frame, and the PSNR of its float
-    image against the phantom image of the same frame (the image error the mode trades)."""
+    piece), as DOTS (2 orthogonal quads = 4 triangles per piece) and as voxels (the pieces
+    voxelized as capsules of radius 0.02 into cells of --voxel-size, walked by the DDA).
+    Reported per mode: build time, frame time and Mrays/s of the exact primary + shadow frame,
+    and the PSNR of its float image against the phantom image of the same frame (the image
+    error the mode trades)."""
     pieces = nseg * 4
     pl = torch.empty(pieces * 8, dtype=torch.float32, device="cuda")
     ctx.curves_to_lines(curves.data_ptr(), nseg, 4, pl.data_ptr(), stream=sp)
@@ -360,14 +363,14 @@ def geometry_modes_block(args, ctx, tlas, cam, W, H, phantom_ms, sp, stream, tor
     phantom_img = fimg.view(H, W, 4)[..., :3].clone()
     image = torch.empty(W * H, dtype=torch.int32, device="cuda")
     out = {}
-    for mode, name in ((V.GEOMETRY_LSS, "lss"), (V.GEOMETRY_DOTS, "dots")):
+    for mode, name in ((V.GEOMETRY_LSS, "lss"), (V.GEOMETRY_DOTS, "dots"), (V.GEOMETRY_VOXELS, "voxels")):
         if mode == V.GEOMETRY_LSS:
             prims = torch.empty(pieces * 8, dtype=torch.float32, device="cuda")
             ctx.generate_lss(pl.data_ptr(), pieces, prims.data_ptr(), stream=sp)
         else:
-            prims = pl  # the DOTS BLAS takes the lines and builds its quads on the device
-        blas = V.BottomLevelAccelerationStructure(ctx, mode, prims.data_ptr(), pieces,
-                                                  leaf_size=args.leaf_size, stream=sp)
+            prims = pl  # the DOTS and voxel BLASes take the lines and build on the device
+        blas = V.BottomLevelAccelerationStructure(ctx, mode, prims.data_ptr(), pieces, leaf_size=args.leaf_size,
+                                                  voxel_size=args.voxel_size, stream=sp)
         mtlas = V.TopLevelAccelerationStructure(ctx, [blas], stream=sp)
         torch.cuda.synchronize()
         info = blas.info()
@@ -395,6 +398,10 @@ def geometry_modes_block(args, ctx, tlas, cam, W, H, phantom_ms, sp, stream, tor
         mrays = (n_primary + n_shadow) / (ms * 1e-3) / 1e6
         log(f"[bench] {name}: {pieces} pieces, build {info['build_ms']:.1f} ms, depth {info['depth']}, "
             f"{ms:.3f} ms/frame, {mrays:.1f} Mrays/s, PSNR vs phantom {psnr:.2f} dB")
+        if mode == V.GEOMETRY_VOXELS:
+            cells, _, cell = blas.voxel_grid()
+            voxel_info = {"voxel_size": cell, "grid": list(cells.shape[::-1]), "filled": int(info["leaf_records"])}
+            log(f"[bench] voxels: grid {voxel_info['grid']} of {cell} cells, {voxel_info['filled']} filled")
         out[name] = {
             "pieces": pieces,
             "triangles": pieces * 4 if mode == V.GEOMETRY_DOTS else None,
@@ -407,6 +414,9 @@ def geometry_modes_block(args, ctx, tlas, cam, W, H, phantom_ms, sp, stream, tor
             "bvh": {"nodes": info["node_count"], "depth": info["depth"], "build_ms": round(info["build_ms"], 3),
                     "device_bytes": info["device_bytes"]},
         }
+        if mode == V.GEOMETRY_VOXELS:
+            out[name]["triangles"] = None
+            out[name]["voxels"] = voxel_info
         mtlas.close()
         blas.close()
         del prims
