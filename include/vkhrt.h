@@ -154,6 +154,9 @@ typedef struct vkhrt_hit {
 /* vkhrt_trace_rays flags */
 #define VKHRT_TRACE_TERMINATE_ON_FIRST_HIT (1u << 0) /* shadow / any-hit rays */
 #define VKHRT_TRACE_FP64_POLISH (1u << 3)            /* FP64 Newton polish of grazing phantom roots */
+#define VKHRT_TRACE_ACCURATE_NORMALS (1u << 5)       /* phantom hits: t, u and normal of the final hit
+                                                        re-evaluated in double-word FP32 (the FP64
+                                                        intersector's iteration; no FP64 used) */
 
 /* Column-major inverse view / projection (CameraUniformData). */
 typedef struct vkhrt_camera {
@@ -166,6 +169,9 @@ typedef struct vkhrt_camera {
 #define VKHRT_RENDER_DEBUG_PALETTE (1u << 1) /* debug.glsl palette by primitive id */
 #define VKHRT_RENDER_RGBA8 (1u << 2)         /* image is RGBA8 (else RGBA32F) */
 #define VKHRT_RENDER_FP64_POLISH (1u << 3)
+#define VKHRT_RENDER_ACCURATE_NORMALS (1u << 5) /* primary phantom hits re-evaluated as with
+                                                   VKHRT_TRACE_ACCURATE_NORMALS (persistent
+                                                   kernels, single-mode TLAS) */
 #define VKHRT_RENDER_NEURAL_SHADOWS (1u << 4) /* shadow visibility from `shadow_mlp`; rays whose
                                                  predicted variance p (1 - p) exceeds
                                                  `shadow_variance_threshold` trace exactly */

@@ -21,6 +21,7 @@
 #include <thread>
 #include <vector>
 
+#include "oracle_ff.hpp"
 #include "oracle_geom.hpp"
 #include "oracle_mlp.hpp"
 #include "oracle_path.hpp"
@@ -37,6 +38,7 @@ enum {
     F_FP64 = 1u << 2,
     F_POLISH = 1u << 3,
     F_REF_NORMAL = 1u << 4,
+    F_ACCURATE = 1u << 5,
 };
 
 struct Hit {
@@ -464,6 +466,34 @@ void normal_to_world(const Instance& I, float n[3]) {
     n[2] = a.z;
 }
 
+// Double-word re-run of Prhi for the final phantom hit (extension): the iteration of the FP64
+// evaluation from the start, in FP32 pairs (oracle_ff.hpp), without the early-out test; t, u
+// and the ray-centric normal are replaced when it converges with t > 0. Polished hits keep
+// their FP64 values.
+void accurate_phantom(const Instance& I, const Ray<float>& w, Hit& best) {
+    const Blas& B = *I.blas;
+    if (B.mode != MODE_PHANTOM || best.v != 0.0f) return;
+    Ray<float> o;
+    obj_ray(I, w, o);
+    const float* c = &B.prims[size_t(best.prim) * 16];
+    PrhiOpts opt;
+    opt.radius_mode = B.per_point ? RADIUS_PER_POINT : RADIUS_CONSTANT;
+    opt.radius = B.radius;
+    opt.cull = false;
+    Curve<FF> cv{mk(FF(c[0]), FF(c[1]), FF(c[2])), mk(FF(c[4]), FF(c[5]), FF(c[6])), mk(FF(c[8]), FF(c[9]), FF(c[10])),
+                 mk(FF(c[12]), FF(c[13]), FF(c[14]))};
+    FF rr[4] = {FF(c[3]), FF(c[7]), FF(c[11]), FF(c[15])};
+    Ray<FF> r{cvt<FF>(o.origin), cvt<FF>(o.direction)};
+    PrhiOut<FF> out = Prhi(r, cv, rr, opt);
+    if (!(out.t > FF(0.0f))) return;
+    best.t = float(out.t);
+    best.u = float(out.u);
+    best.n[0] = float(out.n_rcc.x);
+    best.n[1] = float(out.n_rcc.y);
+    best.n[2] = float(out.n_rcc.z);
+    normal_to_world(I, best.n);
+}
+
 template <class T>
 bool trace_scene(const Scene& S, const Ray<float>& r, float tmin, float tmax, uint32_t flags, Hit& best) {
     bool found = false;
@@ -487,6 +517,8 @@ bool trace_scene(const Scene& S, const Ray<float>& r, float tmin, float tmax, ui
     if (!found) {
         best.t = 0.0f;
         best.prim = best.inst = 0xFFFFFFFFu;
+    } else if (flags & F_ACCURATE) {
+        accurate_phantom(S.inst[best.inst], r, best);
     }
     return found;
 }
@@ -807,10 +839,27 @@ int orc_prhi(const float* ray /*8*/, const float* curve /*16*/, int per_point, f
                         double(p.n_rcc.x), double(p.n_rcc.y), double(p.n_rcc.z), double(p.iters),   p.polished ? 1.0 : 0.0};
         for (int i = 0; i < 10; ++i) out[i] = float(v[i]);
     };
-    if (fp64)
+    if (fp64 == 2) {  // double-word FP32: the final-hit re-run (no early out, no polish)
+        o.cull = false;
+        o.fp64_polish = false;
+        run(FF(0.0f));
+    } else if (fp64) {
         run(0.0);
-    else
+    } else {
         run(0.0f);
+    }
+    return 0;
+}
+
+// Double-word operations for pinning oracle_ff.hpp: a, b, out hold n (hi, lo) pairs;
+// op 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (of a).
+int orc_ff_eval(const float* a, const float* b, uint32_t n, int op, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const FF x(a[2 * i], a[2 * i + 1]), y(b[2 * i], b[2 * i + 1]);
+        const FF z = op == 0 ? x + y : op == 1 ? x - y : op == 2 ? x * y : op == 3 ? x / y : sqrt(x);
+        out[2 * i] = z.hi;
+        out[2 * i + 1] = z.lo;
+    }
     return 0;
 }
 
@@ -1147,7 +1196,7 @@ long long orc_render(orc_scene* sp, const float view_inv[16], const float proj_i
         Ray<float> r{o, d};
         Hit h;
         long long nr = 1;
-        bool hit = trace_scene<float>(S, r, 0.001f, 10000.0f, flags & (F_REF_NORMAL | F_POLISH), h);
+        bool hit = trace_scene<float>(S, r, 0.001f, 10000.0f, flags & (F_REF_NORMAL | F_POLISH | F_ACCURATE), h);
         float rgb[3] = {0.0f, 0.0f, 0.0f};
         if (!hit && env) EnvironmentColor(env, ew, eh, exposure, d, rgb);
         if (hit) {

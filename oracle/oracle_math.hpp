@@ -19,6 +19,19 @@
 
 namespace orc {
 
+// sqrt / fabs found by argument-dependent lookup, so a user scalar type (the double-word FF
+// of oracle_ff.hpp) runs the same templates; for float and double these are std's.
+template <class T>
+inline T osqrt(T x) {
+    using std::sqrt;
+    return sqrt(x);
+}
+template <class T>
+inline T ofabs(T x) {
+    using std::fabs;
+    return fabs(x);
+}
+
 template <class T>
 struct V3 {
     T x, y, z;
@@ -46,10 +59,10 @@ inline V3<T> cross(V3<T> a, V3<T> b) {
     return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
 }
 template <class T>
-inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+inline T length(V3<T> a) { return osqrt(dot(a, a)); }
 // glm::normalize: v * inversesqrt(dot(v, v)), inversesqrt(x) = 1 / sqrt(x).
 template <class T>
-inline V3<T> normalize(V3<T> a) { return a * (T(1) / std::sqrt(dot(a, a))); }
+inline V3<T> normalize(V3<T> a) { return a * (T(1) / osqrt(dot(a, a))); }
 // GLSL min/max definitions (min(x,y) = y < x ? y : x; max(x,y) = x < y ? y : x).
 template <class T>
 inline T gmin(T x, T y) { return y < x ? y : x; }
@@ -134,7 +147,7 @@ struct Ray {
 // ray.glsl:13-17 MakeOrthonormalBasis. Pinned by orthonormality checks.
 template <class T>
 inline void MakeOrthonormalBasis(V3<T>& u, V3<T>& v, V3<T> w) {
-    v = std::fabs(w.x) > std::fabs(w.y) ? normalize(mk(-w.z, T(0), w.x)) : normalize(mk(T(0), w.z, -w.y));
+    v = ofabs(w.x) > ofabs(w.y) ? normalize(mk(-w.z, T(0), w.x)) : normalize(mk(T(0), w.z, -w.y));
     u = cross(v, w);
 }
 
@@ -239,7 +252,7 @@ inline RCI<T> RayConeIntersectRCC(const Cone<T>& cone) {
     r.a = a;
     r.gscale = ddd * r2;
     r.isIntersection = det > T(0);
-    r.s = (b - (r.isIntersection ? std::sqrt(det) : T(0))) / c;
+    r.s = (b - (r.isIntersection ? osqrt(det) : T(0))) / c;
     r.dt = (r.s * cone.axis.z - cdd) / ddd;
     r.dc = r.s * r.s + dp;
     r.sp = cdd / cone.axis.z;
@@ -262,12 +275,12 @@ inline bool RayCylinderIntersect(const Ray<T>& ray, V3<T> p0, V3<T> p1, T radius
     T k0 = baba * dot(oc, oc) - baoc * baoc - radius * radius * baba;
     T h = k1 * k1 - k2 * k0;
     if (h < T(0)) return false;
-    h = std::sqrt(h);
+    h = osqrt(h);
     T t = (-k1 - h) / k2;
     T y = baoc + t * bard;
     if (y > T(0) && y < baba) return true;
     t = ((y < T(0) ? T(0) : baba) - baoc) / bard;
-    if (std::fabs(k1 + k2 * t) < h) return true;
+    if (ofabs(k1 + k2 * t) < h) return true;
     return false;
 }
 
@@ -291,6 +304,7 @@ struct PrhiOpts {
     int radius_mode = RADIUS_CONSTANT;
     float radius = 0.02f;   // constant radius (reference: 0.02)
     bool fp64_polish = false;
+    bool cull = true;       // the early-out cylinder test (:24-33); off to re-run a known hit
 };
 
 // FP64 continuation of a side that ran all iterations without converging while grazing.
@@ -328,8 +342,8 @@ inline bool prhi_side_iterate(const Curve<T>& rcc, const T radii[4], int radius_
         cone.axis = SampleCurveAxis(rcc, t);
         RCI<T> rci = RayConeIntersectRCC(cone);
         ++iters;
-        grazing = std::fabs(rci.a) <= T(1e-2f) * rci.gscale;
-        if (rci.isIntersection && std::fabs(rci.dt) < T(5e-5f)) {
+        grazing = ofabs(rci.a) <= T(1e-2f) * rci.gscale;
+        if (rci.isIntersection && ofabs(rci.dt) < T(5e-5f)) {
             s_out = rci.s;  // relative to cone.center.z
             u_out = t;
             cone_out = cone;
@@ -367,9 +381,11 @@ PrhiOut<T> Prhi(const Ray<T>& ray, const Curve<T>& curve, const T radii[4], cons
     T rpad = const_r;
     if (o.radius_mode == RADIUS_PER_POINT) rpad = gmax(gmax(radii[0], radii[1]), gmax(radii[2], radii[3]));
 
-    T rmax = CurveDistanceToCylinder(curve, SampleCurvePoint(curve, T(0.5f)));
-    rmax += rpad;
-    if (!RayCylinderIntersect(ray, curve.start, curve.end, rmax)) return out;
+    if (o.cull) {
+        T rmax = CurveDistanceToCylinder(curve, SampleCurvePoint(curve, T(0.5f)));
+        rmax += rpad;
+        if (!RayCylinderIntersect(ray, curve.start, curve.end, rmax)) return out;
+    }
 
     M4<T> M = CreateRCCMatrix(ray);
     Curve<T> rcc = TransformCurve(curve, M);
@@ -398,7 +414,7 @@ PrhiOut<T> Prhi(const Ray<T>& ray, const Curve<T>& curve, const T radii[4], cons
             MakeOrthonormalBasis(e1, e2, e3);
             V3<T> nr = mk(-cone.center.x, -cone.center.y, s_rel);
             out.n_rcc = normalize(nr.x * e1 + nr.y * e2 + nr.z * e3);
-        } else if (o.fp64_polish && exhausted && t >= T(0) && t <= T(1) && std::fabs(last_dt) < T(0.05f) && grazing) {
+        } else if (o.fp64_polish && exhausted && t >= T(0) && t <= T(1) && ofabs(last_dt) < T(0.05f) && grazing) {
             // FP64 polish of a grazing side (extension).
             Ray<double> rd{cvt<double>(ray.origin), cvt<double>(ray.direction)};
             Curve<double> cd{cvt<double>(curve.start), cvt<double>(curve.cp1), cvt<double>(curve.cp2),
@@ -446,7 +462,7 @@ inline V3<T> Shade(V3<T> n, T visibility) {
     const V3<T> lightColor = mk(T(0.4f), T(0.2f), T(0.1f));
     const V3<T> lightDirection = mk(T(0), T(-1), T(0));
     const V3<T> ambientColor = mk(T(0.3f), T(0.3f), T(0.3f));
-    T ndl = std::fabs(dot(n, lightDirection));
+    T ndl = ofabs(dot(n, lightDirection));
     V3<T> color = (ndl * visibility) * lightColor;
     return color + ambientColor;
 }

@@ -14,7 +14,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "build", "liboracle.so")
 
 MODE_PHANTOM, MODE_LSS, MODE_DOTS, MODE_VOXELS = 0, 1, 2, 4
-F_ANY_HIT, F_BRUTE, F_FP64, F_POLISH, F_REF_NORMAL = 1, 2, 4, 8, 16
+F_ANY_HIT, F_BRUTE, F_FP64, F_POLISH, F_REF_NORMAL, F_ACCURATE = 1, 2, 4, 8, 16, 32
 
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("u", "<f4"),
                       ("n", "<f4", (3,)), ("v", "<f4")])

@@ -26,12 +26,14 @@ BLAS_VOXEL_REFERENCE_SWEEP = 1 << 4
 
 TRACE_TERMINATE_ON_FIRST_HIT = 1 << 0
 TRACE_FP64_POLISH = 1 << 3
+TRACE_ACCURATE_NORMALS = 1 << 5
 
 RENDER_SHADOWS = 1 << 0
 RENDER_DEBUG_PALETTE = 1 << 1
 RENDER_RGBA8 = 1 << 2
 RENDER_FP64_POLISH = 1 << 3
 RENDER_NEURAL_SHADOWS = 1 << 4
+RENDER_ACCURATE_NORMALS = 1 << 5
 
 SHADOW_MLP_PARAMS = 18753
 

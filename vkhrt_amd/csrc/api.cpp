@@ -119,6 +119,7 @@ struct vkhrt_shadow_mlp {
 struct vkhrt_blas {
     vkhrt_context* ctx = nullptr;
     float4* normals = nullptr;  // triangle meshes: 3 vertex normals per triangle
+    uint32_t* slot_of = nullptr;  // phantom: leaf slot per original primitive id
     uint32_t mode = 0;
     uint32_t radius_mode = 0;
     float radius = 0.02f;
@@ -847,6 +848,14 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
             goto out;
         }
         root_bounds(root, b->lo, b->hi);
+        if (d->mode == VKHRT_GEOMETRY_PHANTOM_CURVES) {  // for the double-word re-run of final hits
+            e = hipMalloc(&b->slot_of, size_t(d->count) * sizeof(uint32_t));
+            if (e == hipSuccess) e = vk::gen_slot_of(b->bvh.prim_ids, d->count, b->slot_of, st);
+            if (e != hipSuccess) {
+                rc = hip_fail(c, e, "slot map");
+                goto out;
+            }
+        }
     } else {
         for (int k = 0; k < 3; ++k) {
             b->lo[k] = INFINITY;
@@ -919,6 +928,7 @@ void vkhrt_blas_destroy(vkhrt_blas* b) {
     hipFree(b->bvh.aux);
     hipFree(b->normals);
     hipFree(b->voxels);
+    hipFree(b->slot_of);
     delete b;
 }
 
@@ -942,6 +952,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         I.blas.radius_mode = b->radius_mode;
         I.blas.radius = b->radius;
         I.blas.normals = b->normals;
+        I.blas.slot_of = b->slot_of;
         I.blas.voxels = b->voxels;
         std::memcpy(I.blas.vorigin, b->vorigin, sizeof(I.blas.vorigin));
         I.blas.vcell = b->vcell;
@@ -1043,8 +1054,10 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
                      uint32_t flags, void* stream) {
     if (!c || !t) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
     if (count && (!rays || !hits)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null rays or hits");
-    if (flags & ~(VKHRT_TRACE_TERMINATE_ON_FIRST_HIT | VKHRT_TRACE_FP64_POLISH))
+    if (flags & ~(VKHRT_TRACE_TERMINATE_ON_FIRST_HIT | VKHRT_TRACE_FP64_POLISH | VKHRT_TRACE_ACCURATE_NORMALS))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown trace flags");
+    const bool accurate = (flags & VKHRT_TRACE_ACCURATE_NORMALS) && !(flags & VKHRT_TRACE_TERMINATE_ON_FIRST_HIT);
+    flags &= ~uint32_t(VKHRT_TRACE_ACCURATE_NORMALS);
     hipStream_t st = pick(c, stream);
     // persistent phantom kernels compact intersection work through keys that need polish_ok
     const bool persistent = c->persistent && (t->polish_ok || !is_phantom(t));
@@ -1064,6 +1077,9 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
                                          reinterpret_cast<vk::HitRec*>(hits), nullptr,
                                          trace_variant(t, (flags & VKHRT_TRACE_FP64_POLISH) != 0),
                                          stack_kind_for(c, t), st);
+    if (e == hipSuccess && accurate)
+        e = vk::launch_accurate_hits(tp, reinterpret_cast<const float*>(rays), count, reinterpret_cast<vk::HitRec*>(hits),
+                                     st);
     tm.stop();
     if (e != hipSuccess) return hip_fail(c, e, "trace launch");
     return VKHRT_SUCCESS;
@@ -1076,7 +1092,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     if (d->width == 0 || d->height == 0 || uint64_t(d->width) * d->height > (1ull << 28))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad image size");
     const uint32_t known = VKHRT_RENDER_SHADOWS | VKHRT_RENDER_DEBUG_PALETTE | VKHRT_RENDER_RGBA8 |
-                           VKHRT_RENDER_FP64_POLISH | VKHRT_RENDER_NEURAL_SHADOWS;
+                           VKHRT_RENDER_FP64_POLISH | VKHRT_RENDER_NEURAL_SHADOWS | VKHRT_RENDER_ACCURATE_NORMALS;
     if (d->flags & ~known) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown render flags");
     const size_t npix = size_t(d->width) * d->height;
     const vkhrt_area_light* al = d->light_visibility ? nullptr : d->area_light;
@@ -1134,6 +1150,11 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
         if (rc != VKHRT_SUCCESS) return rc;
         set_polish(c, rp.tp);
         if (!rp.tp.work) rp.tp.work = c->counters + 1;
+    }
+    if (d->flags & VKHRT_RENDER_ACCURATE_NORMALS) {
+        if (!rp.tp.work || t->variant == vk::kVarMixed)
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "accurate normals need the persistent kernels and a single-mode TLAS");
+        rp.accurate = 1;
     }
     const bool neural = (d->flags & VKHRT_RENDER_NEURAL_SHADOWS) && rp.shadows && !rp.debug_palette &&
                         !d->light_visibility;

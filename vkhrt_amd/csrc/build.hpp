@@ -95,6 +95,7 @@ struct RenderParams {
     AreaLight light;           // shadow queue entries are ids pixel * light.samples + sample
     EnvMap env;                // miss colour
     float4* dirs;              // neural form with an area light: shadow-ray direction per queue entry
+    uint32_t accurate;         // re-evaluate primary phantom hits in double-word FP32
 };
 
 // Compile-time geometry variants of the traversal kernels. A TLAS whose instances share
@@ -201,6 +202,8 @@ hipError_t launch_render(const RenderParams& rp, int variant, int stack_kind, hi
 hipError_t launch_camera_rays(const Camera& cam, float tmin, float tmax, float* rays, hipStream_t st);
 hipError_t launch_shadow_rays(const float* rays, const HitRec* hits, uint32_t n, const float dir[3], float tmin,
                               float tmax, float* out, uint32_t* src, uint32_t* counter, hipStream_t st);
+// Double-word re-run of the final phantom hit of every ray (k_accurate_hits).
+hipError_t launch_accurate_hits(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, hipStream_t st);
 hipError_t launch_count_candidates(const TraceParams& tp, const float* rays, const HitRec* hits, uint32_t n,
                                    uint32_t* counts, hipStream_t st);
 

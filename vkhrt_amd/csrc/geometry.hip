@@ -234,7 +234,21 @@ __global__ void k_mesh_lines(const float* __restrict__ verts, uint32_t nverts, c
     if (miss) atomicAdd(bad, miss);
 }
 
+// Inverse of the leaf order: slot_of[prim_ids[s]] = s.
+__global__ void k_slot_of(const uint32_t* __restrict__ prim_ids, uint32_t n, uint32_t* __restrict__ slot_of) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t p = prim_ids[s];
+    if (p < n) slot_of[p] = s;
+}
+
 }  // namespace
+
+hipError_t gen_slot_of(const uint32_t* prim_ids, uint32_t n, uint32_t* slot_of, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slot_of, grid(n), dim3(kBlock), 0, st, prim_ids, n, slot_of);
+    return hipGetLastError();
+}
 
 hipError_t gen_mesh_lines(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t n, float radius,
                           float4* lines, uint32_t* bad, hipStream_t st) {

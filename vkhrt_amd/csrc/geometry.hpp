@@ -19,6 +19,7 @@ hipError_t gen_dots(const float4* lines, uint32_t n, float constant_radius, floa
 hipError_t lss_chain(const float4* caps, uint32_t n, float4* out, hipStream_t st);
 hipError_t gen_mesh_lines(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t n, float radius,
                           float4* lines, uint32_t* bad, hipStream_t st);
+hipError_t gen_slot_of(const uint32_t* prim_ids, uint32_t n, uint32_t* slot_of, hipStream_t st);
 hipError_t gen_tri_records(const float* verts, uint32_t nverts, const uint32_t* idx, uint32_t ntri, float4* recs,
                            float4* normals, uint32_t* bad, hipStream_t st);
 

@@ -40,6 +40,7 @@ struct BlasDev {
     const uint32_t* prim_ids;
     const WideNode* nodes;
     const float4* normals;  // triangles: 3 vertex normals per original triangle (shading normal)
+    const uint32_t* slot_of;  // phantom: leaf slot of each original primitive id
     uint32_t nprims;    // leaf primitives (curves / capsules / DOTS pieces)
     uint32_t nnodes;
     uint32_t mode;

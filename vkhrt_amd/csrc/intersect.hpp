@@ -8,6 +8,7 @@
 // (RayFrame) instead of once per ray x segment as in the GLSL; the arithmetic is the same.
 #pragma once
 #include "device_math.hpp"
+#include "ffloat.hpp"
 
 namespace vk {
 
@@ -268,6 +269,52 @@ __device__ __forceinline__ bool prhi(const RayFrame& F, const float4* cv, float 
             }
         }
         tStart = 1.0f - tStart;
+    }
+    return false;
+}
+
+// Final-hit re-run of Prhi in double-word FP32 (extension, ffloat.hpp): the iteration of the
+// FP64 evaluation from the start, without the early-out test, for a ray / curve pair the FP32
+// pass already chose. On convergence with s > 0 it returns t, u and the unit normal (ray-
+// centric vector rotated back), rounded once, in `out` (marked polished: the normal is final).
+__device__ __forceinline__ bool prhi_ff(f3 of, f3 df, const float4* cv, int radius_mode, float const_r,
+                                        PhantomHit& out) {
+    const ff3 o = dw::toff(of), d = dw::toff(df);
+    ff3 e1, e2, e3 = nrm3(d);
+    onb(e3, e1, e2);
+    ff M[12];
+    rcc_rows<ff>(e1, e2, e3, o, M);
+    ff3 q[4];
+    ff rr[4];
+    for (int k = 0; k < 4; ++k) {
+        const ff3 p = ff3{ff(cv[k].x), ff(cv[k].y), ff(cv[k].z)};
+        q[k] = ff3{M[0] * p.x + M[1] * p.y + M[2] * p.z + M[3] * ff(1), M[4] * p.x + M[5] * p.y + M[6] * p.z + M[7] * ff(1),
+                   M[8] * p.x + M[9] * p.y + M[10] * p.z + M[11] * ff(1)};
+        rr[k] = ff(cv[k].w);
+    }
+    const ff3 cd = nrm3(sub(q[3], q[0]));
+    const ff dz = cd.x * ff(0) + cd.y * ff(0) + cd.z * ff(1);
+    ff tStart = dz > ff(0) ? ff(0) : ff(1);
+    for (int side = 0; side < 2; ++side) {
+        ff t = tStart, told = ff(0), dt1 = ff(0), dt2 = ff(0), s_rel = ff(0), cx = ff(0), cy = ff(0), cz = ff(0),
+           last = ff(0);
+        const int st = prhi_iterate(q[0], q[1], q[2], q[3], rr, radius_mode, ff(const_r), t, told, dt1, dt2, 0, 8, s_rel,
+                                    cx, cy, cz, last);
+        if (st == 1) {
+            const ff s = s_rel + cz;
+            if (s > ff(0)) {
+                const ff3 n = nrm3(add(add(scl(-cx, e1), scl(-cy, e2)), scl(s_rel, e3)));
+                out.t = s.hi;
+                out.u = t.hi;
+                out.ncx = n.x.hi;
+                out.ncy = n.y.hi;
+                out.nsz = n.z.hi;
+                out.polished = true;
+                out.deferred = false;
+                return true;
+            }
+        }
+        tStart = ff(1) - tStart;
     }
     return false;
 }

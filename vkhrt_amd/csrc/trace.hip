@@ -348,6 +348,14 @@ hipError_t launch_shadow_rays(const float* rays, const HitRec* hits, uint32_t n,
     return hipGetLastError();
 }
 
+hipError_t launch_accurate_hits(const TraceParams& tp, const float* rays, uint32_t n, HitRec* hits, hipStream_t st) {
+    using namespace trace_detail;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_accurate_hits<SrcRays>, dim3((n + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, st, tp,
+                       SrcRays{reinterpret_cast<const float4*>(rays)}, hits, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_count_candidates(const TraceParams& tp, const float* rays, const HitRec* hits, uint32_t n,
                                    uint32_t* counts, hipStream_t st) {
     if (n == 0) return hipSuccess;

@@ -1248,6 +1248,43 @@ struct SrcCamera {
         tmax = 10000.0f;
     }
 };
+// Double-word re-run of each final phantom hit (VKHRT_*_ACCURATE_NORMALS): the ray from `src`,
+// in the instance's object space, against the hit's curve record; t, u and the normal are
+// replaced when the re-run converges. Misses, other modes and FP64-polished roots are kept.
+template <class Src>
+__global__ __launch_bounds__(kTraceBlock) void k_accurate_hits(TraceParams tp, Src src, HitRec* __restrict__ hits,
+                                                               uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    HitRec h = hits[i];
+    if (h.prim_id == 0xFFFFFFFFu || h.v != 0.0f || h.inst_id >= tp.ninst) return;
+    const InstanceDev& I = tp.inst[h.inst_id];
+    if (I.blas.mode != kPhantom || !I.blas.slot_of || h.prim_id >= I.blas.nprims) return;
+    f3 o, d;
+    float tmin, tmax;
+    src.get(i, o, d, tmin, tmax);
+    if (!I.identity) {
+        o = xform_pt(I.inv, o);
+        d = xform_dir(I.inv, d);
+    }
+    const uint32_t slot = I.blas.slot_of[h.prim_id];
+    if (slot >= I.blas.nprims) return;
+    PhantomHit ph;
+    if (!prhi_ff(o, d, I.blas.prims + size_t(slot) * 4, int(I.blas.radius_mode), I.blas.radius, ph)) return;
+    f3 nn{ph.ncx, ph.ncy, ph.nsz};
+    if (!I.identity) {
+        const float* x = I.xf;
+        nn = nrm3(f3{x[0] * nn.x + x[1] * nn.y + x[2] * nn.z, x[4] * nn.x + x[5] * nn.y + x[6] * nn.z,
+                     x[8] * nn.x + x[9] * nn.y + x[10] * nn.z});
+    }
+    h.t = ph.t;
+    h.u = ph.u;
+    h.n[0] = nn.x;
+    h.n[1] = nn.y;
+    h.n[2] = nn.z;
+    hits[i] = h;
+}
+
 // Shadow queue entries are ids pixel * samples + light sample.
 struct SrcShadow {
     Camera cam;
@@ -1633,6 +1670,8 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                    tp.cand_hits, tp.cand_keys);
                 hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
             }
+            if (rp.accurate)
+                hipLaunchKernelGGL(k_accurate_hits<SrcCamera>, grid, block, 0, st, tp, SrcCamera{rp.cam}, rp.hitrec, npix);
             hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kAppendBlock - 1) / kAppendBlock),
                                dim3(kAppendBlock), 0, st, rp);
             bool shadow_timed = false;
