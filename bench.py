@@ -165,6 +165,21 @@ def main():
     kavg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kstats.items()}
     log(f"[bench] kernel avg ms: {kavg}")
 
+    # the same frame with every primary phantom hit re-evaluated in double-word FP32
+    acc_flags = flags | V.RENDER_ACCURATE_NORMALS
+    for _ in range(args.warmup):
+        ctx.render(tlas, cam, W, H, image.data_ptr(), flags=acc_flags, stream=sp)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        ctx.render(tlas, cam, W, H, image.data_ptr(), flags=acc_flags, stream=sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_acc = e0.elapsed_time(e1) / args.steps
+    accurate = {"ms_per_step": round(ms_acc, 4), "value": round(rays / (ms_acc * 1e-3) / 1e6, 3), "unit": "Mrays/s",
+                "frame_time_vs_plain": round(ms_acc / ms, 3)}
+    log(f"[bench] accurate normals: {ms_acc:.3f} ms/frame, {accurate['value']:.1f} Mrays/s")
+
     roofline = None
     if not args.no_roofline:
         roofline = roofline_block(ctx, tlas, cam, W, H, args.polish, kavg, sp, torch, np, V)
@@ -218,6 +233,7 @@ def main():
             "autotune": tuning,
         },
         "roofline": roofline,
+        "accurate_normals": accurate,
         "cpu_baseline": cpu,
         "neural": neural,
         "geometry_modes": modes,
@@ -762,8 +778,46 @@ def cpu_baseline(args, ctx, tlas, cam, W, H, flags, sp, torch, np, V):
            "sample": f"every {stride}th pixel of the {W}x{H} frame ({npix} pixels) traced {reps} times: "
                      f"{nr} primary+shadow rays in {secs:.1f} s; scene generation + BVH2 build "
                      f"({t_build:.1f} s) excluded"}
-    parity = {"pixels": int(npix), "hit_pixels": hits, "primid_agreement": agree}
+    parity = {"pixels": int(npix), "hit_pixels": hits, "primid_agreement": agree,
+              "accuracy_vs_fp64": accuracy_sample(ctx, tlas, cam, W, H, args.polish, sp, torch, np, V, O, curves)}
     return cpu, parity
+
+
+def accuracy_sample(ctx, tlas, cam, W, H, polish, sp, torch, np, V, O, curves, nrays=6000):
+    """Primary hits of the frame against the FP64 evaluation of the same intersector (the
+    oracle's Prhi in double): shading-normal angle and relative t error of the GPU's FP32 hits
+    and of its double-word re-run (VKHRT_TRACE_ACCURATE_NORMALS)."""
+    vi = np.array(cam.view_inverse, np.float32)
+    pi = np.array(cam.proj_inverse, np.float32)
+    rays = O.raygen(vi, pi, W, H)[np.linspace(0, W * H - 1, nrays).astype(np.int64)]
+    d_rays = torch.from_numpy(rays.reshape(-1).copy()).cuda()
+    base = V.TRACE_FP64_POLISH if polish else 0
+    out = {}
+    hits = {}
+    for name, fl in (("fp32", base), ("accurate", base | V.TRACE_ACCURATE_NORMALS)):
+        h = torch.zeros(nrays * 8, dtype=torch.float32, device="cuda")
+        ctx.trace_rays(tlas, d_rays.data_ptr(), nrays, h.data_ptr(), fl, sp)
+        torch.cuda.synchronize()
+        hits[name] = h.cpu().numpy().reshape(nrays, 8)
+    prim = hits["fp32"][:, 1].view(np.uint32)
+    for name, hh in hits.items():
+        ang, rel = [], []
+        for i in np.nonzero(prim != 0xFFFFFFFF)[0]:
+            r64 = O.prhi_f64(rays[i], curves[prim[i]], per_point=True)
+            if r64[0] <= 0:
+                continue
+            n = hh[i, 4:7].astype(np.float64)
+            c = abs(n @ r64[5:8]) / (np.linalg.norm(n) * np.linalg.norm(r64[5:8]))
+            ang.append(math.acos(min(1.0, c)))
+            rel.append(abs(float(hh[i, 0]) - r64[0]) / r64[0])
+        ang, rel = np.array(ang), np.array(rel)
+        out[name] = {"hits": int(len(ang)), "normal_rad_median": float(np.median(ang)),
+                     "normal_rad_p99": float(np.percentile(ang, 99)), "normal_rad_max": float(ang.max()),
+                     "t_rel_p99": float(np.percentile(rel, 99)), "t_rel_max": float(rel.max())}
+        log(f"[bench] vs FP64 ({name}): normal median {out[name]['normal_rad_median']:.2g} p99 "
+            f"{out[name]['normal_rad_p99']:.2g} max {out[name]['normal_rad_max']:.2g} rad; t rel p99 "
+            f"{out[name]['t_rel_p99']:.2g} max {out[name]['t_rel_max']:.2g} ({len(ang)} hits)")
+    return out
 
 
 if __name__ == "__main__":
