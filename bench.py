@@ -774,7 +774,13 @@ def cpu_baseline(args, ctx, tlas, cam, W, H, flags, sp, torch, np, V):
     sel = np.arange(0, W * H, stride)
     agree = float((g[sel] == ids.reshape(-1)[sel]).mean())
     hits = int((ids.reshape(-1)[sel] != 0xFFFFFFFF).sum())
-    cpu = {"value": round(value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    cpu = {"value": round(value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port", "cpu_model": cpu_model,
            "sample": f"every {stride}th pixel of the {W}x{H} frame ({npix} pixels) traced {reps} times: "
                      f"{nr} primary+shadow rays in {secs:.1f} s; scene generation + BVH2 build "
                      f"({t_build:.1f} s) excluded"}
