@@ -9,7 +9,7 @@
 //    library of Hida, Li and Bailey, "sloppy" division); sqrt: s = sqrt(xh) corrected by
 //    (x - s^2) / (2 s) with s^2 formed exactly.
 // The device code (vkhrt_amd/csrc/ffloat.hpp) runs the same algorithms, so both round alike.
-// Pinned against double arithmetic in tests/test_oracle_math.py.
+// Pinned against exact rational arithmetic in tests/test_ff_oracle.py.
 // The type and its functions live in orc::dw and are found by argument-dependent lookup, so
 // an unqualified sqrt / fabs on a float elsewhere in orc still resolves to the float function.
 #pragma once

@@ -443,3 +443,86 @@ def test_cpp_model_loader_mirror(tmp_path):
     r = subprocess.run([cpp_test_binary(), "assetinfo", str(tmp_path / "none.glb")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 0 and r.stdout.startswith("error -1 ") and "cannot read" in r.stdout
+
+
+def _fuzz_cases(seed, n):
+    """Valid files (a GLB and a .gltf with a data: URI, both with nodes, materials and strided
+    attributes) with random byte edits: flips, overwrites with JSON punctuation or digits,
+    truncations and duplications."""
+    rng = np.random.default_rng(seed)
+    pos, idx = _strands(rng, 6, 3)
+    g = GW.Gltf()
+    m = g.material(base_color=[0.5, 0.5, 0.5, 1.0])
+    mesh = g.mesh([{"positions": pos, "indices": idx, "mode": 1, "material": m},
+                   {"positions": pos[:9], "normals": pos[:9], "uv": np.zeros((9, 2), np.float32),
+                    "indices": np.arange(9, dtype=np.uint16), "interleave": True}])
+    child = g.node(mesh=mesh, translation=[1.0, 2.0, 3.0], rotation=[0.0, 0.0, 0.6, 0.8])
+    g.scene([g.node(children=[child], scale=[2.0, 2.0, 2.0]), g.node(mesh=mesh)])
+    bases = [bytearray(g.glb()), bytearray(g.gltf_text(None).encode())]
+    punct = b'{}[]":,0123456789-.eE '
+    for _ in range(n):
+        b = bytearray(bases[int(rng.integers(2))])
+        for _ in range(int(rng.integers(1, 6))):
+            k = int(rng.integers(5))
+            i = int(rng.integers(len(b)))
+            if k == 0:
+                b[i] ^= 1 << int(rng.integers(8))
+            elif k == 1:
+                b[i] = punct[int(rng.integers(len(punct)))]
+            elif k == 2:
+                b = b[:i]
+            elif k == 3:
+                j = min(len(b), i + int(rng.integers(1, 16)))
+                b[i:i] = b[i:j]
+            else:
+                b[i:i + 4] = int(rng.integers(0, 2 ** 32)).to_bytes(4, "little")
+            if not b:
+                break
+        yield bytes(b)
+
+
+def test_mutated_files_load_or_fail_cleanly():
+    """Every mutated file either loads into a consistent asset or fails with a status and a
+    message; the reader never crashes or reads outside the data (a crash ends the test run)."""
+    loaded = failed = 0
+    for data in _fuzz_cases(17, 3000):
+        try:
+            a = V.Asset(data=data)
+        except V.VkhrtError as e:
+            assert e.code in (-1, -4) and str(e)
+            failed += 1
+            continue
+        info = a.info()
+        for i in range(info["mesh_count"]):
+            m = a.mesh(i)
+            nv = m["vertices"].shape[0]
+            assert m["indices"].size == 0 or int(m["indices"].max()) < nv
+            assert m["indices"].size % (2 if m["primitive"] == V.PRIMITIVE_LINES else 3) == 0
+        for i in range(info["node_count"]):
+            n = a.node(i)
+            assert -1 <= n["parent"] < i
+            assert all(0 <= k < info["mesh_count"] for k in n["meshes"])
+        assert info["instance_count"] == sum(len(a.node(i)["meshes"]) for i in range(info["node_count"]))
+        loaded += 1
+    print(f"fuzz: {loaded} loaded, {failed} rejected")
+    assert loaded > 100 and failed > 100
+
+
+def test_mutated_files_under_address_sanitizer(tmp_path):
+    """The same mutated files through the reader built with AddressSanitizer and
+    UndefinedBehaviorSanitizer (tests/cpp/asset_fuzz, host code only): no invalid access."""
+    import os
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = os.path.join(here, "cpp", "asset_fuzz")
+    subprocess.run(["make", "-s", "-C", os.path.join(here, "cpp"), exe], check=True)
+    paths = []
+    for k, data in enumerate(_fuzz_cases(23, 1500)):
+        p = tmp_path / f"m{k}.glb"
+        p.write_bytes(data)
+        paths.append(str(p))
+    r = subprocess.run([exe] + paths, capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="print_stacktrace=1"))
+    assert r.returncode == 0, r.stderr[-3000:]
+    loaded, rejected = int(r.stdout.split()[0]), int(r.stdout.split()[2])
+    assert loaded + rejected == len(paths) and loaded > 50
