@@ -50,7 +50,9 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=1)
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--refill-min", type=int, default=16)
-    ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles")
+    ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles")
+    ap.add_argument("--xcd-chunk", type=int, default=4096,
+                    help="rays handed out per XCD in runs of this many (0: one work counter)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -91,6 +93,7 @@ def main():
     ctx.set_option(V.OPTION_PERSISTENT, args.persistent)
     ctx.set_option(V.OPTION_REFILL_MIN, args.refill_min)
     ctx.set_option(V.OPTION_RAY_ORDER, args.ray_order)
+    ctx.set_option(V.OPTION_XCD_CHUNK, args.xcd_chunk)
 
     NS, SEG, W, H = args.strands, args.segments, args.width, args.height
     nseg_total = NS * SEG
@@ -128,6 +131,7 @@ def main():
         binfo = blas.info()
         tuning = {"bvh_width": tb["bvh_width"], "leaf_size": tb["leaf_size"], "stack_kind": tr["stack_kind"],
                   "blocks_per_cu": tr["blocks_per_cu"], "refill_min": tr["refill_min"], "ray_order": tr["ray_order"],
+                  "xcd_chunk": tr["xcd_chunk"],
                   "trials": tb["trials"] + tr["trials"], "seconds": round(time.perf_counter() - t0, 2),
                   "start_ms": round(tb["baseline_ms"], 4), "tuned_ms": round(tr["best_ms"], 4)}
         log(f"[bench] autotuned: {tuning}")
@@ -230,6 +234,7 @@ def main():
             "stack_kind": tuning["stack_kind"] if tuning else args.stack_kind, "persistent": bool(args.persistent),
             "refill_min": tuning["refill_min"] if tuning else args.refill_min,
             "ray_order": tuning["ray_order"] if tuning else args.ray_order,
+            "xcd_chunk": tuning["xcd_chunk"] if tuning else args.xcd_chunk,
             "autotune": tuning,
         },
         "roofline": roofline,

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define VKHRT_ABI_VERSION 4
+#define VKHRT_ABI_VERSION 5
 
 typedef struct vkhrt_context vkhrt_context;
 typedef struct vkhrt_blas vkhrt_blas;
@@ -235,7 +235,12 @@ int vkhrt_abi_version(void);
 #define VKHRT_OPTION_BVH_WIDTH 7     /* children per node for later builds, 4..8 (default 8; nodes
                                        keep the 8-slot layout) */
 #define VKHRT_OPTION_RAY_ORDER 5   /* vkhrt_render work order: 0 row-major (default), 1 8x8 pixel
-                                     tiles, one tile per wavefront */
+                                     tiles, one tile per wavefront, 2 64x64 tiles of 8x8 tiles */
+#define VKHRT_OPTION_XCD_CHUNK 8   /* persistent traversal (vkhrt_render, vkhrt_render_batch,
+                                     vkhrt_trace_rays): 0 one work counter for the grid; a power
+                                     of two in 64..2^20 (default 4096): the rays are handed out in
+                                     runs of this many, run k by the counter of XCD k mod 8 (each
+                                     XCD's L2 then serves its own image regions) */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
 
 /* ---- acceleration structures */
@@ -388,8 +393,8 @@ int vkhrt_batch_stats_get(vkhrt_context* ctx, vkhrt_batch_stats* out);
  * the context. Each trial runs the workload `repeats` times (>= 1, after one warm-up) and
  * scores the median, timed with HIP events on `stream`. The workload's output buffers are
  * overwritten. Knobs: stack kind, persistent blocks per CU (waves per CU / 4), refill
- * minimum, and per workload: the render's ray order, the batch's sort flags, the BLAS's width
- * and leaf size. Results never depend on the knobs, only the time does. */
+ * minimum, XCD chunk, and per workload: the render's ray order, the batch's sort flags, the
+ * BLAS's width and leaf size. Results never depend on the knobs, only the time does. */
 typedef struct vkhrt_tuning {
     int32_t stack_kind;
     int32_t blocks_per_cu;
@@ -401,13 +406,14 @@ typedef struct vkhrt_tuning {
     uint32_t trials;
     float baseline_ms;     /* the workload with the starting settings */
     float best_ms;         /* ... with the chosen ones (<= baseline_ms) */
+    int32_t xcd_chunk;     /* VKHRT_OPTION_XCD_CHUNK */
 } vkhrt_tuning;
 /* vkhrt_render(tlas, camera, desc) as the workload; sets stack kind, blocks per CU, refill
- * minimum and ray order. */
+ * minimum, ray order and XCD chunk. */
 int vkhrt_autotune_render(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_camera* camera,
                           const vkhrt_render_desc* desc, uint32_t repeats, vkhrt_tuning* out, void* stream);
-/* vkhrt_render_batch as the workload; sets stack kind, blocks per CU and refill minimum and
- * reports the best sort flags in out->batch_flags (desc->flags' sort bits are the start). */
+/* vkhrt_render_batch as the workload; sets stack kind, blocks per CU, refill minimum and XCD
+ * chunk and reports the best sort flags in out->batch_flags (desc->flags' sort bits are the start). */
 int vkhrt_autotune_batch(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_view* views, uint32_t view_count,
                          const vkhrt_batch_desc* desc, uint32_t repeats, vkhrt_tuning* out, void* stream);
 /* Builds `blas` (with its TLAS) for each BVH width in {4, 6, 8} and leaf size in {1, 2, 3},

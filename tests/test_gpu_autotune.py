@@ -31,7 +31,8 @@ def _scene(ctx):
 
 def _check_ranges(t):
     assert 0 <= t["stack_kind"] <= 3 and 0 <= t["blocks_per_cu"] <= 32 and 1 <= t["refill_min"] <= 64
-    assert t["ray_order"] in (0, 1) and 4 <= t["bvh_width"] <= 8 and 1 <= t["leaf_size"] <= 3
+    assert t["ray_order"] in (0, 1, 2) and 4 <= t["bvh_width"] <= 8 and 1 <= t["leaf_size"] <= 3
+    assert t["xcd_chunk"] == 0 or (64 <= t["xcd_chunk"] <= 1 << 20 and t["xcd_chunk"] & (t["xcd_chunk"] - 1) == 0)
     assert 0.0 < t["best_ms"] <= t["baseline_ms"]
 
 

@@ -82,17 +82,24 @@ def _assert_same(gi, gp, ri, rp, what):
         assert np.all(a[..., 3] == 1.0)
 
 
-@pytest.mark.parametrize("polish,sort", [(0, 0), (0, 1), (1, 0), (1, 1), (1, 2), (0, 3)])
-def test_batch_matches_oracle(gpu_ctx, oracle, scene, polish, sort):
-    """sort: bit 0 material-sorted shading, bit 1 ray-sorted bounce queues."""
+@pytest.mark.parametrize("polish,sort,chunk", [(0, 0, 4096), (0, 1, 4096), (1, 0, 4096), (1, 1, 4096), (1, 2, 4096),
+                                              (0, 3, 4096), (1, 0, 0), (0, 3, 64)])
+def test_batch_matches_oracle(gpu_ctx, oracle, scene, polish, sort, chunk):
+    """sort: bit 0 material-sorted shading, bit 1 ray-sorted bounce queues; chunk: the rays
+    of each traversal launch from one work counter (0) or from per-XCD counters in runs of
+    `chunk`."""
     import scenes
     import vkhrt_amd as V
     S, tlas = scene
     views = _views(oracle, scenes.DENSE_VIEWS)
     flags = ((V.BATCH_FP64_POLISH if polish else 0) | (V.BATCH_SORT_MATERIALS if sort & 1 else 0) |
              (V.BATCH_SORT_RAYS if sort & 2 else 0))
-    print(f"[batch parity] polish {polish} sort {sort}: 3 views, 3 bounces", flush=True)
-    gi, gp, (paths, nb, ns, _) = _gpu_batch(gpu_ctx, tlas, views, 3, 21, flags)
+    print(f"[batch parity] polish {polish} sort {sort} chunk {chunk}: 3 views, 3 bounces", flush=True)
+    gpu_ctx.set_option(V.OPTION_XCD_CHUNK, chunk)
+    try:
+        gi, gp, (paths, nb, ns, _) = _gpu_batch(gpu_ctx, tlas, views, 3, 21, flags)
+    finally:
+        gpu_ctx.set_option(V.OPTION_XCD_CHUNK, 4096)
     ri, rp, rnb, rns, _ = S.render_batch(views, bounces=3, seed=21, env=ENV, albedo=TINTS,
                                          flags=oracle.F_POLISH if polish else 0)
     print(f"paths {paths}: bounce rays {nb} (oracle {rnb}), shadow rays {ns} (oracle {rns})")

@@ -213,8 +213,11 @@ def test_phantom_render_fp64_polish(gpu_ctx, G, oracle, curves):
     assert np.array_equal(img[..., :3].view(np.uint32), ref_img.view(np.uint32))
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
-def test_persistent_and_plain_kernels_agree(gpu_ctx, G, oracle, curves, persistent):
+@pytest.mark.parametrize("persistent,chunk", [(0, 0), (1, 0), (1, 64), (1, 4096)])
+def test_persistent_and_plain_kernels_agree(gpu_ctx, G, oracle, curves, persistent, chunk):
+    """One thread per ray, and persistent lanes fed by one work counter or by per-XCD counters
+    in runs of 64 (313 runs, so waves steal from other XCDs' counters) or 4096 rays: every
+    hit equals the oracle's."""
     import scenes
     import vkhrt_amd as V
     rays = scenes.targeted_rays(curves, 20000, seed=3)
@@ -225,12 +228,14 @@ def test_persistent_and_plain_kernels_agree(gpu_ctx, G, oracle, curves, persiste
     tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
     gpu_ctx.set_option(V.OPTION_PERSISTENT, persistent)
     gpu_ctx.set_option(V.OPTION_REFILL_MIN, 8)
+    gpu_ctx.set_option(V.OPTION_XCD_CHUNK, chunk)
     try:
         h = G.trace(gpu_ctx, tlas, rays)
     finally:
         gpu_ctx.set_option(V.OPTION_PERSISTENT, 1)
         gpu_ctx.set_option(V.OPTION_REFILL_MIN, 16)
-    _compare_hits(h, ref, f"persistent={persistent}")
+        gpu_ctx.set_option(V.OPTION_XCD_CHUNK, 4096)
+    _compare_hits(h, ref, f"persistent={persistent} chunk={chunk}")
 
 
 @pytest.mark.parametrize("any_hit", [False, True])
@@ -274,23 +279,30 @@ def test_persistent_many_rays_per_lane(gpu_ctx, G, oracle, curves, any_hit):
         _compare_hits(pers[sample], ref, "persistent many rays")
 
 
-@pytest.mark.parametrize("polish,order", [(False, 0), (True, 0), (False, 1), (True, 1)])
-def test_persistent_render_1024(gpu_ctx, G, oracle, curves, polish, order):
-    """A 1000x1000 render (about 5 pixels per resident lane; 8x8 tiles cross the image edge)
-    with shadows, row-major or tiled work order: persistent and one-thread-per-ray kernels
-    give the same image; primitive ids equal the oracle's."""
+@pytest.mark.parametrize("polish,order,chunk", [(False, 0, 0), (True, 0, 0), (False, 1, 0), (True, 1, 0),
+                                               (False, 2, 64), (True, 2, 4096), (True, 0, 1 << 20)])
+def test_persistent_render_1024(gpu_ctx, G, oracle, curves, polish, order, chunk):
+    """A 1000x1000 render (about 5 pixels per resident lane; 8x8 and 64x64 tiles cross the
+    image edge) with shadows, row-major or tiled work order, rays from one counter or from
+    per-XCD counters in runs of `chunk` (2^20: one run, so seven XCDs take all their rays from
+    the eighth's counter): persistent and one-thread-per-ray kernels give the same image;
+    primitive ids equal the oracle's."""
     import vkhrt_amd as V
     blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
     tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
     W = H = 1000
     cam = V.camera_look_at((0.0, 50.0, 400.0), (0.0, 30.0, 0.0), aspect=1.0)
     flags = V.RENDER_SHADOWS | (V.RENDER_FP64_POLISH if polish else 0)
-    print(f"[persistent_render_1024] polish={polish} order={order}", flush=True)
+    print(f"[persistent_render_1024] polish={polish} order={order} chunk={chunk}", flush=True)
     gpu_ctx.set_option(V.OPTION_RAY_ORDER, order)
+    gpu_ctx.set_option(V.OPTION_XCD_CHUNK, chunk)
     try:
+        # another view first: a pixel the frame under test failed to trace would keep its hit
+        G.render(gpu_ctx, tlas, V.camera_look_at((60.0, 90.0, 300.0), (0.0, 30.0, 0.0), aspect=1.0), W, H, flags)
         img, ids = G.render(gpu_ctx, tlas, cam, W, H, flags)
     finally:
         gpu_ctx.set_option(V.OPTION_RAY_ORDER, 0)
+        gpu_ctx.set_option(V.OPTION_XCD_CHUNK, 4096)
     if not polish:
         gpu_ctx.set_option(V.OPTION_PERSISTENT, 0)
         try:
@@ -309,6 +321,22 @@ def test_persistent_render_1024(gpu_ctx, G, oracle, curves, polish, order):
     assert np.array_equal(ids[stride_mask], rids[stride_mask])
     assert G.psnr(img[..., :3][stride_mask], rimg[stride_mask]) >= 50.0
     assert sel.sum() > 0
+
+
+def test_work_order_options_checked(gpu_ctx):
+    """Ray order 0..2 and XCD chunks of 0 or a power of two in 64..2^20 are accepted; other
+    values fail with INVALID_ARGUMENT."""
+    import vkhrt_amd as V
+    for v in (0, 1, 2):
+        gpu_ctx.set_option(V.OPTION_RAY_ORDER, v)
+    for v in (0, 64, 4096, 1 << 20):
+        gpu_ctx.set_option(V.OPTION_XCD_CHUNK, v)
+    for opt, v in ((V.OPTION_RAY_ORDER, 3), (V.OPTION_RAY_ORDER, -1), (V.OPTION_XCD_CHUNK, 32),
+                   (V.OPTION_XCD_CHUNK, 100), (V.OPTION_XCD_CHUNK, 1 << 21), (V.OPTION_XCD_CHUNK, -64)):
+        with pytest.raises(V.VkhrtError):
+            gpu_ctx.set_option(opt, v)
+    gpu_ctx.set_option(V.OPTION_RAY_ORDER, 0)
+    gpu_ctx.set_option(V.OPTION_XCD_CHUNK, 4096)
 
 
 @pytest.mark.gpu

@@ -35,7 +35,10 @@ def test_library_exports_every_declared_symbol(vk):
     missing = [s for s in syms if not hasattr(L, s)]
     assert not missing, missing
     assert sorted(vk.EXPORTS) == syms
-    assert vk.lib().vkhrt_abi_version() >= 1
+    assert vk.lib().vkhrt_abi_version() == vk.ABI_VERSION
+    hdr = open(os.path.join(ROOT, "include", "vkhrt.h")).read()
+    assert int(re.search(r"#define VKHRT_ABI_VERSION (\d+)", hdr).group(1)) == vk.ABI_VERSION
+    assert ctypes.sizeof(vk.Tuning) == 11 * 4  # vkhrt_tuning: eleven 4-byte fields
 
 
 def test_null_handles_are_rejected(vk):

@@ -37,6 +37,8 @@ RENDER_ACCURATE_NORMALS = 1 << 5
 
 SHADOW_MLP_PARAMS = 18753
 
+ABI_VERSION = 5  # VKHRT_ABI_VERSION of include/vkhrt.h: the struct layouts below
+
 OPTION_STACK_KIND = 1
 OPTION_LEAF_SIZE = 2
 OPTION_PERSISTENT = 3
@@ -44,6 +46,7 @@ OPTION_REFILL_MIN = 4
 OPTION_RAY_ORDER = 5
 OPTION_BLOCKS_PER_CU = 6
 OPTION_BVH_WIDTH = 7
+OPTION_XCD_CHUNK = 8
 
 BATCH_MAX_BOUNCES = 8
 BATCH_MAX_MATERIALS = 16
@@ -158,7 +161,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("stack_kind", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("refill_min", ctypes.c_int32),
                 ("ray_order", ctypes.c_int32), ("bvh_width", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
                 ("batch_flags", ctypes.c_uint32), ("trials", ctypes.c_uint32), ("baseline_ms", ctypes.c_float),
-                ("best_ms", ctypes.c_float)]
+                ("best_ms", ctypes.c_float), ("xcd_chunk", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -297,6 +300,9 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
+    if L.vkhrt_abi_version() != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI {L.vkhrt_abi_version()}, this module expects {ABI_VERSION}: "
+                          "rebuild it with python vkhrt_amd/build.py")
     _lib = L
     return L
 

@@ -39,7 +39,8 @@ struct vkhrt_context {
     int bvh_width = 8;
     bool persistent = true;
     uint32_t refill_min = 16;
-    uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles
+    uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles
+    uint32_t xcd_chunk = 4096;  // render traversal: rays handed out per XCD in runs of this many (0: one counter)
     uint32_t blocks_per_cu = 0;  // persistent grids: 256-thread blocks per CU (0 = occupancy limit)
     bool profiling = false;
     std::vector<PendingTiming> pending;
@@ -163,9 +164,10 @@ int stack_kind_for(const vkhrt_context* c, const vkhrt_tlas* t) {
 // Work items of a frame and the constants of the work item -> pixel division.
 void set_work_order(vk::Camera& cm, uint32_t order) {
     cm.order = order;
-    cm.tiles_x = (cm.width + 7u) / 8u;
-    const uint32_t tiles_y = (cm.height + 7u) / 8u;
-    cm.work = order ? cm.tiles_x * tiles_y * 64u : cm.width * cm.height;
+    const uint32_t side = order == 2 ? 64u : 8u;
+    cm.tiles_x = (cm.width + side - 1u) / side;
+    const uint32_t tiles_y = (cm.height + side - 1u) / side;
+    cm.work = order ? cm.tiles_x * tiles_y * side * side : cm.width * cm.height;
     cm.inv_w = cm.width ? 1.0f / float(cm.width) : 0.0f;
     cm.inv_tiles_x = cm.tiles_x ? 1.0f / float(cm.tiles_x) : 0.0f;
 }
@@ -492,7 +494,7 @@ int vkhrt_context_create(int device, vkhrt_context** out) {
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->status, 16);
-    if (e == hipSuccess) e = hipMalloc(&c->counters, 64);
+    if (e == hipSuccess) e = hipMalloc(&c->counters, vk::kRenderCounterWords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(c->status, 0, 16);
     if (e != hipSuccess) {
         vkhrt_context_destroy(c);
@@ -575,13 +577,19 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
         return VKHRT_SUCCESS;
     }
     if (option == VKHRT_OPTION_RAY_ORDER) {
-        if (value < 0 || value > 1) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "ray order must be 0 or 1");
+        if (value < 0 || value > 2) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "ray order must be 0, 1 or 2");
         c->ray_order = uint32_t(value);
         return VKHRT_SUCCESS;
     }
     if (option == VKHRT_OPTION_BLOCKS_PER_CU) {
         if (value < 0 || value > 32) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "blocks per CU must be 0..32");
         c->blocks_per_cu = uint32_t(value);
+        return VKHRT_SUCCESS;
+    }
+    if (option == VKHRT_OPTION_XCD_CHUNK) {
+        if (value != 0 && (value < 64 || value > (1 << 20) || (value & (value - 1)) != 0))
+            return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "XCD chunk must be 0 or a power of two in 64..2^20");
+        c->xcd_chunk = uint32_t(value);
         return VKHRT_SUCCESS;
     }
     if (option == VKHRT_OPTION_BVH_WIDTH) {
@@ -1063,6 +1071,8 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     const bool persistent = c->persistent && (t->polish_ok || !is_phantom(t));
     vk::TraceParams tp{t->dev, t->count, flags, c->status, persistent ? c->counters + 3 : nullptr, c->refill_min};
     tp.blocks_per_cu = c->blocks_per_cu;
+    tp.xcd_chunk = c->xcd_chunk;
+    tp.xcd_work = c->counters + vk::kXcdCounters + 16 * vk::kXcdStride;
     const bool polish = (flags & VKHRT_TRACE_FP64_POLISH) != 0;
     if (polish) {
         if (!t->polish_ok)
@@ -1138,6 +1148,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     rp.hitbuf = c->hitbuf;
     rp.queue = c->queue;
     rp.counters = c->counters;
+    rp.tp.xcd_chunk = c->xcd_chunk;
     rp.light_vis = const_cast<float*>(d->light_visibility);
     rp.hitrec = c->hitrec;
     rp.vis = c->vis;
@@ -1298,6 +1309,7 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     bp.tp = vk::TraceParams{t->dev, t->count, polish ? uint32_t(vk::kTraceFp64Polish) : 0u, c->status,
                             c->b_counters, c->refill_min};
     bp.tp.blocks_per_cu = c->blocks_per_cu;
+    bp.tp.xcd_chunk = c->xcd_chunk;
     if (polish) {
         bp.tp.defer = c->b_defer;
         bp.tp.defer_cap = uint32_t(std::max<size_t>(c->b_cap, 65536));
@@ -1353,7 +1365,8 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     }
     if (d->flags & VKHRT_BATCH_GRAPH) {
         const std::vector<uint64_t> key = {t->serial, uint64_t(variant), uint64_t(stack), total,         d->bounces,
-                                           bp.sort,   polish,            c->refill_min,   c->blocks_per_cu, c->batch_gen};
+                                           bp.sort,   polish,            c->refill_min,   c->blocks_per_cu, c->batch_gen,
+                                           c->xcd_chunk};
         if (!c->graph_exec || c->graph_key != key) {
             if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
             if (c->graph) hipGraphDestroy(c->graph);
@@ -1500,6 +1513,7 @@ void fill_tuning(const vkhrt_context* c, vkhrt_tuning* out) {
     out->blocks_per_cu = int32_t(c->blocks_per_cu);
     out->refill_min = int32_t(c->refill_min);
     out->ray_order = int32_t(c->ray_order);
+    out->xcd_chunk = int32_t(c->xcd_chunk);
     out->bvh_width = c->bvh_width;
     out->leaf_size = c->leaf_size;
 }
@@ -1523,15 +1537,18 @@ int vkhrt_autotune_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_cam
     *out = vkhrt_tuning{};
     hipStream_t st = pick(c, stream);
     int stack = c->stack_kind, bpc = int(c->blocks_per_cu), refill = int(c->refill_min), order = int(c->ray_order);
+    int chunk = int(c->xcd_chunk);
     std::vector<Knob> knobs = {{&stack, stack_candidates(t)},
                                {&bpc, {0, 6, 4, 3, 2}},
                                {&refill, {4, 8, 16, 32, 48}},
-                               {&order, {0, 1}}};
+                               {&order, {0, 1, 2}},
+                               {&chunk, {0, 512, 4096, 16384}}};
     auto apply = [&] {
         c->stack_kind = stack;
         c->blocks_per_cu = uint32_t(bpc);
         c->refill_min = uint32_t(refill);
         c->ray_order = uint32_t(order);
+        c->xcd_chunk = uint32_t(chunk);
     };
     const int rc = coordinate_descent(c, st, repeats, knobs, apply, [&] { return vkhrt_render(c, t, cam, d, stream); },
                                       out);
@@ -1548,15 +1565,17 @@ int vkhrt_autotune_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view
     const uint32_t sort_bits = VKHRT_BATCH_SORT_MATERIALS | VKHRT_BATCH_SORT_RAYS;
     vkhrt_batch_desc dd = *d;
     int stack = c->stack_kind, bpc = int(c->blocks_per_cu), refill = int(c->refill_min);
-    int sort = int(d->flags & sort_bits);
+    int sort = int(d->flags & sort_bits), chunk = int(c->xcd_chunk);
     std::vector<Knob> knobs = {{&stack, stack_candidates(t)},
                                {&bpc, {0, 6, 4, 3, 2}},
                                {&refill, {4, 8, 16, 32, 48}},
-                               {&sort, {0, int(VKHRT_BATCH_SORT_RAYS), int(VKHRT_BATCH_SORT_MATERIALS), int(sort_bits)}}};
+                               {&sort, {0, int(VKHRT_BATCH_SORT_RAYS), int(VKHRT_BATCH_SORT_MATERIALS), int(sort_bits)}},
+                               {&chunk, {0, 512, 4096, 16384}}};
     auto apply = [&] {
         c->stack_kind = stack;
         c->blocks_per_cu = uint32_t(bpc);
         c->refill_min = uint32_t(refill);
+        c->xcd_chunk = uint32_t(chunk);
         dd.flags = (d->flags & ~sort_bits) | uint32_t(sort);
     };
     const int rc = coordinate_descent(

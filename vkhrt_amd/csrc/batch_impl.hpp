@@ -210,7 +210,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_batch_trace(BatchParams bp, uin
             finish_hit(bp.tp, o, d, best, h);
             bp.hits[qi] = h;
             if (G::kPolish) bp.tp.keys[qi] = best_key(best);
-        });
+        },
+        bp.counters + batch_xcd(b, 0));
 }
 
 // Mixed-mode TLASes (per-instance geometry dispatch): one lane per queue slot, the queue length
@@ -391,7 +392,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_batch_shadow(BatchParams bp, ui
         },
         [&](uint32_t qi, f3, f3, const Best& best) {
             bp.nee[bp.squeue[qi]].w = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
-        });
+        },
+        bp.counters + batch_xcd(b, 1));
 }
 
 template <class G, int STACK>

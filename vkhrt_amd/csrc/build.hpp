@@ -38,19 +38,30 @@ struct TraceParams {
     HitRec* cand_hits;         // per candidate
     unsigned long long* cand_keys;
     uint32_t blocks_per_cu;    // persistent grids: 256-thread blocks per CU, 0 = as many as fit
+    uint32_t xcd_chunk;        // persistent grids: 0 one work counter; a power of two: rays handed
+                               // out per XCD in runs of this many (each launch brings 8 counters)
+    uint32_t* xcd_work;        // the 8 per-XCD counters of vkhrt_trace_rays's persistent launch
 };
 
 struct Camera {
     float view_inv[16];
     float proj_inv[16];
     uint32_t width, height;
-    uint32_t order;    // work item -> pixel: 0 row-major, 1 8x8 tiles (one tile per wavefront)
-    uint32_t tiles_x;  // ceil(width / 8)
-    uint32_t work;     // work items of a frame (order 1: whole tiles, some outside the image)
+    uint32_t order;    // work item -> pixel: 0 row-major, 1 8x8 tiles (one tile per wavefront),
+                       // 2 64x64 tiles made of 8x8 tiles in row-major order
+    uint32_t tiles_x;  // tiles per image row: ceil(width / 8) for order 1, ceil(width / 64) for 2
+    uint32_t work;     // work items of a frame (orders 1, 2: whole tiles, some outside the image)
     float inv_w, inv_tiles_x;  // 1 / width, 1 / tiles_x for the index division
 };
 
 struct MlpDev;
+
+// Context work counters (RenderParams::counters): words [0, 8) per frame; from kXcdCounters
+// the per-XCD counters of the render's primary and shadow traversal launches and of
+// vkhrt_trace_rays's launch, 8 each, kXcdStride words (one 128-byte line) apart.
+constexpr uint32_t kXcdCounters = 64;
+constexpr uint32_t kXcdStride = 32;
+constexpr uint32_t kRenderCounterWords = kXcdCounters + 24 * kXcdStride;
 
 // Equirectangular RGBA32F environment map of the miss shader (miss.rmiss), sampled bilinearly
 // with repeat addressing like the reference's default sampler; texels == null: misses are black.
@@ -80,7 +91,8 @@ struct RenderParams {
     uint32_t* prim_out;   // optional per-pixel primitive id
     float4* hitbuf;       // per-pixel (n.xyz, t) scratch
     uint32_t* queue;      // shadow queue (pixel indices)
-    uint32_t* counters;   // [0] queue length
+    uint32_t* counters;   // [0] queue length; from [kXcdCounters]: 8 + 8 per-XCD work counters of the
+                          // primary and shadow traversal launches, kXcdStride words apart
     float* light_vis;     // optional per-pixel visibility override (neural form), may be null
     // deferred-polish pipeline scratch
     HitRec* hitrec;       // per-pixel primary hit
@@ -127,7 +139,13 @@ enum BatchCounter {
     kBcDeferShadow = 5,  // ... of the shadow trace
 };
 constexpr uint32_t kBatchMatBase = 128;
-constexpr uint32_t kBatchCounterWords = kBatchMatBase + 64 * (kBatchMaxBounces + 1);
+// then the per-XCD work counters of the closest-hit (s = 0) and shadow (s = 1) traces of depth b:
+// 8 counters kXcdStride words apart from batch_xcd(b, s)
+constexpr uint32_t kBatchXcdBase = kBatchMatBase + 64 * (kBatchMaxBounces + 1);
+constexpr uint32_t kBatchCounterWords = kBatchXcdBase + (kBatchMaxBounces + 1) * 2 * 8 * kXcdStride;
+__host__ __device__ constexpr uint32_t batch_xcd(uint32_t b, uint32_t s) {
+    return kBatchXcdBase + (2 * b + s) * 8 * kXcdStride;
+}
 
 // One view of a batch (device table): its camera (width/height set, row-major work order),
 // its first path index and its output buffers.

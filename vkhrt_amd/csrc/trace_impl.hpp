@@ -605,10 +605,18 @@ __device__ __forceinline__ bool work_pixel(const Camera& c, uint32_t r, uint32_t
         x = r - y * c.width;
         return y < c.height;
     }
-    const uint32_t tile = r >> 6, within = r & 63u;
-    const uint32_t ty = tile < (1u << 24) ? udiv_small(tile, c.tiles_x, c.inv_tiles_x) : tile / c.tiles_x;
-    x = (tile - ty * c.tiles_x) * 8u + (within & 7u);
-    y = ty * 8u + (within >> 3);
+    if (c.order == 1) {
+        const uint32_t tile = r >> 6, within = r & 63u;
+        const uint32_t ty = tile < (1u << 24) ? udiv_small(tile, c.tiles_x, c.inv_tiles_x) : tile / c.tiles_x;
+        x = (tile - ty * c.tiles_x) * 8u + (within & 7u);
+        y = ty * 8u + (within >> 3);
+        return x < c.width && y < c.height;
+    }
+    // 64x64 tiles (4096 items) in row-major order, each 64 8x8 tiles in row-major order
+    const uint32_t big = r >> 12, within = r & 4095u, small = within >> 6;
+    const uint32_t by = big < (1u << 24) ? udiv_small(big, c.tiles_x, c.inv_tiles_x) : big / c.tiles_x;
+    x = (big - by * c.tiles_x) * 64u + (small & 7u) * 8u + (within & 7u);
+    y = by * 64u + (small >> 3) * 8u + ((within >> 3) & 7u);
     return x < c.width && y < c.height;
 }
 
@@ -758,6 +766,72 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow(RenderParams rp) 
 }
 
 // ------------------------------------------------------------------ persistent lanes
+// Hands work items [0, total) to the waves of a persistent grid. Without chunks one counter
+// serves the grid. With chunks (a power of two) the items are cut into runs of `chunk`
+// consecutive items; run k belongs to XCD k mod 8 and is handed out by that XCD's own counter
+// (xcd[kXcdStride * (k mod 8)], one 128-byte line each), so the rays of one image region meet
+// the BVH nodes they share in a single XCD's L2 (each of the 8 XCDs has its own). A wave whose
+// XCD's counter has run dry moves on to the next XCD's counter, round the eight, so no XCD
+// idles while another has rays left. The claim that empties a counter marks it in the block's
+// `dry` mask (LDS), and the block's other waves skip it without an atomic: counter atomics to
+// one line are serialized, and a grid of 4096 waves probing 7 empty counters each would cost
+// more than the whole tail. All members are wave-uniform.
+struct WorkClaim {
+    uint32_t* counter;
+    uint32_t* xcd;  // null: one counter
+    uint32_t* dry;  // LDS, bit c: counter c is empty (chunked only)
+    uint32_t total, shift;
+    uint32_t cx, left;  // counter in use; XCD counters not yet found dry
+    bool exhausted;     // no item left anywhere
+
+    __device__ __forceinline__ WorkClaim(uint32_t* c, uint32_t* x, uint32_t chunk, uint32_t n, uint32_t* d)
+        : counter(c), xcd(chunk ? x : nullptr), dry(d), total(n), shift(chunk ? uint32_t(__ffs(chunk) - 1) : 0u),
+          cx(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u),  // hwreg(HW_REG_XCC_ID, 0, 4)
+          left(8u), exhausted(false) {}
+
+    // The idle lanes of the wave (mask `idle`, the same in every lane) claim one item each;
+    // returns the calling lane's item for `rank` = its rank among the idle lanes, or a value
+    // >= total when none was left for it.
+    __device__ __forceinline__ uint32_t claim(uint64_t idle, uint32_t rank) {
+        const uint32_t n = __popcll(idle);
+        const uint32_t leader = __ffsll((unsigned long long)idle) - 1;
+        const uint32_t lane = threadIdx.x & 63u;
+        uint32_t base = 0;
+        if (!xcd) {
+            if (lane == leader) base = atomicAdd(counter, n);
+            base = __shfl(base, int(leader));
+            if (base + n >= total) exhausted = true;
+            return base + rank;
+        }
+        // skip the counters the block already found empty
+        const uint32_t known = __shfl(*reinterpret_cast<volatile uint32_t*>(dry), 0);
+        while (left != 0u && ((known >> cx) & 1u)) {
+            cx = (cx + 1u) & 7u;
+            --left;
+        }
+        if (left == 0u) {
+            exhausted = true;
+            return 0xFFFFFFFFu;
+        }
+        const uint64_t chunk = 1ull << shift;
+        const uint64_t runs = (uint64_t(total) + chunk - 1) >> shift;
+        const uint64_t own = runs > cx ? (runs - 1 - cx) / 8 + 1 : 0;  // runs of XCD cx
+        uint64_t lim = own << shift;                                   // their items
+        if (runs != 0 && (runs - 1) % 8 == cx) lim -= (runs << shift) - total;  // the last run is short
+        if (lane == leader) base = atomicAdd(xcd + kXcdStride * cx, n);
+        base = __shfl(base, int(leader));
+        const uint64_t i = uint64_t(base) + rank;
+        uint32_t r = 0xFFFFFFFFu;
+        if (i < lim) r = uint32_t(((((i >> shift) << 3) + cx) << shift) + (i & (chunk - 1)));
+        if (uint64_t(base) + n >= lim) {
+            if (lane == leader) atomicOr(dry, 1u << cx);
+            cx = (cx + 1u) & 7u;
+            if (--left == 0u) exhausted = true;
+        }
+        return r;
+    }
+};
+
 // Persistent wavefronts with per-lane refill: every iteration each active lane performs one
 // walker step; lanes whose ray finished take new work from a global counter (one atomic per
 // wave per refill), so a wavefront is not held back by its slowest ray. src(r, id, o, d, tmin,
@@ -765,8 +839,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow(RenderParams rp) 
 // entries use); snk(id, o, d, best) consumes the result (called by the lanes that finish in
 // the same iteration, so it may use wave ballots).
 template <class G, bool ANY, int STACK, class Src, class Snk>
-__device__ __forceinline__ void persistent_generic(const TraceParams& tp, uint32_t total, uint32_t* counter,
-                                                   uint2* lds, Src&& src, Snk&& snk) {
+__device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkClaim& work, uint2* lds, Src&& src,
+                                                   Snk&& snk) {
     static_assert(!G::kMixed, "persistent kernels serve single-mode TLASes");
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -776,20 +850,13 @@ __device__ __forceinline__ void persistent_generic(const TraceParams& tp, uint32
     float tmin = 0.0f, tmax = 0.0f;
     uint32_t idx = 0, inst = 0;
     bool active = false, need_inst = false;
-    bool exhausted = false;  // wave-uniform
     for (;;) {
-        if (!exhausted) {
+        if (!work.exhausted) {
             const uint64_t idle = __ballot(!active);
             if (idle && (__popcll(idle) >= tp.refill_min || idle == __ballot(true))) {
-                const uint32_t n = __popcll(idle);
-                const uint32_t leader = __ffsll((unsigned long long)idle) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(counter, n);
-                base = __shfl(base, int(leader));
-                if (base + n >= total) exhausted = true;
+                const uint32_t r = work.claim(idle, __popcll(idle & lt));
                 if (!active) {
-                    const uint32_t r = base + __popcll(idle & lt);
-                    if (r < total) {
+                    if (r < work.total) {
                         src(r, idx, ow, dw, tmin, tmax);
                         best.t = tmax;
                         best.inst = 0xFFFFFFFFu;
@@ -946,8 +1013,8 @@ struct CWalker {
 };
 
 template <class G, bool ANY, int STACK, class Src, class Snk>
-__device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32_t total, uint32_t* counter,
-                                                   uint2* lds, Src&& src, Snk&& snk) {
+__device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkClaim& work, uint2* lds, Src&& src,
+                                                   Snk&& snk) {
     static_assert(G::kMode == kPhantom, "compacted lanes serve the phantom intersector");
     __shared__ uint32_t s_queue[(kTraceBlock / 64) * kCompactQueue];
     __shared__ unsigned long long s_key[kTraceBlock];  // best (t, instance, primitive) of the lane's ray
@@ -970,7 +1037,6 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
     uint32_t idx = 0, inst = 0, home = lane;
     int state = kIdle;
     bool ray_done = false;
-    bool exhausted = false;  // wave-uniform
     uint32_t qn = 0;         // wave-uniform queue length
     // Every wave leaves the loop: a normal frame takes ~10^4 iterations per wave; past the cap
     // the wave reports an error (status[0]) and stops, leaving its rays unwritten.
@@ -989,19 +1055,13 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
                 state = kInst;
             }
         }
-        // (b) refill idle lanes from the global counter
-        if (!exhausted) {
+        // (b) refill idle lanes from the work counters
+        if (!work.exhausted) {
             const uint64_t idle = __ballot(state == kIdle);
             if (idle && (__popcll(idle) >= tp.refill_min || idle == __ballot(true))) {
-                const uint32_t n = __popcll(idle);
-                const uint32_t leader = __ffsll((unsigned long long)idle) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(counter, n);
-                base = __shfl(base, int(leader));
-                if (base + n >= total) exhausted = true;
+                const uint32_t r = work.claim(idle, __popcll(idle & lt));
                 if (state == kIdle) {
-                    const uint32_t r = base + __popcll(idle & lt);
-                    if (r < total) {
+                    if (r < work.total) {
                         src(r, idx, ow, dw, tmin, tmax);
                         tcur = tmax;
                         s_key[threadIdx.x] =
@@ -1061,7 +1121,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
         // (c') once the counter is exhausted, idle lanes take the bottom (largest) stacked node
         // group of a walking lane and continue that ray from there, so the last long rays of
         // the frame are walked by many lanes instead of one
-        if (exhausted) {
+        if (work.exhausted) {
             const bool give = (state == kTrav || state == kHelp) && W.sp > W.sb;
             const uint64_t dm = __ballot(give);
             const uint64_t im = __ballot(state == kIdle);
@@ -1097,7 +1157,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
         // lanes that finished in (c) take new rays in the next iteration: leave only when the
         // counter is exhausted and no lane holds a ray (then no pair is queued either)
         if (!__ballot(state != kIdle)) {
-            if (exhausted) break;
+            if (work.exhausted) break;
             continue;
         }
         // (d) intersection round on up to 64 queued pairs, one per lane
@@ -1194,13 +1254,20 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, uint32
 // (the expensive, iterative test), the per-lane loop for LSS and DOTS.
 template <class G, bool ANY, int STACK, class Src, class Snk>
 __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t total, uint32_t* counter, uint2* lds,
-                                                 Src&& src, Snk&& snk) {
+                                                 Src&& src, Snk&& snk, uint32_t* xcd = nullptr) {
     // an empty queue (uniform over the grid) ends the launch without touching the counter
     if (total == 0) return;
+    const uint32_t chunk = xcd ? tp.xcd_chunk : 0u;
+    __shared__ uint32_t s_dry;
+    if (chunk) {
+        if (threadIdx.x == 0) s_dry = 0u;
+        __syncthreads();
+    }
+    WorkClaim work(counter, xcd, chunk, total, &s_dry);
     if constexpr (G::kMode == kPhantom)
-        persistent_compact<G, ANY, STACK>(tp, total, counter, lds, src, snk);
+        persistent_compact<G, ANY, STACK>(tp, work, lds, src, snk);
     else
-        persistent_generic<G, ANY, STACK>(tp, total, counter, lds, src, snk);
+        persistent_generic<G, ANY, STACK>(tp, work, lds, src, snk);
 }
 
 template <class G, bool ANY, int STACK>
@@ -1224,7 +1291,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TracePara
             finish_hit(tp, o, d, best, h);
             hits[i] = h;
             if (G::kPolish) tp.keys[i] = best_key(best);
-        });
+        },
+        tp.xcd_work);
 }
 
 // ------------------------------------------------------------------ deferred FP64 polish
@@ -1461,7 +1529,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderPara
             finish_hit(rp.tp, o, d, best, h);
             rp.hitrec[pix] = h;
             if (G::kPolish) rp.tp.keys[pix] = best_key(best);
-        });
+        },
+        rp.counters + kXcdCounters);
 }
 
 // Pass 2 (after the optional polish merge): shade misses / palette / unshadowed pixels, queue
@@ -1517,7 +1586,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
         },
         [&](uint32_t qi, f3 o, f3 d, const Best& best) {
             rp.vis[queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
-        });
+        },
+        rp.counters + kXcdCounters + 8 * kXcdStride);
 }
 
 // Shadow-ray origins (and, with an area light, directions) of the queued ids: the MLP's inputs.
@@ -1624,6 +1694,8 @@ hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t 
     if constexpr (!G::kMixed) {
         if (tp.work) {
             hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.work, 1u);
+            if (tp.xcd_chunk && tp.xcd_work)
+                hipLaunchKernelGGL(k_zero, dim3(8 * kXcdStride / 64), dim3(64), 0, st, tp.xcd_work, 8u * kXcdStride);
             if (any)
                 hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK>),
                                    persistent_grid(k_trace_rays_persistent<G, true, STACK>, tp.blocks_per_cu), block, 0, st, tp, rays, n,
@@ -1651,6 +1723,9 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
     // the shadow queue holds up to npix * samples ids
     const dim3 sgrid(uint32_t((uint64_t(npix) * rp.light.samples + kTraceBlock - 1) / kTraceBlock));
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, rp.counters, 8u);
+    if (rp.tp.xcd_chunk)
+        hipLaunchKernelGGL(k_zero, dim3(16 * kXcdStride / 64), dim3(64), 0, st, rp.counters + kXcdCounters,
+                           16u * kXcdStride);
     if constexpr (!G::kMixed) {
         if (rp.tp.work) {
             // Persistent pipeline: the traversal kernels only record hits (their sinks stay
