@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=1)
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--refill-min", type=int, default=16)
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent blocks per CU (0: occupancy limit)")
     ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles")
     ap.add_argument("--xcd-chunk", type=int, default=4096,
                     help="rays handed out per XCD in runs of this many (0: one work counter)")
@@ -94,6 +95,7 @@ def main():
     ctx.set_option(V.OPTION_REFILL_MIN, args.refill_min)
     ctx.set_option(V.OPTION_RAY_ORDER, args.ray_order)
     ctx.set_option(V.OPTION_XCD_CHUNK, args.xcd_chunk)
+    ctx.set_option(V.OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
 
     NS, SEG, W, H = args.strands, args.segments, args.width, args.height
     nseg_total = NS * SEG
@@ -233,6 +235,7 @@ def main():
                     "depth": binfo["depth"], "build_ms": round(binfo["build_ms"], 3)},
             "stack_kind": tuning["stack_kind"] if tuning else args.stack_kind, "persistent": bool(args.persistent),
             "refill_min": tuning["refill_min"] if tuning else args.refill_min,
+            "blocks_per_cu": tuning["blocks_per_cu"] if tuning else args.blocks_per_cu,
             "ray_order": tuning["ray_order"] if tuning else args.ray_order,
             "xcd_chunk": tuning["xcd_chunk"] if tuning else args.xcd_chunk,
             "autotune": tuning,
