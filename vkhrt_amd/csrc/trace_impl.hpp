@@ -1538,6 +1538,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderPara
 // Walks the pixels in the primary pass's work order, so the shadow queue keeps its coherence.
 __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams rp) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    // the primary pass's deferred candidates are merged: the shadow pass starts a new list
+    if (r == 0 && rp.tp.defer) *rp.tp.defer_count = 0u;
     uint32_t x = 0, y = 0;
     const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
     const uint32_t pix = y * rp.cam.width + x;
@@ -1722,10 +1724,9 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
     const dim3 grid((npix + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
     // the shadow queue holds up to npix * samples ids
     const dim3 sgrid(uint32_t((uint64_t(npix) * rp.light.samples + kTraceBlock - 1) / kTraceBlock));
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, rp.counters, 8u);
-    if (rp.tp.xcd_chunk)
-        hipLaunchKernelGGL(k_zero, dim3(16 * kXcdStride / 64), dim3(64), 0, st, rp.counters + kXcdCounters,
-                           16u * kXcdStride);
+    // the frame's counters, and the per-XCD counters of its two traversal launches
+    const uint32_t zero_words = rp.tp.xcd_chunk ? kXcdCounters + 16u * kXcdStride : 8u;
+    hipLaunchKernelGGL(k_zero, dim3((zero_words + 255) / 256), dim3(256), 0, st, rp.counters, zero_words);
     if constexpr (!G::kMixed) {
         if (rp.tp.work) {
             // Persistent pipeline: the traversal kernels only record hits (their sinks stay
@@ -1758,7 +1759,6 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                     // confident ones and the rest (fallback queue, counters[5]) trace it
                     const uint32_t* squeue = rp.queue;
                     const uint32_t* scount = rp.counters;
-                    if (tp.defer) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.defer_count, 1u);
                     if (ev) hipEventRecord(ev[2], st);
                     if (rp.mlp) {
                         hipLaunchKernelGGL(k_shadow_origins, sgrid, block, 0, st, rp);
