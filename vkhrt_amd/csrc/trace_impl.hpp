@@ -115,10 +115,10 @@ __device__ __forceinline__ void node_visit(const WideNode* nodes, uint32_t node,
     const uint32_t fyl = ny ? n2.z : n4.x, fyh = ny ? n2.w : n4.y;
     const uint32_t nzl = nz ? n4.z : n3.x, nzh = nz ? n4.w : n3.y;
     const uint32_t fzl = nz ? n3.x : n4.z, fzh = nz ? n3.y : n4.w;
-    hits = 0;
-    pmask = 0;
     // The slab test only has to be conservative (the quantized boxes round outward and tf is
-    // widened by 8 ulp), so it uses fused multiply-adds; branch-free mask updates.
+    // widened by 8 ulp), so it uses fused multiply-adds. Each slot contributes one bit of a
+    // slot-order hit mask; the metadata is decoded afterwards for the slots hit only.
+    uint32_t lm = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const uint32_t meta = ((s < 4 ? n1.z : n1.w) >> (8 * (s & 3))) & 0xffu;
@@ -127,10 +127,21 @@ __device__ __forceinline__ void node_visit(const WideNode* nodes, uint32_t node,
         const float t0z = fmaf(qbyte(nzl, nzh, s), sz, oz), t1z = fmaf(qbyte(fzl, fzh, s), sz, oz);
         const float tn = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tmin));
         const float tf = fminf(fminf(t1x, t1y), fminf(t1z, tcur)) * 1.00000095f;
-        const bool hit = meta != 0 && tn <= tf;
-        const bool internal = (meta & 0x80u) != 0u;
-        hits |= (hit && internal) ? (1u << (uint32_t(s) ^ oct)) : 0u;
-        pmask |= (hit && !internal) ? (((1u << (meta >> 5)) - 1u) << (meta & 31u)) : 0u;
+        lm |= (meta != 0 && tn <= tf) ? (1u << s) : 0u;
+    }
+    // internal children to octant order: bit j of `hits` is slot j ^ oct, so the lowest set
+    // bit is the nearest child; three conditional swaps apply the XOR to the bit indices
+    uint32_t x = lm & imask;
+    x = (oct & 1u) ? (((x & 0x55u) << 1) | ((x >> 1) & 0x55u)) : x;
+    x = (oct & 2u) ? (((x & 0x33u) << 2) | ((x >> 2) & 0x33u)) : x;
+    x = (oct & 4u) ? (((x & 0x0Fu) << 4) | ((x >> 4) & 0x0Fu)) : x;
+    hits = x;
+    // leaf children hit: their record ranges (count << 5 | first in the metadata byte)
+    pmask = 0;
+    for (uint32_t lf = lm & ~imask; lf; lf &= lf - 1u) {
+        const uint32_t s = uint32_t(__ffs(lf) - 1);
+        const uint32_t meta = ((s < 4u ? n1.z : n1.w) >> (8u * (s & 3u))) & 0xffu;
+        pmask |= ((1u << (meta >> 5)) - 1u) << (meta & 31u);
     }
 }
 
