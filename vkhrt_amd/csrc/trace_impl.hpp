@@ -777,6 +777,19 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow(RenderParams rp) 
 }
 
 // ------------------------------------------------------------------ persistent lanes
+// A copy of a value every lane of the wave reads alike, word by word through readfirstlane,
+// so that it is held in scalar registers.
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    T out;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&v);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return out;
+}
+
 // Hands work items [0, total) to the waves of a persistent grid. Without chunks one counter
 // serves the grid. With chunks (a power of two) the items are cut into runs of `chunk`
 // consecutive items; run k belongs to XCD k mod 8 and is handed out by that XCD's own counter
@@ -849,9 +862,11 @@ struct WorkClaim {
 // tmax) produces work item r and names its ray `id` (the index results and deferred-polish
 // entries use); snk(id, o, d, best) consumes the result (called by the lanes that finish in
 // the same iteration, so it may use wave ballots).
-template <class G, bool ANY, int STACK, class Src, class Snk>
+template <class G, bool ANY, int STACK, bool ONE, class Src, class Snk>
 __device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkClaim& work, uint2* lds, Src&& src,
                                                    Snk&& snk) {
+    BlasDev b1{};  // ONE: as in persistent_compact
+    if constexpr (ONE) b1 = wave_uniform(tp.inst[0].blas);
     static_assert(!G::kMixed, "persistent kernels serve single-mode TLASes");
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -905,7 +920,7 @@ __device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkCl
                     need_inst = false;
                 }
             } else {
-                const int r = W.step(tp.inst[inst].blas, tmin, inst, best, lds, nullptr, tp, idx);
+                const int r = W.step(ONE ? b1 : tp.inst[inst].blas, tmin, inst, best, lds, nullptr, tp, idx);
                 if (r == kStepError) {
                     atomicAdd(&tp.status[0], 1u);
                     finish = true;
@@ -1023,9 +1038,19 @@ struct CWalker {
     }
 };
 
-template <class G, bool ANY, int STACK, class Src, class Snk>
+template <class G, bool ANY, int STACK, bool ONE, class Src, class Snk>
 __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkClaim& work, uint2* lds, Src&& src,
                                                    Snk&& snk) {
+    // ONE: the TLAS has one instance, whose BLAS fields are read once here (before the kernel
+    // stores anything, so they can live in scalar registers) instead of per step and lane
+    BlasDev b1{};
+    if constexpr (ONE) b1 = wave_uniform(tp.inst[0].blas);
+    auto blas = [&](uint32_t i) -> const BlasDev& {
+        if constexpr (ONE)
+            return b1;
+        else
+            return tp.inst[i].blas;
+    };
     static_assert(G::kMode == kPhantom, "compacted lanes serve the phantom intersector");
     __shared__ uint32_t s_queue[(kTraceBlock / 64) * kCompactQueue];
     __shared__ unsigned long long s_key[kTraceBlock];  // best (t, instance, primitive) of the lane's ray
@@ -1188,7 +1213,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
             if (lane < m) {
                 const bool skip = ANY && uint32_t(s_key[os]) != kNoHitLow;  // already occluded
                 if (!skip) {
-                    const BlasDev& B = tp.inst[pinst].blas;
+                    const BlasDev& B = blas(pinst);
                     RayFrame F;
                     make_frame(po, pd, F);
                     const float4* pr = B.prims + size_t(slot) * 4;
@@ -1235,7 +1260,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
             atomicSub(&s_walk[wbase + home], 1u);  // the ray is occluded already
             state = kIdle;
         } else if (state == kTrav || state == kHelp) {
-            const int r = W.step(tp.inst[inst].blas, tmin, tcur, lds, cand);
+            const int r = W.step(blas(inst), tmin, tcur, lds, cand);
             if (r == kStepError) atomicAdd(&tp.status[0], 1u);
             if (state == kHelp) {
                 if (r == kStepError || r == kStepDone) {
@@ -1263,7 +1288,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
 
 // Persistent loop for one geometry variant: compacted intersection rounds for phantom curves
 // (the expensive, iterative test), the per-lane loop for LSS and DOTS.
-template <class G, bool ANY, int STACK, class Src, class Snk>
+template <class G, bool ANY, int STACK, bool ONE = false, class Src, class Snk>
 __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t total, uint32_t* counter, uint2* lds,
                                                  Src&& src, Snk&& snk, uint32_t* xcd = nullptr) {
     // an empty queue (uniform over the grid) ends the launch without touching the counter
@@ -1276,9 +1301,9 @@ __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t
     }
     WorkClaim work(counter, xcd, chunk, total, &s_dry);
     if constexpr (G::kMode == kPhantom)
-        persistent_compact<G, ANY, STACK>(tp, work, lds, src, snk);
+        persistent_compact<G, ANY, STACK, ONE>(tp, work, lds, src, snk);
     else
-        persistent_generic<G, ANY, STACK>(tp, work, lds, src, snk);
+        persistent_generic<G, ANY, STACK, ONE>(tp, work, lds, src, snk);
 }
 
 template <class G, bool ANY, int STACK>
@@ -1523,10 +1548,10 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_live(RenderParams rp) {
     if (live) rp.live[slot] = pix;
 }
 
-template <class G, int STACK>
+template <class G, int STACK, bool ONE>
 __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderParams rp) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    persistent_lanes<G, false, STACK>(
+    persistent_lanes<G, false, STACK, ONE>(
         rp.tp, rp.counters[6], rp.counters + 1, stk + threadIdx.x,
         [&](uint32_t r, uint32_t& pix, f3& o, f3& d, float& tmin, float& tmax) {
             pix = rp.live[r];
@@ -1585,13 +1610,13 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams r
 }
 
 // Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
-template <class G, int STACK>
+template <class G, int STACK, bool ONE>
 __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParams rp, const uint32_t* queue,
                                                                      const uint32_t* count, uint32_t* work) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t total = *count;
     const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
-    persistent_lanes<G, true, STACK>(
+    persistent_lanes<G, true, STACK, ONE>(
         rp.tp, total, work, stk + threadIdx.x,
         [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
             id = qi;
@@ -1749,8 +1774,15 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                0, st, rp);
             // the events bracket the traversal kernels alone (the per-kernel timings)
             if (ev) hipEventRecord(ev[0], st);
-            hipLaunchKernelGGL((k_render_primary_defer<G, STACK>), persistent_grid(k_render_primary_defer<G, STACK>, tp.blocks_per_cu),
-                               block, 0, st, rp);
+            // one-instance TLASes take kernels that read the BLAS fields once
+            const bool one = tp.ninst == 1;
+            if (one)
+                hipLaunchKernelGGL((k_render_primary_defer<G, STACK, true>),
+                                   persistent_grid(k_render_primary_defer<G, STACK, true>, tp.blocks_per_cu), block, 0, st, rp);
+            else
+                hipLaunchKernelGGL((k_render_primary_defer<G, STACK, false>),
+                                   persistent_grid(k_render_primary_defer<G, STACK, false>, tp.blocks_per_cu), block, 0, st,
+                                   rp);
             if (ev) hipEventRecord(ev[1], st);
             if (tp.defer) {
                 hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam},
@@ -1780,9 +1812,14 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                         squeue = rp.fallback_queue;
                         scount = rp.counters + 5;
                     }
-                    hipLaunchKernelGGL((k_render_shadow_defer<G, STACK>),
-                                       persistent_grid(k_render_shadow_defer<G, STACK>, tp.blocks_per_cu), block, 0, st, rp, squeue,
-                                       scount, rp.counters + 2);
+                    if (one)
+                        hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, true>),
+                                           persistent_grid(k_render_shadow_defer<G, STACK, true>, tp.blocks_per_cu), block,
+                                           0, st, rp, squeue, scount, rp.counters + 2);
+                    else
+                        hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, false>),
+                                           persistent_grid(k_render_shadow_defer<G, STACK, false>, tp.blocks_per_cu), block,
+                                           0, st, rp, squeue, scount, rp.counters + 2);
                     if (ev) hipEventRecord(ev[3], st);
                     shadow_timed = true;
                     if (tp.defer)
