@@ -187,6 +187,9 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
     const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
     const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
     for (uint32_t base = wave * 16; base < n; base += nwaves * 16) {
+        // a compiler-only fence: the weight fragments are read from LDS in every pass rather
+        // than hoisted out of the loop into ~150 registers, which would leave one wave per SIMD
+        asm volatile("" ::: "memory");
         const uint32_t q = base + uint32_t(lane & 15);
         float x[6] = {0, 0, 0, 0, 0, 0};
         uint32_t pix = 0;

@@ -1306,12 +1306,12 @@ __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t
         persistent_generic<G, ANY, STACK, ONE>(tp, work, lds, src, snk);
 }
 
-template <class G, bool ANY, int STACK>
+template <class G, bool ANY, int STACK, bool ONE>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TraceParams tp, const float* __restrict__ rays,
                                                                        uint32_t n, HitRec* __restrict__ hits,
                                                                        uint32_t* __restrict__ counter) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    persistent_lanes<G, ANY, STACK>(
+    persistent_lanes<G, ANY, STACK, ONE>(
         tp, n, counter, stk + threadIdx.x,
         [&](uint32_t i, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
             id = i;
@@ -1734,14 +1734,24 @@ hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t 
             hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, tp.work, 1u);
             if (tp.xcd_chunk && tp.xcd_work)
                 hipLaunchKernelGGL(k_zero, dim3(8 * kXcdStride / 64), dim3(64), 0, st, tp.xcd_work, 8u * kXcdStride);
-            if (any)
-                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK>),
-                                   persistent_grid(k_trace_rays_persistent<G, true, STACK>, tp.blocks_per_cu), block, 0, st, tp, rays, n,
-                                   hits, tp.work);
+            // one-instance TLASes take kernels that read the BLAS fields once
+            const bool one = tp.ninst == 1;
+            if (any && one)
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK, true>),
+                                   persistent_grid(k_trace_rays_persistent<G, true, STACK, true>, tp.blocks_per_cu), block, 0,
+                                   st, tp, rays, n, hits, tp.work);
+            else if (any)
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK, false>),
+                                   persistent_grid(k_trace_rays_persistent<G, true, STACK, false>, tp.blocks_per_cu), block, 0,
+                                   st, tp, rays, n, hits, tp.work);
+            else if (one)
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK, true>),
+                                   persistent_grid(k_trace_rays_persistent<G, false, STACK, true>, tp.blocks_per_cu), block,
+                                   0, st, tp, rays, n, hits, tp.work);
             else
-                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK>),
-                                   persistent_grid(k_trace_rays_persistent<G, false, STACK>, tp.blocks_per_cu), block, 0, st, tp, rays, n,
-                                   hits, tp.work);
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK, false>),
+                                   persistent_grid(k_trace_rays_persistent<G, false, STACK, false>, tp.blocks_per_cu), block,
+                                   0, st, tp, rays, n, hits, tp.work);
             return hipGetLastError();
         }
     }
