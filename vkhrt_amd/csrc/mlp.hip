@@ -75,12 +75,16 @@ __global__ void k_pack(const float* __restrict__ master, uint16_t* __restrict__ 
 }
 
 // Feature f = 16 t + 4 g + j of the lane's sample for input dim t: k = 2 g + j / 2,
-// sin for even j, cos for odd j (feature (dim * 8 + k) * 2 + s).
+// sin for even j, cos for odd j (feature (dim * 8 + k) * 2 + s). The odd frequency comes from
+// the even one by the double-angle formulas (a few ulp in FP32, far below the bf16 rounding
+// of the features), which halves the sinpi / cospi evaluations.
 __device__ __forceinline__ void encode(const float x[6], int g, s4 enc[6]) {
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
-        const float v0 = ldexpf(x[t], 2 * g), v1 = ldexpf(x[t], 2 * g + 1);
-        enc[t] = pack4(sinpif(v0), cospif(v0), sinpif(v1), cospif(v1));
+        const float v0 = ldexpf(x[t], 2 * g);
+        const float s0 = sinpif(v0), c0 = cospif(v0);
+        const float s1 = 2.0f * s0 * c0, c1 = (c0 - s0) * (c0 + s0);
+        enc[t] = pack4(s0, c0, s1, c1);
     }
 }
 
