@@ -149,6 +149,33 @@ def test_instances_with_transforms(gpu_ctx, G, oracle, curves):
     _compare_hits(h, ref, "instances")
 
 
+@pytest.mark.parametrize("count", [3, 18])
+def test_render_many_instances(gpu_ctx, G, oracle, curves, count):
+    """Renders of 3 and 18 translated instances with shadows and the FP64 polish: the render
+    kernels that stage the BLAS fields of up to 16 instances in LDS, and those that read
+    them per lane beyond that, give the oracle's primitive ids."""
+    import vkhrt_amd as V
+    S = oracle.Scene()
+    blases = []
+    for i in range(count):
+        # overlapping copies of the head, a few units apart
+        xf = np.array([1, 0, 0, 3.0 * (i % 6) - 7.5, 0, 1, 0, 2.0 * (i // 6), 0, 0, 1, -1.5 * i], np.float32)
+        S.add(oracle.MODE_PHANTOM, curves, per_point=True, transform=xf)
+        blases.append(G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True, transform=xf))
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, blases)
+    W, H = 320, 240
+    cam = V.camera_look_at((0.0, 50.0, 400.0), (0.0, 30.0, 0.0), aspect=W / H)
+    flags = V.RENDER_SHADOWS | V.RENDER_FP64_POLISH
+    print(f"[render_many_instances] count={count}", flush=True)
+    img, ids = G.render(gpu_ctx, tlas, cam, W, H, flags)
+    vi, pi = G.cam_arrays(cam)
+    rimg, rids, _, _ = S.render(vi, pi, W, H, oracle.F_POLISH, shadows=1, pixel_stride=3)
+    mask = (np.arange(W * H).reshape(H, W) % 3) == 0
+    assert (ids[mask] != 0xFFFFFFFF).sum() > 150
+    assert np.array_equal(ids[mask], rids[mask])
+    assert G.psnr(img[..., :3][mask], rimg[mask]) >= 50.0
+
+
 def test_edge_cases(gpu_ctx, G, oracle, curves):
     """Empty BLAS, a single primitive, axis-aligned directions, tmin > tmax."""
     import vkhrt_amd as V

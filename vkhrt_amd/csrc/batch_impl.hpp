@@ -190,11 +190,11 @@ __global__ __launch_bounds__(kAppendBlock) void k_batch_primary(BatchParams bp) 
 
 // Closest hit of the depth-b queue: hit record per queue slot (and the merge key when
 // grazing roots are deferred to the FP64 polish).
-template <class G, int STACK>
+template <class G, int STACK, int IM>
 __global__ __launch_bounds__(kTraceBlock) void k_batch_trace(BatchParams bp, uint32_t b) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t* q = depth_queue(bp, b);
-    persistent_lanes<G, false, STACK>(
+    persistent_lanes<G, false, STACK, IM>(
         bp.tp, bp.counters[bc(b, kBcQueue)], bp.counters + bc(b, kBcWork), stk + threadIdx.x,
         [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
             id = qi;
@@ -380,11 +380,11 @@ __global__ __launch_bounds__(kAppendBlock) void k_batch_shade(BatchParams bp, ui
 }
 
 // Shadow rays of depth b (any hit): visibility into nee.w.
-template <class G, int STACK>
+template <class G, int STACK, int IM>
 __global__ __launch_bounds__(kTraceBlock) void k_batch_shadow(BatchParams bp, uint32_t b) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const SrcBatchShadow src{bp.squeue, bp.ray};
-    persistent_lanes<G, true, STACK>(
+    persistent_lanes<G, true, STACK, IM>(
         bp.tp, bp.counters[bc(b, kBcShadow)], bp.counters + bc(b, kBcShadowWork), stk + threadIdx.x,
         [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
             id = qi;
@@ -450,6 +450,7 @@ hipError_t launch_batch_variant(const BatchParams& bp0, hipStream_t st) {
         const dim3 all(all_blocks), strided(std::min<uint32_t>(all_blocks, 2048u)), pgrid(1024);
         hipLaunchKernelGGL(k_zero, dim3((kBatchCounterWords + 255) / 256), dim3(256), 0, st, bp0.counters,
                            kBatchCounterWords);
+        const int im = instance_mode(bp0.tp.ninst);  // how the traversal reaches the BLAS fields
         hipLaunchKernelGGL(k_batch_primary, all, ab, 0, st, bp0);
         for (uint32_t b = 0; b <= bp0.bounces; ++b) {
             BatchParams bp = bp0;
@@ -463,9 +464,16 @@ hipError_t launch_batch_variant(const BatchParams& bp0, hipStream_t st) {
             if constexpr (G::kMixed)
                 hipLaunchKernelGGL((k_batch_trace_flat<G, STACK>), dim3((total + kTraceBlock - 1) / kTraceBlock), tb, 0,
                                    st, bp, b);
+            else if (im == kInstOne)
+                hipLaunchKernelGGL((k_batch_trace<G, STACK, kInstOne>),
+                                   persistent_grid(k_batch_trace<G, STACK, kInstOne>, bp.tp.blocks_per_cu), tb, 0, st, bp, b);
+            else if (im == kInstFew)
+                hipLaunchKernelGGL((k_batch_trace<G, STACK, kInstFew>),
+                                   persistent_grid(k_batch_trace<G, STACK, kInstFew>, bp.tp.blocks_per_cu), tb, 0, st, bp, b);
             else
-                hipLaunchKernelGGL((k_batch_trace<G, STACK>),
-                                   persistent_grid(k_batch_trace<G, STACK>, bp.tp.blocks_per_cu), tb, 0, st, bp, b);
+                hipLaunchKernelGGL((k_batch_trace<G, STACK, kInstMany>),
+                                   persistent_grid(k_batch_trace<G, STACK, kInstMany>, bp.tp.blocks_per_cu), tb, 0, st, bp,
+                                   b);
             if (bp.tp.defer) {
                 const SrcBatch src{depth_queue(bp, b), bp.ray};
                 hipLaunchKernelGGL(k_polish_closest<SrcBatch>, pgrid, tb, 0, st, bp.tp, src, bp.tp.cand_hits,
@@ -481,9 +489,18 @@ hipError_t launch_batch_variant(const BatchParams& bp0, hipStream_t st) {
             if constexpr (G::kMixed)
                 hipLaunchKernelGGL((k_batch_shadow_flat<G, STACK>), dim3((total + kTraceBlock - 1) / kTraceBlock), tb, 0,
                                    st, bp, b);
+            else if (im == kInstOne)
+                hipLaunchKernelGGL((k_batch_shadow<G, STACK, kInstOne>),
+                                   persistent_grid(k_batch_shadow<G, STACK, kInstOne>, bp.tp.blocks_per_cu), tb, 0, st, bp,
+                                   b);
+            else if (im == kInstFew)
+                hipLaunchKernelGGL((k_batch_shadow<G, STACK, kInstFew>),
+                                   persistent_grid(k_batch_shadow<G, STACK, kInstFew>, bp.tp.blocks_per_cu), tb, 0, st, bp,
+                                   b);
             else
-                hipLaunchKernelGGL((k_batch_shadow<G, STACK>),
-                                   persistent_grid(k_batch_shadow<G, STACK>, bp.tp.blocks_per_cu), tb, 0, st, bp, b);
+                hipLaunchKernelGGL((k_batch_shadow<G, STACK, kInstMany>),
+                                   persistent_grid(k_batch_shadow<G, STACK, kInstMany>, bp.tp.blocks_per_cu), tb, 0, st,
+                                   bp, b);
             if (bp.tp.defer)
                 hipLaunchKernelGGL(k_batch_polish_shadow, pgrid, tb, 0, st, bp.tp, SrcBatchShadow{bp.squeue, bp.ray},
                                    bp.nee);

@@ -777,6 +777,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow(RenderParams rp) 
 }
 
 // ------------------------------------------------------------------ persistent lanes
+// How the persistent loops reach the BLAS fields of a step's instance: through the lane's
+// instance index in global memory (many instances), once into scalar registers (one), or
+// staged per block in LDS (up to kFewInstances).
+enum InstanceMode { kInstMany = 0, kInstOne = 1, kInstFew = 2 };
+constexpr uint32_t kFewInstances = 16;
+inline int instance_mode(uint32_t ninst) {
+    return ninst == 1 ? kInstOne : ninst <= kFewInstances ? kInstFew : kInstMany;
+}
+
 // A copy of a value every lane of the wave reads alike, word by word through readfirstlane,
 // so that it is held in scalar registers.
 template <class T>
@@ -862,11 +871,11 @@ struct WorkClaim {
 // tmax) produces work item r and names its ray `id` (the index results and deferred-polish
 // entries use); snk(id, o, d, best) consumes the result (called by the lanes that finish in
 // the same iteration, so it may use wave ballots).
-template <class G, bool ANY, int STACK, bool ONE, class Src, class Snk>
+template <class G, bool ANY, int STACK, int IM, class Src, class Snk>
 __device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkClaim& work, uint2* lds, Src&& src,
                                                    Snk&& snk) {
-    BlasDev b1{};  // ONE: as in persistent_compact
-    if constexpr (ONE) b1 = wave_uniform(tp.inst[0].blas);
+    BlasDev b1{};  // one instance: as in persistent_compact (a few: read per lane here)
+    if constexpr (IM == kInstOne) b1 = wave_uniform(tp.inst[0].blas);
     static_assert(!G::kMixed, "persistent kernels serve single-mode TLASes");
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -920,7 +929,7 @@ __device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkCl
                     need_inst = false;
                 }
             } else {
-                const int r = W.step(ONE ? b1 : tp.inst[inst].blas, tmin, inst, best, lds, nullptr, tp, idx);
+                const int r = W.step(IM == kInstOne ? b1 : tp.inst[inst].blas, tmin, inst, best, lds, nullptr, tp, idx);
                 if (r == kStepError) {
                     atomicAdd(&tp.status[0], 1u);
                     finish = true;
@@ -1038,16 +1047,26 @@ struct CWalker {
     }
 };
 
-template <class G, bool ANY, int STACK, bool ONE, class Src, class Snk>
+template <class G, bool ANY, int STACK, int IM, class Src, class Snk>
 __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkClaim& work, uint2* lds, Src&& src,
                                                    Snk&& snk) {
-    // ONE: the TLAS has one instance, whose BLAS fields are read once here (before the kernel
-    // stores anything, so they can live in scalar registers) instead of per step and lane
+    // One instance: its BLAS fields are read once here, into scalar registers, instead of per
+    // step and lane. A few instances: the block stages all their BLAS fields in LDS, so a
+    // step reads them from there instead of through two dependent global loads.
     BlasDev b1{};
-    if constexpr (ONE) b1 = wave_uniform(tp.inst[0].blas);
+    const BlasDev* few = nullptr;
+    if constexpr (IM == kInstOne) b1 = wave_uniform(tp.inst[0].blas);
+    if constexpr (IM == kInstFew) {
+        __shared__ BlasDev s_blas[kFewInstances];
+        for (uint32_t i = threadIdx.x; i < tp.ninst && i < kFewInstances; i += blockDim.x) s_blas[i] = tp.inst[i].blas;
+        __syncthreads();
+        few = s_blas;
+    }
     auto blas = [&](uint32_t i) -> const BlasDev& {
-        if constexpr (ONE)
+        if constexpr (IM == kInstOne)
             return b1;
+        else if constexpr (IM == kInstFew)
+            return few[i];
         else
             return tp.inst[i].blas;
     };
@@ -1288,7 +1307,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
 
 // Persistent loop for one geometry variant: compacted intersection rounds for phantom curves
 // (the expensive, iterative test), the per-lane loop for LSS and DOTS.
-template <class G, bool ANY, int STACK, bool ONE = false, class Src, class Snk>
+template <class G, bool ANY, int STACK, int IM = kInstMany, class Src, class Snk>
 __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t total, uint32_t* counter, uint2* lds,
                                                  Src&& src, Snk&& snk, uint32_t* xcd = nullptr) {
     // an empty queue (uniform over the grid) ends the launch without touching the counter
@@ -1301,17 +1320,17 @@ __device__ __forceinline__ void persistent_lanes(const TraceParams& tp, uint32_t
     }
     WorkClaim work(counter, xcd, chunk, total, &s_dry);
     if constexpr (G::kMode == kPhantom)
-        persistent_compact<G, ANY, STACK, ONE>(tp, work, lds, src, snk);
+        persistent_compact<G, ANY, STACK, IM>(tp, work, lds, src, snk);
     else
-        persistent_generic<G, ANY, STACK, ONE>(tp, work, lds, src, snk);
+        persistent_generic<G, ANY, STACK, IM>(tp, work, lds, src, snk);
 }
 
-template <class G, bool ANY, int STACK, bool ONE>
+template <class G, bool ANY, int STACK, int IM>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TraceParams tp, const float* __restrict__ rays,
                                                                        uint32_t n, HitRec* __restrict__ hits,
                                                                        uint32_t* __restrict__ counter) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    persistent_lanes<G, ANY, STACK, ONE>(
+    persistent_lanes<G, ANY, STACK, IM>(
         tp, n, counter, stk + threadIdx.x,
         [&](uint32_t i, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
             id = i;
@@ -1548,10 +1567,10 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_live(RenderParams rp) {
     if (live) rp.live[slot] = pix;
 }
 
-template <class G, int STACK, bool ONE>
+template <class G, int STACK, int IM>
 __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderParams rp) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    persistent_lanes<G, false, STACK, ONE>(
+    persistent_lanes<G, false, STACK, IM>(
         rp.tp, rp.counters[6], rp.counters + 1, stk + threadIdx.x,
         [&](uint32_t r, uint32_t& pix, f3& o, f3& d, float& tmin, float& tmax) {
             pix = rp.live[r];
@@ -1610,13 +1629,13 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams r
 }
 
 // Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
-template <class G, int STACK, bool ONE>
+template <class G, int STACK, int IM>
 __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParams rp, const uint32_t* queue,
                                                                      const uint32_t* count, uint32_t* work) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t total = *count;
     const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
-    persistent_lanes<G, true, STACK, ONE>(
+    persistent_lanes<G, true, STACK, IM>(
         rp.tp, total, work, stk + threadIdx.x,
         [&](uint32_t qi, uint32_t& id, f3& o, f3& d, float& tmin, float& tmax) {
             id = qi;
@@ -1735,22 +1754,22 @@ hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t 
             if (tp.xcd_chunk && tp.xcd_work)
                 hipLaunchKernelGGL(k_zero, dim3(8 * kXcdStride / 64), dim3(64), 0, st, tp.xcd_work, 8u * kXcdStride);
             // one-instance TLASes take kernels that read the BLAS fields once
-            const bool one = tp.ninst == 1;
+            const bool one = instance_mode(tp.ninst) == kInstOne;
             if (any && one)
-                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK, true>),
-                                   persistent_grid(k_trace_rays_persistent<G, true, STACK, true>, tp.blocks_per_cu), block, 0,
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK, kInstOne>),
+                                   persistent_grid(k_trace_rays_persistent<G, true, STACK, kInstOne>, tp.blocks_per_cu), block, 0,
                                    st, tp, rays, n, hits, tp.work);
             else if (any)
-                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK, false>),
-                                   persistent_grid(k_trace_rays_persistent<G, true, STACK, false>, tp.blocks_per_cu), block, 0,
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, true, STACK, kInstMany>),
+                                   persistent_grid(k_trace_rays_persistent<G, true, STACK, kInstMany>, tp.blocks_per_cu), block, 0,
                                    st, tp, rays, n, hits, tp.work);
             else if (one)
-                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK, true>),
-                                   persistent_grid(k_trace_rays_persistent<G, false, STACK, true>, tp.blocks_per_cu), block,
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK, kInstOne>),
+                                   persistent_grid(k_trace_rays_persistent<G, false, STACK, kInstOne>, tp.blocks_per_cu), block,
                                    0, st, tp, rays, n, hits, tp.work);
             else
-                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK, false>),
-                                   persistent_grid(k_trace_rays_persistent<G, false, STACK, false>, tp.blocks_per_cu), block,
+                hipLaunchKernelGGL((k_trace_rays_persistent<G, false, STACK, kInstMany>),
+                                   persistent_grid(k_trace_rays_persistent<G, false, STACK, kInstMany>, tp.blocks_per_cu), block,
                                    0, st, tp, rays, n, hits, tp.work);
             return hipGetLastError();
         }
@@ -1784,15 +1803,21 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                0, st, rp);
             // the events bracket the traversal kernels alone (the per-kernel timings)
             if (ev) hipEventRecord(ev[0], st);
-            // one-instance TLASes take kernels that read the BLAS fields once
-            const bool one = tp.ninst == 1;
-            if (one)
-                hipLaunchKernelGGL((k_render_primary_defer<G, STACK, true>),
-                                   persistent_grid(k_render_primary_defer<G, STACK, true>, tp.blocks_per_cu), block, 0, st, rp);
+            // one-instance TLASes take kernels that read the BLAS fields once, TLASes of up to
+            // kFewInstances instances kernels that stage them in LDS
+            const int im = instance_mode(tp.ninst);
+            if (im == kInstOne)
+                hipLaunchKernelGGL((k_render_primary_defer<G, STACK, kInstOne>),
+                                   persistent_grid(k_render_primary_defer<G, STACK, kInstOne>, tp.blocks_per_cu), block, 0,
+                                   st, rp);
+            else if (im == kInstFew)
+                hipLaunchKernelGGL((k_render_primary_defer<G, STACK, kInstFew>),
+                                   persistent_grid(k_render_primary_defer<G, STACK, kInstFew>, tp.blocks_per_cu), block, 0,
+                                   st, rp);
             else
-                hipLaunchKernelGGL((k_render_primary_defer<G, STACK, false>),
-                                   persistent_grid(k_render_primary_defer<G, STACK, false>, tp.blocks_per_cu), block, 0, st,
-                                   rp);
+                hipLaunchKernelGGL((k_render_primary_defer<G, STACK, kInstMany>),
+                                   persistent_grid(k_render_primary_defer<G, STACK, kInstMany>, tp.blocks_per_cu), block,
+                                   0, st, rp);
             if (ev) hipEventRecord(ev[1], st);
             if (tp.defer) {
                 hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam},
@@ -1822,14 +1847,18 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                         squeue = rp.fallback_queue;
                         scount = rp.counters + 5;
                     }
-                    if (one)
-                        hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, true>),
-                                           persistent_grid(k_render_shadow_defer<G, STACK, true>, tp.blocks_per_cu), block,
-                                           0, st, rp, squeue, scount, rp.counters + 2);
+                    if (im == kInstOne)
+                        hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, kInstOne>),
+                                           persistent_grid(k_render_shadow_defer<G, STACK, kInstOne>, tp.blocks_per_cu),
+                                           block, 0, st, rp, squeue, scount, rp.counters + 2);
+                    else if (im == kInstFew)
+                        hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, kInstFew>),
+                                           persistent_grid(k_render_shadow_defer<G, STACK, kInstFew>, tp.blocks_per_cu),
+                                           block, 0, st, rp, squeue, scount, rp.counters + 2);
                     else
-                        hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, false>),
-                                           persistent_grid(k_render_shadow_defer<G, STACK, false>, tp.blocks_per_cu), block,
-                                           0, st, rp, squeue, scount, rp.counters + 2);
+                        hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, kInstMany>),
+                                           persistent_grid(k_render_shadow_defer<G, STACK, kInstMany>, tp.blocks_per_cu),
+                                           block, 0, st, rp, squeue, scount, rp.counters + 2);
                     if (ev) hipEventRecord(ev[3], st);
                     shadow_timed = true;
                     if (tp.defer)
