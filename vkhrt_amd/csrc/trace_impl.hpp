@@ -70,6 +70,14 @@ __device__ __forceinline__ bool better(float t, uint32_t inst, uint32_t pid, con
 __device__ __forceinline__ float safe_rcp(float x) { return 1.0f / (fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x); }
 __device__ __forceinline__ f3 safe_inv(f3 d) { return f3{safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)}; }
 
+// The same address as a global-memory pointer: loads through it are global_load, not flat
+// loads (a flat load also counts against the LDS counter, so waiting for an LDS access would
+// wait for it too). The BLAS arrays are device allocations.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+    return (const __attribute__((address_space(1))) T*)(p);
+}
+
 __device__ __forceinline__ float qbyte(uint32_t lo, uint32_t hi, int s) {
     uint32_t w = s < 4 ? lo : hi;
     return float((w >> (8 * (s & 3))) & 0xffu);
@@ -97,7 +105,7 @@ __device__ __forceinline__ void node_visit(const WideNode* nodes, uint32_t node,
                                            float tmin, float tcur, uint32_t& hits, uint32_t& pmask,
                                            uint32_t& child_base, uint32_t& imask, uint32_t& pbase) {
     const bool nx = oct & 1u, ny = oct & 2u, nz = oct & 4u;
-    const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+    const auto* np = gmem(reinterpret_cast<const uint4*>(nodes + node));
     const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
     const float sx = __uint_as_float((n0.w & 0xffu) << 23) * inv.x;
     const float sy = __uint_as_float(((n0.w >> 8) & 0xffu) << 23) * inv.y;
@@ -1008,9 +1016,9 @@ struct CWalker {
         if (pmask) {
             const uint32_t slot = pbase + uint32_t(__ffs(pmask) - 1);
             pmask &= pmask - 1;
-            const float4* pr = B.prims + size_t(slot) * 4;
+            const auto* pr = gmem(B.prims + size_t(slot) * 4);
             const float4 a = pr[0], b = pr[3];
-            if (ray_cylinder(o, d, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, B.aux[slot])) cand = slot;
+            if (ray_cylinder(o, d, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, gmem(B.aux)[slot])) cand = slot;
         } else if (have_node) {
             if (node >= B.nnodes) return kStepError;  // malformed structure: report, never read out of bounds
             uint32_t hits, pm, child_base, imask, pb;
@@ -1235,7 +1243,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
                     const BlasDev& B = blas(pinst);
                     RayFrame F;
                     make_frame(po, pd, F);
-                    const float4* pr = B.prims + size_t(slot) * 4;
+                    const auto* pr = gmem(B.prims + size_t(slot) * 4);
                     float4 cv[4] = {pr[0], pr[1], pr[2], pr[3]};
                     hit = prhi<G::kRMode, G::kPolish ? kPolishDefer : kPolishOff, false>(F, cv, 0.0f, B.radius, ph);
                     if (G::kPolish && ph.deferred) {
