@@ -73,9 +73,14 @@ __device__ __forceinline__ f3 safe_inv(f3 d) { return f3{safe_rcp(d.x), safe_rcp
 // The same address as a global-memory pointer: loads through it are global_load, not flat
 // loads (a flat load also counts against the LDS counter, so waiting for an LDS access would
 // wait for it too). The BLAS arrays are device allocations.
+// (The host pass only checks these device bodies; it keeps the generic pointer.)
 template <class T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+__device__ __forceinline__ auto gmem(const T* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
     return (const __attribute__((address_space(1))) T*)(p);
+#else
+    return p;
+#endif
 }
 
 __device__ __forceinline__ float qbyte(uint32_t lo, uint32_t hi, int s) {
