@@ -618,7 +618,7 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
     """Neural shadow form on the 500k-strand x 32-segment model (16M segments) at 1920x1080
     with 2 area-light samples per hit. The MLP is trained (Adam, BF16 MFMA forward/backward) on
     2^24 shadow rays from points drawn uniformly on the strand surfaces toward light points,
-    labelled by the exact tracer; judged on 2^20 held-out rays (another seed); then the frame
+    labelled by the exact tracer; evaluated on 2^20 held-out rays (another seed); then the frame
     is rendered with the variance-gated MLP (exact fallback above the threshold) against the
     exact frame."""
     ns, nseg, seed = 500000, 32, 104
