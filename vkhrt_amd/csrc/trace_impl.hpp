@@ -1612,9 +1612,11 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams r
     const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
     const uint32_t pix = y * rp.cam.width + x;
     bool hit = false;
+    float ht = 0.0f;
     if (valid) {
         const HitRec h = rp.hitrec[pix];
         hit = h.prim_id != 0xFFFFFFFFu;
+        ht = h.t;
         if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
         const f3 n = f3{h.n[0], h.n[1], h.n[2]};
         if (!hit) {
@@ -1637,8 +1639,23 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams r
     }
     const uint32_t S = rp.light.samples;
     const uint32_t slot = block_append(hit, &rp.counters[0], S);
-    if (hit)
+    if (hit) {
         for (uint32_t k = 0; k < S; ++k) rp.queue[slot + k] = pix * S + k;
+        if (rp.mlp) {
+            // neural form: the MLP's inputs of each queued sample, as SrcShadow::get forms the
+            // shadow ray (origin on the primary hit, direction toward the light sample)
+            f3 po, pd;
+            primary_ray(rp.cam, x, y, po, pd);
+            const f3 o = add(po, scl(ht, pd));
+            for (uint32_t k = 0; k < S; ++k) {
+                f3 d;
+                float tmax;
+                light_ray(rp.light, pix, k, o, d, tmax);
+                rp.origins[slot + k] = make_float4(o.x, o.y, o.z, 0.0f);
+                if (rp.dirs) rp.dirs[slot + k] = make_float4(d.x, d.y, d.z, 0.0f);
+            }
+        }
+    }
 }
 
 // Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
@@ -1658,18 +1675,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
             rp.vis[queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
         },
         rp.counters + kXcdCounters + 8 * kXcdStride);
-}
-
-// Shadow-ray origins (and, with an area light, directions) of the queued ids: the MLP's inputs.
-__global__ void k_shadow_origins(RenderParams rp) {
-    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= rp.counters[0]) return;
-    const SrcShadow src{rp.cam, rp.queue, rp.hitbuf, rp.light};
-    f3 o, d;
-    float tmin, tmax;
-    src.get(qi, o, d, tmin, tmax);
-    rp.origins[qi] = make_float4(o.x, o.y, o.z, 0.0f);
-    if (rp.dirs) rp.dirs[qi] = make_float4(d.x, d.y, d.z, 0.0f);
 }
 
 // Shades the queued hit pixels with their light visibility: the mean over the pixel's light
@@ -1852,7 +1857,6 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                     const uint32_t* scount = rp.counters;
                     if (ev) hipEventRecord(ev[2], st);
                     if (rp.mlp) {
-                        hipLaunchKernelGGL(k_shadow_origins, sgrid, block, 0, st, rp);
                         MlpShadowParams sp{rp.queue, rp.counters, rp.origins, {0.0f, 1.0f, 0.0f}, rp.mlp_threshold,
                                            rp.vis, rp.fallback_queue, rp.counters + 5, rp.dirs};
                         const hipError_t e = launch_mlp_shadow_gate(*rp.mlp, sp, npix * rp.light.samples, st);
