@@ -523,3 +523,51 @@ def test_environment_map_render(gpu_ctx, G, oracle, mixed):
     diff = np.abs(got[~hit] - ref[~hit])
     assert diff.max() < 1e-4 and np.median(diff) < 1e-6
     assert np.ptp(ref[~hit][:, 2]) > 0.1  # the map varies across the missed pixels
+
+
+@pytest.mark.parametrize("mode", ["phantom", "lss", "dots"])
+def test_count_candidates_matches_oracle(gpu_ctx, G, oracle, curves, mode):
+    """vkhrt_count_candidates (the N*_prim of the roofline's algorithmic bytes) equals the
+    oracle's count ray for ray: camera rays of the C0 frame against their closest hits, and
+    the shadow rays those hits spawn against their any-hit results."""
+    import vkhrt_amd as V
+    W = H = 256
+    cam = V.camera_look_at((0.0, 112.0, 140.0), (0.0, 105.0, 0.0), fov_deg=22.62, aspect=1.0)
+    vi, pi = G.cam_arrays(cam)
+    S = oracle.Scene()
+    if mode == "phantom":
+        prims, gmode, kw = curves, V.GEOMETRY_PHANTOM_CURVES, {"per_point": True}
+        S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    else:
+        lines = oracle.curves_to_lines(curves, 4)
+        if mode == "lss":
+            prims, gmode = oracle.generate_lss(lines), V.GEOMETRY_LSS
+            S.add(oracle.MODE_LSS, prims)
+        else:
+            prims, gmode = lines, V.GEOMETRY_DOTS
+            S.add(oracle.MODE_DOTS, lines, per_point=True)
+        kw = {"per_point": mode == "dots"}
+    blas = G.build_blas(gpu_ctx, gmode, prims, **kw)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    rays = oracle.raygen(vi, pi, W, H)
+    ref = S.trace(rays)
+    h = G.trace(gpu_ctx, tlas, rays)
+    _compare_hits(h, ref, f"{mode} C0 camera rays")
+    hit = ref["prim"] != 0xFFFFFFFF
+    srays = np.zeros((int(hit.sum()), 8), np.float32)  # shadow rays toward (0, 1, 0)
+    srays[:, :3] = rays[hit, :3] + rays[hit, 4:7] * ref["t"][hit, None]
+    srays[:, 3], srays[:, 5], srays[:, 7] = 0.001, 1.0, 10000.0
+    sref = S.trace(srays, flags=oracle.F_ANY_HIT)
+    sh = G.trace(gpu_ctx, tlas, srays, flags=V.TRACE_TERMINATE_ON_FIRST_HIT)
+    assert np.array_equal(sh["prim"] == 0xFFFFFFFF, sref["prim"] == 0xFFFFFFFF)
+    for what, r, hh, rr in (("primary", rays, h, ref), ("shadow", srays, sh, sref)):
+        d_r, d_h = G.dev(r), G.dev(hh.view(np.float32).reshape(-1))
+        counts = torch.zeros(r.shape[0], dtype=torch.int32, device="cuda")
+        gpu_ctx.count_candidates(tlas, d_r.data_ptr(), d_h.data_ptr(), r.shape[0], counts.data_ptr(),
+                                 stream=G.stream_ptr())
+        torch.cuda.synchronize()
+        g = counts.cpu().numpy().view(np.uint32)
+        o = S.count_candidates(r, rr)
+        print(f"{mode} {what}: {r.shape[0]} rays, N* mean GPU {g.mean():.4f} oracle {o.mean():.4f}")
+        assert g.sum() > 0
+        assert np.array_equal(g, o), f"{mode} {what}: counts differ on {(g != o).sum()} rays"

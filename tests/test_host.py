@@ -77,6 +77,70 @@ def test_camera_look_at_matches_oracle(vk):
         vk.camera_look_at((0, 0, 0), (0, 0, -1), fov_deg=0.0)
 
 
+def _closed_form_dirs(eye, center, up, tan_half, aspect, W, H):
+    """Pixel ray directions from glm's closed forms, in float64: lookAt gives the right-handed
+    basis f = normalize(center - eye), s = normalize(f x up), u = s x f (fly_camera.cpp:27);
+    perspectiveRH_ZO with [1][1] *= -1 (fly_camera.cpp:31-33) maps the NDC point (dx, dy, 1, 1)
+    of ray_gen.rgen:18-24 back to the view-space point (dx*aspect*tan, -dy*tan, -1, 1/far).
+    The world direction is therefore dx*aspect*tan*s - dy*tan*u + f, normalized."""
+    eye, center, up = (np.asarray(v, np.float64) for v in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    x, y = np.meshgrid(np.arange(W), np.arange(H))
+    dx = (x + 0.5) / W * 2.0 - 1.0
+    dy = (y + 0.5) / H * 2.0 - 1.0
+    d = (dx * aspect * tan_half)[..., None] * s - (dy * tan_half)[..., None] * u + f
+    return d / np.linalg.norm(d, axis=-1, keepdims=True), s, u
+
+
+def _raygen_f64(vi, pi, W, H):
+    """ray_gen.rgen:18-24 evaluated in float64 on given inverse matrices (column-major)."""
+    VI = np.asarray(vi, np.float64).reshape(4, 4).T
+    PI = np.asarray(pi, np.float64).reshape(4, 4).T
+    x, y = np.meshgrid(np.arange(W), np.arange(H))
+    ndc = np.stack([(x + 0.5) / W * 2 - 1, (y + 0.5) / H * 2 - 1, np.ones_like(x, float), np.ones_like(x, float)], -1)
+    tgt = ndc @ PI.T
+    t3 = tgt[..., :3] / np.linalg.norm(tgt[..., :3], axis=-1, keepdims=True)
+    d = np.concatenate([t3, np.zeros(t3.shape[:-1] + (1,))], -1) @ VI.T
+    o = VI @ np.array([0, 0, 0, 1.0])
+    return o[:3], d[..., :3]
+
+
+@pytest.mark.parametrize("eye,center,aspect", [((0, 60, 300), (0, 40, 0), 1920 / 1080),
+                                               ((-120.5, 3, 17), (4, 5, -6), 0.75),
+                                               ((10, -40, 5), (0, 0, 0), 1.0)])
+def test_camera_orientation_closed_form(vk, eye, center, aspect):
+    """Which way the image points, from glm's closed forms rather than from the oracle: row 0
+    looks toward +up (the Vulkan Y flip of fly_camera.cpp:33), column 0 toward -right (s = f x up),
+    and every pixel's direction equals the closed form within FP32 rounding. Checked for the
+    oracle's matrices and rays and for the library's vkhrt_camera_look_at. A mirrored image (flip
+    dropped, or the cross product reversed) moves edge rays by about the field of view."""
+    W, H, tan_half, up = 48, 30, 0.5773502691896257, (0.0, 1.0, 0.0)
+    ref, s, u = _closed_form_dirs(eye, center, up, tan_half, aspect, W, H)
+    vi_o, pi_o = O.camera(eye, center, up=up, tan_half_fovy=tan_half, aspect=aspect)
+    cam = vk.camera_look_at(eye, center, up=up, fov_deg=60.0, aspect=aspect)
+    vi_l, pi_l = np.ctypeslib.as_array(cam.view_inverse), np.ctypeslib.as_array(cam.proj_inverse)
+    rays = O.raygen(vi_o, pi_o, W, H).reshape(H, W, 8)
+    for name, o, d in [("oracle raygen", rays[0, 0, :3], rays[..., 4:7].astype(np.float64)),
+                       ("oracle matrices", *_raygen_f64(vi_o, pi_o, W, H)),
+                       ("library matrices", *_raygen_f64(vi_l, pi_l, W, H))]:
+        d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+        assert np.allclose(o, eye, atol=1e-3), name
+        err = np.degrees(np.arccos(np.clip((d * ref).sum(-1), -1, 1)))
+        assert err.max() < 1e-3, (name, err.max())  # FP32 matrices: ~1e-5 degrees
+        # image orientation, stated as signs
+        assert (d[0] @ u > 0).all() and (d[-1] @ u < 0).all(), name          # row 0 is the top
+        assert (d[:, 0] @ s < 0).all() and (d[:, -1] @ s > 0).all(), name    # column 0 is the left
+    # the mirrored alternatives are far from every pixel's ray
+    mirrored_y, _, _ = _closed_form_dirs(eye, center, up, tan_half, aspect, W, H)
+    mirrored_y = mirrored_y[::-1]
+    d = rays[..., 4:7].astype(np.float64)
+    assert np.degrees(np.arccos(np.clip((d[0] * mirrored_y[0]).sum(-1), -1, 1))).min() > 30.0
+
+
 # ------------------------------------------------------------------ oracle traversal
 def _scene(kind):
     curves = scenes.c0_curves(300, 16, seed=11)

@@ -275,6 +275,58 @@ def test_prhi_fp32_vs_fp64_accuracy(dist):
     assert np.median(ang) <= 2.5e-4 and np.percentile(ang, 99) <= 3e-3
 
 
+def _angles(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    c = np.abs((a * b).sum(1)) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1))
+    return np.arccos(np.clip(c, -1.0, 1.0))
+
+
+def test_reference_normal_formula_bounded():
+    """The product's shading normal against the reference's literal formula
+    normalize(o + t*d - SampleCurvePoint(curve, t)) of hair_intersection.rint:74-76 (oracle
+    flag F_REF_NORMAL). Both are the same vector hit - P(u); the product evaluates it in the
+    ray-centric frame. On the C0 hair: identical hits, t and u bits; the two normals within
+    2e-2 rad of each other at most; against the FP64 intersector the product's normal is the
+    closer one at the median; and the C0 frame shaded with either normal differs by at least
+    50 dB PSNR."""
+    import scenes
+    curves = scenes.c0_curves()
+    S = O.Scene()
+    S.add(O.MODE_PHANTOM, curves, per_point=True)
+    rays = scenes.targeted_rays(curves, 8000, seed=5, dist=(20.0, 400.0))
+    h = S.trace(rays)
+    hr = S.trace(rays, flags=O.F_REF_NORMAL)
+    hit = h["prim"] != 0xFFFFFFFF
+    assert hit.sum() > 2000
+    assert np.array_equal(h["prim"], hr["prim"]) and np.array_equal(h["inst"], hr["inst"])
+    for k in ("t", "u"):
+        assert np.array_equal(h[k][hit].view(np.uint32), hr[k][hit].view(np.uint32)), k
+    between = _angles(h["n"][hit], hr["n"][hit])
+    e_prod, e_ref = [], []
+    for i in np.nonzero(hit)[0][:1000]:
+        r64 = O.prhi_f64(rays[i], curves[h["prim"][i]], per_point=True)
+        if r64[0] <= 0:
+            continue
+        e_prod.append(_angles(h["n"][i][None], r64[5:8][None])[0])
+        e_ref.append(_angles(hr["n"][i][None], r64[5:8][None])[0])
+    e_prod, e_ref = np.array(e_prod), np.array(e_ref)
+    print(f"product vs reference normal: median {np.median(between):.2g} p99 {np.percentile(between, 99):.2g} "
+          f"max {between.max():.2g} rad; vs FP64 median product {np.median(e_prod):.2g}, "
+          f"reference formula {np.median(e_ref):.2g} rad")
+    assert between.max() < 2e-2 and np.median(between) < 1e-3
+    assert len(e_prod) > 800 and np.median(e_prod) <= np.median(e_ref)
+    W = H = 256
+    vi, pi = O.camera((0.0, 50.0, 400.0), (0.0, 30.0, 0.0))
+    img, ids, _, _ = S.render(vi, pi, W, H, shadows=1)
+    img_r, ids_r, _, _ = S.render(vi, pi, W, H, flags=O.F_REF_NORMAL, shadows=1)
+    assert np.array_equal(ids, ids_r)
+    mse = float(np.mean((img.astype(np.float64) - img_r) ** 2))
+    psnr = float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)
+    print(f"C0 frame, product vs reference normal: PSNR {psnr:.1f} dB over {(ids != 0xFFFFFFFF).sum()} hits")
+    assert psnr >= 50.0
+
+
 # ------------------------------------------------------------------ LSS / triangles
 def test_lss_capsule_analytic():
     out = np.zeros(6, np.float64)
