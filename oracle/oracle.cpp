@@ -972,6 +972,8 @@ int orc_lss(const float* ray, const float* cap, int cap_a, int fp64, double* out
     return 0;
 }
 
+float orc_det2(float a, float b, float c, float d) { return Det2Kahan(a, b, c, d); }
+
 int orc_triangle(const float* ray, const float* v /*9*/, float* out /*hit, t, b1, b2*/) {
     TriOut<float> o{};
     bool ok = IntersectTriangle(mk(ray[0], ray[1], ray[2]), mk(ray[4], ray[5], ray[6]), mk(v[0], v[1], v[2]),

@@ -131,6 +131,20 @@ inline int max_dim(V3<float> v) {
 template <class T>
 inline T comp(V3<T> v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
+// a*b - c*d in FP32 by Kahan's algorithm with fused multiply-adds (relative error <= 2u,
+// Jeannerod, Louvet and Muller, Math. Comp. 2013): w = c*d rounded, e = w - c*d exactly,
+// f = a*b - w rounded once, result f + e. A nonzero exact value keeps its sign and an exact
+// zero stays zero, which is what the watertight test's re-evaluation of a zero edge function
+// needs (the paper re-evaluates in double; this build keeps the triangle test in FP32).
+inline float Det2Kahan(float a, float b, float c, float d) {
+    const float w = c * d;
+    const float e = std::fma(-c, d, w);
+    const float f = std::fma(a, b, -w);
+    return f + e;
+}
+
+// Watertight ray/triangle (Woop, Benthin, Wald, JCGT 2013), no facing cull; zero edge
+// functions are re-evaluated by Det2Kahan.
 inline bool IntersectTriangle(V3<float> o, V3<float> d, V3<float> v0, V3<float> v1, V3<float> v2, float tmin,
                               float tmax, TriOut<float>& out) {
     int kz = max_dim(d);
@@ -155,9 +169,9 @@ inline bool IntersectTriangle(V3<float> o, V3<float> d, V3<float> v0, V3<float> 
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
-        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
-        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+        U = Det2Kahan(Cx, By, Cy, Bx);
+        V = Det2Kahan(Ax, Cy, Ay, Cx);
+        W = Det2Kahan(Bx, Ay, By, Ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;

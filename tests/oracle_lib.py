@@ -63,6 +63,7 @@ def lib():
         "orc_cone": ([P, f32, P, f32, P], i32),
         "orc_lss": ([P, P, i32, i32, P], i32),
         "orc_triangle": ([P, P, P], i32),
+        "orc_det2": ([f32, f32, f32, f32], f32),
         "orc_perp_stark": ([P, P], i32),
         "orc_shade": ([P, f32, P], i32),
         "orc_debug_color": ([ctypes.c_int32, P], i32),

@@ -77,10 +77,19 @@ def test_accurate_render_matches_oracle(gpu_ctx, oracle):
         assert np.array_equal(ids, rids) and np.array_equal(img[..., :3].view(np.uint32), ref.view(np.uint32))
     plain, _ = G.render(gpu_ctx, tlas, cam, W, H)
     assert not np.array_equal(plain, img)  # the shading follows the re-evaluated normals
-    # a mixed TLAS has no persistent pipeline for it
+    # a mixed TLAS takes the deferred pipeline with one thread per queued ray
     caps = oracle.generate_lss(scenes.lines_from_curves(curves[:100], 2))
     lss = G.build_blas(gpu_ctx, V.GEOMETRY_LSS, caps)
     mixed = V.TopLevelAccelerationStructure(gpu_ctx, [blas, lss])
-    with pytest.raises(V.VkhrtError) as e:
-        G.render(gpu_ctx, mixed, cam, W, H, flags=V.RENDER_SHADOWS | V.RENDER_ACCURATE_NORMALS)
-    assert e.value.code == -4
+    S.add(oracle.MODE_LSS, caps)
+    img, ids = G.render(gpu_ctx, mixed, cam, W, H, flags=V.RENDER_SHADOWS | V.RENDER_ACCURATE_NORMALS)
+    ref, rids, _, _ = S.render(vi, pi, W, H, shadows=1, flags=oracle.F_ACCURATE)
+    assert np.array_equal(ids, rids) and np.array_equal(img[..., :3].view(np.uint32), ref.view(np.uint32))
+    # without the deferred pipeline (persistent option off) the accurate re-run is refused
+    gpu_ctx.set_option(V.OPTION_PERSISTENT, 0)
+    try:
+        with pytest.raises(V.VkhrtError) as e:
+            G.render(gpu_ctx, mixed, cam, W, H, flags=V.RENDER_SHADOWS | V.RENDER_ACCURATE_NORMALS)
+        assert e.value.code == -4
+    finally:
+        gpu_ctx.set_option(V.OPTION_PERSISTENT, 1)

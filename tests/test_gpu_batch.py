@@ -248,15 +248,14 @@ def test_batch_errors(gpu_ctx, oracle, scene):
     b_ph = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
     b_lss = G.build_blas(gpu_ctx, V.GEOMETRY_LSS, caps)
     mixed = V.TopLevelAccelerationStructure(gpu_ctx, [b_ph, b_lss])
-    with pytest.raises(V.VkhrtError) as e:
-        _gpu_batch(gpu_ctx, mixed, views, 1, 1, V.BATCH_FP64_POLISH)
-    assert e.value.code == -4
+    _gpu_batch(gpu_ctx, mixed, views, 1, 1, V.BATCH_FP64_POLISH)  # polish of a mixed TLAS is served
 
 
-@pytest.mark.parametrize("flags", [0, 3], ids=["plain", "sorted+graph"])
+@pytest.mark.parametrize("flags", [0, 3, 4], ids=["plain", "sorted+graph", "polish"])
 def test_mixed_tlas_batch_matches_oracle(gpu_ctx, oracle, flags):
     """Phantom hair, LSS capsules and a triangle mesh in one TLAS (the per-instance dispatch,
-    one lane per path) against the oracle, with material tints per instance."""
+    one lane per path) against the oracle, with material tints per instance; with the FP64
+    polish of the phantom instance's grazing roots too."""
     import gpu_util as G
     import scenes
     import vkhrt_amd as V
@@ -280,11 +279,13 @@ def test_mixed_tlas_batch_matches_oracle(gpu_ctx, oracle, flags):
     tlas = V.TopLevelAccelerationStructure(gpu_ctx, [b_ph, b_lss, b_tri], stream=G.stream_ptr())
     torch.cuda.synchronize()
     views = _views(oracle, scenes.DENSE_VIEWS)
-    gflags = (V.BATCH_SORT_MATERIALS | V.BATCH_SORT_RAYS | V.BATCH_GRAPH) if flags else 0
+    gflags = (V.BATCH_SORT_MATERIALS | V.BATCH_SORT_RAYS | V.BATCH_GRAPH) if flags & 3 else 0
+    gflags |= V.BATCH_FP64_POLISH if flags & 4 else 0
     print(f"[mixed batch] flags {gflags}: 3 views, 3 bounces", flush=True)
     for seed in (4, 5):
         gi, gp, (_, nb, ns, _) = _gpu_batch(gpu_ctx, tlas, views, 3, seed, gflags)
-        ri, rp, rnb, rns, _ = S.render_batch(views, bounces=3, seed=seed, env=ENV, albedo=TINTS)
+        ri, rp, rnb, rns, _ = S.render_batch(views, bounces=3, seed=seed, env=ENV, albedo=TINTS,
+                                             flags=oracle.F_POLISH if flags & 4 else 0)
         assert (nb, ns) == (rnb, rns) and nb > 500 and ns > 500
         _assert_same(gi, gp, ri, rp, f"mixed batch seed {seed}")
     tlas.close()

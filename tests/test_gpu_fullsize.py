@@ -189,7 +189,8 @@ def test_c2_full_frame(gpu_ctx, oracle, c1, mode):
     sel = np.nonzero(rids.reshape(-1) != 0xFFFFFFFF)[0]
     r = np.ascontiguousarray(rays[sel[np.linspace(0, sel.size - 1, 2000).astype(np.int64)]])
     h = G.trace(ctx, tlas, r)
-    acc = accuracy.lss(r, h, prims) if mode == "lss" else accuracy.dots(r, h, oracle.generate_dots(lines))
+    acc = (accuracy.lss(r, h, prims, limit=2000) if mode == "lss"
+           else accuracy.dots(r, h, oracle.generate_dots(lines), limit=2000))
     _log(f"[c2 {mode}] vs FP64: {acc}")
     assert acc["hits"] > 1500 and acc["t_rel_max"] <= 1e-5
     tlas.close()

@@ -149,11 +149,12 @@ def test_instances_with_transforms(gpu_ctx, G, oracle, curves):
     _compare_hits(h, ref, "instances")
 
 
-@pytest.mark.parametrize("count", [3, 18])
+@pytest.mark.parametrize("count", [3, 18, 64])
 def test_render_many_instances(gpu_ctx, G, oracle, curves, count):
-    """Renders of 3 and 18 translated instances with shadows and the FP64 polish: the render
-    kernels that stage the BLAS fields of up to 16 instances in LDS, and those that read
-    them per lane beyond that, give the oracle's primitive ids."""
+    """Renders of 3, 18 and 64 translated instances with shadows and the FP64 polish: the
+    render kernels that stage the BLAS fields of up to 16 instances in LDS, and those that
+    read them per lane beyond that, give the oracle's primitive ids (merge keys carry global
+    primitive ids, so the instance count is not limited)."""
     import vkhrt_amd as V
     S = oracle.Scene()
     blases = []
@@ -560,6 +561,11 @@ def test_count_candidates_matches_oracle(gpu_ctx, G, oracle, curves, mode):
     sref = S.trace(srays, flags=oracle.F_ANY_HIT)
     sh = G.trace(gpu_ctx, tlas, srays, flags=V.TRACE_TERMINATE_ON_FIRST_HIT)
     assert np.array_equal(sh["prim"] == 0xFFFFFFFF, sref["prim"] == 0xFFFFFFFF)
+    # the count runs over [tmin, t_closest]: an any-hit t depends on the traversal order, so the
+    # shadow rays are counted against their closest hits (equal on both sides)
+    sref = S.trace(srays)
+    sh = G.trace(gpu_ctx, tlas, srays)
+    _compare_hits(sh, sref, f"{mode} shadow rays, closest hit")
     for what, r, hh, rr in (("primary", rays, h, ref), ("shadow", srays, sh, sref)):
         d_r, d_h = G.dev(r), G.dev(hh.view(np.float32).reshape(-1))
         counts = torch.zeros(r.shape[0], dtype=torch.int32, device="cuda")
@@ -571,3 +577,76 @@ def test_count_candidates_matches_oracle(gpu_ctx, G, oracle, curves, mode):
         print(f"{mode} {what}: {r.shape[0]} rays, N* mean GPU {g.mean():.4f} oracle {o.mean():.4f}")
         assert g.sum() > 0
         assert np.array_equal(g, o), f"{mode} {what}: counts differ on {(g != o).sum()} rays"
+
+
+def test_polish_trace_64_instances(gpu_ctx, G, oracle, curves):
+    """The C0 hair split into 64 BLASes (one instance each) traced with grazing rays and the
+    FP64 polish: the deferred candidates of every instance merge by global primitive id and
+    equal the oracle's in-place polish bit for bit, polished roots included."""
+    import scenes
+    import vkhrt_amd as V
+    parts = np.array_split(np.arange(curves.shape[0]), 64)
+    S = oracle.Scene()
+    blases = []
+    for idx in parts:
+        sub = np.ascontiguousarray(curves[idx])
+        S.add(oracle.MODE_PHANTOM, sub, per_point=True)
+        blases.append(G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, sub, per_point=True))
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, blases)
+    rays = np.concatenate([scenes.grazing_rays(curves, 6000, seed=41), scenes.targeted_rays(curves, 4000, seed=42)])
+    ref = S.trace(rays, flags=oracle.F_POLISH)
+    h = G.trace(gpu_ctx, tlas, rays, flags=V.TRACE_FP64_POLISH)
+    _compare_hits(h, ref, "64-instance polish")
+    assert np.array_equal(h["v"].view(np.uint32), ref["v"].view(np.uint32))  # polished flag
+    npol = int((ref["v"] != 0).sum())
+    print(f"64 instances: {len(np.unique(ref['inst'][ref['prim'] != 0xFFFFFFFF]))} hit, {npol} polished roots")
+    assert npol >= 50
+
+
+@pytest.mark.parametrize("extra", ["polish", "polish+area+accurate"])
+def test_mixed_tlas_render_polish(gpu_ctx, G, oracle, curves, extra):
+    """A TLAS mixing phantom hair (per-point radius), LSS capsules, DOTS quads and a triangle
+    mesh, rendered with the FP64 polish (the deferred pipeline with one thread per queued
+    ray): bit for bit against the oracle, with an area light and accurate hits too."""
+    import scenes
+    import vkhrt_amd as V
+    xf = np.array([1, 0, 0, 4.0, 0, 1, 0, -2.0, 0, 0, 1, 3.0], np.float32)
+    lines = scenes.lines_from_curves(curves[4000:6000], 4)
+    caps = oracle.generate_lss(lines)
+    verts, tri = scenes.uv_sphere(12.0, 16, 32, (0.0, 96.0, 20.0))
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves[:4000], per_point=True)
+    S.add(oracle.MODE_LSS, caps, transform=xf)
+    S.add(oracle.MODE_DOTS, scenes.lines_from_curves(curves[6000:8000], 4), per_point=True)
+    S.add_mesh(verts, tri)
+    blases = [G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves[:4000], per_point=True),
+              G.build_blas(gpu_ctx, V.GEOMETRY_LSS, caps, transform=xf),
+              G.build_blas(gpu_ctx, V.GEOMETRY_DOTS, scenes.lines_from_curves(curves[6000:8000], 4), per_point=True)]
+    dv, di = G.dev(verts), G.dev(tri)
+    blases.append(V.BottomLevelAccelerationStructure(gpu_ctx, V.GEOMETRY_TRIANGLES, dv.data_ptr(), tri.shape[0],
+                                                     indices_ptr=di.data_ptr(), vertex_count=verts.shape[0],
+                                                     stream=G.stream_ptr()))
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, blases)
+    W, H = 256, 256
+    cam = V.camera_look_at((0.0, 112.0, 140.0), (0.0, 100.0, 0.0), fov_deg=30.0, aspect=1.0)
+    vi, pi = G.cam_arrays(cam)
+    flags, oflags, kw, okw = V.RENDER_SHADOWS | V.RENDER_FP64_POLISH, oracle.F_POLISH, {}, {}
+    if extra != "polish":
+        light = np.array([[0.0, 260.0, 0.0], [60.0, 0.0, 0.0], [0.0, 0.0, 60.0]], np.float32)
+        kw["area_light"] = V.area_light(light[0], light[1], light[2], samples=2, seed=3)
+        okw = {"area_light": light, "light_samples": 2, "light_seed": 3}
+        flags |= V.RENDER_ACCURATE_NORMALS
+        oflags |= oracle.F_ACCURATE
+    img = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    ids = torch.zeros(W * H, dtype=torch.int32, device="cuda")
+    print(f"[mixed render] {extra}", flush=True)
+    gpu_ctx.render(tlas, cam, W, H, img.data_ptr(), flags=flags, prim_ids_ptr=ids.data_ptr(), stream=G.stream_ptr(),
+                   **kw)
+    torch.cuda.synchronize()
+    gpu_ctx.synchronize(G.stream_ptr())
+    got = img.cpu().numpy().reshape(H, W, 4)[..., :3]
+    gids = ids.cpu().numpy().view(np.uint32).reshape(H, W)
+    ref, rids, _, _ = S.render(vi, pi, W, H, flags=oflags, shadows=1, **okw)
+    assert (rids != 0xFFFFFFFF).sum() > 1000
+    assert np.array_equal(gids, rids)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))

@@ -328,6 +328,37 @@ def test_reference_normal_formula_bounded():
 
 
 # ------------------------------------------------------------------ LSS / triangles
+def test_det2_kahan_sign_and_error():
+    """The FP32 re-evaluation of a zero edge function in the watertight triangle test
+    (Det2Kahan: a*b - c*d by Kahan's fused multiply-add algorithm) against exact rational
+    arithmetic: the sign is always the exact one, an exact zero gives zero, and the relative
+    error is at most 2u = 2^-23 (Jeannerod, Louvet, Muller 2013). Cases include products that
+    round to the same float, where a*b - c*d in plain FP32 is 0 but the exact value is not."""
+    from fractions import Fraction
+    L = O.lib()
+    rng = np.random.default_rng(77)
+    u = 2.0 ** -24
+    cases = []
+    for _ in range(4000):
+        a, b = rng.uniform(-200, 200, 2).astype(np.float32)
+        c = np.float32(a * (1 + rng.integers(-3, 4) * 2.0 ** -23))
+        d = np.float32(b * (1 + rng.integers(-3, 4) * 2.0 ** -23))
+        cases.append((a, b, c, d))
+        cases.append((a, b, a, b))  # exact zero
+        cases.append(tuple(rng.uniform(-1e3, 1e3, 4).astype(np.float32)))
+    ties = 0
+    for a, b, c, d in cases:
+        a, b, c, d = (np.float32(x) for x in (a, b, c, d))
+        exact = Fraction(float(a)) * Fraction(float(b)) - Fraction(float(c)) * Fraction(float(d))
+        got = L.orc_det2(float(a), float(b), float(c), float(d))
+        plain = np.float32(np.float32(a * b) - np.float32(c * d))
+        ties += int(plain == 0 and exact != 0)
+        assert (got > 0) == (exact > 0) and (got < 0) == (exact < 0), (a, b, c, d, got, exact)
+        if exact != 0:
+            assert abs(Fraction(got) - exact) <= 2 * u * abs(exact), (a, b, c, d)
+    assert ties > 100  # the plain FP32 difference loses the sign in these cases
+
+
 def test_lss_capsule_analytic():
     out = np.zeros(6, np.float64)
     L = O.lib()

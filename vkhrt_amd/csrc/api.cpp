@@ -174,6 +174,21 @@ void set_work_order(vk::Camera& cm, uint32_t order) {
 
 bool is_phantom(const vkhrt_tlas* t) { return t->variant == vk::kVarPhConst || t->variant == vk::kVarPhPoint; }
 
+// Whether any instance is a phantom-curve BLAS (the FP64 polish has work).
+bool has_phantom(const vkhrt_tlas* t) {
+    for (uint32_t m : t->modes)
+        if (m == VKHRT_GEOMETRY_PHANTOM_CURVES) return true;
+    return false;
+}
+
+// Primitive ids a BLAS's hits report: one per record, four triangles per DOTS piece, one per
+// grid cell of a voxel BLAS.
+uint64_t blas_id_range(const vkhrt_blas* b) {
+    if (b->mode == VKHRT_GEOMETRY_DOTS) return uint64_t(b->records) * 4u;
+    if (b->mode == VKHRT_GEOMETRY_VOXELS) return uint64_t(uint32_t(b->vres[0])) * uint32_t(b->vres[1]) * uint32_t(b->vres[2]);
+    return b->records;
+}
+
 // Kernel variant for a TLAS and the polish flag.
 int trace_variant(const vkhrt_tlas* t, bool polish) {
     int v = t->variant;
@@ -203,7 +218,7 @@ struct Timer {
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
     Timer(vkhrt_context* c_, hipStream_t s_, const char* n) : c(c_), s(s_), name(n) {
-        if (c->profiling) {
+        if (c->profiling && name) {  // null name: not timed (the allocation-free trace creates no events)
             hipEventCreate(&a);
             hipEventCreate(&b);
             hipEventRecord(a, s);
@@ -945,11 +960,14 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
     if (!c || !out || (count && !blases)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     std::vector<vk::InstanceDev> inst(count);
+    uint64_t gid_total = 0;  // global primitive ids of the merge keys, instance by instance
     for (uint32_t i = 0; i < count; ++i) {
         const vkhrt_blas* b = blases[i];
         if (!b) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null blas");
         vk::InstanceDev& I = inst[i];
         std::memset(&I, 0, sizeof(I));
+        I.blas.gbase = uint32_t(std::min<uint64_t>(gid_total, 0xFFFFFFFFull));
+        gid_total += blas_id_range(b);
         I.blas.prims = b->bvh.prims;
         I.blas.aux = b->bvh.aux;
         I.blas.prim_ids = b->bvh.prim_ids;
@@ -1028,9 +1046,11 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
                                                            : vk::kVarPhConst;
             v = (v < 0 || v == vi) ? vi : vk::kVarMixed;
             t->max_depth = std::max(t->max_depth, b->bvh.depth);
-            if (b->records >= (1u << 26)) t->polish_ok = false;
+            // the compacted phantom lanes queue (lane, leaf slot) pairs in 32 bits
+            if (b->mode == VKHRT_GEOMETRY_PHANTOM_CURVES && b->records >= (1u << vk::kCompactSlotBits))
+                t->polish_ok = false;
         }
-        if (count > 31) t->polish_ok = false;
+        if (gid_total >= 0xFFFFFFFFull) t->polish_ok = false;  // merge keys need every global id below 2^32 - 1
         if (t->max_depth > kStackEntries[3]) {
             vkhrt_tlas_destroy(t);
             return fail(c, VKHRT_ERROR_UNSUPPORTED, "BVH deeper than the largest traversal stack (32 levels)");
@@ -1058,34 +1078,59 @@ void vkhrt_tlas_destroy(vkhrt_tlas* t) {
     delete t;
 }
 
-int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits,
-                     uint32_t flags, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Scratch of one trace call: the work counter, the 8 per-XCD counters, and (with FP64 polish)
+// the merge keys and candidate queue.
+struct TraceScratch {
+    uint32_t* work;
+    uint32_t* xcd_work;
+    uint32_t* defer_count;
+    uint4* defer;
+    uint32_t defer_cap;
+    unsigned long long* keys;
+    vk::HitRec* cand_hits;
+    unsigned long long* cand_keys;
+};
+
+int check_trace_args(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits,
+                     uint32_t flags) {
     if (!c || !t) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null argument");
     if (count && (!rays || !hits)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null rays or hits");
     if (flags & ~(VKHRT_TRACE_TERMINATE_ON_FIRST_HIT | VKHRT_TRACE_FP64_POLISH | VKHRT_TRACE_ACCURATE_NORMALS))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "unknown trace flags");
+    if ((flags & VKHRT_TRACE_FP64_POLISH) && !t->polish_ok)
+        return fail(c, VKHRT_ERROR_UNSUPPORTED,
+                    "FP64 polish needs phantom BLASes of < 2^26 records and < 2^32 - 1 primitive ids per TLAS");
+    return VKHRT_SUCCESS;
+}
+
+// The trace launches of vkhrt_trace_rays (arguments checked).
+int trace_launch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits,
+                 uint32_t flags, hipStream_t st, const TraceScratch& s, bool timed) {
     const bool accurate = (flags & VKHRT_TRACE_ACCURATE_NORMALS) && !(flags & VKHRT_TRACE_TERMINATE_ON_FIRST_HIT);
     flags &= ~uint32_t(VKHRT_TRACE_ACCURATE_NORMALS);
-    hipStream_t st = pick(c, stream);
     // persistent phantom kernels compact intersection work through keys that need polish_ok
     const bool persistent = c->persistent && (t->polish_ok || !is_phantom(t));
-    vk::TraceParams tp{t->dev, t->count, flags, c->status, persistent ? c->counters + 3 : nullptr, c->refill_min};
+    vk::TraceParams tp{t->dev, t->count, flags, c->status, persistent ? s.work : nullptr, c->refill_min};
     tp.blocks_per_cu = c->blocks_per_cu;
     tp.xcd_chunk = c->xcd_chunk;
-    tp.xcd_work = c->counters + vk::kXcdCounters + 16 * vk::kXcdStride;
+    tp.xcd_work = s.xcd_work;
     const bool polish = (flags & VKHRT_TRACE_FP64_POLISH) != 0;
     if (polish) {
-        if (!t->polish_ok)
-            return fail(c, VKHRT_ERROR_UNSUPPORTED, "FP64 polish needs < 2^26 primitives per BLAS and <= 31 instances");
-        int rc = ensure_polish_scratch(c, count);
-        if (rc != VKHRT_SUCCESS) return rc;
-        set_polish(c, tp);
-        if (!tp.work) tp.work = c->counters + 3;  // the deferred pipeline runs on persistent kernels
+        tp.defer = s.defer;
+        tp.defer_count = s.defer_count;
+        tp.defer_cap = s.defer_cap;
+        tp.keys = s.keys;
+        tp.cand_hits = s.cand_hits;
+        tp.cand_keys = s.cand_keys;
+        if (!tp.work) tp.work = s.work;  // the deferred pipeline runs on persistent kernels
     }
-    Timer tm(c, st, "trace_rays");
+    Timer tm(c, st, timed ? "trace_rays" : nullptr);
     hipError_t e = vk::launch_trace_rays(tp, reinterpret_cast<const float*>(rays), count,
-                                         reinterpret_cast<vk::HitRec*>(hits), nullptr,
-                                         trace_variant(t, (flags & VKHRT_TRACE_FP64_POLISH) != 0),
+                                         reinterpret_cast<vk::HitRec*>(hits), nullptr, trace_variant(t, polish),
                                          stack_kind_for(c, t), st);
     if (e == hipSuccess && accurate)
         e = vk::launch_accurate_hits(tp, reinterpret_cast<const float*>(rays), count, reinterpret_cast<vk::HitRec*>(hits),
@@ -1093,6 +1138,23 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     tm.stop();
     if (e != hipSuccess) return hip_fail(c, e, "trace launch");
     return VKHRT_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits,
+                     uint32_t flags, void* stream) {
+    int rc = check_trace_args(c, t, rays, count, hits, flags);
+    if (rc != VKHRT_SUCCESS) return rc;
+    if (flags & VKHRT_TRACE_FP64_POLISH) {
+        rc = ensure_polish_scratch(c, count);
+        if (rc != VKHRT_SUCCESS) return rc;
+    }
+    const TraceScratch s{c->counters + 3, c->counters + vk::kXcdCounters + 16 * vk::kXcdStride, c->counters + 4,
+                         c->defer, c->defer_cap, c->keys, c->cand_hits, c->cand_keys};
+    return trace_launch(c, t, rays, count, hits, flags, pick(c, stream), s, true);
 }
 
 int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam, const vkhrt_render_desc* d,
@@ -1109,8 +1171,8 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     const uint32_t S = al ? al->samples : 1u;
     if (al) {
         if (S < 1 || S > 16) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "area light samples must be 1..16");
-        if (t->variant == vk::kVarMixed || !c->persistent || (is_phantom(t) && !t->polish_ok))
-            return fail(c, VKHRT_ERROR_UNSUPPORTED, "area lights need the persistent kernels and a single-mode TLAS");
+        if (!c->persistent || (is_phantom(t) && !t->polish_ok))
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "area lights need the deferred render pipeline (persistent option on)");
         if (npix * S > (size_t(1) << 31)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "too many shadow rays");
     }
     int rc = ensure_render_scratch(c, npix * S);
@@ -1154,17 +1216,17 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     rp.vis = c->vis;
     rp.live = c->live;
     if (d->flags & VKHRT_RENDER_FP64_POLISH) {
-        if (!t->polish_ok || t->variant == vk::kVarMixed)
+        if (!t->polish_ok)
             return fail(c, VKHRT_ERROR_UNSUPPORTED,
-                        "render FP64 polish needs a single-mode TLAS, < 2^26 primitives per BLAS, <= 31 instances");
+                        "render FP64 polish needs phantom BLASes of < 2^26 records and < 2^32 - 1 primitive ids per TLAS");
         rc = ensure_polish_scratch(c, npix * S);
         if (rc != VKHRT_SUCCESS) return rc;
         set_polish(c, rp.tp);
         if (!rp.tp.work) rp.tp.work = c->counters + 1;
     }
     if (d->flags & VKHRT_RENDER_ACCURATE_NORMALS) {
-        if (!rp.tp.work || t->variant == vk::kVarMixed)
-            return fail(c, VKHRT_ERROR_UNSUPPORTED, "accurate normals need the persistent kernels and a single-mode TLAS");
+        if (!rp.tp.work)
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "accurate normals need the deferred render pipeline (persistent option on)");
         rp.accurate = 1;
     }
     const bool neural = (d->flags & VKHRT_RENDER_NEURAL_SHADOWS) && rp.shadows && !rp.debug_palette &&
@@ -1172,8 +1234,8 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     if (neural) {
         if (!d->shadow_mlp) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "neural shadows need shadow_mlp");
         if (d->shadow_mlp->ctx != c) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "shadow_mlp of another context");
-        if (!rp.tp.work || t->variant == vk::kVarMixed)
-            return fail(c, VKHRT_ERROR_UNSUPPORTED, "neural shadows need the persistent kernels and a single-mode TLAS");
+        if (!rp.tp.work)
+            return fail(c, VKHRT_ERROR_UNSUPPORTED, "neural shadows need the deferred render pipeline (persistent option on)");
         if (!(d->shadow_variance_threshold >= 0.0f))
             return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "variance threshold must be >= 0");
         rc = ensure_neural_scratch(c, npix * S);
@@ -1255,11 +1317,9 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     if (d->bounces > VKHRT_BATCH_MAX_BOUNCES) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bounces above the maximum");
     if (d->material_count > VKHRT_BATCH_MAX_MATERIALS || (d->material_count && !d->material_albedo))
         return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "bad material table");
-    if (t->variant == vk::kVarMixed && (d->flags & VKHRT_BATCH_FP64_POLISH))
-        return fail(c, VKHRT_ERROR_UNSUPPORTED, "batched FP64 polish needs a TLAS of one geometry mode");
-    if (is_phantom(t) && !t->polish_ok)
+    if ((is_phantom(t) || (has_phantom(t) && (d->flags & VKHRT_BATCH_FP64_POLISH))) && !t->polish_ok)
         return fail(c, VKHRT_ERROR_UNSUPPORTED,
-                    "batched phantom tracing needs < 2^26 primitives per BLAS and <= 31 instances");
+                    "batched phantom tracing needs phantom BLASes of < 2^26 records and < 2^32 - 1 primitive ids per TLAS");
     uint64_t total = 0;
     for (uint32_t i = 0; i < view_count; ++i) {
         if (!views[i].image) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "null view image");
@@ -1304,7 +1364,7 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     if (e == hipSuccess) e = hipEventRecord(c->b_upload, st);
     if (e != hipSuccess) return hip_fail(c, e, "batch upload");
 
-    const bool polish = (d->flags & VKHRT_BATCH_FP64_POLISH) && is_phantom(t);
+    const bool polish = (d->flags & VKHRT_BATCH_FP64_POLISH) && has_phantom(t);
     vk::BatchParams bp{};
     bp.tp = vk::TraceParams{t->dev, t->count, polish ? uint32_t(vk::kTraceFp64Polish) : 0u, c->status,
                             c->b_counters, c->refill_min};

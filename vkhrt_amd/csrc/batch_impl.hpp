@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_batch_trace(BatchParams bp, uin
             HitRec h;
             finish_hit(bp.tp, o, d, best, h);
             bp.hits[qi] = h;
-            if (G::kPolish) bp.tp.keys[qi] = best_key(best);
+            if (G::kPolish) bp.tp.keys[qi] = best_key(bp.tp, best);
         },
         bp.counters + batch_xcd(b, 0));
 }
@@ -230,6 +230,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_batch_trace_flat(BatchParams bp
     HitRec h;
     finish_hit(bp.tp, o, d, best, h);
     bp.hits[qi] = h;
+    if (bp.tp.keys) bp.tp.keys[qi] = best_key(bp.tp, best);
 }
 
 // Candidate sources of the polish kernels: queue slot -> world ray.

@@ -56,6 +56,10 @@ struct Camera {
 
 struct MlpDev;
 
+// Compacted phantom lanes queue (lane, leaf slot) pairs as lane << kCompactSlotBits | slot, so
+// they serve phantom BLASes of fewer than 2^kCompactSlotBits records.
+constexpr uint32_t kCompactSlotBits = 26;
+
 // Context work counters (RenderParams::counters): words [0, 8) per frame; from kXcdCounters
 // the per-XCD counters of the render's primary and shadow traversal launches and of
 // vkhrt_trace_rays's launch, 8 each, kXcdStride words (one 128-byte line) apart.

@@ -50,6 +50,8 @@ struct BlasDev {
     float vorigin[3];        // voxels: grid origin (object space)
     float vcell;             // voxels: cell edge at the traced level
     int32_t vres[3];         // voxels: cells per axis at the traced level
+    uint32_t gbase;          // per instance: first global primitive id (merge keys); the ids of
+                             // instance i are gbase(i) + [0, id range of its BLAS)
 };
 
 struct InstanceDev {

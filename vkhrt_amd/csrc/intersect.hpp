@@ -421,8 +421,11 @@ __device__ __forceinline__ bool lss_intersect(f3 o, f3 d, f3 a, float ra, f3 b, 
 }
 
 // ------------------------------------------------------------ DOTS triangles
-// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), no facing cull, double
-// fallback for zero edge functions. Per-ray shear constants come from TriRay.
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), no facing cull. The paper
+// re-evaluates zero edge functions in double; here they are re-evaluated in FP32 by Kahan's
+// fused multiply-add 2x2 determinant, whose relative error is at most 2u (Jeannerod, Louvet,
+// Muller, Math. Comp. 2013): a nonzero exact value keeps its sign and an exact zero stays
+// zero. Per-ray shear constants come from TriRay.
 struct TriRay {
     int kx, ky, kz;
     float Sx, Sy, Sz;
@@ -447,6 +450,13 @@ __device__ __forceinline__ TriRay make_tri_ray(f3 d) {
     R.Sz = 1.0f / cmp3(d, kz);
     return R;
 }
+// a*b - c*d: w = c*d rounded, e = w - c*d exactly, f = a*b - w rounded once; f + e.
+__device__ __forceinline__ float det2_kahan(float a, float b, float c, float d) {
+    const float w = c * d;
+    const float e = fmaf(-c, d, w);
+    const float f = fmaf(a, b, -w);
+    return f + e;
+}
 __device__ __forceinline__ bool tri_intersect(const TriRay& R, f3 o, f3 v0, f3 v1, f3 v2, float tmin, float tmax,
                                               float& t, float& b1, float& b2) {
     f3 A = sub(v0, o), B = sub(v1, o), C = sub(v2, o);
@@ -460,9 +470,9 @@ __device__ __forceinline__ bool tri_intersect(const TriRay& R, f3 o, f3 v0, f3 v
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
-        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
-        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+        U = det2_kahan(Cx, By, Cy, Bx);
+        V = det2_kahan(Ax, Cy, Ay, Cx);
+        W = det2_kahan(Bx, Ay, By, Ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;

@@ -49,14 +49,28 @@ struct Best {
     uint32_t extra;    // polished flag / DOTS triangle sub-index
 };
 
-// Merge key of a hit: t (positive float, so its bits order like the value), then instance
-// and primitive (ties resolve to the lower pair). Requires primitive < 2^27, instance < 31.
-__device__ __forceinline__ unsigned long long pack_key(float t, uint32_t inst, uint32_t pid) {
-    return (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | ((inst << 27) | pid);
+// Merge key of a hit: t (a non-negative float, so its bits order like the value), then the
+// hit's global primitive id gbase(instance) + primitive. Global ids order like (instance,
+// primitive) pairs, so equal t resolve to the lower pair. 0xFFFFFFFF (above every global id;
+// the TLAS build checks that) marks no hit.
+__device__ __forceinline__ unsigned long long pack_key(float t, uint32_t gid) {
+    return (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | gid;
 }
-__device__ __forceinline__ unsigned long long best_key(const Best& b) {
+__device__ __forceinline__ unsigned long long best_key(const TraceParams& tp, const Best& b) {
     return (static_cast<unsigned long long>(__float_as_uint(b.t)) << 32) |
-           (b.inst == 0xFFFFFFFFu ? 0xFFFFFFFFu : ((b.inst << 27) | b.pid));
+           (b.inst == 0xFFFFFFFFu ? 0xFFFFFFFFu : tp.inst[b.inst].blas.gbase + b.pid);
+}
+// Instance of a global primitive id: the last instance whose gbase is <= gid.
+__device__ __forceinline__ uint32_t gid_instance(const InstanceDev* inst, uint32_t ninst, uint32_t gid) {
+    uint32_t lo = 0, hi = ninst;  // gbase(lo) <= gid < gbase(hi)
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (inst[mid].blas.gbase <= gid)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 __device__ __forceinline__ bool better(float t, uint32_t inst, uint32_t pid, const Best& B) {
@@ -605,7 +619,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceParams tp, cons
     TravStats ts;
     const bool ok = trace_world<G, ANY, STACK, STATS>(tp, o, d, r0.w, r1.w, best, stk + threadIdx.x, i, &ts);
     if (STATS) stats[i] = make_uint2(ts.nodes, ts.prims);
-    if (tp.keys) tp.keys[i] = best_key(best);
+    if (tp.keys) tp.keys[i] = best_key(tp, best);
     if (!ok) atomicAdd(&tp.status[0], 1u);
     HitRec h;
     finish_hit(tp, o, d, best, h);
@@ -973,7 +987,6 @@ __device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkCl
 // so the closest hit and its tie rule do not depend on the processing order. A lane does not
 // move to its next instance, finish or take a new ray while pairs of its ray are queued.
 constexpr uint32_t kCompactQueue = 128;    // per wave: < 64 left after a round + <= 64 appended per step
-constexpr uint32_t kCompactSlotBits = 26;  // queue entry = lane << 26 | leaf slot
 constexpr uint32_t kNoHitLow = 0xFFFFFFFFu;
 constexpr uint32_t kCompactIterationCap = 1u << 24;
 
@@ -1165,8 +1178,16 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
                 } else {
                     const float4 pay = s_pay[threadIdx.x];
                     best.t = __uint_as_float(uint32_t(kk >> 32));
-                    best.inst = lo >> 27;
-                    best.pid = lo & ((1u << 27) - 1u);
+                    if constexpr (IM == kInstOne) {
+                        best.inst = 0;
+                    } else if constexpr (IM == kInstFew) {
+                        uint32_t k = 0;
+                        for (uint32_t j = 1; j < tp.ninst; ++j) k = few[j].gbase <= lo ? j : k;
+                        best.inst = k;
+                    } else {
+                        best.inst = gid_instance(tp.inst, tp.ninst, lo);
+                    }
+                    best.pid = lo - blas(best.inst).gbase;
                     best.u = pay.x;
                     best.a = pay.y;
                     best.b = pay.z;
@@ -1261,7 +1282,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
                     }
                     hit = hit && ph.t >= ptmin && ph.t <= ptmax;
                     if (hit) {
-                        k = pack_key(ph.t, pinst, B.prim_ids[slot]);
+                        k = pack_key(ph.t, B.gbase + B.prim_ids[slot]);
                         atomicMin(&s_key[os], k);
                     }
                 }
@@ -1358,7 +1379,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays_persistent(TracePara
             HitRec h;
             finish_hit(tp, o, d, best, h);
             hits[i] = h;
-            if (G::kPolish) tp.keys[i] = best_key(best);
+            if (G::kPolish) tp.keys[i] = best_key(tp, best);
         },
         tp.xcd_work);
 }
@@ -1457,7 +1478,7 @@ __device__ __forceinline__ bool eval_candidate(const TraceParams& tp, uint4 c, f
                          : prhi<kRadiusConstant, kPolishInline>(F, cv, I.blas.aux[c.z], I.blas.radius, ph);
     if (!hit || !(ph.t >= tmin && ph.t <= tmax)) return false;
     const uint32_t pid = I.blas.prim_ids[c.z];
-    key = pack_key(ph.t, c.y, pid);
+    key = pack_key(ph.t, I.blas.gbase + pid);
     f3 n = phantom_normal(dd, ph);
     if (!I.identity) {
         const float* x = I.xf;
@@ -1596,7 +1617,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderPara
             HitRec h;
             finish_hit(rp.tp, o, d, best, h);
             rp.hitrec[pix] = h;
-            if (G::kPolish) rp.tp.keys[pix] = best_key(best);
+            if (G::kPolish) rp.tp.keys[pix] = best_key(rp.tp, best);
         },
         rp.counters + kXcdCounters);
 }
@@ -1675,6 +1696,42 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
             rp.vis[queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
         },
         rp.counters + kXcdCounters + 8 * kXcdStride);
+}
+
+// The same two traversal passes for TLASes that mix geometry modes: one thread per queued
+// ray, trace_world's per-instance dispatch (with the FP64 polish candidates of phantom
+// instances deferred like the persistent kernels'); the queue lengths are read on the device.
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_render_primary_flat(RenderParams rp) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t r = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (r >= rp.counters[6]) return;
+    const uint32_t pix = rp.live[r];
+    const uint32_t y = pix < (1u << 24) ? udiv_small(pix, rp.cam.width, rp.cam.inv_w) : pix / rp.cam.width;
+    f3 o, d;
+    primary_ray(rp.cam, pix - y * rp.cam.width, y, o, d);
+    Best best;
+    if (!trace_world<G, false, STACK>(rp.tp, o, d, 0.001f, 10000.0f, best, stk + threadIdx.x, pix))
+        atomicAdd(&rp.tp.status[0], 1u);
+    HitRec h;
+    finish_hit(rp.tp, o, d, best, h);
+    rp.hitrec[pix] = h;
+    if (rp.tp.keys) rp.tp.keys[pix] = best_key(rp.tp, best);
+}
+
+template <class G, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_render_shadow_flat(RenderParams rp, const uint32_t* queue,
+                                                                    const uint32_t* count) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t qi = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (qi >= *count) return;
+    f3 o, d;
+    float tmin, tmax;
+    SrcShadow{rp.cam, queue, rp.hitbuf, rp.light}.get(qi, o, d, tmin, tmax);
+    Best best;
+    if (!trace_world<G, true, STACK>(rp.tp, o, d, tmin, tmax, best, stk + threadIdx.x, qi))
+        atomicAdd(&rp.tp.status[0], 1u);
+    rp.vis[queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
 }
 
 // Shades the queued hit pixels with their light visibility: the mean over the pixel's light
@@ -1810,11 +1867,13 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
     // the frame's counters, and the per-XCD counters of its two traversal launches
     const uint32_t zero_words = rp.tp.xcd_chunk ? kXcdCounters + 16u * kXcdStride : 8u;
     hipLaunchKernelGGL(k_zero, dim3((zero_words + 255) / 256), dim3(256), 0, st, rp.counters, zero_words);
-    if constexpr (!G::kMixed) {
+    {
         if (rp.tp.work) {
-            // Persistent pipeline: the traversal kernels only record hits (their sinks stay
+            // Deferred pipeline: the traversal kernels only record hits (their sinks stay
             // small because a lane finishing any ray makes the whole wave run the sink);
             // shading, the shadow queue and the optional deferred FP64 polish run as passes.
+            // A TLAS of one geometry mode traverses with persistent lanes, a mixed TLAS with one
+            // thread per queued ray.
             const TraceParams& tp = rp.tp;
             const dim3 pgrid(1024);
             hipLaunchKernelGGL(k_render_live, dim3((rp.cam.work + kAppendBlock - 1) / kAppendBlock), dim3(kAppendBlock),
@@ -1824,7 +1883,9 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
             // one-instance TLASes take kernels that read the BLAS fields once, TLASes of up to
             // kFewInstances instances kernels that stage them in LDS
             const int im = instance_mode(tp.ninst);
-            if (im == kInstOne)
+            if constexpr (G::kMixed)
+                hipLaunchKernelGGL((k_render_primary_flat<G, STACK>), grid, block, 0, st, rp);
+            else if (im == kInstOne)
                 hipLaunchKernelGGL((k_render_primary_defer<G, STACK, kInstOne>),
                                    persistent_grid(k_render_primary_defer<G, STACK, kInstOne>, tp.blocks_per_cu), block, 0,
                                    st, rp);
@@ -1864,7 +1925,9 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                         squeue = rp.fallback_queue;
                         scount = rp.counters + 5;
                     }
-                    if (im == kInstOne)
+                    if constexpr (G::kMixed)
+                        hipLaunchKernelGGL((k_render_shadow_flat<G, STACK>), sgrid, block, 0, st, rp, squeue, scount);
+                    else if (im == kInstOne)
                         hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, kInstOne>),
                                            persistent_grid(k_render_shadow_defer<G, STACK, kInstOne>, tp.blocks_per_cu),
                                            block, 0, st, rp, squeue, scount, rp.counters + 2);
