@@ -266,6 +266,16 @@ void vkhrt_tlas_destroy(vkhrt_tlas* tlas);
 /* ---- tracing */
 int vkhrt_trace_rays(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* rays, uint32_t count,
                      vkhrt_hit* hits, uint32_t flags, void* stream);
+/* vkhrt_trace_rays without any device allocation: no counterpart in the reference. For callers
+ * that cannot allocate (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, flags and results as vkhrt_trace_rays; its counters and FP64
+ * polish scratch live in one fixed device array per GPU (16,781,312 bytes, held until the process
+ * ends). With VKHRT_TRACE_FP64_POLISH, count is at most 262,144 (VKHRT_ERROR_INVALID_ARGUMENT
+ * above); without it, any count. Every call on one GPU shares that array: calls that overlap
+ * on one GPU (on different streams, or from different host threads) race and must be ordered
+ * by the caller. Errors found on the device are reported by vkhrt_synchronize. */
+int vkhrt_trace_rays_noalloc(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_ray* rays, uint32_t count,
+                             vkhrt_hit* hits, uint32_t flags, void* stream);
 int vkhrt_render(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_camera* camera,
                  const vkhrt_render_desc* desc, void* stream);
 /* glm::lookAt + perspectiveRH_ZO with the Vulkan Y flip, inverted (host function). */

@@ -65,7 +65,7 @@ STATUS = {0: "SUCCESS", -1: "INVALID_ARGUMENT", -2: "OUT_OF_MEMORY", -3: "DEVICE
 EXPORTS = [
     "vkhrt_abi_version", "vkhrt_context_create", "vkhrt_context_destroy", "vkhrt_last_error",
     "vkhrt_synchronize", "vkhrt_set_option", "vkhrt_blas_build", "vkhrt_blas_info_get",
-    "vkhrt_blas_destroy", "vkhrt_tlas_build", "vkhrt_tlas_destroy", "vkhrt_trace_rays",
+    "vkhrt_blas_destroy", "vkhrt_tlas_build", "vkhrt_tlas_destroy", "vkhrt_trace_rays", "vkhrt_trace_rays_noalloc",
     "vkhrt_render", "vkhrt_camera_look_at", "vkhrt_generate_strands", "vkhrt_generate_lines",
     "vkhrt_generate_curves", "vkhrt_curves_to_lines", "vkhrt_generate_lss", "vkhrt_generate_dots",
     "vkhrt_profiling_enable", "vkhrt_kernel_stats", "vkhrt_status_counters",
@@ -245,6 +245,7 @@ def lib():
         "vkhrt_tlas_build": ([P, ctypes.POINTER(P), u32, P, ctypes.POINTER(P)], i32),
         "vkhrt_tlas_destroy": ([P], None),
         "vkhrt_trace_rays": ([P, P, P, u32, P, u32, P], i32),
+        "vkhrt_trace_rays_noalloc": ([P, P, P, u32, P, u32, P], i32),
         "vkhrt_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderDesc), P], i32),
         "vkhrt_camera_look_at": ([P, P, P, f32, f32, f32, f32, ctypes.POINTER(Camera)], i32),
         "vkhrt_generate_strands": ([P, ctypes.c_uint64, u32, u32, ctypes.POINTER(StrandParams), P, P], i32),

@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvkhrt.so")
-SOURCES = ["api.cpp", "asset.cpp", "bvh_build.hip", "geometry.hip", "voxel.hip", "trace.hip", "trace_phantom_const.hip",
+SOURCES = ["api.cpp", "asset.cpp", "noalloc.cpp", "bvh_build.hip", "geometry.hip", "voxel.hip", "trace.hip", "trace_phantom_const.hip",
            "trace_phantom_point.hip", "trace_lss.hip", "trace_dots.hip", "trace_tris.hip", "trace_mixed.hip", "mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")

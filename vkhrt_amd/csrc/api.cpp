@@ -46,6 +46,7 @@ struct vkhrt_context {
     std::vector<PendingTiming> pending;
     std::map<std::string, std::pair<double, uint32_t>> stats;
     uint32_t* status = nullptr;  // device counters
+    void* noalloc = nullptr;     // device address of this GPU's allocation-free trace scratch
     // render scratch
     float4* hitbuf = nullptr;
     uint32_t* queue = nullptr;
@@ -515,6 +516,9 @@ int vkhrt_context_create(int device, vkhrt_context** out) {
         vkhrt_context_destroy(c);
         return VKHRT_ERROR_DEVICE;
     }
+    // the allocation-free trace's scratch array on this device (left null if the lookup fails;
+    // vkhrt_trace_rays_noalloc then reports it)
+    if (vk::noalloc_scratch_lookup(device, &c->noalloc) != hipSuccess) c->noalloc = nullptr;
     *out = c;
     return VKHRT_SUCCESS;
 }
@@ -1107,7 +1111,7 @@ int check_trace_args(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     return VKHRT_SUCCESS;
 }
 
-// The trace launches of vkhrt_trace_rays (arguments checked).
+// The trace launches of vkhrt_trace_rays and vkhrt_trace_rays_noalloc (arguments checked).
 int trace_launch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count, vkhrt_hit* hits,
                  uint32_t flags, hipStream_t st, const TraceScratch& s, bool timed) {
     const bool accurate = (flags & VKHRT_TRACE_ACCURATE_NORMALS) && !(flags & VKHRT_TRACE_TERMINATE_ON_FIRST_HIT);
@@ -1155,6 +1159,24 @@ int vkhrt_trace_rays(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* ray
     const TraceScratch s{c->counters + 3, c->counters + vk::kXcdCounters + 16 * vk::kXcdStride, c->counters + 4,
                          c->defer, c->defer_cap, c->keys, c->cand_hits, c->cand_keys};
     return trace_launch(c, t, rays, count, hits, flags, pick(c, stream), s, true);
+}
+
+int vkhrt_trace_rays_noalloc(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, uint32_t count,
+                             vkhrt_hit* hits, uint32_t flags, void* stream) {
+    int rc = check_trace_args(c, t, rays, count, hits, flags);
+    if (rc != VKHRT_SUCCESS) return rc;
+    if (!c->noalloc) return fail(c, VKHRT_ERROR_DEVICE, "allocation-free trace: no scratch address on this device");
+    if ((flags & VKHRT_TRACE_FP64_POLISH) && count > vk::kNoAllocMaxPolishRays)
+        return fail(c, VKHRT_ERROR_INVALID_ARGUMENT,
+                    "allocation-free trace with FP64 polish takes at most " + std::to_string(vk::kNoAllocMaxPolishRays) +
+                        " rays per call (asked for " + std::to_string(count) + ")");
+    auto* base = static_cast<unsigned char*>(c->noalloc);
+    auto* ctr = reinterpret_cast<uint32_t*>(base);
+    const TraceScratch s{ctr, ctr + 32, ctr + 1, reinterpret_cast<uint4*>(base + vk::kNoAllocDeferOffset),
+                         vk::kNoAllocMaxPolishRays, reinterpret_cast<unsigned long long*>(base + vk::kNoAllocKeysOffset),
+                         reinterpret_cast<vk::HitRec*>(base + vk::kNoAllocCandHitsOffset),
+                         reinterpret_cast<unsigned long long*>(base + vk::kNoAllocCandKeysOffset)};
+    return trace_launch(c, t, rays, count, hits, flags, pick(c, stream), s, false);
 }
 
 int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam, const vkhrt_render_desc* d,

@@ -56,6 +56,22 @@ struct Camera {
 
 struct MlpDev;
 
+// Allocation-free trace (vkhrt_trace_rays_noalloc): the layout of its per-device scratch array.
+// [0, 4 KiB) counters: word 0 the work counter, word 1 the polish candidate count, words
+// 32.. the 8 per-XCD work counters (kXcdStride apart); then, for traces with FP64 polish of up
+// to kNoAllocMaxPolishRays rays, the merge key per ray and a candidate queue with room for one
+// candidate per ray (queue entry, candidate hit and candidate key).
+constexpr uint32_t kNoAllocMaxPolishRays = 1u << 18;
+constexpr size_t kNoAllocCounterBytes = 4096;
+constexpr size_t kNoAllocKeysOffset = kNoAllocCounterBytes;
+constexpr size_t kNoAllocDeferOffset = kNoAllocKeysOffset + size_t(kNoAllocMaxPolishRays) * 8;
+constexpr size_t kNoAllocCandHitsOffset = kNoAllocDeferOffset + size_t(kNoAllocMaxPolishRays) * 16;
+constexpr size_t kNoAllocCandKeysOffset = kNoAllocCandHitsOffset + size_t(kNoAllocMaxPolishRays) * 32;
+constexpr size_t kNoAllocScratchBytes = kNoAllocCandKeysOffset + size_t(kNoAllocMaxPolishRays) * 8;
+// Device address of the scratch array on `device` (the current device), looked up on the
+// first call for that device.
+hipError_t noalloc_scratch_lookup(int device, void** out);
+
 // Compacted phantom lanes queue (lane, leaf slot) pairs as lane << kCompactSlotBits | slot, so
 // they serve phantom BLASes of fewer than 2^kCompactSlotBits records.
 constexpr uint32_t kCompactSlotBits = 26;
