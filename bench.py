@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the neural shadow form")
     ap.add_argument("--no-modes", action="store_true", help="skip the LSS, DOTS and voxel forms of the frame")
+    ap.add_argument("--no-collapse-study", action="store_true",
+                    help="skip the comparison of the greedy and SAH collapses to wide nodes")
     ap.add_argument("--voxel-size", type=float, default=0.5, help="cell edge of the voxel form")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view path-traced form")
     ap.add_argument("--no-area", action="store_true", help="skip the 16M-segment area-light neural form")
@@ -135,7 +137,8 @@ def main():
         tr = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags, repeats=5, stream=sp)
         torch.cuda.synchronize()
         binfo = blas.info()
-        tuning = {"bvh_width": tb["bvh_width"], "leaf_size": tb["leaf_size"], "stack_kind": tr["stack_kind"],
+        tuning = {"bvh_width": tb["bvh_width"], "leaf_size": tb["leaf_size"], "bvh_collapse": tb["bvh_collapse"],
+                  "stack_kind": tr["stack_kind"],
                   "blocks_per_cu": tr["blocks_per_cu"], "refill_min": tr["refill_min"], "ray_order": tr["ray_order"],
                   "xcd_chunk": tr["xcd_chunk"],
                   "trials": tb["trials"] + tr["trials"], "seconds": round(time.perf_counter() - t0, 2),
@@ -197,6 +200,11 @@ def main():
     if not args.no_roofline:
         roofline = roofline_block(ctx, tlas, cam, W, H, args.polish, kavg, sp, torch, np, V)
 
+    collapse = None
+    if not args.no_collapse_study:
+        collapse = collapse_block(args, ctx, cam, W, H, flags, sp, stream, torch, np, V, curves, nseg_total)
+        ctx.set_option(V.OPTION_BVH_COLLAPSE, tuning["bvh_collapse"] if tuning else 0)
+
     modes = None
     if not args.no_modes:
         modes = geometry_modes_block(args, ctx, tlas, cam, W, H, ms, sp, stream, torch, V, curves, nseg_total)
@@ -239,6 +247,8 @@ def main():
             "accurate_normals": bool(args.accurate),
             "rays_per_frame": rays, "primary_rays": n_primary, "shadow_rays": n_shadow,
             "bvh": {"width": tuning["bvh_width"] if tuning else 8,
+                    "collapse": ("greedy" if tuning["bvh_collapse"] == 0 else
+                                 f"sah_prim_cost_{tuning['bvh_collapse'] / 4:g}") if tuning else "greedy",
                     "leaf_size": tuning["leaf_size"] if tuning else args.leaf_size, "nodes": binfo["node_count"],
                     "depth": binfo["depth"], "build_ms": round(binfo["build_ms"], 3)},
             "stack_kind": tuning["stack_kind"] if tuning else args.stack_kind, "persistent": bool(args.persistent),
@@ -253,6 +263,7 @@ def main():
         "cpu_baseline": cpu,
         "neural": neural,
         "geometry_modes": modes,
+        "bvh_collapse": collapse,
         "batch": batch,
         "neural_area_light": area,
     }
@@ -453,6 +464,58 @@ def geometry_modes_block(args, ctx, tlas, cam, W, H, phantom_ms, sp, stream, tor
         mtlas.close()
         blas.close()
         del prims
+    return out
+
+
+def collapse_block(args, ctx, cam, W, H, flags, sp, stream, torch, np, V, curves, nseg):
+    """The binary tree collapsed to 8-wide nodes by the greedy rule (open the largest-area
+    child) and by the SAH-optimal rule at primitive costs 0.25, 0.5 and 1 (node cost 1): wide
+    nodes, build time, nodes visited and records tested per primary ray of the frame (one
+    thread per ray, closest hit), and the frame time (the headline frame's flags) of the phantom
+    hair and of its LSS and DOTS forms (12.8M pieces)."""
+    n = W * H
+    rays = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+    hits = torch.empty(n * 8, dtype=torch.float32, device="cuda")
+    stats = torch.empty(n * 2, dtype=torch.int32, device="cuda")
+    image = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.generate_camera_rays(cam, W, H, rays.data_ptr(), stream=sp)
+    pieces = nseg * 4
+    pl = torch.empty(pieces * 8, dtype=torch.float32, device="cuda")
+    ctx.curves_to_lines(curves.data_ptr(), nseg, 4, pl.data_ptr(), stream=sp)
+    caps = torch.empty(pieces * 8, dtype=torch.float32, device="cuda")
+    ctx.generate_lss(pl.data_ptr(), pieces, caps.data_ptr(), stream=sp)
+    forms = [("phantom", V.GEOMETRY_PHANTOM_CURVES, curves, nseg, V.BLAS_PER_POINT_RADIUS, flags),
+             ("lss", V.GEOMETRY_LSS, caps, pieces, 0, V.RENDER_SHADOWS | V.RENDER_RGBA8),
+             ("dots", V.GEOMETRY_DOTS, pl, pieces, V.BLAS_PER_POINT_RADIUS, V.RENDER_SHADOWS | V.RENDER_RGBA8)]
+    out = {}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for name, mode, prims, count, bflags, rflags in forms:
+        rows = {}
+        for coll in (0, 1, 2, 4):
+            ctx.set_option(V.OPTION_BVH_COLLAPSE, coll)
+            blas = V.BottomLevelAccelerationStructure(ctx, mode, prims.data_ptr(), count, flags=bflags, stream=sp)
+            tlas = V.TopLevelAccelerationStructure(ctx, [blas], stream=sp)
+            torch.cuda.synchronize()
+            info = blas.info()
+            ctx.trace_stats(tlas, rays.data_ptr(), n, hits.data_ptr(), stats.data_ptr(), stream=sp)
+            torch.cuda.synchronize()
+            st = stats.view(n, 2).to(torch.float64).mean(0).tolist()
+            for _ in range(args.warmup):
+                ctx.render(tlas, cam, W, H, image.data_ptr(), flags=rflags, stream=sp)
+            e0.record(stream)
+            for _ in range(args.steps):
+                ctx.render(tlas, cam, W, H, image.data_ptr(), flags=rflags, stream=sp)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            key = "greedy" if coll == 0 else f"sah_prim_cost_{coll / 4:g}"
+            rows[key] = {"nodes": info["node_count"], "depth": info["depth"], "build_ms": round(info["build_ms"], 3),
+                         "nodes_visited_per_ray": round(st[0], 3), "records_tested_per_ray": round(st[1], 3),
+                         "ms_per_frame": round(ms, 4)}
+            log(f"[bench] collapse {name} {key}: {rows[key]}")
+            tlas.close()
+            blas.close()
+        out[name] = rows
     return out
 
 

@@ -234,6 +234,9 @@ int vkhrt_abi_version(void);
                                        per CU, 0 (default) = as many as the LDS stack allows */
 #define VKHRT_OPTION_BVH_WIDTH 7     /* children per node for later builds, 4..8 (default 8; nodes
                                        keep the 8-slot layout) */
+#define VKHRT_OPTION_BVH_COLLAPSE 9  /* how later builds collapse the binary tree into wide nodes: 0 greedy
+                                       (open the largest-area child until the width is reached; default),
+                                       c in 1..8 SAH-optimal (node cost 1, primitive cost c / 4) */
 #define VKHRT_OPTION_RAY_ORDER 5   /* vkhrt_render work order: 0 row-major (default), 1 8x8 pixel
                                      tiles, one tile per wavefront, 2 64x64 tiles of 8x8 tiles */
 #define VKHRT_OPTION_XCD_CHUNK 8   /* persistent traversal (vkhrt_render, vkhrt_render_batch,
@@ -417,6 +420,7 @@ typedef struct vkhrt_tuning {
     float baseline_ms;     /* the workload with the starting settings */
     float best_ms;         /* ... with the chosen ones (<= baseline_ms) */
     int32_t xcd_chunk;     /* VKHRT_OPTION_XCD_CHUNK */
+    int32_t bvh_collapse;  /* VKHRT_OPTION_BVH_COLLAPSE (vkhrt_autotune_blas) */
 } vkhrt_tuning;
 /* vkhrt_render(tlas, camera, desc) as the workload; sets stack kind, blocks per CU, refill
  * minimum, ray order and XCD chunk. */

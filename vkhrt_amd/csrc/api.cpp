@@ -37,6 +37,7 @@ struct vkhrt_context {
     int stack_kind = 1;
     int leaf_size = 1;
     int bvh_width = 8;
+    int bvh_collapse = 0;  // greedy largest-area collapse (fewer node visits per ray on hair than SAH)
     bool persistent = true;
     uint32_t refill_min = 16;
     uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles
@@ -611,6 +612,11 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
         c->xcd_chunk = uint32_t(value);
         return VKHRT_SUCCESS;
     }
+    if (option == VKHRT_OPTION_BVH_COLLAPSE) {
+        if (value < 0 || value > 8) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "BVH collapse must be 0..8");
+        c->bvh_collapse = value;
+        return VKHRT_SUCCESS;
+    }
     if (option == VKHRT_OPTION_BVH_WIDTH) {
         if (value < 4 || value > 8) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "BVH width must be 4..8");
         c->bvh_width = value;
@@ -861,7 +867,8 @@ int vkhrt_blas_build(vkhrt_context* c, const vkhrt_blas_desc* d, void* stream, v
         {
             Timer tm(c, st, "blas_build");
             e = vk::build_bvh(int(d->mode), records, d->count, int(b->radius_mode), d->radius,
-                              d->leaf_size ? int(d->leaf_size) : c->leaf_size, c->bvh_width, st, &b->bvh, err);
+                              d->leaf_size ? int(d->leaf_size) : c->leaf_size, c->bvh_width, c->bvh_collapse, st,
+                              &b->bvh, err);
         }
         if (e != hipSuccess) {
             rc = fail(c, e == hipErrorOutOfMemory ? VKHRT_ERROR_OUT_OF_MEMORY : VKHRT_ERROR_DEVICE, "bvh build: " + err);
@@ -1597,6 +1604,7 @@ void fill_tuning(const vkhrt_context* c, vkhrt_tuning* out) {
     out->ray_order = int32_t(c->ray_order);
     out->xcd_chunk = int32_t(c->xcd_chunk);
     out->bvh_width = c->bvh_width;
+    out->bvh_collapse = c->bvh_collapse;
     out->leaf_size = c->leaf_size;
 }
 
@@ -1673,9 +1681,10 @@ int vkhrt_autotune_blas(vkhrt_context* c, const vkhrt_blas_desc* bd, const vkhrt
     if (repeats < 1 || repeats > 100) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "repeats must be 1..100");
     *out = vkhrt_tuning{};
     hipStream_t st = pick(c, stream);
-    const int width0 = c->bvh_width, leaf0 = c->leaf_size;
-    // one trial: build with (width, leaf), render, time
-    auto trial = [&](int width, int leaf, float& ms) {
+    const int width0 = c->bvh_width, leaf0 = c->leaf_size, coll0 = c->bvh_collapse;
+    // one trial: build with (collapse, width, leaf), render, time
+    auto trial = [&](int coll, int width, int leaf, float& ms) {
+        c->bvh_collapse = coll;
         c->bvh_width = width;
         vkhrt_blas_desc b = *bd;
         b.leaf_size = uint32_t(leaf);
@@ -1691,33 +1700,38 @@ int vkhrt_autotune_blas(vkhrt_context* c, const vkhrt_blas_desc* bd, const vkhrt
         return rc;
     };
     float best = 0.0f;
-    int rc = trial(width0, leaf0, best);
+    int rc = trial(coll0, width0, leaf0, best);
     if (rc != VKHRT_SUCCESS) {
         c->bvh_width = width0;
+        c->bvh_collapse = coll0;
         return rc;
     }
     out->baseline_ms = best;
     out->trials = 1;
-    int bw = width0, bl = leaf0;
-    for (int width : {4, 6, 8})
-        for (int leaf : {1, 2, 3}) {
-            if (width == width0 && leaf == leaf0) continue;
-            float ms = 0.0f;
-            rc = trial(width, leaf, ms);
-            if (rc != VKHRT_SUCCESS) {
-                c->bvh_width = width0;
-                c->leaf_size = leaf0;
-                return rc;
+    int bw = width0, bl = leaf0, bc = coll0;
+    for (int coll : {0, 2, 4})
+        for (int width : {4, 6, 8})
+            for (int leaf : {1, 2, 3}) {
+                if (coll == coll0 && width == width0 && leaf == leaf0) continue;
+                float ms = 0.0f;
+                rc = trial(coll, width, leaf, ms);
+                if (rc != VKHRT_SUCCESS) {
+                    c->bvh_width = width0;
+                    c->leaf_size = leaf0;
+                    c->bvh_collapse = coll0;
+                    return rc;
+                }
+                ++out->trials;
+                if (ms < best) {
+                    best = ms;
+                    bw = width;
+                    bl = leaf;
+                    bc = coll;
+                }
             }
-            ++out->trials;
-            if (ms < best) {
-                best = ms;
-                bw = width;
-                bl = leaf;
-            }
-        }
     c->bvh_width = bw;
     c->leaf_size = bl;
+    c->bvh_collapse = bc;
     out->best_ms = best;
     fill_tuning(c, out);
     return VKHRT_SUCCESS;

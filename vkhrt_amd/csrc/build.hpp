@@ -19,9 +19,10 @@ struct BuiltBvh {
 };
 
 // Builds the 8-wide BVH over `n` primitive records (device pointer) on stream `st`.
-// `width` (2..8) caps the children per wide node.
+// `width` (2..8) caps the children per wide node. `collapse` 0: greedy largest-area opening;
+// c >= 1: SAH-optimal collapse with node cost 1 and primitive cost c / 4.
 hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mode, float radius, int leaf_max,
-                     int width, hipStream_t st, BuiltBvh* out, std::string& err);
+                     int width, int collapse, hipStream_t st, BuiltBvh* out, std::string& err);
 
 struct TraceParams {
     const InstanceDev* inst;

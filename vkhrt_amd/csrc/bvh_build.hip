@@ -9,10 +9,15 @@
 //      scan, LDS-ranked stable scatter (wave ballots rank equal digits)
 //   4. Karras (HPG 2012) binary radix tree, one thread per internal node
 //   5. bottom-up box refit in level-synchronous rounds (one launch per tree level)
-//   6. top-down collapse to 8-wide nodes, one level per launch: each binary subtree is
-//      opened greedily by largest surface area until `width` (<= 8) children; subtrees with at most
-//      `leaf_max` primitives become leaf children. Child slots are ordered per ray octant;
-//      child boxes are quantized to 8 bits against power-of-two scales (outward rounding)
+//   6. top-down collapse to 8-wide nodes, one level per launch, by one of two rules:
+//      SAH: the refit rounds also compute, for every binary node, the least SAH cost
+//      of covering its subtree with at most j child slots (j = 1..width), and each wide node is
+//      opened along those choices (Ylitie, Karras, Laine, HPG 2017);
+//      greedy (default, fewer node visits per ray on hair): each binary subtree is opened by
+//      largest surface area until `width` (<= 8)
+//      children, and subtrees with at most `leaf_max` primitives become leaf children.
+//      Child slots are ordered per ray octant; child boxes are quantized to 8 bits against
+//      power-of-two scales (outward rounding)
 //   7. gather primitive records into leaf order
 #include <hip/hip_runtime.h>
 
@@ -370,14 +375,92 @@ __device__ __forceinline__ void child_box(uint32_t ref, const float4* blo, const
     }
 }
 
+__device__ __forceinline__ float box_area(const float lo[3], const float hi[3]) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// SAH-optimal collapse (Ylitie, Karras, Laine, "Efficient Incoherent Ray Traversal on GPUs
+// Through Compressed Wide BVHs", HPG 2017). Every binary node n gets the least SAH cost C(n, j)
+// of covering its subtree with at most j child slots of a wide node (j = 1..width):
+//   C(n, 1) = min(as a leaf slot: A(n) cnt(n) c_prim, if cnt(n) <= leaf_max;
+//                 as a wide node: A(n) c_node + D(n, width))
+//   C(n, j) = min(C(n, j - 1), D(n, j)),   D(n, j) = min over k of C(left, k) + C(right, j - k)
+// with A the half surface area of the box; a binary leaf (one primitive) costs A c_prim for any
+// j. The choices are recorded, and the top-down pass opens each wide node along them.
+struct SahCost {
+    float c[8];      // C(n, j), j = 1..width
+    uint8_t dec[8];  // dec[0]: 1 leaf slot, 0 wide node; dec[j - 1] (j >= 2): 0 = C(n, j - 1), else the k of D(n, j)
+    uint32_t kint;   // the k of D(n, width): slots of the wide node rooted at n from the left child
+    uint32_t pad;
+};
+struct SahParams {
+    SahCost* cost;  // null: greedy collapse
+    const uint2* range;
+    int leaf_max, width;
+    float c_node, c_prim;
+};
+
+__device__ __forceinline__ float sah_child(const SahParams& sp, uint32_t ref, int j, const float lo[3],
+                                           const float hi[3]) {
+    return (ref & kLeafFlag) ? box_area(lo, hi) * sp.c_prim : sp.cost[ref].c[j - 1];
+}
+
+__device__ __forceinline__ void sah_node(const SahParams& sp, int i, uint2 c, const float lo[3], const float hi[3],
+                                         const float alo[3], const float ahi[3], const float blo[3],
+                                         const float bhi[3]) {
+    const int W = sp.width;
+    float cl[8], cr[8], dist[9];
+    uint8_t dk[9];
+#pragma unroll
+    for (int j = 1; j <= 8; ++j) {
+        cl[j - 1] = j <= W ? sah_child(sp, c.x, j, alo, ahi) : 0.0f;
+        cr[j - 1] = j <= W ? sah_child(sp, c.y, j, blo, bhi) : 0.0f;
+    }
+    for (int j = 2; j <= W; ++j) {
+        float best = INFINITY;
+        uint8_t bk = 1;
+        for (int k = 1; k < j; ++k) {
+            const float v = cl[k - 1] + cr[j - k - 1];
+            if (v < best) {
+                best = v;
+                bk = uint8_t(k);
+            }
+        }
+        dist[j] = best;
+        dk[j] = bk;
+    }
+    const float A = box_area(lo, hi);
+    const uint2 rg = sp.range[i];
+    const uint32_t cnt = rg.y - rg.x + 1;
+    const float c_leaf = cnt <= uint32_t(sp.leaf_max) ? A * float(cnt) * sp.c_prim : INFINITY;
+    const float c_int = A * sp.c_node + dist[W];
+    SahCost out;
+    out.c[0] = fminf(c_leaf, c_int);
+    out.dec[0] = c_leaf <= c_int ? 1 : 0;
+    for (int j = 2; j <= 8; ++j) {
+        if (j <= W && dist[j] < out.c[j - 2]) {
+            out.c[j - 1] = dist[j];
+            out.dec[j - 1] = dk[j];
+        } else {
+            out.c[j - 1] = out.c[j - 2];
+            out.dec[j - 1] = 0;
+        }
+    }
+    out.kint = dk[W];
+    out.pad = 0;
+    sp.cost[i] = out;
+}
+
 // Level-synchronous refit, one round per launch: an internal node whose children were
 // finished in earlier rounds (leaves always are) gets its box and records the round. Rounds =
 // tree height. The kernel boundary makes the children's boxes visible, so the rounds need no
 // atomics or fences (a per-node climb needs agent-scope release/acquire pairs, which write back
 // and invalidate L2 across the XCDs at every step).
+// With SAH costs requested, the round also computes the node's costs from its children's.
 __global__ void k_refit_round(int nint, const uint2* __restrict__ child, const uint32_t* __restrict__ vals,
                               const float4* __restrict__ blo, const float4* __restrict__ bhi, float* __restrict__ nlo,
-                              float* __restrict__ nhi, uint32_t* __restrict__ done, uint32_t round) {
+                              float* __restrict__ nhi, uint32_t* __restrict__ done, uint32_t round, SahParams sp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nint || done[i]) return;
     const uint2 c = child[i];
@@ -387,10 +470,14 @@ __global__ void k_refit_round(int nint, const uint2* __restrict__ child, const u
     float alo[3], ahi[3], blo3[3], bhi3[3];
     child_box(c.x, blo, bhi, vals, nlo, nhi, alo, ahi);
     child_box(c.y, blo, bhi, vals, nlo, nhi, blo3, bhi3);
+    float lo[3], hi[3];
     for (int k = 0; k < 3; ++k) {
-        nlo[i * 3 + k] = fminf(alo[k], blo3[k]);
-        nhi[i * 3 + k] = fmaxf(ahi[k], bhi3[k]);
+        lo[k] = fminf(alo[k], blo3[k]);
+        hi[k] = fmaxf(ahi[k], bhi3[k]);
+        nlo[i * 3 + k] = lo[k];
+        nhi[i * 3 + k] = hi[k];
     }
+    if (sp.cost) sah_node(sp, i, c, lo, hi, alo, ahi, blo3, bhi3);
     done[i] = round;
 }
 
@@ -422,7 +509,7 @@ __global__ void k_collapse(const Task* __restrict__ tin, const uint32_t* __restr
                            const uint2* __restrict__ range, const uint32_t* __restrict__ vals,
                            const float4* __restrict__ blo, const float4* __restrict__ bhi, const float* __restrict__ nlo,
                            const float* __restrict__ nhi, WideNode* __restrict__ nodes,
-                           uint32_t* __restrict__ prim_slots) {
+                           uint32_t* __restrict__ prim_slots, SahParams sp) {
     uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
     if (ti >= *ntin) return;
     Task task = tin[ti];
@@ -452,16 +539,45 @@ __global__ void k_collapse(const Task* __restrict__ tin, const uint32_t* __restr
             }
         }
     };
+    bool is_int[8];
     if (task.bref & kLeafFlag) {
         load(0, task.bref);
         ne = 1;
+    } else if (sp.cost) {
+        // the slots recorded by the cost pass: D(root, width) split, each side expanded along
+        // its choices (a binary leaf, or one slot, ends the expansion)
+        uint2 stk[16];
+        int sn = 0;
+        const uint2 c = child[task.bref];
+        const int k = int(sp.cost[task.bref].kint);
+        stk[sn++] = make_uint2(c.y, uint32_t(width - k));
+        stk[sn++] = make_uint2(c.x, uint32_t(k));
+        while (sn > 0 && ne < 8) {
+            const uint2 t = stk[--sn];
+            const uint32_t m = t.x;
+            const int j = int(t.y);
+            if ((m & kLeafFlag) || j <= 1) {
+                load(ne, m);
+                is_int[ne] = !(m & kLeafFlag) && sp.cost[m].dec[0] == 0;
+                ++ne;
+                continue;
+            }
+            const int d = sp.cost[m].dec[j - 1];
+            if (d == 0) {
+                stk[sn++] = make_uint2(m, uint32_t(j - 1));
+            } else if (sn + 2 <= 16) {
+                const uint2 cm = child[m];
+                stk[sn++] = make_uint2(cm.y, uint32_t(j - d));
+                stk[sn++] = make_uint2(cm.x, uint32_t(d));
+            }
+        }
     } else {
         uint2 c = child[task.bref];
         load(0, c.x);
         load(1, c.y);
         ne = 2;
     }
-    while (ne < width) {
+    while (!sp.cost && ne < width) {
         int best = -1;
         float ba = -1.0f;
         for (int e = 0; e < ne; ++e) {
@@ -538,9 +654,8 @@ __global__ void k_collapse(const Task* __restrict__ tin, const uint32_t* __restr
     uint32_t slot_entry[8];
     for (int s = 0; s < 8; ++s) slot_entry[s] = 0xFFu;
     for (int e = 0; e < ne; ++e) slot_entry[slot_of[e]] = uint32_t(e);
-    bool is_int[8];
     for (int e = 0; e < ne; ++e) {
-        is_int[e] = !(ref[e] & kLeafFlag) && cnt[e] > uint32_t(leaf_max);
+        if (!sp.cost || (task.bref & kLeafFlag)) is_int[e] = !(ref[e] & kLeafFlag) && cnt[e] > uint32_t(leaf_max);
         if (is_int[e])
             ++n_int;
         else
@@ -666,7 +781,7 @@ inline uint32_t blocks(uint64_t n, int b = kBlock) { return uint32_t((n + b - 1)
     } while (0)
 
 hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mode, float radius, int leaf_max,
-                     int width, hipStream_t st, BuiltBvh* out, std::string& err) {
+                     int width, int collapse, hipStream_t st, BuiltBvh* out, std::string& err) {
     const int rec = mode == kLss ? 2 : mode == kTriangles ? 3 : 4;
     float4 *blo = nullptr, *bhi = nullptr;
     uint32_t* cb = nullptr;
@@ -676,6 +791,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     uint32_t *par_i = nullptr, *par_l = nullptr, *counters = nullptr;
     float *nlo = nullptr, *nhi = nullptr;
     Task *ta = nullptr, *tb = nullptr;
+    SahParams sp{};
     uint32_t* cnt = nullptr;  // [0] ntasks a, [1] ntasks b, [2] nodes, [3] prims
     WideNode* nodes = nullptr;
     uint32_t* slots = nullptr;
@@ -717,6 +833,14 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
     VK_TRY(dalloc(&slots, n));
     VK_TRY(dalloc(&prims_sorted, size_t(n) * rec));
     if (mode == kPhantom) VK_TRY(dalloc(&aux, n));
+    if (collapse > 0 && n >= 2) {
+        VK_TRY(dalloc(&sp.cost, n));
+        sp.range = range;
+        sp.leaf_max = leaf_max;
+        sp.width = width;
+        sp.c_node = 1.0f;
+        sp.c_prim = 0.25f * float(collapse);
+    }
 
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, cb);
     hipLaunchKernelGGL(k_prim_boxes, dim3(std::min<uint32_t>(blocks(n), 2048u)), dim3(kBlock), 0, st, prims_in, n, mode,
@@ -750,7 +874,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
             }
             for (uint32_t r = round; r < round + 8; ++r)
                 hipLaunchKernelGGL(k_refit_round, dim3(blocks(n - 1)), dim3(kBlock), 0, st, int(n - 1), child, v0, blo,
-                                   bhi, nlo, nhi, counters, r);
+                                   bhi, nlo, nhi, counters, r, sp);
             VK_TRY(hipGetLastError());
             VK_TRY(hipMemcpyAsync(&root_done, counters, 4, hipMemcpyDeviceToHost, st));
             VK_TRY(hipStreamSynchronize(st));
@@ -766,7 +890,7 @@ hipError_t build_bvh(int mode, const float4* prims_in, uint32_t n, int radius_mo
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, nout, 0u, 1u);
         hipLaunchKernelGGL(k_collapse, dim3(blocks(cur, 64)), dim3(64), 0, st, (level & 1) ? tb : ta, nin,
                            (level & 1) ? ta : tb, nout, cnt + 2, cnt + 3, int(n), leaf_max, width, child, range, v0, blo, bhi,
-                           nlo, nhi, nodes, slots);
+                           nlo, nhi, nodes, slots, sp);
         VK_TRY(hipGetLastError());
         VK_TRY(hipMemcpyAsync(&cur, nout, 4, hipMemcpyDeviceToHost, st));
         VK_TRY(hipStreamSynchronize(st));
@@ -827,6 +951,7 @@ done:
     hipFree(slots);
     hipFree(prims_sorted);
     hipFree(aux);
+    hipFree(sp.cost);
     return rc;
 }
 

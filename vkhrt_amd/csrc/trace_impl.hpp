@@ -1409,10 +1409,7 @@ struct SrcCamera {
 // in the instance's object space, against the hit's curve record; t, u and the normal are
 // replaced when the re-run converges. Misses, other modes and FP64-polished roots are kept.
 template <class Src>
-__global__ __launch_bounds__(kTraceBlock) void k_accurate_hits(TraceParams tp, Src src, HitRec* __restrict__ hits,
-                                                               uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void accurate_one(const TraceParams& tp, const Src& src, uint32_t i, HitRec* hits) {
     HitRec h = hits[i];
     if (h.prim_id == 0xFFFFFFFFu || h.v != 0.0f || h.inst_id >= tp.ninst) return;
     const InstanceDev& I = tp.inst[h.inst_id];
@@ -1440,6 +1437,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_accurate_hits(TraceParams tp, S
     h.n[1] = nn.y;
     h.n[2] = nn.z;
     hits[i] = h;
+}
+template <class Src>
+__global__ __launch_bounds__(kTraceBlock) void k_accurate_hits(TraceParams tp, Src src, HitRec* __restrict__ hits,
+                                                               uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) accurate_one(tp, src, i, hits);
 }
 
 // Shadow queue entries are ids pixel * samples + light sample.
@@ -1682,9 +1685,11 @@ __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams r
 // Pass 3: shadow rays (FP32, any hit) -> per-pixel visibility.
 template <class G, int STACK, int IM>
 __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParams rp, const uint32_t* queue,
-                                                                     const uint32_t* count, uint32_t* work) {
+                                                                     const uint32_t* count, uint32_t* work,
+                                                                     uint32_t skip_upto) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t total = *count;
+    if (total <= skip_upto) return;  // a short queue is walked by k_render_shadow_wave
     const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
     persistent_lanes<G, true, STACK, IM>(
         rp.tp, total, work, stk + threadIdx.x,
@@ -1696,6 +1701,156 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
             rp.vis[queue[qi]] = best.inst == 0xFFFFFFFFu ? 1.0f : 0.0f;
         },
         rp.counters + kXcdCounters + 8 * kXcdStride);
+}
+
+// Accurate hits of the render: the primary phantom hits (not FP64-polished) gathered in work
+// order from the live rays, so the double-word re-run fills whole wavefronts.
+__global__ __launch_bounds__(kAppendBlock) void k_accurate_gather(RenderParams rp) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    uint32_t pix = 0;
+    if (r < rp.counters[6]) {
+        pix = rp.live[r];
+        const HitRec& h = rp.hitrec[pix];
+        take = h.prim_id != 0xFFFFFFFFu && h.v == 0.0f && h.inst_id < rp.tp.ninst &&
+               rp.tp.inst[h.inst_id].blas.mode == kPhantom;
+    }
+    const uint32_t slot = block_append(take, &rp.counters[7]);
+    if (take) rp.queue[slot] = pix;
+}
+
+template <class Src>
+__global__ __launch_bounds__(kTraceBlock) void k_accurate_queued(TraceParams tp, Src src, HitRec* __restrict__ hits,
+                                                                 const uint32_t* __restrict__ queue,
+                                                                 const uint32_t* __restrict__ count) {
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= *count) return;
+    accurate_one(tp, src, queue[qi], hits);
+}
+
+// ------------------------------------------------------------------ one wave per shadow ray
+// Any-hit traversal of a short shadow queue (the neural form's exact fallback): the whole
+// wavefront walks one ray. Persistent lanes take one ray each and hide memory latency behind
+// the other rays of a long queue; with a few thousand rays the frame waits for the slowest
+// ray, whose walk is a chain of dependent node loads. Here the 64 lanes pop up to 64 node
+// groups of the ray at once from a stack shared in LDS, test them in parallel and push the
+// internal children they hit; the leaf records hit are early-out tested and intersected 64 at a
+// time. The walk ends at the first hit in [tmin, tmax] (the ray is occluded) or when the stack
+// is empty (visible). Grazing candidates are deferred to the FP64 polish as in the other
+// kernels, so the visibility equals the persistent kernel's for every ray.
+constexpr uint32_t kWaveStack = 6144;   // node entries per wave (24 KB)
+constexpr uint32_t kWaveLeaves = 1600;  // leaf records of one step: <= 64 nodes x 24 records
+constexpr uint32_t kWaveRayLimit = 16384;  // queues up to this long take the wave-per-ray kernel
+constexpr uint32_t kWaveGrid = 4096;       // one-wave blocks of the wave-per-ray launch
+
+template <class G, int IM>
+__global__ __launch_bounds__(64) void k_render_shadow_wave(RenderParams rp, const uint32_t* queue, const uint32_t* count,
+                                                           uint32_t limit) {
+    static_assert(G::kMode == kPhantom && !G::kMixed, "wave-per-ray shadows serve phantom TLASes");
+    __shared__ uint32_t s_stack[kWaveStack];
+    __shared__ uint32_t s_leaf[kWaveLeaves];
+    const uint32_t total = *count;
+    if (total == 0 || total > limit) return;
+    const TraceParams& tp = rp.tp;
+    const uint32_t lane = threadIdx.x;
+    const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
+    for (uint32_t qi = blockIdx.x; qi < total; qi += gridDim.x) {
+        f3 ow, dw;
+        float tmin, tmax;
+        src.get(qi, ow, dw, tmin, tmax);
+        bool occluded = false;
+        for (uint32_t inst = 0; inst < tp.ninst && !occluded; ++inst) {
+            const InstanceDev& I = tp.inst[inst];
+            if (I.blas.nprims == 0 || !ray_box(ow, dw, I.lo, I.hi, tmin, tmax)) continue;
+            const BlasDev B = IM == kInstOne ? wave_uniform(I.blas) : I.blas;
+            f3 o = ow, d = dw;
+            if (!I.identity) {
+                o = xform_pt(I.inv, ow);
+                d = xform_dir(I.inv, dw);
+            }
+            const f3 inv = safe_inv(d);
+            const uint32_t oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
+            RayFrame F;
+            make_frame(o, d, F);
+            uint32_t sp = 1;  // wave-uniform; the root is node 0
+            if (lane == 0) s_stack[0] = 0u;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            bool error = false;
+            while (sp > 0 && !occluded && !error) {
+                const uint32_t m = sp < 64u ? sp : 64u;
+                const uint32_t node = lane < m ? s_stack[sp - m + lane] : 0xFFFFFFFFu;
+                sp -= m;
+                uint32_t hits = 0, pmask = 0, child_base = 0, imask = 0, pbase = 0;
+                if (node != 0xFFFFFFFFu) {
+                    if (node < B.nnodes) {
+                        node_visit(B.nodes, node, o, inv, oct, tmin, tmax, hits, pmask, child_base, imask, pbase);
+                        if (pmask && pbase + (32u - __clz(pmask)) > B.nprims) {
+                            error = true;
+                            hits = pmask = 0;
+                        }
+                    } else {
+                        error = true;
+                    }
+                }
+                error = __ballot(error) != 0;
+                // push the internal children hit (octant bit j -> slot j ^ oct -> child index)
+                const uint32_t nc = __popc(hits), np = __popc(pmask);
+                uint32_t cpre = nc, ppre = np;  // inclusive prefix sums over the lanes
+#pragma unroll
+                for (int k = 1; k < 64; k <<= 1) {
+                    const uint32_t a = __shfl_up(cpre, k), b = __shfl_up(ppre, k);
+                    if (int(lane) >= k) {
+                        cpre += a;
+                        ppre += b;
+                    }
+                }
+                const uint32_t ctot = __shfl(cpre, 63), ptot = __shfl(ppre, 63);
+                if (sp + ctot > kWaveStack || ptot > kWaveLeaves) {
+                    error = true;
+                    break;
+                }
+                uint32_t w = sp + cpre - nc;
+                for (uint32_t hb = hits; hb; hb &= hb - 1u) {
+                    const uint32_t j = __ffs(hb) - 1, slot = j ^ oct;
+                    s_stack[w++] = child_base + __popc(imask & ((1u << slot) - 1u));
+                }
+                w = ppre - np;
+                for (uint32_t pm = pmask; pm; pm &= pm - 1u) s_leaf[w++] = pbase + uint32_t(__ffs(pm) - 1);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                sp += ctot;
+                // the leaf records of this step, 64 at a time: early out, then Prhi
+                for (uint32_t base = 0; base < ptot && !occluded; base += 64u) {
+                    bool hit = false;
+                    if (base + lane < ptot) {
+                        const uint32_t slot = s_leaf[base + lane];
+                        const auto* pr = gmem(B.prims + size_t(slot) * 4);
+                        const float4 a = pr[0], b = pr[3];
+                        if (ray_cylinder(o, d, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, gmem(B.aux)[slot])) {
+                            float4 cv[4] = {a, pr[1], pr[2], b};
+                            PhantomHit ph;
+                            hit = prhi<G::kRMode, G::kPolish ? kPolishDefer : kPolishOff, false>(F, cv, 0.0f, B.radius,
+                                                                                                  ph);
+                            if (G::kPolish && ph.deferred) {
+                                const uint32_t q = atomicAdd(tp.defer_count, 1u);
+                                if (q < tp.defer_cap)
+                                    tp.defer[q] = make_uint4(qi, inst, slot, 0u);
+                                else
+                                    atomicAdd(&tp.status[1], 1u);
+                            }
+                            hit = hit && ph.t >= tmin && ph.t <= tmax;
+                        }
+                    }
+                    occluded = __ballot(hit) != 0;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (error && lane == 0) atomicAdd(&tp.status[0], 1u);
+        }
+        if (lane == 0) rp.vis[queue[qi]] = occluded ? 0.0f : 1.0f;
+    }
 }
 
 // The same two traversal passes for TLASes that mix geometry modes: one thread per queued
@@ -1903,8 +2058,12 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                    tp.cand_hits, tp.cand_keys);
                 hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
             }
-            if (rp.accurate)
-                hipLaunchKernelGGL(k_accurate_hits<SrcCamera>, grid, block, 0, st, tp, SrcCamera{rp.cam}, rp.hitrec, npix);
+            if (rp.accurate) {
+                hipLaunchKernelGGL(k_accurate_gather, dim3((npix + kAppendBlock - 1) / kAppendBlock), dim3(kAppendBlock), 0,
+                                   st, rp);
+                hipLaunchKernelGGL(k_accurate_queued<SrcCamera>, grid, block, 0, st, tp, SrcCamera{rp.cam}, rp.hitrec,
+                                   rp.queue, rp.counters + 7);
+            }
             hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kAppendBlock - 1) / kAppendBlock),
                                dim3(kAppendBlock), 0, st, rp);
             bool shadow_timed = false;
@@ -1925,20 +2084,35 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                         squeue = rp.fallback_queue;
                         scount = rp.counters + 5;
                     }
+                    // the neural form's fallback queue is short: a queue of up to kWaveRayLimit
+                    // rays is walked one ray per wavefront, a longer one by the persistent lanes
+                    // (both kernels read the length on the device; one of them returns at once)
+                    uint32_t skip = 0;
+                    if constexpr (!G::kMixed && G::kMode == kPhantom) {
+                        if (rp.mlp) {
+                            skip = kWaveRayLimit;
+                            if (im == kInstOne)
+                                hipLaunchKernelGGL((k_render_shadow_wave<G, kInstOne>), dim3(kWaveGrid), dim3(64), 0, st, rp,
+                                                   squeue, scount, skip);
+                            else
+                                hipLaunchKernelGGL((k_render_shadow_wave<G, kInstMany>), dim3(kWaveGrid), dim3(64), 0, st,
+                                                   rp, squeue, scount, skip);
+                        }
+                    }
                     if constexpr (G::kMixed)
                         hipLaunchKernelGGL((k_render_shadow_flat<G, STACK>), sgrid, block, 0, st, rp, squeue, scount);
                     else if (im == kInstOne)
                         hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, kInstOne>),
                                            persistent_grid(k_render_shadow_defer<G, STACK, kInstOne>, tp.blocks_per_cu),
-                                           block, 0, st, rp, squeue, scount, rp.counters + 2);
+                                           block, 0, st, rp, squeue, scount, rp.counters + 2, skip);
                     else if (im == kInstFew)
                         hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, kInstFew>),
                                            persistent_grid(k_render_shadow_defer<G, STACK, kInstFew>, tp.blocks_per_cu),
-                                           block, 0, st, rp, squeue, scount, rp.counters + 2);
+                                           block, 0, st, rp, squeue, scount, rp.counters + 2, skip);
                     else
                         hipLaunchKernelGGL((k_render_shadow_defer<G, STACK, kInstMany>),
                                            persistent_grid(k_render_shadow_defer<G, STACK, kInstMany>, tp.blocks_per_cu),
-                                           block, 0, st, rp, squeue, scount, rp.counters + 2);
+                                           block, 0, st, rp, squeue, scount, rp.counters + 2, skip);
                     if (ev) hipEventRecord(ev[3], st);
                     shadow_timed = true;
                     if (tp.defer)
