@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s, 100k-strand 1080p primary+shadow; exact vs neural shadows @PSNR>=40dB"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 vector
+# FP32 operations of one 8-wide node test (node_visit in vkhrt_amd/csrc/trace_impl.hpp): 3 scale
+# products, 3 origin differences and products, and per child slot 6 dequantizing fused
+# multiply-adds (2 each) and the far-plane widening product: 3 + 6 + 8 x 13 = 113
+NODE_TEST_FLOP = 113
 # frozen view of the 100k-strand hair ball (fills the frame height)
 CAMERA_EYE = (0.0, 60.0, 300.0)
 CAMERA_TARGET = (0.0, 40.0, 0.0)
@@ -198,7 +203,7 @@ def main():
 
     roofline = None
     if not args.no_roofline:
-        roofline = roofline_block(ctx, tlas, cam, W, H, args.polish, kavg, sp, torch, np, V)
+        roofline = roofline_block(ctx, tlas, cam, W, H, args.polish, kavg, sp, torch, np, V, args.accurate)
 
     collapse = None
     if not args.no_collapse_study:
@@ -269,6 +274,8 @@ def main():
     }
     if parity is not None:
         out["parity_sample"] = parity
+    if roofline is not None and roofline.get("work") and parity is not None and parity.get("flop_census"):
+        roofline["valu"] = valu_block(roofline, parity["flop_census"])
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -533,7 +540,47 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}), flush=True)
 
 
-def roofline_block(ctx, tlas, cam, W, H, polish, kavg, sp, torch, np, V):
+def work_counts(args_frames=3, accurate=1):
+    """Work counters of the same frame from the counting build (vkhrt_amd/workcount.py in a
+    child process, so its library stays out of this one): per traversal kind the node tests,
+    leaf early-out tests, Prhi calls, lane use and the split of wave clock cycles."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-m", "vkhrt_amd.workcount", "--frames", str(args_frames),
+                        "--accurate", str(int(accurate))], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        log(f"[bench] work counters unavailable: {r.stderr.strip()[-300:]}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def valu_block(roofline, census):
+    """Useful FP32 work of the dominant kernel against the FP32 peak: the GPU's counts of node
+    tests, leaf early-out tests, Prhi calls and rays (counting build, same frame) times the FLOP
+    each takes (node test: NODE_TEST_FLOP from the kernel's arithmetic; ray frame, early-out and
+    Prhi: the oracle's templates run with an operation-counting float on sampled rays of the
+    frame, Prhi averaged over its calls), divided by the kernel's average time."""
+    w = roofline["work"]["primary"]["per_frame"]
+    fl = {"node_test": NODE_TEST_FLOP,
+          "ray_frame": census["frame_flop"] / census["rays"],
+          "early_out": census["early_out_flop"] / max(census["early_out_tests"], 1),
+          "prhi_call": census["prhi_flop"] / max(census["prhi_calls"], 1)}
+    parts = {"node_tests": w["node_tests"] * fl["node_test"], "early_out": w["leaf_tests"] * fl["early_out"],
+             "prhi": w["prhi_calls"] * fl["prhi_call"], "ray_frames": w["rays"] * fl["ray_frame"]}
+    total = sum(parts.values())
+    ms = roofline["kernel_avg_ms"]["render_primary"]
+    achieved = total / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "kernel": "render_primary", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+            "useful_flop_per_launch": round(total), "flop_each": {k: round(v, 1) for k, v in fl.items()},
+            "share": {k: round(v / total, 4) for k, v in parts.items()},
+            "prhi_iterations_per_call": round(census["prhi_iterations"] / max(census["prhi_calls"], 1), 2),
+            "lane_use": roofline["work"]["primary"]["lane_use"],
+            "wave_cycle_share": roofline["work"]["primary"]["cycle_share"],
+            "note": "useful = the arithmetic the method needs; the kernel also spends VALU slots on idle lanes, "
+                    "integer decoding, queues and control (see wave_cycle_share, lane_use)"}
+
+
+def roofline_block(ctx, tlas, cam, W, H, polish, kavg, sp, torch, np, V, accurate=1):
     """Algorithmic bytes per launch of the dominant kernel / its average duration."""
     n = W * H
     rays = torch.empty(n * 8, dtype=torch.float32, device="cuda")
@@ -566,6 +613,7 @@ def roofline_block(ctx, tlas, cam, W, H, polish, kavg, sp, torch, np, V):
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9 if ms_dom > 0 else 0.0
     log(f"[bench] N*_prim primary {nstar_primary / n:.3f}/ray, shadow {nstar_shadow / max(1, ns):.3f}/ray; "
         f"dominant {dom}: {bytes_dom / 1e6:.1f} MB in {ms_dom:.3f} ms -> {achieved:.1f} GB/s")
+    work = work_counts(args_frames=3, accurate=accurate)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "render_primary_phantom_c1_traffic.json")
     if os.path.exists(tpath):
@@ -586,6 +634,7 @@ def roofline_block(ctx, tlas, cam, W, H, polish, kavg, sp, torch, np, V):
         "nstar_per_ray": {"primary": round(nstar_primary / n, 4),
                           "shadow": round(nstar_shadow / max(1, ns), 4)},
         "kernel_avg_ms": {k: round(v, 4) for k, v in kavg.items()},
+        "work": work,
     }
 
 
@@ -876,7 +925,12 @@ def cpu_baseline(args, ctx, tlas, cam, W, H, flags, sp, torch, np, V, curves_d):
            "sample": f"every {stride}th pixel of the {W}x{H} frame ({npix} pixels) traced {reps} times: "
                      f"{nr} primary+shadow rays in {secs:.1f} s; scene generation + BVH2 build "
                      f"({t_build:.1f} s) excluded"}
-    parity = {"pixels": int(npix), "hit_pixels": hits, "primid_agreement": agree,
+    census_rays = O.raygen(vi, pi, W, H)[::97].copy()
+    census = S.flop_census(census_rays)
+    log(f"[bench] FLOP census on {int(census['rays'])} rays: frame {census['frame_flop'] / census['rays']:.0f}, "
+        f"early-out {census['early_out_flop'] / max(census['early_out_tests'], 1):.1f} per test, Prhi "
+        f"{census['prhi_flop'] / max(census['prhi_calls'], 1):.0f} per call")
+    parity = {"pixels": int(npix), "hit_pixels": hits, "primid_agreement": agree, "flop_census": census,
               "accuracy_vs_fp64": accuracy_sample(ctx, tlas, cam, W, H, args.polish, sp, torch, np, V, O, curves,
                                                   curves_d)}
     return cpu, parity

@@ -596,6 +596,13 @@ int vkhrt_profiling_enable(vkhrt_context* ctx, int enable);
 /* name: "render_primary", "render_shadow", "render_batch", "trace_rays", "blas_build". */
 int vkhrt_kernel_stats(vkhrt_context* ctx, const char* name, double* total_ms, uint32_t* launches);
 int vkhrt_status_counters(vkhrt_context* ctx, uint32_t* stack_overflows);
+/* Work counters of the counting build (the library compiled with VKHRT_WORK_COUNTERS, e.g.
+ * vkhrt_amd/libvkhrt_count.so): per traversal kind, loop iterations, intersection rounds,
+ * traversal steps, node and leaf tests, Prhi calls, busy lanes and shader-clock cycles per
+ * phase, summed over waves since the last reset (layout: vk::WorkCounter in
+ * vkhrt_amd/csrc/build.hpp). Waits for the device. The product build returns
+ * VKHRT_ERROR_UNSUPPORTED. */
+int vkhrt_work_counters(vkhrt_context* ctx, uint64_t* out, uint32_t count, int reset);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "oracle_ff.hpp"
+#include "oracle_count.hpp"
 #include "oracle_geom.hpp"
 #include "oracle_mlp.hpp"
 #include "oracle_path.hpp"
@@ -1142,6 +1143,76 @@ int orc_count_candidates(orc_scene* sp, const float* rays, const void* hits, uin
         }
         counts[i] = c;
     });
+    return 0;
+}
+
+// FP32 operation census of the phantom intersector (test infrastructure; the oracle's own
+// templates run with the counting float of oracle_count.hpp). For each ray against instance 0
+// (a phantom BLAS), over the segments whose exact box overlaps [tmin, t_closest] (the
+// candidate set of orc_count_candidates):
+//   out[0] rays, out[1] FLOP of the ray frame (MakeOrthonormalBasis + the generic inverse of
+//   CreateRCCMatrix, ray.glsl:13-33), summed over rays;
+//   out[2] early-out tests, out[3] their FLOP (RayCylinderIntersect, cylinder.glsl:8-46, with
+//   the precomputed radius rmax);
+//   out[4] Prhi calls (candidates past the early-out), out[5] their FLOP after the early-out
+//   (curve transform, start end, cone iterations, hit and normal; hair_intersection.rint:38-126,
+//   the frame excluded), out[6] their cone iterations;
+//   out[7..10] the compare/min/max/abs operations of the frame, early-out and Prhi classes
+//   (not FLOP, reported apart).
+int orc_flop_census(orc_scene* sp, const float* rays, uint32_t n, double* out) {
+    Scene& S = *reinterpret_cast<Scene*>(sp);
+    for (int i = 0; i < 11; ++i) out[i] = 0.0;
+    if (S.inst.empty() || S.inst[0].blas->mode != MODE_PHANTOM) return -1;
+    const Blas& B = *S.inst[0].blas;
+    using orc::cnt::counts;
+    for (uint32_t i = 0; i < n; ++i) {
+        float tmin, tmax;
+        Ray<float> rw = load_ray(rays + size_t(i) * 8, tmin, tmax);
+        Ray<float> r;
+        obj_ray(S.inst[0], rw, r);
+        Hit best{};
+        bool found = false;
+        float tc = tmax;
+        trace_blas<float>(B, 0, r, tmin, tc, 0, best, found, nullptr);
+        std::vector<uint32_t> cand;
+        Hit dummy{};
+        bool f2 = false;
+        float te = found ? best.t : tmax;
+        trace_blas<float>(B, 0, r, tmin, te, 0, dummy, f2, &cand);
+        const Ray<CountF> rc{cvt<CountF>(r.origin), cvt<CountF>(r.direction)};
+        counts() = {};
+        (void)CreateRCCMatrix(rc);
+        const uint64_t frame = counts().flops();
+        out[0] += 1;
+        out[1] += double(frame);
+        out[7] += double(counts().cmp);
+        for (uint32_t p : cand) {
+            const float* c = &B.prims[size_t(p) * 16];
+            const Curve<float> cf{mk(c[0], c[1], c[2]), mk(c[4], c[5], c[6]), mk(c[8], c[9], c[10]), mk(c[12], c[13], c[14])};
+            const float radf[4] = {c[3], c[7], c[11], c[15]};
+            float rpad = B.radius;
+            if (B.per_point) rpad = gmax(gmax(radf[0], radf[1]), gmax(radf[2], radf[3]));
+            const float rmax = CurveDistanceToCylinder(cf, SampleCurvePoint(cf, 0.5f)) + rpad;
+            const Curve<CountF> cc{cvt<CountF>(cf.start), cvt<CountF>(cf.cp1), cvt<CountF>(cf.cp2), cvt<CountF>(cf.end)};
+            counts() = {};
+            const bool pass = RayCylinderIntersect(rc, cc.start, cc.end, CountF(rmax));
+            out[2] += 1;
+            out[3] += double(counts().flops());
+            out[8] += double(counts().cmp);
+            if (!pass) continue;
+            const CountF radc[4] = {CountF(radf[0]), CountF(radf[1]), CountF(radf[2]), CountF(radf[3])};
+            PrhiOpts o;
+            o.radius_mode = B.per_point ? RADIUS_PER_POINT : RADIUS_CONSTANT;
+            o.radius = B.radius;
+            o.cull = false;
+            counts() = {};
+            const PrhiOut<CountF> po = Prhi(rc, cc, radc, o);
+            out[4] += 1;
+            out[5] += double(counts().flops()) - double(frame);
+            out[6] += double(po.iters);
+            out[9] += double(counts().cmp);
+        }
+    }
     return 0;
 }
 

@@ -73,6 +73,7 @@ def lib():
         "orc_scene_stats": ([P, u32, P], i32),
         "orc_trace": ([P, P, u32, P, u32, i32], i32),
         "orc_count_candidates": ([P, P, P, u32, P, i32], i32),
+        "orc_flop_census": ([P, P, u32, P], i32),
         "orc_render": ([P, P, P, u32, u32, u32, i32, P, P, i32, u32, P, P, u32, u64, P, u32, u32, f32],
                        ctypes.c_longlong),
         "orc_scene_set_material": ([P, u32, u32], i32),
@@ -241,6 +242,17 @@ class Scene:
         hits = np.zeros(rays.shape[0], HIT_DTYPE)
         lib().orc_trace(self.h, ptr(rays), rays.shape[0], ptr(hits), flags, nthreads)
         return hits
+
+    def flop_census(self, rays):
+        """FP32 operations of the phantom intersector on these rays against instance 0 (the
+        counting build of the oracle's templates): dict of per-class calls and FLOP."""
+        rays = f32(rays)
+        out = np.zeros(11, np.float64)
+        if lib().orc_flop_census(self.h, ptr(rays), rays.shape[0], out.ctypes.data_as(ctypes.c_void_p)) != 0:
+            raise ValueError("instance 0 is not a phantom BLAS")
+        return {"rays": out[0], "frame_flop": out[1], "early_out_tests": out[2], "early_out_flop": out[3],
+                "prhi_calls": out[4], "prhi_flop": out[5], "prhi_iterations": out[6], "frame_cmp": out[7],
+                "early_out_cmp": out[8], "prhi_cmp": out[9]}
 
     def count_candidates(self, rays, hits, nthreads=0):
         rays = f32(rays)

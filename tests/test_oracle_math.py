@@ -510,3 +510,30 @@ def test_camera_center_ray_hits_target():
     bot = rays.reshape(H, W, 8)[-1, W // 2, 4:7]
     ang = math.degrees(math.acos(np.clip(top @ bot, -1, 1)))
     assert abs(ang - 60.0 * (H - 1) / H) < 0.5
+
+
+def test_flop_census_counts():
+    """The operation census (the oracle's templates run with a counting float): the ray frame
+    (basis + generic 4x4 inverse) has no data-dependent branch, so its FLOP per ray is one
+    integer for any ray set; every Prhi call runs between 1 and 16 cone iterations; a call's
+    FLOP grows with its iterations; and the counting float computes like float, so the census's
+    candidate and call counts equal those of the plain traversal (orc_count_candidates)."""
+    import scenes
+    curves = scenes.c0_curves()
+    S = O.Scene()
+    S.add(O.MODE_PHANTOM, curves, per_point=True)
+    per_ray = []
+    for seed in (1, 2):
+        rays = scenes.targeted_rays(curves, 1500, seed=seed)
+        c = S.flop_census(rays)
+        assert c["rays"] == 1500
+        f = c["frame_flop"] / c["rays"]
+        assert f == int(f) and f > 100
+        per_ray.append(f)
+        assert c["prhi_calls"] > 200 and c["early_out_tests"] >= c["prhi_calls"]
+        it = c["prhi_iterations"] / c["prhi_calls"]
+        assert 1.0 <= it <= 16.0
+        assert c["prhi_flop"] / c["prhi_calls"] > 50 * it  # each cone iteration is dozens of FLOP
+        hits = S.trace(rays)
+        assert c["early_out_tests"] == S.count_candidates(rays, hits).sum()
+    assert per_ray[0] == per_ray[1]

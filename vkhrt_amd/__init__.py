@@ -14,7 +14,9 @@ import math
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvkhrt.so")
+# VKHRT_LIBRARY selects another build of the library, e.g. the counting build
+# libvkhrt_count.so (work counters for measurement; same kernels and results)
+LIB_PATH = os.environ.get("VKHRT_LIBRARY") or os.path.join(HERE, "libvkhrt.so")
 
 GEOMETRY_PHANTOM_CURVES, GEOMETRY_LSS, GEOMETRY_DOTS, GEOMETRY_TRIANGLES, GEOMETRY_VOXELS = 0, 1, 2, 3, 4
 
@@ -66,7 +68,7 @@ STATUS = {0: "SUCCESS", -1: "INVALID_ARGUMENT", -2: "OUT_OF_MEMORY", -3: "DEVICE
 EXPORTS = [
     "vkhrt_abi_version", "vkhrt_context_create", "vkhrt_context_destroy", "vkhrt_last_error",
     "vkhrt_synchronize", "vkhrt_set_option", "vkhrt_blas_build", "vkhrt_blas_info_get",
-    "vkhrt_blas_destroy", "vkhrt_tlas_build", "vkhrt_tlas_destroy", "vkhrt_trace_rays", "vkhrt_trace_rays_noalloc",
+    "vkhrt_blas_destroy", "vkhrt_tlas_build", "vkhrt_tlas_destroy", "vkhrt_trace_rays", "vkhrt_trace_rays_noalloc", "vkhrt_work_counters",
     "vkhrt_render", "vkhrt_camera_look_at", "vkhrt_generate_strands", "vkhrt_generate_lines",
     "vkhrt_generate_curves", "vkhrt_curves_to_lines", "vkhrt_generate_lss", "vkhrt_generate_dots",
     "vkhrt_profiling_enable", "vkhrt_kernel_stats", "vkhrt_status_counters",
@@ -247,6 +249,7 @@ def lib():
         "vkhrt_tlas_destroy": ([P], None),
         "vkhrt_trace_rays": ([P, P, P, u32, P, u32, P], i32),
         "vkhrt_trace_rays_noalloc": ([P, P, P, u32, P, u32, P], i32),
+        "vkhrt_work_counters": ([P, P, u32, i32], i32),
         "vkhrt_render": ([P, P, ctypes.POINTER(Camera), ctypes.POINTER(RenderDesc), P], i32),
         "vkhrt_camera_look_at": ([P, P, P, f32, f32, f32, f32, ctypes.POINTER(Camera)], i32),
         "vkhrt_generate_strands": ([P, ctypes.c_uint64, u32, u32, ctypes.POINTER(StrandParams), P, P], i32),

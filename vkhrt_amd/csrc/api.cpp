@@ -48,6 +48,7 @@ struct vkhrt_context {
     std::map<std::string, std::pair<double, uint32_t>> stats;
     uint32_t* status = nullptr;  // device counters
     void* noalloc = nullptr;     // device address of this GPU's allocation-free trace scratch
+    unsigned long long* work_counts = nullptr;  // counting build: the kernels' work counters
     // render scratch
     float4* hitbuf = nullptr;
     uint32_t* queue = nullptr;
@@ -513,6 +514,10 @@ int vkhrt_context_create(int device, vkhrt_context** out) {
     if (e == hipSuccess) e = hipMalloc(&c->status, 16);
     if (e == hipSuccess) e = hipMalloc(&c->counters, vk::kRenderCounterWords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(c->status, 0, 16);
+#ifdef VKHRT_WORK_COUNTERS
+    if (e == hipSuccess) e = hipMalloc(&c->work_counts, vk::kWorkCounters * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->work_counts, 0, vk::kWorkCounters * sizeof(unsigned long long));
+#endif
     if (e != hipSuccess) {
         vkhrt_context_destroy(c);
         return VKHRT_ERROR_DEVICE;
@@ -533,6 +538,7 @@ void vkhrt_context_destroy(vkhrt_context* c) {
         hipEventDestroy(p.b);
     }
     hipFree(c->status);
+    hipFree(c->work_counts);
     hipFree(c->counters);
     hipFree(c->hitbuf);
     hipFree(c->queue);
@@ -1126,6 +1132,7 @@ int trace_launch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, u
     // persistent phantom kernels compact intersection work through keys that need polish_ok
     const bool persistent = c->persistent && (t->polish_ok || !is_phantom(t));
     vk::TraceParams tp{t->dev, t->count, flags, c->status, persistent ? s.work : nullptr, c->refill_min};
+    tp.work_counts = c->work_counts;
     tp.blocks_per_cu = c->blocks_per_cu;
     tp.xcd_chunk = c->xcd_chunk;
     tp.xcd_work = s.xcd_work;
@@ -1226,6 +1233,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
                             c->status, c->persistent && (t->polish_ok || !is_phantom(t)) ? c->counters + 1 : nullptr,
                             c->refill_min};
     rp.tp.blocks_per_cu = c->blocks_per_cu;
+    rp.tp.work_counts = c->work_counts;
     std::memcpy(rp.cam.view_inv, cam->view_inverse, 64);
     std::memcpy(rp.cam.proj_inv, cam->proj_inverse, 64);
     rp.cam.width = d->width;
@@ -2211,6 +2219,16 @@ int vkhrt_kernel_stats(vkhrt_context* c, const char* name, double* total_ms, uin
     if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.first;
     if (launches) *launches = it == c->stats.end() ? 0u : it->second.second;
     return VKHRT_SUCCESS;
+}
+
+int vkhrt_work_counters(vkhrt_context* c, uint64_t* out, uint32_t count, int reset) {
+    if (!c || (count && !out)) return VKHRT_ERROR_INVALID_ARGUMENT;
+    if (!c->work_counts) return fail(c, VKHRT_ERROR_UNSUPPORTED, "work counters exist in the counting build only");
+    const uint32_t n = std::min<uint32_t>(count, vk::kWorkCounters);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && n) e = hipMemcpy(out, c->work_counts, n * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && reset) e = hipMemset(c->work_counts, 0, vk::kWorkCounters * sizeof(unsigned long long));
+    return e == hipSuccess ? VKHRT_SUCCESS : hip_fail(c, e, "work counters");
 }
 
 int vkhrt_status_counters(vkhrt_context* c, uint32_t* overflows) {

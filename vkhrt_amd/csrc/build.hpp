@@ -42,7 +42,31 @@ struct TraceParams {
     uint32_t xcd_chunk;        // persistent grids: 0 one work counter; a power of two: rays handed
                                // out per XCD in runs of this many (each launch brings 8 counters)
     uint32_t* xcd_work;        // the 8 per-XCD counters of vkhrt_trace_rays's persistent launch
+    unsigned long long* work_counts;  // counting build (VKHRT_WORK_COUNTERS): kWorkCounters words, else null
 };
+
+// Work counters of the counting build (compiled with VKHRT_WORK_COUNTERS; the product build
+// leaves them out). Per traversal kind (closest hit from 0, any hit from kWorkAnyHit; the wave-
+// per-ray shadow kernel from kWorkWave), wave-level sums over the persistent loop:
+enum WorkCounter {
+    kWcIterations = 0,   // loop iterations of all waves
+    kWcRounds = 1,       // intersection rounds (iterations that run Prhi on the queued pairs)
+    kWcSteps = 2,        // traversal-step iterations
+    kWcNodeTests = 3,    // lanes that tested a node in a step
+    kWcLeafTests = 4,    // lanes that ran a leaf record's early-out test in a step
+    kWcPairs = 5,        // (ray, record) pairs that passed the early-out (Prhi calls)
+    kWcRoundLanes = 6,   // lanes busy in intersection rounds (sum of the pairs per round)
+    kWcCyclesRounds = 7, // shader clock cycles spent in rounds, summed over waves
+    kWcCyclesSteps = 8,  // ... in traversal steps
+    kWcCyclesOther = 9,  // ... in refill, instance selection, tail hand-over and sinks
+    kWcWaves = 10,       // waves that ran the loop
+    kWcRays = 11,        // rays finished
+    kWcHandovers = 12,   // stacked node groups handed to idle lanes (tail splitting)
+    kWcStepLanes = 13,   // lanes walking (node test or leaf test) summed over step iterations
+};
+constexpr uint32_t kWorkAnyHit = 16, kWorkWave = 32, kWorkCounters = 48;
+// wave-per-ray kernel: kWorkWave + 0 rays, 1 steps, 2 node tests, 3 leaf records tested,
+// 4 Prhi calls, 5 iterations that ran records, 6 cycles
 
 struct Camera {
     float view_inv[16];

@@ -6,12 +6,15 @@
 // (pi 2^k x) = 96 features -> 4 x (64, ReLU) -> logit -> sigmoid = visibility. Parameters,
 // FP32, flat: W1[64][96] b1 W2[64][64] b2 W3 b3 W4 b4 w5[64] b5 (18,753 values).
 //
-// MFMA mapping (v_mfma_f32_16x16x16_bf16): a layer computes D = W X^T for 16 outputs x 16
-// samples. Lane l = 16 g + c holds D[4 g + j][c] (j = 0..3), i.e. outputs 4g..4g+3 of sample
-// c, which is exactly the B operand slice (K rows 4g..4g+3, column c) of the next layer. The
-// forward pass and the back-propagated deltas (dX = W^T dZ) therefore chain from layer to
-// layer in registers. Weight gradients (dW = dZ X^T, K = samples) need the transposed
-// operands and are staged through LDS.
+// MFMA mapping (gfx950 v_mfma_f32_16x16x32_bf16, K = 32): a layer computes D = W X^T for 16
+// outputs x 16 samples. Lane l = 16 g + c holds D[4 g + j][c] (j = 0..3), i.e. outputs
+// 4g..4g+3 of sample c. The B operand of a K = 32 step holds 8 K rows per lane (8g..8g+7 of
+// column c); a step covers two 16-row input tiles t0, t1, and physical row 8g + i stands for
+// input 16 t0 + 4g + i (i < 4) or 16 t1 + 4g + i - 4 (i >= 4). With that order of K, the
+// outputs one lane holds of two output tiles are exactly its B operand of the next layer: the
+// forward pass and the back-propagated deltas (dX = W^T dZ) chain from layer to layer in
+// registers, and the weight fragments are packed in the same K order. Weight gradients
+// (dW = dZ X^T, K = samples) take samples in plain order and are staged through LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -22,15 +25,17 @@ namespace mlp_detail {
 namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kEnc = 96, kHid = 64;
 constexpr int kOffW1 = 0, kOffB1 = 6144, kOffW2 = 6208, kOffB2 = 10304, kOffW3 = 10368, kOffB3 = 14464;
 constexpr int kOffW4 = 14528, kOffB4 = 18624, kOffW5 = 18688, kOffB5 = 18752;
-// packed bf16 fragments: [frag][64 lanes][4]; forward W1 (4x6 frags), W2..W4 (4x4 each), then
-// the transposed W2..W4 (4x4 each) for back-propagation
-constexpr int kFragW1 = 0, kFragW2 = 24, kFragW3 = 40, kFragW4 = 56, kFragT2 = 72, kFragT3 = 88, kFragT4 = 104;
-constexpr int kFragsFwd = 72, kFragsAll = 120;
+// packed bf16 fragments: [frag][64 lanes][8]; forward W1 (4 output tiles x 3 K steps), W2..W4
+// (4 x 2 each), then the transposed W2..W4 (4 input tiles x 2 K steps each) for back-propagation
+constexpr int kFragW1 = 0, kFragW2 = 12, kFragW3 = 20, kFragW4 = 28, kFragT2 = 36, kFragT3 = 44, kFragT4 = 52;
+constexpr int kFragsFwd = 36, kFragsAll = 60;
 constexpr int kSmall = 4 * kHid + kHid + 1;  // biases b1..b4, w5, b5 (FP32)
 
 __device__ __forceinline__ uint32_t bf16_bits(float x) {
@@ -47,31 +52,36 @@ __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
     v.w = short(bf16_bits(d));
     return v;
 }
-__device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ s8 cat(s4 a, s4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
+__device__ __forceinline__ f4 mfma(s8 a, s8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
 
-// Source index in the flat parameters of packed element (frag, lane, j).
-__device__ __forceinline__ int packed_source(int frag, int lane, int j) {
+// Source index in the flat parameters of packed element (frag, lane, i): lane 16 g + c, K
+// step p over the 16-row tiles 2p, 2p + 1, physical K row 8 g + i = logical row k below.
+__device__ __forceinline__ int packed_source(int frag, int lane, int i) {
     const int g = lane >> 4, c = lane & 15;
-    if (frag < kFragW2) {  // W1 [m][t]
-        const int m = frag / 6, t = frag % 6;
-        return kOffW1 + (16 * m + c) * kEnc + 16 * t + 4 * g + j;
+    auto krow = [&](int p) { return i < 4 ? 32 * p + 4 * g + i : 32 * p + 16 + 4 * g + (i - 4); };
+    if (frag < kFragW2) {  // W1 [m][p]: row = output 16 m + c, column = input feature k
+        const int m = frag / 3, p = frag % 3;
+        return kOffW1 + (16 * m + c) * kEnc + krow(p);
     }
-    if (frag < kFragT2) {  // W2..W4 [m][t]
-        const int L = (frag - kFragW2) / 16, q = (frag - kFragW2) % 16, m = q / 4, t = q % 4;
+    if (frag < kFragT2) {  // W2..W4 [m][p]
+        const int L = (frag - kFragW2) / 8, q = (frag - kFragW2) % 8, m = q / 2, p = q % 2;
         const int off = L == 0 ? kOffW2 : (L == 1 ? kOffW3 : kOffW4);
-        return off + (16 * m + c) * kHid + 16 * t + 4 * g + j;
+        return off + (16 * m + c) * kHid + krow(p);
     }
-    // transposed W2..W4 [t][m]: row = input 16 t + c, k = output 16 m + 4 g + j
-    const int L = (frag - kFragT2) / 16, q = (frag - kFragT2) % 16, t = q / 4, m = q % 4;
+    // transposed W2..W4 [t][p]: row = input 16 t + c, K = output k
+    const int L = (frag - kFragT2) / 8, q = (frag - kFragT2) % 8, t = q / 2, p = q % 2;
     const int off = L == 0 ? kOffW2 : (L == 1 ? kOffW3 : kOffW4);
-    return off + (16 * m + 4 * g + j) * kHid + 16 * t + c;
+    return off + krow(p) * kHid + 16 * t + c;
 }
 
 __global__ void k_pack(const float* __restrict__ master, uint16_t* __restrict__ packed) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;  // element
-    if (e >= kFragsAll * 256) return;
-    const int frag = e >> 8, lane = (e >> 2) & 63, j = e & 3;
-    packed[e] = uint16_t(bf16_bits(master[packed_source(frag, lane, j)]));
+    if (e >= kFragsAll * 512) return;
+    const int frag = e >> 9, lane = (e >> 3) & 63, i = e & 7;
+    packed[e] = uint16_t(bf16_bits(master[packed_source(frag, lane, i)]));
 }
 
 // Feature f = 16 t + 4 g + j of the lane's sample for input dim t: k = 2 g + j / 2,
@@ -99,14 +109,14 @@ __device__ __forceinline__ void load_sample(const float4* s, float4 c, float4 ih
 }
 
 // Forward of 16 samples (lane l: sample l & 15); h[L][m] = post-ReLU FP32 outputs 16 m + 4 g + j.
-__device__ __forceinline__ float forward16(const s4* __restrict__ W, const float* __restrict__ small, const s4 enc[6],
+__device__ __forceinline__ float forward16(const s8* __restrict__ W, const float* __restrict__ small, const s4 enc[6],
                                            int lane, f4 h[4][4]) {
     const int g = lane >> 4;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int t = 0; t < 6; ++t) acc = mfma(W[(kFragW1 + m * 6 + t) * 64 + lane], enc[t], acc);
+        for (int p = 0; p < 3; ++p) acc = mfma(W[(kFragW1 + m * 3 + p) * 64 + lane], cat(enc[2 * p], enc[2 * p + 1]), acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j) h[0][m][j] = fmaxf(acc[j] + small[16 * m + 4 * g + j], 0.0f);
     }
@@ -119,7 +129,8 @@ __device__ __forceinline__ float forward16(const s4* __restrict__ W, const float
         for (int m = 0; m < 4; ++m) {
             f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc = mfma(W[(kFragW2 + (L - 1) * 16 + m * 4 + t) * 64 + lane], xin[t], acc);
+            for (int p = 0; p < 2; ++p)
+                acc = mfma(W[(kFragW2 + (L - 1) * 8 + m * 2 + p) * 64 + lane], cat(xin[2 * p], xin[2 * p + 1]), acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) h[L][m][j] = fmaxf(acc[j] + small[kHid * L + 16 * m + 4 * g + j], 0.0f);
         }
@@ -138,8 +149,8 @@ __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-
 
 // Stages the forward fragments and the FP32 small parameters into LDS (all threads).
 __device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ packed, const float* __restrict__ master,
-                                              s4* lw, float* lsmall, int nfrags) {
-    const s4* src = reinterpret_cast<const s4*>(packed);
+                                              s8* lw, float* lsmall, int nfrags) {
+    const s8* src = reinterpret_cast<const s8*>(packed);
     for (int i = threadIdx.x; i < nfrags * 64; i += blockDim.x) lw[i] = src[i];
     for (int i = threadIdx.x; i < kSmall; i += blockDim.x) {
         int s;
@@ -156,7 +167,7 @@ __device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ packe
 
 __global__ __launch_bounds__(256) void k_infer(MlpDev m, const float4* __restrict__ samples, uint32_t n,
                                                float* __restrict__ out) {
-    __shared__ s4 lw[kFragsFwd * 64];
+    __shared__ s8 lw[kFragsFwd * 64];
     __shared__ float lsmall[kSmall];
     stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
     const int lane = threadIdx.x & 63;
@@ -179,7 +190,7 @@ __global__ __launch_bounds__(256) void k_infer(MlpDev m, const float4* __restric
 // direction). Confident predictions (p (1 - p) <= threshold) are written as the pixel's
 // visibility; the rest are appended to the exact-fallback queue.
 __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
-    __shared__ s4 lw[kFragsFwd * 64];
+    __shared__ s8 lw[kFragsFwd * 64];
     __shared__ float lsmall[kSmall];
     __shared__ uint32_t s_fb[4][128];  // per-wave fallback pixels, flushed 64+ at a time
     stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
@@ -245,7 +256,7 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
 // network and back, staging bf16 inputs X_L and deltas dZ_L per layer in LDS; then wave w
 // accumulates dW of layer w + 1 over the chunk's 64 samples with MFMAs (K = samples).
 struct TrainLds {
-    alignas(16) s4 w[kFragsAll * 64];       // 61,440 B
+    alignas(16) s8 w[kFragsAll * 64];       // 61,440 B
     alignas(16) float small[kSmall];
     alignas(16) uint16_t x1[kEnc * 64];     // [feature][sample]
     alignas(16) uint16_t x[3][kHid * 64];   // h1, h2, h3 as inputs of layers 2..4
@@ -258,7 +269,7 @@ __global__ __launch_bounds__(256) void k_train(MlpDev m, MlpTrainParams tp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     TrainLds& S = *reinterpret_cast<TrainLds*>(lds_raw);
     {
-        const s4* src = reinterpret_cast<const s4*>(m.packed);
+        const s8* src = reinterpret_cast<const s8*>(m.packed);
         for (int i = threadIdx.x; i < kFragsAll * 64; i += blockDim.x) S.w[i] = src[i];
         for (int i = threadIdx.x; i < kSmall; i += blockDim.x) {
             int s;
@@ -354,7 +365,8 @@ __global__ __launch_bounds__(256) void k_train(MlpDev m, MlpTrainParams tp) {
                 for (int t = 0; t < 4; ++t) {
                     f4 a = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                    for (int mm = 0; mm < 4; ++mm) a = mfma(S.w[(tb + t * 4 + mm) * 64 + lane], dzb[mm], a);
+                    for (int p = 0; p < 2; ++p)
+                        a = mfma(S.w[(tb + t * 2 + p) * 64 + lane], cat(dzb[2 * p], dzb[2 * p + 1]), a);
                     dh[t] = a;
                 }
             }
@@ -365,14 +377,14 @@ __global__ __launch_bounds__(256) void k_train(MlpDev m, MlpTrainParams tp) {
         const uint16_t* xs = w == 0 ? S.x1 : S.x[w - 1];
         const int ntiles = w == 0 ? 6 : 4;
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
+        for (int ks = 0; ks < 2; ++ks) {
 #pragma unroll
             for (int mo = 0; mo < 4; ++mo) {
-                const s4 a = *reinterpret_cast<const s4*>(dz + (16 * mo + cs) * 64 + 16 * ks + 4 * g);
+                const s8 a = *reinterpret_cast<const s8*>(dz + (16 * mo + cs) * 64 + 32 * ks + 8 * g);
 #pragma unroll
                 for (int ti = 0; ti < 6; ++ti) {
                     if (ti < ntiles) {
-                        const s4 b = *reinterpret_cast<const s4*>(xs + (16 * ti + cs) * 64 + 16 * ks + 4 * g);
+                        const s8 b = *reinterpret_cast<const s8*>(xs + (16 * ti + cs) * 64 + 32 * ks + 8 * g);
                         acc[mo * 6 + ti] = mfma(a, b, acc[mo * 6 + ti]);
                     }
                 }
@@ -409,13 +421,25 @@ __global__ __launch_bounds__(256) void k_train(MlpDev m, MlpTrainParams tp) {
     if (threadIdx.x == 0) atomicAdd(tp.loss, S.loss);
 }
 
-// Sums the block partials; either returns the gradient or applies one Adam step.
-__global__ void k_reduce_adam(MlpDev m, const float* __restrict__ partial, uint32_t nblocks, float* grad_out,
-                              AdamParams ap) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpParamCount) return;
+// Sums the block partials; either returns the gradient or applies one Adam step. A block
+// covers 64 parameters with 16 wavefronts: wavefront r sums the partial rows b = r mod 16
+// (coalesced 256-byte rows), and the 16 sums meet in LDS in a fixed order, so the result does
+// not depend on timing.
+constexpr int kReduceParams = 64, kReduceRows = 16;
+__global__ __launch_bounds__(kReduceParams * kReduceRows) void k_reduce_adam(MlpDev m, const float* __restrict__ partial,
+                                                                             uint32_t nblocks, float* grad_out,
+                                                                             AdamParams ap) {
+    __shared__ float s_sum[kReduceRows][kReduceParams];
+    const int lane = threadIdx.x & (kReduceParams - 1), r = threadIdx.x / kReduceParams;
+    const int p = blockIdx.x * kReduceParams + lane;
+    float part = 0.0f;
+    if (p < kMlpParamCount)
+        for (uint32_t b = uint32_t(r); b < nblocks; b += kReduceRows) part += partial[size_t(b) * kMlpParamCount + p];
+    s_sum[r][lane] = part;
+    __syncthreads();
+    if (r != 0 || p >= kMlpParamCount) return;
     float g = 0.0f;
-    for (uint32_t b = 0; b < nblocks; ++b) g += partial[size_t(b) * kMlpParamCount + p];
+    for (int k = 0; k < kReduceRows; ++k) g += s_sum[k][lane];
     if (grad_out) {
         grad_out[p] = g;
         return;
@@ -445,7 +469,7 @@ __global__ void k_collect(const float4* __restrict__ rays, const HitRec* __restr
 size_t mlp_train_lds_bytes() { return sizeof(mlp_detail::TrainLds); }
 
 hipError_t launch_mlp_pack(const MlpDev& m, hipStream_t st) {
-    hipLaunchKernelGGL(mlp_detail::k_pack, dim3((mlp_detail::kFragsAll * 256 + 255) / 256), dim3(256), 0, st, m.master,
+    hipLaunchKernelGGL(mlp_detail::k_pack, dim3((mlp_detail::kFragsAll * 512 + 255) / 256), dim3(256), 0, st, m.master,
                        m.packed);
     return hipGetLastError();
 }
@@ -481,8 +505,10 @@ hipError_t launch_mlp_shadow_gate(const MlpDev& m, const MlpShadowParams& sp, ui
 hipError_t launch_mlp_train_step(const MlpDev& m, const MlpTrainParams& tp, uint32_t nblocks, float* grad_out,
                                  const AdamParams& ap, hipStream_t st) {
     hipLaunchKernelGGL(mlp_detail::k_train, dim3(nblocks), dim3(256), mlp_train_lds_bytes(), st, m, tp);
-    hipLaunchKernelGGL(mlp_detail::k_reduce_adam, dim3((kMlpParamCount + 255) / 256), dim3(256), 0, st, m, tp.partial,
-                       nblocks, grad_out, ap);
+    hipLaunchKernelGGL(mlp_detail::k_reduce_adam,
+                       dim3((kMlpParamCount + mlp_detail::kReduceParams - 1) / mlp_detail::kReduceParams),
+                       dim3(mlp_detail::kReduceParams * mlp_detail::kReduceRows), 0, st, m, tp.partial, nblocks, grad_out,
+                       ap);
     if (!grad_out) return launch_mlp_pack(m, st);
     return hipGetLastError();
 }

@@ -1119,9 +1119,28 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
     int state = kIdle;
     bool ray_done = false;
     uint32_t qn = 0;         // wave-uniform queue length
+#ifdef VKHRT_WORK_COUNTERS
+    unsigned long long wc[14] = {};  // WorkCounter sums of this wave (wave-uniform)
+    unsigned long long t_it = clock64(), t_mid = t_it;
+    wc[kWcWaves] = 1;
+#define VK_WC(i, v) (wc[i] += (unsigned long long)(v))
+#define VK_WC_MARK() (t_mid = clock64())
+#define VK_WC_END(kind)                                   \
+    do {                                                  \
+        const unsigned long long t_now = clock64();       \
+        wc[kWcCyclesOther] += t_mid - t_it;               \
+        wc[kind] += t_now - t_mid;                        \
+        t_it = t_mid = t_now;                             \
+    } while (0)
+#else
+#define VK_WC(i, v) ((void)0)
+#define VK_WC_MARK() ((void)0)
+#define VK_WC_END(kind) ((void)0)
+#endif
     // Every wave leaves the loop: a normal frame takes ~10^4 iterations per wave; past the cap
     // the wave reports an error (status[0]) and stops, leaving its rays unwritten.
     for (uint32_t guard = 0;; ++guard) {
+        VK_WC(kWcIterations, 1);
         if (guard >= kCompactIterationCap) {
             if (lane == 0) atomicAdd(&tp.status[0], 1u);
             break;
@@ -1159,6 +1178,9 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
             }
         }
         // (c) next instance whose world box the ray overlaps (top_level_acceleration_structure.cpp:23-43)
+#ifdef VKHRT_WORK_COUNTERS
+        bool finished = false;
+#endif
         if (state == kInst) {
             while (inst < tp.ninst) {
                 const InstanceDev& I = tp.inst[inst];
@@ -1196,6 +1218,9 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
                 }
                 snk(idx, ow, dw, best);
                 state = kIdle;
+#ifdef VKHRT_WORK_COUNTERS
+                finished = true;
+#endif
             } else {
                 const InstanceDev& I = tp.inst[inst];
                 f3 oo = ow, dd = dw;
@@ -1207,6 +1232,9 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
                 state = kTrav;
             }
         }
+#ifdef VKHRT_WORK_COUNTERS
+        VK_WC(kWcRays, __popcll(__ballot(finished)));
+#endif
         // (c') once the counter is exhausted, idle lanes take the bottom (largest) stacked node
         // group of a walking lane and continue that ray from there, so the last long rays of
         // the frame are walked by many lanes instead of one
@@ -1241,6 +1269,7 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 if (give && kd < n) ++W.sb;  // the group now belongs to the helper
+                VK_WC(kWcHandovers, n);
             }
         }
         // lanes that finished in (c) take new rays in the next iteration: leave only when the
@@ -1250,8 +1279,11 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
             continue;
         }
         // (d) intersection round on up to 64 queued pairs, one per lane
+        VK_WC_MARK();
         if (qn >= 64u || (qn != 0u && !__ballot(state == kTrav || state == kHelp))) {
             const uint32_t m = qn < 64u ? qn : 64u;
+            VK_WC(kWcRounds, 1);
+            VK_WC(kWcRoundLanes, m);
             const uint32_t e = lane < m ? queue[qn - m + lane] : 0u;
             const int owner = int(e >> kCompactSlotBits);
             const uint32_t slot = e & ((1u << kCompactSlotBits) - 1u);
@@ -1305,9 +1337,18 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
                     state = kWait;
                 }
             }
+            VK_WC_END(kWcCyclesRounds);
             continue;
         }
         // (e) one traversal step per traversing lane; survivors of the early-out are queued
+#ifdef VKHRT_WORK_COUNTERS
+        {
+            const bool wk = state == kTrav || state == kHelp;
+            VK_WC(kWcNodeTests, __popcll(__ballot(wk && W.pmask == 0u && W.have_node)));
+            VK_WC(kWcLeafTests, __popcll(__ballot(wk && W.pmask != 0u)));
+            VK_WC(kWcStepLanes, __popcll(__ballot(wk)));
+        }
+#endif
         uint32_t cand = 0xFFFFFFFFu;
         if (state == kHelp && ANY && uint32_t(s_key[wbase + home]) != kNoHitLow) {
             atomicSub(&s_walk[wbase + home], 1u);  // the ray is occluded already
@@ -1334,9 +1375,22 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
             atomicAdd(&s_cnt[wbase + home], 1u);
         }
         qn += __popcll(am);
+        VK_WC(kWcSteps, 1);
+        VK_WC(kWcPairs, __popcll(am));
+        VK_WC_END(kWcCyclesSteps);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
+#ifdef VKHRT_WORK_COUNTERS
+    if (lane == 0 && tp.work_counts) {
+        const uint32_t base = ANY ? kWorkAnyHit : 0u;
+        for (int i = 0; i < 14; ++i)
+            if (wc[i]) atomicAdd(&tp.work_counts[base + i], wc[i]);
+    }
+#endif
+#undef VK_WC
+#undef VK_WC_MARK
+#undef VK_WC_END
 }
 
 // Persistent loop for one geometry variant: compacted intersection rounds for phantom curves
@@ -1732,16 +1786,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_accurate_queued(TraceParams tp,
 // Any-hit traversal of a short shadow queue (the neural form's exact fallback): the whole
 // wavefront walks one ray. Persistent lanes take one ray each and hide memory latency behind
 // the other rays of a long queue; with a few thousand rays the frame waits for the slowest
-// ray, whose walk is a chain of dependent node loads. Here the 64 lanes pop up to 64 node
-// groups of the ray at once from a stack shared in LDS, test them in parallel and push the
-// internal children they hit; the leaf records hit are early-out tested and intersected 64 at a
-// time. The walk ends at the first hit in [tmin, tmax] (the ray is occluded) or when the stack
-// is empty (visible). Grazing candidates are deferred to the FP64 polish as in the other
-// kernels, so the visibility equals the persistent kernel's for every ray.
-constexpr uint32_t kWaveStack = 6144;   // node entries per wave (24 KB)
-constexpr uint32_t kWaveLeaves = 1600;  // leaf records of one step: <= 64 nodes x 24 records
+// ray, whose walk is a chain of dependent loads. Here each iteration of the wave pops up to 64
+// nodes of its ray from a stack in LDS and takes up to 64 pending leaf records from a queue in
+// LDS, one per lane: the loads of both are issued together, so an iteration costs one memory
+// round trip. Node tests push the internal children they hit and append their leaf records;
+// records run the early-out and Prhi. The walk ends at the first hit in [tmin, tmax]
+// (occluded) or when both are empty (visible). Grazing candidates are deferred to the FP64
+// polish as in the other kernels, so the visibility equals the persistent kernel's for every
+// ray. Nodes are popped only while the record queue has room for all they can append.
+constexpr uint32_t kWaveStack = 2048;      // pending nodes per wave (8 KB); a walk past it reports an overflow
+constexpr uint32_t kWaveLeaves = 2048;     // pending leaf records per wave (8 KB)
 constexpr uint32_t kWaveRayLimit = 16384;  // queues up to this long take the wave-per-ray kernel
-constexpr uint32_t kWaveGrid = 4096;       // one-wave blocks of the wave-per-ray launch
+constexpr uint32_t kWaveGrid = 8192;       // one-wave blocks of the wave-per-ray launch
 
 template <class G, int IM>
 __global__ __launch_bounds__(64) void k_render_shadow_wave(RenderParams rp, const uint32_t* queue, const uint32_t* count,
@@ -1754,7 +1810,14 @@ __global__ __launch_bounds__(64) void k_render_shadow_wave(RenderParams rp, cons
     const TraceParams& tp = rp.tp;
     const uint32_t lane = threadIdx.x;
     const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
+#ifdef VKHRT_WORK_COUNTERS
+    unsigned long long wc[7] = {};
+#endif
     for (uint32_t qi = blockIdx.x; qi < total; qi += gridDim.x) {
+#ifdef VKHRT_WORK_COUNTERS
+        const unsigned long long t_ray = clock64();
+        ++wc[0];
+#endif
         f3 ow, dw;
         float tmin, tmax;
         src.get(qi, ow, dw, tmin, tmax);
@@ -1772,15 +1835,32 @@ __global__ __launch_bounds__(64) void k_render_shadow_wave(RenderParams rp, cons
             const uint32_t oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
             RayFrame F;
             make_frame(o, d, F);
-            uint32_t sp = 1;  // wave-uniform; the root is node 0
+            uint32_t sp = 1, lh = 0, lt = 0;  // wave-uniform: stack size, record queue head and tail
             if (lane == 0) s_stack[0] = 0u;
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             bool error = false;
-            while (sp > 0 && !occluded && !error) {
-                const uint32_t m = sp < 64u ? sp : 64u;
+            while ((sp > 0 || lh < lt) && !occluded && !error) {
+                // this iteration's nodes (only while the record queue can take 64 nodes' records)
+                const uint32_t m = lt + 64u * 24u <= kWaveLeaves ? (sp < 64u ? sp : 64u) : 0u;
                 const uint32_t node = lane < m ? s_stack[sp - m + lane] : 0xFFFFFFFFu;
                 sp -= m;
+                // this iteration's records
+                const uint32_t nl = lt - lh < 64u ? lt - lh : 64u;
+                const uint32_t slot = lane < nl ? s_leaf[lh + lane] : 0xFFFFFFFFu;
+                lh += nl;
+                if (lh == lt) lh = lt = 0;
+                // issue the record's loads before the node test so both are in flight together
+                float4 cv[4] = {};
+                float rmax = 0.0f;
+                if (slot != 0xFFFFFFFFu) {
+                    const auto* pr = gmem(B.prims + size_t(slot) * 4);
+                    cv[0] = pr[0];
+                    cv[1] = pr[1];
+                    cv[2] = pr[2];
+                    cv[3] = pr[3];
+                    rmax = gmem(B.aux)[slot];
+                }
                 uint32_t hits = 0, pmask = 0, child_base = 0, imask = 0, pbase = 0;
                 if (node != 0xFFFFFFFFu) {
                     if (node < B.nnodes) {
@@ -1793,64 +1873,75 @@ __global__ __launch_bounds__(64) void k_render_shadow_wave(RenderParams rp, cons
                         error = true;
                     }
                 }
+                // the record: early out, then Prhi
+                bool hit = false, pass = false;
+                if (slot != 0xFFFFFFFFu) {
+                    pass = ray_cylinder(o, d, f3{cv[0].x, cv[0].y, cv[0].z}, f3{cv[3].x, cv[3].y, cv[3].z}, rmax);
+                    if (pass) {
+                        PhantomHit ph;
+                        hit = prhi<G::kRMode, G::kPolish ? kPolishDefer : kPolishOff, false>(F, cv, 0.0f, B.radius, ph);
+                        if (G::kPolish && ph.deferred) {
+                            const uint32_t q = atomicAdd(tp.defer_count, 1u);
+                            if (q < tp.defer_cap)
+                                tp.defer[q] = make_uint4(qi, inst, slot, 0u);
+                            else
+                                atomicAdd(&tp.status[1], 1u);
+                        }
+                        hit = hit && ph.t >= tmin && ph.t <= tmax;
+                    }
+                }
+                occluded = __ballot(hit) != 0;
                 error = __ballot(error) != 0;
-                // push the internal children hit (octant bit j -> slot j ^ oct -> child index)
+#ifdef VKHRT_WORK_COUNTERS
+                ++wc[1];
+                wc[2] += m;
+                wc[3] += nl;
+                wc[4] += __popcll(__ballot(pass));
+                wc[5] += nl ? 1u : 0u;
+#else
+                (void)pass;
+#endif
+                // push the internal children hit (octant bit j -> slot j ^ oct -> child index) and
+                // append the leaf records hit (inclusive prefix sums over the lanes)
                 const uint32_t nc = __popc(hits), np = __popc(pmask);
-                uint32_t cpre = nc, ppre = np;  // inclusive prefix sums over the lanes
+                uint32_t cpre = nc, ppre = np;
 #pragma unroll
                 for (int k = 1; k < 64; k <<= 1) {
-                    const uint32_t a = __shfl_up(cpre, k), b = __shfl_up(ppre, k);
+                    const uint32_t x = __shfl_up(cpre, k), y = __shfl_up(ppre, k);
                     if (int(lane) >= k) {
-                        cpre += a;
-                        ppre += b;
+                        cpre += x;
+                        ppre += y;
                     }
                 }
                 const uint32_t ctot = __shfl(cpre, 63), ptot = __shfl(ppre, 63);
-                if (sp + ctot > kWaveStack || ptot > kWaveLeaves) {
+                if (sp + ctot > kWaveStack) {
                     error = true;
                     break;
                 }
                 uint32_t w = sp + cpre - nc;
                 for (uint32_t hb = hits; hb; hb &= hb - 1u) {
-                    const uint32_t j = __ffs(hb) - 1, slot = j ^ oct;
-                    s_stack[w++] = child_base + __popc(imask & ((1u << slot) - 1u));
+                    const uint32_t j = __ffs(hb) - 1, sl = j ^ oct;
+                    s_stack[w++] = child_base + __popc(imask & ((1u << sl) - 1u));
                 }
-                w = ppre - np;
+                w = lt + ppre - np;
                 for (uint32_t pm = pmask; pm; pm &= pm - 1u) s_leaf[w++] = pbase + uint32_t(__ffs(pm) - 1);
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                __builtin_amdgcn_wave_barrier();
                 sp += ctot;
-                // the leaf records of this step, 64 at a time: early out, then Prhi
-                for (uint32_t base = 0; base < ptot && !occluded; base += 64u) {
-                    bool hit = false;
-                    if (base + lane < ptot) {
-                        const uint32_t slot = s_leaf[base + lane];
-                        const auto* pr = gmem(B.prims + size_t(slot) * 4);
-                        const float4 a = pr[0], b = pr[3];
-                        if (ray_cylinder(o, d, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, gmem(B.aux)[slot])) {
-                            float4 cv[4] = {a, pr[1], pr[2], b};
-                            PhantomHit ph;
-                            hit = prhi<G::kRMode, G::kPolish ? kPolishDefer : kPolishOff, false>(F, cv, 0.0f, B.radius,
-                                                                                                  ph);
-                            if (G::kPolish && ph.deferred) {
-                                const uint32_t q = atomicAdd(tp.defer_count, 1u);
-                                if (q < tp.defer_cap)
-                                    tp.defer[q] = make_uint4(qi, inst, slot, 0u);
-                                else
-                                    atomicAdd(&tp.status[1], 1u);
-                            }
-                            hit = hit && ph.t >= tmin && ph.t <= tmax;
-                        }
-                    }
-                    occluded = __ballot(hit) != 0;
-                }
+                lt += ptot;
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
             if (error && lane == 0) atomicAdd(&tp.status[0], 1u);
         }
         if (lane == 0) rp.vis[queue[qi]] = occluded ? 0.0f : 1.0f;
+#ifdef VKHRT_WORK_COUNTERS
+        wc[6] += clock64() - t_ray;
+#endif
     }
+#ifdef VKHRT_WORK_COUNTERS
+    if (lane == 0 && tp.work_counts)
+        for (int i = 0; i < 7; ++i)
+            if (wc[i]) atomicAdd(&tp.work_counts[kWorkWave + i], wc[i]);
+#endif
 }
 
 // The same two traversal passes for TLASes that mix geometry modes: one thread per queued
