@@ -76,10 +76,15 @@ def main():
                     "refill minimum and ray order with the autotuner before timing (1) or use the flags (0)")
     ap.add_argument("--neural-views", type=int, default=32, help="training views for the shadow MLP")
     ap.add_argument("--neural-epochs", type=int, default=80)
+    ap.add_argument("--neural-thresholds", type=str, default="",
+                    help="comma-separated variance thresholds of the neural sweeps (default: the fixed list)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain float32 copy kernel over MIB MiB three times and exit "
                          "(the known-size reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.neural_thresholds:
+        global NEURAL_THRESHOLDS
+        NEURAL_THRESHOLDS = tuple(float(x) for x in args.neural_thresholds.split(","))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration << 20)
         return
