@@ -189,16 +189,15 @@ inline bool IntersectTriangle(V3<float> o, V3<float> d, V3<float> v0, V3<float> 
     return true;
 }
 
-// geometry_processor.cpp:201-213 PerpStark and :216-219 BuildFrame.
+// geometry_processor.cpp:201-213 PerpStark and :216-219 BuildFrame: Stark's branch-free
+// perpendicular (JGT 14(1), 2009), u crossed with the axis of its smallest-magnitude component
+// (x only if strictly smallest, then y if strictly below z, else z), normalized. For IEEE floats
+// with gradual underflow, a < b decides exactly as the reference's (a - b) < 0.
 inline V3<float> PerpStark(V3<float> u) {
-    V3<float> a = mk(std::fabs(u.x), std::fabs(u.y), std::fabs(u.z));
-    uint32_t uyx = (a.x - a.y) < 0 ? 1 : 0;
-    uint32_t uzx = (a.x - a.z) < 0 ? 1 : 0;
-    uint32_t uzy = (a.y - a.z) < 0 ? 1 : 0;
-    uint32_t xm = uyx & uzx;
-    uint32_t ym = (1 ^ xm) & uzy;
-    uint32_t zm = 1 ^ (xm | ym);
-    return normalize(cross(u, mk(float(xm), float(ym), float(zm))));
+    const float ax = std::fabs(u.x), ay = std::fabs(u.y), az = std::fabs(u.z);
+    const bool along_x = ax < ay && ax < az;
+    const bool along_y = !along_x && ay < az;
+    return normalize(cross(u, mk(along_x ? 1.0f : 0.0f, along_y ? 1.0f : 0.0f, along_x || along_y ? 0.0f : 1.0f)));
 }
 
 // DOTS piece: two orthogonal quads through the line (geometry_processor.cpp:221-271).

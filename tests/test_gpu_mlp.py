@@ -59,7 +59,11 @@ def test_forward_matches_oracle(gpu_ctx, O):
     err = np.abs(got - ref)
     print(f"forward |gpu - oracle(bf16)| max {err.max():.3g} mean {err.mean():.3g}; "
           f"|gpu - fp64| max {np.abs(got - exact).max():.3g}")
-    assert err.max() < 1e-2 and err.mean() < 1e-3
+    # Both sides round every activation to bf16 and accumulate in fp32; the MFMA and the oracle
+    # sum in different orders, so an activation near a bf16 rounding boundary can round to the
+    # neighbouring value (one bf16 ulp, 2^-8 relative). Such flips are rare and bounded: the mean
+    # error stays at fp32 rounding level, the largest at a few 1e-4 (measured 2.2e-4, mean 1e-7).
+    assert err.max() < 2e-3 and err.mean() < 1e-5
     assert np.abs(got - exact).max() < 0.05
 
 
@@ -80,15 +84,15 @@ def test_gradient_matches_oracle(gpu_ctx, O):
     g = grad.cpu().numpy()
     center, inv_half = _box(O)
     gref, lref = O.mlp_grad(P, center, inv_half, smp, tgt)
-    assert abs(loss - lref) < 1e-3 * max(1.0, abs(lref)), (loss, lref)
+    assert abs(loss - lref) < 1e-5 * max(1.0, abs(lref)), (loss, lref)
     rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
     print(f"gradient relative L2 error vs oracle {rel:.3g}; loss {loss:.6f} vs {lref:.6f}")
-    assert rel < 2e-2
+    assert rel < 1e-4  # measured 5.9e-6: the same bf16 rounding flips as the forward pass
     for name, size in [("W1", 6144), ("W2", 4096), ("W3", 4096), ("W4", 4096), ("w5", 64)]:
         a = g[O.MLP_OFF[name]:O.MLP_OFF[name] + size]
         b = gref[O.MLP_OFF[name]:O.MLP_OFF[name] + size]
         cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
-        assert cos > 0.999, (name, cos)
+        assert cos > 0.99999, (name, cos)
     # the same parameters are untouched by a gradient evaluation
     assert np.array_equal(np.array(mlp.get_params(), np.float32), P)
 

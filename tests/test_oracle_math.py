@@ -426,6 +426,24 @@ def test_perp_stark_and_dots_pieces():
         v = np.zeros(3, np.float32)
         L.orc_perp_stark(O.ptr(u), O.ptr(v))
         assert abs(np.linalg.norm(v) - 1) < 1e-6 and abs(u @ v) < 1e-6
+        # Stark's choice: the axis of the smallest |component| keeps |u x e| >= sqrt(2/3)
+        e = np.zeros(3)
+        e[int(np.argmin(np.abs(u)))] = 1.0
+        assert abs(abs(v @ (np.cross(u, e) / np.linalg.norm(np.cross(u, e)))) - 1) < 1e-6
+    # ties (geometry_processor.cpp:201-213): x only if strictly smallest, then y if strictly
+    # below z, else z; closed forms of u x axis, normalized
+    cases = [((1, 1, 1), lambda u: (u[1], -u[0], 0)),      # all equal: z
+             ((1, 1, 2), lambda u: (-u[2], 0, u[0])),      # x ties y, both below z: y
+             ((2, 1, 1), lambda u: (u[1], -u[0], 0)),      # y ties z: z
+             ((1, 2, 2), lambda u: (0, u[2], -u[1])),      # x strictly smallest: x
+             ((-3, 1, -1), lambda u: (u[1], -u[0], 0))]    # |y| ties |z|: z
+    for raw, form in cases:
+        u = np.array(raw, np.float64)
+        u = (u / np.linalg.norm(u)).astype(np.float32)
+        v = np.zeros(3, np.float32)
+        L.orc_perp_stark(O.ptr(u), O.ptr(v))
+        want = np.array(form(u.astype(np.float64)))
+        assert np.abs(v - want / np.linalg.norm(want)).max() < 1e-6, (raw, v)
     lines = np.array([[0, 0, 0, 0.04, 1.5, 0.3, 0.2, 0.03]], np.float32)
     pieces = O.generate_dots(lines)
     tris = O.dots_triangles(pieces)

@@ -143,16 +143,17 @@ __global__ void k_lss(const float4* __restrict__ L, uint32_t n, float constant_r
     out[size_t(i) * 2 + 1] = b;
 }
 
-// PerpStark (geometry_processor.cpp:201-213).
+// Branch-free perpendicular of M. M. Stark ("Efficient construction of perpendicular vectors
+// without branching", Journal of Graphics, GPU, and Game Tools 14(1), 2009), the frame vector of
+// the DOTS quads (PerpStark, geometry_processor.cpp:201-213): u crossed with the coordinate axis of
+// u's smallest-magnitude component, x only if strictly smallest, then y if strictly below z, else
+// z; normalized. The order of the tests fixes which axis a tie picks.
 __device__ __forceinline__ f3 perp_stark(f3 u) {
-    f3 a = f3{fabsf(u.x), fabsf(u.y), fabsf(u.z)};
-    uint32_t uyx = (a.x - a.y) < 0.0f ? 1u : 0u;
-    uint32_t uzx = (a.x - a.z) < 0.0f ? 1u : 0u;
-    uint32_t uzy = (a.y - a.z) < 0.0f ? 1u : 0u;
-    uint32_t xm = uyx & uzx;
-    uint32_t ym = (1u ^ xm) & uzy;
-    uint32_t zm = 1u ^ (xm | ym);
-    return nrm3(cross3(u, f3{float(xm), float(ym), float(zm)}));
+    const float ax = fabsf(u.x), ay = fabsf(u.y), az = fabsf(u.z);
+    const bool along_x = ax < ay && ax < az;
+    const bool along_y = !along_x && ay < az;
+    const f3 axis = f3{along_x ? 1.0f : 0.0f, along_y ? 1.0f : 0.0f, along_x || along_y ? 0.0f : 1.0f};
+    return nrm3(cross3(u, axis));
 }
 
 // DOTS piece per line (geometry_processor.cpp:221-271): frame s = PerpStark(fwd),
