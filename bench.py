@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles")
     ap.add_argument("--xcd-chunk", type=int, default=4096,
                     help="rays handed out per XCD in runs of this many (0: one work counter)")
+    ap.add_argument("--shadow-group-lanes", type=int, default=8,
+                    help="lanes per ray of the neural fallback's lane-group walker (8, 16, 32 or 64)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
@@ -111,6 +113,7 @@ def main():
     ctx.set_option(V.OPTION_RAY_ORDER, args.ray_order)
     ctx.set_option(V.OPTION_XCD_CHUNK, args.xcd_chunk)
     ctx.set_option(V.OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
+    ctx.set_option(V.OPTION_SHADOW_GROUP_LANES, args.shadow_group_lanes)
 
     NS, SEG, W, H = args.strands, args.segments, args.width, args.height
     nseg_total = NS * SEG
@@ -870,6 +873,7 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
                    "frame_speedup_vs_exact": round(exact_ms / neural_ms, 3),
                    "choice": "largest threshold of the sweep whose frame traces part of the queries exactly "
                              "and keeps PSNR >= 40 dB against the exact frame",
+                   "fallback_walk": {"lanes_per_ray": args.shadow_group_lanes, "queue_limit": 16384},
                    "threshold_sweep": sweep},
         "train": {"samples": ntrain, "epochs": epochs, "batch": 65536, "seconds": round(train_s, 2),
                   "trace_seconds": round(collect_s, 2), "final_loss": round(loss, 5),

@@ -244,6 +244,11 @@ int vkhrt_abi_version(void);
                                      of two in 64..2^20 (default 4096): the rays are handed out in
                                      runs of this many, run k by the counter of XCD k mod 8 (each
                                      XCD's L2 then serves its own image regions) */
+#define VKHRT_OPTION_SHADOW_WAVE_LIMIT 10 /* neural shadow form: a fallback queue of up to this many
+                                            rays (0..2^20, default 16384) is walked by lane groups, one
+                                            ray per group, a longer one by the persistent lanes; 0 =
+                                            always the persistent lanes. Results do not depend on it */
+#define VKHRT_OPTION_SHADOW_GROUP_LANES 11 /* lanes per ray of that walk: 8 (default), 16, 32 or 64 */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
 
 /* ---- acceleration structures */

@@ -1460,4 +1460,10 @@ int orc_mlp_grad(const void* params, const float center[3], const float inv_half
 
 int orc_mlp_param_count(void) { return kMlpParams; }
 
+// The kernels' FP32 sin(pi v), cos(pi v) of the MLP features, elementwise.
+int orc_sincospi_f32(const float* v, uint32_t n, float* s, float* c) {
+    for (uint32_t i = 0; i < n; ++i) sincospi_f32(v[i], s[i], c[i]);
+    return 0;
+}
+
 }  // extern "C"

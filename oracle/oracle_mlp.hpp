@@ -41,6 +41,28 @@ inline float bf16_round(float x) {
 inline double sinpi_d(double v) { return std::sin(M_PI * (v - 2.0 * std::nearbyint(0.5 * v))); }
 inline double cospi_d(double v) { return std::cos(M_PI * (v - 2.0 * std::nearbyint(0.5 * v))); }
 
+// The kernels' FP32 evaluation of sin(pi v), cos(pi v) (|v| < 2^22), operation for operation:
+// v = f + n / 2 with n = rint(2 v), |f| <= 1/4 (exact); odd / even polynomials in f whose
+// coefficients are a least-squares minimax fit on [0, 1/4] rounded to FP32; quadrant n mod 4.
+// Its accuracy against sin / cos in double is pinned by tests/test_mlp_oracle.py.
+inline void sincospi_f32(float v, float& s, float& c) {
+    const float n = std::nearbyint(2.0f * v);  // round half to even, as v_rndne_f32
+    const float f = std::fma(-0.5f, n, v);
+    const float u = f * f;
+    float ps = std::fma(u, -0.587813f, 2.549629f);
+    ps = std::fma(u, ps, -5.1677036f);
+    ps = std::fma(u, ps, 3.1415925f);
+    const float sf = f * ps;
+    float pc = std::fma(u, 0.23142953f, -1.3350585f);
+    pc = std::fma(u, pc, 4.0587077f);
+    pc = std::fma(u, pc, -4.934802f);
+    const float cf = std::fma(u, pc, 1.0f);
+    const int q = int(n) & 3;
+    const float a = (q & 1) ? cf : sf, b = (q & 1) ? sf : cf;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
 template <class T, bool BF16>
 struct MlpRef {
     static T rd(T x) {
@@ -57,11 +79,23 @@ struct MlpRef {
         for (int i = 0; i < 3; ++i) x[3 + i] = T(sample[4 + i]);
         for (int i = 0; i < kMlpIn; ++i)
             for (int k = 0; k < kMlpFreq; ++k) {
-                const double v = std::ldexp(double(x[i]), k);
-                T s = T(sinpi_d(v)), c = T(cospi_d(v));
+                T s, c;
                 if constexpr (BF16) {
-                    s = T(float(sinpi_d(v)));
-                    c = T(float(cospi_d(v)));
+                    // the kernels' FP32 features: the polynomial at even k, the double-angle
+                    // formulas from frequency k - 1 at odd k
+                    float s0, c0;
+                    sincospi_f32(std::ldexp(float(x[i]), k & ~1), s0, c0);
+                    if (k & 1) {
+                        const float s1 = 2.0f * s0 * c0, c1 = (c0 - s0) * (c0 + s0);
+                        s0 = s1;
+                        c0 = c1;
+                    }
+                    s = T(s0);
+                    c = T(c0);
+                } else {
+                    const double v = std::ldexp(double(x[i]), k);
+                    s = T(sinpi_d(v));
+                    c = T(cospi_d(v));
                 }
                 enc[(i * kMlpFreq + k) * 2 + 0] = rd(s);
                 enc[(i * kMlpFreq + k) * 2 + 1] = rd(c);

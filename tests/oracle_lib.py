@@ -85,6 +85,7 @@ def lib():
         "orc_mlp_forward": ([P, P, P, P, u32, i32, P], i32),
         "orc_mlp_grad": ([P, P, P, P, P, u32, i32, P, ctypes.POINTER(ctypes.c_double)], i32),
         "orc_mlp_param_count": ([], i32),
+        "orc_sincospi_f32": ([P, u32, P, P], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

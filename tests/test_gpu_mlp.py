@@ -204,3 +204,54 @@ def test_neural_shadow_render(gpu_ctx):
     assert p_trained >= 40.0
     assert tr < q  # the network answers part of the queries
     assert tr < utr or p_trained > p_untrained + 3.0
+
+
+@pytest.mark.parametrize("area", [False, True], ids=["directional", "area2"])
+def test_wave_fallback_equals_persistent_lanes(gpu_ctx, area):
+    """The neural form's exact fallback gives the same image whether its queue is walked one
+    ray per wavefront (the short-queue kernel, forced for every queue length here) or by the
+    persistent lanes (wave limit 0): bit for bit at every threshold, FP64 polish on. At
+    threshold 0 every query takes the fallback, and the image equals the exact frame. An
+    untrained network is deterministic and uncertain, so the thresholds split the queries."""
+    import gpu_util as G
+    import vkhrt_amd as V
+    ctx = gpu_ctx
+    blas, tlas = _hair_tlas(ctx, 20000, 78)
+    mlp = V.ShadowMLP(ctx, SCENE_MIN, SCENE_MAX, seed=12)
+    W = H = 384
+    cam = V.camera_look_at((120.0, 90.0, 330.0), (0.0, 40.0, 0.0), aspect=1.0)
+    light = V.area_light((0.0, 260.0, 0.0), (100.0, 0.0, 0.0), (0.0, 0.0, 100.0), samples=2, seed=3) if area else None
+    flags = V.RENDER_SHADOWS | V.RENDER_FP64_POLISH
+
+    def frame(net, thr):
+        img = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+        f = flags | (V.RENDER_NEURAL_SHADOWS if net is not None else 0)
+        ctx.render(tlas, cam, W, H, img.data_ptr(), flags=f, shadow_mlp=net, variance_threshold=thr,
+                   area_light=light, stream=G.stream_ptr())
+        torch.cuda.synchronize()
+        _, q, tr = ctx.render_stats()
+        return img.cpu().numpy(), q, tr
+
+    exact, _, _ = frame(None, 0.0)
+    partial = False
+    try:
+        for thr in (0.0, 0.2, 0.24, 0.245, 0.249, 0.2499):
+            ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 1 << 20)
+            a, q, tr = frame(mlp, thr)
+            ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 0)
+            b, q2, tr2 = frame(mlp, thr)
+            print(f"[wave fallback] threshold {thr}: {tr}/{q} queries traced", flush=True)
+            assert (q, tr) == (q2, tr2)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), thr
+            if thr == 0.0:
+                assert tr == q and np.array_equal(a.view(np.uint32), exact.view(np.uint32))
+            partial = partial or 0 < tr < q
+    finally:
+        ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 16384)
+    assert partial
+    for opt, v in ((V.OPTION_SHADOW_WAVE_LIMIT, -1), (V.OPTION_SHADOW_WAVE_LIMIT, (1 << 20) + 1)):
+        with pytest.raises(V.VkhrtError):
+            ctx.set_option(opt, v)
+    mlp.close()
+    tlas.close()
+    blas.close()

@@ -90,3 +90,34 @@ def test_bf16_gradient_close_to_exact():
     g32, l32 = O.mlp_grad(P, CENTER, INV_HALF, smp, tgt)
     assert abs(l64 - l32) < 0.02
     assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 0.05
+
+
+def test_feature_sincospi_accuracy_and_closed_forms():
+    """The FP32 sin(pi v) / cos(pi v) that the kernels and the oracle's bf16 form evaluate:
+    within 1.5e-7 of the float64 functions over the feature range |v| <= 64, exact at the
+    multiples of 1/2, and odd / even in v."""
+    L = O.lib()
+    rng = np.random.default_rng(5)
+    v = np.concatenate([np.linspace(-64, 64, 400001), rng.uniform(-64, 64, 100000),
+                        np.arange(-256, 257) / 4.0,                     # quadrant boundaries and 1/4 ties
+                        np.arange(-256, 257) / 4.0 + 1e-6, [0.0, -0.0]]).astype(np.float32)
+    s = np.zeros_like(v)
+    c = np.zeros_like(v)
+    L.orc_sincospi_f32(O.ptr(v), len(v), O.ptr(s), O.ptr(c))
+    vd = v.astype(np.float64)
+    es = np.abs(s - np.sin(np.pi * vd)).max()
+    ec = np.abs(c - np.cos(np.pi * vd)).max()
+    assert es < 1.5e-7 and ec < 1.5e-7, (es, ec)
+    half = np.arange(-128, 129) / 2.0
+    hv = half.astype(np.float32)
+    hs = np.zeros_like(hv)
+    hc = np.zeros_like(hv)
+    L.orc_sincospi_f32(O.ptr(hv), len(hv), O.ptr(hs), O.ptr(hc))
+    assert np.array_equal(hs, np.round(np.sin(np.pi * half)).astype(np.float32))
+    assert np.array_equal(hc, np.round(np.cos(np.pi * half)).astype(np.float32))
+    w = rng.uniform(0, 64, 20000).astype(np.float32)
+    a, b, a2, b2 = (np.zeros_like(w) for _ in range(4))
+    nw = -w
+    L.orc_sincospi_f32(O.ptr(w), len(w), O.ptr(a), O.ptr(b))
+    L.orc_sincospi_f32(O.ptr(nw), len(w), O.ptr(a2), O.ptr(b2))
+    assert np.array_equal(a2, -a) and np.array_equal(b2, b)

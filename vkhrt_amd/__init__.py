@@ -50,6 +50,8 @@ OPTION_BLOCKS_PER_CU = 6
 OPTION_BVH_WIDTH = 7
 OPTION_BVH_COLLAPSE = 9
 OPTION_XCD_CHUNK = 8
+OPTION_SHADOW_WAVE_LIMIT = 10
+OPTION_SHADOW_GROUP_LANES = 11
 
 BATCH_MAX_BOUNCES = 8
 BATCH_MAX_MATERIALS = 16

@@ -43,6 +43,8 @@ struct vkhrt_context {
     uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles
     uint32_t xcd_chunk = 4096;  // render traversal: rays handed out per XCD in runs of this many (0: one counter)
     uint32_t blocks_per_cu = 0;  // persistent grids: 256-thread blocks per CU (0 = occupancy limit)
+    uint32_t shadow_wave_limit = 16384;  // neural fallback queues up to this long: lane groups per ray
+    uint32_t shadow_group_lanes = 8;     // lanes per fallback ray of that walk
     bool profiling = false;
     std::vector<PendingTiming> pending;
     std::map<std::string, std::pair<double, uint32_t>> stats;
@@ -616,6 +618,18 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
         if (value != 0 && (value < 64 || value > (1 << 20) || (value & (value - 1)) != 0))
             return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "XCD chunk must be 0 or a power of two in 64..2^20");
         c->xcd_chunk = uint32_t(value);
+        return VKHRT_SUCCESS;
+    }
+    if (option == VKHRT_OPTION_SHADOW_GROUP_LANES) {
+        if (value != 8 && value != 16 && value != 32 && value != 64)
+            return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "shadow group lanes must be 8, 16, 32 or 64");
+        c->shadow_group_lanes = uint32_t(value);
+        return VKHRT_SUCCESS;
+    }
+    if (option == VKHRT_OPTION_SHADOW_WAVE_LIMIT) {
+        if (value < 0 || value > (1 << 20))
+            return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "shadow wave limit must be 0..2^20");
+        c->shadow_wave_limit = uint32_t(value);
         return VKHRT_SUCCESS;
     }
     if (option == VKHRT_OPTION_BVH_COLLAPSE) {
@@ -1280,6 +1294,8 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
         rp.mlp = &d->shadow_mlp->dev;
         if (al) rp.dirs = c->dirs;
         rp.mlp_threshold = d->shadow_variance_threshold;
+        rp.wave_limit = c->shadow_wave_limit;
+        rp.group_lanes = c->shadow_group_lanes;
         rp.origins = c->origins;
         rp.fallback_queue = c->fallback;
     }

@@ -65,8 +65,9 @@ enum WorkCounter {
     kWcStepLanes = 13,   // lanes walking (node test or leaf test) summed over step iterations
 };
 constexpr uint32_t kWorkAnyHit = 16, kWorkWave = 32, kWorkCounters = 48;
-// wave-per-ray kernel: kWorkWave + 0 rays, 1 steps, 2 node tests, 3 leaf records tested,
-// 4 Prhi calls, 5 iterations that ran records, 6 cycles
+// lane-group shadow walker: kWorkWave + 0 rays, 1 group iterations, 2 node tests, 3 leaf records tested,
+// 4 Prhi calls, 5 iterations that ran records, 6 cycles from a ray's start to its end (summed),
+// 7 the largest of those
 
 struct Camera {
     float view_inv[16];
@@ -145,6 +146,8 @@ struct RenderParams {
     // neural shadow form (optional): the MLP gates the shadow queue, uncertain pixels fall back
     const MlpDev* mlp;         // host-side descriptor (read by the launch code only), null = exact
     float mlp_threshold;       // exact fallback when p (1 - p) > threshold
+    uint32_t wave_limit;       // fallback queues up to this long are walked by lane groups
+    uint32_t group_lanes;      // lanes per fallback ray of that walk: 8, 16, 32 or 64
     float4* origins;           // shadow-ray origin per queue entry
     uint32_t* fallback_queue;  // pixels left to the exact shadow ray ([5] of counters = length)
     uint32_t* live;            // persistent pipeline: pixels whose primary ray enters an instance box

@@ -27,7 +27,10 @@ namespace {
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s2 __attribute__((ext_vector_type(2)));
 
 constexpr int kEnc = 96, kHid = 64;
 constexpr int kOffW1 = 0, kOffB1 = 6144, kOffW2 = 6208, kOffB2 = 10304, kOffW3 = 10368, kOffB3 = 14464;
@@ -44,13 +47,20 @@ __device__ __forceinline__ uint32_t bf16_bits(float x) {
     return u >> 16;
 }
 __device__ __forceinline__ float bf16_val(float x) { return __uint_as_float(bf16_bits(x) << 16); }
+// Two floats to bf16, round to nearest even, a in the low half (gfx950's v_cvt_pk_bf16_f32:
+// the same bits as bf16_bits for finite values).
+__device__ __forceinline__ s2 pack2(float a, float b) {
+    return __builtin_bit_cast(s2, __builtin_convertvector(f2{a, b}, bf2));
+}
 __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
-    s4 v;
-    v.x = short(bf16_bits(a));
-    v.y = short(bf16_bits(b));
-    v.z = short(bf16_bits(c));
-    v.w = short(bf16_bits(d));
-    return v;
+    return __builtin_shufflevector(pack2(a, b), pack2(c, d), 0, 1, 2, 3);
+}
+// ReLU of four accumulators as bf16: max(x, 0) then rounding equals rounding then max, and a
+// negative bf16 (sign bit set) is a negative int16, so the packed integer max clears it.
+__device__ __forceinline__ s4 relu_pack4(f4 a) {
+    const s2 lo = __builtin_elementwise_max(pack2(a[0], a[1]), s2{0, 0});
+    const s2 hi = __builtin_elementwise_max(pack2(a[2], a[3]), s2{0, 0});
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3);
 }
 __device__ __forceinline__ s8 cat(s4 a, s4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
 __device__ __forceinline__ f4 mfma(s8 a, s8 b, f4 c) {
@@ -84,15 +94,40 @@ __global__ void k_pack(const float* __restrict__ master, uint16_t* __restrict__ 
     packed[e] = uint16_t(bf16_bits(master[packed_source(frag, lane, i)]));
 }
 
+// sin(pi v) and cos(pi v) for |v| < 2^22: v = f + n / 2 with n = rint(2 v) and |f| <= 1/4,
+// both exact; odd and even polynomials in f (coefficients of a least-squares minimax fit on
+// [0, 1/4] rounded to FP32: absolute error below 9e-8 for sin and 6e-8 for cos, far below the
+// bf16 rounding of the features), then the quadrant n mod 4. The oracle evaluates the same
+// operations, so the features agree bit for bit.
+constexpr float kSinPi1 = 3.1415925f, kSinPi3 = -5.1677036f, kSinPi5 = 2.549629f, kSinPi7 = -0.587813f;
+constexpr float kCosPi2 = -4.934802f, kCosPi4 = 4.0587077f, kCosPi6 = -1.3350585f, kCosPi8 = 0.23142953f;
+__device__ __forceinline__ void sincospi(float v, float& s, float& c) {
+    const float n = rintf(2.0f * v);
+    const float f = fmaf(-0.5f, n, v);
+    const float u = f * f;
+    float ps = fmaf(u, kSinPi7, kSinPi5);
+    ps = fmaf(u, ps, kSinPi3);
+    ps = fmaf(u, ps, kSinPi1);
+    const float sf = f * ps;
+    float pc = fmaf(u, kCosPi8, kCosPi6);
+    pc = fmaf(u, pc, kCosPi4);
+    pc = fmaf(u, pc, kCosPi2);
+    const float cf = fmaf(u, pc, 1.0f);
+    const int q = int(n) & 3;  // sin(pi f + q pi / 2), cos(pi f + q pi / 2)
+    const float a = (q & 1) ? cf : sf, b = (q & 1) ? sf : cf;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
 // Feature f = 16 t + 4 g + j of the lane's sample for input dim t: k = 2 g + j / 2,
 // sin for even j, cos for odd j (feature (dim * 8 + k) * 2 + s). The odd frequency comes from
-// the even one by the double-angle formulas (a few ulp in FP32, far below the bf16 rounding
-// of the features), which halves the sinpi / cospi evaluations.
+// the even one by the double-angle formulas (a few ulp in FP32), which halves the polynomial
+// evaluations.
 __device__ __forceinline__ void encode(const float x[6], int g, s4 enc[6]) {
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
-        const float v0 = ldexpf(x[t], 2 * g);
-        const float s0 = sinpif(v0), c0 = cospif(v0);
+        float s0, c0;
+        sincospi(ldexpf(x[t], 2 * g), s0, c0);
         const float s1 = 2.0f * s0 * c0, c1 = (c0 - s0) * (c0 + s0);
         enc[t] = pack4(s0, c0, s1, c1);
     }
@@ -145,6 +180,82 @@ __device__ __forceinline__ float forward16(const s8* __restrict__ W, const float
     return z + small[4 * kHid + kHid];
 }
 
+// Inference forward of 32 samples, two 16-sample tiles u (lane l: samples l & 15 of each
+// tile): every weight fragment read from LDS serves both tiles. The bias is the accumulator's
+// start value, hidden outputs go to bf16 through relu_pack4, and the output layer runs in FP32.
+// z[u] is the logit of the lane's sample of tile u (in all four lane groups).
+__device__ __forceinline__ void forward32(const s8* __restrict__ W, const float* __restrict__ small,
+                                          const s4 enc[2][6], int lane, float z[2]) {
+    const int g = lane >> 4;
+    s4 xin[2][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const f4 b = *reinterpret_cast<const f4*>(small + 16 * m + 4 * g);
+        f4 a0 = b, a1 = b;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const s8 w = W[(kFragW1 + m * 3 + p) * 64 + lane];
+            a0 = mfma(w, cat(enc[0][2 * p], enc[0][2 * p + 1]), a0);
+            a1 = mfma(w, cat(enc[1][2 * p], enc[1][2 * p + 1]), a1);
+        }
+        xin[0][m] = relu_pack4(a0);
+        xin[1][m] = relu_pack4(a1);
+    }
+#pragma unroll
+    for (int L = 1; L < 3; ++L) {
+        s4 nx[2][4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const f4 b = *reinterpret_cast<const f4*>(small + kHid * L + 16 * m + 4 * g);
+            f4 a0 = b, a1 = b;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const s8 w = W[(kFragW2 + (L - 1) * 8 + m * 2 + p) * 64 + lane];
+                a0 = mfma(w, cat(xin[0][2 * p], xin[0][2 * p + 1]), a0);
+                a1 = mfma(w, cat(xin[1][2 * p], xin[1][2 * p + 1]), a1);
+            }
+            nx[0][m] = relu_pack4(a0);
+            nx[1][m] = relu_pack4(a1);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) xin[u][m] = nx[u][m];
+    }
+    float zz[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const f4 b = *reinterpret_cast<const f4*>(small + 3 * kHid + 16 * m + 4 * g);
+        const f4 w5 = *reinterpret_cast<const f4*>(small + 4 * kHid + 16 * m + 4 * g);
+        f4 a0 = b, a1 = b;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const s8 w = W[(kFragW4 + m * 2 + p) * 64 + lane];
+            a0 = mfma(w, cat(xin[0][2 * p], xin[0][2 * p + 1]), a0);
+            a1 = mfma(w, cat(xin[1][2 * p], xin[1][2 * p + 1]), a1);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            zz[0] = fmaf(w5[j], fmaxf(a0[j], 0.0f), zz[0]);
+            zz[1] = fmaf(w5[j], fmaxf(a1[j], 0.0f), zz[1]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        zz[u] += __shfl_xor(zz[u], 16);
+        zz[u] += __shfl_xor(zz[u], 32);
+        z[u] = zz[u] + small[4 * kHid + kHid];
+    }
+}
+
+// Loads sample s of the inference input (zeros past n) and encodes it for lane group g.
+__device__ __forceinline__ void encode_sample(const float4* __restrict__ samples, uint32_t s, uint32_t n, float4 c,
+                                              float4 ih, int g, s4 enc[6]) {
+    float x[6] = {0, 0, 0, 0, 0, 0};
+    if (s < n) load_sample(samples + size_t(s) * 2, c, ih, x);
+    encode(x, g, enc);
+}
+
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
 
 // Stages the forward fragments and the FP32 small parameters into LDS (all threads).
@@ -168,73 +279,83 @@ __device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ packe
 __global__ __launch_bounds__(256) void k_infer(MlpDev m, const float4* __restrict__ samples, uint32_t n,
                                                float* __restrict__ out) {
     __shared__ s8 lw[kFragsFwd * 64];
-    __shared__ float lsmall[kSmall];
+    __shared__ __align__(16) float lsmall[kSmall];
     stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
     const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
     const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
-    for (uint32_t base = wave * 16; base < n; base += nwaves * 16) {
-        const uint32_t s = base + uint32_t(lane & 15);
-        float x[6] = {0, 0, 0, 0, 0, 0};
-        if (s < n) load_sample(samples + size_t(s) * 2, c, ih, x);
-        s4 enc[6];
-        encode(x, lane >> 4, enc);
-        f4 h[4][4];
-        const float z = forward16(lw, lsmall, enc, lane, h);
-        if (s < n && lane < 16) out[s] = sigmoid(z);
+    for (uint32_t base = wave * 32; base < n; base += nwaves * 32) {
+        asm volatile("" ::: "memory");  // weight fragments are re-read from LDS in every pass
+        s4 enc[2][6];
+        encode_sample(samples, base + uint32_t(lane & 15), n, c, ih, g, enc[0]);
+        encode_sample(samples, base + 16u + uint32_t(lane & 15), n, c, ih, g, enc[1]);
+        float z[2];
+        forward32(lw, lsmall, enc, lane, z);
+        // lane groups 0 and 1 write tiles 0 and 1
+        const uint32_t s = base + uint32_t(16 * g + (lane & 15));
+        if (g < 2 && s < n) out[s] = sigmoid(g == 0 ? z[0] : z[1]);
     }
 }
 
 // Render form: for each queued shadow pixel, the MLP's visibility p at (hit point, light
 // direction). Confident predictions (p (1 - p) <= threshold) are written as the pixel's
 // visibility; the rest are appended to the exact-fallback queue.
+__device__ __forceinline__ void gate_input(const MlpShadowParams& sp, uint32_t q, uint32_t n, float4 c, float4 ih,
+                                           float x[6]) {
+    x[0] = x[1] = x[2] = x[3] = x[4] = x[5] = 0.0f;
+    if (q >= n) return;
+    const float4 o = sp.origins[q];
+    x[0] = (o.x - c.x) * ih.x;
+    x[1] = (o.y - c.y) * ih.y;
+    x[2] = (o.z - c.z) * ih.z;
+    if (sp.dirs) {
+        const float4 d = sp.dirs[q];
+        x[3] = d.x;
+        x[4] = d.y;
+        x[5] = d.z;
+    } else {
+        x[3] = sp.dir[0];
+        x[4] = sp.dir[1];
+        x[5] = sp.dir[2];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
     __shared__ s8 lw[kFragsFwd * 64];
-    __shared__ float lsmall[kSmall];
+    __shared__ __align__(16) float lsmall[kSmall];
     __shared__ uint32_t s_fb[4][128];  // per-wave fallback pixels, flushed 64+ at a time
     stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
     uint32_t* fb = s_fb[threadIdx.x >> 6];
     uint32_t fbn = 0;  // wave-uniform
     const uint32_t n = *sp.count;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
     const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
     const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
-    for (uint32_t base = wave * 16; base < n; base += nwaves * 16) {
+    for (uint32_t base = wave * 32; base < n; base += nwaves * 32) {
         // a compiler-only fence: the weight fragments are read from LDS in every pass rather
         // than hoisted out of the loop into ~150 registers, which would leave one wave per SIMD
         asm volatile("" ::: "memory");
-        const uint32_t q = base + uint32_t(lane & 15);
-        float x[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t pix = 0;
-        if (q < n) {
-            pix = sp.queue[q];
-            const float4 o = sp.origins[q];
-            x[0] = (o.x - c.x) * ih.x;
-            x[1] = (o.y - c.y) * ih.y;
-            x[2] = (o.z - c.z) * ih.z;
-            if (sp.dirs) {
-                const float4 d = sp.dirs[q];
-                x[3] = d.x;
-                x[4] = d.y;
-                x[5] = d.z;
-            } else {
-                x[3] = sp.dir[0];
-                x[4] = sp.dir[1];
-                x[5] = sp.dir[2];
-            }
+        s4 enc[2][6];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            float x[6];
+            gate_input(sp, base + uint32_t(16 * u + (lane & 15)), n, c, ih, x);
+            encode(x, g, enc[u]);
         }
-        s4 enc[6];
-        encode(x, lane >> 4, enc);
-        f4 h[4][4];
-        const float p = sigmoid(forward16(lw, lsmall, enc, lane, h));
-        const bool own = q < n && lane < 16;
+        float z[2];
+        forward32(lw, lsmall, enc, lane, z);
+        // lane groups 0 and 1 own the samples of tiles 0 and 1
+        const uint32_t q = base + uint32_t(16 * g + (lane & 15));
+        const bool own = g < 2 && q < n;
+        const float p = sigmoid(g == 0 ? z[0] : z[1]);
+        const uint32_t pix = own ? sp.queue[q] : 0u;
         const bool fallback = own && p * (1.0f - p) > sp.threshold;
         if (own && !fallback) sp.vis[pix] = p;
         const uint64_t fm = __ballot(fallback);
         if (fallback) fb[fbn + __popcll(fm & ((1ull << lane) - 1ull))] = pix;
-        fbn += uint32_t(__popcll(fm));  // <= 16 per pass: at most 63 + 16 < 128 before a flush
+        fbn += uint32_t(__popcll(fm));  // <= 32 per pass: at most 63 + 32 < 128 before a flush
         if (fbn >= 64u) {
             uint32_t b = 0;
             if (lane == 0) b = atomicAdd(sp.fallback_count, fbn);
@@ -476,7 +597,7 @@ hipError_t launch_mlp_pack(const MlpDev& m, hipStream_t st) {
 
 hipError_t launch_mlp_infer(const MlpDev& m, const float* samples, uint32_t n, float* out, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const uint32_t groups = (n + 15u) / 16u;
+    const uint32_t groups = (n + 31u) / 32u;
     const uint32_t blocks = std::min<uint32_t>((groups + 3u) / 4u, 2048u);
     hipLaunchKernelGGL(mlp_detail::k_infer, dim3(blocks), dim3(256), 0, st, m,
                        reinterpret_cast<const float4*>(samples), n, out);
@@ -485,17 +606,17 @@ hipError_t launch_mlp_infer(const MlpDev& m, const float* samples, uint32_t n, f
 
 hipError_t launch_mlp_shadow_gate(const MlpDev& m, const MlpShadowParams& sp, uint32_t max_count, hipStream_t st) {
     if (max_count == 0) return hipSuccess;
-    // as many resident blocks as the registers and LDS weights allow (2 per CU at 255 VGPRs):
-    // the second wave per SIMD hides the per-sample loads and the encoding (the neural 16M-segment
-    // frame: 1.44 ms against 1.47 ms with one block per CU); each wavefront still walks many
-    // 16-sample groups, so its fallback buffer reaches the queue with few atomics
+    // as many resident blocks as the registers and LDS weights allow (up to 4 per CU): the
+    // waves of a SIMD overlap one another's encoding (VALU) with their matrix products (MFMA);
+    // each wavefront still walks many 32-sample groups, so its fallback buffer reaches the
+    // queue with few atomics
     int dev = 0, cus = 0, per_cu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(mlp_detail::k_shadow_gate), 256,
                                                  0);
     per_cu = std::max(1, std::min(per_cu, 4));
-    const uint32_t groups = (max_count + 15u) / 16u;
+    const uint32_t groups = (max_count + 31u) / 32u;
     const uint32_t blocks =
         std::max(1u, std::min<uint32_t>((groups + 3u) / 4u, uint32_t(std::max(cus, 1) * per_cu)));
     hipLaunchKernelGGL(mlp_detail::k_shadow_gate, dim3(blocks), dim3(256), 0, st, m, sp);

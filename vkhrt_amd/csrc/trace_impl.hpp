@@ -1743,7 +1743,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
                                                                      uint32_t skip_upto) {
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t total = *count;
-    if (total <= skip_upto) return;  // a short queue is walked by k_render_shadow_wave
+    if (total <= skip_upto) return;  // a short queue is walked by k_render_shadow_groups
     const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
     persistent_lanes<G, true, STACK, IM>(
         rp.tp, total, work, stk + threadIdx.x,
@@ -1782,165 +1782,240 @@ __global__ __launch_bounds__(kTraceBlock) void k_accurate_queued(TraceParams tp,
     accurate_one(tp, src, queue[qi], hits);
 }
 
-// ------------------------------------------------------------------ one wave per shadow ray
-// Any-hit traversal of a short shadow queue (the neural form's exact fallback): the whole
-// wavefront walks one ray. Persistent lanes take one ray each and hide memory latency behind
-// the other rays of a long queue; with a few thousand rays the frame waits for the slowest
-// ray, whose walk is a chain of dependent loads. Here each iteration of the wave pops up to 64
-// nodes of its ray from a stack in LDS and takes up to 64 pending leaf records from a queue in
-// LDS, one per lane: the loads of both are issued together, so an iteration costs one memory
-// round trip. Node tests push the internal children they hit and append their leaf records;
-// records run the early-out and Prhi. The walk ends at the first hit in [tmin, tmax]
-// (occluded) or when both are empty (visible). Grazing candidates are deferred to the FP64
-// polish as in the other kernels, so the visibility equals the persistent kernel's for every
-// ray. Nodes are popped only while the record queue has room for all they can append.
-constexpr uint32_t kWaveStack = 2048;      // pending nodes per wave (8 KB); a walk past it reports an overflow
-constexpr uint32_t kWaveLeaves = 2048;     // pending leaf records per wave (8 KB)
-constexpr uint32_t kWaveRayLimit = 16384;  // queues up to this long take the wave-per-ray kernel
-constexpr uint32_t kWaveGrid = 8192;       // one-wave blocks of the wave-per-ray launch
+// ------------------------------------------------------------------ lane groups per shadow ray
+// Any-hit traversal of a short shadow queue (the neural form's exact fallback). Persistent lanes
+// take one ray each and hide memory latency behind the other rays of a long queue; with a few
+// thousand rays the frame waits for the slowest ray, whose walk is a chain of dependent loads.
+// Here a group of LPR lanes walks one ray, 64 / LPR rays per wavefront. Each iteration a group
+// pops up to LPR nodes of its ray from its stack in LDS and takes up to LPR pending leaf records
+// from its record queue in LDS, one per lane, and issues the loads of both together: an
+// iteration costs one memory round trip. Node tests push the internal children they hit and
+// append their leaf records; records run the early-out and Prhi, and a grazing root runs the
+// FP64 continuation at once (kPolishInline: the few thousand walks have the latency to spare,
+// and the FP64 polish launch after them finds nothing to do). A walk ends at the first hit in
+// [tmin, tmax] (occluded) or when stack and queue are empty in every instance the ray enters
+// (visible); the group then takes its next ray. The visibility equals the persistent kernel's
+// for every ray. A hair walk's frontier holds about 5 nodes per step, so a group of 8 lanes
+// idles few of them; a whole wavefront per ray spent most of its issue slots on idle lanes.
+//
+// Nodes are popped only while the record queue has room for all they can append (24 records
+// per node: 8 slots of up to 3) and the stack has room for their children (a popped node adds
+// at most 7 entries). Past kGrpStack - kGrpReserve a group pops one node at a time, a
+// depth-first walk whose stack grows by at most 7 entries per level, so the reserve covers
+// trees 32 levels deep; past kGrpStack the walk reports an overflow.
+constexpr uint32_t kGrpReserve = 7u * 32u + 8u;
+constexpr uint32_t kGrpNodeRecords = 24;   // leaf records one node can append
+constexpr uint32_t kGrpGrid = 8192;        // one-wave blocks of the launch
+template <int LPR>
+struct GroupShape {
+    static constexpr uint32_t kGroups = 64u / LPR;                     // rays per wavefront
+    static constexpr uint32_t kStack = LPR >= 32 ? 1024u : (LPR == 16 ? 512u : 384u);
+    static constexpr uint32_t kLeaves = LPR >= 32 ? 512u : (LPR == 16 ? 512u : 256u);
+};
 
-template <class G, int IM>
-__global__ __launch_bounds__(64) void k_render_shadow_wave(RenderParams rp, const uint32_t* queue, const uint32_t* count,
-                                                           uint32_t limit) {
-    static_assert(G::kMode == kPhantom && !G::kMixed, "wave-per-ray shadows serve phantom TLASes");
-    __shared__ uint32_t s_stack[kWaveStack];
-    __shared__ uint32_t s_leaf[kWaveLeaves];
+template <class G, int IM, int LPR>
+__global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, const uint32_t* queue, const uint32_t* count,
+                                                             uint32_t limit) {
+    static_assert(G::kMode == kPhantom && !G::kMixed, "lane-group shadows serve phantom TLASes");
+    static_assert(LPR == 8 || LPR == 16 || LPR == 32 || LPR == 64, "lanes per ray: 8, 16, 32 or 64");
+    using S = GroupShape<LPR>;
+    __shared__ uint32_t s_stack[S::kGroups][S::kStack];
+    __shared__ uint32_t s_leaf[S::kGroups][S::kLeaves];
     const uint32_t total = *count;
     if (total == 0 || total > limit) return;
     const TraceParams& tp = rp.tp;
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x, grp = lane / LPR, sub = lane % LPR, last = grp * LPR + LPR - 1u;
+    const uint64_t gmask = (LPR == 64 ? ~0ull : ((1ull << LPR) - 1ull)) << (grp * LPR);
+    uint32_t* stk = s_stack[grp];
+    uint32_t* lf = s_leaf[grp];
     const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
+    const uint32_t qstride = gridDim.x * S::kGroups;
 #ifdef VKHRT_WORK_COUNTERS
     unsigned long long wc[7] = {};
+    unsigned long long t_ray = 0, t_max = 0;
 #endif
-    for (uint32_t qi = blockIdx.x; qi < total; qi += gridDim.x) {
+    // group state: the same value in every lane of the group
+    uint32_t qi = blockIdx.x * S::kGroups + grp;
+    bool have_ray = false, walking = false, occluded = false, error = false;
+    uint32_t inst = 0, sp = 0, lh = 0, lt = 0;
+    f3 ow{}, dw{}, o{}, d{}, inv{};
+    float tmin = 0.0f, tmax = 0.0f;
+    uint32_t oct = 0;
+    RayFrame F;
+    BlasDev B{};
+    while (true) {
+        // per group: take a ray, then enter its next instance (or finish the ray)
+        if (!have_ray && qi < total) {
+            src.get(qi, ow, dw, tmin, tmax);
+            have_ray = true;
+            inst = 0;
+            occluded = error = false;
 #ifdef VKHRT_WORK_COUNTERS
-        const unsigned long long t_ray = clock64();
-        ++wc[0];
+            t_ray = clock64();
+            if (sub == 0) ++wc[0];
 #endif
-        f3 ow, dw;
-        float tmin, tmax;
-        src.get(qi, ow, dw, tmin, tmax);
-        bool occluded = false;
-        for (uint32_t inst = 0; inst < tp.ninst && !occluded; ++inst) {
-            const InstanceDev& I = tp.inst[inst];
-            if (I.blas.nprims == 0 || !ray_box(ow, dw, I.lo, I.hi, tmin, tmax)) continue;
-            const BlasDev B = IM == kInstOne ? wave_uniform(I.blas) : I.blas;
-            f3 o = ow, d = dw;
-            if (!I.identity) {
-                o = xform_pt(I.inv, ow);
-                d = xform_dir(I.inv, dw);
+        }
+        while (have_ray && !walking) {
+            if (inst < tp.ninst && !occluded && !error) {
+                const InstanceDev& I = tp.inst[inst];
+                if (I.blas.nprims == 0 || !ray_box(ow, dw, I.lo, I.hi, tmin, tmax)) {
+                    ++inst;
+                    continue;
+                }
+                B = IM == kInstOne ? wave_uniform(I.blas) : I.blas;
+                o = ow;
+                d = dw;
+                if (!I.identity) {
+                    o = xform_pt(I.inv, ow);
+                    d = xform_dir(I.inv, dw);
+                }
+                inv = safe_inv(d);
+                oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
+                make_frame(o, d, F);
+                if (sub == 0) stk[0] = 0u;  // the root
+                sp = 1;
+                lh = lt = 0;
+                walking = true;
+            } else {
+                if (error && sub == 0) atomicAdd(&tp.status[0], 1u);
+                if (sub == 0) rp.vis[queue[qi]] = occluded ? 0.0f : 1.0f;
+#ifdef VKHRT_WORK_COUNTERS
+                if (sub == 0) {
+                    const unsigned long long c = clock64() - t_ray;
+                    wc[6] += c;
+                    t_max = c > t_max ? c : t_max;
+                }
+#endif
+                have_ray = false;
+                qi += qstride;
+                if (qi < total) {
+                    src.get(qi, ow, dw, tmin, tmax);
+                    have_ray = true;
+                    inst = 0;
+                    occluded = error = false;
+#ifdef VKHRT_WORK_COUNTERS
+                    t_ray = clock64();
+                    if (sub == 0) ++wc[0];
+#endif
+                }
             }
-            const f3 inv = safe_inv(d);
-            const uint32_t oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
-            RayFrame F;
-            make_frame(o, d, F);
-            uint32_t sp = 1, lh = 0, lt = 0;  // wave-uniform: stack size, record queue head and tail
-            if (lane == 0) s_stack[0] = 0u;
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            bool error = false;
-            while ((sp > 0 || lh < lt) && !occluded && !error) {
-                // this iteration's nodes (only while the record queue can take 64 nodes' records)
-                const uint32_t m = lt + 64u * 24u <= kWaveLeaves ? (sp < 64u ? sp : 64u) : 0u;
-                const uint32_t node = lane < m ? s_stack[sp - m + lane] : 0xFFFFFFFFu;
-                sp -= m;
-                // this iteration's records
-                const uint32_t nl = lt - lh < 64u ? lt - lh : 64u;
-                const uint32_t slot = lane < nl ? s_leaf[lh + lane] : 0xFFFFFFFFu;
-                lh += nl;
-                if (lh == lt) lh = lt = 0;
-                // issue the record's loads before the node test so both are in flight together
-                float4 cv[4] = {};
-                float rmax = 0.0f;
-                if (slot != 0xFFFFFFFFu) {
-                    const auto* pr = gmem(B.prims + size_t(slot) * 4);
-                    cv[0] = pr[0];
-                    cv[1] = pr[1];
-                    cv[2] = pr[2];
-                    cv[3] = pr[3];
-                    rmax = gmem(B.aux)[slot];
+        }
+        if (__ballot(walking) == 0) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // one iteration of every walking group: its nodes (as many as its record queue and stack
+        // can take) and its records
+        uint32_t m = 0, nl = 0;
+        if (walking) {
+            const uint32_t by_stack = sp + kGrpReserve <= S::kStack ? (S::kStack - kGrpReserve - sp) / 7u : 0u;
+            const uint32_t by_leaves = (S::kLeaves - lt) / kGrpNodeRecords;
+            m = min(min(sp, uint32_t(LPR)), min(by_stack > 1u ? by_stack : 1u, by_leaves));
+            nl = min(lt - lh, uint32_t(LPR));
+        }
+        const uint32_t node = sub < m ? stk[sp - m + sub] : 0xFFFFFFFFu;
+        const uint32_t slot = sub < nl ? lf[lh + sub] : 0xFFFFFFFFu;
+        if (walking) {
+            sp -= m;
+            lh += nl;
+            if (lh == lt) lh = lt = 0;
+        }
+        // issue the record's loads before the node test so both are in flight together
+        float4 cv[4] = {};
+        if (slot != 0xFFFFFFFFu) {
+            const auto* pr = gmem(B.prims + size_t(slot) * 4);
+            cv[0] = pr[0];
+            cv[1] = pr[1];
+            cv[2] = pr[2];
+            cv[3] = pr[3];
+        }
+        const float rmax = slot != 0xFFFFFFFFu ? gmem(B.aux)[slot] : 0.0f;
+        uint32_t hits = 0, pmask = 0, child_base = 0, imask = 0, pbase = 0;
+        bool bad = false;
+        if (node != 0xFFFFFFFFu) {
+            if (node < B.nnodes) {
+                node_visit(B.nodes, node, o, inv, oct, tmin, tmax, hits, pmask, child_base, imask, pbase);
+                if (pmask && pbase + (32u - __clz(pmask)) > B.nprims) {
+                    bad = true;
+                    hits = pmask = 0;
                 }
-                uint32_t hits = 0, pmask = 0, child_base = 0, imask = 0, pbase = 0;
-                if (node != 0xFFFFFFFFu) {
-                    if (node < B.nnodes) {
-                        node_visit(B.nodes, node, o, inv, oct, tmin, tmax, hits, pmask, child_base, imask, pbase);
-                        if (pmask && pbase + (32u - __clz(pmask)) > B.nprims) {
-                            error = true;
-                            hits = pmask = 0;
-                        }
-                    } else {
-                        error = true;
-                    }
-                }
-                // the record: early out, then Prhi
-                bool hit = false, pass = false;
-                if (slot != 0xFFFFFFFFu) {
-                    pass = ray_cylinder(o, d, f3{cv[0].x, cv[0].y, cv[0].z}, f3{cv[3].x, cv[3].y, cv[3].z}, rmax);
-                    if (pass) {
-                        PhantomHit ph;
-                        hit = prhi<G::kRMode, G::kPolish ? kPolishDefer : kPolishOff, false>(F, cv, 0.0f, B.radius, ph);
-                        if (G::kPolish && ph.deferred) {
-                            const uint32_t q = atomicAdd(tp.defer_count, 1u);
-                            if (q < tp.defer_cap)
-                                tp.defer[q] = make_uint4(qi, inst, slot, 0u);
-                            else
-                                atomicAdd(&tp.status[1], 1u);
-                        }
-                        hit = hit && ph.t >= tmin && ph.t <= tmax;
-                    }
-                }
-                occluded = __ballot(hit) != 0;
-                error = __ballot(error) != 0;
+            } else {
+                bad = true;
+            }
+        }
+        // the record: early out, then Prhi with the FP64 continuation of grazing roots
+        bool hit = false, pass = false;
+        if (slot != 0xFFFFFFFFu) {
+            pass = ray_cylinder(o, d, f3{cv[0].x, cv[0].y, cv[0].z}, f3{cv[3].x, cv[3].y, cv[3].z}, rmax);
+            if (pass) {
+                PhantomHit ph;
+                hit = prhi<G::kRMode, G::kPolish ? kPolishInline : kPolishOff, false>(F, cv, 0.0f, B.radius, ph);
+                hit = hit && ph.t >= tmin && ph.t <= tmax;
+            }
+        }
+        const uint64_t hb = __ballot(hit), eb = __ballot(bad);
 #ifdef VKHRT_WORK_COUNTERS
-                ++wc[1];
-                wc[2] += m;
-                wc[3] += nl;
-                wc[4] += __popcll(__ballot(pass));
-                wc[5] += nl ? 1u : 0u;
+        const uint64_t pb = __ballot(pass);
+        if (walking && sub == 0) {
+            ++wc[1];
+            wc[2] += m;
+            wc[3] += nl;
+            wc[4] += __popcll(pb & gmask);
+            wc[5] += nl ? 1u : 0u;
+        }
 #else
-                (void)pass;
+        (void)pass;
 #endif
-                // push the internal children hit (octant bit j -> slot j ^ oct -> child index) and
-                // append the leaf records hit (inclusive prefix sums over the lanes)
-                const uint32_t nc = __popc(hits), np = __popc(pmask);
-                uint32_t cpre = nc, ppre = np;
+        // push the internal children hit (octant bit j -> slot j ^ oct -> child index) and
+        // append the leaf records hit (inclusive prefix sums over the group's lanes)
+        const uint32_t nc = __popc(hits), np = __popc(pmask);
+        uint32_t cpre = nc, ppre = np;
 #pragma unroll
-                for (int k = 1; k < 64; k <<= 1) {
-                    const uint32_t x = __shfl_up(cpre, k), y = __shfl_up(ppre, k);
-                    if (int(lane) >= k) {
-                        cpre += x;
-                        ppre += y;
-                    }
-                }
-                const uint32_t ctot = __shfl(cpre, 63), ptot = __shfl(ppre, 63);
-                if (sp + ctot > kWaveStack) {
-                    error = true;
-                    break;
-                }
+        for (int k = 1; k < LPR; k <<= 1) {
+            const uint32_t x = __shfl_up(cpre, k), y = __shfl_up(ppre, k);
+            if (int(sub) >= k) {
+                cpre += x;
+                ppre += y;
+            }
+        }
+        const uint32_t ctot = __shfl(cpre, last), ptot = __shfl(ppre, last);
+        if (walking) {
+            occluded = (hb & gmask) != 0;
+            error = (eb & gmask) != 0 || sp + ctot > S::kStack || lt + ptot > S::kLeaves;
+            if (!error) {
                 uint32_t w = sp + cpre - nc;
-                for (uint32_t hb = hits; hb; hb &= hb - 1u) {
-                    const uint32_t j = __ffs(hb) - 1, sl = j ^ oct;
-                    s_stack[w++] = child_base + __popc(imask & ((1u << sl) - 1u));
+                for (uint32_t h = hits; h; h &= h - 1u) {
+                    const uint32_t j = __ffs(h) - 1, sl = j ^ oct;
+                    stk[w++] = child_base + __popc(imask & ((1u << sl) - 1u));
                 }
                 w = lt + ppre - np;
-                for (uint32_t pm = pmask; pm; pm &= pm - 1u) s_leaf[w++] = pbase + uint32_t(__ffs(pm) - 1);
+                for (uint32_t pm = pmask; pm; pm &= pm - 1u) lf[w++] = pbase + uint32_t(__ffs(pm) - 1);
                 sp += ctot;
                 lt += ptot;
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                __builtin_amdgcn_wave_barrier();
             }
-            if (error && lane == 0) atomicAdd(&tp.status[0], 1u);
+            // the instance is done when the ray is occluded, an error stopped it, or nothing is left
+            if (occluded || error || (sp == 0 && lh == lt)) {
+                walking = false;
+                ++inst;
+            }
         }
-        if (lane == 0) rp.vis[queue[qi]] = occluded ? 0.0f : 1.0f;
-#ifdef VKHRT_WORK_COUNTERS
-        wc[6] += clock64() - t_ray;
-#endif
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 #ifdef VKHRT_WORK_COUNTERS
-    if (lane == 0 && tp.work_counts)
+    for (int i = 0; i < 7; ++i) {
+        unsigned long long v = wc[i];
+        for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+        wc[i] = v;
+    }
+    for (int k = 32; k >= 1; k >>= 1) {
+        const unsigned long long v = __shfl_xor(t_max, k);
+        t_max = v > t_max ? v : t_max;
+    }
+    if (lane == 0 && tp.work_counts) {
         for (int i = 0; i < 7; ++i)
             if (wc[i]) atomicAdd(&tp.work_counts[kWorkWave + i], wc[i]);
+        atomicMax(&tp.work_counts[kWorkWave + 7], t_max);
+    }
 #endif
 }
 
@@ -2102,6 +2177,27 @@ hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t 
     return hipGetLastError();
 }
 
+// The neural fallback's lane-group walker: rp.group_lanes (8, 16, 32 or 64) lanes per ray.
+template <class G>
+hipError_t launch_shadow_groups(const RenderParams& rp, const uint32_t* squeue, const uint32_t* scount, uint32_t limit,
+                                int im, hipStream_t st) {
+    using namespace trace_detail;
+    const dim3 grid(kGrpGrid), block(64);
+#define VK_GROUPS(L)                                                                                               \
+    if (im == kInstOne)                                                                                            \
+        hipLaunchKernelGGL((k_render_shadow_groups<G, kInstOne, L>), grid, block, 0, st, rp, squeue, scount, limit); \
+    else                                                                                                           \
+        hipLaunchKernelGGL((k_render_shadow_groups<G, kInstMany, L>), grid, block, 0, st, rp, squeue, scount, limit);
+    switch (rp.group_lanes) {
+        case 16: VK_GROUPS(16) break;
+        case 32: VK_GROUPS(32) break;
+        case 64: VK_GROUPS(64) break;
+        default: VK_GROUPS(8) break;
+    }
+#undef VK_GROUPS
+    return hipGetLastError();
+}
+
 template <class G, int STACK>
 hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEvent_t* ev) {
     using namespace trace_detail;
@@ -2175,19 +2271,15 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                         squeue = rp.fallback_queue;
                         scount = rp.counters + 5;
                     }
-                    // the neural form's fallback queue is short: a queue of up to kWaveRayLimit
+                    // the neural form's fallback queue is short: a queue of up to rp.wave_limit
                     // rays is walked one ray per wavefront, a longer one by the persistent lanes
                     // (both kernels read the length on the device; one of them returns at once)
                     uint32_t skip = 0;
                     if constexpr (!G::kMixed && G::kMode == kPhantom) {
-                        if (rp.mlp) {
-                            skip = kWaveRayLimit;
-                            if (im == kInstOne)
-                                hipLaunchKernelGGL((k_render_shadow_wave<G, kInstOne>), dim3(kWaveGrid), dim3(64), 0, st, rp,
-                                                   squeue, scount, skip);
-                            else
-                                hipLaunchKernelGGL((k_render_shadow_wave<G, kInstMany>), dim3(kWaveGrid), dim3(64), 0, st,
-                                                   rp, squeue, scount, skip);
+                        if (rp.mlp && rp.wave_limit) {
+                            skip = rp.wave_limit;
+                            const hipError_t e = launch_shadow_groups<G>(rp, squeue, scount, skip, im, st);
+                            if (e != hipSuccess) return e;
                         }
                     }
                     if constexpr (G::kMixed)

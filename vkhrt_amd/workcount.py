@@ -5,7 +5,7 @@ of the library (libvkhrt_count.so: the same kernels compiled with VKHRT_WORK_COU
 
 Renders the frame (100,000 seeded strands x 32 tapered segments, 1920x1080, a shadow ray per
 hit, FP64 polish) and prints one JSON object: per frame and per traversal kind (closest hit,
-any hit, and with --area 1 the wave-per-ray kernel of the 16M-segment area-light frame's neural
+any hit, and with --area 1 the lane-group walker of the 16M-segment area-light frame's neural
 fallback) the loop iterations, intersection rounds, traversal steps, node and leaf tests, Prhi
 calls, lane use and the shares of shader-clock cycles spent in rounds, steps and bookkeeping.
 The counters add clock reads and atomics to the kernels, so their frame times are not the
@@ -52,6 +52,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--accurate", type=int, default=1)
     ap.add_argument("--area", type=int, default=0, help="also the 16M-segment area-light frame's neural fallback")
+    ap.add_argument("--group-lanes", type=int, default=8, help="lanes per ray of the fallback's lane-group walker")
     args = ap.parse_args()
 
     import ctypes
@@ -62,6 +63,7 @@ def main():
     assert torch.cuda.is_available(), "needs a GPU"
     sp = torch.cuda.current_stream().cuda_stream
     ctx = V.Context(0)
+    ctx.set_option(V.OPTION_SHADOW_GROUP_LANES, args.group_lanes)
     raw = (ctypes.c_uint64 * 48)()
 
     def read(reset=True):
@@ -125,9 +127,10 @@ def main():
         w = {k: r[32 + i] / args.frames for i, k in enumerate(WAVE_NAMES)}
         rays_w = max(w["rays"], 1.0)
         out["area_neural_fallback"] = {
-            "threshold": thr, "queries": q, "traced": tr,
+            "threshold": thr, "queries": q, "traced": tr, "lanes_per_ray": args.group_lanes,
             "per_ray": {k: round(w[k] / rays_w, 2) for k in ("steps", "node_tests", "leaf_tests", "prhi_calls",
                                                             "record_steps", "cycles")},
+            "slowest_ray_cycles": int(r[32 + len(WAVE_NAMES)]),
             "primary": summarize(r, 0, args.frames)}
         mlp.close()
     print(json.dumps(out), flush=True)
