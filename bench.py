@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles")
     ap.add_argument("--xcd-chunk", type=int, default=4096,
                     help="rays handed out per XCD in runs of this many (0: one work counter)")
-    ap.add_argument("--shadow-group-lanes", type=int, default=8,
+    ap.add_argument("--shadow-group-lanes", type=int, default=16,
                     help="lanes per ray of the neural fallback's lane-group walker (8, 16, 32 or 64)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -217,6 +217,12 @@ def main():
     if not args.no_collapse_study:
         collapse = collapse_block(args, ctx, cam, W, H, flags, sp, stream, torch, np, V, curves, nseg_total)
         ctx.set_option(V.OPTION_BVH_COLLAPSE, tuning["bvh_collapse"] if tuning else 0)
+
+    # the autotuned BVH settings belong to the 100k-strand scene; the other workloads build
+    # their BVHs with the library defaults (8 children, leaf size 1, greedy collapse)
+    ctx.set_option(V.OPTION_BVH_WIDTH, 8)
+    ctx.set_option(V.OPTION_LEAF_SIZE, 1)
+    ctx.set_option(V.OPTION_BVH_COLLAPSE, 0)
 
     modes = None
     if not args.no_modes:

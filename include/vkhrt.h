@@ -29,7 +29,8 @@
  * aborts, no exceptions across the ABI); vkhrt_last_error() describes the last failure of
  * a context. Buffers are device pointers unless a flag says otherwise. `stream` is a
  * hipStream_t (NULL = the HIP null stream). A context is not internally locked: use one
- * per host thread.
+ * per host thread. Calls on one context share its scratch memory (counters, queues, hit and
+ * polish buffers): issue them on one stream, or give each stream its own context.
  */
 #ifndef VKHRT_H
 #define VKHRT_H
@@ -248,7 +249,7 @@ int vkhrt_abi_version(void);
                                             rays (0..2^20, default 16384) is walked by lane groups, one
                                             ray per group, a longer one by the persistent lanes; 0 =
                                             always the persistent lanes. Results do not depend on it */
-#define VKHRT_OPTION_SHADOW_GROUP_LANES 11 /* lanes per ray of that walk: 8 (default), 16, 32 or 64 */
+#define VKHRT_OPTION_SHADOW_GROUP_LANES 11 /* lanes per ray of that walk: 8, 16 (default), 32 or 64 */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
 
 /* ---- acceleration structures */

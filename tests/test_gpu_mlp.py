@@ -208,9 +208,9 @@ def test_neural_shadow_render(gpu_ctx):
 
 @pytest.mark.parametrize("area", [False, True], ids=["directional", "area2"])
 def test_wave_fallback_equals_persistent_lanes(gpu_ctx, area):
-    """The neural form's exact fallback gives the same image whether its queue is walked one
-    ray per wavefront (the short-queue kernel, forced for every queue length here) or by the
-    persistent lanes (wave limit 0): bit for bit at every threshold, FP64 polish on. At
+    """The neural form's exact fallback gives the same image whether its queue is walked by lane
+    groups (8, 16 or 64 lanes per ray, forced for every queue length here) or by the persistent
+    lanes (wave limit 0): bit for bit at every threshold, FP64 polish on. At
     threshold 0 every query takes the fallback, and the image equals the exact frame. An
     untrained network is deterministic and uncertain, so the thresholds split the queries."""
     import gpu_util as G
@@ -236,20 +236,24 @@ def test_wave_fallback_equals_persistent_lanes(gpu_ctx, area):
     partial = False
     try:
         for thr in (0.0, 0.2, 0.24, 0.245, 0.249, 0.2499):
-            ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 1 << 20)
-            a, q, tr = frame(mlp, thr)
             ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 0)
-            b, q2, tr2 = frame(mlp, thr)
-            print(f"[wave fallback] threshold {thr}: {tr}/{q} queries traced", flush=True)
-            assert (q, tr) == (q2, tr2)
-            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), thr
+            b, q, tr = frame(mlp, thr)
+            ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 1 << 20)
+            for lanes in (8, 16, 64):
+                ctx.set_option(V.OPTION_SHADOW_GROUP_LANES, lanes)
+                a, q2, tr2 = frame(mlp, thr)
+                print(f"[wave fallback] threshold {thr}, {lanes} lanes per ray: {tr2}/{q2} queries traced", flush=True)
+                assert (q, tr) == (q2, tr2)
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (thr, lanes)
             if thr == 0.0:
                 assert tr == q and np.array_equal(a.view(np.uint32), exact.view(np.uint32))
             partial = partial or 0 < tr < q
     finally:
         ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 16384)
+        ctx.set_option(V.OPTION_SHADOW_GROUP_LANES, 16)
     assert partial
-    for opt, v in ((V.OPTION_SHADOW_WAVE_LIMIT, -1), (V.OPTION_SHADOW_WAVE_LIMIT, (1 << 20) + 1)):
+    for opt, v in ((V.OPTION_SHADOW_WAVE_LIMIT, -1), (V.OPTION_SHADOW_WAVE_LIMIT, (1 << 20) + 1),
+                   (V.OPTION_SHADOW_GROUP_LANES, 4), (V.OPTION_SHADOW_GROUP_LANES, 24)):
         with pytest.raises(V.VkhrtError):
             ctx.set_option(opt, v)
     mlp.close()

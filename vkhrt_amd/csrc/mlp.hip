@@ -113,10 +113,10 @@ __device__ __forceinline__ void sincospi(float v, float& s, float& c) {
     pc = fmaf(u, pc, kCosPi4);
     pc = fmaf(u, pc, kCosPi2);
     const float cf = fmaf(u, pc, 1.0f);
-    const int q = int(n) & 3;  // sin(pi f + q pi / 2), cos(pi f + q pi / 2)
+    const int q = int(n) & 3;  // sin(pi f + q pi / 2), cos(pi f + q pi / 2): swap, then sign flips
     const float a = (q & 1) ? cf : sf, b = (q & 1) ? sf : cf;
-    s = (q & 2) ? -a : a;
-    c = ((q + 1) & 2) ? -b : b;
+    s = __uint_as_float(__float_as_uint(a) ^ (uint32_t(q & 2) << 30));
+    c = __uint_as_float(__float_as_uint(b) ^ (uint32_t((q + 1) & 2) << 30));
 }
 
 // Feature f = 16 t + 4 g + j of the lane's sample for input dim t: k = 2 g + j / 2,
@@ -257,6 +257,8 @@ __device__ __forceinline__ void encode_sample(const float4* __restrict__ samples
 }
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+// Inference form: the hardware exp2 and reciprocal (a few ulp, far below the network's error).
+__device__ __forceinline__ float sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // Stages the forward fragments and the FP32 small parameters into LDS (all threads).
 __device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ packed, const float* __restrict__ master,
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(256) void k_infer(MlpDev m, const float4* __restric
         forward32(lw, lsmall, enc, lane, z);
         // lane groups 0 and 1 write tiles 0 and 1
         const uint32_t s = base + uint32_t(16 * g + (lane & 15));
-        if (g < 2 && s < n) out[s] = sigmoid(g == 0 ? z[0] : z[1]);
+        if (g < 2 && s < n) out[s] = sigmoid_fast(g == 0 ? z[0] : z[1]);
     }
 }
 
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
         // lane groups 0 and 1 own the samples of tiles 0 and 1
         const uint32_t q = base + uint32_t(16 * g + (lane & 15));
         const bool own = g < 2 && q < n;
-        const float p = sigmoid(g == 0 ? z[0] : z[1]);
+        const float p = sigmoid_fast(g == 0 ? z[0] : z[1]);
         const uint32_t pix = own ? sp.queue[q] : 0u;
         const bool fallback = own && p * (1.0f - p) > sp.threshold;
         if (own && !fallback) sp.vis[pix] = p;

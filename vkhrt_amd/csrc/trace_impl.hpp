@@ -1790,27 +1790,40 @@ __global__ __launch_bounds__(kTraceBlock) void k_accurate_queued(TraceParams tp,
 // pops up to LPR nodes of its ray from its stack in LDS and takes up to LPR pending leaf records
 // from its record queue in LDS, one per lane, and issues the loads of both together: an
 // iteration costs one memory round trip. Node tests push the internal children they hit and
-// append their leaf records; records run the early-out and Prhi, and a grazing root runs the
-// FP64 continuation at once (kPolishInline: the few thousand walks have the latency to spare,
-// and the FP64 polish launch after them finds nothing to do). A walk ends at the first hit in
-// [tmin, tmax] (occluded) or when stack and queue are empty in every instance the ray enters
-// (visible); the group then takes its next ray. The visibility equals the persistent kernel's
-// for every ray. A hair walk's frontier holds about 5 nodes per step, so a group of 8 lanes
-// idles few of them; a whole wavefront per ray spent most of its issue slots on idle lanes.
+// append their leaf records; records run the early-out test, and the (group, record) pairs
+// that pass wait in a queue of the wavefront. Prhi runs in rounds over that queue, one pair per
+// lane at full wave width, when 32 pairs wait or no group can step on without their results; a
+// grazing root runs the FP64 continuation at once (kPolishInline: the few thousand walks have
+// the latency to spare, and the FP64 polish launch after them finds nothing to do). Without the
+// queue every group paid for any group's Prhi call: a wavefront of 8 rays ran Prhi in most
+// iterations. A walk ends at the first hit in [tmin, tmax] (occluded; its waiting pairs are
+// dropped) or when stack, record queue and waiting pairs are empty in every instance the ray
+// enters (visible); the group then takes its next ray. The visibility equals the persistent
+// kernel's for every ray. On the 16M-segment area-light frame's fallback (5k rays) the walk
+// takes 62 us with 16 lanes per ray (the default) and 86-100 us with 8 (deferred Prhi results
+// let occluded walks run longer); a whole wavefront per ray without the pair queue took 84-97 us.
 //
 // Nodes are popped only while the record queue has room for all they can append (24 records
 // per node: 8 slots of up to 3) and the stack has room for their children (a popped node adds
-// at most 7 entries). Past kGrpStack - kGrpReserve a group pops one node at a time, a
-// depth-first walk whose stack grows by at most 7 entries per level, so the reserve covers
-// trees 32 levels deep; past kGrpStack the walk reports an overflow.
+// at most 7 entries). Past kStack - kGrpReserve a group pops one node at a time, a depth-first
+// walk whose stack grows by at most 7 entries per level, so the reserve covers trees 32 levels
+// deep; past kStack the walk reports an overflow.
 constexpr uint32_t kGrpReserve = 7u * 32u + 8u;
-constexpr uint32_t kGrpNodeRecords = 24;   // leaf records one node can append
-constexpr uint32_t kGrpGrid = 8192;        // one-wave blocks of the launch
+constexpr uint32_t kGrpNodeRecords = 24;  // leaf records one node can append
+constexpr uint32_t kGrpGrid = 8192;       // one-wave blocks of the launch
+constexpr uint32_t kGrpRound = 32;        // waiting pairs that start a Prhi round
+constexpr uint32_t kGrpSlotBits = 26;     // pair = group << 26 | leaf slot (phantom BLASes stay below 2^26 records)
 template <int LPR>
 struct GroupShape {
-    static constexpr uint32_t kGroups = 64u / LPR;                     // rays per wavefront
+    static constexpr uint32_t kGroups = 64u / LPR;  // rays per wavefront
     static constexpr uint32_t kStack = LPR >= 32 ? 1024u : (LPR == 16 ? 512u : 384u);
     static constexpr uint32_t kLeaves = LPR >= 32 ? 512u : (LPR == 16 ? 512u : 256u);
+};
+// A group's ray as the Prhi rounds read it: frame, interval and the instance's records.
+struct GroupRay {
+    RayFrame F;
+    float tmin, tmax, radius;
+    const float4* prims;
 };
 
 template <class G, int IM, int LPR>
@@ -1819,30 +1832,36 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
     static_assert(G::kMode == kPhantom && !G::kMixed, "lane-group shadows serve phantom TLASes");
     static_assert(LPR == 8 || LPR == 16 || LPR == 32 || LPR == 64, "lanes per ray: 8, 16, 32 or 64");
     using S = GroupShape<LPR>;
-    __shared__ uint32_t s_stack[S::kGroups][S::kStack];
-    __shared__ uint32_t s_leaf[S::kGroups][S::kLeaves];
+    constexpr uint32_t NG = S::kGroups;
+    __shared__ uint32_t s_stack[NG][S::kStack];
+    __shared__ uint32_t s_leaf[NG][S::kLeaves];
+    __shared__ uint32_t s_pairs[128];  // waiting (group, slot) pairs: fewer than 64 + 64
+    __shared__ GroupRay s_ray[NG];
+    __shared__ uint32_t s_hit[NG], s_done[NG];
     const uint32_t total = *count;
     if (total == 0 || total > limit) return;
     const TraceParams& tp = rp.tp;
     const uint32_t lane = threadIdx.x, grp = lane / LPR, sub = lane % LPR, last = grp * LPR + LPR - 1u;
     const uint64_t gmask = (LPR == 64 ? ~0ull : ((1ull << LPR) - 1ull)) << (grp * LPR);
+    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t* stk = s_stack[grp];
     uint32_t* lf = s_leaf[grp];
     const SrcShadow src{rp.cam, queue, rp.hitbuf, rp.light};
-    const uint32_t qstride = gridDim.x * S::kGroups;
+    const uint32_t qstride = gridDim.x * NG;
+    if (lane < NG) s_hit[lane] = s_done[lane] = 0u;
 #ifdef VKHRT_WORK_COUNTERS
     unsigned long long wc[7] = {};
     unsigned long long t_ray = 0, t_max = 0;
 #endif
     // group state: the same value in every lane of the group
-    uint32_t qi = blockIdx.x * S::kGroups + grp;
+    uint32_t qi = blockIdx.x * NG + grp;
     bool have_ray = false, walking = false, occluded = false, error = false;
-    uint32_t inst = 0, sp = 0, lh = 0, lt = 0;
+    uint32_t inst = 0, sp = 0, lh = 0, lt = 0, pend = 0;
     f3 ow{}, dw{}, o{}, d{}, inv{};
     float tmin = 0.0f, tmax = 0.0f;
     uint32_t oct = 0;
-    RayFrame F;
     BlasDev B{};
+    uint32_t npairs = 0;  // wave-uniform
     while (true) {
         // per group: take a ray, then enter its next instance (or finish the ray)
         if (!have_ray && qi < total) {
@@ -1871,10 +1890,18 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
                 }
                 inv = safe_inv(d);
                 oct = uint32_t(signbit(d.x)) | (uint32_t(signbit(d.y)) << 1) | (uint32_t(signbit(d.z)) << 2);
-                make_frame(o, d, F);
-                if (sub == 0) stk[0] = 0u;  // the root
+                if (sub == 0) {
+                    GroupRay& R = s_ray[grp];
+                    make_frame(o, d, R.F);
+                    R.tmin = tmin;
+                    R.tmax = tmax;
+                    R.radius = B.radius;
+                    R.prims = B.prims;
+                    stk[0] = 0u;  // the root
+                }
                 sp = 1;
                 lh = lt = 0;
+                pend = 0;
                 walking = true;
             } else {
                 if (error && sub == 0) atomicAdd(&tp.status[0], 1u);
@@ -1903,8 +1930,8 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
         if (__ballot(walking) == 0) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        // one iteration of every walking group: its nodes (as many as its record queue and stack
-        // can take) and its records
+        // one step of every walking group: its nodes (as many as its record queue and stack can
+        // take) and its records
         uint32_t m = 0, nl = 0;
         if (walking) {
             const uint32_t by_stack = sp + kGrpReserve <= S::kStack ? (S::kStack - kGrpReserve - sp) / 7u : 0u;
@@ -1919,16 +1946,15 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
             lh += nl;
             if (lh == lt) lh = lt = 0;
         }
-        // issue the record's loads before the node test so both are in flight together
-        float4 cv[4] = {};
+        // issue the early-out's loads (the record's end points and radius) before the node test
+        float4 e0{}, e3{};
+        float rmax = 0.0f;
         if (slot != 0xFFFFFFFFu) {
             const auto* pr = gmem(B.prims + size_t(slot) * 4);
-            cv[0] = pr[0];
-            cv[1] = pr[1];
-            cv[2] = pr[2];
-            cv[3] = pr[3];
+            e0 = pr[0];
+            e3 = pr[3];
+            rmax = gmem(B.aux)[slot];
         }
-        const float rmax = slot != 0xFFFFFFFFu ? gmem(B.aux)[slot] : 0.0f;
         uint32_t hits = 0, pmask = 0, child_base = 0, imask = 0, pbase = 0;
         bool bad = false;
         if (node != 0xFFFFFFFFu) {
@@ -1942,19 +1968,13 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
                 bad = true;
             }
         }
-        // the record: early out, then Prhi with the FP64 continuation of grazing roots
-        bool hit = false, pass = false;
-        if (slot != 0xFFFFFFFFu) {
-            pass = ray_cylinder(o, d, f3{cv[0].x, cv[0].y, cv[0].z}, f3{cv[3].x, cv[3].y, cv[3].z}, rmax);
-            if (pass) {
-                PhantomHit ph;
-                hit = prhi<G::kRMode, G::kPolish ? kPolishInline : kPolishOff, false>(F, cv, 0.0f, B.radius, ph);
-                hit = hit && ph.t >= tmin && ph.t <= tmax;
-            }
-        }
-        const uint64_t hb = __ballot(hit), eb = __ballot(bad);
-#ifdef VKHRT_WORK_COUNTERS
+        const bool pass = slot != 0xFFFFFFFFu && ray_cylinder(o, d, f3{e0.x, e0.y, e0.z}, f3{e3.x, e3.y, e3.z}, rmax);
+        // the pairs that passed join the wavefront's queue (npairs < 64 here, so they fit)
         const uint64_t pb = __ballot(pass);
+        if (pass) s_pairs[npairs + uint32_t(__popcll(pb & below))] = (grp << kGrpSlotBits) | slot;
+        npairs += uint32_t(__popcll(pb));
+        const uint64_t eb = __ballot(bad);
+#ifdef VKHRT_WORK_COUNTERS
         if (walking && sub == 0) {
             ++wc[1];
             wc[2] += m;
@@ -1962,8 +1982,6 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
             wc[4] += __popcll(pb & gmask);
             wc[5] += nl ? 1u : 0u;
         }
-#else
-        (void)pass;
 #endif
         // push the internal children hit (octant bit j -> slot j ^ oct -> child index) and
         // append the leaf records hit (inclusive prefix sums over the group's lanes)
@@ -1979,7 +1997,7 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
         }
         const uint32_t ctot = __shfl(cpre, last), ptot = __shfl(ppre, last);
         if (walking) {
-            occluded = (hb & gmask) != 0;
+            pend += uint32_t(__popcll(pb & gmask));
             error = (eb & gmask) != 0 || sp + ctot > S::kStack || lt + ptot > S::kLeaves;
             if (!error) {
                 uint32_t w = sp + cpre - nc;
@@ -1992,14 +2010,78 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
                 sp += ctot;
                 lt += ptot;
             }
-            // the instance is done when the ray is occluded, an error stopped it, or nothing is left
-            if (occluded || error || (sp == 0 && lh == lt)) {
-                walking = false;
-                ++inst;
-            }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        // a Prhi round when enough pairs wait, or when no walking group can step on without them
+        // (every group with pairs waiting has an empty stack and record queue)
+        const bool can_step = walking && !error && (sp > 0 || lh < lt);
+        if (npairs >= kGrpRound || (npairs > 0 && __ballot(can_step) == 0)) {
+            const uint32_t k = npairs < 64u ? npairs : 64u;
+            if (lane < k) {
+                const uint32_t e = s_pairs[lane];
+                const uint32_t g = e >> kGrpSlotBits;
+                const uint32_t sl = e & ((1u << kGrpSlotBits) - 1u);
+                const GroupRay& R = s_ray[g];
+                const auto* pr = gmem(R.prims + size_t(sl) * 4);
+                float4 cv[4] = {pr[0], pr[1], pr[2], pr[3]};
+                PhantomHit ph;
+                const bool hit = prhi<G::kRMode, G::kPolish ? kPolishInline : kPolishOff, false>(R.F, cv, 0.0f,
+                                                                                                 R.radius, ph) &&
+                                 ph.t >= R.tmin && ph.t <= R.tmax;
+                if (hit) s_hit[g] = 1u;
+                atomicAdd(&s_done[g], 1u);
+            }
+            // the pairs not taken move to the front ([64, npairs) -> [0, npairs - 64): no overlap)
+            const uint32_t rest = npairs - k;
+            const uint32_t moved = lane < rest ? s_pairs[64u + lane] : 0u;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane < rest) s_pairs[lane] = moved;
+            npairs = rest;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (walking) {
+                occluded = occluded || s_hit[grp] != 0u;
+                pend -= s_done[grp];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane < NG) s_hit[lane] = s_done[lane] = 0u;
+        }
+        // an occluded or failed walk drops its waiting pairs (the others keep their order)
+        const bool drop = walking && (occluded || error);
+        if (__ballot(drop) != 0) {
+            if (lane < NG) s_hit[lane] = 0u;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (drop && sub == 0) s_hit[grp] = 1u;  // groups whose pairs go
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            uint32_t kept_total = 0;
+            for (uint32_t base = 0; base < npairs; base += 64u) {
+                const uint32_t i = base + lane;
+                const uint32_t e = i < npairs ? s_pairs[i] : 0u;
+                const bool keep = i < npairs && s_hit[e >> kGrpSlotBits] == 0u;
+                const uint64_t kb = __ballot(keep);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (keep) s_pairs[kept_total + uint32_t(__popcll(kb & below))] = e;
+                kept_total += uint32_t(__popcll(kb));
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            npairs = kept_total;
+            if (lane < NG) s_hit[lane] = 0u;
+            if (drop) pend = 0;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+        // the instance is done when the ray is occluded, an error stopped it, or nothing is left
+        if (walking && (occluded || error || (sp == 0 && lh == lt && pend == 0))) {
+            walking = false;
+            ++inst;
+        }
     }
 #ifdef VKHRT_WORK_COUNTERS
     for (int i = 0; i < 7; ++i) {
@@ -2177,7 +2259,7 @@ hipError_t launch_trace_pass(const TraceParams& tp, const float* rays, uint32_t 
     return hipGetLastError();
 }
 
-// The neural fallback's lane-group walker: rp.group_lanes (8, 16, 32 or 64) lanes per ray.
+// The neural fallback's lane-group walker: rp.group_lanes (8, 16, 32 or 64; 16 otherwise) lanes per ray.
 template <class G>
 hipError_t launch_shadow_groups(const RenderParams& rp, const uint32_t* squeue, const uint32_t* scount, uint32_t limit,
                                 int im, hipStream_t st) {
@@ -2189,10 +2271,10 @@ hipError_t launch_shadow_groups(const RenderParams& rp, const uint32_t* squeue, 
     else                                                                                                           \
         hipLaunchKernelGGL((k_render_shadow_groups<G, kInstMany, L>), grid, block, 0, st, rp, squeue, scount, limit);
     switch (rp.group_lanes) {
-        case 16: VK_GROUPS(16) break;
+        case 8: VK_GROUPS(8) break;
         case 32: VK_GROUPS(32) break;
         case 64: VK_GROUPS(64) break;
-        default: VK_GROUPS(8) break;
+        default: VK_GROUPS(16) break;
     }
 #undef VK_GROUPS
     return hipGetLastError();

@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--accurate", type=int, default=1)
     ap.add_argument("--area", type=int, default=0, help="also the 16M-segment area-light frame's neural fallback")
-    ap.add_argument("--group-lanes", type=int, default=8, help="lanes per ray of the fallback's lane-group walker")
+    ap.add_argument("--group-lanes", type=int, default=16, help="lanes per ray of the fallback's lane-group walker")
     args = ap.parse_args()
 
     import ctypes
