@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--ray-order", type=int, default=0, help="0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles")
     ap.add_argument("--xcd-chunk", type=int, default=4096,
                     help="rays handed out per XCD in runs of this many (0: one work counter)")
+    ap.add_argument("--steps-per-iteration", type=int, default=3,
+                    help="persistent phantom traversal: traversal steps per bookkeeping pass (1..8)")
     ap.add_argument("--shadow-group-lanes", type=int, default=16,
                     help="lanes per ray of the neural fallback's lane-group walker (8, 16, 32 or 64)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
@@ -114,6 +116,7 @@ def main():
     ctx.set_option(V.OPTION_XCD_CHUNK, args.xcd_chunk)
     ctx.set_option(V.OPTION_BLOCKS_PER_CU, args.blocks_per_cu)
     ctx.set_option(V.OPTION_SHADOW_GROUP_LANES, args.shadow_group_lanes)
+    ctx.set_option(V.OPTION_STEPS_PER_ITERATION, args.steps_per_iteration)
 
     NS, SEG, W, H = args.strands, args.segments, args.width, args.height
     nseg_total = NS * SEG
@@ -147,13 +150,13 @@ def main():
         blas = V.BottomLevelAccelerationStructure(ctx, V.GEOMETRY_PHANTOM_CURVES, curves.data_ptr(),
                                                   nseg_total, flags=V.BLAS_PER_POINT_RADIUS, stream=sp)
         tlas = V.TopLevelAccelerationStructure(ctx, [blas], stream=sp)
-        tr = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags, repeats=5, stream=sp)
+        tr = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags, repeats=9, stream=sp)
         torch.cuda.synchronize()
         binfo = blas.info()
         tuning = {"bvh_width": tb["bvh_width"], "leaf_size": tb["leaf_size"], "bvh_collapse": tb["bvh_collapse"],
                   "stack_kind": tr["stack_kind"],
                   "blocks_per_cu": tr["blocks_per_cu"], "refill_min": tr["refill_min"], "ray_order": tr["ray_order"],
-                  "xcd_chunk": tr["xcd_chunk"],
+                  "xcd_chunk": tr["xcd_chunk"], "steps_per_iteration": tr["steps_per_iteration"],
                   "trials": tb["trials"] + tr["trials"], "seconds": round(time.perf_counter() - t0, 2),
                   "start_ms": round(tb["baseline_ms"], 4), "tuned_ms": round(tr["best_ms"], 4)}
         log(f"[bench] autotuned: {tuning}")
@@ -275,6 +278,7 @@ def main():
             "blocks_per_cu": tuning["blocks_per_cu"] if tuning else args.blocks_per_cu,
             "ray_order": tuning["ray_order"] if tuning else args.ray_order,
             "xcd_chunk": tuning["xcd_chunk"] if tuning else args.xcd_chunk,
+            "steps_per_iteration": tuning["steps_per_iteration"] if tuning else args.steps_per_iteration,
             "autotune": tuning,
         },
         "roofline": roofline,

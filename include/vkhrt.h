@@ -250,6 +250,9 @@ int vkhrt_abi_version(void);
                                             ray per group, a longer one by the persistent lanes; 0 =
                                             always the persistent lanes. Results do not depend on it */
 #define VKHRT_OPTION_SHADOW_GROUP_LANES 11 /* lanes per ray of that walk: 8, 16 (default), 32 or 64 */
+#define VKHRT_OPTION_STEPS_PER_ITERATION 12 /* persistent phantom traversal: 1..8 traversal steps per
+                                              pass over the lanes' refill and finish bookkeeping
+                                              (default 3) */
 int vkhrt_set_option(vkhrt_context* ctx, int option, int value);
 
 /* ---- acceleration structures */
@@ -427,13 +430,15 @@ typedef struct vkhrt_tuning {
     float best_ms;         /* ... with the chosen ones (<= baseline_ms) */
     int32_t xcd_chunk;     /* VKHRT_OPTION_XCD_CHUNK */
     int32_t bvh_collapse;  /* VKHRT_OPTION_BVH_COLLAPSE (vkhrt_autotune_blas) */
+    int32_t steps_per_iteration; /* VKHRT_OPTION_STEPS_PER_ITERATION */
 } vkhrt_tuning;
 /* vkhrt_render(tlas, camera, desc) as the workload; sets stack kind, blocks per CU, refill
- * minimum, ray order and XCD chunk. */
+ * minimum, ray order, XCD chunk and steps per iteration. */
 int vkhrt_autotune_render(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_camera* camera,
                           const vkhrt_render_desc* desc, uint32_t repeats, vkhrt_tuning* out, void* stream);
-/* vkhrt_render_batch as the workload; sets stack kind, blocks per CU, refill minimum and XCD
- * chunk and reports the best sort flags in out->batch_flags (desc->flags' sort bits are the start). */
+/* vkhrt_render_batch as the workload; sets stack kind, blocks per CU, refill minimum, XCD chunk
+ * and steps per iteration and reports the best sort flags in out->batch_flags (desc->flags' sort
+ * bits are the start). */
 int vkhrt_autotune_batch(vkhrt_context* ctx, const vkhrt_tlas* tlas, const vkhrt_view* views, uint32_t view_count,
                          const vkhrt_batch_desc* desc, uint32_t repeats, vkhrt_tuning* out, void* stream);
 /* Builds `blas` (with its TLAS) for each BVH width in {4, 6, 8} and leaf size in {1, 2, 3},

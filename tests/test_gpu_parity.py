@@ -266,6 +266,40 @@ def test_persistent_and_plain_kernels_agree(gpu_ctx, G, oracle, curves, persiste
     _compare_hits(h, ref, f"persistent={persistent} chunk={chunk}")
 
 
+@pytest.mark.parametrize("steps", [2, 3, 4, 8])
+def test_steps_per_iteration_leave_results_unchanged(gpu_ctx, G, oracle, curves, steps):
+    """Several traversal steps per pass over the persistent loop's bookkeeping: closest hits
+    equal the oracle's bit for bit, any-hit decisions too, and the C0 render with shadows and
+    FP64 polish equals the one-step render pixel for pixel."""
+    import scenes
+    import vkhrt_amd as V
+    rays = scenes.targeted_rays(curves, 20000, seed=12)
+    S = oracle.Scene()
+    S.add(oracle.MODE_PHANTOM, curves, per_point=True)
+    ref = S.trace(rays)
+    rs = S.trace(rays, flags=oracle.F_ANY_HIT)
+    blas = G.build_blas(gpu_ctx, V.GEOMETRY_PHANTOM_CURVES, curves, per_point=True)
+    tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
+    cam = V.camera_look_at((0.0, 112.0, 140.0), (0.0, 105.0, 0.0), fov_deg=22.6, aspect=1.0)
+    flags = V.RENDER_SHADOWS | V.RENDER_FP64_POLISH
+    try:
+        gpu_ctx.set_option(V.OPTION_STEPS_PER_ITERATION, 1)
+        one, one_ids = G.render(gpu_ctx, tlas, cam, 256, 256, flags)
+        gpu_ctx.set_option(V.OPTION_STEPS_PER_ITERATION, steps)
+        h = G.trace(gpu_ctx, tlas, rays)
+        hs = G.trace(gpu_ctx, tlas, rays, flags=V.TRACE_TERMINATE_ON_FIRST_HIT)
+        img, ids = G.render(gpu_ctx, tlas, cam, 256, 256, flags)
+    finally:
+        gpu_ctx.set_option(V.OPTION_STEPS_PER_ITERATION, 3)
+    _compare_hits(h, ref, f"steps per iteration {steps}")
+    assert np.array_equal(hs["prim"] == 0xFFFFFFFF, rs["prim"] == 0xFFFFFFFF)
+    assert np.array_equal(ids, one_ids) and np.array_equal(img.view(np.uint32), one.view(np.uint32))
+    assert (ids != 0xFFFFFFFF).sum() > 1000
+    for v in (0, 9):
+        with pytest.raises(V.VkhrtError):
+            gpu_ctx.set_option(V.OPTION_STEPS_PER_ITERATION, v)
+
+
 @pytest.mark.parametrize("any_hit", [False, True])
 def test_persistent_many_rays_per_lane(gpu_ctx, G, oracle, curves, any_hit):
     """1.5M rays (several per resident lane), led by 300k rays that miss everything so whole

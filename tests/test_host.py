@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol(vk):
     assert vk.lib().vkhrt_abi_version() == vk.ABI_VERSION
     hdr = open(os.path.join(ROOT, "include", "vkhrt.h")).read()
     assert int(re.search(r"#define VKHRT_ABI_VERSION (\d+)", hdr).group(1)) == vk.ABI_VERSION
-    assert ctypes.sizeof(vk.Tuning) == 12 * 4  # vkhrt_tuning: twelve 4-byte fields
+    assert ctypes.sizeof(vk.Tuning) == 13 * 4  # vkhrt_tuning: thirteen 4-byte fields
 
 
 def test_null_handles_are_rejected(vk):

@@ -52,6 +52,7 @@ OPTION_BVH_COLLAPSE = 9
 OPTION_XCD_CHUNK = 8
 OPTION_SHADOW_WAVE_LIMIT = 10
 OPTION_SHADOW_GROUP_LANES = 11
+OPTION_STEPS_PER_ITERATION = 12
 
 BATCH_MAX_BOUNCES = 8
 BATCH_MAX_MATERIALS = 16
@@ -166,7 +167,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("stack_kind", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("refill_min", ctypes.c_int32),
                 ("ray_order", ctypes.c_int32), ("bvh_width", ctypes.c_int32), ("leaf_size", ctypes.c_int32),
                 ("batch_flags", ctypes.c_uint32), ("trials", ctypes.c_uint32), ("baseline_ms", ctypes.c_float),
-                ("best_ms", ctypes.c_float), ("xcd_chunk", ctypes.c_int32), ("bvh_collapse", ctypes.c_int32)]
+                ("best_ms", ctypes.c_float), ("xcd_chunk", ctypes.c_int32), ("bvh_collapse", ctypes.c_int32),
+                ("steps_per_iteration", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

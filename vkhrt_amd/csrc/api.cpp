@@ -45,6 +45,7 @@ struct vkhrt_context {
     uint32_t blocks_per_cu = 0;  // persistent grids: 256-thread blocks per CU (0 = occupancy limit)
     uint32_t shadow_wave_limit = 16384;  // neural fallback queues up to this long: lane groups per ray
     uint32_t shadow_group_lanes = 16;    // lanes per fallback ray of that walk
+    uint32_t steps_per_iter = 3;         // compacted phantom loop: traversal steps per bookkeeping pass
     bool profiling = false;
     std::vector<PendingTiming> pending;
     std::map<std::string, std::pair<double, uint32_t>> stats;
@@ -620,6 +621,11 @@ int vkhrt_set_option(vkhrt_context* c, int option, int value) {
         c->xcd_chunk = uint32_t(value);
         return VKHRT_SUCCESS;
     }
+    if (option == VKHRT_OPTION_STEPS_PER_ITERATION) {
+        if (value < 1 || value > 8) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "steps per iteration must be 1..8");
+        c->steps_per_iter = uint32_t(value);
+        return VKHRT_SUCCESS;
+    }
     if (option == VKHRT_OPTION_SHADOW_GROUP_LANES) {
         if (value != 8 && value != 16 && value != 32 && value != 64)
             return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "shadow group lanes must be 8, 16, 32 or 64");
@@ -1149,6 +1155,7 @@ int trace_launch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_ray* rays, u
     tp.work_counts = c->work_counts;
     tp.blocks_per_cu = c->blocks_per_cu;
     tp.xcd_chunk = c->xcd_chunk;
+    tp.steps_per_iter = c->steps_per_iter;
     tp.xcd_work = s.xcd_work;
     const bool polish = (flags & VKHRT_TRACE_FP64_POLISH) != 0;
     if (polish) {
@@ -1262,6 +1269,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     rp.queue = c->queue;
     rp.counters = c->counters;
     rp.tp.xcd_chunk = c->xcd_chunk;
+    rp.tp.steps_per_iter = c->steps_per_iter;
     rp.light_vis = const_cast<float*>(d->light_visibility);
     rp.hitrec = c->hitrec;
     rp.vis = c->vis;
@@ -1423,6 +1431,7 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
                             c->b_counters, c->refill_min};
     bp.tp.blocks_per_cu = c->blocks_per_cu;
     bp.tp.xcd_chunk = c->xcd_chunk;
+    bp.tp.steps_per_iter = c->steps_per_iter;
     if (polish) {
         bp.tp.defer = c->b_defer;
         bp.tp.defer_cap = uint32_t(std::max<size_t>(c->b_cap, 65536));
@@ -1479,7 +1488,7 @@ int vkhrt_render_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view* 
     if (d->flags & VKHRT_BATCH_GRAPH) {
         const std::vector<uint64_t> key = {t->serial, uint64_t(variant), uint64_t(stack), total,         d->bounces,
                                            bp.sort,   polish,            c->refill_min,   c->blocks_per_cu, c->batch_gen,
-                                           c->xcd_chunk};
+                                           c->xcd_chunk, c->steps_per_iter};
         if (!c->graph_exec || c->graph_key != key) {
             if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
             if (c->graph) hipGraphDestroy(c->graph);
@@ -1630,6 +1639,7 @@ void fill_tuning(const vkhrt_context* c, vkhrt_tuning* out) {
     out->bvh_width = c->bvh_width;
     out->bvh_collapse = c->bvh_collapse;
     out->leaf_size = c->leaf_size;
+    out->steps_per_iteration = int32_t(c->steps_per_iter);
 }
 
 // Stack kinds that can hold the TLAS's deepest tree (smaller ones are raised to the same).
@@ -1651,18 +1661,20 @@ int vkhrt_autotune_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_cam
     *out = vkhrt_tuning{};
     hipStream_t st = pick(c, stream);
     int stack = c->stack_kind, bpc = int(c->blocks_per_cu), refill = int(c->refill_min), order = int(c->ray_order);
-    int chunk = int(c->xcd_chunk);
+    int chunk = int(c->xcd_chunk), steps = int(c->steps_per_iter);
     std::vector<Knob> knobs = {{&stack, stack_candidates(t)},
                                {&bpc, {0, 6, 4, 3, 2}},
                                {&refill, {4, 8, 16, 32, 48}},
                                {&order, {0, 1, 2}},
-                               {&chunk, {0, 512, 4096, 16384}}};
+                               {&chunk, {0, 512, 4096, 16384}},
+                               {&steps, {1, 2, 3, 4, 6}}};
     auto apply = [&] {
         c->stack_kind = stack;
         c->blocks_per_cu = uint32_t(bpc);
         c->refill_min = uint32_t(refill);
         c->ray_order = uint32_t(order);
         c->xcd_chunk = uint32_t(chunk);
+        c->steps_per_iter = uint32_t(steps);
     };
     const int rc = coordinate_descent(c, st, repeats, knobs, apply, [&] { return vkhrt_render(c, t, cam, d, stream); },
                                       out);
@@ -1679,17 +1691,19 @@ int vkhrt_autotune_batch(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_view
     const uint32_t sort_bits = VKHRT_BATCH_SORT_MATERIALS | VKHRT_BATCH_SORT_RAYS;
     vkhrt_batch_desc dd = *d;
     int stack = c->stack_kind, bpc = int(c->blocks_per_cu), refill = int(c->refill_min);
-    int sort = int(d->flags & sort_bits), chunk = int(c->xcd_chunk);
+    int sort = int(d->flags & sort_bits), chunk = int(c->xcd_chunk), steps = int(c->steps_per_iter);
     std::vector<Knob> knobs = {{&stack, stack_candidates(t)},
                                {&bpc, {0, 6, 4, 3, 2}},
                                {&refill, {4, 8, 16, 32, 48}},
                                {&sort, {0, int(VKHRT_BATCH_SORT_RAYS), int(VKHRT_BATCH_SORT_MATERIALS), int(sort_bits)}},
-                               {&chunk, {0, 512, 4096, 16384}}};
+                               {&chunk, {0, 512, 4096, 16384}},
+                               {&steps, {1, 2, 3, 4, 6}}};
     auto apply = [&] {
         c->stack_kind = stack;
         c->blocks_per_cu = uint32_t(bpc);
         c->refill_min = uint32_t(refill);
         c->xcd_chunk = uint32_t(chunk);
+        c->steps_per_iter = uint32_t(steps);
         dd.flags = (d->flags & ~sort_bits) | uint32_t(sort);
     };
     const int rc = coordinate_descent(

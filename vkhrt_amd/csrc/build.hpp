@@ -35,7 +35,7 @@ struct TraceParams {
     uint4* defer;
     uint32_t* defer_count;
     uint32_t defer_cap;
-    unsigned long long* keys;  // per ray (t bits << 32 | instance << 27 | primitive) for the merge
+    unsigned long long* keys;  // per ray (t bits << 32 | global primitive id) for the merge
     HitRec* cand_hits;         // per candidate
     unsigned long long* cand_keys;
     uint32_t blocks_per_cu;    // persistent grids: 256-thread blocks per CU, 0 = as many as fit
@@ -43,11 +43,12 @@ struct TraceParams {
                                // out per XCD in runs of this many (each launch brings 8 counters)
     uint32_t* xcd_work;        // the 8 per-XCD counters of vkhrt_trace_rays's persistent launch
     unsigned long long* work_counts;  // counting build (VKHRT_WORK_COUNTERS): kWorkCounters words, else null
+    uint32_t steps_per_iter;   // compacted phantom loop: traversal steps between bookkeeping passes (0 = 1)
 };
 
 // Work counters of the counting build (compiled with VKHRT_WORK_COUNTERS; the product build
-// leaves them out). Per traversal kind (closest hit from 0, any hit from kWorkAnyHit; the wave-
-// per-ray shadow kernel from kWorkWave), wave-level sums over the persistent loop:
+// leaves them out). Per traversal kind (closest hit from 0, any hit from kWorkAnyHit; the lane-
+// group shadow walker from kWorkWave), wave-level sums over the persistent loop:
 enum WorkCounter {
     kWcIterations = 0,   // loop iterations of all waves
     kWcRounds = 1,       // intersection rounds (iterations that run Prhi on the queued pairs)

@@ -1340,46 +1340,52 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
             VK_WC_END(kWcCyclesRounds);
             continue;
         }
-        // (e) one traversal step per traversing lane; survivors of the early-out are queued
+        // (e) traversal steps, one per traversing lane each; survivors of the early-out are queued.
+        // tp.steps_per_iter > 1 runs more steps before the bookkeeping above comes round again,
+        // while the queue can take a full step of pairs and some lane still walks.
+        const uint32_t nsteps = tp.steps_per_iter > 1u ? tp.steps_per_iter : 1u;
+        for (uint32_t si = 0; si < nsteps; ++si) {
+            if (si > 0u && (qn >= 64u || !__ballot(state == kTrav || state == kHelp))) break;
 #ifdef VKHRT_WORK_COUNTERS
-        {
-            const bool wk = state == kTrav || state == kHelp;
-            VK_WC(kWcNodeTests, __popcll(__ballot(wk && W.pmask == 0u && W.have_node)));
-            VK_WC(kWcLeafTests, __popcll(__ballot(wk && W.pmask != 0u)));
-            VK_WC(kWcStepLanes, __popcll(__ballot(wk)));
-        }
-#endif
-        uint32_t cand = 0xFFFFFFFFu;
-        if (state == kHelp && ANY && uint32_t(s_key[wbase + home]) != kNoHitLow) {
-            atomicSub(&s_walk[wbase + home], 1u);  // the ray is occluded already
-            state = kIdle;
-        } else if (state == kTrav || state == kHelp) {
-            const int r = W.step(blas(inst), tmin, tcur, lds, cand);
-            if (r == kStepError) atomicAdd(&tp.status[0], 1u);
-            if (state == kHelp) {
-                if (r == kStepError || r == kStepDone) {
-                    atomicSub(&s_walk[wbase + home], 1u);
-                    state = kIdle;
-                }
-            } else if (r == kStepError) {
-                ray_done = true;
-                state = kWait;
-            } else if (r == kStepDone) {
-                state = kWait;
+            {
+                const bool wk = state == kTrav || state == kHelp;
+                VK_WC(kWcNodeTests, __popcll(__ballot(wk && W.pmask == 0u && W.have_node)));
+                VK_WC(kWcLeafTests, __popcll(__ballot(wk && W.pmask != 0u)));
+                VK_WC(kWcStepLanes, __popcll(__ballot(wk)));
             }
+#endif
+            uint32_t cand = 0xFFFFFFFFu;
+            if (state == kHelp && ANY && uint32_t(s_key[wbase + home]) != kNoHitLow) {
+                atomicSub(&s_walk[wbase + home], 1u);  // the ray is occluded already
+                state = kIdle;
+            } else if (state == kTrav || state == kHelp) {
+                const int r = W.step(blas(inst), tmin, tcur, lds, cand);
+                if (r == kStepError) atomicAdd(&tp.status[0], 1u);
+                if (state == kHelp) {
+                    if (r == kStepError || r == kStepDone) {
+                        atomicSub(&s_walk[wbase + home], 1u);
+                        state = kIdle;
+                    }
+                } else if (r == kStepError) {
+                    ray_done = true;
+                    state = kWait;
+                } else if (r == kStepDone) {
+                    state = kWait;
+                }
+            }
+            const bool app = cand != 0xFFFFFFFFu;
+            const uint64_t am = __ballot(app);
+            if (app) {
+                queue[qn + __popcll(am & lt)] = (home << kCompactSlotBits) | cand;
+                atomicAdd(&s_cnt[wbase + home], 1u);
+            }
+            qn += __popcll(am);
+            VK_WC(kWcSteps, 1);
+            VK_WC(kWcPairs, __popcll(am));
+            VK_WC_END(kWcCyclesSteps);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
-        const bool app = cand != 0xFFFFFFFFu;
-        const uint64_t am = __ballot(app);
-        if (app) {
-            queue[qn + __popcll(am & lt)] = (home << kCompactSlotBits) | cand;
-            atomicAdd(&s_cnt[wbase + home], 1u);
-        }
-        qn += __popcll(am);
-        VK_WC(kWcSteps, 1);
-        VK_WC(kWcPairs, __popcll(am));
-        VK_WC_END(kWcCyclesSteps);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
     }
 #ifdef VKHRT_WORK_COUNTERS
     if (lane == 0 && tp.work_counts) {
