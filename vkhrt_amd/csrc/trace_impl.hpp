@@ -1341,8 +1341,9 @@ __device__ __forceinline__ void persistent_compact(const TraceParams& tp, WorkCl
             continue;
         }
         // (e) traversal steps, one per traversing lane each; survivors of the early-out are queued.
-        // tp.steps_per_iter > 1 runs more steps before the bookkeeping above comes round again,
-        // while the queue can take a full step of pairs and some lane still walks.
+        // tp.steps_per_iter > 1 runs up to that many steps before the bookkeeping above comes
+        // round again, while the queue can take a full step of pairs and some lane still walks.
+        // (Stopping early once refill_min lanes have stopped walking measured no better.)
         const uint32_t nsteps = tp.steps_per_iter > 1u ? tp.steps_per_iter : 1u;
         for (uint32_t si = 0; si < nsteps; ++si) {
             if (si > 0u && (qn >= 64u || !__ballot(state == kTrav || state == kHelp))) break;
