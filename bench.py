@@ -798,6 +798,14 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
         ctx.synchronize(sp)
         return e0.elapsed_time(e1) / args.steps, ctx.render_stats()
 
+    # the autotuner picks the traversal options on this workload's exact frame (both frames use them)
+    area_tuning = None
+    if args.autotune:
+        t = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags, repeats=5, stream=sp, area_light=light)
+        area_tuning = {k: t[k] for k in ("stack_kind", "blocks_per_cu", "refill_min", "ray_order", "xcd_chunk",
+                                         "steps_per_iteration", "trials")}
+        area_tuning["tuned_ms"] = round(t["best_ms"], 4)
+        log(f"[bench] area light autotuned: {area_tuning}")
     exact_ms, (prim, queries, _) = timed()
     log(f"[bench] area light: 16M segments, exact {exact_ms:.3f} ms/frame, {prim} primary + {queries} shadow rays "
         f"-> {(prim + queries) / (exact_ms * 1e-3) / 1e6:.1f} Mrays/s")
@@ -875,7 +883,7 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
     blas.close()
     return {
         "workload": "phantom_500k_strands_x32_segments_1920x1080_area_light_2_samples",
-        "segments": n, "light_samples": 2,
+        "segments": n, "light_samples": 2, "autotune": area_tuning,
         "exact": {"ms_per_step": round(exact_ms, 4), "primary_rays": prim, "shadow_rays": queries,
                   "mrays_per_s": round((prim + queries) / (exact_ms * 1e-3) / 1e6, 1)},
         "neural": {"ms_per_step": round(neural_ms, 4), "variance_threshold": thr,
@@ -883,7 +891,7 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
                    "frame_speedup_vs_exact": round(exact_ms / neural_ms, 3),
                    "choice": "largest threshold of the sweep whose frame traces part of the queries exactly "
                              "and keeps PSNR >= 40 dB against the exact frame",
-                   "fallback_walk": {"lanes_per_ray": args.shadow_group_lanes, "queue_limit": 16384},
+                   "fallback_walk": {"lanes_per_ray": args.shadow_group_lanes, "queue_limit": 32768},
                    "threshold_sweep": sweep},
         "train": {"samples": ntrain, "epochs": epochs, "batch": 65536, "seconds": round(train_s, 2),
                   "trace_seconds": round(collect_s, 2), "final_loss": round(loss, 5),

@@ -88,6 +88,67 @@ def test_area_light_neural_threshold_zero_is_exact(gpu_ctx, oracle, ball):
     assert not np.array_equal(out["exact"][0], exact.reshape(-1))
 
 
+def test_area_light_gate_matches_infer(gpu_ctx, oracle, ball):
+    """The gate of the neural form, at threshold 0.25 where the network answers every query: each
+    hit pixel's visibility (read back through Shade) is the mean of the network's p over the
+    pixel's two light samples, with p evaluated by the inference kernel on the queries formed
+    from the traced primary hits and the light points (SplitMix64 through the oracle). The gate
+    evaluates both samples of a hit in one wavefront column pair and shares their position
+    features; a wrong pairing changes the mean."""
+    import gpu_util as G
+    import vkhrt_amd as V
+    S, tlas, curves = ball
+    W, H = 128, 96
+    cam = V.camera_look_at((0.0, 20.0, 45.0), (0.0, 8.0, 0.0), aspect=W / H)
+    pts = curves.reshape(-1, 4)[:, :3]
+    lo, hi = pts.min(0) - 1.0, pts.max(0) + 1.0
+    mlp = V.ShadowMLP(gpu_ctx, lo, hi, seed=7)
+    seed = 11
+    L = _light(V, 2, seed)
+    img = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    gpu_ctx.render(tlas, cam, W, H, img.data_ptr(), flags=V.RENDER_SHADOWS | V.RENDER_NEURAL_SHADOWS,
+                   stream=G.stream_ptr(), area_light=L, shadow_mlp=mlp, variance_threshold=0.25)
+    torch.cuda.synchronize()
+    _, queries, traced = gpu_ctx.render_stats()
+    assert traced == 0 and queries > 1000
+    img = img.cpu().numpy().reshape(H * W, 4)
+    rays_d = torch.empty(W * H * 8, dtype=torch.float32, device="cuda")
+    gpu_ctx.generate_camera_rays(cam, W, H, rays_d.data_ptr(), stream=G.stream_ptr())
+    torch.cuda.synchronize()
+    rays = rays_d.cpu().numpy().reshape(-1, 8)
+    hits = G.trace(gpu_ctx, tlas, rays)
+    hit = np.nonzero(hits["prim"] != 0xFFFFFFFF)[0]
+    assert 2 * len(hit) == queries
+    # the queries as the render forms them, operation for operation in float32: origin o + t d,
+    # light point c + a u + b v, direction w * (1 / sqrt(w . w))
+    f32 = np.float32
+    o = rays[hit, 0:3] + hits["t"][hit][:, None] * rays[hit, 4:7]
+    smp = np.zeros((2 * len(hit), 8), np.float32)
+    for s_ in range(2):
+        a = np.array([f32(2.0) * f32(oracle.path_rand(seed, int(p), s_, 0)) - f32(1.0) for p in hit], np.float32)
+        b = np.array([f32(2.0) * f32(oracle.path_rand(seed, int(p), s_, 1)) - f32(1.0) for p in hit], np.float32)
+        y = (LIGHT[0][None, :] + a[:, None] * LIGHT[1][None, :]) + b[:, None] * LIGHT[2][None, :]
+        w = y - o
+        dd = (w[:, 0] * w[:, 0] + w[:, 1] * w[:, 1]) + w[:, 2] * w[:, 2]
+        r = f32(1.0) / np.sqrt(dd)
+        smp[s_::2, 0:3] = o
+        smp[s_::2, 4:7] = w * r[:, None]
+    sd = torch.from_numpy(smp).cuda()
+    pd = torch.empty(len(smp), dtype=torch.float32, device="cuda")
+    mlp.infer(sd.data_ptr(), len(smp), pd.data_ptr(), stream=G.stream_ptr())
+    torch.cuda.synchronize()
+    p = pd.cpu().numpy()
+    v_want = (p[0::2] + p[1::2]) / 2.0
+    ndl = np.abs(hits["n"][hit][:, 1])
+    ok = ndl > 0.2
+    v_got = (img[hit, 0] - 0.3) / (0.4 * ndl)
+    err = np.abs(v_got - v_want)[ok]
+    print(f"[gate vs infer] {ok.sum()} pixels, visibility error max {err.max():.3g}, mean {err.mean():.3g}; "
+          f"p spread {p.min():.3f}..{p.max():.3f}")
+    assert ok.sum() > 300 and err.max() < 1e-4
+    mlp.close()
+
+
 def _straight_fibres(n, rng):
     """n straight fibres (evenly spaced control points) with linearly tapered radii."""
     a = rng.uniform(-20, 20, (n, 3))

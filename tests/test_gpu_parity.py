@@ -106,8 +106,11 @@ def test_geometry_front_end(gpu_ctx, G, oracle):
         assert np.array_equal(gh.view(np.uint32), r.view(np.uint32)), name
 
 
+@pytest.mark.parametrize("steps", [1, 3, 6])
 @pytest.mark.parametrize("mode", ["lss", "dots"])
-def test_lss_dots_trace(gpu_ctx, G, oracle, curves, mode):
+def test_lss_dots_trace(gpu_ctx, G, oracle, curves, mode, steps):
+    """Closest hits of capsules and triangle pieces bit for bit, with 1, 3 and 6 traversal steps
+    per iteration of the persistent loop; any-hit decisions too."""
     import scenes
     import vkhrt_amd as V
     lines = scenes.lines_from_curves(curves[:4000], 4)
@@ -121,9 +124,16 @@ def test_lss_dots_trace(gpu_ctx, G, oracle, curves, mode):
         S.add(oracle.MODE_DOTS, lines, per_point=True)
         blas = G.build_blas(gpu_ctx, V.GEOMETRY_DOTS, lines, per_point=True)
     ref = S.trace(rays)
+    rs = S.trace(rays, flags=oracle.F_ANY_HIT)
     tlas = V.TopLevelAccelerationStructure(gpu_ctx, [blas])
-    h = G.trace(gpu_ctx, tlas, rays)
-    _compare_hits(h, ref, mode)
+    gpu_ctx.set_option(V.OPTION_STEPS_PER_ITERATION, steps)
+    try:
+        h = G.trace(gpu_ctx, tlas, rays)
+        hs = G.trace(gpu_ctx, tlas, rays, flags=V.TRACE_TERMINATE_ON_FIRST_HIT)
+    finally:
+        gpu_ctx.set_option(V.OPTION_STEPS_PER_ITERATION, 3)
+    _compare_hits(h, ref, f"{mode}, {steps} steps per iteration")
+    assert np.array_equal(hs["prim"] == 0xFFFFFFFF, rs["prim"] == 0xFFFFFFFF)
 
 
 def test_instances_with_transforms(gpu_ctx, G, oracle, curves):

@@ -412,8 +412,10 @@ class Context:
                    "render_batch")
 
     # autotuner: times the given workload under candidate settings, keeps the fastest
-    def autotune_render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS, repeats=3, stream=None):
-        d = RenderDesc(width, height, flags, image_ptr, None, None, None, 0.0)
+    def autotune_render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS, repeats=3, stream=None,
+                        area_light=None):
+        d = RenderDesc(width, height, flags, image_ptr, None, None, None, 0.0,
+                       ctypes.pointer(area_light) if area_light is not None else None)
         t = Tuning()
         self.check(lib().vkhrt_autotune_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), repeats,
                                                ctypes.byref(t), stream), "autotune_render")

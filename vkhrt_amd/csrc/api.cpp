@@ -43,7 +43,7 @@ struct vkhrt_context {
     uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles
     uint32_t xcd_chunk = 4096;  // render traversal: rays handed out per XCD in runs of this many (0: one counter)
     uint32_t blocks_per_cu = 0;  // persistent grids: 256-thread blocks per CU (0 = occupancy limit)
-    uint32_t shadow_wave_limit = 16384;  // neural fallback queues up to this long: lane groups per ray
+    uint32_t shadow_wave_limit = 32768;  // neural fallback queues up to this long: lane groups per ray
     uint32_t shadow_group_lanes = 16;    // lanes per fallback ray of that walk
     uint32_t steps_per_iter = 3;         // compacted phantom loop: traversal steps per bookkeeping pass
     bool profiling = false;

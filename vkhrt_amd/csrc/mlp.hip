@@ -123,9 +123,9 @@ __device__ __forceinline__ void sincospi(float v, float& s, float& c) {
 // sin for even j, cos for odd j (feature (dim * 8 + k) * 2 + s). The odd frequency comes from
 // the even one by the double-angle formulas (a few ulp in FP32), which halves the polynomial
 // evaluations.
-__device__ __forceinline__ void encode(const float x[6], int g, s4 enc[6]) {
+__device__ __forceinline__ void encode(const float x[6], int g, s4 enc[6], int t0 = 0, int t1 = 6) {
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {
+    for (int t = t0; t < t1; ++t) {
         float s0, c0;
         sincospi(ldexpf(x[t], 2 * g), s0, c0);
         const float s1 = 2.0f * s0 * c0, c1 = (c0 - s0) * (c0 + s0);
@@ -332,6 +332,7 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
     uint32_t* fb = s_fb[threadIdx.x >> 6];
     uint32_t fbn = 0;  // wave-uniform
     const uint32_t n = *sp.count;
+    const bool paired = sp.paired != 0u && sp.dirs != nullptr;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
     const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
     const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
@@ -340,16 +341,33 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
         // than hoisted out of the loop into ~150 registers, which would leave one wave per SIMD
         asm volatile("" ::: "memory");
         s4 enc[2][6];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        if (paired) {
+            // tile u holds light sample u of hits base / 2 + c: one origin, its features once
             float x[6];
-            gate_input(sp, base + uint32_t(16 * u + (lane & 15)), n, c, ih, x);
-            encode(x, g, enc[u]);
+            const uint32_t q0 = base + 2u * uint32_t(lane & 15);
+            gate_input(sp, q0, n, c, ih, x);
+            encode(x, g, enc[0]);
+#pragma unroll
+            for (int t = 0; t < 3; ++t) enc[1][t] = enc[0][t];
+            if (q0 + 1u < n) {
+                const float4 d = sp.dirs[q0 + 1u];
+                x[3] = d.x;
+                x[4] = d.y;
+                x[5] = d.z;
+            }
+            encode(x, g, enc[1], 3, 6);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                float x[6];
+                gate_input(sp, base + uint32_t(16 * u + (lane & 15)), n, c, ih, x);
+                encode(x, g, enc[u]);
+            }
         }
         float z[2];
         forward32(lw, lsmall, enc, lane, z);
         // lane groups 0 and 1 own the samples of tiles 0 and 1
-        const uint32_t q = base + uint32_t(16 * g + (lane & 15));
+        const uint32_t q = paired ? base + 2u * uint32_t(lane & 15) + uint32_t(g) : base + uint32_t(16 * g + (lane & 15));
         const bool own = g < 2 && q < n;
         const float p = sigmoid_fast(g == 0 ? z[0] : z[1]);
         const uint32_t pix = own ? sp.queue[q] : 0u;

@@ -45,6 +45,7 @@ struct MlpShadowParams {
     uint32_t* fallback_queue; // ids left to the exact shadow ray
     uint32_t* fallback_count;
     const float4* dirs;       // per queue entry directions (area light), null = `dir` for all
+    uint32_t paired;          // 1: entries 2k, 2k + 1 are the two light samples of one hit (one origin)
 };
 
 size_t mlp_train_lds_bytes();

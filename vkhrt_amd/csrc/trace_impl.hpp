@@ -934,28 +934,33 @@ __device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkCl
             }
         }
         if (!__ballot(active)) break;
-        if (active) {
-            bool finish = false;
-            if (need_inst) {
-                // next instance whose world box the ray overlaps (top_level_acceleration_structure.cpp:23-43)
-                while (inst < tp.ninst) {
-                    const InstanceDev& I = tp.inst[inst];
-                    if (I.blas.nprims != 0 && ray_box(ow, dw, I.lo, I.hi, tmin, best.t)) break;
-                    ++inst;
-                }
-                if (inst >= tp.ninst) {
-                    finish = true;
-                } else {
-                    const InstanceDev& I = tp.inst[inst];
-                    f3 oo = ow, dd = dw;
-                    if (!I.identity) {
-                        oo = xform_pt(I.inv, ow);
-                        dd = xform_dir(I.inv, dw);
-                    }
-                    W.begin(oo, dd);
-                    need_inst = false;
-                }
+        bool finish = false;
+        if (active && need_inst) {
+            // next instance whose world box the ray overlaps (top_level_acceleration_structure.cpp:23-43)
+            while (inst < tp.ninst) {
+                const InstanceDev& I = tp.inst[inst];
+                if (I.blas.nprims != 0 && ray_box(ow, dw, I.lo, I.hi, tmin, best.t)) break;
+                ++inst;
+            }
+            if (inst >= tp.ninst) {
+                finish = true;
             } else {
+                const InstanceDev& I = tp.inst[inst];
+                f3 oo = ow, dd = dw;
+                if (!I.identity) {
+                    oo = xform_pt(I.inv, ow);
+                    dd = xform_dir(I.inv, dw);
+                }
+                W.begin(oo, dd);
+                need_inst = false;
+            }
+        }
+        // up to tp.steps_per_iter traversal steps before the refill and finish bookkeeping again
+        const uint32_t nsteps = tp.steps_per_iter > 1u ? tp.steps_per_iter : 1u;
+        for (uint32_t si = 0; si < nsteps; ++si) {
+            const bool walk = active && !need_inst && !finish;
+            if (si > 0u && !__ballot(walk)) break;
+            if (walk) {
                 const int r = W.step(IM == kInstOne ? b1 : tp.inst[inst].blas, tmin, inst, best, lds, nullptr, tp, idx);
                 if (r == kStepError) {
                     atomicAdd(&tp.status[0], 1u);
@@ -967,10 +972,10 @@ __device__ __forceinline__ void persistent_generic(const TraceParams& tp, WorkCl
                     need_inst = true;
                 }
             }
-            if (finish) {
-                snk(idx, ow, dw, best);
-                active = false;
-            }
+        }
+        if (active && finish) {
+            snk(idx, ow, dw, best);
+            active = false;
         }
     }
 }
@@ -1799,7 +1804,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_accurate_queued(TraceParams tp,
 // iteration costs one memory round trip. Node tests push the internal children they hit and
 // append their leaf records; records run the early-out test, and the (group, record) pairs
 // that pass wait in a queue of the wavefront. Prhi runs in rounds over that queue, one pair per
-// lane at full wave width, when 32 pairs wait or no group can step on without their results; a
+// lane at full wave width, when 64 pairs wait or no group can step on without their results (on
+// the 5k-ray fallback: 59 us, against 60, 71 and 67 us for rounds from 32, 16 and 8 pairs); a
 // grazing root runs the FP64 continuation at once (kPolishInline: the few thousand walks have
 // the latency to spare, and the FP64 polish launch after them finds nothing to do). Without the
 // queue every group paid for any group's Prhi call: a wavefront of 8 rays ran Prhi in most
@@ -1818,7 +1824,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_accurate_queued(TraceParams tp,
 constexpr uint32_t kGrpReserve = 7u * 32u + 8u;
 constexpr uint32_t kGrpNodeRecords = 24;  // leaf records one node can append
 constexpr uint32_t kGrpGrid = 8192;       // one-wave blocks of the launch
-constexpr uint32_t kGrpRound = 32;        // waiting pairs that start a Prhi round
+constexpr uint32_t kGrpRound = 64;        // waiting pairs that start a Prhi round (a full wavefront)
 constexpr uint32_t kGrpSlotBits = 26;     // pair = group << 26 | leaf slot (phantom BLASes stay below 2^26 records)
 template <int LPR>
 struct GroupShape {
@@ -2353,8 +2359,10 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                     const uint32_t* scount = rp.counters;
                     if (ev) hipEventRecord(ev[2], st);
                     if (rp.mlp) {
-                        MlpShadowParams sp{rp.queue, rp.counters, rp.origins, {0.0f, 1.0f, 0.0f}, rp.mlp_threshold,
-                                           rp.vis, rp.fallback_queue, rp.counters + 5, rp.dirs};
+                        MlpShadowParams sp{rp.queue,          rp.counters,      rp.origins,
+                                           {0.0f, 1.0f, 0.0f}, rp.mlp_threshold, rp.vis,
+                                           rp.fallback_queue, rp.counters + 5, rp.dirs,
+                                           rp.light.samples == 2u ? 1u : 0u};
                         const hipError_t e = launch_mlp_shadow_gate(*rp.mlp, sp, npix * rp.light.samples, st);
                         if (e != hipSuccess) return e;
                         squeue = rp.fallback_queue;
