@@ -28,6 +28,7 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef float f2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s2 __attribute__((ext_vector_type(2)));
@@ -53,7 +54,7 @@ __device__ __forceinline__ s2 pack2(float a, float b) {
     return __builtin_bit_cast(s2, __builtin_convertvector(f2{a, b}, bf2));
 }
 __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
-    return __builtin_shufflevector(pack2(a, b), pack2(c, d), 0, 1, 2, 3);
+    return __builtin_bit_cast(s4, __builtin_convertvector(f4{a, b, c, d}, bf4));
 }
 // ReLU of four accumulators as bf16: max(x, 0) then rounding equals rounding then max, and a
 // negative bf16 (sign bit set) is a negative int16, so the packed integer max clears it.
@@ -62,6 +63,9 @@ __device__ __forceinline__ s4 relu_pack4(f4 a) {
     const s2 hi = __builtin_elementwise_max(pack2(a[2], a[3]), s2{0, 0});
     return __builtin_shufflevector(lo, hi, 0, 1, 2, 3);
 }
+// max(x, 0) of a finite FP32 value as one integer max: the bit patterns of non-negative floats
+// are non-negative integers in the same order, those of negative floats (and -0) negative.
+__device__ __forceinline__ float relu_bits(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 __device__ __forceinline__ s8 cat(s4 a, s4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
 __device__ __forceinline__ f4 mfma(s8 a, s8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
@@ -236,8 +240,8 @@ __device__ __forceinline__ void forward32(const s8* __restrict__ W, const float*
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            zz[0] = fmaf(w5[j], fmaxf(a0[j], 0.0f), zz[0]);
-            zz[1] = fmaf(w5[j], fmaxf(a1[j], 0.0f), zz[1]);
+            zz[0] = fmaf(w5[j], relu_bits(a0[j]), zz[0]);
+            zz[1] = fmaf(w5[j], relu_bits(a1[j]), zz[1]);
         }
     }
 #pragma unroll
@@ -303,74 +307,105 @@ __global__ __launch_bounds__(256) void k_infer(MlpDev m, const float4* __restric
 // Render form: for each queued shadow pixel, the MLP's visibility p at (hit point, light
 // direction). Confident predictions (p (1 - p) <= threshold) are written as the pixel's
 // visibility; the rest are appended to the exact-fallback queue.
-__device__ __forceinline__ void gate_input(const MlpShadowParams& sp, uint32_t q, uint32_t n, float4 c, float4 ih,
-                                           float x[6]) {
-    x[0] = x[1] = x[2] = x[3] = x[4] = x[5] = 0.0f;
-    if (q >= n) return;
-    const float4 o = sp.origins[q];
+//
+// A lane's inputs of one 32-query pass: the origins and directions of its sample column in
+// the two tiles (paired: one origin, the two light samples' directions) and the pixel id of
+// the query it owns. They are loaded one pass ahead, so their memory latency overlaps the
+// previous pass's network.
+struct GateIn {
+    float4 o[2], d[2];
+    uint32_t pix;
+};
+
+__device__ __forceinline__ void gate_load(const MlpShadowParams& sp, bool paired, uint32_t base, uint32_t n, int lane,
+                                          GateIn& in) {
+    const int g = lane >> 4;
+    const uint32_t c = uint32_t(lane & 15);
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    in.o[0] = in.o[1] = in.d[0] = in.d[1] = z;
+    if (paired) {
+        const uint32_t q0 = base + 2u * c;
+        if (q0 < n) {
+            in.o[0] = in.o[1] = sp.origins[q0];
+            in.d[0] = sp.dirs[q0];
+        }
+        if (q0 + 1u < n) in.d[1] = sp.dirs[q0 + 1u];
+    } else {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint32_t q = base + 16u * uint32_t(u) + c;
+            if (q < n) {
+                in.o[u] = sp.origins[q];
+                if (sp.dirs)
+                    in.d[u] = sp.dirs[q];
+                else
+                    in.d[u] = make_float4(sp.dir[0], sp.dir[1], sp.dir[2], 0.0f);
+            }
+        }
+    }
+    // lane groups 0 and 1 own the queries of tiles 0 and 1
+    const uint32_t q = paired ? base + 2u * c + uint32_t(g) : base + 16u * uint32_t(g) + c;
+    in.pix = 0u;
+    if (g < 2 && q < n) in.pix = sp.queue[q];
+}
+
+__device__ __forceinline__ void gate_features(const float4& o, const float4& d, float4 c, float4 ih, float x[6]) {
     x[0] = (o.x - c.x) * ih.x;
     x[1] = (o.y - c.y) * ih.y;
     x[2] = (o.z - c.z) * ih.z;
-    if (sp.dirs) {
-        const float4 d = sp.dirs[q];
-        x[3] = d.x;
-        x[4] = d.y;
-        x[5] = d.z;
-    } else {
-        x[3] = sp.dir[0];
-        x[4] = sp.dir[1];
-        x[5] = sp.dir[2];
-    }
+    x[3] = d.x;
+    x[4] = d.y;
+    x[5] = d.z;
 }
 
+// kPaired: queue entries 2k, 2k + 1 are the two light samples of one hit (area light, two
+// samples per hit), so tile u of a pass holds light sample u of hits base / 2 + c.
+template <bool kPaired>
 __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
     __shared__ s8 lw[kFragsFwd * 64];
     __shared__ __align__(16) float lsmall[kSmall];
     __shared__ uint32_t s_fb[4][128];  // per-wave fallback pixels, flushed 64+ at a time
-    stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
+    __shared__ uint32_t s_fbn[4], s_fbbase;
     const int lane = threadIdx.x & 63, g = lane >> 4;
+    const uint32_t n = *sp.count;
+    constexpr bool paired = kPaired;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    GateIn cur;
+    if (wave * 32u < n) gate_load(sp, paired, wave * 32u, n, lane, cur);
+    stage_weights(m.packed, m.master, lw, lsmall, kFragsFwd);
     uint32_t* fb = s_fb[threadIdx.x >> 6];
     uint32_t fbn = 0;  // wave-uniform
-    const uint32_t n = *sp.count;
-    const bool paired = sp.paired != 0u && sp.dirs != nullptr;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
     const float4 c = make_float4(m.center[0], m.center[1], m.center[2], 0.0f);
     const float4 ih = make_float4(m.inv_half[0], m.inv_half[1], m.inv_half[2], 0.0f);
     for (uint32_t base = wave * 32; base < n; base += nwaves * 32) {
         // a compiler-only fence: the weight fragments are read from LDS in every pass rather
         // than hoisted out of the loop into ~150 registers, which would leave one wave per SIMD
         asm volatile("" ::: "memory");
+        const GateIn in = cur;
+        const uint32_t next = base + nwaves * 32;
+        if (next < n) gate_load(sp, paired, next, n, lane, cur);
         s4 enc[2][6];
+        float x[6];
+        gate_features(in.o[0], in.d[0], c, ih, x);
+        encode(x, g, enc[0]);
         if (paired) {
             // tile u holds light sample u of hits base / 2 + c: one origin, its features once
-            float x[6];
-            const uint32_t q0 = base + 2u * uint32_t(lane & 15);
-            gate_input(sp, q0, n, c, ih, x);
-            encode(x, g, enc[0]);
 #pragma unroll
             for (int t = 0; t < 3; ++t) enc[1][t] = enc[0][t];
-            if (q0 + 1u < n) {
-                const float4 d = sp.dirs[q0 + 1u];
-                x[3] = d.x;
-                x[4] = d.y;
-                x[5] = d.z;
-            }
+            x[3] = in.d[1].x;
+            x[4] = in.d[1].y;
+            x[5] = in.d[1].z;
             encode(x, g, enc[1], 3, 6);
         } else {
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                float x[6];
-                gate_input(sp, base + uint32_t(16 * u + (lane & 15)), n, c, ih, x);
-                encode(x, g, enc[u]);
-            }
+            gate_features(in.o[1], in.d[1], c, ih, x);
+            encode(x, g, enc[1]);
         }
         float z[2];
         forward32(lw, lsmall, enc, lane, z);
-        // lane groups 0 and 1 own the samples of tiles 0 and 1
         const uint32_t q = paired ? base + 2u * uint32_t(lane & 15) + uint32_t(g) : base + uint32_t(16 * g + (lane & 15));
         const bool own = g < 2 && q < n;
         const float p = sigmoid_fast(g == 0 ? z[0] : z[1]);
-        const uint32_t pix = own ? sp.queue[q] : 0u;
+        const uint32_t pix = in.pix;
         const bool fallback = own && p * (1.0f - p) > sp.threshold;
         if (own && !fallback) sp.vis[pix] = p;
         const uint64_t fm = __ballot(fallback);
@@ -384,12 +419,20 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
             fbn = 0;
         }
     }
-    if (fbn) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(sp.fallback_count, fbn);
-        b = __shfl(b, 0);
-        for (uint32_t i = uint32_t(lane); i < fbn; i += 64u) sp.fallback_queue[b + i] = fb[i];
+    // The waves' last partial buffers go out with one atomic per block: all waves end at about
+    // the same time, and one returning atomic per wave on the single queue counter serializes
+    // (about 4k of them took ~16 us with a few thousand fallback queries).
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane == 0) s_fbn[w] = fbn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = s_fbn[0] + s_fbn[1] + s_fbn[2] + s_fbn[3];
+        s_fbbase = total ? atomicAdd(sp.fallback_count, total) : 0u;
     }
+    __syncthreads();
+    uint32_t b = s_fbbase;
+    for (uint32_t k = 0; k < w; ++k) b += s_fbn[k];
+    for (uint32_t i = uint32_t(lane); i < fbn; i += 64u) sp.fallback_queue[b + i] = fb[i];
 }
 
 // Forward + backward of the batch; each block accumulates its share of the gradient and
@@ -633,13 +676,16 @@ hipError_t launch_mlp_shadow_gate(const MlpDev& m, const MlpShadowParams& sp, ui
     int dev = 0, cus = 0, per_cu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(mlp_detail::k_shadow_gate), 256,
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(mlp_detail::k_shadow_gate<true>), 256,
                                                  0);
     per_cu = std::max(1, std::min(per_cu, 4));
     const uint32_t groups = (max_count + 31u) / 32u;
     const uint32_t blocks =
         std::max(1u, std::min<uint32_t>((groups + 3u) / 4u, uint32_t(std::max(cus, 1) * per_cu)));
-    hipLaunchKernelGGL(mlp_detail::k_shadow_gate, dim3(blocks), dim3(256), 0, st, m, sp);
+    if (sp.paired && sp.dirs)
+        hipLaunchKernelGGL(mlp_detail::k_shadow_gate<true>, dim3(blocks), dim3(256), 0, st, m, sp);
+    else
+        hipLaunchKernelGGL(mlp_detail::k_shadow_gate<false>, dim3(blocks), dim3(256), 0, st, m, sp);
     return hipGetLastError();
 }
 
