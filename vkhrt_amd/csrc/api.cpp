@@ -57,12 +57,14 @@ struct vkhrt_context {
     uint32_t* queue = nullptr;
     uint32_t* counters = nullptr;
     size_t cap_pixels = 0;
+    size_t cap_work = 0;  // work items the live mask covers
     uint64_t last_pixels = 0;
     bool last_shadows = false;
     hipStream_t last_stream = nullptr;
     vk::HitRec* hitrec = nullptr;  // per pixel (deferred-polish render)
     float* vis = nullptr;          // per pixel
     uint32_t* live = nullptr;      // per pixel: primary rays that enter an instance box
+    uint64_t* live_mask = nullptr; // the same as one bit per pixel in the camera's work order
     // deferred FP64 polish
     uint4* defer = nullptr;
     vk::HitRec* cand_hits = nullptr;
@@ -334,26 +336,34 @@ void root_bounds(const vk::WideNode& N, float lo[3], float hi[3]) {
     }
 }
 
-int ensure_render_scratch(vkhrt_context* c, size_t npix) {
-    if (npix <= c->cap_pixels) return VKHRT_SUCCESS;
+// npix: pixels times light samples; nwork: work items of the frame's camera in any work order.
+int ensure_render_scratch(vkhrt_context* c, size_t npix, size_t nwork) {
+    if (npix <= c->cap_pixels && nwork <= c->cap_work) return VKHRT_SUCCESS;
+    npix = std::max(npix, c->cap_pixels);
+    nwork = std::max(nwork, c->cap_work);
     hipFree(c->hitbuf);
     hipFree(c->queue);
     hipFree(c->hitrec);
     hipFree(c->vis);
     hipFree(c->live);
+    hipFree(c->live_mask);
     c->live = nullptr;
+    c->live_mask = nullptr;
     c->hitbuf = nullptr;
     c->queue = nullptr;
     c->hitrec = nullptr;
     c->vis = nullptr;
     c->cap_pixels = 0;
+    c->cap_work = 0;
     hipError_t e = hipMalloc(&c->hitbuf, npix * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(&c->queue, npix * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&c->hitrec, npix * sizeof(vk::HitRec));
     if (e == hipSuccess) e = hipMalloc(&c->vis, npix * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&c->live, npix * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->live_mask, (nwork + 63) / 64 * sizeof(uint64_t));
     if (e != hipSuccess) return hip_fail(c, e, "render scratch");
     c->cap_pixels = npix;
+    c->cap_work = nwork;
     return VKHRT_SUCCESS;
 }
 
@@ -548,6 +558,7 @@ void vkhrt_context_destroy(vkhrt_context* c) {
     hipFree(c->hitrec);
     hipFree(c->vis);
     hipFree(c->live);
+    hipFree(c->live_mask);
     hipFree(c->defer);
     hipFree(c->cand_hits);
     hipFree(c->cand_keys);
@@ -1232,7 +1243,9 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
             return fail(c, VKHRT_ERROR_UNSUPPORTED, "area lights need the deferred render pipeline (persistent option on)");
         if (npix * S > (size_t(1) << 31)) return fail(c, VKHRT_ERROR_INVALID_ARGUMENT, "too many shadow rays");
     }
-    int rc = ensure_render_scratch(c, npix * S);
+    // the largest work count of the camera's orders: whole 64 x 64 tiles
+    const size_t nwork = size_t((d->width + 63u) / 64u) * 64u * size_t((d->height + 63u) / 64u) * 64u;
+    int rc = ensure_render_scratch(c, npix * S, nwork);
     if (rc != VKHRT_SUCCESS) return rc;
     hipStream_t st = pick(c, stream);
     vk::RenderParams rp{};
@@ -1274,6 +1287,7 @@ int vkhrt_render(vkhrt_context* c, const vkhrt_tlas* t, const vkhrt_camera* cam,
     rp.hitrec = c->hitrec;
     rp.vis = c->vis;
     rp.live = c->live;
+    rp.live_mask = c->live_mask;
     if (d->flags & VKHRT_RENDER_FP64_POLISH) {
         if (!t->polish_ok)
             return fail(c, VKHRT_ERROR_UNSUPPORTED,

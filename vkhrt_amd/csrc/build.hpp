@@ -152,7 +152,8 @@ struct RenderParams {
     float4* origins;           // shadow-ray origin per queue entry
     uint32_t* fallback_queue;  // pixels left to the exact shadow ray ([5] of counters = length)
     uint32_t* live;            // persistent pipeline: pixels whose primary ray enters an instance box
-                               // ([6] of counters = length); the rest are recorded as misses up front
+                               // ([6] of counters = length); the rest are misses
+    uint64_t* live_mask;       // the same as bits over the camera's work order (bit r & 63 of word r >> 6)
     AreaLight light;           // shadow queue entries are ids pixel * light.samples + sample
     EnvMap env;                // miss colour
     float4* dirs;              // neural form with an area light: shadow-ray direction per queue entry

@@ -1640,34 +1640,84 @@ __device__ __forceinline__ uint32_t block_append(bool flag, uint32_t* counter, u
     return slot;
 }
 
-// Persistent render, pass 0: primary rays that miss every instance box are recorded as misses
-// here; the others are appended (in work order within each wavefront) to the live queue, so
-// the persistent lanes only take rays that enter the scene.
-__global__ __launch_bounds__(kAppendBlock) void k_render_live(RenderParams rp) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t x = 0, y = 0;
-    const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
-    const uint32_t pix = y * rp.cam.width + x;
-    bool live = false;
-    if (valid) {
-        f3 o, d;
-        primary_ray(rp.cam, x, y, o, d);
-        for (uint32_t i = 0; i < rp.tp.ninst && !live; ++i) {
-            const InstanceDev& I = rp.tp.inst[i];
-            live = I.blas.nprims != 0 && ray_box(o, d, I.lo, I.hi, 0.001f, 10000.0f);
-        }
-        if (!live) {
-            HitRec h;
-            h.t = 0.0f;
-            h.prim_id = h.inst_id = 0xFFFFFFFFu;
-            h.u = h.v = 0.0f;
-            h.n[0] = h.n[1] = h.n[2] = 0.0f;
-            rp.hitrec[pix] = h;
-            if (rp.tp.keys) rp.tp.keys[pix] = (static_cast<unsigned long long>(__float_as_uint(10000.0f)) << 32) | 0xFFFFFFFFull;
-        }
+// Ordered append of kAppendItems work items per thread with one atomic per block: item i of
+// thread t is work index blockIdx.x * kAppendBlock * kAppendItems + i * kAppendBlock + t, and
+// the slots follow that work order. Every kernel of a frame that appends this way ends with all
+// its blocks updating the one queue counter at about the same time; device-wide atomics on one
+// address serialize, so fewer, larger blocks shorten the kernel. Called by all threads of the
+// block (kAppendBlock threads).
+constexpr int kAppendItems = 4;
+__device__ __forceinline__ void block_append_items(const bool (&flag)[kAppendItems], uint32_t* counter, uint32_t m,
+                                                   uint32_t (&slot)[kAppendItems]) {
+    constexpr uint32_t NW = kAppendBlock / 64;
+    static_assert(kAppendItems * NW <= 64, "one wavefront scans the per-wave counts");
+    __shared__ uint32_t s_cnt[kAppendItems * NW + 1];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t bal[kAppendItems];
+#pragma unroll
+    for (int i = 0; i < kAppendItems; ++i) {
+        bal[i] = __ballot(flag[i]);
+        if (lane == 0) s_cnt[i * NW + wave] = uint32_t(__popcll(bal[i]));
     }
-    const uint32_t slot = block_append(live, &rp.counters[6]);
-    if (live) rp.live[slot] = pix;
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive scan of the counts in work order (item-major, then wavefront)
+        const uint32_t c = lane < kAppendItems * NW ? s_cnt[lane] : 0u;
+        uint32_t v = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(v, off);
+            if (lane >= uint32_t(off)) v += u;
+        }
+        const uint32_t sum = __shfl(v, 63);
+        uint32_t base = 0;
+        if (lane == 0 && sum) base = atomicAdd(counter, sum * m);
+        base = __shfl(base, 0);
+        if (lane < kAppendItems * NW) s_cnt[lane] = v - c;
+        if (lane == 0) s_cnt[kAppendItems * NW] = base;
+    }
+    __syncthreads();
+    const uint32_t base = s_cnt[kAppendItems * NW];
+#pragma unroll
+    for (int i = 0; i < kAppendItems; ++i)
+        slot[i] = base + (s_cnt[i * NW + wave] + uint32_t(__popcll(bal[i] & below))) * m;
+    __syncthreads();
+}
+
+// Persistent render, pass 0: the primary rays that enter an instance box are appended (in
+// work order) to the live queue, so the persistent lanes only take rays that enter the scene,
+// and marked in the live mask (bit r & 63 of word r >> 6 for work index r). The others are
+// misses: classify reads the mask and writes their pixels without a hit record.
+__global__ __launch_bounds__(kAppendBlock) void k_render_live(RenderParams rp) {
+    bool live[kAppendItems];
+    uint32_t pixs[kAppendItems];
+#pragma unroll
+    for (int k = 0; k < kAppendItems; ++k) {
+        const uint32_t r = (blockIdx.x * kAppendItems + k) * kAppendBlock + threadIdx.x;
+        uint32_t x = 0, y = 0;
+        const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
+        const uint32_t pix = y * rp.cam.width + x;
+        bool lv = false;
+        if (valid) {
+            f3 o, d;
+            primary_ray(rp.cam, x, y, o, d);
+            for (uint32_t i = 0; i < rp.tp.ninst && !lv; ++i) {
+                const InstanceDev& I = rp.tp.inst[i];
+                lv = I.blas.nprims != 0 && ray_box(o, d, I.lo, I.hi, 0.001f, 10000.0f);
+            }
+        }
+        // a wavefront's lanes hold 64 consecutive work indexes, the first a multiple of 64
+        const uint64_t bits = __ballot(lv);
+        if ((threadIdx.x & 63u) == 0u && r < rp.cam.work) rp.live_mask[r >> 6] = bits;
+        live[k] = lv;
+        pixs[k] = pix;
+    }
+    uint32_t slot[kAppendItems];
+    block_append_items(live, &rp.counters[6], 1u, slot);
+#pragma unroll
+    for (int k = 0; k < kAppendItems; ++k)
+        if (live[k]) rp.live[slot[k]] = pixs[k];
 }
 
 template <class G, int STACK, int IM>
@@ -1695,54 +1745,78 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_primary_defer(RenderPara
 // the rest for shadow rays.
 // Walks the pixels in the primary pass's work order, so the shadow queue keeps its coherence.
 __global__ __launch_bounds__(kAppendBlock) void k_render_classify(RenderParams rp) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     // the primary pass's deferred candidates are merged: the shadow pass starts a new list
-    if (r == 0 && rp.tp.defer) *rp.tp.defer_count = 0u;
-    uint32_t x = 0, y = 0;
-    const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
-    const uint32_t pix = y * rp.cam.width + x;
-    bool hit = false;
-    float ht = 0.0f;
-    if (valid) {
-        const HitRec h = rp.hitrec[pix];
-        hit = h.prim_id != 0xFFFFFFFFu;
-        ht = h.t;
-        if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
-        const f3 n = f3{h.n[0], h.n[1], h.n[2]};
-        if (!hit) {
-            f3 c{0.0f, 0.0f, 0.0f};
-            if (rp.env.texels) {
-                f3 o, d;
-                primary_ray(rp.cam, x, y, o, d);
-                c = env_color(rp.env, d);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && rp.tp.defer) *rp.tp.defer_count = 0u;
+    bool hits[kAppendItems];
+    uint32_t xs[kAppendItems], ys[kAppendItems];
+    float hts[kAppendItems];
+#pragma unroll
+    for (int k = 0; k < kAppendItems; ++k) {
+        const uint32_t r = (blockIdx.x * kAppendItems + k) * kAppendBlock + threadIdx.x;
+        uint32_t x = 0, y = 0;
+        const bool valid = r < rp.cam.work && work_pixel(rp.cam, r, x, y);
+        const uint32_t pix = y * rp.cam.width + x;
+        bool hit = false;
+        float ht = 0.0f;
+        // the same word for the whole wavefront (64 consecutive work indexes)
+        const uint64_t mask = r < rp.cam.work ? rp.live_mask[r >> 6] : 0ull;
+        if (valid) {
+            HitRec h;
+            if ((mask >> (r & 63u)) & 1ull) {
+                h = rp.hitrec[pix];
+            } else {  // the ray misses every instance box
+                h.t = 0.0f;
+                h.prim_id = h.inst_id = 0xFFFFFFFFu;
+                h.u = h.v = 0.0f;
+                h.n[0] = h.n[1] = h.n[2] = 0.0f;
             }
-            store_px(rp, pix, c);
-        } else if (rp.debug_palette) {
-            store_px(rp, pix, debug_color(h.prim_id));
-            hit = false;
-        } else if (!rp.shadows) {
-            store_px(rp, pix, shade(n, 1.0f));
-            hit = false;
-        } else {
-            rp.hitbuf[pix] = make_float4(n.x, n.y, n.z, h.t);
+            hit = h.prim_id != 0xFFFFFFFFu;
+            ht = h.t;
+            if (rp.prim_out) rp.prim_out[pix] = h.prim_id;
+            const f3 n = f3{h.n[0], h.n[1], h.n[2]};
+            if (!hit) {
+                f3 c{0.0f, 0.0f, 0.0f};
+                if (rp.env.texels) {
+                    f3 o, d;
+                    primary_ray(rp.cam, x, y, o, d);
+                    c = env_color(rp.env, d);
+                }
+                store_px(rp, pix, c);
+            } else if (rp.debug_palette) {
+                store_px(rp, pix, debug_color(h.prim_id));
+                hit = false;
+            } else if (!rp.shadows) {
+                store_px(rp, pix, shade(n, 1.0f));
+                hit = false;
+            } else {
+                rp.hitbuf[pix] = make_float4(n.x, n.y, n.z, h.t);
+            }
         }
+        hits[k] = hit;
+        xs[k] = x;
+        ys[k] = y;
+        hts[k] = ht;
     }
     const uint32_t S = rp.light.samples;
-    const uint32_t slot = block_append(hit, &rp.counters[0], S);
-    if (hit) {
-        for (uint32_t k = 0; k < S; ++k) rp.queue[slot + k] = pix * S + k;
+    uint32_t slots[kAppendItems];
+    block_append_items(hits, &rp.counters[0], S, slots);
+#pragma unroll
+    for (int k = 0; k < kAppendItems; ++k) {
+        if (!hits[k]) continue;
+        const uint32_t x = xs[k], y = ys[k], pix = y * rp.cam.width + x, slot = slots[k];
+        for (uint32_t j = 0; j < S; ++j) rp.queue[slot + j] = pix * S + j;
         if (rp.mlp) {
             // neural form: the MLP's inputs of each queued sample, as SrcShadow::get forms the
             // shadow ray (origin on the primary hit, direction toward the light sample)
             f3 po, pd;
             primary_ray(rp.cam, x, y, po, pd);
-            const f3 o = add(po, scl(ht, pd));
-            for (uint32_t k = 0; k < S; ++k) {
+            const f3 o = add(po, scl(hts[k], pd));
+            for (uint32_t j = 0; j < S; ++j) {
                 f3 d;
                 float tmax;
-                light_ray(rp.light, pix, k, o, d, tmax);
-                rp.origins[slot + k] = make_float4(o.x, o.y, o.z, 0.0f);
-                if (rp.dirs) rp.dirs[slot + k] = make_float4(d.x, d.y, d.z, 0.0f);
+                light_ray(rp.light, pix, j, o, d, tmax);
+                rp.origins[slot + j] = make_float4(o.x, o.y, o.z, 0.0f);
+                if (rp.dirs) rp.dirs[slot + j] = make_float4(d.x, d.y, d.z, 0.0f);
             }
         }
     }
@@ -1772,17 +1846,27 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_shadow_defer(RenderParam
 // Accurate hits of the render: the primary phantom hits (not FP64-polished) gathered in work
 // order from the live rays, so the double-word re-run fills whole wavefronts.
 __global__ __launch_bounds__(kAppendBlock) void k_accurate_gather(RenderParams rp) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    bool take = false;
-    uint32_t pix = 0;
-    if (r < rp.counters[6]) {
-        pix = rp.live[r];
-        const HitRec& h = rp.hitrec[pix];
-        take = h.prim_id != 0xFFFFFFFFu && h.v == 0.0f && h.inst_id < rp.tp.ninst &&
-               rp.tp.inst[h.inst_id].blas.mode == kPhantom;
+    bool take[kAppendItems];
+    uint32_t pixs[kAppendItems];
+    const uint32_t nlive = rp.counters[6];
+#pragma unroll
+    for (int k = 0; k < kAppendItems; ++k) {
+        const uint32_t r = (blockIdx.x * kAppendItems + k) * kAppendBlock + threadIdx.x;
+        take[k] = false;
+        pixs[k] = 0u;
+        if (r < nlive) {
+            const uint32_t pix = rp.live[r];
+            const HitRec& h = rp.hitrec[pix];
+            take[k] = h.prim_id != 0xFFFFFFFFu && h.v == 0.0f && h.inst_id < rp.tp.ninst &&
+                      rp.tp.inst[h.inst_id].blas.mode == kPhantom;
+            pixs[k] = pix;
+        }
     }
-    const uint32_t slot = block_append(take, &rp.counters[7]);
-    if (take) rp.queue[slot] = pix;
+    uint32_t slot[kAppendItems];
+    block_append_items(take, &rp.counters[7], 1u, slot);
+#pragma unroll
+    for (int k = 0; k < kAppendItems; ++k)
+        if (take[k]) rp.queue[slot[k]] = pixs[k];
 }
 
 template <class Src>
@@ -2313,7 +2397,7 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
             // thread per queued ray.
             const TraceParams& tp = rp.tp;
             const dim3 pgrid(1024);
-            hipLaunchKernelGGL(k_render_live, dim3((rp.cam.work + kAppendBlock - 1) / kAppendBlock), dim3(kAppendBlock),
+            hipLaunchKernelGGL(k_render_live, dim3((rp.cam.work + kAppendBlock * kAppendItems - 1) / (kAppendBlock * kAppendItems)), dim3(kAppendBlock),
                                0, st, rp);
             // the events bracket the traversal kernels alone (the per-kernel timings)
             if (ev) hipEventRecord(ev[0], st);
@@ -2341,12 +2425,12 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                 hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
             }
             if (rp.accurate) {
-                hipLaunchKernelGGL(k_accurate_gather, dim3((npix + kAppendBlock - 1) / kAppendBlock), dim3(kAppendBlock), 0,
+                hipLaunchKernelGGL(k_accurate_gather, dim3((npix + kAppendBlock * kAppendItems - 1) / (kAppendBlock * kAppendItems)), dim3(kAppendBlock), 0,
                                    st, rp);
                 hipLaunchKernelGGL(k_accurate_queued<SrcCamera>, grid, block, 0, st, tp, SrcCamera{rp.cam}, rp.hitrec,
                                    rp.queue, rp.counters + 7);
             }
-            hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kAppendBlock - 1) / kAppendBlock),
+            hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kAppendBlock * kAppendItems - 1) / (kAppendBlock * kAppendItems)),
                                dim3(kAppendBlock), 0, st, rp);
             bool shadow_timed = false;
             if (rp.shadows && !rp.debug_palette) {
