@@ -68,7 +68,8 @@ enum WorkCounter {
 constexpr uint32_t kWorkAnyHit = 16, kWorkWave = 32, kWorkCounters = 48;
 // lane-group shadow walker: kWorkWave + 0 rays, 1 group iterations, 2 node tests, 3 leaf records tested,
 // 4 Prhi calls, 5 iterations that ran records, 6 cycles from a ray's start to its end (summed),
-// 7 the largest of those
+// 7 the largest of those; over the wavefronts that walked: 8 cycles in Prhi rounds, 9 rounds,
+// 10 loop iterations, 11 cycles of the whole loop, 12 wavefronts
 
 struct Camera {
     float view_inv[16];

@@ -360,12 +360,15 @@ __device__ __forceinline__ void gate_features(const float4& o, const float4& d, 
 
 // kPaired: queue entries 2k, 2k + 1 are the two light samples of one hit (area light, two
 // samples per hit), so tile u of a pass holds light sample u of hits base / 2 + c.
+// One 1024-thread block per CU (16 wavefronts, four per SIMD at the kernel's registers): one
+// copy of the weight fragments in LDS per CU instead of four, a quarter of the staging reads.
+constexpr int kGateWaves = 16;
 template <bool kPaired>
-__global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
+__global__ __launch_bounds__(kGateWaves * 64) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
     __shared__ s8 lw[kFragsFwd * 64];
     __shared__ __align__(16) float lsmall[kSmall];
-    __shared__ uint32_t s_fb[4][128];  // per-wave fallback pixels, flushed 64+ at a time
-    __shared__ uint32_t s_fbn[4], s_fbbase;
+    __shared__ uint32_t s_fb[kGateWaves][128];  // per-wave fallback pixels, flushed 64+ at a time
+    __shared__ uint32_t s_fbn[kGateWaves], s_fbbase;
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint32_t n = *sp.count;
     constexpr bool paired = kPaired;
@@ -426,7 +429,8 @@ __global__ __launch_bounds__(256) void k_shadow_gate(MlpDev m, MlpShadowParams s
     if (lane == 0) s_fbn[w] = fbn;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t total = s_fbn[0] + s_fbn[1] + s_fbn[2] + s_fbn[3];
+        uint32_t total = 0;
+        for (int k = 0; k < kGateWaves; ++k) total += s_fbn[k];
         s_fbbase = total ? atomicAdd(sp.fallback_count, total) : 0u;
     }
     __syncthreads();
@@ -669,23 +673,18 @@ hipError_t launch_mlp_infer(const MlpDev& m, const float* samples, uint32_t n, f
 
 hipError_t launch_mlp_shadow_gate(const MlpDev& m, const MlpShadowParams& sp, uint32_t max_count, hipStream_t st) {
     if (max_count == 0) return hipSuccess;
-    // as many resident blocks as the registers and LDS weights allow (up to 4 per CU): the
-    // waves of a SIMD overlap one another's encoding (VALU) with their matrix products (MFMA);
-    // each wavefront still walks many 32-sample groups, so its fallback buffer reaches the
-    // queue with few atomics
-    int dev = 0, cus = 0, per_cu = 0;
+    // one resident block per CU: the four waves of a SIMD overlap one another's encoding
+    // (VALU) with their matrix products (MFMA); each wavefront walks many 32-sample groups
+    constexpr uint32_t W = mlp_detail::kGateWaves;
+    int dev = 0, cus = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(mlp_detail::k_shadow_gate<true>), 256,
-                                                 0);
-    per_cu = std::max(1, std::min(per_cu, 4));
     const uint32_t groups = (max_count + 31u) / 32u;
-    const uint32_t blocks =
-        std::max(1u, std::min<uint32_t>((groups + 3u) / 4u, uint32_t(std::max(cus, 1) * per_cu)));
+    const uint32_t blocks = std::max(1u, std::min<uint32_t>((groups + W - 1u) / W, uint32_t(std::max(cus, 1))));
     if (sp.paired && sp.dirs)
-        hipLaunchKernelGGL(mlp_detail::k_shadow_gate<true>, dim3(blocks), dim3(256), 0, st, m, sp);
+        hipLaunchKernelGGL(mlp_detail::k_shadow_gate<true>, dim3(blocks), dim3(W * 64), 0, st, m, sp);
     else
-        hipLaunchKernelGGL(mlp_detail::k_shadow_gate<false>, dim3(blocks), dim3(256), 0, st, m, sp);
+        hipLaunchKernelGGL(mlp_detail::k_shadow_gate<false>, dim3(blocks), dim3(W * 64), 0, st, m, sp);
     return hipGetLastError();
 }
 

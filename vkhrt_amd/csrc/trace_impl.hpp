@@ -1949,6 +1949,9 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
 #ifdef VKHRT_WORK_COUNTERS
     unsigned long long wc[7] = {};
     unsigned long long t_ray = 0, t_max = 0;
+    // per wavefront: cycles in Prhi rounds, rounds, loop iterations, cycles of the whole loop
+    unsigned long long c_round = 0, n_round = 0, n_iter = 0;
+    const unsigned long long t_loop = clock64();
 #endif
     // group state: the same value in every lane of the group
     uint32_t qi = blockIdx.x * NG + grp;
@@ -2025,6 +2028,9 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
             }
         }
         if (__ballot(walking) == 0) break;
+#ifdef VKHRT_WORK_COUNTERS
+        ++n_iter;
+#endif
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         // one step of every walking group: its nodes (as many as its record queue and stack can
@@ -2114,6 +2120,9 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
         // (every group with pairs waiting has an empty stack and record queue)
         const bool can_step = walking && !error && (sp > 0 || lh < lt);
         if (npairs >= kGrpRound || (npairs > 0 && __ballot(can_step) == 0)) {
+#ifdef VKHRT_WORK_COUNTERS
+            const unsigned long long t_round = clock64();
+#endif
             const uint32_t k = npairs < 64u ? npairs : 64u;
             if (lane < k) {
                 const uint32_t e = s_pairs[lane];
@@ -2145,6 +2154,10 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
             if (lane < NG) s_hit[lane] = s_done[lane] = 0u;
+#ifdef VKHRT_WORK_COUNTERS
+            c_round += clock64() - t_round;
+            ++n_round;
+#endif
         }
         // an occluded or failed walk drops its waiting pairs (the others keep their order)
         const bool drop = walking && (occluded || error);
@@ -2194,6 +2207,13 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
         for (int i = 0; i < 7; ++i)
             if (wc[i]) atomicAdd(&tp.work_counts[kWorkWave + i], wc[i]);
         atomicMax(&tp.work_counts[kWorkWave + 7], t_max);
+        if (n_iter) {
+            atomicAdd(&tp.work_counts[kWorkWave + 8], c_round);
+            atomicAdd(&tp.work_counts[kWorkWave + 9], n_round);
+            atomicAdd(&tp.work_counts[kWorkWave + 10], n_iter);
+            atomicAdd(&tp.work_counts[kWorkWave + 11], clock64() - t_loop);
+            atomicAdd(&tp.work_counts[kWorkWave + 12], 1ull);
+        }
     }
 #endif
 }

@@ -42,6 +42,16 @@ def summarize(raw, base, frames):
     }
 
 
+def _wave_loop(r, frames):
+    """Loop timing of the lane-group walker's wavefronts (those that walked a ray)."""
+    round_cyc, rounds, iters, loop_cyc, waves = (r[32 + 8 + i] / frames for i in range(5))
+    waves = max(waves, 1.0)
+    return {"waves": round(waves, 1), "iterations": round(iters / waves, 2), "rounds": round(rounds / waves, 2),
+            "loop_cycles": round(loop_cyc / waves), "cycles_per_iteration": round(loop_cyc / max(iters, 1.0)),
+            "cycles_per_round": round(round_cyc / max(rounds, 1.0)),
+            "round_share": round(round_cyc / max(loop_cyc, 1.0), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--frames", type=int, default=5)
@@ -131,6 +141,7 @@ def main():
             "per_ray": {k: round(w[k] / rays_w, 2) for k in ("steps", "node_tests", "leaf_tests", "prhi_calls",
                                                             "record_steps", "cycles")},
             "slowest_ray_cycles": int(r[32 + len(WAVE_NAMES)]),
+            "per_wave": _wave_loop(r, args.frames),
             "primary": summarize(r, 0, args.frames)}
         mlp.close()
     print(json.dumps(out), flush=True)
