@@ -358,11 +358,12 @@ __device__ __forceinline__ void gate_features(const float4& o, const float4& d, 
     x[5] = d.z;
 }
 
-// kPaired: queue entries 2k, 2k + 1 are the two light samples of one hit (area light, two
-// samples per hit), so tile u of a pass holds light sample u of hits base / 2 + c.
 // One 1024-thread block per CU (16 wavefronts, four per SIMD at the kernel's registers): one
 // copy of the weight fragments in LDS per CU instead of four, a quarter of the staging reads.
 constexpr int kGateWaves = 16;
+
+// kPaired: queue entries 2k, 2k + 1 are the two light samples of one hit (area light, two
+// samples per hit), so tile u of a pass holds light sample u of hits base / 2 + c.
 template <bool kPaired>
 __global__ __launch_bounds__(kGateWaves * 64) void k_shadow_gate(MlpDev m, MlpShadowParams sp) {
     __shared__ s8 lw[kFragsFwd * 64];
