@@ -1475,11 +1475,10 @@ struct SrcCamera {
 // in the instance's object space, against the hit's curve record; t, u and the normal are
 // replaced when the re-run converges. Misses, other modes and FP64-polished roots are kept.
 template <class Src>
-__device__ __forceinline__ void accurate_one(const TraceParams& tp, const Src& src, uint32_t i, HitRec* hits) {
-    HitRec h = hits[i];
-    if (h.prim_id == 0xFFFFFFFFu || h.v != 0.0f || h.inst_id >= tp.ninst) return;
+__device__ __forceinline__ bool accurate_refine(const TraceParams& tp, const Src& src, uint32_t i, HitRec& h) {
+    if (h.prim_id == 0xFFFFFFFFu || h.v != 0.0f || h.inst_id >= tp.ninst) return false;
     const InstanceDev& I = tp.inst[h.inst_id];
-    if (I.blas.mode != kPhantom || !I.blas.slot_of || h.prim_id >= I.blas.nprims) return;
+    if (I.blas.mode != kPhantom || !I.blas.slot_of || h.prim_id >= I.blas.nprims) return false;
     f3 o, d;
     float tmin, tmax;
     src.get(i, o, d, tmin, tmax);
@@ -1488,9 +1487,9 @@ __device__ __forceinline__ void accurate_one(const TraceParams& tp, const Src& s
         d = xform_dir(I.inv, d);
     }
     const uint32_t slot = I.blas.slot_of[h.prim_id];
-    if (slot >= I.blas.nprims) return;
+    if (slot >= I.blas.nprims) return false;
     PhantomHit ph;
-    if (!prhi_ff(o, d, I.blas.prims + size_t(slot) * 4, int(I.blas.radius_mode), I.blas.radius, ph)) return;
+    if (!prhi_ff(o, d, I.blas.prims + size_t(slot) * 4, int(I.blas.radius_mode), I.blas.radius, ph)) return false;
     f3 nn{ph.ncx, ph.ncy, ph.nsz};
     if (!I.identity) {
         const float* x = I.xf;
@@ -1502,7 +1501,14 @@ __device__ __forceinline__ void accurate_one(const TraceParams& tp, const Src& s
     h.n[0] = nn.x;
     h.n[1] = nn.y;
     h.n[2] = nn.z;
-    hits[i] = h;
+    return true;
+}
+// The double-word re-run of ray i's final hit (not FP64-polished, phantom): t, u and the
+// normal are replaced; records it does not apply to are left as they are.
+template <class Src>
+__device__ __forceinline__ void accurate_one(const TraceParams& tp, const Src& src, uint32_t i, HitRec* hits) {
+    HitRec h = hits[i];
+    if (accurate_refine(tp, src, i, h)) hits[i] = h;
 }
 template <class Src>
 __global__ __launch_bounds__(kTraceBlock) void k_accurate_hits(TraceParams tp, Src src, HitRec* __restrict__ hits,
@@ -1585,6 +1591,43 @@ __global__ void k_polish_closest(TraceParams tp, Src src, HitRec* __restrict__ c
     }
 }
 
+// Accurate render with the FP64 polish: the closest-hit merge of the deferred candidates
+// (k_polish_closest's work) and the double-word re-run of the queued primary hits in one
+// launch, so the candidates' long FP64 chains run beside the re-run instead of before it. The
+// first `polish_blocks` blocks take the candidates, the others the queue. The queue was
+// gathered before the merge, from the FP32 hits: a queued hit that a polished candidate later
+// replaces is re-run in vain (k_polish_fixup_accurate overwrites it), and a candidate that
+// wins unpolished (its FP64 side failed and the other side converged in FP32) is re-run there,
+// so every final unpolished phantom hit gets the re-run, as when the merge comes first.
+// Held to four waves per SIMD (128 registers, the re-run's occupancy on its own): without the
+// bound the FP64 candidate path and the double-word re-run share one allocation of 162.
+template <class Src>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void k_accurate_polish(TraceParams tp, Src src, HitRec* __restrict__ hits, const uint32_t* __restrict__ queue,
+                       const uint32_t* __restrict__ count, HitRec* __restrict__ cand_hits,
+                       unsigned long long* __restrict__ cand_keys, uint32_t polish_blocks) {
+    if (blockIdx.x < polish_blocks) {
+        const uint32_t n = min(*tp.defer_count, tp.defer_cap);
+        for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < n; ci += polish_blocks * blockDim.x) {
+            const uint4 c = tp.defer[ci];
+            f3 o, d;
+            float tmin, tmax;
+            src.get(c.x, o, d, tmin, tmax);
+            HitRec h;
+            unsigned long long key = ~0ull;
+            if (eval_candidate(tp, c, o, d, tmin, tmax, h, key)) {
+                atomicMin(&tp.keys[c.x], key);
+                cand_hits[ci] = h;
+            }
+            cand_keys[ci] = key;
+        }
+        return;
+    }
+    const uint32_t qi = (blockIdx.x - polish_blocks) * blockDim.x + threadIdx.x;
+    if (qi >= *count) return;
+    accurate_one(tp, src, queue[qi], hits);
+}
+
 // The candidate that won the merge writes its hit record.
 __global__ void k_polish_fixup(TraceParams tp, const HitRec* __restrict__ cand_hits,
                                const unsigned long long* __restrict__ cand_keys, HitRec* __restrict__ out) {
@@ -1593,6 +1636,22 @@ __global__ void k_polish_fixup(TraceParams tp, const HitRec* __restrict__ cand_h
         const unsigned long long k = cand_keys[ci];
         const uint4 c = tp.defer[ci];
         if (k != ~0ull && k == tp.keys[c.x]) out[c.x] = cand_hits[ci];
+    }
+}
+
+// k_polish_fixup after k_accurate_polish: a winner that the FP64 continuation did not polish
+// gets the double-word re-run the queued hits had.
+template <class Src>
+__global__ void k_polish_fixup_accurate(TraceParams tp, Src src, const HitRec* __restrict__ cand_hits,
+                                        const unsigned long long* __restrict__ cand_keys, HitRec* __restrict__ out) {
+    const uint32_t n = min(*tp.defer_count, tp.defer_cap);
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < n; ci += gridDim.x * blockDim.x) {
+        const unsigned long long k = cand_keys[ci];
+        const uint4 c = tp.defer[ci];
+        if (k == ~0ull || k != tp.keys[c.x]) continue;
+        HitRec h = cand_hits[ci];
+        accurate_refine(tp, src, c.x, h);  // unpolished winners only (h.v == 0)
+        out[c.x] = h;
     }
 }
 
@@ -2439,16 +2498,28 @@ hipError_t launch_render_variant(const RenderParams& rp, hipStream_t st, hipEven
                                    persistent_grid(k_render_primary_defer<G, STACK, kInstMany>, tp.blocks_per_cu), block,
                                    0, st, rp);
             if (ev) hipEventRecord(ev[1], st);
-            if (tp.defer) {
-                hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam},
-                                   tp.cand_hits, tp.cand_keys);
-                hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
-            }
-            if (rp.accurate) {
-                hipLaunchKernelGGL(k_accurate_gather, dim3((npix + kAppendBlock * kAppendItems - 1) / (kAppendBlock * kAppendItems)), dim3(kAppendBlock), 0,
-                                   st, rp);
-                hipLaunchKernelGGL(k_accurate_queued<SrcCamera>, grid, block, 0, st, tp, SrcCamera{rp.cam}, rp.hitrec,
-                                   rp.queue, rp.counters + 7);
+            const dim3 agrid((npix + kAppendBlock * kAppendItems - 1) / (kAppendBlock * kAppendItems));
+            if (rp.accurate && tp.defer) {
+                // the FP64 merge beside the double-word re-run (one block per CU for the
+                // candidates), then the winners' records
+                constexpr uint32_t kPolishBlocks = 256;
+                hipLaunchKernelGGL(k_accurate_gather, agrid, dim3(kAppendBlock), 0, st, rp);
+                hipLaunchKernelGGL(k_accurate_polish<SrcCamera>, dim3(grid.x + kPolishBlocks), block, 0, st, tp,
+                                   SrcCamera{rp.cam}, rp.hitrec, rp.queue, rp.counters + 7, tp.cand_hits, tp.cand_keys,
+                                   kPolishBlocks);
+                hipLaunchKernelGGL(k_polish_fixup_accurate<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam},
+                                   tp.cand_hits, tp.cand_keys, rp.hitrec);
+            } else {
+                if (tp.defer) {
+                    hipLaunchKernelGGL(k_polish_closest<SrcCamera>, pgrid, block, 0, st, tp, SrcCamera{rp.cam},
+                                       tp.cand_hits, tp.cand_keys);
+                    hipLaunchKernelGGL(k_polish_fixup, pgrid, block, 0, st, tp, tp.cand_hits, tp.cand_keys, rp.hitrec);
+                }
+                if (rp.accurate) {
+                    hipLaunchKernelGGL(k_accurate_gather, agrid, dim3(kAppendBlock), 0, st, rp);
+                    hipLaunchKernelGGL(k_accurate_queued<SrcCamera>, grid, block, 0, st, tp, SrcCamera{rp.cam},
+                                       rp.hitrec, rp.queue, rp.counters + 7);
+                }
             }
             hipLaunchKernelGGL(k_render_classify, dim3((rp.cam.work + kAppendBlock * kAppendItems - 1) / (kAppendBlock * kAppendItems)),
                                dim3(kAppendBlock), 0, st, rp);
