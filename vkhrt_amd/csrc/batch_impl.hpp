@@ -263,12 +263,14 @@ __global__ void k_batch_polish_shadow(TraceParams tp, SrcBatchShadow src, float4
     const uint32_t n = min(*tp.defer_count, tp.defer_cap);
     for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < n; ci += gridDim.x * blockDim.x) {
         const uint4 c = tp.defer[ci];
+        const uint32_t p = src.sq[c.x];
+        if (nee[p].w == 0.0f) continue;  // already occluded by an FP32 hit of the traversal
         f3 o, d;
         float tmin, tmax;
         src.get(c.x, o, d, tmin, tmax);
         HitRec h;
         unsigned long long key;
-        if (eval_candidate(tp, c, o, d, tmin, tmax, h, key)) nee[src.sq[c.x]].w = 0.0f;
+        if (eval_candidate(tp, c, o, d, tmin, tmax, h, key)) nee[p].w = 0.0f;
     }
 }
 

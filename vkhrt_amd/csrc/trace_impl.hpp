@@ -1655,17 +1655,20 @@ __global__ void k_polish_fixup_accurate(TraceParams tp, Src src, const HitRec* _
     }
 }
 
-// Shadow rays: any polished hit in range occludes.
+// Shadow rays: any polished hit in range occludes. A ray the traversal already found occluded
+// (an FP32 hit after the candidate was deferred) needs no FP64 continuation.
 __global__ void k_polish_shadow(TraceParams tp, SrcShadow src, float* __restrict__ vis) {
     const uint32_t n = min(*tp.defer_count, tp.defer_cap);
     for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < n; ci += gridDim.x * blockDim.x) {
         const uint4 c = tp.defer[ci];
+        const uint32_t id = src.queue[c.x];
+        if (vis[id] == 0.0f) continue;
         f3 o, d;
         float tmin, tmax;
         src.get(c.x, o, d, tmin, tmax);
         HitRec h;
         unsigned long long key;
-        if (eval_candidate(tp, c, o, d, tmin, tmax, h, key)) vis[src.queue[c.x]] = 0.0f;
+        if (eval_candidate(tp, c, o, d, tmin, tmax, h, key)) vis[id] = 0.0f;
     }
 }
 
