@@ -1972,6 +1972,34 @@ constexpr uint32_t kGrpNodeRecords = 24;  // leaf records one node can append
 constexpr uint32_t kGrpGrid = 8192;       // one-wave blocks of the launch
 constexpr uint32_t kGrpRound = 64;        // waiting pairs that start a Prhi round (a full wavefront)
 constexpr uint32_t kGrpSlotBits = 26;     // pair = group << 26 | leaf slot (phantom BLASes stay below 2^26 records)
+// Value of lane (l - k) for lane l, for the prefix sums over a lane group of LPR lanes (the
+// caller uses it only where l - k lies in l's group). A group of 8 or 16 lanes lies inside one
+// 16-lane row, so its shifts go through the DPP row shift (no LDS round trip; lanes whose
+// source is outside the row read 0). A row shift does not cross rows, so groups of 32 or 64
+// lanes, whose lanes take sources from the row below, use the LDS permute for every shift.
+template <int LPR>
+__device__ __forceinline__ uint32_t group_shift_up(uint32_t v, int k) {
+    if (LPR > 16) return uint32_t(__shfl_up(int(v), k));
+    switch (k) {
+        case 1: return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xF, 0xF, true));  // row_shr:1
+        case 2: return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xF, 0xF, true));  // row_shr:2
+        case 4: return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xF, 0xF, true));  // row_shr:4
+        case 8: return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xF, 0xF, true));  // row_shr:8
+        default: return uint32_t(__shfl_up(int(v), k));
+    }
+}
+// The value of the last lane of lane group `grp` (LPR lanes), in every lane: one scalar
+// read per group and a select, no LDS round trip.
+template <int LPR>
+__device__ __forceinline__ uint32_t group_last(uint32_t v, uint32_t grp) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int g = 0; g < 64 / LPR; ++g) {
+        const uint32_t t = uint32_t(__builtin_amdgcn_readlane(int(v), g * LPR + LPR - 1));
+        r = grp == uint32_t(g) ? t : r;
+    }
+    return r;
+}
 template <int LPR>
 struct GroupShape {
     static constexpr uint32_t kGroups = 64u / LPR;  // rays per wavefront
@@ -2154,13 +2182,13 @@ __global__ __launch_bounds__(64) void k_render_shadow_groups(RenderParams rp, co
         uint32_t cpre = nc, ppre = np;
 #pragma unroll
         for (int k = 1; k < LPR; k <<= 1) {
-            const uint32_t x = __shfl_up(cpre, k), y = __shfl_up(ppre, k);
+            const uint32_t x = group_shift_up<LPR>(cpre, k), y = group_shift_up<LPR>(ppre, k);
             if (int(sub) >= k) {
                 cpre += x;
                 ppre += y;
             }
         }
-        const uint32_t ctot = __shfl(cpre, last), ptot = __shfl(ppre, last);
+        const uint32_t ctot = group_last<LPR>(cpre, grp), ptot = group_last<LPR>(ppre, grp);
         if (walking) {
             pend += uint32_t(__popcll(pb & gmask));
             error = (eb & gmask) != 0 || sp + ctot > S::kStack || lt + ptot > S::kLeaves;
