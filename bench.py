@@ -784,8 +784,14 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
     flags = V.RENDER_SHADOWS | V.RENDER_RGBA8 | (V.RENDER_FP64_POLISH if args.polish else 0)
     image = torch.empty(W * H, dtype=torch.int32, device="cuda")
 
+    # each form runs with the traversal options the autotuner picked on its own frame
+    tunings = {}
+
     def timed(**kw):
         nflags = flags | (V.RENDER_NEURAL_SHADOWS if kw else 0)
+        tuning = tunings.get("neural" if kw else "exact")
+        if tuning:
+            ctx.apply_tuning(tuning)
         for _ in range(args.warmup):
             ctx.render(tlas, cam, W, H, image.data_ptr(), flags=nflags, stream=sp, area_light=light, **kw)
         torch.cuda.synchronize()
@@ -798,12 +804,14 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
         ctx.synchronize(sp)
         return e0.elapsed_time(e1) / args.steps, ctx.render_stats()
 
-    # the autotuner picks the traversal options on this workload's exact frame (both frames use them)
-    area_tuning = None
+    # the autotuner picks the traversal options on this workload's exact frame, and later on its
+    # neural frame; each form is timed with its own
+    tune_keys = ("stack_kind", "blocks_per_cu", "refill_min", "ray_order", "xcd_chunk", "steps_per_iteration", "trials")
+    area_tuning = neural_tuning = None
     if args.autotune:
-        t = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags, repeats=5, stream=sp, area_light=light)
-        area_tuning = {k: t[k] for k in ("stack_kind", "blocks_per_cu", "refill_min", "ray_order", "xcd_chunk",
-                                         "steps_per_iteration", "trials")}
+        t = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags, repeats=9, stream=sp, area_light=light)
+        tunings["exact"] = t
+        area_tuning = {k: t[k] for k in tune_keys}
         area_tuning["tuned_ms"] = round(t["best_ms"], 4)
         log(f"[bench] area light autotuned: {area_tuning}")
     exact_ms, (prim, queries, _) = timed()
@@ -850,6 +858,14 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
     log(f"[bench] area light: trained on 2^24 surface rays ({collect_s:.2f} s to trace) for {epochs} epochs in "
         f"{train_s:.2f} s, loss {loss:.4f}; held-out {held}")
 
+    if args.autotune:
+        t = ctx.autotune_render(tlas, cam, W, H, image.data_ptr(), flags=flags | V.RENDER_NEURAL_SHADOWS, repeats=9,
+                                stream=sp, area_light=light, shadow_mlp=mlp, variance_threshold=NEURAL_THRESHOLDS[0])
+        tunings["neural"] = t
+        neural_tuning = {k: t[k] for k in tune_keys}
+        neural_tuning["tuned_ms"] = round(t["best_ms"], 4)
+        neural_tuning["threshold"] = NEURAL_THRESHOLDS[0]
+        log(f"[bench] area light neural autotuned: {neural_tuning}")
     fimg = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
     fexact = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
     ctx.render(tlas, cam, W, H, fexact.data_ptr(), flags=flags & ~V.RENDER_RGBA8, stream=sp, area_light=light)
@@ -891,7 +907,7 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
     blas.close()
     return {
         "workload": "phantom_500k_strands_x32_segments_1920x1080_area_light_2_samples",
-        "segments": n, "light_samples": 2, "autotune": area_tuning,
+        "segments": n, "light_samples": 2, "autotune": area_tuning, "autotune_neural": neural_tuning,
         "exact": {"ms_per_step": round(exact_ms, 4), "primary_rays": prim, "shadow_rays": queries,
                   "mrays_per_s": round((prim + queries) / (exact_ms * 1e-3) / 1e6, 1),
                   "timing": "median of every exact-frame timing of the block"},

@@ -413,13 +413,23 @@ class Context:
 
     # autotuner: times the given workload under candidate settings, keeps the fastest
     def autotune_render(self, tlas, camera, width, height, image_ptr, flags=RENDER_SHADOWS, repeats=3, stream=None,
-                        area_light=None):
-        d = RenderDesc(width, height, flags, image_ptr, None, None, None, 0.0,
+                        area_light=None, shadow_mlp=None, variance_threshold=0.0):
+        """Times the render (the same arguments as render(), neural form included) under candidate
+        traversal settings and leaves the context on the fastest."""
+        d = RenderDesc(width, height, flags, image_ptr, None, None,
+                       shadow_mlp.h if shadow_mlp is not None else None, variance_threshold,
                        ctypes.pointer(area_light) if area_light is not None else None)
         t = Tuning()
         self.check(lib().vkhrt_autotune_render(self.h, tlas.h, ctypes.byref(camera), ctypes.byref(d), repeats,
                                                ctypes.byref(t), stream), "autotune_render")
         return t.as_dict()
+
+    def apply_tuning(self, t):
+        """Sets the traversal options of a tuning (autotune_render's result) on the context."""
+        for opt, key in ((OPTION_STACK_KIND, "stack_kind"), (OPTION_BLOCKS_PER_CU, "blocks_per_cu"),
+                         (OPTION_REFILL_MIN, "refill_min"), (OPTION_RAY_ORDER, "ray_order"),
+                         (OPTION_XCD_CHUNK, "xcd_chunk"), (OPTION_STEPS_PER_ITERATION, "steps_per_iteration")):
+            self.set_option(opt, int(t[key]))
 
     def autotune_batch(self, tlas, views, bounces=4, seed=0, environment=(0.0, 0.0, 0.0), flags=0,
                        material_albedo=None, repeats=3, stream=None):
