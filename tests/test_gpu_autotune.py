@@ -120,3 +120,51 @@ def test_autotune_batch_keeps_the_images(fresh_ctx):
     torch.cuda.synchronize()
     after = [im.cpu().numpy() for im in imgs]
     assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(before, after))
+
+
+def test_autotune_neural_render_keeps_the_image(fresh_ctx):
+    """The neural form (area light, two samples, exact fallback above the threshold) tunes like
+    the exact one, and switching between the two tunings with apply_tuning renders both forms
+    bit for bit as before."""
+    import gpu_util as G
+    import vkhrt_amd as V
+    ctx = fresh_ctx
+    curves, blas, tlas = _scene(ctx)
+    cam = V.camera_look_at((0.0, 20.0, 45.0), (0.0, 8.0, 0.0), aspect=1.0)
+    light = V.area_light((0.0, 60.0, 0.0), (20.0, 0.0, 0.0), (0.0, 0.0, 20.0), samples=2, seed=3)
+    mlp = V.ShadowMLP(ctx, [-30.0, -10.0, -30.0], [30.0, 40.0, 30.0], seed=5)  # untrained: p near 0.5
+    W = H = 256
+    flags = V.RENDER_SHADOWS | V.RENDER_FP64_POLISH
+
+    def frame(neural):
+        img = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+        kw = dict(shadow_mlp=mlp, variance_threshold=0.2499) if neural else {}
+        ctx.render(tlas, cam, W, H, img.data_ptr(), flags=flags | (V.RENDER_NEURAL_SHADOWS if neural else 0),
+                   stream=G.stream_ptr(), area_light=light, **kw)
+        torch.cuda.synchronize()
+        ctx.synchronize(G.stream_ptr())
+        return img.cpu().numpy().view(np.uint32), ctx.render_stats()
+
+    try:
+        exact0, _ = frame(False)
+        neural0, (_, q0, tr0) = frame(True)
+        print(f"[autotune neural render] {tr0}/{q0} queries traced exactly", flush=True)
+        assert tr0 > 0  # the exact fallback runs
+        img = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+        print("[autotune neural render]", flush=True)
+        te = ctx.autotune_render(tlas, cam, W, H, img.data_ptr(), flags=flags, repeats=2, stream=G.stream_ptr(),
+                                 area_light=light)
+        tn = ctx.autotune_render(tlas, cam, W, H, img.data_ptr(), flags=flags | V.RENDER_NEURAL_SHADOWS, repeats=2,
+                                 stream=G.stream_ptr(), area_light=light, shadow_mlp=mlp, variance_threshold=0.2499)
+        print(te, tn)
+        _check_ranges(te)
+        _check_ranges(tn)
+        for _ in range(2):
+            ctx.apply_tuning(tn)
+            neural1, (_, q1, tr1) = frame(True)
+            ctx.apply_tuning(te)
+            exact1, _ = frame(False)
+            assert (q1, tr1) == (q0, tr0)
+            assert np.array_equal(neural0, neural1) and np.array_equal(exact0, exact1)
+    finally:
+        mlp.close()
