@@ -8,11 +8,13 @@ hair surface (24 epochs of 65,536 samples, as the bench does) or loads its param
 file written by an earlier run, and renders `--frames` neural frames at each threshold. The
 parameters of one training can be reused by later runs (a profiler run, for one), so that those
 see the same network and the same exact-fallback rays. Prints one line per threshold with the
-frame's query and fallback counts.
+frame's query and fallback counts and the image's CRC-32 (the same network gives the same
+image under every traversal setting).
 """
 import argparse
 import os
 import sys
+import zlib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
@@ -81,7 +83,9 @@ def main():
                        area_light=light, stream=sp)
         torch.cuda.synchronize()
         prim, queries, traced = ctx.render_stats()
-        print(f"[neural_frames] threshold {thr}: {queries} queries, {traced} traced exactly", flush=True)
+        crc = zlib.crc32(image.cpu().numpy().tobytes())
+        print(f"[neural_frames] threshold {thr}: {queries} queries, {traced} traced exactly, image crc32 {crc:08x}",
+              flush=True)
     mlp.close()
     ctx.close()
 
