@@ -917,7 +917,7 @@ def neural_area_block(args, ctx, sp, stream, torch, np, V):
                    "frame_speedup_vs_exact": chosen["frame_speedup_vs_exact"],
                    "choice": "largest threshold of the sweep whose frame traces part of the queries exactly "
                              "and keeps PSNR >= 40 dB against the exact frame",
-                   "fallback_walk": {"lanes_per_ray": args.shadow_group_lanes, "queue_limit": 32768},
+                   "fallback_walk": {"lanes_per_ray": args.shadow_group_lanes, "queue_limit": 65536},
                    "threshold_sweep": sweep},
         "train": {"samples": ntrain, "epochs": epochs, "batch": 65536, "seconds": round(train_s, 2),
                   "trace_seconds": round(collect_s, 2), "final_loss": round(loss, 5),

@@ -246,7 +246,7 @@ int vkhrt_abi_version(void);
                                      runs of this many, run k by the counter of XCD k mod 8 (each
                                      XCD's L2 then serves its own image regions) */
 #define VKHRT_OPTION_SHADOW_WAVE_LIMIT 10 /* neural shadow form: a fallback queue of up to this many
-                                            rays (0..2^20, default 32768) is walked by lane groups, one
+                                            rays (0..2^20, default 65536) is walked by lane groups, one
                                             ray per group, a longer one by the persistent lanes; 0 =
                                             always the persistent lanes. Results do not depend on it */
 #define VKHRT_OPTION_SHADOW_GROUP_LANES 11 /* lanes per ray of that walk: 8, 16 (default), 32 or 64 */

@@ -249,7 +249,7 @@ def test_wave_fallback_equals_persistent_lanes(gpu_ctx, area):
                 assert tr == q and np.array_equal(a.view(np.uint32), exact.view(np.uint32))
             partial = partial or 0 < tr < q
     finally:
-        ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 32768)
+        ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, 65536)
         ctx.set_option(V.OPTION_SHADOW_GROUP_LANES, 16)
     assert partial
     for opt, v in ((V.OPTION_SHADOW_WAVE_LIMIT, -1), (V.OPTION_SHADOW_WAVE_LIMIT, (1 << 20) + 1),

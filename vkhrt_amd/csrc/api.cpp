@@ -43,7 +43,7 @@ struct vkhrt_context {
     uint32_t ray_order = 0;  // render work order: 0 row-major, 1 8x8 tiles, 2 64x64 tiles of 8x8 tiles
     uint32_t xcd_chunk = 4096;  // render traversal: rays handed out per XCD in runs of this many (0: one counter)
     uint32_t blocks_per_cu = 0;  // persistent grids: 256-thread blocks per CU (0 = occupancy limit)
-    uint32_t shadow_wave_limit = 32768;  // neural fallback queues up to this long: lane groups per ray
+    uint32_t shadow_wave_limit = 65536;  // neural fallback queues up to this long: lane groups per ray
     uint32_t shadow_group_lanes = 16;    // lanes per fallback ray of that walk
     uint32_t steps_per_iter = 3;         // compacted phantom loop: traversal steps per bookkeeping pass
     bool profiling = false;
@@ -140,6 +140,7 @@ struct vkhrt_blas {
     float lo[3], hi[3];  // object-space bounds
     vkhrt_blas_info info{};
     uint32_t* voxels = nullptr;  // voxel BLAS: occupancy bits of the traced level
+    uint32_t* vbricks = nullptr;  // voxel BLAS: one bit per 4x4x4 brick of the traced level
     float vorigin[3] = {0, 0, 0};
     float vcell = 0.0f;
     int32_t vres[3] = {0, 0, 0};
@@ -683,6 +684,7 @@ int build_voxels(vkhrt_context* c, const vkhrt_blas_desc* d, hipStream_t st, vkh
     float4* lines = nullptr;
     bool own = false;
     uint32_t* bits = nullptr;
+    uint32_t* bricks = nullptr;
     uint32_t* scratch = nullptr;  // [0] overflow, [1] filled count, [2..7] bounds keys
     int rc = VKHRT_SUCCESS;
     hipError_t e = hipMalloc(&scratch, 8 * sizeof(uint32_t));
@@ -766,10 +768,32 @@ int build_voxels(vkhrt_context* c, const vkhrt_blas_desc* d, hipStream_t st, vkh
             rc = hip_fail(c, e, "voxel count");
             goto done;
         }
-        b->info.device_bytes = uint64_t(nwords) * 4u;
+        // Brick occupancy: two 2x downsamplings give one bit per 4x4x4 cells (brick (X, Y, Z)
+        // covers cells 4X..4X+3 per axis, clipped to the grid).
+        int32_t hres[3], bres[3];
+        for (int k = 0; k < 3; ++k) {
+            hres[k] = (res[k] + 1) / 2;
+            bres[k] = (hres[k] + 1) / 2;
+        }
+        const size_t hwords = (size_t(hres[0]) * size_t(hres[1]) * size_t(hres[2]) + 31) / 32;
+        const size_t bwords = (size_t(bres[0]) * size_t(bres[1]) * size_t(bres[2]) + 31) / 32;
+        uint32_t* half = nullptr;
+        e = hipMalloc(&half, hwords * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMalloc(&bricks, bwords * sizeof(uint32_t));
+        if (e == hipSuccess) e = vk::voxel_downsample(bits, res, half, hres, st);
+        if (e == hipSuccess) e = vk::voxel_downsample(half, hres, bricks, bres, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        hipFree(half);
+        if (e != hipSuccess) {
+            rc = hip_fail(c, e, "voxel bricks");
+            goto done;
+        }
+        b->info.device_bytes = uint64_t(nwords) * 4u + uint64_t(bwords) * 4u;
     }
     b->voxels = bits;
     bits = nullptr;
+    b->vbricks = bricks;
+    bricks = nullptr;
     b->records = filled;
     b->vcell = cell;
     std::memcpy(b->vres, res, sizeof(res));
@@ -791,6 +815,7 @@ done:
     hipStreamSynchronize(st);
     if (own) hipFree(lines);
     hipFree(bits);
+    hipFree(bricks);
     hipFree(scratch);
     return rc;
 }
@@ -999,6 +1024,7 @@ void vkhrt_blas_destroy(vkhrt_blas* b) {
     hipFree(b->bvh.aux);
     hipFree(b->normals);
     hipFree(b->voxels);
+    hipFree(b->vbricks);
     hipFree(b->slot_of);
     delete b;
 }
@@ -1028,6 +1054,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         I.blas.normals = b->normals;
         I.blas.slot_of = b->slot_of;
         I.blas.voxels = b->voxels;
+        I.blas.vbricks = b->vbricks;
         std::memcpy(I.blas.vorigin, b->vorigin, sizeof(I.blas.vorigin));
         I.blas.vcell = b->vcell;
         std::memcpy(I.blas.vres, b->vres, sizeof(I.blas.vres));

@@ -7,6 +7,8 @@
 //                 DOTS piece    : 4 x float4 (start,r0 ; end,r1 ; s ; t)      64 B
 //                 triangle      : 3 x float4 (vertex positions)               48 B
 //   voxels    : voxel BLAS only, one occupancy bit per cell (x fastest) in u32 words; no nodes
+//   vbricks   : voxel BLAS only, one bit per 4x4x4-cell brick (x fastest), set when any of its
+//               cells is filled; the walk reads no cell words inside an empty brick
 //   prim_ids  : original primitive index per leaf slot (u32)
 //   nodes     : 8-wide BVH nodes with 8-bit quantized child boxes, 80 B each
 #pragma once
@@ -47,6 +49,7 @@ struct BlasDev {
     uint32_t radius_mode;  // phantom: 0 constant radius, 1 per control point
     float radius;          // phantom constant radius
     const uint32_t* voxels;  // voxels: occupancy bits of the traced level of detail
+    const uint32_t* vbricks;  // voxels: one bit per 4x4x4 brick of cells (null: read every cell)
     float vorigin[3];        // voxels: grid origin (object space)
     float vcell;             // voxels: cell edge at the traced level
     int32_t vres[3];         // voxels: cells per axis at the traced level

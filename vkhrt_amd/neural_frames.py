@@ -27,6 +27,9 @@ def main():
     ap.add_argument("--strands", type=int, default=500000)
     ap.add_argument("--save-params", default=None, help="write the trained parameters (numpy .npy)")
     ap.add_argument("--load-params", default=None, help="read the parameters instead of training")
+    ap.add_argument("--wave-limit", type=int, default=None,
+                    help="fallback queues up to this long are walked by lane groups (VKHRT_OPTION_SHADOW_WAVE_LIMIT)")
+    ap.add_argument("--group-lanes", type=int, default=None, help="lanes per ray of that walk: 8, 16, 32 or 64")
     args = ap.parse_args()
     thresholds = args.threshold or [0.2499]
 
@@ -76,16 +79,28 @@ def main():
         print("[neural_frames] trained", flush=True)
         if args.save_params:
             np.save(args.save_params, np.array(mlp.get_params(), np.float32))
+    if args.wave_limit is not None:
+        ctx.set_option(V.OPTION_SHADOW_WAVE_LIMIT, args.wave_limit)
+    if args.group_lanes is not None:
+        ctx.set_option(V.OPTION_SHADOW_GROUP_LANES, args.group_lanes)
     flags = V.RENDER_SHADOWS | V.RENDER_RGBA8 | V.RENDER_FP64_POLISH | V.RENDER_NEURAL_SHADOWS
     for thr in thresholds:
-        for _ in range(args.frames):
+        def frame():
             ctx.render(tlas, cam, W, H, image.data_ptr(), flags=flags, shadow_mlp=mlp, variance_threshold=thr,
                        area_light=light, stream=sp)
+        frame()  # warm-up
         torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(args.frames):
+            frame()
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / max(args.frames, 1)
         prim, queries, traced = ctx.render_stats()
         crc = zlib.crc32(image.cpu().numpy().tobytes())
-        print(f"[neural_frames] threshold {thr}: {queries} queries, {traced} traced exactly, image crc32 {crc:08x}",
-              flush=True)
+        print(f"[neural_frames] threshold {thr}: {queries} queries, {traced} traced exactly, {ms:.3f} ms/frame, "
+              f"image crc32 {crc:08x}", flush=True)
     mlp.close()
     ctx.close()
 
