@@ -1,8 +1,9 @@
-// The voxel walk's brick jump (voxel_walk in vkhrt_amd/csrc/trace_impl.hpp) against the plain
+// The voxel walk's block moves (voxel_walk in vkhrt_amd/csrc/trace_impl.hpp) against the plain
 // cell-by-cell 3D DDA, on the CPU with the same float arithmetic: random grids with sparse and
-// clustered cells, 4x4x4 bricks, and rays that are axis-parallel, diagonal (equal face t on
-// several axes), start on cell boundaries or end early. Prints the mismatch count. Argument
-// "flip" plants a wrong tie rule in the jump (the higher axis first) to show that the rays find it.
+// clustered cells, empty 16x16x16 and 4x4x4 blocks left in one move, and rays that are
+// axis-parallel, diagonal (equal face t on several axes), start on cell boundaries or end early.
+// Prints the mismatch count. Argument "flip" plants a wrong tie rule in the move (the higher
+// axis first) to show that the rays find it.
 #include <cstdio>
 #include <cstdint>
 #include <cmath>
@@ -11,7 +12,7 @@
 #include <algorithm>
 #include <cstring>
 struct Res { float t; uint32_t idx; int ax; bool hit; };
-struct Grid { int R[3]; float O[3]; float c; std::vector<uint8_t> occ; std::vector<uint8_t> brick; int bx, by, bz; };
+struct Grid { int R[3]; float O[3]; float c; std::vector<uint8_t> occ, brick, super_; int bx, by, bz, sx, sy, sz; };
 static bool filled(const Grid& g, int x, int y, int z) { return g.occ[x + y * g.R[0] + (size_t)z * g.R[0] * g.R[1]]; }
 static bool g_flip = false;
 template <bool JUMP>
@@ -38,18 +39,21 @@ Res walk(const Grid& g, const float oo[3], const float dd[3], float tmin, float 
     }
     float te = t0;
     for (int it = 0, lim = R[0] + R[1] + R[2] + 3; it < lim; ++it) {
-        const int bi = (i[0] >> 2) + (i[1] >> 2) * g.bx + (i[2] >> 2) * g.bx * g.by;
-        const bool bfull = !JUMP || g.brick[bi];
-        if (bfull && filled(g, i[0], i[1], i[2])) {
+        int S = 1;  // size of the empty block around the cell: 16, 4, or 1 (the cell itself)
+        if (JUMP) {
+            if (!g.super_[(i[0] >> 4) + (i[1] >> 4) * g.sx + (i[2] >> 4) * g.sx * g.sy]) S = 16;
+            else if (!g.brick[(i[0] >> 2) + (i[1] >> 2) * g.bx + (i[2] >> 2) * g.bx * g.by]) S = 4;
+        }
+        if (S == 1 && filled(g, i[0], i[1], i[2])) {
             int ax = ea;
             if (ax < 0) { ax = 0; if (fabsf(dd[1]) > fabsf(dd[ax])) ax = 1; if (fabsf(dd[2]) > fabsf(dd[ax])) ax = 2; }
             r.t = te; r.idx = i[0] + i[1] * R[0] + i[2] * R[0] * R[1]; r.ax = ax; r.hit = true; return r;
         }
-        if (!bfull) {
+        if (S > 1) {
             float fe = INFINITY; int a = -1, ja = 0; bool nan = false;
             for (int k = 0; k < 3; ++k) {
                 if (step[k] == 0) continue;
-                const int j = step[k] > 0 ? std::min(i[k] | 3, R[k] - 1) : (i[k] & ~3);
+                const int j = step[k] > 0 ? std::min(i[k] | (S - 1), R[k] - 1) : (i[k] & ~(S - 1));
                 const float f = (O[k] + float(j + (step[k] > 0 ? 1 : 0)) * c - oo[k]) * inv[k];
                 if (f != f) nan = true;
                 if (a < 0 || f < fe) { fe = f; a = k; ja = j; }
@@ -58,7 +62,7 @@ Res walk(const Grid& g, const float oo[3], const float dd[3], float tmin, float 
                 if (!(fe <= t1)) return r;
                 for (int k = 0; k < 3; ++k) {
                     if (k == a || step[k] == 0) continue;
-                    for (int s = 0; s < 3 && (tn[k] < fe || (tn[k] == fe && (g_flip ? k > a : k < a))); ++s) {
+                    for (int s = 0; s < S - 1 && (tn[k] < fe || (tn[k] == fe && (g_flip ? k > a : k < a))); ++s) {
                         i[k] += step[k];
                         tn[k] = (O[k] + float(i[k] + (step[k] > 0 ? 1 : 0)) * c - oo[k]) * inv[k];
                     }
@@ -70,6 +74,7 @@ Res walk(const Grid& g, const float oo[3], const float dd[3], float tmin, float 
                 tn[a] = (O[a] + float(i[a] + (step[a] > 0 ? 1 : 0)) * c - oo[a]) * inv[a];
                 continue;
             }
+            if (filled(g, i[0], i[1], i[2])) return r;  // unreachable: an empty block holds no filled cell
         }
         int a = 0;
         if (tn[1] < tn[a]) a = 1;
@@ -90,7 +95,7 @@ int main(int argc, char** argv) {
     long long bad = 0, total = 0, hits = 0;
     for (int gi = 0; gi < 60; ++gi) {
         Grid g;
-        for (int k = 0; k < 3; ++k) g.R[k] = 1 + int(U(rng) * 70);
+        for (int k = 0; k < 3; ++k) g.R[k] = 1 + int(U(rng) * 110);
         g.c = gi % 3 == 0 ? 0.5f : gi % 3 == 1 ? 0.1f : 0.37f;
         for (int k = 0; k < 3; ++k) g.O[k] = (U(rng) - 0.5f) * 20;
         size_t n = size_t(g.R[0]) * g.R[1] * g.R[2];
@@ -108,6 +113,10 @@ int main(int argc, char** argv) {
         g.brick.assign(size_t(g.bx) * g.by * g.bz, 0);
         for (int z = 0; z < g.R[2]; ++z) for (int y = 0; y < g.R[1]; ++y) for (int x = 0; x < g.R[0]; ++x)
             if (filled(g, x, y, z)) g.brick[(x >> 2) + (y >> 2) * g.bx + (z >> 2) * g.bx * g.by] = 1;
+        g.sx = (g.R[0] + 15) >> 4; g.sy = (g.R[1] + 15) >> 4; g.sz = (g.R[2] + 15) >> 4;
+        g.super_.assign(size_t(g.sx) * g.sy * g.sz, 0);
+        for (int z = 0; z < g.R[2]; ++z) for (int y = 0; y < g.R[1]; ++y) for (int x = 0; x < g.R[0]; ++x)
+            if (filled(g, x, y, z)) g.super_[(x >> 4) + (y >> 4) * g.sx + (z >> 4) * g.sx * g.sy] = 1;
         for (int ri = 0; ri < 40000; ++ri) {
             float o[3], d[3];
             const int kind = ri % 6;

@@ -13,7 +13,7 @@ and the DDA walk that replaces the placeholder voxel_intersection.rint):
 - each level of detail is the OR of its 2x2x2 children;
 - the walk against a brute-force slab test over every filled cell, and closed forms for rays
   along an axis;
-- the device walk's brick jump against the cell walk (host C++ restatement of both)."""
+- the device walk's block moves against the cell walk (host C++ restatement of both)."""
 import numpy as np
 import pytest
 
@@ -204,10 +204,11 @@ def test_walk_closed_forms_along_axes(oracle):
 
 
 def test_brick_jump_equals_cell_walk():
-    """The device walk crosses an empty 4x4x4 brick in one move (trace_impl.hpp, voxel_walk).
-    tests/cpp/voxel_brick_walk.cpp restates that move and the cell-by-cell DDA in host C++ with
-    the same float arithmetic, and compares hit, cell, entry t (bits) and face over 2.4M rays on
-    60 random grids; with the tie rule flipped on purpose ("flip") it must find mismatches."""
+    """The device walk crosses an empty 16x16x16 block or 4x4x4 brick in one move
+    (trace_impl.hpp, voxel_walk). tests/cpp/voxel_brick_walk.cpp restates those moves and the
+    cell-by-cell DDA in host C++ with the same float arithmetic, and compares hit, cell, entry t
+    (bits) and face over 2.4M rays on 60 random grids; with the tie rule flipped on purpose
+    ("flip") it must find mismatches."""
     import os
     import subprocess
     here = os.path.dirname(os.path.abspath(__file__))

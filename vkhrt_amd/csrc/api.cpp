@@ -141,6 +141,7 @@ struct vkhrt_blas {
     vkhrt_blas_info info{};
     uint32_t* voxels = nullptr;  // voxel BLAS: occupancy bits of the traced level
     uint32_t* vbricks = nullptr;  // voxel BLAS: one bit per 4x4x4 brick of the traced level
+    uint32_t* vblocks = nullptr;  // voxel BLAS: one bit per 16x16x16 block of the traced level
     float vorigin[3] = {0, 0, 0};
     float vcell = 0.0f;
     int32_t vres[3] = {0, 0, 0};
@@ -685,6 +686,7 @@ int build_voxels(vkhrt_context* c, const vkhrt_blas_desc* d, hipStream_t st, vkh
     bool own = false;
     uint32_t* bits = nullptr;
     uint32_t* bricks = nullptr;
+    uint32_t* blocks = nullptr;
     uint32_t* scratch = nullptr;  // [0] overflow, [1] filled count, [2..7] bounds keys
     int rc = VKHRT_SUCCESS;
     hipError_t e = hipMalloc(&scratch, 8 * sizeof(uint32_t));
@@ -768,32 +770,40 @@ int build_voxels(vkhrt_context* c, const vkhrt_blas_desc* d, hipStream_t st, vkh
             rc = hip_fail(c, e, "voxel count");
             goto done;
         }
-        // Brick occupancy: two 2x downsamplings give one bit per 4x4x4 cells (brick (X, Y, Z)
-        // covers cells 4X..4X+3 per axis, clipped to the grid).
-        int32_t hres[3], bres[3];
+        // Empty-space levels: pairs of 2x downsamplings give one bit per 4x4x4 cells (brick
+        // (X, Y, Z) covers cells 4X..4X+3 per axis, clipped to the grid), then per 16x16x16.
+        int32_t hres[3], bres[3], h2res[3], kres[3];
         for (int k = 0; k < 3; ++k) {
             hres[k] = (res[k] + 1) / 2;
             bres[k] = (hres[k] + 1) / 2;
+            h2res[k] = (bres[k] + 1) / 2;
+            kres[k] = (h2res[k] + 1) / 2;
         }
         const size_t hwords = (size_t(hres[0]) * size_t(hres[1]) * size_t(hres[2]) + 31) / 32;
         const size_t bwords = (size_t(bres[0]) * size_t(bres[1]) * size_t(bres[2]) + 31) / 32;
+        const size_t kwords = (size_t(kres[0]) * size_t(kres[1]) * size_t(kres[2]) + 31) / 32;
         uint32_t* half = nullptr;
         e = hipMalloc(&half, hwords * sizeof(uint32_t));
         if (e == hipSuccess) e = hipMalloc(&bricks, bwords * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMalloc(&blocks, kwords * sizeof(uint32_t));
         if (e == hipSuccess) e = vk::voxel_downsample(bits, res, half, hres, st);
         if (e == hipSuccess) e = vk::voxel_downsample(half, hres, bricks, bres, st);
+        if (e == hipSuccess) e = vk::voxel_downsample(bricks, bres, half, h2res, st);
+        if (e == hipSuccess) e = vk::voxel_downsample(half, h2res, blocks, kres, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         hipFree(half);
         if (e != hipSuccess) {
             rc = hip_fail(c, e, "voxel bricks");
             goto done;
         }
-        b->info.device_bytes = uint64_t(nwords) * 4u + uint64_t(bwords) * 4u;
+        b->info.device_bytes = uint64_t(nwords) * 4u + uint64_t(bwords) * 4u + uint64_t(kwords) * 4u;
     }
     b->voxels = bits;
     bits = nullptr;
     b->vbricks = bricks;
     bricks = nullptr;
+    b->vblocks = blocks;
+    blocks = nullptr;
     b->records = filled;
     b->vcell = cell;
     std::memcpy(b->vres, res, sizeof(res));
@@ -816,6 +826,7 @@ done:
     if (own) hipFree(lines);
     hipFree(bits);
     hipFree(bricks);
+    hipFree(blocks);
     hipFree(scratch);
     return rc;
 }
@@ -1025,6 +1036,7 @@ void vkhrt_blas_destroy(vkhrt_blas* b) {
     hipFree(b->normals);
     hipFree(b->voxels);
     hipFree(b->vbricks);
+    hipFree(b->vblocks);
     hipFree(b->slot_of);
     delete b;
 }
@@ -1055,6 +1067,7 @@ int vkhrt_tlas_build(vkhrt_context* c, const vkhrt_blas* const* blases, uint32_t
         I.blas.slot_of = b->slot_of;
         I.blas.voxels = b->voxels;
         I.blas.vbricks = b->vbricks;
+        I.blas.vblocks = b->vblocks;
         std::memcpy(I.blas.vorigin, b->vorigin, sizeof(I.blas.vorigin));
         I.blas.vcell = b->vcell;
         std::memcpy(I.blas.vres, b->vres, sizeof(I.blas.vres));

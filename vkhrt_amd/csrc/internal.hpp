@@ -8,7 +8,8 @@
 //                 triangle      : 3 x float4 (vertex positions)               48 B
 //   voxels    : voxel BLAS only, one occupancy bit per cell (x fastest) in u32 words; no nodes
 //   vbricks   : voxel BLAS only, one bit per 4x4x4-cell brick (x fastest), set when any of its
-//               cells is filled; the walk reads no cell words inside an empty brick
+//               cells is filled, and vblocks the same per 16x16x16 cells; the walk crosses an
+//               empty brick or block in one move
 //   prim_ids  : original primitive index per leaf slot (u32)
 //   nodes     : 8-wide BVH nodes with 8-bit quantized child boxes, 80 B each
 #pragma once
@@ -50,6 +51,7 @@ struct BlasDev {
     float radius;          // phantom constant radius
     const uint32_t* voxels;  // voxels: occupancy bits of the traced level of detail
     const uint32_t* vbricks;  // voxels: one bit per 4x4x4 brick of cells (null: read every cell)
+    const uint32_t* vblocks;  // voxels: one bit per 16x16x16 block of cells (null with vbricks)
     float vorigin[3];        // voxels: grid origin (object space)
     float vcell;             // voxels: cell edge at the traced level
     int32_t vres[3];         // voxels: cells per axis at the traced level

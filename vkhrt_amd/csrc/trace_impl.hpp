@@ -377,12 +377,13 @@ __device__ __forceinline__ bool ray_box(f3 o, f3 d, const float* lo, const float
 // axis. Cell boundaries are origin + i * size in float, like the reference's voxel boxes
 // (GetVoxelWorldPosition, geometry_processor.cpp:438-449); the primitive id is the cell's
 // linear index (its AABB index there, gl_PrimitiveID in voxel_closest_hit.rchit:20).
-// Empty space is crossed a 4x4x4 brick at a time, with the result of the cell-by-cell walk.
-// That walk visits the face crossings of the three axes in the order of the key (t of the face,
-// axis), each axis's t being non-decreasing along its steps: it leaves a brick at the least key
-// among the brick's three exit faces, and each other axis has by then taken exactly its
-// crossings with smaller keys (at most 3 inside a brick). An empty brick (its bit clear) is
-// left in one move to that state: the cell, the entry t and the face are the walk's own.
+// Empty space is crossed a 16x16x16 block or a 4x4x4 brick at a time, with the result of the
+// cell-by-cell walk. That walk visits the face crossings of the three axes in the order of the
+// key (t of the face, axis), each axis's t being non-decreasing along its steps: it leaves an
+// aligned S-cell block at the least key among the block's three exit faces, and each other axis
+// has by then taken exactly its crossings with smaller keys (at most S - 1 inside the block). An
+// empty block (its bit clear) is left in one move to that state: the cell, the entry t and the
+// face are the walk's own.
 template <bool ANY>
 __device__ __forceinline__ void voxel_walk(const BlasDev& B, f3 o, f3 d, float tmin, uint32_t inst, Best& best,
                                            bool& done) {
@@ -422,23 +423,29 @@ __device__ __forceinline__ void voxel_walk(const BlasDev& B, f3 o, f3 d, float t
         tn[k] = step[k] == 0 ? INFINITY : (O[k] + float(i[k] + (step[k] > 0 ? 1 : 0)) * c - oo[k]) * inv[k];
     }
     const uint32_t rx = uint32_t(R[0]), ry = uint32_t(R[1]);
-    const uint32_t bx = (rx + 3u) >> 2, by = (ry + 3u) >> 2;
+    const uint32_t bx = (rx + 3u) >> 2, by = (ry + 3u) >> 2, kx = (rx + 15u) >> 4, ky = (ry + 15u) >> 4;
     float te = t0;
     for (int it = 0, lim = R[0] + R[1] + R[2] + 3; it < lim; ++it) {
         const uint32_t idx = uint32_t(i[0]) + uint32_t(i[1]) * rx + uint32_t(i[2]) * (rx * ry);
-        bool brick_full = true;
-        if (B.vbricks) {
-            const uint32_t bi = (uint32_t(i[0]) >> 2) + (uint32_t(i[1]) >> 2) * bx + (uint32_t(i[2]) >> 2) * (bx * by);
-            brick_full = (B.vbricks[bi >> 5] >> (bi & 31u)) & 1u;
+        int S = 1;  // the empty aligned block around the cell: 16, 4, or 1 (not empty)
+        if (B.vbricks && B.vblocks) {
+            const uint32_t ki = (uint32_t(i[0]) >> 4) + (uint32_t(i[1]) >> 4) * kx + (uint32_t(i[2]) >> 4) * (kx * ky);
+            if (!((B.vblocks[ki >> 5] >> (ki & 31u)) & 1u)) {
+                S = 16;
+            } else {
+                const uint32_t bi =
+                    (uint32_t(i[0]) >> 2) + (uint32_t(i[1]) >> 2) * bx + (uint32_t(i[2]) >> 2) * (bx * by);
+                if (!((B.vbricks[bi >> 5] >> (bi & 31u)) & 1u)) S = 4;
+            }
         }
-        if (!brick_full) {
+        if (S > 1) {
             float fe = INFINITY;
             int a = -1, ja = 0;
             bool nan = false;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 if (step[k] == 0) continue;
-                const int j = step[k] > 0 ? min(i[k] | 3, R[k] - 1) : (i[k] & ~3);
+                const int j = step[k] > 0 ? min(i[k] | (S - 1), R[k] - 1) : (i[k] & ~(S - 1));
                 const float f = (O[k] + float(j + (step[k] > 0 ? 1 : 0)) * c - oo[k]) * inv[k];
                 nan = nan || f != f;
                 if (a < 0 || f < fe) {
@@ -452,7 +459,7 @@ __device__ __forceinline__ void voxel_walk(const BlasDev& B, f3 o, f3 d, float t
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     if (k == a || step[k] == 0) continue;
-                    for (int s = 0; s < 3 && (tn[k] < fe || (tn[k] == fe && k < a)); ++s) {
+                    for (int s = 0; s < S - 1 && (tn[k] < fe || (tn[k] == fe && k < a)); ++s) {
                         i[k] += step[k];
                         tn[k] = (O[k] + float(i[k] + (step[k] > 0 ? 1 : 0)) * c - oo[k]) * inv[k];
                     }
